@@ -1,0 +1,229 @@
+/*
+ * within_amd.h -- C ABI of the MI355X fixed-effects projection library (libwithin_amd.so).
+ *
+ * The library solves the weighted fixed-effects normal equations D'W D x = D'W z with
+ * (modified) LSMR and an additive-Schwarz / diagonal / no preconditioner, on one AMD
+ * Instinct MI355X (gfx950), and returns x and the demeaned output z - D x.
+ *
+ * Every entry point replaces one item of the reference's public surface (a Rust crate
+ * plus a PyO3 module); the comment above each declaration names the reference
+ * interface (file:line, relative to the reference repository root).
+ *
+ * Conventions
+ *   - Plain pointers and sizes only.  Host pointers unless the name ends in `_device`.
+ *   - Inputs are borrowed for the duration of the call and never modified.
+ *   - Outputs are written into caller-provided buffers.  Matrices are column-major
+ *     (Fortran order) with an explicit leading dimension.
+ *   - Handles are immutable after construction and reference counted; every handle
+ *     returned through an out-pointer must be released with its `*_release`.
+ *   - All functions return a wx_status; on failure `wx_last_error()` returns a
+ *     thread-local message naming the offending input.
+ *   - Calls on one handle from several threads are safe: each solve owns its
+ *     workspace and HIP stream.
+ */
+#ifndef WITHIN_AMD_H
+#define WITHIN_AMD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Status codes mirror the reference error taxonomy:
+ *   crates/within/src/error.rs:12-146 (BuildError, WithinError) and
+ *   crates/schwarz-precond/src/error.rs:22-90 (SolveError, LocalSolveError). */
+typedef enum wx_status {
+    WX_OK = 0,
+    WX_ERR_INVALID_INPUT = 1,  /* SolveError::InvalidInput -> Python ValueError            */
+    WX_ERR_BUILD = 2,          /* BuildError::*            -> Python ValueError            */
+    WX_ERR_RUNTIME = 3,        /* other SolveError         -> Python RuntimeError          */
+    WX_ERR_DEVICE = 4,         /* HIP runtime failure      -> Python RuntimeError          */
+    WX_ERR_NOT_POSITIVE_DEF = 5, /* SolveError::InvalidInput "preconditioner not positive
+                                    definite" (lsmr/bidiag.rs:99-111) -> ValueError        */
+    WX_ERR_INDEX = 6           /* out-of-range layout lookup -> Python IndexError          */
+} wx_status;
+
+/* LSMR stop reasons, crates/schwarz-precond/src/lsmr.rs:54-72 (declaration order kept). */
+typedef enum wx_stop_reason {
+    WX_STOP_ZERO_RHS = 0,
+    WX_STOP_INITIAL_NORMAL_EQ_ZERO = 1,
+    WX_STOP_RESIDUAL_TOLERANCE = 2,
+    WX_STOP_NORMAL_EQ_TOLERANCE = 3,
+    WX_STOP_WARM_START_EXACT = 4,
+    WX_STOP_MAX_ITERATIONS = 5
+} wx_stop_reason;
+
+/* Arithmetic of the preconditioner apply.  The Krylov vectors, recurrences and the
+ * final residual check are FP64 in both modes. */
+typedef enum wx_precision {
+    WX_PRECISION_FP64 = 0,   /* FP64 preconditioner coefficients and arithmetic            */
+    WX_PRECISION_MIXED = 1   /* FP32 preconditioner coefficients (factors, C, 1/D), FP64
+                                vector arithmetic                                           */
+} wx_precision;
+
+/* Preconditioner kind, crates/within/src/config.rs:133-158 (PreconditionerConfig). */
+typedef enum wx_precond_kind {
+    WX_PRECOND_DEFAULT = 0,   /* Additive with LocalSolverConfig::default()                */
+    WX_PRECOND_OFF = 1,
+    WX_PRECOND_DIAGONAL = 2,
+    WX_PRECOND_ADDITIVE = 3
+} wx_precond_kind;
+
+/* Schur mode, crates/within/src/config.rs:47-59. */
+typedef enum wx_schur_mode { WX_SCHUR_APPROXIMATE = 0, WX_SCHUR_EXACT = 1 } wx_schur_mode;
+
+/* LocalSolverConfig / ApproxCholConfig / ApproxSchurConfig / ScalingConfig,
+ * crates/within/src/config.rs:20-131.  `dense_threshold < 0` selects the device cost
+ * model (the default): a kept block is factored densely and exactly when that is the
+ * cheaper device apply; otherwise approximate Cholesky is used.  A value >= 0 keeps the
+ * reference rule (dense exact factor iff n_kept <= dense_threshold). */
+typedef struct wx_precond_config {
+    int32_t kind;              /* wx_precond_kind */
+    int32_t schur;             /* wx_schur_mode */
+    uint64_t approx_chol_seed;
+    int32_t split_merge;       /* <= 0: None */
+    uint64_t schur_seed;
+    int32_t schur_split;       /* >= 1 */
+    int64_t dense_threshold;
+    double scaling_tolerance;
+    int64_t scaling_max_sweeps;
+    int32_t scaling_error_on_failure; /* 0 Warn, 1 Error */
+    int32_t reduction;         /* ReductionStrategy; accepted, the device combine is always
+                                  the deterministic gather */
+} wx_precond_config;
+
+/* LsmrOptions, crates/within/src/config.rs:160-178, plus the device precision. */
+typedef struct wx_lsmr_options {
+    double tol;
+    int64_t maxiter;
+    int64_t local_size;        /* <= 0: None */
+    int32_t precision;         /* wx_precision */
+    int32_t check_every;       /* host convergence poll interval in iterations; <= 0: auto */
+} wx_lsmr_options;
+
+/* Per-RHS result, the per-column fields of BatchSolveResult (crates/within/src/solver.rs:248-293)
+ * plus the FP64 true normal-equation residual ||D'W(z - Dx)|| / ||D'Wz||. */
+typedef struct wx_rhs_stats {
+    int32_t converged;
+    int32_t stop_reason;       /* wx_stop_reason */
+    int64_t iterations;
+    double residual;           /* LSMR estimate |zeta_k| / |zeta_0| (M-metric when preconditioned) */
+    double residual_norm;      /* |phi_bar| */
+    double true_residual;      /* ||D'W(z - D x)|| / ||D'W z|| computed in FP64 on the device */
+    double time_solve;         /* seconds */
+} wx_rhs_stats;
+
+typedef struct wx_design wx_design;
+typedef struct wx_solver wx_solver;
+typedef struct wx_precond wx_precond;
+
+/* Thread-local message of the last failed call on this thread ("" if none). */
+const char *wx_last_error(void);
+
+/* Library / device information: fills `name` (NUL terminated) and returns the
+ * device count seen by the HIP runtime (0 when no GPU is visible). */
+int32_t wx_device_info(char *name, uint64_t name_len);
+
+/* Default configs (config.rs:74-131, 160-178). */
+void wx_precond_config_default(wx_precond_config *cfg);
+void wx_lsmr_options_default(wx_lsmr_options *opts);
+
+/* ---- design ---------------------------------------------------------------- */
+
+/* Replaces `IntoDesign for ArrayView2<u32>` (crates/within/src/solver.rs:34-48) +
+ * `Design::from_frame` (crates/within/src/domain.rs:174-178): intercept-only factors,
+ * element (i, f) at codes[i*row_stride + f*col_stride]. */
+wx_status wx_design_from_codes(const uint32_t *codes, uint64_t n_obs, uint64_t n_factors,
+                               int64_t row_stride, int64_t col_stride, wx_design **out);
+
+/* Replaces `Design::new(effects)` (crates/within/src/domain.rs:156-171) with
+ * `Effect::new(levels, intercept, slopes)` (crates/within/src/domain/effect.rs:13-42).
+ * levels[e] has n_obs codes; effect e has n_slopes[e] slope columns taken in order from
+ * the flat list `slopes` (each n_obs long). */
+wx_status wx_design_from_effects(uint64_t n_obs, uint64_t n_effects, const uint32_t *const *levels,
+                                 const uint8_t *intercept, const uint64_t *n_slopes,
+                                 const double *const *slopes, wx_design **out);
+
+void wx_design_release(wx_design *d);
+
+/* ---- solver ------------------------------------------------------------------ */
+
+/* Replaces `Solver::new(design, weights, preconditioner)` (crates/within/src/solver.rs:361-411).
+ * Exactly one of `cfg` / `prebuilt` may be non-NULL; both NULL = PreconditionerInput::Default. */
+wx_status wx_solver_new(const wx_design *design, const double *weights, const wx_precond_config *cfg,
+                        const wx_precond *prebuilt, wx_solver **out);
+
+void wx_solver_release(wx_solver *s);
+
+/* Replaces `Solver::n_obs / n_dofs` (crates/within/src/solver.rs:586-596). */
+uint64_t wx_solver_n_obs(const wx_solver *s);
+uint64_t wx_solver_n_dofs(const wx_solver *s);
+
+/* Seconds spent in wx_solver_new (design + reparametrisation + preconditioner build). */
+double wx_solver_time_setup(const wx_solver *s);
+
+/* Replaces `Solver::solve_batch` (crates/within/src/solver.rs:530-577) and `Solver::solve`
+ * (solver.rs:503-527, k = 1).  Y is n_obs x k column-major with leading dimension ldy;
+ * X (n_dofs x k, ld n_dofs) and demeaned (n_obs x k, ld n_obs) are column-major and may be
+ * NULL when not wanted.  `opts` NULL = LsmrOptions::default(). */
+wx_status wx_solver_solve_batch(const wx_solver *s, const double *Y, uint64_t ldy, uint64_t k,
+                                const wx_lsmr_options *opts, double *X, double *demeaned,
+                                wx_rhs_stats *stats);
+
+/* Same as wx_solver_solve_batch with every array in device memory (HBM) and the work
+ * enqueued on `stream` (a hipStream_t, NULL = a library-owned stream).  Returns after the
+ * solve has completed.  No reference counterpart: the reference has no device boundary;
+ * this is the entry point for callers whose data already lives on the GPU. */
+wx_status wx_solver_solve_batch_device(const wx_solver *s, const double *Y_device, uint64_t ldy, uint64_t k,
+                                       const wx_lsmr_options *opts, double *X_device, double *demeaned_device,
+                                       wx_rhs_stats *stats, void *stream);
+
+/* Replaces `Operator::apply` / `Operator::apply_adjoint` of the weighted design operator
+ * A = W^{1/2} D (crates/schwarz-precond/src/lib.rs:48-57 implemented by
+ * crates/within/src/operator/design.rs:92-117), in the caller's row order:
+ * adjoint = 0: Y (n_obs x k) = A X (n_dofs x k);  adjoint = 1: Y (n_dofs x k) = A' X (n_obs x k).
+ * Column-major host arrays with leading dimensions n_obs / n_dofs. */
+wx_status wx_solver_apply_operator(const wx_solver *s, int32_t adjoint, const double *X, uint64_t k, double *Y);
+
+/* Replaces `Solver::preconditioner()` (crates/within/src/solver.rs:580-583): *out is a new
+ * reference (NULL when the solver runs unpreconditioned). */
+wx_status wx_solver_preconditioner(const wx_solver *s, wx_precond **out);
+
+/* Replaces `CoefficientLayout` (crates/within/src/solver.rs:125-200): per-term
+ * (offset, n_levels, n_columns).  Arrays must hold wx_solver_n_terms() entries. */
+uint64_t wx_solver_n_terms(const wx_solver *s);
+void wx_solver_layout(const wx_solver *s, uint64_t *offset, uint64_t *n_levels, uint64_t *n_columns);
+
+/* Replaces `SolveResult::unidentified` (crates/within/src/solver.rs:221-222,
+ * solver/reparam.rs): triples (term, level, column), `count` in/out. */
+wx_status wx_solver_unidentified(const wx_solver *s, uint64_t *triples, uint64_t *count);
+
+/* Replaces `Solver::warnings` (crates/within/src/solver.rs:415-419, BuildWarning
+ * error.rs:107-134): number of warnings and the text of warning i. */
+uint64_t wx_solver_n_warnings(const wx_solver *s);
+const char *wx_solver_warning(const wx_solver *s, uint64_t i);
+
+/* ---- preconditioner --------------------------------------------------------- */
+
+void wx_precond_release(wx_precond *p);
+
+/* Replaces `Preconditioner::nrows/ncols/variant_name` (crates/within/src/operator/schwarz.rs:129-147). */
+uint64_t wx_precond_n(const wx_precond *p);
+const char *wx_precond_variant(const wx_precond *p);
+
+/* Replaces `Preconditioner::apply` (crates/within/src/operator/schwarz.rs:150-152):
+ * y = M^{-1} x for n x k column-major host arrays (FP64 coefficients). */
+wx_status wx_precond_apply(const wx_precond *p, const double *x, uint64_t k, double *y);
+
+/* Replaces the postcard persistence of `Preconditioner` (crates/within-py/src/config.rs:367-388):
+ * a versioned little-endian binary format of this library.  The buffer is allocated by
+ * the library and freed with wx_buffer_free. */
+wx_status wx_precond_serialize(const wx_precond *p, uint8_t **buf, uint64_t *len);
+wx_status wx_precond_deserialize(const uint8_t *buf, uint64_t len, wx_precond **out);
+void wx_buffer_free(uint8_t *buf);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* WITHIN_AMD_H */

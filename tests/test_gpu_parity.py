@@ -1,0 +1,251 @@
+"""GPU parity tests: the HIP path (through the C ABI) against the CPU oracle and closed forms."""
+import os
+import pickle
+
+import numpy as np
+import pytest
+
+import within_amd as wa
+from within_amd import _lib
+from oracle.oracle import exact_demean, normal_eq_residual, sparse_design
+from tests._helpers import oracle_schwarz_from_bytes, parse_precond
+from tests.conftest import cfg1_data
+
+pytestmark = pytest.mark.gpu
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+SMALL = np.load(os.path.join(HERE, "golden", "small_projections.npz"))
+CASES = ["two_factor", "three_factor_weighted", "disconnected"]
+
+
+def small(case):
+    w = SMALL[f"{case}__w"] if f"{case}__w" in SMALL.files else None
+    return np.asfortranarray(SMALL[f"{case}__codes"]), np.asfortranarray(SMALL[f"{case}__y"]), w
+
+
+def apply_op(solver, X, adjoint):
+    import ctypes as C
+    X = np.asfortranarray(X, dtype=np.float64)
+    k = X.shape[1]
+    rows = solver.n_dofs if adjoint else solver.n_obs
+    Y = np.empty((rows, k), order="F")
+    _lib.check(_lib.lib().wx_solver_apply_operator(solver._h, 1 if adjoint else 0, X.ctypes.data, k, Y.ctypes.data))
+    return Y
+
+
+def rel(a, b):
+    return np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300)
+
+
+# ---------------------------------------------------------------------------
+# operators D, D' (bit-exact known answers, closed forms, adjoint identity)
+# ---------------------------------------------------------------------------
+
+def test_operator_known_answers(gpu):
+    # crates/within/src/operator/design/tests.rs:6-57
+    codes = np.asfortranarray(np.array([[0, 1, 1, 2, 0], [0, 0, 1, 2, 3]], dtype=np.uint32).T)
+    s = wa.Solver(codes, preconditioner=wa.PreconditionerConfig.Off)
+    Dx = apply_op(s, np.array([[1.0, 2, 3, 10, 20, 30, 40]]).T, False)[:, 0]
+    assert Dx.tolist() == [11.0, 12.0, 22.0, 33.0, 41.0]
+    Dtr = apply_op(s, np.array([[1.0, 2, 3, 4, 5]]).T, True)[:, 0]
+    assert Dtr.tolist() == [6.0, 5.0, 4.0, 3.0, 3.0, 4.0, 5.0]
+
+
+@pytest.mark.parametrize("weighted", [False, True])
+@pytest.mark.parametrize("k", [1, 3, 16])
+def test_operator_matches_oracle_bitwise(gpu, oracle, weighted, k):
+    rng = np.random.default_rng(17 + k)
+    n = 5000
+    # unsorted dominant factor -> locality sort; Zipf-like head to exercise split levels
+    a = np.minimum((rng.pareto(1.1, n) * 3).astype(np.int64), 399)
+    codes = np.asfortranarray(np.column_stack([a, rng.integers(0, 37, n), rng.integers(0, 5, n)]).astype(np.uint32))
+    w = rng.uniform(0.5, 2.0, n) if weighted else None
+    s = wa.Solver(codes, weights=w, preconditioner=wa.PreconditionerConfig.Off)
+    d = oracle.design(codes)
+    X = rng.standard_normal((s.n_dofs, k))
+    U = rng.standard_normal((n, k))
+    sw = None if w is None else np.sqrt(w)
+    got = apply_op(s, X, False)
+    got_t = apply_op(s, U, True)
+    for j in range(k):
+        # the gather's per-row sum order is the reference's: bit-exact
+        assert np.array_equal(got[:, j], oracle.gather(d, X[:, j], sw)), j
+        # the reduction sums each level in code-sorted row order (split levels pairwise)
+        np.testing.assert_allclose(got_t[:, j], oracle.scatter(d, U[:, j], sw), rtol=1e-13, atol=1e-12)
+    lhs = np.sum(got * U, axis=0)
+    rhs = np.sum(X * got_t, axis=0)
+    np.testing.assert_allclose(lhs, rhs, rtol=1e-11)
+
+
+# ---------------------------------------------------------------------------
+# full solves
+# ---------------------------------------------------------------------------
+
+@pytest.mark.parametrize("case", CASES)
+@pytest.mark.parametrize("prec", ["off", "diagonal"])
+def test_solve_matches_oracle_same_algorithm(gpu, oracle, case, prec):
+    codes, y, w = small(case)
+    pc = wa.PreconditionerConfig.Off if prec == "off" else wa.PreconditionerConfig.Diagonal
+    opts = wa.LsmrOptions(tol=1e-10, maxiter=5000)
+    r = wa.solve_batch(codes, y, weights=w, options=opts, preconditioner=pc)
+    d = oracle.design(codes)
+    _, Dm, res = oracle.solve_batch(d, y, w, prec=prec, tol=1e-10, maxiter=5000)
+    assert all(r.converged)
+    np.testing.assert_allclose(r.demeaned, Dm, atol=1e-9)
+    np.testing.assert_allclose(r.demeaned, SMALL[f"{case}__demeaned"], atol=1e-7)
+    assert max(abs(a - b.iterations) for a, b in zip(r.iterations, res)) <= 2
+
+
+@pytest.mark.parametrize("case", CASES)
+def test_additive_schwarz_apply_matches_oracle(gpu, oracle, case):
+    codes, y, w = small(case)
+    s = wa.Solver(codes, weights=w)
+    p = s.preconditioner
+    assert repr(p).startswith("Preconditioner(Additive")
+    sch = oracle_schwarz_from_bytes(oracle, p.to_bytes())
+    rng = np.random.default_rng(1)
+    for _ in range(3):
+        x = rng.standard_normal(s.n_dofs)
+        np.testing.assert_allclose(p.apply(x), sch.apply(x), rtol=1e-11, atol=1e-12)
+
+
+@pytest.mark.parametrize("case", CASES)
+def test_additive_solve_matches_oracle_and_exact(gpu, oracle, case):
+    codes, y, w = small(case)
+    s = wa.Solver(codes, weights=w)
+    r = s.solve_batch(y, wa.LsmrOptions(tol=1e-10))
+    sch = oracle_schwarz_from_bytes(oracle, s.preconditioner.to_bytes())
+    d = oracle.design(codes)
+    # the oracle solves in the internal (locality-sorted) order: permute like the product
+    _, Dm, res = oracle.solve_batch(d, y, w, prec="schwarz", schwarz=sch, tol=1e-10)
+    assert all(r.converged) and all(q.converged for q in res)
+    np.testing.assert_allclose(r.demeaned, Dm, atol=1e-9)
+    np.testing.assert_allclose(r.demeaned, SMALL[f"{case}__demeaned"], atol=1e-8)
+    np.testing.assert_allclose(r.true_residual, normal_eq_residual(codes, y, r.demeaned, w), rtol=1e-6, atol=1e-14)
+
+
+def test_additive_schwarz_is_spd(gpu):
+    codes, y, w = small("three_factor_weighted")
+    p = wa.Solver(codes, weights=w).preconditioner
+    rng = np.random.default_rng(2)
+    n = p.nrows
+    X = rng.standard_normal((n, 6))
+    MX = np.column_stack([p.apply(X[:, j].copy()) for j in range(6)])
+    G = X.T @ MX
+    np.testing.assert_allclose(G, G.T, rtol=1e-10, atol=1e-12)
+    assert np.all(np.linalg.eigvalsh((G + G.T) / 2) > -1e-10)
+
+
+def test_config1_parity(gpu, oracle):
+    """README data, default Additive, tol 1e-8, maxiter 1000."""
+    fe, y = cfg1_data()
+    r = wa.solve(fe, y)
+    assert r.converged
+    ex = exact_demean(sparse_design(fe), y)
+    print(f"[cfg1] iterations={r.iterations} rel_err={rel(r.demeaned, ex):.3e} "
+          f"true_residual={r.true_residual:.3e}", flush=True)
+    assert rel(r.demeaned, ex) <= 1e-8
+    assert r.true_residual <= 1e-7
+    np.testing.assert_allclose(r.true_residual, normal_eq_residual(fe, y, r.demeaned)[0], rtol=1e-6)
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_batch_equals_columnwise_and_is_deterministic(gpu, precision):
+    rng = np.random.default_rng(9)
+    n = 20000
+    codes = np.asfortranarray(np.column_stack([rng.integers(0, 900, n), rng.integers(0, 60, n),
+                                               rng.integers(0, 8, n)]).astype(np.uint32))
+    w = rng.uniform(0.5, 2.0, n)
+    Y = np.asfortranarray(rng.standard_normal((n, 5)))
+    s = wa.Solver(codes, weights=w)
+    opts = wa.LsmrOptions(tol=1e-10, precision=precision)
+    b1 = s.solve_batch(Y, opts)
+    b2 = s.solve_batch(Y, opts)
+    assert np.array_equal(b1.demeaned, b2.demeaned) and np.array_equal(b1.x, b2.x)
+    for j in [0, 3]:
+        r = s.solve(Y[:, j].copy(), opts)
+        # a single column reproduces its batch column (same per-column arithmetic)
+        np.testing.assert_allclose(r.demeaned, b1.demeaned[:, j], rtol=0, atol=1e-9)
+    ex = exact_demean(sparse_design(codes), Y, w)
+    assert rel(b1.demeaned, ex) <= 1e-8
+
+
+def test_edge_cases(gpu):
+    rng = np.random.default_rng(4)
+    n = 3000
+    codes = np.asfortranarray(np.column_stack([rng.integers(0, 50, n), rng.integers(0, 7, n)]).astype(np.uint32))
+    s = wa.Solver(codes)
+    # zero RHS -> ZeroRhs, x = 0
+    r = s.solve(np.zeros(n))
+    assert r.converged and r.iterations == 0 and r.stop_reason == "ZeroRhs" and not r.x.any()
+    # y in the column space of D -> demeaned ~ 0
+    xt = rng.standard_normal(s.n_dofs)
+    D = sparse_design(codes)
+    r = s.solve(D @ xt, wa.LsmrOptions(tol=1e-12))
+    assert np.abs(r.demeaned).max() < 1e-8
+    # maxiter exhausted
+    r = s.solve(rng.standard_normal(n), wa.LsmrOptions(tol=1e-14, maxiter=2))
+    assert not r.converged and r.iterations == 2 and r.stop_reason == "MaxIterations"
+    # empty batch
+    b = s.solve_batch(np.zeros((n, 0)))
+    assert b.x.shape == (s.n_dofs, 0) and b.demeaned.shape == (n, 0)
+    # non-finite response -> ValueError
+    yb = rng.standard_normal(n)
+    yb[7] = np.nan
+    with pytest.raises(ValueError, match="finite"):
+        s.solve(yb)
+    # zero weights exclude rows (edge_cases.rs:74-160): equal to dropping them
+    w = np.ones(n)
+    w[:500] = 0.0
+    y = rng.standard_normal(n)
+    r = wa.solve(codes, y, weights=w, options=wa.LsmrOptions(tol=1e-11))
+    ex = exact_demean(D, y, w)
+    np.testing.assert_allclose(r.demeaned[500:], ex[500:], atol=1e-8)
+    # uniform weights == unweighted (edge_cases.rs:323-351)
+    r1 = wa.solve(codes, y, weights=np.full(n, 2.5), options=wa.LsmrOptions(tol=1e-11))
+    r0 = wa.solve(codes, y, options=wa.LsmrOptions(tol=1e-11))
+    np.testing.assert_allclose(r1.demeaned, r0.demeaned, atol=1e-8)
+
+
+def test_locality_sort_is_transparent(gpu):
+    rng = np.random.default_rng(12)
+    n = 8000
+    codes = np.column_stack([rng.integers(0, 400, n), rng.integers(0, 30, n)]).astype(np.uint32)
+    y = rng.standard_normal(n)
+    order = np.argsort(codes[:, 0], kind="stable")
+    r_uns = wa.solve(np.asfortranarray(codes), y, options=wa.LsmrOptions(tol=1e-11))
+    r_srt = wa.solve(np.asfortranarray(codes[order]), y[order], options=wa.LsmrOptions(tol=1e-11))
+    np.testing.assert_allclose(r_uns.demeaned[order], r_srt.demeaned, atol=1e-8)
+    with pytest.warns(UserWarning, match="F-contiguous"):
+        wa.solve(np.ascontiguousarray(codes), y)
+
+
+def test_preconditioner_pickle_round_trip(gpu):
+    codes, y, w = small("three_factor_weighted")
+    s = wa.Solver(codes, weights=w)
+    p = s.preconditioner
+    q = pickle.loads(pickle.dumps(p))
+    assert repr(q) == repr(p) and q.nrows == p.nrows
+    x = np.random.default_rng(0).standard_normal(p.nrows)
+    assert np.array_equal(p.apply(x), q.apply(x))
+    s2 = wa.Solver(codes, weights=w, preconditioner=q)
+    a = s.solve_batch(y)
+    b = s2.solve_batch(y)
+    assert np.array_equal(a.demeaned, b.demeaned)
+    with pytest.raises(ValueError, match="dimension"):
+        wa.Solver(codes[:, :2].copy(order="F"), preconditioner=q)
+    d = wa.Solver(codes, preconditioner=wa.PreconditionerConfig.Diagonal).preconditioner
+    assert repr(d).startswith("Preconditioner(Diagonal") and parse_precond(d.to_bytes())["kind"] == 0
+    assert wa.Solver(codes, preconditioner=wa.PreconditionerConfig.Off).preconditioner is None
+
+
+def test_single_factor_runs_plain_lsmr(gpu):
+    # one factor: no factor pair, the Additive build falls back to plain LSMR (schwarz.rs:249-252)
+    rng = np.random.default_rng(8)
+    codes = np.asfortranarray(rng.integers(0, 40, (2000, 1)).astype(np.uint32))
+    y = rng.standard_normal(2000)
+    s = wa.Solver(codes)
+    assert s.preconditioner is None
+    r = s.solve(y)
+    means = np.bincount(codes[:, 0], weights=y) / np.bincount(codes[:, 0])
+    np.testing.assert_allclose(r.demeaned, y - means[codes[:, 0]], atol=1e-9)

@@ -1,0 +1,398 @@
+// capi.cpp -- the extern "C" boundary declared in include/within_amd.h.
+#include <hip/hip_runtime.h>
+
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <new>
+
+#include "schwarz.hpp"
+#include "solver.hpp"
+
+using namespace wx;
+
+struct wx_design {
+    std::atomic<int> refs{1};
+    uint64_t n_obs = 0;
+    std::vector<std::vector<uint32_t>> columns;
+    std::vector<std::vector<int64_t>> structure;
+    std::vector<std::vector<double>> slopes;
+};
+
+struct wx_solver {
+    std::atomic<int> refs{1};
+    Solver s;
+};
+
+struct wx_precond {
+    std::atomic<int> refs{1};
+    std::shared_ptr<Precond> p;
+};
+
+namespace {
+
+thread_local std::string g_last_error;
+
+template <typename F> wx_status guard(F &&f) {
+    try {
+        g_last_error.clear();
+        f();
+        return WX_OK;
+    } catch (const Error &e) {
+        g_last_error = e.what();
+        return e.code;
+    } catch (const std::bad_alloc &) {
+        g_last_error = "out of host memory";
+        return WX_ERR_RUNTIME;
+    } catch (const std::exception &e) {
+        g_last_error = e.what();
+        return WX_ERR_RUNTIME;
+    }
+}
+
+void require_device() {
+    int count = 0;
+    if (hipGetDeviceCount(&count) != hipSuccess || count == 0)
+        fail(WX_ERR_DEVICE, "no AMD GPU visible to the HIP runtime: within_amd runs its solves on the device only");
+}
+
+struct Stream {
+    hipStream_t s = nullptr;
+    bool own = false;
+    explicit Stream(void *ext) {
+        if (ext) s = (hipStream_t)ext;
+        else {
+            WX_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+            own = true;
+        }
+    }
+    ~Stream() {
+        if (own && s) (void)hipStreamDestroy(s);
+    }
+};
+
+LocalSolverSettings settings_from(const wx_precond_config &c) {
+    LocalSolverSettings st;
+    st.approx_chol_seed = c.approx_chol_seed;
+    st.split_merge = c.split_merge;
+    st.schur_exact = c.schur == WX_SCHUR_EXACT;
+    st.schur_seed = c.schur_seed;
+    st.schur_split = c.schur_split;
+    st.dense_threshold = c.dense_threshold;
+    st.scaling_tolerance = c.scaling_tolerance;
+    st.scaling_max_sweeps = c.scaling_max_sweeps;
+    st.scaling_error = c.scaling_error_on_failure != 0;
+    return st;
+}
+
+} // namespace
+
+extern "C" {
+
+const char *wx_last_error(void) { return g_last_error.c_str(); }
+
+int32_t wx_device_info(char *name, uint64_t name_len) {
+    int count = 0;
+    if (hipGetDeviceCount(&count) != hipSuccess) count = 0;
+    if (name && name_len) {
+        name[0] = 0;
+        if (count > 0) {
+            hipDeviceProp_t prop;
+            if (hipGetDeviceProperties(&prop, 0) == hipSuccess) {
+                std::snprintf(name, name_len, "%s (%s)", prop.name, prop.gcnArchName);
+            }
+        }
+    }
+    return count;
+}
+
+void wx_precond_config_default(wx_precond_config *cfg) {
+    // crates/within/src/config.rs:74-131 (LocalSolverConfig::default)
+    std::memset(cfg, 0, sizeof(*cfg));
+    cfg->kind = WX_PRECOND_DEFAULT;
+    cfg->schur = WX_SCHUR_APPROXIMATE;
+    cfg->approx_chol_seed = 0;
+    cfg->split_merge = 2;
+    cfg->schur_seed = 0;
+    cfg->schur_split = 1;
+    cfg->dense_threshold = -1;
+    cfg->scaling_tolerance = 1e-9;
+    cfg->scaling_max_sweeps = 2048;
+    cfg->scaling_error_on_failure = 0;
+    cfg->reduction = 0;
+}
+
+void wx_lsmr_options_default(wx_lsmr_options *o) {
+    // crates/within/src/config.rs:170-177
+    o->tol = 1e-8;
+    o->maxiter = 1000;
+    o->local_size = 0;
+    o->precision = WX_PRECISION_FP64;
+    o->check_every = 0;
+}
+
+wx_status wx_design_from_codes(const uint32_t *codes, uint64_t n_obs, uint64_t n_factors, int64_t rs, int64_t cs,
+                               wx_design **out) {
+    return guard([&] {
+        if (!out) fail(WX_ERR_INVALID_INPUT, "null output pointer");
+        auto *d = new wx_design();
+        d->n_obs = n_obs;
+        d->columns.assign(n_factors, std::vector<uint32_t>(n_obs));
+        for (uint64_t f = 0; f < n_factors; ++f) {
+            uint32_t *dst = d->columns[f].data();
+            if (rs == 1) std::memcpy(dst, codes + (int64_t)f * cs, n_obs * 4);
+            else
+                for (uint64_t i = 0; i < n_obs; ++i) dst[i] = codes[(int64_t)i * rs + (int64_t)f * cs];
+        }
+        d->structure.assign(n_factors, std::vector<int64_t>{-1});
+        *out = d;
+    });
+}
+
+wx_status wx_design_from_effects(uint64_t n_obs, uint64_t n_effects, const uint32_t *const *levels,
+                                 const uint8_t *intercept, const uint64_t *n_slopes, const double *const *slopes,
+                                 wx_design **out) {
+    return guard([&] {
+        if (!out) fail(WX_ERR_INVALID_INPUT, "null output pointer");
+        auto d = std::make_unique<wx_design>();
+        d->n_obs = n_obs;
+        uint64_t sp = 0;
+        for (uint64_t e = 0; e < n_effects; ++e) {
+            // Effect::new (crates/within/src/domain/effect.rs:13-42)
+            std::vector<int64_t> cols;
+            if (intercept[e]) cols.push_back(-1);
+            for (uint64_t s = 0; s < n_slopes[e]; ++s) {
+                const double *z = slopes[sp + s];
+                for (uint64_t i = 0; i < n_obs; ++i)
+                    if (!std::isfinite(z[i]))
+                        fail(WX_ERR_BUILD, "slope " + std::to_string(s) + " loading at index " + std::to_string(i) +
+                                               " must be finite, got " + std::to_string(z[i]));
+                cols.push_back((int64_t)d->slopes.size());
+                d->slopes.emplace_back(z, z + n_obs);
+            }
+            sp += n_slopes[e];
+            if (cols.empty()) fail(WX_ERR_BUILD, "an effect must have an intercept or at least one slope");
+            d->columns.emplace_back(levels[e], levels[e] + n_obs);
+            d->structure.push_back(std::move(cols));
+        }
+        *out = d.release();
+    });
+}
+
+void wx_design_release(wx_design *d) {
+    if (d && --d->refs == 0) delete d;
+}
+
+wx_status wx_solver_new(const wx_design *design, const double *weights, const wx_precond_config *cfg,
+                        const wx_precond *prebuilt, wx_solver **out) {
+    return guard([&] {
+        if (!design || !out) fail(WX_ERR_INVALID_INPUT, "null design or output pointer");
+        if (cfg && prebuilt) fail(WX_ERR_INVALID_INPUT, "pass either a preconditioner config or a prebuilt one");
+        require_device();
+        auto t0 = std::chrono::steady_clock::now();
+        auto h = std::make_unique<wx_solver>();
+        Solver &s = h->s;
+        // crates/within/src/solver.rs:361-411
+        s.design = build_design(design->n_obs, design->columns, design->structure, design->slopes, true);
+        std::vector<double> w;
+        if (weights) {
+            validate_weights(s.design, weights, s.design.n_obs);
+            w.resize(s.design.n_obs);
+            permute_in(s.design, weights, w.data());
+        }
+        s.weighted = weights != nullptr;
+        if (s.design.has_slopes()) s.reparam = build_reparam(s.design, weights ? w.data() : nullptr);
+        if (prebuilt) {
+            if (prebuilt->p->n != s.design.n_dofs)
+                fail(WX_ERR_BUILD, "preconditioner dimension mismatch: expected " + std::to_string(s.design.n_dofs) +
+                                       ", got " + std::to_string(prebuilt->p->n) + "x" +
+                                       std::to_string(prebuilt->p->n));
+            s.precond = prebuilt->p;
+        } else {
+            wx_precond_config c;
+            if (cfg) c = *cfg;
+            else wx_precond_config_default(&c);
+            if (c.kind == WX_PRECOND_OFF) s.precond = nullptr;
+            else if (c.kind == WX_PRECOND_DIAGONAL) s.precond = build_diagonal(s.design, w);
+            else s.precond = build_additive(s.design, w, settings_from(c), s.warnings);
+        }
+        upload_design(s, w);
+        s.time_setup = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+        *out = h.release();
+    });
+}
+
+void wx_solver_release(wx_solver *s) {
+    if (s && --s->refs == 0) delete s;
+}
+
+uint64_t wx_solver_n_obs(const wx_solver *s) { return s ? s->s.design.n_obs : 0; }
+uint64_t wx_solver_n_dofs(const wx_solver *s) { return s ? s->s.design.n_dofs : 0; }
+double wx_solver_time_setup(const wx_solver *s) { return s ? s->s.time_setup : 0.0; }
+
+wx_status wx_solver_solve_batch_device(const wx_solver *s, const double *Y, uint64_t ldy, uint64_t k,
+                                       const wx_lsmr_options *opts, double *X, double *Dm, wx_rhs_stats *stats,
+                                       void *stream) {
+    return guard([&] {
+        if (!s || !stats) fail(WX_ERR_INVALID_INPUT, "null solver or stats pointer");
+        wx_lsmr_options o;
+        if (opts) o = *opts;
+        else wx_lsmr_options_default(&o);
+        if (o.local_size > 0)
+            fail(WX_ERR_INVALID_INPUT, "local_size (windowed reorthogonalization) is not supported on the device");
+        Stream st(stream);
+        solve_batch_device(s->s, Y, ldy, k, o, X, Dm, stats, st.s);
+        WX_HIP(hipStreamSynchronize(st.s));
+    });
+}
+
+wx_status wx_solver_solve_batch(const wx_solver *s, const double *Y, uint64_t ldy, uint64_t k,
+                                const wx_lsmr_options *opts, double *X, double *demeaned, wx_rhs_stats *stats) {
+    return guard([&] {
+        if (!s || !stats || (!Y && k)) fail(WX_ERR_INVALID_INPUT, "null solver, response or stats pointer");
+        const uint64_t n = s->s.design.n_obs, N = s->s.design.n_dofs;
+        if (ldy < n) fail(WX_ERR_INVALID_INPUT, "leading dimension of Y is smaller than n_obs");
+        // validate the response on the host (solver.rs:424-442)
+        for (uint64_t j = 0; j < k; ++j)
+            for (uint64_t i = 0; i < n; ++i) {
+                const double v = Y[j * ldy + i];
+                if (!std::isfinite(v))
+                    fail(WX_ERR_INVALID_INPUT, "Solver::solve: response at index " + std::to_string(i) +
+                                                   " must be finite, got " + std::to_string(v));
+            }
+        if (k == 0) return;
+        wx_lsmr_options o;
+        if (opts) o = *opts;
+        else wx_lsmr_options_default(&o);
+        if (o.local_size > 0)
+            fail(WX_ERR_INVALID_INPUT, "local_size (windowed reorthogonalization) is not supported on the device");
+        Stream st(nullptr);
+        DevBuf dY, dX, dD;
+        dY.alloc(n * k * 8);
+        for (uint64_t j = 0; j < k; ++j)
+            WX_HIP(hipMemcpyAsync(dY.as<double>() + j * n, Y + j * ldy, n * 8, hipMemcpyHostToDevice, st.s));
+        if (X) dX.alloc(N * k * 8);
+        if (demeaned) dD.alloc(n * k * 8);
+        solve_batch_device(s->s, dY.as<double>(), n, k, o, X ? dX.as<double>() : nullptr,
+                           demeaned ? dD.as<double>() : nullptr, stats, st.s);
+        if (X) WX_HIP(hipMemcpyAsync(X, dX.p, N * k * 8, hipMemcpyDeviceToHost, st.s));
+        if (demeaned) WX_HIP(hipMemcpyAsync(demeaned, dD.p, n * k * 8, hipMemcpyDeviceToHost, st.s));
+        WX_HIP(hipStreamSynchronize(st.s));
+    });
+}
+
+wx_status wx_solver_apply_operator(const wx_solver *s, int32_t adjoint, const double *X, uint64_t k, double *Y) {
+    return guard([&] {
+        if (!s || (k && (!X || !Y))) fail(WX_ERR_INVALID_INPUT, "null pointer");
+        if (k == 0) return;
+        const uint64_t n = s->s.design.n_obs, N = s->s.design.n_dofs;
+        const uint64_t rin = adjoint ? n : N, rout = adjoint ? N : n;
+        Stream st(nullptr);
+        DevBuf dx, dy;
+        dx.alloc(rin * k * 8);
+        dy.alloc(rout * k * 8);
+        WX_HIP(hipMemcpyAsync(dx.p, X, rin * k * 8, hipMemcpyHostToDevice, st.s));
+        design_apply_device(s->s, adjoint != 0, dx.as<double>(), k, dy.as<double>(), st.s);
+        WX_HIP(hipMemcpyAsync(Y, dy.p, rout * k * 8, hipMemcpyDeviceToHost, st.s));
+        WX_HIP(hipStreamSynchronize(st.s));
+    });
+}
+
+wx_status wx_solver_preconditioner(const wx_solver *s, wx_precond **out) {
+    return guard([&] {
+        if (!s || !out) fail(WX_ERR_INVALID_INPUT, "null pointer");
+        if (!s->s.precond) {
+            *out = nullptr;
+            return;
+        }
+        auto *p = new wx_precond();
+        p->p = s->s.precond;
+        *out = p;
+    });
+}
+
+uint64_t wx_solver_n_terms(const wx_solver *s) { return s ? s->s.design.terms.size() : 0; }
+
+void wx_solver_layout(const wx_solver *s, uint64_t *offset, uint64_t *n_levels, uint64_t *n_columns) {
+    if (!s) return;
+    const auto &T = s->s.design.terms;
+    for (size_t t = 0; t < T.size(); ++t) {
+        if (offset) offset[t] = T[t].offset;
+        if (n_levels) n_levels[t] = T[t].n_levels;
+        if (n_columns) n_columns[t] = T[t].ncols();
+    }
+}
+
+wx_status wx_solver_unidentified(const wx_solver *s, uint64_t *triples, uint64_t *count) {
+    return guard([&] {
+        if (!s || !count) fail(WX_ERR_INVALID_INPUT, "null pointer");
+        const uint64_t have = s->s.reparam ? s->s.reparam->unidentified.size() / 3 : 0;
+        if (triples) {
+            const uint64_t m = std::min(have, *count);
+            if (m) std::memcpy(triples, s->s.reparam->unidentified.data(), m * 3 * sizeof(uint64_t));
+        }
+        *count = have;
+    });
+}
+
+uint64_t wx_solver_n_warnings(const wx_solver *s) { return s ? s->s.warnings.size() : 0; }
+const char *wx_solver_warning(const wx_solver *s, uint64_t i) {
+    return (s && i < s->s.warnings.size()) ? s->s.warnings[i].c_str() : "";
+}
+
+void wx_precond_release(wx_precond *p) {
+    if (p && --p->refs == 0) delete p;
+}
+
+uint64_t wx_precond_n(const wx_precond *p) { return p ? p->p->n : 0; }
+
+const char *wx_precond_variant(const wx_precond *p) {
+    if (!p) return "";
+    return p->p->kind == Precond::ADDITIVE ? "Additive" : "Diagonal";
+}
+
+wx_status wx_precond_apply(const wx_precond *p, const double *x, uint64_t k, double *y) {
+    return guard([&] {
+        if (!p || (!x && k) || (!y && k)) fail(WX_ERR_INVALID_INPUT, "null pointer");
+        require_device();
+        const uint64_t n = p->p->n;
+        if (k == 0) return;
+        Stream st(nullptr);
+        DevBuf dx, dy;
+        dx.alloc(n * k * 8);
+        dy.alloc(n * k * 8);
+        WX_HIP(hipMemcpyAsync(dx.p, x, n * k * 8, hipMemcpyHostToDevice, st.s));
+        precond_apply_device(*p->p, dx.as<double>(), k, dy.as<double>(), st.s);
+        WX_HIP(hipMemcpyAsync(y, dy.p, n * k * 8, hipMemcpyDeviceToHost, st.s));
+        WX_HIP(hipStreamSynchronize(st.s));
+    });
+}
+
+wx_status wx_precond_serialize(const wx_precond *p, uint8_t **buf, uint64_t *len) {
+    return guard([&] {
+        if (!p || !buf || !len) fail(WX_ERR_INVALID_INPUT, "null pointer");
+        std::vector<uint8_t> v = serialize_precond(*p->p);
+        *buf = (uint8_t *)std::malloc(v.size() ? v.size() : 1);
+        if (!*buf) throw std::bad_alloc();
+        if (!v.empty()) std::memcpy(*buf, v.data(), v.size());
+        *len = v.size();
+    });
+}
+
+wx_status wx_precond_deserialize(const uint8_t *buf, uint64_t len, wx_precond **out) {
+    return guard([&] {
+        if (!buf || !out) fail(WX_ERR_INVALID_INPUT, "null pointer");
+        auto *p = new wx_precond();
+        try {
+            p->p = deserialize_precond(buf, len);
+        } catch (...) {
+            delete p;
+            throw;
+        }
+        *out = p;
+    });
+}
+
+void wx_buffer_free(uint8_t *buf) { std::free(buf); }
+
+} // extern "C"

@@ -1,0 +1,1065 @@
+// lsmr.hip -- device D / D' operators and the batched (modified) LSMR loop for gfx950.
+//
+// Vectors of a batch of k right-hand sides are interleaved: observation-space arrays are
+// n x k row-major (row i holds its k columns contiguously), DOF-space arrays N x k.  One
+// code load then serves all k columns and every access of a k-wide row is one coalesced
+// transaction.  Scalars of the LSMR recurrence live on the device, one lane per column,
+// so an iteration never waits for the host; the host polls the per-column "active" flags
+// every few iterations.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <numeric>
+
+#include "schwarz.hpp"
+#include "solver.hpp"
+
+namespace wx {
+
+namespace {
+
+constexpr int BLK = 256;
+constexpr uint32_t SEG_CHUNK = 256; // rows per D' work item before a level is split
+constexpr uint32_t MAX_PART_BLOCKS = 1024;
+
+// Per-column LSMR state (structure of arrays, each of length MAX_BATCH).
+struct ColState {
+    double *alpha, *beta, *beta_prev_inv, *beta_inv, *p_coeff, *gscale;
+    double *alpha_bar, *phi_bar, *c_bar, *s_bar, *zeta_bar, *zeta0, *a_norm_sq, *abs_tol;
+    double *prev_rho, *prev_rho_bar, *t_x, *t_hbar, *t_h, *inv_alpha, *bnorm_scale;
+    int32_t *active, *upd, *brk, *stop, *iters, *err;
+};
+constexpr int N_DOUBLE_FIELDS = 21;
+constexpr int N_INT_FIELDS = 6;
+
+__device__ inline double term_load(const DesignArgs &d, int32_t li, uint64_t i) {
+    return li < 0 ? 1.0 : d.slopes[(uint64_t)li * d.n + i];
+}
+
+// Sum of the k-lane block reduction: threads sharing (tid & (kp-1)) hold the same column.
+__device__ inline double block_reduce_cols(double v, uint32_t kp, double *sh) {
+    sh[threadIdx.x] = v;
+    __syncthreads();
+    for (uint32_t s = BLK / 2; s >= kp && s > 0; s >>= 1) {
+        if (threadIdx.x < s) sh[threadIdx.x] += sh[threadIdx.x + s];
+        __syncthreads();
+    }
+    return sh[threadIdx.x];
+}
+
+// ---------------------------------------------------------------------------
+// D v gather fused with the u update:  u <- sqrt(w) .* (D v) + gscale_j * u,  ||u||^2 partials.
+// The per-row sum follows crates/within/src/operator/design/gather.rs:10-78 (terms summed in
+// order from 0.0, each term folded from -0.0, weight applied after the terms) and the update
+// axpy_with_sq_norm crates/schwarz-precond/src/lsmr/bidiag.rs:24-43.
+// ---------------------------------------------------------------------------
+template <bool WEIGHTED>
+__global__ __launch_bounds__(BLK) void k_gather_axpy(DesignArgs d, const double *__restrict__ sw,
+                                                     const double *__restrict__ v, double *__restrict__ u,
+                                                     ColState cs, double *__restrict__ part, uint32_t k,
+                                                     uint32_t kshift) {
+    __shared__ double sh[BLK];
+    const uint32_t kp = 1u << kshift;
+    const uint32_t j = threadIdx.x & (kp - 1);
+    const uint64_t rstride = ((uint64_t)gridDim.x * BLK) >> kshift;
+    double sq = 0.0;
+    const bool act = j < k && cs.active[j];
+    if (act) {
+        const double scale = cs.gscale[j];
+        for (uint64_t i = ((uint64_t)blockIdx.x * BLK + threadIdx.x) >> kshift; i < d.n; i += rstride) {
+            double acc = 0.0;
+            for (uint32_t t = 0; t < d.T; ++t) {
+                const TermInfo &ti = d.terms[t];
+                const uint64_t code = d.codes[(uint64_t)t * d.n + i];
+                double tv = -0.0;
+                for (uint32_t c = 0; c < ti.ncols; ++c)
+                    tv = tv + v[(ti.offset + c * ti.n_levels + code) * k + j] * term_load(d, ti.load[c], i);
+                acc += tv;
+            }
+            if (WEIGHTED) acc *= sw[i];
+            const double nu = acc + scale * u[i * k + j];
+            u[i * k + j] = nu;
+            sq += nu * nu;
+        }
+    }
+    double tot = block_reduce_cols(sq, kp, sh);
+    if (threadIdx.x < kp) part[(uint64_t)blockIdx.x * kp + threadIdx.x] = tot;
+}
+
+// ---------------------------------------------------------------------------
+// D' reduction: out[off + c*L + l] = sum over rows of level l (code-sorted order) of
+// load_c(row) * scale(row) * src[row].  Deterministic: each item sums its rows in order,
+// split levels add their partials in slot order.  Replaces the strategies of
+// crates/within/src/operator/design/scatter.rs:14-242 (sequential / fold / CAS atomics /
+// run-coalesced atomics) with one atomic-free code-sorted segmented sum.
+// ---------------------------------------------------------------------------
+template <int NC, int SCALE> // SCALE: 0 none, 1 sqrt-weights, 2 weights (= scale^2)
+__global__ __launch_bounds__(BLK) void k_segreduce(DesignArgs d, TermInfo ti, const uint32_t *__restrict__ perm,
+                                                   const SegItem *__restrict__ items, uint32_t n_items,
+                                                   const double *__restrict__ scale, const double *__restrict__ src,
+                                                   double *__restrict__ out, double *__restrict__ partials,
+                                                   const int32_t *__restrict__ active, uint32_t k, uint32_t kshift) {
+    const uint32_t kp = 1u << kshift;
+    const uint32_t j = threadIdx.x & (kp - 1);
+    if (j >= k || (active && !active[j])) return;
+    const uint32_t groups = (gridDim.x * BLK) >> kshift;
+    for (uint32_t it = (blockIdx.x * BLK + threadIdx.x) >> kshift; it < n_items; it += groups) {
+        const SegItem item = items[it];
+        double acc[NC];
+#pragma unroll
+        for (int c = 0; c < NC; ++c) acc[c] = 0.0;
+        for (uint32_t p = item.begin; p < item.end; ++p) {
+            const uint64_t row = perm ? perm[p] : p;
+            double b = src[row * k + j];
+            if (SCALE == 1) b = scale[row] * b;
+            if (SCALE == 2) b = (scale[row] * scale[row]) * b;
+#pragma unroll
+            for (int c = 0; c < NC; ++c) acc[c] += (ti.load[c] < 0 ? b : term_load(d, ti.load[c], row) * b);
+        }
+        if (item.dest >= 0) {
+#pragma unroll
+            for (int c = 0; c < NC; ++c) out[(ti.offset + c * ti.n_levels + item.level) * k + j] = acc[c];
+        } else {
+            const uint64_t slot = (uint64_t)(-item.dest - 1);
+#pragma unroll
+            for (int c = 0; c < NC; ++c) partials[(slot * NC + c) * k + j] = acc[c];
+        }
+    }
+}
+
+template <int NC>
+__global__ void k_segfix(TermInfo ti, const SegFix *__restrict__ fixes, uint32_t n_fixes,
+                         const double *__restrict__ partials, double *__restrict__ out,
+                         const int32_t *__restrict__ active, uint32_t k, uint32_t kshift) {
+    const uint32_t kp = 1u << kshift;
+    const uint32_t j = threadIdx.x & (kp - 1);
+    if (j >= k || (active && !active[j])) return;
+    const uint32_t g = (blockIdx.x * BLK + threadIdx.x) >> kshift;
+    if (g >= n_fixes) return;
+    const SegFix f = fixes[g];
+#pragma unroll
+    for (int c = 0; c < NC; ++c) {
+        double acc = 0.0;
+        for (uint32_t q = 0; q < f.count; ++q) acc += partials[((uint64_t)(f.first + q) * NC + c) * k + j];
+        out[(ti.offset + c * ti.n_levels + f.level) * k + j] = acc;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Right-hand side loading and vec_norm (crates/schwarz-precond/src/lsmr.rs:24-36)
+// ---------------------------------------------------------------------------
+// u[i,j] = sqrt(w_i) * Y[caller(i), j]; per-block max|u| partials and a non-finite count.
+template <bool WEIGHTED>
+__global__ __launch_bounds__(BLK) void k_load_rhs(uint64_t n, const uint32_t *__restrict__ obs_perm,
+                                                  const double *__restrict__ sw, const double *__restrict__ Y,
+                                                  uint64_t ldy, double *__restrict__ u, double *__restrict__ part,
+                                                  int32_t *__restrict__ bad, uint32_t k, uint32_t kshift) {
+    __shared__ double sh[BLK];
+    const uint32_t kp = 1u << kshift;
+    const uint32_t j = threadIdx.x & (kp - 1);
+    const uint64_t rstride = ((uint64_t)gridDim.x * BLK) >> kshift;
+    double mx = 0.0;
+    if (j < k) {
+        for (uint64_t i = ((uint64_t)blockIdx.x * BLK + threadIdx.x) >> kshift; i < n; i += rstride) {
+            const uint64_t src = obs_perm ? obs_perm[i] : i;
+            const double y = Y[src + j * ldy];
+            if (!isfinite(y)) atomicAdd(bad, 1);
+            const double b = WEIGHTED ? sw[i] * y : y;
+            u[i * k + j] = b;
+            mx = fmax(mx, fabs(b));
+        }
+    }
+    sh[threadIdx.x] = mx;
+    __syncthreads();
+    for (uint32_t s = BLK / 2; s >= kp && s > 0; s >>= 1) {
+        if (threadIdx.x < s) sh[threadIdx.x] = fmax(sh[threadIdx.x], sh[threadIdx.x + s]);
+        __syncthreads();
+    }
+    if (threadIdx.x < kp) part[(uint64_t)blockIdx.x * kp + threadIdx.x] = sh[threadIdx.x];
+}
+
+// Per-column reduction of block partials (one block per column, fixed tree order).
+__device__ inline double reduce_partials(const double *part, uint32_t nb, uint32_t kp, uint32_t stride_fields,
+                                         uint32_t field, uint32_t j, double *sh, bool is_max) {
+    double acc = 0.0;
+    for (uint32_t b = threadIdx.x; b < nb; b += BLK) {
+        double v = part[((uint64_t)b * kp + j) * stride_fields + field];
+        acc = is_max ? fmax(acc, v) : acc + v;
+    }
+    sh[threadIdx.x] = acc;
+    __syncthreads();
+    for (uint32_t s = BLK / 2; s > 0; s >>= 1) {
+        if (threadIdx.x < s)
+            sh[threadIdx.x] = is_max ? fmax(sh[threadIdx.x], sh[threadIdx.x + s]) : sh[threadIdx.x] + sh[threadIdx.x + s];
+        __syncthreads();
+    }
+    double r = sh[0];
+    __syncthreads();
+    return r;
+}
+
+__global__ __launch_bounds__(BLK) void k_colmax(const double *part, uint32_t nb, uint32_t kp, ColState cs) {
+    __shared__ double sh[BLK];
+    const uint32_t j = blockIdx.x;
+    double m = reduce_partials(part, nb, kp, 1, 0, j, sh, true);
+    if (threadIdx.x == 0) cs.bnorm_scale[j] = m;
+}
+
+// sum((u/scale)^2) partials for vec_norm
+__global__ __launch_bounds__(BLK) void k_scaled_sumsq(uint64_t n, const double *__restrict__ u, ColState cs,
+                                                      double *__restrict__ part, uint32_t k, uint32_t kshift) {
+    __shared__ double sh[BLK];
+    const uint32_t kp = 1u << kshift;
+    const uint32_t j = threadIdx.x & (kp - 1);
+    const uint64_t rstride = ((uint64_t)gridDim.x * BLK) >> kshift;
+    double sq = 0.0;
+    if (j < k && cs.bnorm_scale[j] > 0.0) {
+        const double sc = cs.bnorm_scale[j];
+        for (uint64_t i = ((uint64_t)blockIdx.x * BLK + threadIdx.x) >> kshift; i < n; i += rstride) {
+            double q = u[i * k + j] / sc;
+            sq += q * q;
+        }
+    }
+    double tot = block_reduce_cols(sq, kp, sh);
+    if (threadIdx.x < kp) part[(uint64_t)blockIdx.x * kp + threadIdx.x] = tot;
+}
+
+// beta_1 = vec_norm(b); zero rhs -> stop ZeroRhs (lsmr.rs:266-282).
+__global__ __launch_bounds__(BLK) void k_beta1(const double *part, uint32_t nb, uint32_t kp, ColState cs, double tol,
+                                               uint32_t k) {
+    __shared__ double sh[BLK];
+    const uint32_t j = blockIdx.x;
+    double s = reduce_partials(part, nb, kp, 1, 0, j, sh, false);
+    if (threadIdx.x == 0) {
+        double sc = cs.bnorm_scale[j];
+        double beta = sc == 0.0 ? 0.0 : sc * sqrt(s);
+        cs.beta[j] = beta;
+        cs.abs_tol[j] = tol * beta;
+        cs.iters[j] = 0;
+        cs.err[j] = 0;
+        cs.brk[j] = 0;
+        cs.upd[j] = 0;
+        if (beta == 0.0) {
+            cs.active[j] = 0;
+            cs.stop[j] = WX_STOP_ZERO_RHS;
+            cs.beta_inv[j] = 0.0;
+        } else {
+            cs.active[j] = 1;
+            cs.stop[j] = -1;
+            cs.beta_inv[j] = 1.0 / beta;
+        }
+    }
+}
+
+__global__ __launch_bounds__(BLK) void k_scale_obs(uint64_t n, double *__restrict__ u, ColState cs, uint32_t k,
+                                                   uint32_t kshift) {
+    const uint32_t kp = 1u << kshift;
+    const uint32_t j = threadIdx.x & (kp - 1);
+    if (j >= k || !cs.active[j]) return;
+    const double s = cs.beta_inv[j];
+    const uint64_t rstride = ((uint64_t)gridDim.x * BLK) >> kshift;
+    for (uint64_t i = ((uint64_t)blockIdx.x * BLK + threadIdx.x) >> kshift; i < n; i += rstride) u[i * k + j] *= s;
+}
+
+// ---------------------------------------------------------------------------
+// DOF-space kernels
+// ---------------------------------------------------------------------------
+// PREC: 0 identity, 1 diagonal, 2 external (only p~ is written).  INIT: p~ already holds A'u.
+// p~ <- A'u * (1/beta) - (beta/alpha) p~   (bidiag.rs:475-485), v = M^{-1} p~, and partial
+// sums of <v,p~>, <v,v>, <p~,p~> for alpha_from_vp (bidiag.rs:99-111).
+template <int PREC, bool INIT>
+__global__ __launch_bounds__(BLK) void k_ptilde(uint64_t N, const double *__restrict__ atu, double *__restrict__ pt,
+                                                double *__restrict__ v, const double *__restrict__ inv_diag,
+                                                ColState cs, double *__restrict__ part, uint32_t k, uint32_t kshift) {
+    __shared__ double sh[BLK];
+    const uint32_t kp = 1u << kshift;
+    const uint32_t j = threadIdx.x & (kp - 1);
+    const uint64_t rstride = ((uint64_t)gridDim.x * BLK) >> kshift;
+    double vp = 0.0, vv = 0.0, pp = 0.0;
+    if (j < k && cs.active[j]) {
+        const bool brk = !INIT && cs.brk[j];
+        const double bi = INIT ? 1.0 : cs.beta_inv[j];
+        const double pc = INIT ? 0.0 : cs.p_coeff[j];
+        for (uint64_t dof = ((uint64_t)blockIdx.x * BLK + threadIdx.x) >> kshift; dof < N; dof += rstride) {
+            const uint64_t q = dof * k + j;
+            double p;
+            if (brk) p = 0.0;
+            else if (INIT) p = pt[q];
+            else p = bi * atu[q] + (-pc) * pt[q];
+            if (!INIT) pt[q] = p;
+            if (PREC != 2) {
+                const double z = brk ? 0.0 : (PREC == 1 ? inv_diag[dof] * p : p);
+                v[q] = z;
+                vp += z * p;
+                vv += z * z;
+                pp += p * p;
+            } else if (brk) {
+                v[q] = 0.0;
+            }
+        }
+    }
+    double a = block_reduce_cols(vp, kp, sh);
+    __syncthreads();
+    double b = block_reduce_cols(vv, kp, sh);
+    __syncthreads();
+    double c = block_reduce_cols(pp, kp, sh);
+    if (threadIdx.x < kp) {
+        double *o = part + ((uint64_t)blockIdx.x * kp + threadIdx.x) * 3;
+        o[0] = a;
+        o[1] = b;
+        o[2] = c;
+    }
+}
+
+// <v,p~>, <v,v>, <p~,p~> partials after an external preconditioner apply.
+__global__ __launch_bounds__(BLK) void k_dots(uint64_t N, const double *__restrict__ pt, const double *__restrict__ v,
+                                              ColState cs, double *__restrict__ part, uint32_t k, uint32_t kshift) {
+    __shared__ double sh[BLK];
+    const uint32_t kp = 1u << kshift;
+    const uint32_t j = threadIdx.x & (kp - 1);
+    const uint64_t rstride = ((uint64_t)gridDim.x * BLK) >> kshift;
+    double vp = 0.0, vv = 0.0, pp = 0.0;
+    if (j < k && cs.active[j]) {
+        for (uint64_t dof = ((uint64_t)blockIdx.x * BLK + threadIdx.x) >> kshift; dof < N; dof += rstride) {
+            const double z = v[dof * k + j], p = pt[dof * k + j];
+            vp += z * p;
+            vv += z * z;
+            pp += p * p;
+        }
+    }
+    double a = block_reduce_cols(vp, kp, sh);
+    __syncthreads();
+    double b = block_reduce_cols(vv, kp, sh);
+    __syncthreads();
+    double c = block_reduce_cols(pp, kp, sh);
+    if (threadIdx.x < kp) {
+        double *o = part + ((uint64_t)blockIdx.x * kp + threadIdx.x) * 3;
+        o[0] = a;
+        o[1] = b;
+        o[2] = c;
+    }
+}
+
+__device__ inline void givens(double a, double b, double &c, double &s, double &r) {
+    // crates/schwarz-precond/src/lsmr/recurrence.rs:20-27
+    r = hypot(a, b);
+    if (r > 0.0) {
+        c = a / r;
+        s = b / r;
+    } else {
+        c = 1.0;
+        s = 0.0;
+    }
+}
+
+__device__ inline double alpha_from_parts(double vp, double vv, double pp, bool &indefinite) {
+    // crates/schwarz-precond/src/lsmr/bidiag.rs:99-111
+    indefinite = false;
+    if (vp < 0.0) {
+        double bound = sqrt(2.220446049250313e-16) * sqrt(vv * pp);
+        if (vp < -bound) indefinite = true;
+    }
+    return sqrt(vp > 0.0 ? vp : 0.0);
+}
+
+// beta = ||u|| after the gather; breakdown flags and the p~ update coefficients.
+__global__ __launch_bounds__(BLK) void k_beta(const double *part, uint32_t nb, uint32_t kp, ColState cs) {
+    __shared__ double sh[BLK];
+    const uint32_t j = blockIdx.x;
+    double s = reduce_partials(part, nb, kp, 1, 0, j, sh, false);
+    if (threadIdx.x == 0 && cs.active[j]) {
+        double beta = sqrt(s);
+        cs.beta[j] = beta;
+        if (beta == 0.0) { // lucky breakdown (bidiag.rs:282-291)
+            cs.brk[j] = 1;
+            cs.beta_inv[j] = 0.0;
+            cs.p_coeff[j] = 0.0;
+        } else {
+            cs.brk[j] = 0;
+            cs.beta_inv[j] = 1.0 / beta;
+            cs.p_coeff[j] = beta / cs.alpha[j];
+        }
+    }
+}
+
+// First alpha (ModifiedGolubKahan::init, bidiag.rs:428-472) and the recurrence seed
+// (LsmrRecurrenceState::init, recurrence.rs:68-79; lsmr_from_bidiag early exit lsmr.rs:306-316).
+__global__ __launch_bounds__(BLK) void k_alpha1(const double *part, uint32_t nb, uint32_t kp, ColState cs) {
+    __shared__ double sh[BLK];
+    const uint32_t j = blockIdx.x;
+    double vp = reduce_partials(part, nb, kp, 3, 0, j, sh, false);
+    double vv = reduce_partials(part, nb, kp, 3, 1, j, sh, false);
+    double pp = reduce_partials(part, nb, kp, 3, 2, j, sh, false);
+    if (threadIdx.x == 0 && cs.active[j]) {
+        bool indef;
+        double alpha = alpha_from_parts(vp, vv, pp, indef);
+        if (indef) {
+            cs.err[j] = 1;
+            cs.active[j] = 0;
+            return;
+        }
+        double beta = cs.beta[j];
+        cs.alpha[j] = alpha;
+        cs.inv_alpha[j] = alpha > 0.0 ? 1.0 / alpha : 0.0;
+        cs.beta_prev_inv[j] = 1.0;
+        if (alpha == 0.0) {
+            cs.stop[j] = WX_STOP_INITIAL_NORMAL_EQ_ZERO;
+            cs.active[j] = 0;
+            cs.phi_bar[j] = beta;
+            cs.zeta_bar[j] = 0.0;
+            cs.zeta0[j] = 1.0;
+            return;
+        }
+        cs.alpha_bar[j] = alpha;
+        cs.phi_bar[j] = beta;
+        cs.c_bar[j] = 1.0;
+        cs.s_bar[j] = 0.0;
+        double zb = alpha * beta;
+        cs.zeta_bar[j] = zb;
+        cs.zeta0[j] = fmax(fabs(zb), 2.2250738585072014e-308);
+        cs.a_norm_sq[j] = alpha * alpha;
+        cs.prev_rho[j] = 1.0;
+        cs.prev_rho_bar[j] = 1.0;
+        cs.gscale[j] = -(alpha * 1.0);
+    }
+}
+
+// v <- v / alpha_1, h = v, x = 0, h_bar = 0 (SolutionState::init, recurrence.rs:134-141).
+__global__ __launch_bounds__(BLK) void k_init_solution(uint64_t N, double *__restrict__ v, double *__restrict__ h,
+                                                       double *__restrict__ hb, double *__restrict__ x, ColState cs,
+                                                       uint32_t k, uint32_t kshift) {
+    const uint32_t kp = 1u << kshift;
+    const uint32_t j = threadIdx.x & (kp - 1);
+    if (j >= k) return;
+    const bool act = cs.active[j];
+    const double ia = cs.inv_alpha[j];
+    const uint64_t rstride = ((uint64_t)gridDim.x * BLK) >> kshift;
+    for (uint64_t dof = ((uint64_t)blockIdx.x * BLK + threadIdx.x) >> kshift; dof < N; dof += rstride) {
+        const uint64_t q = dof * k + j;
+        x[q] = 0.0;
+        hb[q] = 0.0;
+        if (act) {
+            double z = v[q] * ia;
+            v[q] = z;
+            h[q] = z;
+        } else {
+            h[q] = 0.0;
+        }
+    }
+}
+
+// One LSMR scalar step per column: alpha from <v,p~>, the two Givens chains, the solution
+// coefficients and the dual stopping test.  Follows lsmr_from_bidiag (lsmr.rs:323-361),
+// LsmrRecurrenceState::step (recurrence.rs:81-107), SolutionState::update's coefficients
+// (recurrence.rs:143-161) and ConvergenceState::{observe, check} (recurrence.rs:239-255).
+__global__ __launch_bounds__(BLK) void k_alpha_step(const double *part, uint32_t nb, uint32_t kp, ColState cs,
+                                                    double rel_tol, int32_t itn) {
+    __shared__ double sh[BLK];
+    const uint32_t j = blockIdx.x;
+    double vp = reduce_partials(part, nb, kp, 3, 0, j, sh, false);
+    double vv = reduce_partials(part, nb, kp, 3, 1, j, sh, false);
+    double pp = reduce_partials(part, nb, kp, 3, 2, j, sh, false);
+    if (threadIdx.x != 0) return;
+    if (!cs.active[j]) {
+        cs.upd[j] = 0;
+        return;
+    }
+    double alpha, beta = cs.beta[j];
+    if (cs.brk[j]) {
+        alpha = 0.0;
+    } else {
+        bool indef;
+        alpha = alpha_from_parts(vp, vv, pp, indef);
+        if (indef) {
+            cs.err[j] = 1;
+            cs.active[j] = 0;
+            cs.upd[j] = 0;
+            cs.iters[j] = itn;
+            return;
+        }
+    }
+    cs.inv_alpha[j] = alpha > 0.0 ? 1.0 / alpha : 0.0;
+    // convergence.observe
+    double a_norm_sq = cs.a_norm_sq[j] + alpha * alpha + beta * beta;
+    cs.a_norm_sq[j] = a_norm_sq;
+    // recurrence.step
+    double ph_c, ph_s, ph_r;
+    givens(cs.alpha_bar[j], beta, ph_c, ph_s, ph_r);
+    const double theta_new = ph_s * alpha;
+    const double alpha_bar_new = -ph_c * alpha;
+    const double phi_bar_new = ph_s * cs.phi_bar[j];
+    const double theta_bar = cs.s_bar[j] * ph_r;
+    double pb_c, pb_s, pb_r;
+    givens(cs.c_bar[j] * ph_r, theta_new, pb_c, pb_s, pb_r);
+    const double zeta_bar = cs.zeta_bar[j];
+    const double zeta = pb_c * zeta_bar;
+    const double zeta_bar_new = -pb_s * zeta_bar;
+    cs.alpha_bar[j] = alpha_bar_new;
+    cs.phi_bar[j] = phi_bar_new;
+    cs.c_bar[j] = pb_c;
+    cs.s_bar[j] = pb_s;
+    cs.zeta_bar[j] = zeta_bar_new;
+    // solution.update coefficients
+    const double eps = 2.220446049250313e-16;
+    const double t_x_denom = ph_r * pb_r;
+    cs.t_x[j] = fabs(t_x_denom) > eps ? zeta / t_x_denom : 0.0;
+    const double t_hbar_denom = cs.prev_rho[j] * cs.prev_rho_bar[j];
+    cs.t_hbar[j] = fabs(t_hbar_denom) > eps ? theta_bar * ph_r / t_hbar_denom : 0.0;
+    cs.t_h[j] = fabs(ph_r) > eps ? theta_new / ph_r : 0.0;
+    cs.prev_rho[j] = ph_r;
+    cs.prev_rho_bar[j] = pb_r;
+    // bidiagonal state for the next step
+    cs.gscale[j] = -(alpha * cs.beta_inv[j]); // -(alpha_k / beta_k); 0 after a breakdown
+    cs.alpha[j] = alpha;
+    // convergence.check
+    const double residual = fabs(phi_bar_new);
+    int32_t stop = -1;
+    if (residual <= cs.abs_tol[j]) stop = WX_STOP_RESIDUAL_TOLERANCE;
+    else {
+        const double a_norm = fmax(sqrt(a_norm_sq), 2.2250738585072014e-308);
+        const double normar = fabs(zeta_bar_new);
+        if (normar / (a_norm * fmax(residual, 2.2250738585072014e-308)) <= rel_tol) stop = WX_STOP_NORMAL_EQ_TOLERANCE;
+    }
+    cs.upd[j] = 1;
+    cs.iters[j] = itn;
+    if (stop >= 0) {
+        cs.stop[j] = stop;
+        cs.active[j] = 0;
+    }
+}
+
+// x/h/h_bar update with the normalized v (recurrence.rs:163-190); v <- v / alpha first
+// (bidiag.rs:505-507).
+__global__ __launch_bounds__(BLK) void k_update_solution(uint64_t N, double *__restrict__ v, double *__restrict__ h,
+                                                         double *__restrict__ hb, double *__restrict__ x, ColState cs,
+                                                         uint32_t k, uint32_t kshift) {
+    const uint32_t kp = 1u << kshift;
+    const uint32_t j = threadIdx.x & (kp - 1);
+    if (j >= k || !cs.upd[j]) return;
+    const double ia = cs.inv_alpha[j], t_x = cs.t_x[j], t_hbar = cs.t_hbar[j], t_h = cs.t_h[j];
+    const bool scale = ia > 0.0;
+    const uint64_t rstride = ((uint64_t)gridDim.x * BLK) >> kshift;
+    for (uint64_t dof = ((uint64_t)blockIdx.x * BLK + threadIdx.x) >> kshift; dof < N; dof += rstride) {
+        const uint64_t q = dof * k + j;
+        double vq = v[q];
+        if (scale) {
+            vq *= ia;
+            v[q] = vq;
+        }
+        const double h_old = h[q];
+        const double hbq = h_old - t_hbar * hb[q];
+        hb[q] = hbq;
+        x[q] += t_x * hbq;
+        h[q] = vq - t_h * h_old;
+    }
+}
+
+// demeaned[caller(i), j] = y[caller(i), j] - (D x)[i, j]; the internal-order copy goes to
+// dm_int (n x k interleaved) for the residual check.  solver.rs:469-473, 483.
+__global__ __launch_bounds__(BLK) void k_demean(DesignArgs d, const uint32_t *__restrict__ obs_perm,
+                                                const double *__restrict__ x, const double *__restrict__ Y,
+                                                uint64_t ldy, double *__restrict__ Dm, uint64_t ldd,
+                                                double *__restrict__ dm_int, double *__restrict__ y_int, uint32_t k,
+                                                uint32_t kshift) {
+    const uint32_t kp = 1u << kshift;
+    const uint32_t j = threadIdx.x & (kp - 1);
+    if (j >= k) return;
+    const uint64_t rstride = ((uint64_t)gridDim.x * BLK) >> kshift;
+    for (uint64_t i = ((uint64_t)blockIdx.x * BLK + threadIdx.x) >> kshift; i < d.n; i += rstride) {
+        double acc = 0.0;
+        for (uint32_t t = 0; t < d.T; ++t) {
+            const TermInfo &ti = d.terms[t];
+            const uint64_t code = d.codes[(uint64_t)t * d.n + i];
+            double tv = -0.0;
+            for (uint32_t c = 0; c < ti.ncols; ++c)
+                tv = tv + x[(ti.offset + c * ti.n_levels + code) * k + j] * term_load(d, ti.load[c], i);
+            acc += tv;
+        }
+        const uint64_t src = obs_perm ? obs_perm[i] : i;
+        const double y = Y[src + j * ldy];
+        const double r = y - acc;
+        if (Dm) Dm[src + j * ldd] = r;
+        dm_int[i * k + j] = r;
+        y_int[i * k + j] = y;
+    }
+}
+
+// X[dof + j*ldx] = x[dof*k + j]
+__global__ void k_store_x(uint64_t N, const double *__restrict__ x, double *__restrict__ X, uint64_t ldx, uint32_t k) {
+    const uint64_t idx = (uint64_t)blockIdx.x * BLK + threadIdx.x;
+    if (idx >= N * k) return;
+    const uint64_t dof = idx / k, j = idx % k;
+    X[dof + j * ldx] = x[idx];
+}
+
+// sum of squares per column of an N x k interleaved array (block partials)
+__global__ __launch_bounds__(BLK) void k_sumsq(uint64_t N, const double *__restrict__ a, double *__restrict__ part,
+                                               uint32_t k, uint32_t kshift) {
+    __shared__ double sh[BLK];
+    const uint32_t kp = 1u << kshift;
+    const uint32_t j = threadIdx.x & (kp - 1);
+    const uint64_t rstride = ((uint64_t)gridDim.x * BLK) >> kshift;
+    double sq = 0.0;
+    if (j < k)
+        for (uint64_t dof = ((uint64_t)blockIdx.x * BLK + threadIdx.x) >> kshift; dof < N; dof += rstride) {
+            double q = a[dof * k + j];
+            sq += q * q;
+        }
+    double tot = block_reduce_cols(sq, kp, sh);
+    if (threadIdx.x < kp) part[(uint64_t)blockIdx.x * kp + threadIdx.x] = tot;
+}
+
+__global__ __launch_bounds__(BLK) void k_reduce_to(const double *part, uint32_t nb, uint32_t kp, double *outv) {
+    __shared__ double sh[BLK];
+    const uint32_t j = blockIdx.x;
+    double s = reduce_partials(part, nb, kp, 1, 0, j, sh, false);
+    if (threadIdx.x == 0) outv[j] = s;
+}
+
+uint32_t grid_for(uint64_t rows, uint32_t kshift, uint32_t cap = MAX_PART_BLOCKS) {
+    const uint64_t rows_per_block = BLK >> kshift;
+    uint64_t g = (rows + rows_per_block - 1) / rows_per_block;
+    if (g < 1) g = 1;
+    if (g > cap) g = cap;
+    return (uint32_t)g;
+}
+
+} // namespace
+
+DesignArgs DevDesign::args() const {
+    DesignArgs a{};
+    a.n = n;
+    a.T = (uint32_t)terms.size();
+    a.codes = codes.as<uint32_t>();
+    a.slopes = slopes.as<double>();
+    for (size_t t = 0; t < terms.size(); ++t) a.terms[t] = terms[t].info;
+    return a;
+}
+
+// ---------------------------------------------------------------------------
+// Design upload: codes, slopes, weights and the per-term D' segment plans
+// ---------------------------------------------------------------------------
+void upload_design(Solver &s, const std::vector<double> &weights_internal) {
+    const Design &d = s.design;
+    DevDesign &dv = s.dev;
+    if (d.terms.size() > MAX_TERMS)
+        fail(WX_ERR_INVALID_INPUT, "the device path supports at most " + std::to_string(MAX_TERMS) + " terms");
+    dv.n = d.n_obs;
+    dv.N = d.n_dofs;
+    dv.codes.upload(d.codes.data(), d.codes.size());
+    dv.slopes.upload(d.slopes.data(), d.slopes.size());
+    if (!weights_internal.empty()) {
+        std::vector<double> sw(weights_internal.size());
+        for (size_t i = 0; i < sw.size(); ++i) sw[i] = std::sqrt(weights_internal[i]);
+        dv.sw.upload(sw.data(), sw.size());
+        dv.w.upload(weights_internal.data(), weights_internal.size());
+    }
+    if (!d.obs_perm.empty()) dv.obs_perm.upload(d.obs_perm.data(), d.obs_perm.size());
+    dv.terms.clear();
+    dv.terms.resize(d.terms.size());
+    dv.max_partials = 0;
+    for (size_t t = 0; t < d.terms.size(); ++t) {
+        const Term &tm = d.terms[t];
+        DevTerm &dt = dv.terms[t];
+        if (tm.ncols() > MAX_COLS)
+            fail(WX_ERR_INVALID_INPUT, "the device path supports at most " + std::to_string(MAX_COLS) +
+                                           " coefficient columns per effect");
+        dt.info.n_levels = tm.n_levels;
+        dt.info.offset = tm.offset;
+        dt.info.ncols = (uint32_t)tm.ncols();
+        for (uint32_t c = 0; c < MAX_COLS; ++c) dt.info.load[c] = c < tm.ncols() ? (int32_t)tm.loads[c] : -1;
+        const uint32_t *col = d.level_column(t);
+        std::vector<uint64_t> start(tm.n_levels + 1, 0);
+        for (uint64_t i = 0; i < d.n_obs; ++i) start[col[i] + 1]++;
+        for (uint64_t l = 0; l < tm.n_levels; ++l) start[l + 1] += start[l];
+        if (!tm.sorted) {
+            std::vector<uint32_t> perm = stable_argsort(col, d.n_obs, tm.n_levels);
+            dt.perm.upload(perm.data(), perm.size());
+        }
+        std::vector<SegItem> items;
+        std::vector<SegFix> fixes;
+        uint32_t partials = 0;
+        items.reserve(tm.n_levels + d.n_obs / SEG_CHUNK + 1);
+        for (uint64_t l = 0; l < tm.n_levels; ++l) {
+            uint64_t b = start[l], e = start[l + 1];
+            if (e - b <= SEG_CHUNK) {
+                items.push_back({(uint32_t)l, (uint32_t)b, (uint32_t)e, (int32_t)l});
+            } else {
+                uint32_t first = partials;
+                uint32_t count = 0;
+                for (uint64_t p = b; p < e; p += SEG_CHUNK) {
+                    items.push_back({(uint32_t)l, (uint32_t)p, (uint32_t)std::min<uint64_t>(e, p + SEG_CHUNK),
+                                     -(int32_t)(partials + 1)});
+                    ++partials;
+                    ++count;
+                }
+                fixes.push_back({(uint32_t)l, first, count, 0});
+            }
+        }
+        dt.items.upload(items.data(), items.size());
+        dt.n_items = (uint32_t)items.size();
+        dt.fixes.upload(fixes.data(), fixes.size());
+        dt.n_fixes = (uint32_t)fixes.size();
+        dt.n_partials = partials;
+        dv.max_partials = std::max(dv.max_partials, partials * dt.info.ncols);
+    }
+}
+
+namespace {
+
+// Launches the D' reduction of every term: out (N x k) = D' (scale .* src).
+void launch_adjoint(const DevDesign &dv, const DesignArgs &da, const double *scale, int scale_mode,
+                    const double *src, double *out, double *partials, const int32_t *active, uint32_t k,
+                    uint32_t kshift, hipStream_t st) {
+    for (const DevTerm &dt : dv.terms) {
+        const uint32_t groups_per_block = BLK >> kshift;
+        uint32_t grid = (dt.n_items + groups_per_block - 1) / groups_per_block;
+        grid = std::max<uint32_t>(1, std::min<uint32_t>(grid, 65535u * 8));
+        const uint32_t *perm = dt.perm.p ? dt.perm.as<uint32_t>() : nullptr;
+#define WX_SEG(NC, SM)                                                                                           \
+    hipLaunchKernelGGL((k_segreduce<NC, SM>), dim3(grid), dim3(BLK), 0, st, da, dt.info, perm,                   \
+                       dt.items.as<SegItem>(), dt.n_items, scale, src, out, partials, active, k, kshift)
+#define WX_SEG_NC(SM)                                                                                            \
+    switch (dt.info.ncols) {                                                                                     \
+    case 1: WX_SEG(1, SM); break;                                                                                \
+    case 2: WX_SEG(2, SM); break;                                                                                \
+    case 3: WX_SEG(3, SM); break;                                                                                \
+    default: WX_SEG(4, SM); break;                                                                               \
+    }
+        if (scale_mode == 0 || !scale) { WX_SEG_NC(0) }
+        else if (scale_mode == 1) { WX_SEG_NC(1) }
+        else { WX_SEG_NC(2) }
+#undef WX_SEG_NC
+#undef WX_SEG
+        if (dt.n_fixes) {
+            const uint32_t fgrid = (dt.n_fixes + groups_per_block - 1) / groups_per_block;
+            switch (dt.info.ncols) {
+            case 1: hipLaunchKernelGGL((k_segfix<1>), dim3(fgrid), dim3(BLK), 0, st, dt.info, dt.fixes.as<SegFix>(), dt.n_fixes, partials, out, active, k, kshift); break;
+            case 2: hipLaunchKernelGGL((k_segfix<2>), dim3(fgrid), dim3(BLK), 0, st, dt.info, dt.fixes.as<SegFix>(), dt.n_fixes, partials, out, active, k, kshift); break;
+            case 3: hipLaunchKernelGGL((k_segfix<3>), dim3(fgrid), dim3(BLK), 0, st, dt.info, dt.fixes.as<SegFix>(), dt.n_fixes, partials, out, active, k, kshift); break;
+            default: hipLaunchKernelGGL((k_segfix<4>), dim3(fgrid), dim3(BLK), 0, st, dt.info, dt.fixes.as<SegFix>(), dt.n_fixes, partials, out, active, k, kshift); break;
+            }
+        }
+    }
+    WX_HIP(hipGetLastError());
+}
+
+struct Workspace {
+    DevBuf u, v, pt, atu, h, hb, x, part, segpart, cols, ints, scratch_y;
+    ColState cs{};
+};
+
+void setup_colstate(Workspace &ws) {
+    ws.cols.alloc(sizeof(double) * MAX_BATCH * N_DOUBLE_FIELDS);
+    ws.ints.alloc(sizeof(int32_t) * MAX_BATCH * N_INT_FIELDS);
+    WX_HIP(hipMemset(ws.cols.p, 0, ws.cols.bytes));
+    WX_HIP(hipMemset(ws.ints.p, 0, ws.ints.bytes));
+    double *dp = ws.cols.as<double>();
+    double **fields[N_DOUBLE_FIELDS] = {&ws.cs.alpha, &ws.cs.beta, &ws.cs.beta_prev_inv, &ws.cs.beta_inv,
+                                        &ws.cs.p_coeff, &ws.cs.gscale, &ws.cs.alpha_bar, &ws.cs.phi_bar,
+                                        &ws.cs.c_bar, &ws.cs.s_bar, &ws.cs.zeta_bar, &ws.cs.zeta0,
+                                        &ws.cs.a_norm_sq, &ws.cs.abs_tol, &ws.cs.prev_rho, &ws.cs.prev_rho_bar,
+                                        &ws.cs.t_x, &ws.cs.t_hbar, &ws.cs.t_h, &ws.cs.inv_alpha,
+                                        &ws.cs.bnorm_scale};
+    for (int f = 0; f < N_DOUBLE_FIELDS; ++f) *fields[f] = dp + f * MAX_BATCH;
+    int32_t *ip = ws.ints.as<int32_t>();
+    int32_t **ifields[N_INT_FIELDS] = {&ws.cs.active, &ws.cs.upd, &ws.cs.brk, &ws.cs.stop, &ws.cs.iters, &ws.cs.err};
+    for (int f = 0; f < N_INT_FIELDS; ++f) *ifields[f] = ip + f * MAX_BATCH;
+}
+
+} // namespace
+
+// ---------------------------------------------------------------------------
+// Batched solve (Solver::solve_batch, crates/within/src/solver.rs:423-577)
+// ---------------------------------------------------------------------------
+void solve_batch_device(const Solver &s, const double *Y, uint64_t ldy, uint64_t k_total, const wx_lsmr_options &opts,
+                        double *X, double *Dm, wx_rhs_stats *stats, hipStream_t st) {
+    const DevDesign &dv = s.dev;
+    const uint64_t n = dv.n, N = dv.N;
+    if (!std::isfinite(opts.tol) || opts.tol < 0.0)
+        fail(WX_ERR_INVALID_INPUT, "lsmr: tolerance must be finite and nonnegative, got " + std::to_string(opts.tol));
+    if (ldy < n) fail(WX_ERR_INVALID_INPUT, "leading dimension of Y is smaller than n_obs");
+    if (k_total == 0) return;
+    const DesignArgs da = dv.args();
+    const bool weighted = dv.sw.p != nullptr;
+    const uint32_t *obs_perm = dv.obs_perm.p ? dv.obs_perm.as<uint32_t>() : nullptr;
+    const int maxiter = (int)std::max<int64_t>(0, opts.maxiter);
+    const Precond *pc = s.precond.get();
+    const int prec_kind = pc == nullptr ? 0 : (pc->kind == Precond::DIAGONAL ? 1 : 2);
+    DevBuf inv_diag;
+    if (prec_kind == 1) inv_diag.upload(pc->inv_diag.data(), pc->inv_diag.size(), st);
+    std::unique_ptr<SchwarzRun> schwarz_run;
+
+    for (uint64_t c0 = 0; c0 < k_total; c0 += MAX_BATCH) {
+        const uint32_t k = (uint32_t)std::min<uint64_t>(MAX_BATCH, k_total - c0);
+        const uint32_t kshift = pow2_shift(k);
+        const uint32_t kp = 1u << kshift;
+        auto t0 = std::chrono::steady_clock::now();
+        Workspace ws;
+        setup_colstate(ws);
+        ws.u.alloc(n * k * 8);
+        ws.v.alloc(N * k * 8);
+        ws.pt.alloc(N * k * 8);
+        ws.atu.alloc(N * k * 8);
+        ws.h.alloc(N * k * 8);
+        ws.hb.alloc(N * k * 8);
+        ws.x.alloc(N * k * 8);
+        ws.part.alloc((size_t)MAX_PART_BLOCKS * kp * 3 * 8);
+        ws.segpart.alloc((size_t)std::max<uint32_t>(dv.max_partials, 1) * k * 8);
+        ColState cs = ws.cs;
+        if (prec_kind == 2 && !schwarz_run) schwarz_run = make_schwarz_run(*pc, opts.precision, st);
+        const uint32_t gn = grid_for(n, kshift), gN = grid_for(N, kshift);
+        double *part = ws.part.as<double>();
+        const double *Yc = Y + c0 * ldy;
+
+        // --- right-hand side: b = sqrt(W) y (internal order), beta_1 = vec_norm(b) ---
+        DevBuf bad;
+        bad.alloc(4);
+        WX_HIP(hipMemsetAsync(bad.p, 0, 4, st));
+        if (weighted)
+            hipLaunchKernelGGL((k_load_rhs<true>), dim3(gn), dim3(BLK), 0, st, n, obs_perm, dv.sw.as<double>(), Yc, ldy,
+                               ws.u.as<double>(), part, bad.as<int32_t>(), k, kshift);
+        else
+            hipLaunchKernelGGL((k_load_rhs<false>), dim3(gn), dim3(BLK), 0, st, n, obs_perm, nullptr, Yc, ldy,
+                               ws.u.as<double>(), part, bad.as<int32_t>(), k, kshift);
+        hipLaunchKernelGGL(k_colmax, dim3(k), dim3(BLK), 0, st, part, gn, kp, cs);
+        hipLaunchKernelGGL(k_scaled_sumsq, dim3(gn), dim3(BLK), 0, st, n, ws.u.as<double>(), cs, part, k, kshift);
+        hipLaunchKernelGGL(k_beta1, dim3(k), dim3(BLK), 0, st, part, gn, kp, cs, opts.tol, k);
+        int32_t nbad = 0;
+        WX_HIP(hipMemcpyAsync(&nbad, bad.p, 4, hipMemcpyDeviceToHost, st));
+        WX_HIP(hipStreamSynchronize(st));
+        if (nbad) fail(WX_ERR_INVALID_INPUT, "Solver::solve: response contains non-finite values");
+        hipLaunchKernelGGL(k_scale_obs, dim3(gn), dim3(BLK), 0, st, n, ws.u.as<double>(), cs, k, kshift);
+
+        // --- init: p~ = A'u, v = M^{-1} p~, alpha_1 ---
+        launch_adjoint(dv, da, weighted ? dv.sw.as<double>() : nullptr, weighted ? 1 : 0, ws.u.as<double>(),
+                       ws.pt.as<double>(), ws.segpart.as<double>(), cs.active, k, kshift, st);
+        if (prec_kind == 2) {
+            schwarz_run->apply(ws.pt.as<double>(), ws.v.as<double>(), k, st);
+            hipLaunchKernelGGL(k_dots, dim3(gN), dim3(BLK), 0, st, N, ws.pt.as<double>(), ws.v.as<double>(), cs, part, k, kshift);
+        } else if (prec_kind == 1) {
+            hipLaunchKernelGGL((k_ptilde<1, true>), dim3(gN), dim3(BLK), 0, st, N, nullptr, ws.pt.as<double>(),
+                               ws.v.as<double>(), inv_diag.as<double>(), cs, part, k, kshift);
+        } else {
+            hipLaunchKernelGGL((k_ptilde<0, true>), dim3(gN), dim3(BLK), 0, st, N, nullptr, ws.pt.as<double>(),
+                               ws.v.as<double>(), nullptr, cs, part, k, kshift);
+        }
+        hipLaunchKernelGGL(k_alpha1, dim3(k), dim3(BLK), 0, st, part, gN, kp, cs);
+        hipLaunchKernelGGL(k_init_solution, dim3(gN), dim3(BLK), 0, st, N, ws.v.as<double>(), ws.h.as<double>(),
+                           ws.hb.as<double>(), ws.x.as<double>(), cs, k, kshift);
+        WX_HIP(hipGetLastError());
+
+        // --- iterations ---
+        int32_t host_active[MAX_BATCH];
+        int check_every = opts.check_every > 0 ? opts.check_every : 4;
+        int itn_done = 0;
+        for (int itn = 1; itn <= maxiter; ++itn) {
+            if (weighted)
+                hipLaunchKernelGGL((k_gather_axpy<true>), dim3(gn), dim3(BLK), 0, st, da, dv.sw.as<double>(),
+                                   ws.v.as<double>(), ws.u.as<double>(), cs, part, k, kshift);
+            else
+                hipLaunchKernelGGL((k_gather_axpy<false>), dim3(gn), dim3(BLK), 0, st, da, nullptr, ws.v.as<double>(),
+                                   ws.u.as<double>(), cs, part, k, kshift);
+            hipLaunchKernelGGL(k_beta, dim3(k), dim3(BLK), 0, st, part, gn, kp, cs);
+            launch_adjoint(dv, da, weighted ? dv.sw.as<double>() : nullptr, weighted ? 1 : 0, ws.u.as<double>(),
+                           ws.atu.as<double>(), ws.segpart.as<double>(), cs.active, k, kshift, st);
+            if (prec_kind == 2) {
+                hipLaunchKernelGGL((k_ptilde<2, false>), dim3(gN), dim3(BLK), 0, st, N, ws.atu.as<double>(),
+                                   ws.pt.as<double>(), ws.v.as<double>(), nullptr, cs, part, k, kshift);
+                schwarz_run->apply(ws.pt.as<double>(), ws.v.as<double>(), k, st);
+                hipLaunchKernelGGL(k_dots, dim3(gN), dim3(BLK), 0, st, N, ws.pt.as<double>(), ws.v.as<double>(), cs,
+                                   part, k, kshift);
+            } else if (prec_kind == 1) {
+                hipLaunchKernelGGL((k_ptilde<1, false>), dim3(gN), dim3(BLK), 0, st, N, ws.atu.as<double>(),
+                                   ws.pt.as<double>(), ws.v.as<double>(), inv_diag.as<double>(), cs, part, k, kshift);
+            } else {
+                hipLaunchKernelGGL((k_ptilde<0, false>), dim3(gN), dim3(BLK), 0, st, N, ws.atu.as<double>(),
+                                   ws.pt.as<double>(), ws.v.as<double>(), nullptr, cs, part, k, kshift);
+            }
+            hipLaunchKernelGGL(k_alpha_step, dim3(k), dim3(BLK), 0, st, part, gN, kp, cs, opts.tol, itn);
+            hipLaunchKernelGGL(k_update_solution, dim3(gN), dim3(BLK), 0, st, N, ws.v.as<double>(), ws.h.as<double>(),
+                               ws.hb.as<double>(), ws.x.as<double>(), cs, k, kshift);
+            itn_done = itn;
+            if (itn % check_every == 0 || itn == maxiter) {
+                WX_HIP(hipGetLastError());
+                WX_HIP(hipMemcpyAsync(host_active, cs.active, k * 4, hipMemcpyDeviceToHost, st));
+                WX_HIP(hipStreamSynchronize(st));
+                bool any = false;
+                for (uint32_t j = 0; j < k; ++j) any |= host_active[j] != 0;
+                if (!any) break;
+                if (check_every < 32 && itn >= 8 * check_every) check_every *= 2;
+            }
+        }
+        (void)itn_done;
+
+        // --- outputs: x, demeaned (caller order), FP64 true residual ---
+        double *dm_int = ws.u.as<double>();
+        double *y_int = ws.atu.p && N >= n ? ws.atu.as<double>() : nullptr;
+        DevBuf yint_buf;
+        if (!y_int) {
+            yint_buf.alloc(n * k * 8);
+            y_int = yint_buf.as<double>();
+        }
+        const uint32_t gd = grid_for(n, kshift, 65535);
+        hipLaunchKernelGGL(k_demean, dim3(gd), dim3(BLK), 0, st, da, obs_perm, ws.x.as<double>(), Yc, ldy,
+                           Dm ? Dm + c0 * n : nullptr, n, dm_int, y_int, k, kshift);
+        if (X && !s.reparam) {
+            uint64_t tot = N * k;
+            hipLaunchKernelGGL(k_store_x, dim3((tot + BLK - 1) / BLK), dim3(BLK), 0, st, N, ws.x.as<double>(),
+                               X + c0 * N, N, k);
+        }
+        // residual numerator D'W(z - Dx) and denominator D'W z
+        const double *wv = weighted ? dv.w.as<double>() : nullptr;
+        launch_adjoint(dv, da, wv, weighted ? 1 : 0, dm_int, ws.h.as<double>(), ws.segpart.as<double>(), nullptr, k,
+                       kshift, st);
+        launch_adjoint(dv, da, wv, weighted ? 1 : 0, y_int, ws.hb.as<double>(), ws.segpart.as<double>(), nullptr, k,
+                       kshift, st);
+        double num[MAX_BATCH], den[MAX_BATCH];
+        DevBuf red;
+        red.alloc(2 * MAX_BATCH * 8);
+        hipLaunchKernelGGL(k_sumsq, dim3(gN), dim3(BLK), 0, st, N, ws.h.as<double>(), part, k, kshift);
+        hipLaunchKernelGGL(k_reduce_to, dim3(k), dim3(BLK), 0, st, part, gN, kp, red.as<double>());
+        hipLaunchKernelGGL(k_sumsq, dim3(gN), dim3(BLK), 0, st, N, ws.hb.as<double>(), part, k, kshift);
+        hipLaunchKernelGGL(k_reduce_to, dim3(k), dim3(BLK), 0, st, part, gN, kp, red.as<double>() + MAX_BATCH);
+        WX_HIP(hipGetLastError());
+        double host_d[N_DOUBLE_FIELDS * MAX_BATCH];
+        int32_t host_i[N_INT_FIELDS * MAX_BATCH];
+        WX_HIP(hipMemcpyAsync(num, red.as<double>(), MAX_BATCH * 8, hipMemcpyDeviceToHost, st));
+        WX_HIP(hipMemcpyAsync(den, red.as<double>() + MAX_BATCH, MAX_BATCH * 8, hipMemcpyDeviceToHost, st));
+        WX_HIP(hipMemcpyAsync(host_d, ws.cols.p, sizeof(host_d), hipMemcpyDeviceToHost, st));
+        WX_HIP(hipMemcpyAsync(host_i, ws.ints.p, sizeof(host_i), hipMemcpyDeviceToHost, st));
+        WX_HIP(hipStreamSynchronize(st));
+        if (X && s.reparam) {
+            std::vector<double> xi(N * k), xo(N);
+            WX_HIP(hipMemcpy(xi.data(), ws.x.p, N * k * 8, hipMemcpyDeviceToHost));
+            for (uint32_t j = 0; j < k; ++j) {
+                for (uint64_t q = 0; q < N; ++q) xo[q] = xi[q * k + j];
+                s.reparam->back_transform(s.design, xo.data());
+                WX_HIP(hipMemcpy(X + (c0 + j) * N, xo.data(), N * 8, hipMemcpyHostToDevice));
+            }
+        }
+        auto t1 = std::chrono::steady_clock::now();
+        const double secs = std::chrono::duration<double>(t1 - t0).count();
+        const double *phi_bar = host_d + 7 * MAX_BATCH, *zeta_bar = host_d + 10 * MAX_BATCH,
+                     *zeta0 = host_d + 11 * MAX_BATCH;
+        const int32_t *active = host_i, *stop = host_i + 3 * MAX_BATCH, *iters = host_i + 4 * MAX_BATCH,
+                      *err = host_i + 5 * MAX_BATCH;
+        for (uint32_t j = 0; j < k; ++j) {
+            if (err[j])
+                fail(WX_ERR_NOT_POSITIVE_DEF, "mlsmr: preconditioner not positive definite (<v, Mv> < 0)");
+            wx_rhs_stats &o = stats[c0 + j];
+            int32_t sr = stop[j];
+            if (active[j] || sr < 0) {
+                sr = WX_STOP_MAX_ITERATIONS;
+                o.converged = 0;
+                o.iterations = maxiter;
+            } else {
+                o.converged = 1;
+                o.iterations = iters[j];
+            }
+            o.stop_reason = sr;
+            if (sr == WX_STOP_ZERO_RHS) {
+                o.residual = 0.0;
+                o.residual_norm = 0.0;
+            } else if (sr == WX_STOP_INITIAL_NORMAL_EQ_ZERO) {
+                o.residual = 0.0;
+                o.residual_norm = phi_bar[j];
+            } else {
+                o.residual = std::fabs(zeta_bar[j]) / zeta0[j];
+                o.residual_norm = std::fabs(phi_bar[j]);
+            }
+            o.true_residual = den[j] > 0.0 ? std::sqrt(num[j]) / std::sqrt(den[j]) : 0.0;
+            o.time_solve = secs / k;
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Standalone operator applies (the Operator seam, crates/schwarz-precond/src/lib.rs:48-57,
+// implemented by DesignOperator, crates/within/src/operator/design.rs:92-117), in caller row order.
+// ---------------------------------------------------------------------------
+namespace {
+// xi[d*k + j] = X[d + j*ld]
+__global__ void k_interleave(uint64_t rows, const double *__restrict__ X, uint64_t ld,
+                             const uint32_t *__restrict__ perm, double *__restrict__ xi, uint32_t k) {
+    const uint64_t idx = (uint64_t)blockIdx.x * BLK + threadIdx.x;
+    if (idx >= rows * k) return;
+    const uint64_t r = idx / k, j = idx % k;
+    xi[idx] = X[(perm ? perm[r] : r) + j * ld];
+}
+// Y[perm(r) + j*ld] = yi[r*k + j]
+__global__ void k_deinterleave(uint64_t rows, const double *__restrict__ yi, double *__restrict__ Y, uint64_t ld,
+                               const uint32_t *__restrict__ perm, uint32_t k) {
+    const uint64_t idx = (uint64_t)blockIdx.x * BLK + threadIdx.x;
+    if (idx >= rows * k) return;
+    const uint64_t r = idx / k, j = idx % k;
+    Y[(perm ? perm[r] : r) + j * ld] = yi[idx];
+}
+// y = sqrt(w) .* (D x), no update term
+template <bool WEIGHTED>
+__global__ __launch_bounds__(BLK) void k_gather_plain(DesignArgs d, const double *__restrict__ sw,
+                                                      const double *__restrict__ x, double *__restrict__ y, uint32_t k,
+                                                      uint32_t kshift) {
+    const uint32_t kp = 1u << kshift;
+    const uint32_t j = threadIdx.x & (kp - 1);
+    if (j >= k) return;
+    const uint64_t rstride = ((uint64_t)gridDim.x * BLK) >> kshift;
+    for (uint64_t i = ((uint64_t)blockIdx.x * BLK + threadIdx.x) >> kshift; i < d.n; i += rstride) {
+        double acc = 0.0;
+        for (uint32_t t = 0; t < d.T; ++t) {
+            const TermInfo &ti = d.terms[t];
+            const uint64_t code = d.codes[(uint64_t)t * d.n + i];
+            double tv = -0.0;
+            for (uint32_t c = 0; c < ti.ncols; ++c)
+                tv = tv + x[(ti.offset + c * ti.n_levels + code) * k + j] * term_load(d, ti.load[c], i);
+            acc += tv;
+        }
+        if (WEIGHTED) acc *= sw[i];
+        y[i * k + j] = acc;
+    }
+}
+} // namespace
+
+void design_apply_device(const Solver &s, bool adjoint, const double *X, uint64_t k_total, double *Y,
+                         hipStream_t st) {
+    const DevDesign &dv = s.dev;
+    const uint64_t n = dv.n, N = dv.N;
+    const DesignArgs da = dv.args();
+    const bool weighted = dv.sw.p != nullptr;
+    const uint32_t *obs_perm = dv.obs_perm.p ? dv.obs_perm.as<uint32_t>() : nullptr;
+    for (uint64_t c0 = 0; c0 < k_total; c0 += MAX_BATCH) {
+        const uint32_t k = (uint32_t)std::min<uint64_t>(MAX_BATCH, k_total - c0);
+        const uint32_t kshift = pow2_shift(k);
+        DevBuf in, out, part;
+        in.alloc((adjoint ? n : N) * k * 8);
+        out.alloc((adjoint ? N : n) * k * 8);
+        part.alloc((uint64_t)std::max<uint32_t>(dv.max_partials, 1) * k * 8);
+        if (!adjoint) {
+            hipLaunchKernelGGL(k_interleave, dim3((N * k + BLK - 1) / BLK), dim3(BLK), 0, st, N, X + c0 * N, N,
+                               nullptr, in.as<double>(), k);
+            const uint32_t g = grid_for(n, kshift, 65535);
+            if (weighted)
+                hipLaunchKernelGGL((k_gather_plain<true>), dim3(g), dim3(BLK), 0, st, da, dv.sw.as<double>(),
+                                   in.as<double>(), out.as<double>(), k, kshift);
+            else
+                hipLaunchKernelGGL((k_gather_plain<false>), dim3(g), dim3(BLK), 0, st, da, nullptr, in.as<double>(),
+                                   out.as<double>(), k, kshift);
+            hipLaunchKernelGGL(k_deinterleave, dim3((n * k + BLK - 1) / BLK), dim3(BLK), 0, st, n, out.as<double>(),
+                               Y + c0 * n, n, obs_perm, k);
+        } else {
+            hipLaunchKernelGGL(k_interleave, dim3((n * k + BLK - 1) / BLK), dim3(BLK), 0, st, n, X + c0 * n, n,
+                               obs_perm, in.as<double>(), k);
+            launch_adjoint(dv, da, weighted ? dv.sw.as<double>() : nullptr, weighted ? 1 : 0, in.as<double>(),
+                           out.as<double>(), part.as<double>(), nullptr, k, kshift, st);
+            hipLaunchKernelGGL(k_deinterleave, dim3((N * k + BLK - 1) / BLK), dim3(BLK), 0, st, N, out.as<double>(),
+                               Y + c0 * N, N, nullptr, k);
+        }
+        WX_HIP(hipGetLastError());
+        WX_HIP(hipStreamSynchronize(st));
+    }
+}
+
+} // namespace wx

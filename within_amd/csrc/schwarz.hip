@@ -1,0 +1,546 @@
+// schwarz.hip -- device apply of the additive Schwarz preconditioner (gfx950).
+//
+// z = sum_c R_c' W_c A_c^+ W_c R_c r over factor-pair components, for k interleaved columns.
+// The local solve of a component follows crates/within/src/block_elim/solver.rs:350-466 with
+// the subdomain wrapper crates/schwarz-precond/src/local_solve.rs:106-147:
+//   restrict + fold (negate the kept block) -> subtract the component mean -> eliminate the
+//   larger block (1/D_e scaling, C' SpMV into the kept block) -> kept-block solve ->
+//   back-substitution x_e = D_e^{-1}(b_e + C x_k) -> subtract the mean -> unfold -> prolong.
+// The combine is a deterministic gather over a DOF -> slot map (no atomics), replacing the
+// reference's atomic / per-worker-buffer reductions (schwarz/executor.rs:25-165).
+// Coefficients are stored in the precision of the run (double, or float in mixed mode);
+// every vector and every accumulation is FP64, so the operator stays exactly linear and
+// symmetric up to FP64 rounding of fixed coefficients.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <numeric>
+
+#include "schwarz.hpp"
+
+namespace wx {
+
+namespace {
+
+template <typename T> inline T *dp(const DevBuf &b) { return static_cast<T *>(b.p); }
+
+constexpr int BLK = 256;
+constexpr uint32_t ROW_CHUNK = 256;   // CSR entries per work item before a row is split
+constexpr uint32_t MEAN_CHUNK = 256;  // slots per partial-sum chunk
+
+struct RowItem {
+    uint32_t row;   // slot id of the output row
+    uint32_t pad;
+    uint64_t begin, end; // CSR entry range
+    int64_t dest;   // >= 0 final row, < 0 partial slot -dest-1
+};
+struct RowFix {
+    uint32_t row;
+    uint32_t count;
+    uint64_t first;
+};
+struct Chunk {
+    uint32_t comp;
+    uint32_t begin, end; // slot range
+    uint32_t pad;
+};
+
+template <typename CT>
+struct SchwarzDev {
+    uint64_t ns = 0, nc = 0, N = 0;
+    DevBuf glob, sgnpw, inv_de, slot_comp;
+    // C' over kept rows (entry index = eliminated slot) and C over eliminated rows (entry = kept slot)
+    DevBuf ct_idx, ct_val, ct_items, ct_fix;
+    uint32_t n_ct_items = 0, n_ct_fix = 0, n_ct_part = 0;
+    DevBuf c_idx, c_val, c_items, c_fix;
+    uint32_t n_c_items = 0, n_c_fix = 0, n_c_part = 0;
+    // means: chunks over all slots (per component) and over kept slots of sparse components
+    DevBuf all_chunks, all_cptr, kept_chunks, kept_cptr;
+    uint32_t n_all_chunks = 0, n_kept_chunks = 0;
+    DevBuf comp_count, comp_kcount;
+    // dense kept blocks: one entry per kept row: (matrix row offset, first kept slot, nk)
+    DevBuf drow_off, drow_base, drow_nk, drow_slot, dmat;
+    uint32_t n_drows = 0;
+    // sparse kept blocks (one workgroup per component): levels and factor rows
+    DevBuf sp_comp, sp_lvl_f_ptr, sp_lvl_b_ptr, sp_lvl_f_rows, sp_lvl_b_rows, sp_meta;
+    DevBuf l_ptr, l_idx, l_val, lt_ptr, lt_idx, lt_val, piv;
+    uint32_t n_sparse = 0;
+    // prolongation: DOF -> slots
+    DevBuf d_ptr, d_slot;
+    bool has_sparse = false;
+};
+
+// per-sparse-component metadata
+struct SpMeta {
+    uint32_t comp;
+    uint32_t kept_base; // first kept slot
+    uint32_t nk;
+    uint32_t nlev_f;
+    uint32_t nlev_b;
+    uint32_t pad;
+    uint64_t lvl_f_off; // into lvl_f_ptr
+    uint64_t lvl_b_off;
+};
+
+// ---------------------------------------------------------------------------
+// kernels (kp lanes per row; lane j = RHS column)
+// ---------------------------------------------------------------------------
+template <typename CT>
+__global__ __launch_bounds__(BLK) void sk_restrict(uint64_t ns, const uint32_t *__restrict__ glob,
+                                                   const CT *__restrict__ sgnpw, const double *__restrict__ r,
+                                                   double *__restrict__ loc, uint32_t k, uint32_t kshift) {
+    const uint32_t kp = 1u << kshift, j = threadIdx.x & (kp - 1);
+    if (j >= k) return;
+    const uint64_t stride = ((uint64_t)gridDim.x * BLK) >> kshift;
+    for (uint64_t s = ((uint64_t)blockIdx.x * BLK + threadIdx.x) >> kshift; s < ns; s += stride)
+        loc[s * k + j] = (double)sgnpw[s] * r[(uint64_t)glob[s] * k + j];
+}
+
+// partial sums of src over chunks of slots
+__global__ __launch_bounds__(BLK) void sk_chunk_sum(const Chunk *__restrict__ chunks, uint32_t n_chunks,
+                                                    const double *__restrict__ src, double *__restrict__ part,
+                                                    uint32_t k, uint32_t kshift) {
+    const uint32_t kp = 1u << kshift, j = threadIdx.x & (kp - 1);
+    if (j >= k) return;
+    const uint32_t g = (blockIdx.x * BLK + threadIdx.x) >> kshift;
+    if (g >= n_chunks) return;
+    const Chunk ch = chunks[g];
+    double acc = 0.0;
+    for (uint32_t s = ch.begin; s < ch.end; ++s) acc += src[(uint64_t)s * k + j];
+    part[(uint64_t)g * k + j] = acc;
+}
+
+// mean per component = sum of its chunk partials / count
+__global__ __launch_bounds__(BLK) void sk_comp_mean(uint64_t nc, const uint32_t *__restrict__ cptr,
+                                                    const double *__restrict__ part,
+                                                    const uint32_t *__restrict__ count, double *__restrict__ mu,
+                                                    uint32_t k, uint32_t kshift) {
+    const uint32_t kp = 1u << kshift, j = threadIdx.x & (kp - 1);
+    if (j >= k) return;
+    const uint64_t c = ((uint64_t)blockIdx.x * BLK + threadIdx.x) >> kshift;
+    if (c >= nc) return;
+    double acc = 0.0;
+    for (uint32_t q = cptr[c]; q < cptr[c + 1]; ++q) acc += part[(uint64_t)q * k + j];
+    const uint32_t n = count[c];
+    mu[c * k + j] = n ? acc / (double)n : 0.0;
+}
+
+// MODE 0: reduced rhs of kept rows:  red[row] = (loc[row]-mu) + sum_p ct[p]*inv_de[e]*(loc[e]-mu)
+// MODE 1: back-substitution of eliminated rows: xs[row] = inv_de[row]*((loc[row]-mu) + sum_p c[p]*xs[kept])
+template <typename CT, int MODE>
+__device__ inline double row_term(const uint32_t *idx, const CT *val, uint64_t p, const double *loc,
+                                  const double *xs, const CT *inv_de, const double *mu_c, uint32_t k, uint32_t j) {
+    const uint64_t e = idx[p];
+    if (MODE == 0) return (double)val[p] * ((double)inv_de[e] * (loc[e * k + j] - mu_c[j]));
+    return (double)val[p] * xs[e * k + j];
+}
+
+template <typename CT, int MODE>
+__device__ inline void row_finish(uint32_t row, double sum, const double *loc, double *out, const CT *inv_de,
+                                  const double *mu_c, uint32_t k, uint32_t j) {
+    const double b = loc[(uint64_t)row * k + j] - mu_c[j];
+    if (MODE == 0) out[(uint64_t)row * k + j] = b + sum;
+    else {
+        const double id = (double)inv_de[row];
+        out[(uint64_t)row * k + j] = id * b + id * sum;
+    }
+}
+
+template <typename CT, int MODE>
+__global__ __launch_bounds__(BLK) void sk_rows(const RowItem *__restrict__ items, uint32_t n_items,
+                                               const uint32_t *__restrict__ idx, const CT *__restrict__ val,
+                                               const uint32_t *__restrict__ slot_comp, const double *__restrict__ loc,
+                                               const double *__restrict__ xs, const CT *__restrict__ inv_de,
+                                               const double *__restrict__ mu, double *__restrict__ out,
+                                               double *__restrict__ part, uint32_t k, uint32_t kshift) {
+    const uint32_t kp = 1u << kshift, j = threadIdx.x & (kp - 1);
+    if (j >= k) return;
+    const uint32_t groups = (gridDim.x * BLK) >> kshift;
+    for (uint32_t it = (blockIdx.x * BLK + threadIdx.x) >> kshift; it < n_items; it += groups) {
+        const RowItem item = items[it];
+        const double *mu_c = mu + (uint64_t)slot_comp[item.row] * k;
+        double acc = 0.0;
+        for (uint64_t p = item.begin; p < item.end; ++p)
+            acc += row_term<CT, MODE>(idx, val, p, loc, xs, inv_de, mu_c, k, j);
+        if (item.dest >= 0) row_finish<CT, MODE>(item.row, acc, loc, out, inv_de, mu_c, k, j);
+        else part[(uint64_t)(-item.dest - 1) * k + j] = acc;
+    }
+}
+
+template <typename CT, int MODE>
+__global__ __launch_bounds__(BLK) void sk_rows_fix(const RowFix *__restrict__ fixes, uint32_t n_fix,
+                                                   const uint32_t *__restrict__ slot_comp,
+                                                   const double *__restrict__ loc, const CT *__restrict__ inv_de,
+                                                   const double *__restrict__ mu, const double *__restrict__ part,
+                                                   double *__restrict__ out, uint32_t k, uint32_t kshift) {
+    const uint32_t kp = 1u << kshift, j = threadIdx.x & (kp - 1);
+    if (j >= k) return;
+    const uint32_t g = (blockIdx.x * BLK + threadIdx.x) >> kshift;
+    if (g >= n_fix) return;
+    const RowFix f = fixes[g];
+    double acc = 0.0;
+    for (uint32_t q = 0; q < f.count; ++q) acc += part[(f.first + q) * k + j];
+    row_finish<CT, MODE>(f.row, acc, loc, out, inv_de, mu + (uint64_t)slot_comp[f.row] * k, k, j);
+}
+
+// Dense kept-block solve: xs[row] = sum_l S^+[i, l] * red[l]   (S^+ = P S_g^{-1} P)
+template <typename CT>
+__global__ __launch_bounds__(BLK) void sk_dense(uint32_t n_rows, const uint64_t *__restrict__ row_off,
+                                                const uint32_t *__restrict__ row_base,
+                                                const uint32_t *__restrict__ row_nk,
+                                                const uint32_t *__restrict__ row_slot, const CT *__restrict__ mat,
+                                                const double *__restrict__ red, double *__restrict__ xs, uint32_t k,
+                                                uint32_t kshift) {
+    const uint32_t kp = 1u << kshift, j = threadIdx.x & (kp - 1);
+    if (j >= k) return;
+    const uint32_t groups = (gridDim.x * BLK) >> kshift;
+    for (uint32_t r = (blockIdx.x * BLK + threadIdx.x) >> kshift; r < n_rows; r += groups) {
+        const CT *m = mat + row_off[r];
+        const uint64_t base = row_base[r];
+        const uint32_t nk = row_nk[r];
+        double acc = 0.0;
+        for (uint32_t l = 0; l < nk; ++l) acc += (double)m[l] * red[(base + l) * k + j];
+        xs[(uint64_t)row_slot[r] * k + j] = acc;
+    }
+}
+
+// z[dof] = sum over the DOF's slots of sgnpw[s] * (xs[s] - mu3[comp(s)]); uncovered DOFs -> 0.
+template <typename CT>
+__global__ __launch_bounds__(BLK) void sk_prolong(uint64_t N, const uint64_t *__restrict__ d_ptr,
+                                                  const uint32_t *__restrict__ d_slot, const CT *__restrict__ sgnpw,
+                                                  const uint32_t *__restrict__ slot_comp,
+                                                  const double *__restrict__ xs, const double *__restrict__ mu3,
+                                                  double *__restrict__ z, uint32_t k, uint32_t kshift) {
+    const uint32_t kp = 1u << kshift, j = threadIdx.x & (kp - 1);
+    if (j >= k) return;
+    const uint64_t stride = ((uint64_t)gridDim.x * BLK) >> kshift;
+    for (uint64_t d = ((uint64_t)blockIdx.x * BLK + threadIdx.x) >> kshift; d < N; d += stride) {
+        double acc = 0.0;
+        for (uint64_t q = d_ptr[d]; q < d_ptr[d + 1]; ++q) {
+            const uint64_t s = d_slot[q];
+            acc += (double)sgnpw[s] * (xs[s * k + j] - mu3[(uint64_t)slot_comp[s] * k + j]);
+        }
+        z[d * k + j] = acc;
+    }
+}
+
+uint32_t grid_rows(uint64_t rows, uint32_t kshift, uint32_t cap = 65535) {
+    const uint64_t per = BLK >> kshift;
+    uint64_t g = (rows + per - 1) / per;
+    return (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(g, cap));
+}
+
+template <typename T, typename S> std::vector<T> cast_vec(const std::vector<S> &v) {
+    std::vector<T> o(v.size());
+    for (size_t i = 0; i < v.size(); ++i) o[i] = (T)v[i];
+    return o;
+}
+
+// Splits CSR rows into work items of at most ROW_CHUNK entries.
+void make_items(const std::vector<uint32_t> &rows, const std::vector<uint64_t> &begin,
+                const std::vector<uint64_t> &end, std::vector<RowItem> &items, std::vector<RowFix> &fixes,
+                uint32_t &n_part) {
+    n_part = 0;
+    for (size_t r = 0; r < rows.size(); ++r) {
+        const uint64_t b = begin[r], e = end[r];
+        if (e - b <= ROW_CHUNK) {
+            items.push_back({rows[r], 0, b, e, (int64_t)rows[r]});
+        } else {
+            RowFix f{rows[r], 0, n_part};
+            for (uint64_t p = b; p < e; p += ROW_CHUNK) {
+                items.push_back({rows[r], 0, p, std::min<uint64_t>(e, p + ROW_CHUNK), -(int64_t)(n_part + 1)});
+                ++n_part;
+                ++f.count;
+            }
+            fixes.push_back(f);
+        }
+    }
+}
+
+template <typename CT>
+std::shared_ptr<SchwarzDev<CT>> build_dev(const Precond::Schwarz &S) {
+    auto D = std::make_shared<SchwarzDev<CT>>();
+    const uint64_t ns = S.n_slots(), nc = S.n_comp();
+    D->ns = ns;
+    D->nc = nc;
+    D->N = S.n_dofs;
+    if (ns >= 0xFFFFFFFFull) fail(WX_ERR_INVALID_INPUT, "too many Schwarz slots for the device path");
+    std::vector<uint32_t> slot_comp(ns);
+    std::vector<CT> sgnpw(ns);
+    for (uint64_t c = 0; c < nc; ++c)
+        for (uint64_t s = S.slot_start[c]; s < S.slot_start[c + 1]; ++s) {
+            slot_comp[s] = (uint32_t)c;
+            const bool kept = s >= S.slot_start[c] + S.ne[c];
+            sgnpw[s] = (CT)(kept ? -S.pw[s] : S.pw[s]);
+        }
+    D->glob.upload(S.glob.data(), ns);
+    D->sgnpw.upload(sgnpw.data(), ns);
+    {
+        auto v = cast_vec<CT>(S.inv_de);
+        D->inv_de.upload(v.data(), v.size());
+    }
+    D->slot_comp.upload(slot_comp.data(), ns);
+    // C over eliminated rows: entries -> kept slot ids; C' over kept rows: entries -> elim slot ids
+    std::vector<uint32_t> c_idx(S.c_idx.size());
+    std::vector<uint32_t> c_rows;
+    std::vector<uint64_t> c_b, c_e;
+    uint64_t ct_nnz = S.c_idx.size();
+    std::vector<uint64_t> ct_ptr(ns + 1, 0);
+    for (uint64_t c = 0; c < nc; ++c) {
+        const uint64_t base = S.slot_start[c], kb = base + S.ne[c];
+        for (uint64_t e = base; e < kb; ++e) {
+            c_rows.push_back((uint32_t)e);
+            c_b.push_back(S.c_ptr[e]);
+            c_e.push_back(S.c_ptr[e + 1]);
+            for (uint64_t p = S.c_ptr[e]; p < S.c_ptr[e + 1]; ++p) {
+                c_idx[p] = (uint32_t)(kb + S.c_idx[p]);
+                ct_ptr[kb + S.c_idx[p] + 1]++;
+            }
+        }
+    }
+    for (uint64_t s = 0; s < ns; ++s) ct_ptr[s + 1] += ct_ptr[s];
+    std::vector<uint32_t> ct_idx(ct_nnz);
+    std::vector<double> ct_val(ct_nnz);
+    {
+        std::vector<uint64_t> cur(ct_ptr.begin(), ct_ptr.end() - 1);
+        for (uint64_t c = 0; c < nc; ++c) {
+            const uint64_t base = S.slot_start[c], kb = base + S.ne[c];
+            for (uint64_t e = base; e < kb; ++e)
+                for (uint64_t p = S.c_ptr[e]; p < S.c_ptr[e + 1]; ++p) {
+                    const uint64_t row = kb + S.c_idx[p];
+                    ct_idx[cur[row]] = (uint32_t)e;
+                    ct_val[cur[row]] = S.c_val[p];
+                    cur[row]++;
+                }
+        }
+    }
+    std::vector<uint32_t> ct_rows;
+    std::vector<uint64_t> ct_b, ct_e;
+    for (uint64_t c = 0; c < nc; ++c)
+        for (uint64_t s = S.slot_start[c] + S.ne[c]; s < S.slot_start[c + 1]; ++s) {
+            ct_rows.push_back((uint32_t)s);
+            ct_b.push_back(ct_ptr[s]);
+            ct_e.push_back(ct_ptr[s + 1]);
+        }
+    {
+        std::vector<RowItem> items;
+        std::vector<RowFix> fixes;
+        make_items(c_rows, c_b, c_e, items, fixes, D->n_c_part);
+        D->c_items.upload(items.data(), items.size());
+        D->n_c_items = (uint32_t)items.size();
+        D->c_fix.upload(fixes.data(), fixes.size());
+        D->n_c_fix = (uint32_t)fixes.size();
+        items.clear();
+        fixes.clear();
+        make_items(ct_rows, ct_b, ct_e, items, fixes, D->n_ct_part);
+        D->ct_items.upload(items.data(), items.size());
+        D->n_ct_items = (uint32_t)items.size();
+        D->ct_fix.upload(fixes.data(), fixes.size());
+        D->n_ct_fix = (uint32_t)fixes.size();
+    }
+    D->c_idx.upload(c_idx.data(), c_idx.size());
+    {
+        auto v = cast_vec<CT>(S.c_val);
+        D->c_val.upload(v.data(), v.size());
+    }
+    D->ct_idx.upload(ct_idx.data(), ct_idx.size());
+    {
+        auto v = cast_vec<CT>(ct_val);
+        D->ct_val.upload(v.data(), v.size());
+    }
+    // mean chunks
+    std::vector<Chunk> all, kept;
+    std::vector<uint32_t> all_cptr(nc + 1, 0), kept_cptr(nc + 1, 0), cnt(nc), kcnt(nc);
+    for (uint64_t c = 0; c < nc; ++c) {
+        const uint32_t b = (uint32_t)S.slot_start[c], e = (uint32_t)S.slot_start[c + 1];
+        for (uint32_t s = b; s < e; s += MEAN_CHUNK) all.push_back({(uint32_t)c, s, std::min(e, s + MEAN_CHUNK), 0});
+        all_cptr[c + 1] = (uint32_t)all.size();
+        cnt[c] = e - b;
+        const uint32_t kb = b + S.ne[c];
+        if (S.fac_kind[c] == 1)
+            for (uint32_t s = kb; s < e; s += MEAN_CHUNK) kept.push_back({(uint32_t)c, s, std::min(e, s + MEAN_CHUNK), 0});
+        kept_cptr[c + 1] = (uint32_t)kept.size();
+        kcnt[c] = e - kb;
+    }
+    D->all_chunks.upload(all.data(), all.size());
+    D->n_all_chunks = (uint32_t)all.size();
+    D->all_cptr.upload(all_cptr.data(), all_cptr.size());
+    D->kept_chunks.upload(kept.data(), kept.size());
+    D->n_kept_chunks = (uint32_t)kept.size();
+    D->kept_cptr.upload(kept_cptr.data(), kept_cptr.size());
+    D->comp_count.upload(cnt.data(), cnt.size());
+    D->comp_kcount.upload(kcnt.data(), kcnt.size());
+    // dense rows
+    std::vector<uint64_t> drow_off;
+    std::vector<uint32_t> drow_base, drow_nk, drow_slot;
+    for (uint64_t c = 0; c < nc; ++c) {
+        if (S.fac_kind[c] != 0) continue;
+        const uint32_t nk = S.nk[c];
+        const uint32_t kb = (uint32_t)(S.slot_start[c] + S.ne[c]);
+        for (uint32_t i = 0; i < nk; ++i) {
+            drow_off.push_back(S.dense_off[c] + (uint64_t)i * nk);
+            drow_base.push_back(kb);
+            drow_nk.push_back(nk);
+            drow_slot.push_back(kb + i);
+        }
+    }
+    D->n_drows = (uint32_t)drow_off.size();
+    D->drow_off.upload(drow_off.data(), drow_off.size());
+    D->drow_base.upload(drow_base.data(), drow_base.size());
+    D->drow_nk.upload(drow_nk.data(), drow_nk.size());
+    D->drow_slot.upload(drow_slot.data(), drow_slot.size());
+    {
+        auto v = cast_vec<CT>(S.dense);
+        D->dmat.upload(v.data(), v.size());
+    }
+    // prolongation map
+    std::vector<uint64_t> d_ptr(S.n_dofs + 1, 0);
+    for (uint64_t s = 0; s < ns; ++s) d_ptr[S.glob[s] + 1]++;
+    for (uint64_t d = 0; d < S.n_dofs; ++d) d_ptr[d + 1] += d_ptr[d];
+    std::vector<uint32_t> d_slot(ns);
+    {
+        std::vector<uint64_t> cur(d_ptr.begin(), d_ptr.end() - 1);
+        for (uint64_t s = 0; s < ns; ++s) d_slot[cur[S.glob[s]]++] = (uint32_t)s;
+    }
+    D->d_ptr.upload(d_ptr.data(), d_ptr.size());
+    D->d_slot.upload(d_slot.data(), d_slot.size());
+    for (uint64_t c = 0; c < nc; ++c) D->has_sparse |= S.fac_kind[c] == 1;
+    if (D->has_sparse) fail(WX_ERR_INVALID_INPUT, "sparse kept-block factors are not enabled on the device yet");
+    return D;
+}
+
+template <typename CT>
+struct SchwarzRunImpl : SchwarzRun {
+    std::shared_ptr<SchwarzDev<CT>> D;
+    uint32_t kcap = 0;
+    DevBuf loc, red, xs, part, rowpart, mu1, mu2, mu3;
+
+    void ensure(uint32_t k) {
+        if (k <= kcap) return;
+        kcap = k;
+        const uint64_t ns = D->ns;
+        loc.alloc(ns * k * 8);
+        red.alloc(ns * k * 8);
+        xs.alloc(ns * k * 8);
+        part.alloc((uint64_t)std::max(D->n_all_chunks, D->n_kept_chunks) * k * 8 + 8);
+        rowpart.alloc((uint64_t)std::max(D->n_c_part, D->n_ct_part) * k * 8 + 8);
+        mu1.alloc(D->nc * k * 8);
+        mu2.alloc(D->nc * k * 8);
+        mu3.alloc(D->nc * k * 8);
+    }
+
+    void mean(const DevBuf &chunks, uint32_t n_chunks, const DevBuf &cptr, const DevBuf &count, const double *src,
+              double *mu, uint32_t k, uint32_t kshift, hipStream_t st) {
+        hipLaunchKernelGGL(sk_chunk_sum, dim3(grid_rows(n_chunks, kshift)), dim3(BLK), 0, st, dp<Chunk>(chunks),
+                           n_chunks, src, dp<double>(part), k, kshift);
+        hipLaunchKernelGGL(sk_comp_mean, dim3(grid_rows(D->nc, kshift)), dim3(BLK), 0, st, D->nc, dp<uint32_t>(cptr),
+                           dp<double>(part), dp<uint32_t>(count), mu, k, kshift);
+    }
+
+    void apply(const double *r, double *z, uint32_t k, hipStream_t st) override {
+        ensure(k);
+        const uint32_t kshift = pow2_shift(k);
+        const SchwarzDev<CT> &d = *D;
+        double *locp = dp<double>(loc), *redp = dp<double>(red), *xsp = dp<double>(xs);
+        hipLaunchKernelGGL((sk_restrict<CT>), dim3(grid_rows(d.ns, kshift)), dim3(BLK), 0, st, d.ns,
+                           dp<uint32_t>(d.glob), dp<CT>(d.sgnpw), r, locp, k, kshift);
+        mean(d.all_chunks, d.n_all_chunks, d.all_cptr, d.comp_count, locp, dp<double>(mu1), k, kshift, st);
+        // reduced right-hand side on kept rows
+        hipLaunchKernelGGL((sk_rows<CT, 0>), dim3(grid_rows(d.n_ct_items, kshift)), dim3(BLK), 0, st,
+                           dp<RowItem>(d.ct_items), d.n_ct_items, dp<uint32_t>(d.ct_idx), dp<CT>(d.ct_val),
+                           dp<uint32_t>(d.slot_comp), locp, xsp, dp<CT>(d.inv_de), dp<double>(mu1), redp,
+                           dp<double>(rowpart), k, kshift);
+        if (d.n_ct_fix)
+            hipLaunchKernelGGL((sk_rows_fix<CT, 0>), dim3(grid_rows(d.n_ct_fix, kshift)), dim3(BLK), 0, st,
+                               dp<RowFix>(d.ct_fix), d.n_ct_fix, dp<uint32_t>(d.slot_comp), locp, dp<CT>(d.inv_de),
+                               dp<double>(mu1), dp<double>(rowpart), redp, k, kshift);
+        // kept-block solves
+        if (d.n_drows)
+            hipLaunchKernelGGL((sk_dense<CT>), dim3(grid_rows(d.n_drows, kshift)), dim3(BLK), 0, st, d.n_drows,
+                               dp<uint64_t>(d.drow_off), dp<uint32_t>(d.drow_base), dp<uint32_t>(d.drow_nk),
+                               dp<uint32_t>(d.drow_slot), dp<CT>(d.dmat), redp, xsp, k, kshift);
+        // back-substitution on eliminated rows
+        hipLaunchKernelGGL((sk_rows<CT, 1>), dim3(grid_rows(d.n_c_items, kshift)), dim3(BLK), 0, st,
+                           dp<RowItem>(d.c_items), d.n_c_items, dp<uint32_t>(d.c_idx), dp<CT>(d.c_val),
+                           dp<uint32_t>(d.slot_comp), locp, xsp, dp<CT>(d.inv_de), dp<double>(mu1), xsp,
+                           dp<double>(rowpart), k, kshift);
+        if (d.n_c_fix)
+            hipLaunchKernelGGL((sk_rows_fix<CT, 1>), dim3(grid_rows(d.n_c_fix, kshift)), dim3(BLK), 0, st,
+                               dp<RowFix>(d.c_fix), d.n_c_fix, dp<uint32_t>(d.slot_comp), locp, dp<CT>(d.inv_de),
+                               dp<double>(mu1), dp<double>(rowpart), xsp, k, kshift);
+        mean(d.all_chunks, d.n_all_chunks, d.all_cptr, d.comp_count, xsp, dp<double>(mu3), k, kshift, st);
+        hipLaunchKernelGGL((sk_prolong<CT>), dim3(grid_rows(d.N, kshift)), dim3(BLK), 0, st, d.N,
+                           dp<uint64_t>(d.d_ptr), dp<uint32_t>(d.d_slot), dp<CT>(d.sgnpw),
+                           dp<uint32_t>(d.slot_comp), xsp, dp<double>(mu3), z, k, kshift);
+        WX_HIP(hipGetLastError());
+    }
+};
+
+template <typename CT> std::shared_ptr<SchwarzDev<CT>> get_dev(const Precond &p, int slot) {
+    std::lock_guard<std::mutex> lk(p.dev_mu);
+    if (!p.dev_cache[slot]) p.dev_cache[slot] = build_dev<CT>(*p.schwarz);
+    return std::static_pointer_cast<SchwarzDev<CT>>(p.dev_cache[slot]);
+}
+
+// y[d + j*n] <-> interleaved conversions for the standalone apply
+__global__ void sk_to_interleaved(uint64_t n, const double *__restrict__ x, double *__restrict__ xi, uint32_t k) {
+    const uint64_t idx = (uint64_t)blockIdx.x * BLK + threadIdx.x;
+    if (idx >= n * k) return;
+    xi[idx] = x[(idx % k) * n + idx / k];
+}
+__global__ void sk_from_interleaved(uint64_t n, const double *__restrict__ yi, double *__restrict__ y, uint32_t k) {
+    const uint64_t idx = (uint64_t)blockIdx.x * BLK + threadIdx.x;
+    if (idx >= n * k) return;
+    y[(idx % k) * n + idx / k] = yi[idx];
+}
+__global__ void sk_diag(uint64_t n, const double *__restrict__ inv, const double *__restrict__ x,
+                        double *__restrict__ y, uint32_t k) {
+    const uint64_t idx = (uint64_t)blockIdx.x * BLK + threadIdx.x;
+    if (idx >= n * k) return;
+    y[idx] = inv[idx % n] * x[idx];
+}
+
+} // namespace
+
+std::unique_ptr<SchwarzRun> make_schwarz_run(const Precond &p, int32_t precision, hipStream_t st) {
+    (void)st;
+    if (precision == WX_PRECISION_MIXED) {
+        auto r = std::make_unique<SchwarzRunImpl<float>>();
+        r->D = get_dev<float>(p, 1);
+        return r;
+    }
+    auto r = std::make_unique<SchwarzRunImpl<double>>();
+    r->D = get_dev<double>(p, 0);
+    return r;
+}
+
+void precond_apply_device(const Precond &p, const double *x, uint64_t k, double *y, hipStream_t st) {
+    const uint64_t n = p.n;
+    if (p.kind == Precond::DIAGONAL) {
+        DevBuf inv;
+        inv.upload(p.inv_diag.data(), n, st);
+        const uint64_t tot = n * k;
+        hipLaunchKernelGGL(sk_diag, dim3((tot + BLK - 1) / BLK), dim3(BLK), 0, st, n, dp<double>(inv), x, y,
+                           (uint32_t)k);
+        WX_HIP(hipStreamSynchronize(st));
+        return;
+    }
+    auto run = make_schwarz_run(p, WX_PRECISION_FP64, st);
+    for (uint64_t c0 = 0; c0 < k; c0 += MAX_BATCH) {
+        const uint32_t kk = (uint32_t)std::min<uint64_t>(MAX_BATCH, k - c0);
+        DevBuf xi, yi;
+        xi.alloc(n * kk * 8);
+        yi.alloc(n * kk * 8);
+        const uint64_t tot = n * kk;
+        hipLaunchKernelGGL(sk_to_interleaved, dim3((tot + BLK - 1) / BLK), dim3(BLK), 0, st, n, x + c0 * n,
+                           dp<double>(xi), kk);
+        run->apply(dp<double>(xi), dp<double>(yi), kk, st);
+        hipLaunchKernelGGL(sk_from_interleaved, dim3((tot + BLK - 1) / BLK), dim3(BLK), 0, st, n, dp<double>(yi),
+                           y + c0 * n, kk);
+        WX_HIP(hipStreamSynchronize(st));
+    }
+}
+
+} // namespace wx

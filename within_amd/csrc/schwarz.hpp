@@ -1,0 +1,79 @@
+// schwarz.hpp -- additive Schwarz preconditioner: host representation and device apply.
+#pragma once
+
+#include <memory>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "solver.hpp"
+
+namespace wx {
+
+// Local solver settings (crates/within/src/config.rs:20-131).
+struct LocalSolverSettings {
+    uint64_t approx_chol_seed = 0;
+    int32_t split_merge = 2;     // <= 0: None
+    bool schur_exact = false;
+    uint64_t schur_seed = 0;
+    int32_t schur_split = 1;
+    int64_t dense_threshold = -1; // < 0: device cost model
+    double scaling_tolerance = 1e-9;
+    int64_t scaling_max_sweeps = 2048;
+    bool scaling_error = false;
+};
+
+// Host form of a one-level additive Schwarz preconditioner over factor-pair components.
+// Every component is stored eliminated-block first: slots [slot_start[c], +ne[c]) are the
+// eliminated levels, the next nk[c] slots the kept levels.
+struct Precond::Schwarz {
+    uint64_t n_dofs = 0;
+    // components
+    std::vector<uint64_t> slot_start; // n_comp + 1
+    std::vector<uint32_t> ne, nk;
+    std::vector<uint8_t> fac_kind;    // 0 dense pseudo-inverse of the kept block, 1 sparse LDL'
+    // slots
+    std::vector<uint32_t> glob;       // global DOF
+    std::vector<double> pw;           // partition weight
+    std::vector<double> inv_de;       // 1/D_elim (0 on kept slots)
+    // C (eliminated x kept, local kept index) as CSR over slots; kept slots have empty rows
+    std::vector<uint64_t> c_ptr;      // n_slots + 1
+    std::vector<uint32_t> c_idx;
+    std::vector<double> c_val;
+    // dense kept-block operators: S^+ (nk x nk row-major) at dense_off[c]
+    std::vector<uint64_t> dense_off;  // n_comp
+    std::vector<double> dense;
+    // sparse LDL' factors: L strictly lower, CSR by rows (local indices), pivots d
+    std::vector<uint64_t> l_row_off;  // n_comp: offset of the component's first row pointer
+    std::vector<uint64_t> l_ptr;
+    std::vector<uint32_t> l_idx;
+    std::vector<double> l_val;
+    std::vector<uint64_t> piv_off;    // n_comp
+    std::vector<double> piv;
+
+    uint64_t n_comp() const { return ne.size(); }
+    uint64_t n_slots() const { return glob.size(); }
+};
+
+// Device apply of an additive Schwarz preconditioner for one batch width.
+struct SchwarzRun {
+    virtual ~SchwarzRun() = default;
+    // v = M^{-1} pt for N x k interleaved FP64 arrays, enqueued on `st`.
+    virtual void apply(const double *pt, double *v, uint32_t k, hipStream_t st) = 0;
+};
+
+std::unique_ptr<SchwarzRun> make_schwarz_run(const Precond &p, int32_t precision, hipStream_t st);
+
+// Builds the additive Schwarz preconditioner of a design (operator/schwarz.rs:232-271).
+// Returns nullptr when the design has no factor pair (plain LSMR, schwarz.rs:249-252).
+std::shared_ptr<Precond> build_additive(const Design &d, const std::vector<double> &weights,
+                                        const LocalSolverSettings &cfg, std::vector<std::string> &warnings);
+
+// Diagonal preconditioner (operator/schwarz.rs:185-229).
+std::shared_ptr<Precond> build_diagonal(const Design &d, const std::vector<double> &weights);
+
+// Versioned binary format of a preconditioner.
+std::vector<uint8_t> serialize_precond(const Precond &p);
+std::shared_ptr<Precond> deserialize_precond(const uint8_t *buf, uint64_t len);
+
+} // namespace wx

@@ -1,0 +1,105 @@
+// solver.hpp -- the Solver / Preconditioner objects behind the C ABI.
+#pragma once
+
+#include <atomic>
+#include <memory>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "device.hpp"
+
+namespace wx {
+
+constexpr uint32_t MAX_TERMS = 8;
+constexpr uint32_t MAX_COLS = 4;   // coefficient columns per term on the device path
+constexpr uint32_t MAX_BATCH = 64; // RHS columns interleaved per device pass
+
+// Per-term description passed by value to kernels.
+struct TermInfo {
+    uint64_t n_levels;
+    uint64_t offset;
+    uint32_t ncols;
+    int32_t load[MAX_COLS]; // -1 intercept, else slope column index
+};
+
+struct DesignArgs {
+    uint64_t n;
+    uint32_t T;
+    const uint32_t *codes;  // T x n
+    const double *slopes;   // S x n
+    TermInfo terms[MAX_TERMS];
+};
+
+// Work item of the deterministic D' segmented reduction: rows at positions [begin, end)
+// of the term's code-sorted order sum into `dest` (>= 0: the level itself; < 0: partial
+// slot -dest-1, combined afterwards in a fixed order).
+struct SegItem {
+    uint32_t level;
+    uint32_t begin;
+    uint32_t end;
+    int32_t dest;
+};
+struct SegFix {
+    uint32_t level;
+    uint32_t first;
+    uint32_t count;
+    uint32_t pad;
+};
+
+struct DevTerm {
+    TermInfo info;
+    DevBuf perm;   // position -> internal row, empty when the term is sorted
+    DevBuf items;  // SegItem[n_items]
+    uint32_t n_items = 0;
+    DevBuf fixes;  // SegFix[n_fixes]
+    uint32_t n_fixes = 0;
+    uint32_t n_partials = 0;
+};
+
+struct DevDesign {
+    uint64_t n = 0, N = 0;
+    DevBuf codes, slopes, sw, w, obs_perm;
+    std::vector<DevTerm> terms;
+    uint32_t max_partials = 0;
+    DesignArgs args() const;
+};
+
+// Preconditioner (host data + lazily built device form).
+struct Precond {
+    enum Kind { DIAGONAL = 0, ADDITIVE = 1 } kind = DIAGONAL;
+    uint64_t n = 0;
+    std::vector<double> inv_diag;
+    struct Schwarz;                 // defined in schwarz.hpp
+    std::shared_ptr<Schwarz> schwarz;
+    // device forms of the Schwarz data, built on first use per precision (0 FP64, 1 mixed)
+    mutable std::mutex dev_mu;
+    mutable std::shared_ptr<void> dev_cache[2];
+    virtual ~Precond() = default;
+};
+
+struct Solver {
+    Design design;
+    std::unique_ptr<SlopeReparam> reparam;
+    bool weighted = false;
+    std::shared_ptr<Precond> precond;
+    std::vector<std::string> warnings;
+    double time_setup = 0.0;
+    DevDesign dev;
+};
+
+// Uploads the design (codes, slopes, weights, D' segment plans) to the current device.
+void upload_design(Solver &s, const std::vector<double> &weights_internal);
+
+// Runs the batched solve with device-resident inputs/outputs on `stream`.
+void solve_batch_device(const Solver &s, const double *Y, uint64_t ldy, uint64_t k, const wx_lsmr_options &opts,
+                        double *X, double *Dm, wx_rhs_stats *stats, hipStream_t stream);
+
+// Y = sqrt(W) D X (adjoint: Y = D' sqrt(W) X) for column-major device arrays in caller
+// row order (n x k and N x k).
+void design_apply_device(const Solver &s, bool adjoint, const double *X, uint64_t k, double *Y, hipStream_t stream);
+
+// y = M^{-1} x on the device for n x k column-major arrays (FP64 coefficients).
+void precond_apply_device(const Precond &p, const double *x, uint64_t k, double *y, hipStream_t stream);
+
+} // namespace wx
