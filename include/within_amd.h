@@ -206,6 +206,13 @@ const char *wx_solver_warning(const wx_solver *s, uint64_t i);
 
 /* ---- preconditioner --------------------------------------------------------- */
 
+/* Replaces `build_preconditioner(design, weights, config)` (crates/within/src/operator/schwarz.rs:232-271):
+ * builds a preconditioner on the host only (no GPU needed), for offline construction and
+ * persistence; *out is NULL when the config resolves to no preconditioner (Off, or an
+ * Additive build without factor pairs). */
+wx_status wx_precond_build(const wx_design *design, const double *weights, const wx_precond_config *cfg,
+                           wx_precond **out);
+
 void wx_precond_release(wx_precond *p);
 
 /* Replaces `Preconditioner::nrows/ncols/variant_name` (crates/within/src/operator/schwarz.rs:129-147). */

@@ -93,6 +93,7 @@ EXPORTS = {
     "wx_solver_unidentified": ([C.c_void_p, C.c_void_p, C.POINTER(C.c_uint64)], C.c_int),
     "wx_solver_n_warnings": ([C.c_void_p], C.c_uint64),
     "wx_solver_warning": ([C.c_void_p, C.c_uint64], C.c_char_p),
+    "wx_precond_build": ([C.c_void_p, C.c_void_p, C.c_void_p, C.POINTER(C.c_void_p)], C.c_int),
     "wx_precond_release": ([C.c_void_p], None),
     "wx_precond_n": ([C.c_void_p], C.c_uint64),
     "wx_precond_variant": ([C.c_void_p], C.c_char_p),

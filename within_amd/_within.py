@@ -502,6 +502,23 @@ class Preconditioner:
         return f"Preconditioner({_lib.lib().wx_precond_variant(self._h).decode()}, n={self.nrows})"
 
 
+def build_preconditioner(design, weights: Optional[np.ndarray] = None, preconditioner=None) -> Optional[Preconditioner]:
+    """Builds a preconditioner on the host only (no GPU needed), e.g. for offline
+    construction and pickling; ``None`` when the config resolves to no preconditioner."""
+    L = _lib.lib()
+    if weights is not None:
+        weights = np.ascontiguousarray(_f64_1d("weights", weights))
+    cfg = _native_config(preconditioner)
+    d = _make_design(design)
+    try:
+        out = C.c_void_p()
+        _lib.check(L.wx_precond_build(d, None if weights is None else weights.ctypes.data,
+                                      None if cfg is None else C.byref(cfg), C.byref(out)))
+    finally:
+        L.wx_design_release(d)
+    return Preconditioner(out) if out.value else None
+
+
 def _check_options(options) -> LsmrOptions:
     if options is None:
         return LsmrOptions()

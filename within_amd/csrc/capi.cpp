@@ -222,6 +222,36 @@ wx_status wx_solver_new(const wx_design *design, const double *weights, const wx
     });
 }
 
+wx_status wx_precond_build(const wx_design *design, const double *weights, const wx_precond_config *cfg,
+                           wx_precond **out) {
+    return guard([&] {
+        if (!design || !out) fail(WX_ERR_INVALID_INPUT, "null design or output pointer");
+        Design d = build_design(design->n_obs, design->columns, design->structure, design->slopes, true);
+        std::vector<double> w;
+        if (weights) {
+            validate_weights(d, weights, d.n_obs);
+            w.resize(d.n_obs);
+            permute_in(d, weights, w.data());
+        }
+        std::unique_ptr<SlopeReparam> rp;
+        if (d.has_slopes()) rp = build_reparam(d, weights ? w.data() : nullptr);
+        wx_precond_config c;
+        if (cfg) c = *cfg;
+        else wx_precond_config_default(&c);
+        std::shared_ptr<Precond> p;
+        std::vector<std::string> warnings;
+        if (c.kind == WX_PRECOND_DIAGONAL) p = build_diagonal(d, w);
+        else if (c.kind != WX_PRECOND_OFF) p = build_additive(d, w, settings_from(c), warnings);
+        if (!p) {
+            *out = nullptr;
+            return;
+        }
+        auto *h = new wx_precond();
+        h->p = p;
+        *out = h;
+    });
+}
+
 void wx_solver_release(wx_solver *s) {
     if (s && --s->refs == 0) delete s;
 }

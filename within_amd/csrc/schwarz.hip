@@ -15,6 +15,8 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <numeric>
 
@@ -82,6 +84,8 @@ struct SpMeta {
     uint32_t pad;
     uint64_t lvl_f_off; // into lvl_f_ptr
     uint64_t lvl_b_off;
+    uint64_t row_off;   // first row pointer of the component in l_ptr / lt_ptr
+    uint64_t piv_off;   // first pivot
 };
 
 // ---------------------------------------------------------------------------
@@ -203,6 +207,63 @@ __global__ __launch_bounds__(BLK) void sk_dense(uint32_t n_rows, const uint64_t 
         double acc = 0.0;
         for (uint32_t l = 0; l < nk; ++l) acc += (double)m[l] * red[(base + l) * k + j];
         xs[(uint64_t)row_slot[r] * k + j] = acc;
+    }
+}
+
+// Sparse kept-block solve, one workgroup per component: forward L y = red - mu2 level by
+// level, y /= d (a zero pivot -- the floating gauge -- gives 0), backward L' x = y level by
+// level.  Rows within a level are independent; __syncthreads() orders the levels.  Row and
+// column ids are global slot ids; y and x live in xs (in place).
+template <typename CT>
+__global__ __launch_bounds__(BLK) void sk_sparse(const SpMeta *__restrict__ meta, const uint64_t *__restrict__ lvl_f_ptr,
+                                                 const uint32_t *__restrict__ lvl_f_rows,
+                                                 const uint64_t *__restrict__ lvl_b_ptr,
+                                                 const uint32_t *__restrict__ lvl_b_rows,
+                                                 const uint64_t *__restrict__ l_ptr, const uint32_t *__restrict__ l_idx,
+                                                 const CT *__restrict__ l_val, const uint64_t *__restrict__ lt_ptr,
+                                                 const uint32_t *__restrict__ lt_idx, const CT *__restrict__ lt_val,
+                                                 const CT *__restrict__ piv, const double *__restrict__ red,
+                                                 const double *__restrict__ mu2, double *xs, uint32_t k,
+                                                 uint32_t kshift) {
+    const uint32_t kp = 1u << kshift, j = threadIdx.x & (kp - 1);
+    const uint32_t lane_row = threadIdx.x >> kshift, rows_per_pass = BLK >> kshift;
+    const SpMeta m = meta[blockIdx.x];
+    const bool on = j < k;
+    const double mu = on ? mu2[(uint64_t)m.comp * k + j] : 0.0;
+    // forward substitution; l_ptr / piv are indexed by (row slot - kept_base) + row offset
+    for (uint32_t lv = 0; lv < m.nlev_f; ++lv) {
+        const uint64_t b = lvl_f_ptr[m.lvl_f_off + lv], e = lvl_f_ptr[m.lvl_f_off + lv + 1];
+        if (on)
+            for (uint64_t q = b + lane_row; q < e; q += rows_per_pass) {
+                const uint32_t row = lvl_f_rows[q];
+                const uint64_t lr = (uint64_t)(row - m.kept_base) + m.row_off;
+                double acc = red[(uint64_t)row * k + j] - mu;
+                for (uint64_t p = l_ptr[lr]; p < l_ptr[lr + 1]; ++p) acc -= (double)l_val[p] * xs[(uint64_t)l_idx[p] * k + j];
+                xs[(uint64_t)row * k + j] = acc;
+            }
+        __syncthreads();
+    }
+    // diagonal
+    if (on)
+        for (uint32_t r = lane_row; r < m.nk; r += rows_per_pass) {
+            const double dv = (double)piv[m.piv_off + r];
+            const uint64_t q = (uint64_t)(m.kept_base + r) * k + j;
+            xs[q] = dv != 0.0 ? xs[q] / dv : 0.0;
+        }
+    __syncthreads();
+    // backward substitution with L' (rows of L' = columns of L)
+    for (uint32_t lv = 0; lv < m.nlev_b; ++lv) {
+        const uint64_t b = lvl_b_ptr[m.lvl_b_off + lv], e = lvl_b_ptr[m.lvl_b_off + lv + 1];
+        if (on)
+            for (uint64_t q = b + lane_row; q < e; q += rows_per_pass) {
+                const uint32_t row = lvl_b_rows[q];
+                const uint64_t lr = (uint64_t)(row - m.kept_base) + m.row_off;
+                double acc = xs[(uint64_t)row * k + j];
+                for (uint64_t p = lt_ptr[lr]; p < lt_ptr[lr + 1]; ++p)
+                    acc -= (double)lt_val[p] * xs[(uint64_t)lt_idx[p] * k + j];
+                xs[(uint64_t)row * k + j] = acc;
+            }
+        __syncthreads();
     }
 }
 
@@ -407,7 +468,105 @@ std::shared_ptr<SchwarzDev<CT>> build_dev(const Precond::Schwarz &S) {
     D->d_ptr.upload(d_ptr.data(), d_ptr.size());
     D->d_slot.upload(d_slot.data(), d_slot.size());
     for (uint64_t c = 0; c < nc; ++c) D->has_sparse |= S.fac_kind[c] == 1;
-    if (D->has_sparse) fail(WX_ERR_INVALID_INPUT, "sparse kept-block factors are not enabled on the device yet");
+    if (D->has_sparse) {
+        // level schedules of L (forward) and L' (backward) per sparse component
+        std::vector<SpMeta> metas;
+        std::vector<uint64_t> lptr, ltptr, lvf_ptr, lvb_ptr;
+        std::vector<uint32_t> lidx, ltidx, lvf_rows, lvb_rows;
+        std::vector<CT> lval, ltval, piv;
+        uint64_t max_depth = 0, total_rows = 0;
+        for (uint64_t c = 0; c < nc; ++c) {
+            if (S.fac_kind[c] != 1) continue;
+            const uint32_t nk = S.nk[c];
+            const uint32_t kb = (uint32_t)(S.slot_start[c] + S.ne[c]);
+            const uint64_t ro = S.l_row_off[c];
+            SpMeta m{};
+            m.comp = (uint32_t)c;
+            m.kept_base = kb;
+            m.nk = nk;
+            m.row_off = lptr.size();
+            m.piv_off = piv.size();
+            // L rows
+            const uint64_t lbase = lidx.size();
+            for (uint32_t i = 0; i <= nk; ++i) lptr.push_back(lbase + (S.l_ptr[ro + i] - S.l_ptr[ro]));
+            for (uint64_t p = S.l_ptr[ro]; p < S.l_ptr[ro + nk]; ++p) {
+                lidx.push_back(kb + S.l_idx[p]);
+                lval.push_back((CT)S.l_val[p]);
+            }
+            for (uint32_t i = 0; i < nk; ++i) piv.push_back((CT)S.piv[S.piv_off[c] + i]);
+            // L' rows (column structure of L)
+            std::vector<uint64_t> tp(nk + 1, 0);
+            for (uint64_t p = S.l_ptr[ro]; p < S.l_ptr[ro + nk]; ++p) tp[S.l_idx[p] + 1]++;
+            for (uint32_t i = 0; i < nk; ++i) tp[i + 1] += tp[i];
+            std::vector<uint32_t> ti(tp[nk]);
+            std::vector<CT> tv(tp[nk]);
+            {
+                std::vector<uint64_t> cur(tp.begin(), tp.end() - 1);
+                for (uint32_t i = 0; i < nk; ++i)
+                    for (uint64_t p = S.l_ptr[ro + i]; p < S.l_ptr[ro + i + 1]; ++p) {
+                        const uint32_t col = S.l_idx[p];
+                        ti[cur[col]] = kb + i;
+                        tv[cur[col]] = (CT)S.l_val[p];
+                        cur[col]++;
+                    }
+            }
+            const uint64_t tbase = ltidx.size();
+            for (uint32_t i = 0; i <= nk; ++i) ltptr.push_back(tbase + tp[i]);
+            ltidx.insert(ltidx.end(), ti.begin(), ti.end());
+            ltval.insert(ltval.end(), tv.begin(), tv.end());
+            // levels: forward lvl(i) = 1 + max lvl(j), j in row i of L; backward over L' in reverse
+            std::vector<uint32_t> lf(nk, 0), lb(nk, 0);
+            uint32_t df = 0, db = 0;
+            for (uint32_t i = 0; i < nk; ++i) {
+                uint32_t v = 0;
+                for (uint64_t p = S.l_ptr[ro + i]; p < S.l_ptr[ro + i + 1]; ++p) v = std::max(v, lf[S.l_idx[p]] + 1);
+                lf[i] = v;
+                df = std::max(df, v + 1);
+            }
+            for (uint32_t i = nk; i-- > 0;) {
+                uint32_t v = 0;
+                for (uint64_t p = tp[i]; p < tp[i + 1]; ++p) v = std::max(v, lb[ti[p] - kb] + 1);
+                lb[i] = v;
+                db = std::max(db, v + 1);
+            }
+            auto bucket = [&](const std::vector<uint32_t> &lvl, uint32_t depth, std::vector<uint64_t> &ptr,
+                              std::vector<uint32_t> &rows) {
+                const uint64_t off = ptr.size();
+                std::vector<uint64_t> cnt(depth + 1, 0);
+                for (uint32_t i = 0; i < nk; ++i) cnt[lvl[i] + 1]++;
+                for (uint32_t l = 0; l < depth; ++l) cnt[l + 1] += cnt[l];
+                const uint64_t rbase = rows.size();
+                rows.resize(rbase + nk);
+                std::vector<uint64_t> cur(cnt.begin(), cnt.end() - 1);
+                for (uint32_t i = 0; i < nk; ++i) rows[rbase + cur[lvl[i]]++] = kb + i;
+                for (uint32_t l = 0; l <= depth; ++l) ptr.push_back(rbase + cnt[l]);
+                return off;
+            };
+            m.nlev_f = df;
+            m.nlev_b = db;
+            m.lvl_f_off = bucket(lf, df, lvf_ptr, lvf_rows);
+            m.lvl_b_off = bucket(lb, db, lvb_ptr, lvb_rows);
+            max_depth = std::max<uint64_t>(max_depth, std::max(df, db));
+            total_rows += nk;
+            metas.push_back(m);
+        }
+        D->n_sparse = (uint32_t)metas.size();
+        D->sp_meta.upload(metas.data(), metas.size());
+        D->l_ptr.upload(lptr.data(), lptr.size());
+        D->l_idx.upload(lidx.data(), lidx.size());
+        D->l_val.upload(lval.data(), lval.size());
+        D->lt_ptr.upload(ltptr.data(), ltptr.size());
+        D->lt_idx.upload(ltidx.data(), ltidx.size());
+        D->lt_val.upload(ltval.data(), ltval.size());
+        D->piv.upload(piv.data(), piv.size());
+        D->sp_lvl_f_ptr.upload(lvf_ptr.data(), lvf_ptr.size());
+        D->sp_lvl_f_rows.upload(lvf_rows.data(), lvf_rows.size());
+        D->sp_lvl_b_ptr.upload(lvb_ptr.data(), lvb_ptr.size());
+        D->sp_lvl_b_rows.upload(lvb_rows.data(), lvb_rows.size());
+        if (std::getenv("WITHIN_AMD_VERBOSE"))
+            std::fprintf(stderr, "[within_amd] sparse kept blocks: %u components, %lu rows, nnz(L)=%zu, max depth %lu\n",
+                         D->n_sparse, (unsigned long)total_rows, lidx.size(), (unsigned long)max_depth);
+    }
     return D;
 }
 
@@ -457,6 +616,14 @@ struct SchwarzRunImpl : SchwarzRun {
                                dp<RowFix>(d.ct_fix), d.n_ct_fix, dp<uint32_t>(d.slot_comp), locp, dp<CT>(d.inv_de),
                                dp<double>(mu1), dp<double>(rowpart), redp, k, kshift);
         // kept-block solves
+        if (d.n_sparse) {
+            mean(d.kept_chunks, d.n_kept_chunks, d.kept_cptr, d.comp_kcount, redp, dp<double>(mu2), k, kshift, st);
+            hipLaunchKernelGGL((sk_sparse<CT>), dim3(d.n_sparse), dim3(BLK), 0, st, dp<SpMeta>(d.sp_meta),
+                               dp<uint64_t>(d.sp_lvl_f_ptr), dp<uint32_t>(d.sp_lvl_f_rows),
+                               dp<uint64_t>(d.sp_lvl_b_ptr), dp<uint32_t>(d.sp_lvl_b_rows), dp<uint64_t>(d.l_ptr),
+                               dp<uint32_t>(d.l_idx), dp<CT>(d.l_val), dp<uint64_t>(d.lt_ptr), dp<uint32_t>(d.lt_idx),
+                               dp<CT>(d.lt_val), dp<CT>(d.piv), redp, dp<double>(mu2), xsp, k, kshift);
+        }
         if (d.n_drows)
             hipLaunchKernelGGL((sk_dense<CT>), dim3(grid_rows(d.n_drows, kshift)), dim3(BLK), 0, st, d.n_drows,
                                dp<uint64_t>(d.drow_off), dp<uint32_t>(d.drow_base), dp<uint32_t>(d.drow_nk),
