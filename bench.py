@@ -1,0 +1,201 @@
+"""Benchmark: batched weighted solve_batch on one MI355X (config 2 of the hot path).
+
+Workload (default): n = 5,000,000 observations, factors A (200,000 levels, Zipf(1.1)
+frequencies), B (10,000 uniform levels), C (50 uniform levels); weights ~ U(0.5, 2);
+k = 16 float64 right-hand sides ~ N(0,1); tol 1e-8; default additive Schwarz.  A "step" is
+one solve_batch of the 16 right-hand sides from inputs resident in HBM.  The headline value
+is the mixed-precision form (FP32 preconditioner coefficients, FP64 Krylov); the plain
+FP64 form is reported beside it.
+
+Prints ONE JSON line.  The CPU baseline is the CPU oracle (a C restatement of the
+reference's LSMR + additive Schwarz) solving a bounded sample of the same workload on
+the host's cores with the same preconditioner data.
+"""
+from __future__ import annotations
+
+import argparse
+import ctypes as C
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def cfg2_data(n=5_000_000, k=16, seed=2):
+    rng = np.random.default_rng(seed)
+    p = (np.arange(200_000) + 1.0) ** -1.1
+    p /= p.sum()
+    A = rng.choice(200_000, n, p=p)
+    B = rng.integers(0, 10_000, n)
+    Cc = rng.integers(0, 50, n)
+    codes = np.asfortranarray(np.column_stack([A, B, Cc]).astype(np.uint32))
+    w = rng.uniform(0.5, 2.0, n)
+    Y = np.asfortranarray(rng.standard_normal((n, k)))
+    return codes, w, Y
+
+
+def algorithmic_bytes(n, F, weighted, V, k, N):
+    """Compulsory bytes of one D v gather (with the fused u update) and one D' reduction."""
+    w = 1 if weighted else 0
+    b_d = n * (4 * F + 8 * w + 8 * V + 16 * k) + 8 * k * N
+    b_dt = n * (4 * F + 8 * w + 8 * V + 8 * k) + 8 * k * N
+    return b_d, b_dt
+
+
+def run_gpu(solver, Y_dev, k, n, N, precision, steps, warmup, tol):
+    import torch
+    from within_amd import _lib
+    L = _lib.lib()
+    opts = _lib.LsmrOpts()
+    L.wx_lsmr_options_default(C.byref(opts))
+    opts.tol = tol
+    opts.precision = 1 if precision == "mixed" else 0
+    X = torch.empty((k, N), dtype=torch.float64, device="cuda")   # column-major N x k
+    Dm = torch.empty((k, n), dtype=torch.float64, device="cuda")  # column-major n x k
+    stats = (_lib.RhsStats * k)()
+    stream = torch.cuda.current_stream().cuda_stream
+
+    def step(timing=False):
+        opts.timing = 1 if timing else 0
+        _lib.check(L.wx_solver_solve_batch_device(solver._h, Y_dev.data_ptr(), n, k, C.byref(opts), X.data_ptr(),
+                                                  Dm.data_ptr(), stats, C.c_void_p(stream)))
+
+    for _ in range(warmup):
+        step()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        step()
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    # one more step with HIP-event timing of the phases (not part of the timed region)
+    step(timing=True)
+    tm = (C.c_double * 8)()
+    L.wx_last_timing(tm, 8)
+    phases = {name: (tm[2 * i], int(tm[2 * i + 1])) for i, name in
+              enumerate(["gather", "adjoint", "precond", "iteration"])}
+    st = [stats[j] for j in range(k)]
+    return elapsed, st, phases, Dm
+
+
+def cpu_baseline(solver, codes, w, Y, seconds_budget=30.0):
+    """Oracle (C restatement of the reference algorithm) on the host, bounded sample."""
+    from oracle.oracle import OracleLib
+    from tests._helpers import oracle_schwarz_from_bytes
+    lib = OracleLib()
+    threads = min(os.cpu_count() or 1, Y.shape[1], 16)
+    # the reference's default local solver rule (dense exact factor only for n_kept <= 24,
+    # sampled Schur + approximate Cholesky otherwise), built by this library's host builder
+    import within_amd as wa
+    from within_amd._within import build_preconditioner
+    cfg = wa.config.AdditiveSchwarz(wa.config.LocalSolverConfig(dense_threshold=24))
+    p = build_preconditioner(codes, w, cfg)
+    sch = oracle_schwarz_from_bytes(lib, p.to_bytes())
+    d = lib.design(codes)
+    Ys = np.asfortranarray(Y[:, :threads])
+    t0 = time.perf_counter()
+    _, _, res = lib.solve_batch(d, Ys, w, prec="schwarz", schwarz=sch, tol=1e-8, threads=threads)
+    dt = time.perf_counter() - t0
+    return {"value": threads / dt, "unit": "RHS/s", "cores": threads, "kind": "port",
+            "sample": f"{threads} of the 16 RHS of the same workload (full n), one RHS per thread, "
+                      f"additive Schwarz with the reference's default local-solver rule (approximate Cholesky), FP64, "
+                      f"{max(r.iterations for r in res)} max iterations, {dt:.1f} s wall"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--n", type=int, default=5_000_000)
+    ap.add_argument("--k", type=int, default=16)
+    ap.add_argument("--tol", type=float, default=1e-8)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-fp64", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only: --gpus {args.gpus} is not supported by this build "
+              "(the solve batches are independent; see the sharding note in DESIGN.md)", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    import within_amd as wa
+    from within_amd import _lib
+    n_dev, name = _lib.device_info()
+    log(f"[bench] device {name}; generating data n={args.n} k={args.k}")
+    codes, w, Y = cfg2_data(args.n, args.k)
+    t0 = time.perf_counter()
+    solver = wa.Solver(codes, weights=w)
+    setup = time.perf_counter() - t0
+    n, k, N = args.n, args.k, solver.n_dofs
+    log(f"[bench] setup (design + preconditioner build + upload) {setup:.2f} s, n_dofs={N}")
+    Y_dev = torch.from_numpy(np.ascontiguousarray(Y.T)).cuda()  # (k, n) row-major == (n, k) column-major
+
+    res = {}
+    for precision in (["mixed"] if args.no_fp64 else ["mixed", "fp64"]):
+        el, st, ph, Dm = run_gpu(solver, Y_dev, k, n, N, precision, args.steps, args.warmup, args.tol)
+        res[precision] = (el, st, ph)
+        log(f"[bench] {precision}: {el / args.steps * 1e3:.1f} ms/step, iterations {[s.iterations for s in st]}, "
+            f"max true residual {max(s.true_residual for s in st):.2e}, phases {ph}")
+    el, st, ph = res["mixed"]
+    value = k * args.steps / el
+    b_d, b_dt = algorithmic_bytes(n, codes.shape[1], True, 0, k, N)
+    g_ms, g_cnt = ph["gather"]
+    a_ms, a_cnt = ph["adjoint"]
+    # dominant of the two design-operator phases by total time
+    if g_ms >= a_ms:
+        kern, bytes_per, dur = "k_gather_axpy (D v + u update)", b_d, g_ms / max(g_cnt, 1) * 1e-3
+    else:
+        kern, bytes_per, dur = "k_segreduce (D' reduction, all terms)", b_dt, a_ms / max(a_cnt, 1) * 1e-3
+    achieved = bytes_per / dur / 1e9
+    out = {
+        "metric": "RHS solved/s to tol 1e-8 (batched mixed-prec vs plain FP64 GPU); D/D' HBM GB/s",
+        "value": value,
+        "unit": "RHS/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": el / args.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64 Krylov + f32 preconditioner coefficients",
+        "data": "synthetic (seeded generator of the weighted solve_batch workload)",
+        "config": {"workload": "weighted solve_batch: n=5,000,000; A 200,000 Zipf(1.1), B 10,000 uniform, "
+                               "C 50 uniform; k=16; weights U(0.5,2); tol 1e-8",
+                   "n": n, "k": k, "n_dofs": N, "precision": "mixed"},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": kern,
+                     "algorithmic_bytes_per_launch": bytes_per, "avg_launch_ms": dur * 1e3},
+        "iterations": [s.iterations for s in st],
+        "true_residual_max": max(s.true_residual for s in st),
+        "setup_s": setup,
+        "phases_ms": {p: {"total_ms": v[0], "launches": v[1]} for p, v in ph.items()},
+    }
+    if "fp64" in res:
+        el64, st64, ph64 = res["fp64"]
+        out["fp64_value"] = k * args.steps / el64
+        out["mixed_over_fp64"] = out["value"] / out["fp64_value"]
+        out["fp64_iterations"] = [s.iterations for s in st64]
+    if not args.no_cpu:
+        try:
+            out["cpu_baseline"] = cpu_baseline(solver, codes, w, Y)
+        except Exception as e:  # the baseline is reported, never the measured value
+            log(f"[bench] cpu baseline failed: {e!r}")
+            out["cpu_baseline"] = None
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -100,6 +100,7 @@ typedef struct wx_lsmr_options {
     int64_t local_size;        /* <= 0: None */
     int32_t precision;         /* wx_precision */
     int32_t check_every;       /* host convergence poll interval in iterations; <= 0: auto */
+    int32_t timing;            /* != 0: time the solve's kernels with HIP events (wx_last_timing) */
 } wx_lsmr_options;
 
 /* Per-RHS result, the per-column fields of BatchSolveResult (crates/within/src/solver.rs:248-293)
@@ -117,6 +118,13 @@ typedef struct wx_rhs_stats {
 typedef struct wx_design wx_design;
 typedef struct wx_solver wx_solver;
 typedef struct wx_precond wx_precond;
+
+/* Per-phase device times of this thread's last solve run with `timing` set, measured with
+ * HIP events on the solve stream: out[2p] = total milliseconds, out[2p+1] = launches, for
+ * p = 0 D v gather (fused u update), 1 D' reduction (all terms), 2 preconditioner apply,
+ * 3 whole iterations.  Returns the number of phases written (<= n / 2).  No reference
+ * counterpart: the measurement hook of the benchmark. */
+uint64_t wx_last_timing(double *out, uint64_t n);
 
 /* Thread-local message of the last failed call on this thread ("" if none). */
 const char *wx_last_error(void);

@@ -52,6 +52,7 @@ class LsmrOpts(C.Structure):
         ("local_size", C.c_int64),
         ("precision", C.c_int32),
         ("check_every", C.c_int32),
+        ("timing", C.c_int32),
     ]
 
 
@@ -69,6 +70,7 @@ class RhsStats(C.Structure):
 
 EXPORTS = {
     "wx_last_error": ([], C.c_char_p),
+    "wx_last_timing": ([C.c_void_p, C.c_uint64], C.c_uint64),
     "wx_device_info": ([C.c_char_p, C.c_uint64], C.c_int32),
     "wx_precond_config_default": ([C.POINTER(PrecondConfig)], None),
     "wx_lsmr_options_default": ([C.POINTER(LsmrOpts)], None),

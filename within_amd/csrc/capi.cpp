@@ -129,6 +129,14 @@ void wx_lsmr_options_default(wx_lsmr_options *o) {
     o->local_size = 0;
     o->precision = WX_PRECISION_FP64;
     o->check_every = 0;
+    o->timing = 0;
+}
+
+uint64_t wx_last_timing(double *out, uint64_t n) {
+    const auto &t = last_timing();
+    const uint64_t m = std::min<uint64_t>(n / 2, t.size() / 2);
+    for (uint64_t i = 0; i < 2 * m; ++i) out[i] = t[i];
+    return m;
 }
 
 wx_status wx_design_from_codes(const uint32_t *codes, uint64_t n_obs, uint64_t n_factors, int64_t rs, int64_t cs,

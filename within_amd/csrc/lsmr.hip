@@ -747,6 +747,42 @@ void launch_adjoint(const DevDesign &dv, const DesignArgs &da, const double *sca
     WX_HIP(hipGetLastError());
 }
 
+// HIP-event timer of the solve phases (enabled by wx_lsmr_options::timing).
+struct PhaseTimer {
+    static constexpr int NP = 4;
+    bool on = false;
+    hipStream_t st = nullptr;
+    std::vector<std::pair<hipEvent_t, hipEvent_t>> ev[NP];
+    hipEvent_t open[NP] = {};
+    void begin(int p) {
+        if (!on) return;
+        hipEvent_t e;
+        WX_HIP(hipEventCreate(&e));
+        WX_HIP(hipEventRecord(e, st));
+        open[p] = e;
+    }
+    void end(int p) {
+        if (!on) return;
+        hipEvent_t e;
+        WX_HIP(hipEventCreate(&e));
+        WX_HIP(hipEventRecord(e, st));
+        ev[p].push_back({open[p], e});
+    }
+    void collect(std::vector<double> &out) {
+        out.assign(2 * NP, 0.0);
+        for (int p = 0; p < NP; ++p) {
+            for (auto &pr : ev[p]) {
+                float ms = 0.f;
+                if (hipEventElapsedTime(&ms, pr.first, pr.second) == hipSuccess) out[2 * p] += ms;
+                out[2 * p + 1] += 1.0;
+                (void)hipEventDestroy(pr.first);
+                (void)hipEventDestroy(pr.second);
+            }
+            ev[p].clear();
+        }
+    }
+};
+
 struct Workspace {
     DevBuf u, v, pt, atu, h, hb, x, part, segpart, cols, ints, scratch_y;
     ColState cs{};
@@ -792,6 +828,9 @@ void solve_batch_device(const Solver &s, const double *Y, uint64_t ldy, uint64_t
     DevBuf inv_diag;
     if (prec_kind == 1) inv_diag.upload(pc->inv_diag.data(), pc->inv_diag.size(), st);
     std::unique_ptr<SchwarzRun> schwarz_run;
+    PhaseTimer timer;
+    timer.on = opts.timing != 0;
+    timer.st = st;
 
     for (uint64_t c0 = 0; c0 < k_total; c0 += MAX_BATCH) {
         const uint32_t k = (uint32_t)std::min<uint64_t>(MAX_BATCH, k_total - c0);
@@ -857,19 +896,26 @@ void solve_batch_device(const Solver &s, const double *Y, uint64_t ldy, uint64_t
         int check_every = opts.check_every > 0 ? opts.check_every : 4;
         int itn_done = 0;
         for (int itn = 1; itn <= maxiter; ++itn) {
+            timer.begin(3);
+            timer.begin(0);
             if (weighted)
                 hipLaunchKernelGGL((k_gather_axpy<true>), dim3(gn), dim3(BLK), 0, st, da, dv.sw.as<double>(),
                                    ws.v.as<double>(), ws.u.as<double>(), cs, part, k, kshift);
             else
                 hipLaunchKernelGGL((k_gather_axpy<false>), dim3(gn), dim3(BLK), 0, st, da, nullptr, ws.v.as<double>(),
                                    ws.u.as<double>(), cs, part, k, kshift);
+            timer.end(0);
             hipLaunchKernelGGL(k_beta, dim3(k), dim3(BLK), 0, st, part, gn, kp, cs);
+            timer.begin(1);
             launch_adjoint(dv, da, weighted ? dv.sw.as<double>() : nullptr, weighted ? 1 : 0, ws.u.as<double>(),
                            ws.atu.as<double>(), ws.segpart.as<double>(), cs.active, k, kshift, st);
+            timer.end(1);
             if (prec_kind == 2) {
                 hipLaunchKernelGGL((k_ptilde<2, false>), dim3(gN), dim3(BLK), 0, st, N, ws.atu.as<double>(),
                                    ws.pt.as<double>(), ws.v.as<double>(), nullptr, cs, part, k, kshift);
+                timer.begin(2);
                 schwarz_run->apply(ws.pt.as<double>(), ws.v.as<double>(), k, st);
+                timer.end(2);
                 hipLaunchKernelGGL(k_dots, dim3(gN), dim3(BLK), 0, st, N, ws.pt.as<double>(), ws.v.as<double>(), cs,
                                    part, k, kshift);
             } else if (prec_kind == 1) {
@@ -882,6 +928,7 @@ void solve_batch_device(const Solver &s, const double *Y, uint64_t ldy, uint64_t
             hipLaunchKernelGGL(k_alpha_step, dim3(k), dim3(BLK), 0, st, part, gN, kp, cs, opts.tol, itn);
             hipLaunchKernelGGL(k_update_solution, dim3(gN), dim3(BLK), 0, st, N, ws.v.as<double>(), ws.h.as<double>(),
                                ws.hb.as<double>(), ws.x.as<double>(), cs, k, kshift);
+            timer.end(3);
             itn_done = itn;
             if (itn % check_every == 0 || itn == maxiter) {
                 WX_HIP(hipGetLastError());
@@ -975,6 +1022,15 @@ void solve_batch_device(const Solver &s, const double *Y, uint64_t ldy, uint64_t
             o.time_solve = secs / k;
         }
     }
+    if (timer.on) {
+        WX_HIP(hipStreamSynchronize(st));
+        timer.collect(last_timing());
+    }
+}
+
+std::vector<double> &last_timing() {
+    static thread_local std::vector<double> t;
+    return t;
 }
 
 // ---------------------------------------------------------------------------

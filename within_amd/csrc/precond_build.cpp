@@ -22,6 +22,10 @@
 
 namespace wx {
 
+// dense_gpu.hip
+bool gpu_available();
+bool dense_laplacian_pinv_gpu(std::vector<double> &S, uint32_t n);
+
 std::shared_ptr<Precond> build_diagonal(const Design &d, const std::vector<double> &w) {
     // crates/within/src/operator/schwarz.rs:185-229
     auto p = std::make_shared<Precond>();
@@ -542,13 +546,26 @@ std::shared_ptr<Precond> build_additive(const Design &d, const std::vector<doubl
     S->pw.resize(ns);
     S->inv_de.assign(ns, 0.0);
     S->c_ptr.assign(ns + 1, 0);
-    // factor choice per component
-    const int64_t dense_cap = 4096;
-    std::vector<uint8_t> dense(nc);
+    // factor choice per component.  Device cost model (dense_threshold < 0): small kept blocks
+    // and dense-ish reduced graphs get the exact dense pseudo-inverse -- one parallel
+    // symmetric product per apply -- while large sparse reduced graphs get approximate
+    // Cholesky, whose triangular solves run level by level.  The density estimate is the
+    // sampled Schur complement's edge count sum_e (deg(e) - 1) over nk^2 / 2.
+    const bool have_gpu = gpu_available();
+    std::vector<uint8_t> dense(nc), big(nc, 0);
     for (uint64_t c = 0; c < nc; ++c) {
-        const uint32_t nk = comps[c].nk;
+        const uint64_t nk = comps[c].nk;
         if (cfg.dense_threshold >= 0) dense[c] = nk <= (uint64_t)cfg.dense_threshold;
-        else dense[c] = nk <= dense_cap;
+        else if (nk <= 4096) dense[c] = 1;
+        else if (nk <= 16384 && have_gpu) {
+            double m = 0.0;
+            for (uint32_t e = 0; e < comps[c].ne; ++e) {
+                const uint64_t deg = comps[c].cptr[e + 1] - comps[c].cptr[e];
+                if (deg > 1) m += (double)(deg - 1);
+            }
+            dense[c] = 2.0 * m / ((double)nk * nk) >= 0.005;
+        } else dense[c] = 0;
+        big[c] = dense[c] && nk > 1024 && have_gpu;
         S->fac_kind[c] = dense[c] ? 0 : 1;
     }
     // per-component factorisation (parallel)
@@ -561,7 +578,7 @@ std::shared_ptr<Precond> build_additive(const Design &d, const std::vector<doubl
         const uint32_t ne = cm.ne, nk = cm.nk;
         for (uint32_t e = 0; e < ne; ++e)
             if (!(cm.diag[e] > 0.0)) failed[c] = 1;
-        if (failed[c]) continue;
+        if (failed[c] || big[c]) continue;
         if (dense[c]) {
             // exact Schur complement S = D_k - C' D_e^{-1} C (schur.rs:29-61)
             std::vector<double> Sm((size_t)nk * nk, 0.0);
@@ -600,6 +617,41 @@ std::shared_ptr<Precond> build_additive(const Design &d, const std::vector<doubl
             acs[c] = approx_cholesky(nk, edges, mix_seed(cfg.approx_chol_seed, c, 0xC401),
                                      cfg.split_merge > 0 ? cfg.split_merge : 1);
         }
+    }
+    // large dense kept blocks: exact Schur complement accumulated row-parallel on the host
+    // (each thread owns kept rows, entries ordered (w*v)*inv as in schur.rs:112-113), then the
+    // pseudo-inverse on the GPU
+    for (uint64_t c = 0; c < nc; ++c) {
+        if (!big[c] || failed[c]) continue;
+        const Component &cm = comps[c];
+        const uint32_t ne = cm.ne, nk = cm.nk;
+        std::vector<uint64_t> tp(nk + 1, 0);
+        for (uint32_t k : cm.cidx) tp[k + 1]++;
+        for (uint32_t k = 0; k < nk; ++k) tp[k + 1] += tp[k];
+        std::vector<uint32_t> te(cm.cidx.size());
+        std::vector<double> tv(cm.cidx.size());
+        {
+            std::vector<uint64_t> cur(tp.begin(), tp.end() - 1);
+            for (uint32_t e = 0; e < ne; ++e)
+                for (uint64_t p = cm.cptr[e]; p < cm.cptr[e + 1]; ++p) {
+                    te[cur[cm.cidx[p]]] = e;
+                    tv[cur[cm.cidx[p]]] = cm.cval[p];
+                    cur[cm.cidx[p]]++;
+                }
+        }
+        std::vector<double> Sm((size_t)nk * nk, 0.0);
+#pragma omp parallel for schedule(dynamic, 8)
+        for (int64_t i = 0; i < (int64_t)nk; ++i) {
+            double *row = &Sm[(size_t)i * nk];
+            row[i] = cm.diag[ne + i];
+            for (uint64_t q = tp[i]; q < tp[i + 1]; ++q) {
+                const uint32_t e = te[q];
+                const double wi = tv[q], inv = 1.0 / cm.diag[e];
+                for (uint64_t p = cm.cptr[e]; p < cm.cptr[e + 1]; ++p) row[cm.cidx[p]] -= (wi * cm.cval[p]) * inv;
+            }
+        }
+        if (!dense_laplacian_pinv_gpu(Sm, nk)) failed[c] = 2;
+        dense_mats[c] = std::move(Sm);
     }
     for (uint64_t c = 0; c < nc; ++c) {
         if (failed[c] == 1) fail(WX_ERR_BUILD, "zero diagonal in preconditioner at index " + std::to_string(c));

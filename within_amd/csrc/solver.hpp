@@ -95,6 +95,9 @@ void upload_design(Solver &s, const std::vector<double> &weights_internal);
 void solve_batch_device(const Solver &s, const double *Y, uint64_t ldy, uint64_t k, const wx_lsmr_options &opts,
                         double *X, double *Dm, wx_rhs_stats *stats, hipStream_t stream);
 
+// Per-phase (ms, launches) pairs of this thread's last timed solve (see wx_last_timing).
+std::vector<double> &last_timing();
+
 // Y = sqrt(W) D X (adjoint: Y = D' sqrt(W) X) for column-major device arrays in caller
 // row order (n x k and N x k).
 void design_apply_device(const Solver &s, bool adjoint, const double *X, uint64_t k, double *Y, hipStream_t stream);
