@@ -187,8 +187,9 @@ def test_edge_cases(gpu):
     D = sparse_design(codes)
     r = s.solve(D @ xt, wa.LsmrOptions(tol=1e-12))
     assert np.abs(r.demeaned).max() < 1e-8
-    # maxiter exhausted
-    r = s.solve(rng.standard_normal(n), wa.LsmrOptions(tol=1e-14, maxiter=2))
+    # maxiter exhausted (unpreconditioned: the exact two-factor Schwarz converges in one step)
+    s_off = wa.Solver(codes, preconditioner=wa.PreconditionerConfig.Off)
+    r = s_off.solve(rng.standard_normal(n), wa.LsmrOptions(tol=1e-14, maxiter=2))
     assert not r.converged and r.iterations == 2 and r.stop_reason == "MaxIterations"
     # empty batch
     b = s.solve_batch(np.zeros((n, 0)))
@@ -250,7 +251,7 @@ def test_single_factor_runs_plain_lsmr(gpu):
     y = rng.standard_normal(2000)
     s = wa.Solver(codes)
     assert s.preconditioner is None
-    r = s.solve(y)
+    r = s.solve(y, wa.LsmrOptions(tol=1e-12))
     means = np.bincount(codes[:, 0], weights=y) / np.bincount(codes[:, 0])
     np.testing.assert_allclose(r.demeaned, y - means[codes[:, 0]], atol=1e-9)
 

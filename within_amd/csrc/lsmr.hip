@@ -134,44 +134,77 @@ template <int NC>
 __global__ void k_segfix(TermInfo ti, const SegFix *__restrict__ fixes, uint32_t n_fixes,
                          const double *__restrict__ partials, double *__restrict__ out,
                          const int32_t *__restrict__ active, uint32_t k, uint32_t kshift) {
+    // one workgroup per split level: row groups stride over the partials, then a fixed-order
+    // LDS tree across the groups
+    __shared__ double sh[BLK];
     const uint32_t kp = 1u << kshift;
-    const uint32_t j = threadIdx.x & (kp - 1);
-    if (j >= k || (active && !active[j])) return;
-    const uint32_t g = (blockIdx.x * BLK + threadIdx.x) >> kshift;
-    if (g >= n_fixes) return;
-    const SegFix f = fixes[g];
-#pragma unroll
+    const uint32_t j = threadIdx.x & (kp - 1), grp = threadIdx.x >> kshift;
+    const bool on = j < k && !(active && !active[j]);
+    const SegFix f = fixes[blockIdx.x];
     for (int c = 0; c < NC; ++c) {
         double acc = 0.0;
-        for (uint32_t q = 0; q < f.count; ++q) acc += partials[((uint64_t)(f.first + q) * NC + c) * k + j];
-        out[(ti.offset + c * ti.n_levels + f.level) * k + j] = acc;
+        if (on)
+            for (uint32_t q = grp; q < f.count; q += BLK >> kshift)
+                acc += partials[((uint64_t)(f.first + q) * NC + c) * k + j];
+        sh[threadIdx.x] = acc;
+        __syncthreads();
+        for (uint32_t s = BLK / 2; s >= kp && s > 0; s >>= 1) {
+            if (threadIdx.x < s) sh[threadIdx.x] += sh[threadIdx.x + s];
+            __syncthreads();
+        }
+        if (threadIdx.x < kp && on) out[(ti.offset + c * ti.n_levels + f.level) * k + j] = sh[threadIdx.x];
+        __syncthreads();
     }
 }
 
 // ---------------------------------------------------------------------------
 // Right-hand side loading and vec_norm (crates/schwarz-precond/src/lsmr.rs:24-36)
 // ---------------------------------------------------------------------------
-// u[i,j] = sqrt(w_i) * Y[caller(i), j]; per-block max|u| partials and a non-finite count.
+// Caller-order column-major Y -> internal-order interleaved u = sqrt(w) .* y and y_int = y,
+// through a 64-row LDS tile: Y is read along rows (coalesced) and each internal row's k
+// values are written as one contiguous segment.  Non-finite entries are counted.
 template <bool WEIGHTED>
-__global__ __launch_bounds__(BLK) void k_load_rhs(uint64_t n, const uint32_t *__restrict__ obs_perm,
+__global__ __launch_bounds__(BLK) void k_load_rhs(uint64_t n, const uint32_t *__restrict__ inv_perm,
                                                   const double *__restrict__ sw, const double *__restrict__ Y,
-                                                  uint64_t ldy, double *__restrict__ u, double *__restrict__ part,
-                                                  int32_t *__restrict__ bad, uint32_t k, uint32_t kshift) {
+                                                  uint64_t ldy, double *__restrict__ u, double *__restrict__ y_int,
+                                                  int32_t *__restrict__ bad, uint32_t k) {
+    __shared__ double tile[64][65];
+    const uint64_t r0 = (uint64_t)blockIdx.x * 64;
+    int32_t nbad = 0;
+    for (uint32_t q = threadIdx.x; q < 64 * k; q += BLK) {
+        const uint32_t c = q >> 6, rr = q & 63;
+        const uint64_t r = r0 + rr;
+        if (r < n) {
+            const double y = Y[r + c * ldy];
+            nbad += !isfinite(y);
+            tile[rr][c] = y;
+        }
+    }
+    if (nbad) atomicAdd(bad, nbad);
+    __syncthreads();
+    for (uint32_t q = threadIdx.x; q < 64 * k; q += BLK) {
+        const uint32_t rr = q / k, c = q - rr * k;
+        const uint64_t r = r0 + rr;
+        if (r < n) {
+            const uint64_t i = inv_perm ? inv_perm[r] : r;
+            const double y = tile[rr][c];
+            y_int[i * k + c] = y;
+            u[i * k + c] = WEIGHTED ? sw[i] * y : y;
+        }
+    }
+}
+
+// Per-block max|u| partials (for vec_norm's scale).
+__global__ __launch_bounds__(BLK) void k_absmax(uint64_t n, const double *__restrict__ u, double *__restrict__ part,
+                                                uint32_t k, uint32_t kshift) {
     __shared__ double sh[BLK];
     const uint32_t kp = 1u << kshift;
     const uint32_t j = threadIdx.x & (kp - 1);
     const uint64_t rstride = ((uint64_t)gridDim.x * BLK) >> kshift;
     double mx = 0.0;
-    if (j < k) {
-        for (uint64_t i = ((uint64_t)blockIdx.x * BLK + threadIdx.x) >> kshift; i < n; i += rstride) {
-            const uint64_t src = obs_perm ? obs_perm[i] : i;
-            const double y = Y[src + j * ldy];
-            if (!isfinite(y)) atomicAdd(bad, 1);
-            const double b = WEIGHTED ? sw[i] * y : y;
-            u[i * k + j] = b;
-            mx = fmax(mx, fabs(b));
-        }
-    }
+    if (j < k)
+        for (uint64_t i = ((uint64_t)blockIdx.x * BLK + threadIdx.x) >> kshift; i < n; i += rstride)
+            mx = fmax(mx, fabs(u[i * k + j]));
     sh[threadIdx.x] = mx;
     __syncthreads();
     for (uint32_t s = BLK / 2; s >= kp && s > 0; s >>= 1) {
@@ -557,13 +590,11 @@ __global__ __launch_bounds__(BLK) void k_update_solution(uint64_t N, double *__r
     }
 }
 
-// demeaned[caller(i), j] = y[caller(i), j] - (D x)[i, j]; the internal-order copy goes to
-// dm_int (n x k interleaved) for the residual check.  solver.rs:469-473, 483.
-__global__ __launch_bounds__(BLK) void k_demean(DesignArgs d, const uint32_t *__restrict__ obs_perm,
-                                                const double *__restrict__ x, const double *__restrict__ Y,
-                                                uint64_t ldy, double *__restrict__ Dm, uint64_t ldd,
-                                                double *__restrict__ dm_int, double *__restrict__ y_int, uint32_t k,
-                                                uint32_t kshift) {
+// dm_int[i, j] = y_int[i, j] - (D x)[i, j] in internal order (solver.rs:469-473; the gather
+// is unweighted and folds each term from -0.0 like the operator's).
+__global__ __launch_bounds__(BLK) void k_demean(DesignArgs d, const double *__restrict__ x,
+                                                const double *__restrict__ y_int, double *__restrict__ dm_int,
+                                                uint32_t k, uint32_t kshift) {
     const uint32_t kp = 1u << kshift;
     const uint32_t j = threadIdx.x & (kp - 1);
     if (j >= k) return;
@@ -578,12 +609,27 @@ __global__ __launch_bounds__(BLK) void k_demean(DesignArgs d, const uint32_t *__
                 tv = tv + x[(ti.offset + c * ti.n_levels + code) * k + j] * term_load(d, ti.load[c], i);
             acc += tv;
         }
-        const uint64_t src = obs_perm ? obs_perm[i] : i;
-        const double y = Y[src + j * ldy];
-        const double r = y - acc;
-        if (Dm) Dm[src + j * ldd] = r;
-        dm_int[i * k + j] = r;
-        y_int[i * k + j] = y;
+        dm_int[i * k + j] = y_int[i * k + j] - acc;
+    }
+}
+
+// Internal-order interleaved dm_int -> caller-order column-major Dm (solver.rs:483), through a
+// 64-row LDS tile so the column-major writes are coalesced.
+__global__ __launch_bounds__(BLK) void k_store_caller(uint64_t n, const uint32_t *__restrict__ inv_perm,
+                                                      const double *__restrict__ dm_int, double *__restrict__ Dm,
+                                                      uint64_t ldd, uint32_t k) {
+    __shared__ double tile[64][65];
+    const uint64_t r0 = (uint64_t)blockIdx.x * 64;
+    for (uint32_t q = threadIdx.x; q < 64 * k; q += BLK) {
+        const uint32_t rr = q / k, c = q - rr * k;
+        const uint64_t r = r0 + rr;
+        if (r < n) tile[rr][c] = dm_int[(inv_perm ? inv_perm[r] : r) * k + c];
+    }
+    __syncthreads();
+    for (uint32_t q = threadIdx.x; q < 64 * k; q += BLK) {
+        const uint32_t c = q >> 6, rr = q & 63;
+        const uint64_t r = r0 + rr;
+        if (r < n) Dm[r + c * ldd] = tile[rr][c];
     }
 }
 
@@ -657,7 +703,14 @@ void upload_design(Solver &s, const std::vector<double> &weights_internal) {
         dv.sw.upload(sw.data(), sw.size());
         dv.w.upload(weights_internal.data(), weights_internal.size());
     }
-    if (!d.obs_perm.empty()) dv.obs_perm.upload(d.obs_perm.data(), d.obs_perm.size());
+    if (!d.obs_perm.empty()) {
+        dv.obs_perm.upload(d.obs_perm.data(), d.obs_perm.size());
+        std::vector<uint32_t> inv(d.obs_perm.size());
+        for (uint64_t i = 0; i < inv.size(); ++i) inv[d.obs_perm[i]] = (uint32_t)i;
+        dv.inv_perm.upload(inv.data(), inv.size());
+    }
+    if (s.precond && s.precond->kind == Precond::DIAGONAL)
+        dv.inv_diag.upload(s.precond->inv_diag.data(), s.precond->inv_diag.size());
     dv.terms.clear();
     dv.terms.resize(d.terms.size());
     dv.max_partials = 0;
@@ -735,7 +788,7 @@ void launch_adjoint(const DevDesign &dv, const DesignArgs &da, const double *sca
 #undef WX_SEG_NC
 #undef WX_SEG
         if (dt.n_fixes) {
-            const uint32_t fgrid = (dt.n_fixes + groups_per_block - 1) / groups_per_block;
+            const uint32_t fgrid = dt.n_fixes;
             switch (dt.info.ncols) {
             case 1: hipLaunchKernelGGL((k_segfix<1>), dim3(fgrid), dim3(BLK), 0, st, dt.info, dt.fixes.as<SegFix>(), dt.n_fixes, partials, out, active, k, kshift); break;
             case 2: hipLaunchKernelGGL((k_segfix<2>), dim3(fgrid), dim3(BLK), 0, st, dt.info, dt.fixes.as<SegFix>(), dt.n_fixes, partials, out, active, k, kshift); break;
@@ -783,16 +836,27 @@ struct PhaseTimer {
     }
 };
 
+// Device workspace of one batch width, reused across solves of a Solver (a pool keyed by
+// width; a solve takes one out and returns it, so concurrent solves never share one).
 struct Workspace {
-    DevBuf u, v, pt, atu, h, hb, x, part, segpart, cols, ints, scratch_y;
+    uint32_t k = 0;
+    DevBuf u, v, pt, atu, h, hb, x, yint, part, segpart, cols, ints, bad, red;
     ColState cs{};
+    std::unique_ptr<SchwarzRun> runs[2];
+    // pinned copies of the per-column "active" flags of the last two iterations, and the
+    // events that mark them complete (the host polls one iteration behind the device)
+    int32_t *hflags = nullptr;
+    hipEvent_t flag_ev[2] = {nullptr, nullptr};
+    ~Workspace() {
+        if (hflags) (void)hipHostFree(hflags);
+        for (auto &e : flag_ev)
+            if (e) (void)hipEventDestroy(e);
+    }
 };
 
 void setup_colstate(Workspace &ws) {
     ws.cols.alloc(sizeof(double) * MAX_BATCH * N_DOUBLE_FIELDS);
     ws.ints.alloc(sizeof(int32_t) * MAX_BATCH * N_INT_FIELDS);
-    WX_HIP(hipMemset(ws.cols.p, 0, ws.cols.bytes));
-    WX_HIP(hipMemset(ws.ints.p, 0, ws.ints.bytes));
     double *dp = ws.cols.as<double>();
     double **fields[N_DOUBLE_FIELDS] = {&ws.cs.alpha, &ws.cs.beta, &ws.cs.beta_prev_inv, &ws.cs.beta_inv,
                                         &ws.cs.p_coeff, &ws.cs.gscale, &ws.cs.alpha_bar, &ws.cs.phi_bar,
@@ -805,6 +869,49 @@ void setup_colstate(Workspace &ws) {
     int32_t **ifields[N_INT_FIELDS] = {&ws.cs.active, &ws.cs.upd, &ws.cs.brk, &ws.cs.stop, &ws.cs.iters, &ws.cs.err};
     for (int f = 0; f < N_INT_FIELDS; ++f) *ifields[f] = ip + f * MAX_BATCH;
 }
+
+std::shared_ptr<Workspace> new_workspace(const DevDesign &dv, uint32_t k) {
+    auto w = std::make_shared<Workspace>();
+    const uint64_t n = dv.n, N = dv.N, kp = 1u << pow2_shift(k);
+    w->k = k;
+    setup_colstate(*w);
+    w->u.alloc(n * k * 8);
+    w->yint.alloc(n * k * 8);
+    for (DevBuf *b : {&w->v, &w->pt, &w->atu, &w->h, &w->hb, &w->x}) b->alloc(N * k * 8);
+    w->part.alloc((size_t)MAX_PART_BLOCKS * kp * 3 * 8);
+    w->segpart.alloc((size_t)std::max<uint32_t>(dv.max_partials, 1) * k * 8);
+    w->bad.alloc(4);
+    w->red.alloc(2 * MAX_BATCH * 8);
+    WX_HIP(hipHostMalloc((void **)&w->hflags, 2 * MAX_BATCH * sizeof(int32_t), hipHostMallocDefault));
+    for (auto &e : w->flag_ev) WX_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    return w;
+}
+
+// Takes a workspace of width k out of the solver's pool (or makes one) and returns it on scope exit.
+struct PooledWorkspace {
+    const Solver &s;
+    std::shared_ptr<Workspace> ws;
+    PooledWorkspace(const Solver &solver, uint32_t k) : s(solver) {
+        {
+            std::lock_guard<std::mutex> lk(s.pool_mu);
+            for (size_t i = 0; i < s.pool.size(); ++i)
+                if (s.pool_k[i] == k) {
+                    ws = std::static_pointer_cast<Workspace>(s.pool[i]);
+                    s.pool.erase(s.pool.begin() + i);
+                    s.pool_k.erase(s.pool_k.begin() + i);
+                    break;
+                }
+        }
+        if (!ws) ws = new_workspace(s.dev, k);
+    }
+    ~PooledWorkspace() {
+        std::lock_guard<std::mutex> lk(s.pool_mu);
+        if (s.pool.size() < 4) {
+            s.pool.push_back(ws);
+            s.pool_k.push_back(ws->k);
+        }
+    }
+};
 
 } // namespace
 
@@ -821,13 +928,12 @@ void solve_batch_device(const Solver &s, const double *Y, uint64_t ldy, uint64_t
     if (k_total == 0) return;
     const DesignArgs da = dv.args();
     const bool weighted = dv.sw.p != nullptr;
-    const uint32_t *obs_perm = dv.obs_perm.p ? dv.obs_perm.as<uint32_t>() : nullptr;
+    const uint32_t *inv_perm = dv.inv_perm.p ? dv.inv_perm.as<uint32_t>() : nullptr;
     const int maxiter = (int)std::max<int64_t>(0, opts.maxiter);
     const Precond *pc = s.precond.get();
     const int prec_kind = pc == nullptr ? 0 : (pc->kind == Precond::DIAGONAL ? 1 : 2);
-    DevBuf inv_diag;
-    if (prec_kind == 1) inv_diag.upload(pc->inv_diag.data(), pc->inv_diag.size(), st);
-    std::unique_ptr<SchwarzRun> schwarz_run;
+    const double *inv_diag = prec_kind == 1 ? dv.inv_diag.as<double>() : nullptr;
+    const int prec_slot = opts.precision == WX_PRECISION_MIXED ? 1 : 0;
     PhaseTimer timer;
     timer.on = opts.timing != 0;
     timer.st = st;
@@ -837,38 +943,30 @@ void solve_batch_device(const Solver &s, const double *Y, uint64_t ldy, uint64_t
         const uint32_t kshift = pow2_shift(k);
         const uint32_t kp = 1u << kshift;
         auto t0 = std::chrono::steady_clock::now();
-        Workspace ws;
-        setup_colstate(ws);
-        ws.u.alloc(n * k * 8);
-        ws.v.alloc(N * k * 8);
-        ws.pt.alloc(N * k * 8);
-        ws.atu.alloc(N * k * 8);
-        ws.h.alloc(N * k * 8);
-        ws.hb.alloc(N * k * 8);
-        ws.x.alloc(N * k * 8);
-        ws.part.alloc((size_t)MAX_PART_BLOCKS * kp * 3 * 8);
-        ws.segpart.alloc((size_t)std::max<uint32_t>(dv.max_partials, 1) * k * 8);
+        PooledWorkspace pw(s, k);
+        Workspace &ws = *pw.ws;
+        if (prec_kind == 2 && !ws.runs[prec_slot]) ws.runs[prec_slot] = make_schwarz_run(*pc, opts.precision, st);
+        SchwarzRun *schwarz_run = prec_kind == 2 ? ws.runs[prec_slot].get() : nullptr;
         ColState cs = ws.cs;
-        if (prec_kind == 2 && !schwarz_run) schwarz_run = make_schwarz_run(*pc, opts.precision, st);
         const uint32_t gn = grid_for(n, kshift), gN = grid_for(N, kshift);
         double *part = ws.part.as<double>();
         const double *Yc = Y + c0 * ldy;
 
         // --- right-hand side: b = sqrt(W) y (internal order), beta_1 = vec_norm(b) ---
-        DevBuf bad;
-        bad.alloc(4);
-        WX_HIP(hipMemsetAsync(bad.p, 0, 4, st));
+        WX_HIP(hipMemsetAsync(ws.bad.p, 0, 4, st));
+        const uint32_t gt = (uint32_t)((n + 63) / 64);
         if (weighted)
-            hipLaunchKernelGGL((k_load_rhs<true>), dim3(gn), dim3(BLK), 0, st, n, obs_perm, dv.sw.as<double>(), Yc, ldy,
-                               ws.u.as<double>(), part, bad.as<int32_t>(), k, kshift);
+            hipLaunchKernelGGL((k_load_rhs<true>), dim3(gt), dim3(BLK), 0, st, n, inv_perm, dv.sw.as<double>(), Yc, ldy,
+                               ws.u.as<double>(), ws.yint.as<double>(), ws.bad.as<int32_t>(), k);
         else
-            hipLaunchKernelGGL((k_load_rhs<false>), dim3(gn), dim3(BLK), 0, st, n, obs_perm, nullptr, Yc, ldy,
-                               ws.u.as<double>(), part, bad.as<int32_t>(), k, kshift);
+            hipLaunchKernelGGL((k_load_rhs<false>), dim3(gt), dim3(BLK), 0, st, n, inv_perm, nullptr, Yc, ldy,
+                               ws.u.as<double>(), ws.yint.as<double>(), ws.bad.as<int32_t>(), k);
+        hipLaunchKernelGGL(k_absmax, dim3(gn), dim3(BLK), 0, st, n, ws.u.as<double>(), part, k, kshift);
         hipLaunchKernelGGL(k_colmax, dim3(k), dim3(BLK), 0, st, part, gn, kp, cs);
         hipLaunchKernelGGL(k_scaled_sumsq, dim3(gn), dim3(BLK), 0, st, n, ws.u.as<double>(), cs, part, k, kshift);
         hipLaunchKernelGGL(k_beta1, dim3(k), dim3(BLK), 0, st, part, gn, kp, cs, opts.tol, k);
         int32_t nbad = 0;
-        WX_HIP(hipMemcpyAsync(&nbad, bad.p, 4, hipMemcpyDeviceToHost, st));
+        WX_HIP(hipMemcpyAsync(&nbad, ws.bad.p, 4, hipMemcpyDeviceToHost, st));
         WX_HIP(hipStreamSynchronize(st));
         if (nbad) fail(WX_ERR_INVALID_INPUT, "Solver::solve: response contains non-finite values");
         hipLaunchKernelGGL(k_scale_obs, dim3(gn), dim3(BLK), 0, st, n, ws.u.as<double>(), cs, k, kshift);
@@ -881,7 +979,7 @@ void solve_batch_device(const Solver &s, const double *Y, uint64_t ldy, uint64_t
             hipLaunchKernelGGL(k_dots, dim3(gN), dim3(BLK), 0, st, N, ws.pt.as<double>(), ws.v.as<double>(), cs, part, k, kshift);
         } else if (prec_kind == 1) {
             hipLaunchKernelGGL((k_ptilde<1, true>), dim3(gN), dim3(BLK), 0, st, N, nullptr, ws.pt.as<double>(),
-                               ws.v.as<double>(), inv_diag.as<double>(), cs, part, k, kshift);
+                               ws.v.as<double>(), inv_diag, cs, part, k, kshift);
         } else {
             hipLaunchKernelGGL((k_ptilde<0, true>), dim3(gN), dim3(BLK), 0, st, N, nullptr, ws.pt.as<double>(),
                                ws.v.as<double>(), nullptr, cs, part, k, kshift);
@@ -893,7 +991,7 @@ void solve_batch_device(const Solver &s, const double *Y, uint64_t ldy, uint64_t
 
         // --- iterations ---
         int32_t host_active[MAX_BATCH];
-        int check_every = opts.check_every > 0 ? opts.check_every : 4;
+        const int check_every = opts.check_every > 0 ? opts.check_every : 1;
         int itn_done = 0;
         for (int itn = 1; itn <= maxiter; ++itn) {
             timer.begin(3);
@@ -920,7 +1018,7 @@ void solve_batch_device(const Solver &s, const double *Y, uint64_t ldy, uint64_t
                                    part, k, kshift);
             } else if (prec_kind == 1) {
                 hipLaunchKernelGGL((k_ptilde<1, false>), dim3(gN), dim3(BLK), 0, st, N, ws.atu.as<double>(),
-                                   ws.pt.as<double>(), ws.v.as<double>(), inv_diag.as<double>(), cs, part, k, kshift);
+                                   ws.pt.as<double>(), ws.v.as<double>(), inv_diag, cs, part, k, kshift);
             } else {
                 hipLaunchKernelGGL((k_ptilde<0, false>), dim3(gN), dim3(BLK), 0, st, N, ws.atu.as<double>(),
                                    ws.pt.as<double>(), ws.v.as<double>(), nullptr, cs, part, k, kshift);
@@ -930,29 +1028,37 @@ void solve_batch_device(const Solver &s, const double *Y, uint64_t ldy, uint64_t
                                ws.hb.as<double>(), ws.x.as<double>(), cs, k, kshift);
             timer.end(3);
             itn_done = itn;
-            if (itn % check_every == 0 || itn == maxiter) {
+            if (check_every == 1) {
+                // lag-one polling: record this iteration's flags, then wait for the previous
+                // iteration's while the device already runs this one
+                const int slot = itn & 1;
+                WX_HIP(hipMemcpyAsync(ws.hflags + slot * MAX_BATCH, cs.active, k * 4, hipMemcpyDeviceToHost, st));
+                WX_HIP(hipEventRecord(ws.flag_ev[slot], st));
+                if (itn >= 2) {
+                    WX_HIP(hipEventSynchronize(ws.flag_ev[slot ^ 1]));
+                    bool any = false;
+                    for (uint32_t j = 0; j < k; ++j) any |= ws.hflags[(slot ^ 1) * MAX_BATCH + j] != 0;
+                    if (!any) break;
+                }
+            } else if (itn % check_every == 0 || itn == maxiter) {
                 WX_HIP(hipGetLastError());
                 WX_HIP(hipMemcpyAsync(host_active, cs.active, k * 4, hipMemcpyDeviceToHost, st));
                 WX_HIP(hipStreamSynchronize(st));
                 bool any = false;
                 for (uint32_t j = 0; j < k; ++j) any |= host_active[j] != 0;
                 if (!any) break;
-                if (check_every < 32 && itn >= 8 * check_every) check_every *= 2;
             }
         }
+        WX_HIP(hipGetLastError());
         (void)itn_done;
 
         // --- outputs: x, demeaned (caller order), FP64 true residual ---
         double *dm_int = ws.u.as<double>();
-        double *y_int = ws.atu.p && N >= n ? ws.atu.as<double>() : nullptr;
-        DevBuf yint_buf;
-        if (!y_int) {
-            yint_buf.alloc(n * k * 8);
-            y_int = yint_buf.as<double>();
-        }
+        double *y_int = ws.yint.as<double>();
         const uint32_t gd = grid_for(n, kshift, 65535);
-        hipLaunchKernelGGL(k_demean, dim3(gd), dim3(BLK), 0, st, da, obs_perm, ws.x.as<double>(), Yc, ldy,
-                           Dm ? Dm + c0 * n : nullptr, n, dm_int, y_int, k, kshift);
+        hipLaunchKernelGGL(k_demean, dim3(gd), dim3(BLK), 0, st, da, ws.x.as<double>(), y_int, dm_int, k, kshift);
+        if (Dm)
+            hipLaunchKernelGGL(k_store_caller, dim3(gt), dim3(BLK), 0, st, n, inv_perm, dm_int, Dm + c0 * n, n, k);
         if (X && !s.reparam) {
             uint64_t tot = N * k;
             hipLaunchKernelGGL(k_store_x, dim3((tot + BLK - 1) / BLK), dim3(BLK), 0, st, N, ws.x.as<double>(),
@@ -965,8 +1071,7 @@ void solve_batch_device(const Solver &s, const double *Y, uint64_t ldy, uint64_t
         launch_adjoint(dv, da, wv, weighted ? 1 : 0, y_int, ws.hb.as<double>(), ws.segpart.as<double>(), nullptr, k,
                        kshift, st);
         double num[MAX_BATCH], den[MAX_BATCH];
-        DevBuf red;
-        red.alloc(2 * MAX_BATCH * 8);
+        DevBuf &red = ws.red;
         hipLaunchKernelGGL(k_sumsq, dim3(gN), dim3(BLK), 0, st, N, ws.h.as<double>(), part, k, kshift);
         hipLaunchKernelGGL(k_reduce_to, dim3(k), dim3(BLK), 0, st, part, gN, kp, red.as<double>());
         hipLaunchKernelGGL(k_sumsq, dim3(gN), dim3(BLK), 0, st, N, ws.hb.as<double>(), part, k, kshift);

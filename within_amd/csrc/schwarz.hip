@@ -31,6 +31,7 @@ template <typename T> inline T *dp(const DevBuf &b) { return static_cast<T *>(b.
 constexpr int BLK = 256;
 constexpr uint32_t ROW_CHUNK = 256;   // CSR entries per work item before a row is split
 constexpr uint32_t MEAN_CHUNK = 256;  // slots per partial-sum chunk
+constexpr uint32_t DENSE_TILED_MIN = 256; // kept blocks at least this large use the tiled product
 
 struct RowItem {
     uint32_t row;   // slot id of the output row
@@ -65,6 +66,10 @@ struct SchwarzDev {
     // dense kept blocks: one entry per kept row: (matrix row offset, first kept slot, nk)
     DevBuf drow_off, drow_base, drow_nk, drow_slot, dmat;
     uint32_t n_drows = 0;
+    // large dense kept blocks: split-K tiles, per-component segment counts and row starts
+    DevBuf dtiles, dcomps, drow_start;
+    uint32_t n_dtiles = 0, n_dcomps = 0;
+    uint64_t dense_part_rows = 0;
     // sparse kept blocks (one workgroup per component): levels and factor rows
     DevBuf sp_comp, sp_lvl_f_ptr, sp_lvl_b_ptr, sp_lvl_f_rows, sp_lvl_b_rows, sp_meta;
     DevBuf l_ptr, l_idx, l_val, lt_ptr, lt_idx, lt_val, piv;
@@ -116,19 +121,27 @@ __global__ __launch_bounds__(BLK) void sk_chunk_sum(const Chunk *__restrict__ ch
     part[(uint64_t)g * k + j] = acc;
 }
 
-// mean per component = sum of its chunk partials / count
-__global__ __launch_bounds__(BLK) void sk_comp_mean(uint64_t nc, const uint32_t *__restrict__ cptr,
-                                                    const double *__restrict__ part,
+// mean per component = sum of its chunk partials / count; one workgroup per component,
+// row groups stride over the chunks, then a fixed-order LDS tree across the groups
+__global__ __launch_bounds__(BLK) void sk_comp_mean(const uint32_t *__restrict__ cptr, const double *__restrict__ part,
                                                     const uint32_t *__restrict__ count, double *__restrict__ mu,
                                                     uint32_t k, uint32_t kshift) {
-    const uint32_t kp = 1u << kshift, j = threadIdx.x & (kp - 1);
-    if (j >= k) return;
-    const uint64_t c = ((uint64_t)blockIdx.x * BLK + threadIdx.x) >> kshift;
-    if (c >= nc) return;
+    __shared__ double sh[BLK];
+    const uint32_t kp = 1u << kshift, j = threadIdx.x & (kp - 1), grp = threadIdx.x >> kshift;
+    const uint32_t c = blockIdx.x;
     double acc = 0.0;
-    for (uint32_t q = cptr[c]; q < cptr[c + 1]; ++q) acc += part[(uint64_t)q * k + j];
-    const uint32_t n = count[c];
-    mu[c * k + j] = n ? acc / (double)n : 0.0;
+    if (j < k)
+        for (uint32_t q = cptr[c] + grp; q < cptr[c + 1]; q += BLK >> kshift) acc += part[(uint64_t)q * k + j];
+    sh[threadIdx.x] = acc;
+    __syncthreads();
+    for (uint32_t s = BLK / 2; s >= kp && s > 0; s >>= 1) {
+        if (threadIdx.x < s) sh[threadIdx.x] += sh[threadIdx.x + s];
+        __syncthreads();
+    }
+    if (threadIdx.x < k) {
+        const uint32_t n = count[c];
+        mu[(uint64_t)c * k + threadIdx.x] = n ? sh[threadIdx.x] / (double)n : 0.0;
+    }
 }
 
 // MODE 0: reduced rhs of kept rows:  red[row] = (loc[row]-mu) + sum_p ct[p]*inv_de[e]*(loc[e]-mu)
@@ -189,7 +202,91 @@ __global__ __launch_bounds__(BLK) void sk_rows_fix(const RowFix *__restrict__ fi
     row_finish<CT, MODE>(f.row, acc, loc, out, inv_de, mu + (uint64_t)slot_comp[f.row] * k, k, j);
 }
 
-// Dense kept-block solve: xs[row] = sum_l S^+[i, l] * red[l]   (S^+ = P S_g^{-1} P)
+// Large dense kept blocks: xs = S^+ red as a split-K tiled product.  A tile is 64 rows x a
+// segment of the columns of S^+; each 64 x 64 sub-block of S^+ is staged in LDS (coalesced
+// row reads), the matching 64 x k slab of red likewise, and 256 threads each keep
+// ceil(k/4) FP64 accumulators (row = tid/4, columns tid%4 + 4i).  Segment partials are
+// summed in a fixed order by sk_dense_reduce, so the result is deterministic.
+struct DenseTile {
+    uint64_t mat_off;   // S^+ of the component (row-major nk x nk)
+    uint32_t nk, kept_base;
+    uint32_t row0, l0, l1;
+    uint32_t seg;       // segment index of [l0, l1)
+    uint64_t part_off;  // partial rows of this component: part[(part_off + seg*nk + i)*k + c]
+};
+struct DenseComp {
+    uint32_t nk, kept_base, n_seg, pad;
+    uint64_t part_off;
+};
+
+template <typename CT, int KC>
+__global__ __launch_bounds__(256) void sk_dense_tiled(const DenseTile *__restrict__ tiles, const CT *__restrict__ mat,
+                                                      const double *__restrict__ red, double *__restrict__ part,
+                                                      uint32_t kstride, uint32_t c0, uint32_t k) {
+    __shared__ CT Ms[64][65];
+    __shared__ double Rs[64 * 4 * KC];
+    const DenseTile T = tiles[blockIdx.x];
+    const uint32_t t = threadIdx.x, r = t >> 2, g = t & 3;
+    double acc[KC];
+#pragma unroll
+    for (int i = 0; i < KC; ++i) acc[i] = 0.0;
+    const CT *M = mat + T.mat_off;
+    for (uint32_t lc = T.l0; lc < T.l1; lc += 64) {
+        for (uint32_t q = t; q < 64 * 64; q += 256) {
+            const uint32_t rr = q >> 6, ll = q & 63, row = T.row0 + rr, col = lc + ll;
+            Ms[rr][ll] = (row < T.nk && col < T.l1) ? M[(uint64_t)row * T.nk + col] : (CT)0;
+        }
+        for (uint32_t q = t; q < 64 * k; q += 256) {
+            const uint32_t ll = q / k, c = q - ll * k, l = lc + ll;
+            Rs[ll * k + c] = l < T.l1 ? red[(uint64_t)(T.kept_base + l) * kstride + c0 + c] : 0.0;
+        }
+        __syncthreads();
+#pragma unroll 4
+        for (uint32_t ll = 0; ll < 64; ++ll) {
+            const double m = (double)Ms[r][ll];
+#pragma unroll
+            for (int i = 0; i < KC; ++i) {
+                const uint32_t c = g + 4 * i;
+                if (c < k) acc[i] += m * Rs[ll * k + c];
+            }
+        }
+        __syncthreads();
+    }
+    const uint32_t row = T.row0 + r;
+    if (row < T.nk) {
+#pragma unroll
+        for (int i = 0; i < KC; ++i) {
+            const uint32_t c = g + 4 * i;
+            if (c < k) part[(T.part_off + (uint64_t)T.seg * T.nk + row) * kstride + c0 + c] = acc[i];
+        }
+    }
+}
+
+// xs[kept_base + i] = sum over segments (in order) of the partials
+__global__ __launch_bounds__(BLK) void sk_dense_reduce(const DenseComp *__restrict__ comps, uint32_t n_comps,
+                                                       const uint64_t *__restrict__ row_start,
+                                                       const double *__restrict__ part, double *__restrict__ xs,
+                                                       uint32_t k, uint32_t kshift) {
+    const uint32_t kp = 1u << kshift, j = threadIdx.x & (kp - 1);
+    if (j >= k) return;
+    const uint64_t total = row_start[n_comps];
+    const uint64_t stride = ((uint64_t)gridDim.x * BLK) >> kshift;
+    for (uint64_t q = ((uint64_t)blockIdx.x * BLK + threadIdx.x) >> kshift; q < total; q += stride) {
+        uint32_t lo = 0, hi = n_comps - 1;
+        while (lo < hi) { // component owning global dense row q
+            const uint32_t mid = (lo + hi + 1) / 2;
+            if (row_start[mid] <= q) lo = mid;
+            else hi = mid - 1;
+        }
+        const DenseComp C = comps[lo];
+        const uint64_t i = q - row_start[lo];
+        double acc = 0.0;
+        for (uint32_t s = 0; s < C.n_seg; ++s) acc += part[(C.part_off + (uint64_t)s * C.nk + i) * k + j];
+        xs[(uint64_t)(C.kept_base + i) * k + j] = acc;
+    }
+}
+
+// Dense kept-block solve for small blocks: xs[row] = sum_l S^+[i, l] * red[l]   (S^+ = P S_g^{-1} P)
 template <typename CT>
 __global__ __launch_bounds__(BLK) void sk_dense(uint32_t n_rows, const uint64_t *__restrict__ row_off,
                                                 const uint32_t *__restrict__ row_base,
@@ -433,13 +530,38 @@ std::shared_ptr<SchwarzDev<CT>> build_dev(const Precond::Schwarz &S) {
     D->kept_cptr.upload(kept_cptr.data(), kept_cptr.size());
     D->comp_count.upload(cnt.data(), cnt.size());
     D->comp_kcount.upload(kcnt.data(), kcnt.size());
-    // dense rows
+    // dense rows: small blocks row by row, blocks with nk >= DENSE_TILED_MIN as split-K tiles
     std::vector<uint64_t> drow_off;
     std::vector<uint32_t> drow_base, drow_nk, drow_slot;
+    std::vector<DenseComp> dcomps;
+    std::vector<uint64_t> drow_start{0};
+    uint64_t row_tiles = 0;
+    for (uint64_t c = 0; c < nc; ++c)
+        if (S.fac_kind[c] == 0 && S.nk[c] >= DENSE_TILED_MIN) row_tiles += (S.nk[c] + 63) / 64;
+    std::vector<DenseTile> tiles;
+    uint64_t part_rows = 0;
     for (uint64_t c = 0; c < nc; ++c) {
         if (S.fac_kind[c] != 0) continue;
         const uint32_t nk = S.nk[c];
         const uint32_t kb = (uint32_t)(S.slot_start[c] + S.ne[c]);
+        if (nk >= DENSE_TILED_MIN) {
+            // enough (row tile, column segment) pairs to fill the device, segments >= 256 columns
+            const uint32_t max_seg = (nk + 255) / 256;
+            const uint32_t want = (uint32_t)std::max<uint64_t>(1, (2048 + row_tiles - 1) / row_tiles);
+            const uint32_t n_seg = std::min(max_seg, want);
+            const uint32_t seg_len = ((nk + n_seg - 1) / n_seg + 63) / 64 * 64;
+            DenseComp dc{nk, kb, 0, 0, part_rows};
+            for (uint32_t s = 0; s * seg_len < nk; ++s) {
+                for (uint32_t r0 = 0; r0 < nk; r0 += 64)
+                    tiles.push_back({S.dense_off[c], nk, kb, r0, s * seg_len, std::min(nk, (s + 1) * seg_len), s,
+                                     part_rows});
+                dc.n_seg++;
+            }
+            part_rows += (uint64_t)dc.n_seg * nk;
+            dcomps.push_back(dc);
+            drow_start.push_back(drow_start.back() + nk);
+            continue;
+        }
         for (uint32_t i = 0; i < nk; ++i) {
             drow_off.push_back(S.dense_off[c] + (uint64_t)i * nk);
             drow_base.push_back(kb);
@@ -447,6 +569,12 @@ std::shared_ptr<SchwarzDev<CT>> build_dev(const Precond::Schwarz &S) {
             drow_slot.push_back(kb + i);
         }
     }
+    D->n_dtiles = (uint32_t)tiles.size();
+    D->dtiles.upload(tiles.data(), tiles.size());
+    D->n_dcomps = (uint32_t)dcomps.size();
+    D->dcomps.upload(dcomps.data(), dcomps.size());
+    D->drow_start.upload(drow_start.data(), drow_start.size());
+    D->dense_part_rows = part_rows;
     D->n_drows = (uint32_t)drow_off.size();
     D->drow_off.upload(drow_off.data(), drow_off.size());
     D->drow_base.upload(drow_base.data(), drow_base.size());
@@ -574,7 +702,7 @@ template <typename CT>
 struct SchwarzRunImpl : SchwarzRun {
     std::shared_ptr<SchwarzDev<CT>> D;
     uint32_t kcap = 0;
-    DevBuf loc, red, xs, part, rowpart, mu1, mu2, mu3;
+    DevBuf loc, red, xs, part, rowpart, mu1, mu2, mu3, dpart;
 
     void ensure(uint32_t k) {
         if (k <= kcap) return;
@@ -588,14 +716,15 @@ struct SchwarzRunImpl : SchwarzRun {
         mu1.alloc(D->nc * k * 8);
         mu2.alloc(D->nc * k * 8);
         mu3.alloc(D->nc * k * 8);
+        dpart.alloc(D->dense_part_rows * k * 8 + 8);
     }
 
     void mean(const DevBuf &chunks, uint32_t n_chunks, const DevBuf &cptr, const DevBuf &count, const double *src,
               double *mu, uint32_t k, uint32_t kshift, hipStream_t st) {
         hipLaunchKernelGGL(sk_chunk_sum, dim3(grid_rows(n_chunks, kshift)), dim3(BLK), 0, st, dp<Chunk>(chunks),
                            n_chunks, src, dp<double>(part), k, kshift);
-        hipLaunchKernelGGL(sk_comp_mean, dim3(grid_rows(D->nc, kshift)), dim3(BLK), 0, st, D->nc, dp<uint32_t>(cptr),
-                           dp<double>(part), dp<uint32_t>(count), mu, k, kshift);
+        hipLaunchKernelGGL(sk_comp_mean, dim3((uint32_t)D->nc), dim3(BLK), 0, st, dp<uint32_t>(cptr), dp<double>(part),
+                           dp<uint32_t>(count), mu, k, kshift);
     }
 
     void apply(const double *r, double *z, uint32_t k, hipStream_t st) override {
@@ -623,6 +752,23 @@ struct SchwarzRunImpl : SchwarzRun {
                                dp<uint64_t>(d.sp_lvl_b_ptr), dp<uint32_t>(d.sp_lvl_b_rows), dp<uint64_t>(d.l_ptr),
                                dp<uint32_t>(d.l_idx), dp<CT>(d.l_val), dp<uint64_t>(d.lt_ptr), dp<uint32_t>(d.lt_idx),
                                dp<CT>(d.lt_val), dp<CT>(d.piv), redp, dp<double>(mu2), xsp, k, kshift);
+        }
+        if (d.n_dtiles) {
+            for (uint32_t c0 = 0; c0 < k; c0 += 32) {
+                const uint32_t kc = std::min<uint32_t>(32, k - c0);
+                const uint32_t KC = (kc + 3) / 4;
+#define WX_DT(N)                                                                                                   \
+    hipLaunchKernelGGL((sk_dense_tiled<CT, N>), dim3(d.n_dtiles), dim3(256), 0, st, dp<DenseTile>(d.dtiles),      \
+                       dp<CT>(d.dmat), redp, dp<double>(dpart), k, c0, kc)
+                if (KC <= 1) WX_DT(1);
+                else if (KC <= 2) WX_DT(2);
+                else if (KC <= 4) WX_DT(4);
+                else WX_DT(8);
+#undef WX_DT
+            }
+            hipLaunchKernelGGL(sk_dense_reduce, dim3(grid_rows(d.dense_part_rows, kshift)), dim3(BLK), 0, st,
+                               dp<DenseComp>(d.dcomps), d.n_dcomps, dp<uint64_t>(d.drow_start), dp<double>(dpart), xsp,
+                               k, kshift);
         }
         if (d.n_drows)
             hipLaunchKernelGGL((sk_dense<CT>), dim3(grid_rows(d.n_drows, kshift)), dim3(BLK), 0, st, d.n_drows,

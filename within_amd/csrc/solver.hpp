@@ -59,7 +59,7 @@ struct DevTerm {
 
 struct DevDesign {
     uint64_t n = 0, N = 0;
-    DevBuf codes, slopes, sw, w, obs_perm;
+    DevBuf codes, slopes, sw, w, obs_perm, inv_perm, inv_diag;
     std::vector<DevTerm> terms;
     uint32_t max_partials = 0;
     DesignArgs args() const;
@@ -86,6 +86,10 @@ struct Solver {
     std::vector<std::string> warnings;
     double time_setup = 0.0;
     DevDesign dev;
+    // pool of reusable device workspaces (see lsmr.hip), keyed by batch width
+    mutable std::mutex pool_mu;
+    mutable std::vector<std::shared_ptr<void>> pool;
+    mutable std::vector<uint32_t> pool_k;
 };
 
 // Uploads the design (codes, slopes, weights, D' segment plans) to the current device.
