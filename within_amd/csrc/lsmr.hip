@@ -23,7 +23,7 @@ namespace {
 
 constexpr int BLK = 256;
 constexpr uint32_t SEG_CHUNK = 256; // rows per D' work item before a level is split
-constexpr uint32_t MAX_PART_BLOCKS = 1024;
+constexpr uint32_t MAX_PART_BLOCKS = 4096;
 
 // Per-column LSMR state (structure of arrays, each of length MAX_BATCH).
 struct ColState {
@@ -69,20 +69,42 @@ __global__ __launch_bounds__(BLK) void k_gather_axpy(DesignArgs d, const double 
     const bool act = j < k && cs.active[j];
     if (act) {
         const double scale = cs.gscale[j];
-        for (uint64_t i = ((uint64_t)blockIdx.x * BLK + threadIdx.x) >> kshift; i < d.n; i += rstride) {
-            double acc = 0.0;
+        // R rows per thread per pass, their loads issued together for memory-level parallelism
+        constexpr int R = 4;
+        for (uint64_t i0 = ((uint64_t)blockIdx.x * BLK + threadIdx.x) >> kshift; i0 < d.n; i0 += R * rstride) {
+            uint64_t rows[R];
+            bool ok[R];
+            double acc[R], uo[R];
+#pragma unroll
+            for (int q = 0; q < R; ++q) {
+                rows[q] = i0 + q * rstride;
+                ok[q] = rows[q] < d.n;
+                acc[q] = 0.0;
+                uo[q] = ok[q] ? u[rows[q] * k + j] : 0.0;
+            }
             for (uint32_t t = 0; t < d.T; ++t) {
                 const TermInfo &ti = d.terms[t];
-                const uint64_t code = d.codes[(uint64_t)t * d.n + i];
-                double tv = -0.0;
-                for (uint32_t c = 0; c < ti.ncols; ++c)
-                    tv = tv + v[(ti.offset + c * ti.n_levels + code) * k + j] * term_load(d, ti.load[c], i);
-                acc += tv;
+                uint64_t code[R];
+#pragma unroll
+                for (int q = 0; q < R; ++q) code[q] = ok[q] ? d.codes[(uint64_t)t * d.n + rows[q]] : 0;
+#pragma unroll
+                for (int q = 0; q < R; ++q) {
+                    double tv = -0.0;
+                    for (uint32_t c = 0; c < ti.ncols; ++c)
+                        tv = tv + v[(ti.offset + c * ti.n_levels + code[q]) * k + j] *
+                                      (ok[q] ? term_load(d, ti.load[c], rows[q]) : 0.0);
+                    acc[q] += tv;
+                }
             }
-            if (WEIGHTED) acc *= sw[i];
-            const double nu = acc + scale * u[i * k + j];
-            u[i * k + j] = nu;
-            sq += nu * nu;
+#pragma unroll
+            for (int q = 0; q < R; ++q) {
+                if (!ok[q]) continue;
+                double a = acc[q];
+                if (WEIGHTED) a *= sw[rows[q]];
+                const double nu = a + scale * uo[q];
+                u[rows[q] * k + j] = nu;
+                sq += nu * nu;
+            }
         }
     }
     double tot = block_reduce_cols(sq, kp, sh);
@@ -111,7 +133,26 @@ __global__ __launch_bounds__(BLK) void k_segreduce(DesignArgs d, TermInfo ti, co
         double acc[NC];
 #pragma unroll
         for (int c = 0; c < NC; ++c) acc[c] = 0.0;
-        for (uint32_t p = item.begin; p < item.end; ++p) {
+        // rows are read four at a time (independent loads in flight) but summed strictly in order
+        uint32_t p = item.begin;
+        for (; p + 4 <= item.end; p += 4) {
+            uint64_t row[4];
+            double b[4];
+#pragma unroll
+            for (int q = 0; q < 4; ++q) row[q] = perm ? perm[p + q] : p + q;
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                b[q] = src[row[q] * k + j];
+                if (SCALE == 1) b[q] = scale[row[q]] * b[q];
+                if (SCALE == 2) b[q] = (scale[row[q]] * scale[row[q]]) * b[q];
+            }
+#pragma unroll
+            for (int q = 0; q < 4; ++q)
+#pragma unroll
+                for (int c = 0; c < NC; ++c)
+                    acc[c] += (ti.load[c] < 0 ? b[q] : term_load(d, ti.load[c], row[q]) * b[q]);
+        }
+        for (; p < item.end; ++p) {
             const uint64_t row = perm ? perm[p] : p;
             double b = src[row * k + j];
             if (SCALE == 1) b = scale[row] * b;

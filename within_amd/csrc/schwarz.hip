@@ -179,8 +179,17 @@ __global__ __launch_bounds__(BLK) void sk_rows(const RowItem *__restrict__ items
         const RowItem item = items[it];
         const double *mu_c = mu + (uint64_t)slot_comp[item.row] * k;
         double acc = 0.0;
-        for (uint64_t p = item.begin; p < item.end; ++p)
-            acc += row_term<CT, MODE>(idx, val, p, loc, xs, inv_de, mu_c, k, j);
+        uint64_t p = item.begin;
+        for (; p + 4 <= item.end; p += 4) { // four independent gathers in flight, summed in order
+            double t[4];
+#pragma unroll
+            for (int q = 0; q < 4; ++q) t[q] = row_term<CT, MODE>(idx, val, p + q, loc, xs, inv_de, mu_c, k, j);
+            acc += t[0];
+            acc += t[1];
+            acc += t[2];
+            acc += t[3];
+        }
+        for (; p < item.end; ++p) acc += row_term<CT, MODE>(idx, val, p, loc, xs, inv_de, mu_c, k, j);
         if (item.dest >= 0) row_finish<CT, MODE>(item.row, acc, loc, out, inv_de, mu_c, k, j);
         else part[(uint64_t)(-item.dest - 1) * k + j] = acc;
     }
@@ -202,13 +211,14 @@ __global__ __launch_bounds__(BLK) void sk_rows_fix(const RowFix *__restrict__ fi
     row_finish<CT, MODE>(f.row, acc, loc, out, inv_de, mu + (uint64_t)slot_comp[f.row] * k, k, j);
 }
 
-// Large dense kept blocks: xs = S^+ red as a split-K tiled product.  A tile is 64 rows x a
-// segment of the columns of S^+; each 64 x 64 sub-block of S^+ is staged in LDS (coalesced
-// row reads), the matching 64 x k slab of red likewise, and 256 threads each keep
-// ceil(k/4) FP64 accumulators (row = tid/4, columns tid%4 + 4i).  Segment partials are
+// Large dense kept blocks: xs = S^+ red as a split-K product that streams S^+ once.  S^+ is
+// symmetric, so column i of S^+ is row i: one wave owns 64 consecutive output rows (one per
+// lane) and walks a segment [l0, l1) of l, reading S^+[l][row0 + lane] -- 64 consecutive
+// coefficients, one coalesced load per l -- while red[l][c0 .. c0+KC) is the same for every
+// lane (a wave-uniform load).  Each lane keeps KC FP64 accumulators.  Segment partials are
 // summed in a fixed order by sk_dense_reduce, so the result is deterministic.
 struct DenseTile {
-    uint64_t mat_off;   // S^+ of the component (row-major nk x nk)
+    uint64_t mat_off;   // S^+ of the component (row-major nk x nk, symmetric)
     uint32_t nk, kept_base;
     uint32_t row0, l0, l1;
     uint32_t seg;       // segment index of [l0, l1)
@@ -220,45 +230,39 @@ struct DenseComp {
 };
 
 template <typename CT, int KC>
-__global__ __launch_bounds__(256) void sk_dense_tiled(const DenseTile *__restrict__ tiles, const CT *__restrict__ mat,
-                                                      const double *__restrict__ red, double *__restrict__ part,
-                                                      uint32_t kstride, uint32_t c0, uint32_t k) {
-    __shared__ CT Ms[64][65];
-    __shared__ double Rs[64 * 4 * KC];
+__global__ __launch_bounds__(64) void sk_dense_sym(const DenseTile *__restrict__ tiles, const CT *__restrict__ mat,
+                                                   const double *__restrict__ red, double *__restrict__ part,
+                                                   uint32_t kstride, uint32_t c0) {
     const DenseTile T = tiles[blockIdx.x];
-    const uint32_t t = threadIdx.x, r = t >> 2, g = t & 3;
+    const uint32_t row = T.row0 + threadIdx.x;
+    const bool on = row < T.nk;
+    const CT *M = mat + T.mat_off + (on ? row : 0);
+    const double *R = red + (uint64_t)T.kept_base * kstride + c0;
     double acc[KC];
 #pragma unroll
-    for (int i = 0; i < KC; ++i) acc[i] = 0.0;
-    const CT *M = mat + T.mat_off;
-    for (uint32_t lc = T.l0; lc < T.l1; lc += 64) {
-        for (uint32_t q = t; q < 64 * 64; q += 256) {
-            const uint32_t rr = q >> 6, ll = q & 63, row = T.row0 + rr, col = lc + ll;
-            Ms[rr][ll] = (row < T.nk && col < T.l1) ? M[(uint64_t)row * T.nk + col] : (CT)0;
-        }
-        for (uint32_t q = t; q < 64 * k; q += 256) {
-            const uint32_t ll = q / k, c = q - ll * k, l = lc + ll;
-            Rs[ll * k + c] = l < T.l1 ? red[(uint64_t)(T.kept_base + l) * kstride + c0 + c] : 0.0;
-        }
-        __syncthreads();
-#pragma unroll 4
-        for (uint32_t ll = 0; ll < 64; ++ll) {
-            const double m = (double)Ms[r][ll];
+    for (int c = 0; c < KC; ++c) acc[c] = 0.0;
+    uint32_t l = T.l0;
+    for (; l + 4 <= T.l1; l += 4) {
+        double m[4];
 #pragma unroll
-            for (int i = 0; i < KC; ++i) {
-                const uint32_t c = g + 4 * i;
-                if (c < k) acc[i] += m * Rs[ll * k + c];
-            }
+        for (int q = 0; q < 4; ++q) m[q] = (double)M[(uint64_t)(l + q) * T.nk];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const double *rl = R + (uint64_t)(l + q) * kstride;
+#pragma unroll
+            for (int c = 0; c < KC; ++c) acc[c] += m[q] * rl[c];
         }
-        __syncthreads();
     }
-    const uint32_t row = T.row0 + r;
-    if (row < T.nk) {
+    for (; l < T.l1; ++l) {
+        const double m = (double)M[(uint64_t)l * T.nk];
+        const double *rl = R + (uint64_t)l * kstride;
 #pragma unroll
-        for (int i = 0; i < KC; ++i) {
-            const uint32_t c = g + 4 * i;
-            if (c < k) part[(T.part_off + (uint64_t)T.seg * T.nk + row) * kstride + c0 + c] = acc[i];
-        }
+        for (int c = 0; c < KC; ++c) acc[c] += m * rl[c];
+    }
+    if (on) {
+        double *o = part + (T.part_off + (uint64_t)T.seg * T.nk + row) * kstride + c0;
+#pragma unroll
+        for (int c = 0; c < KC; ++c) o[c] = acc[c];
     }
 }
 
@@ -547,7 +551,7 @@ std::shared_ptr<SchwarzDev<CT>> build_dev(const Precond::Schwarz &S) {
         if (nk >= DENSE_TILED_MIN) {
             // enough (row tile, column segment) pairs to fill the device, segments >= 256 columns
             const uint32_t max_seg = (nk + 255) / 256;
-            const uint32_t want = (uint32_t)std::max<uint64_t>(1, (2048 + row_tiles - 1) / row_tiles);
+            const uint32_t want = (uint32_t)std::max<uint64_t>(1, (4096 + row_tiles - 1) / row_tiles);
             const uint32_t n_seg = std::min(max_seg, want);
             const uint32_t seg_len = ((nk + n_seg - 1) / n_seg + 63) / 64 * 64;
             DenseComp dc{nk, kb, 0, 0, part_rows};
@@ -754,17 +758,21 @@ struct SchwarzRunImpl : SchwarzRun {
                                dp<CT>(d.lt_val), dp<CT>(d.piv), redp, dp<double>(mu2), xsp, k, kshift);
         }
         if (d.n_dtiles) {
-            for (uint32_t c0 = 0; c0 < k; c0 += 32) {
-                const uint32_t kc = std::min<uint32_t>(32, k - c0);
-                const uint32_t KC = (kc + 3) / 4;
-#define WX_DT(N)                                                                                                   \
-    hipLaunchKernelGGL((sk_dense_tiled<CT, N>), dim3(d.n_dtiles), dim3(256), 0, st, dp<DenseTile>(d.dtiles),      \
-                       dp<CT>(d.dmat), redp, dp<double>(dpart), k, c0, kc)
-                if (KC <= 1) WX_DT(1);
-                else if (KC <= 2) WX_DT(2);
-                else if (KC <= 4) WX_DT(4);
-                else WX_DT(8);
-#undef WX_DT
+            // column windows of exact power-of-two widths (<= 16): the wave-uniform red loads
+            // never read past column k
+            for (uint32_t c0 = 0; c0 < k;) {
+                uint32_t kc = 16;
+                while (kc > k - c0) kc >>= 1;
+#define WX_DS(N)                                                                                                   \
+    hipLaunchKernelGGL((sk_dense_sym<CT, N>), dim3(d.n_dtiles), dim3(64), 0, st, dp<DenseTile>(d.dtiles),         \
+                       dp<CT>(d.dmat), redp, dp<double>(dpart), k, c0)
+                if (kc == 16) WX_DS(16);
+                else if (kc == 8) WX_DS(8);
+                else if (kc == 4) WX_DS(4);
+                else if (kc == 2) WX_DS(2);
+                else WX_DS(1);
+#undef WX_DS
+                c0 += kc;
             }
             hipLaunchKernelGGL(sk_dense_reduce, dim3(grid_rows(d.dense_part_rows, kshift)), dim3(BLK), 0, st,
                                dp<DenseComp>(d.dcomps), d.n_dcomps, dp<uint64_t>(d.drow_start), dp<double>(dpart), xsp,
