@@ -53,6 +53,13 @@ def algorithmic_bytes(n, F, weighted, V, k, N):
     return b_d, b_dt
 
 
+def sorted_term_levels(codes):
+    """Levels of the term the observations are ordered by: the largest, ties to the last."""
+    levels = [int(codes[:, t].max()) + 1 for t in range(codes.shape[1])]
+    best = max(levels)
+    return levels[max(i for i, L in enumerate(levels) if L == best)]
+
+
 def run_gpu(solver, Y_dev, k, n, N, precision, steps, warmup, tol):
     import torch
     from within_amd import _lib
@@ -89,6 +96,34 @@ def run_gpu(solver, Y_dev, k, n, N, precision, steps, warmup, tol):
     return elapsed, st, phases, Dm
 
 
+def run_plain_fp64(solver, Y_dev, n, N, tol, n_rhs):
+    """Plain FP64 form: each right-hand side solved on its own (k = 1), FP64 coefficients."""
+    import torch
+    from within_amd import _lib
+    L = _lib.lib()
+    opts = _lib.LsmrOpts()
+    L.wx_lsmr_options_default(C.byref(opts))
+    opts.tol = tol
+    opts.precision = 0
+    X = torch.empty((1, N), dtype=torch.float64, device="cuda")
+    Dm = torch.empty((1, n), dtype=torch.float64, device="cuda")
+    stats = (_lib.RhsStats * 1)()
+    stream = torch.cuda.current_stream().cuda_stream
+
+    def one(j):
+        ptr = Y_dev.data_ptr() + j * n * 8
+        _lib.check(L.wx_solver_solve_batch_device(solver._h, ptr, n, 1, C.byref(opts), X.data_ptr(), Dm.data_ptr(),
+                                                  stats, C.c_void_p(stream)))
+        return stats[0].iterations
+
+    one(0)  # warm-up (workspace of width 1)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    its = [one(j) for j in range(n_rhs)]
+    torch.cuda.synchronize()
+    return n_rhs / (time.perf_counter() - t0), its
+
+
 def cpu_baseline(solver, codes, w, Y, seconds_budget=30.0):
     """Oracle (C restatement of the reference algorithm) on the host, bounded sample."""
     from oracle.oracle import OracleLib
@@ -123,6 +158,7 @@ def main():
     ap.add_argument("--tol", type=float, default=1e-8)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
+    ap.add_argument("--no-plain", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only: --gpus {args.gpus} is not supported by this build "
@@ -150,15 +186,22 @@ def main():
             f"max true residual {max(s.true_residual for s in st):.2e}, phases {ph}")
     el, st, ph = res["mixed"]
     value = k * args.steps / el
-    b_d, b_dt = algorithmic_bytes(n, codes.shape[1], True, 0, k, N)
-    g_ms, g_cnt = ph["gather"]
-    a_ms, a_cnt = ph["adjoint"]
-    # dominant of the two design-operator phases by total time
-    if g_ms >= a_ms:
-        kern, bytes_per, dur = "k_gather_axpy (D v + u update)", b_d, g_ms / max(g_cnt, 1) * 1e-3
-    else:
-        kern, bytes_per, dur = "k_segreduce (D' reduction, all terms)", b_dt, a_ms / max(a_cnt, 1) * 1e-3
+    F = codes.shape[1]
+    b_d, b_dt = algorithmic_bytes(n, F, True, 0, k, N)
+    # the timed solve launches one gather per iteration plus one after the last column has
+    # converged, which returns at once; the per-launch duration is over the working launches
+    iters = max(s.iterations for s in st)
+    g_ms, _ = ph["gather"]
+    a_ms, _ = ph["adjoint"]
+    n_sorted = sorted_term_levels(codes)
+    # k_gather_seg: the gather with the u update (B_D) plus the sorted term's D' output
+    kern = "k_gather_seg (D v + u update + D' of the sorted term)"
+    bytes_per = b_d + 8 * k * n_sorted
+    dur = g_ms / iters * 1e-3
     achieved = bytes_per / dur / 1e9
+    # the remaining terms' D' (read u, w and codes once; write their levels)
+    b_rest = n * (8 * k + 8 + 4 * (F - 1)) + 8 * k * (N - n_sorted)
+    adj_dur = a_ms / iters * 1e-3
     out = {
         "metric": "RHS solved/s to tol 1e-8 (batched mixed-prec vs plain FP64 GPU); D/D' HBM GB/s",
         "value": value,
@@ -178,6 +221,10 @@ def main():
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": kern,
                      "algorithmic_bytes_per_launch": bytes_per, "avg_launch_ms": dur * 1e3},
+        "roofline_adjoint_rest": {"achieved": b_rest / adj_dur / 1e9, "unit": "GB/s",
+                                  "frac": b_rest / adj_dur / 1e9 / HBM_PEAK_GBS,
+                                  "algorithmic_bytes_per_iteration": b_rest, "ms_per_iteration": adj_dur * 1e3,
+                                  "what": "D' of the other terms (k_segreduce + k_segfix launches)"},
         "iterations": [s.iterations for s in st],
         "true_residual_max": max(s.true_residual for s in st),
         "setup_s": setup,
@@ -188,6 +235,12 @@ def main():
         out["fp64_value"] = k * args.steps / el64
         out["mixed_over_fp64"] = out["value"] / out["fp64_value"]
         out["fp64_iterations"] = [s.iterations for s in st64]
+    if not args.no_plain:
+        pv, pits = run_plain_fp64(solver, Y_dev, n, N, args.tol, k)
+        out["plain_fp64_value"] = pv
+        out["plain_fp64_iterations"] = pits
+        out["mixed_over_plain_fp64"] = out["value"] / pv
+        log(f"[bench] plain fp64 (one RHS per solve): {pv:.1f} RHS/s")
     if not args.no_cpu:
         try:
             out["cpu_baseline"] = cpu_baseline(solver, codes, w, Y)

@@ -30,9 +30,10 @@ struct ColState {
     double *alpha, *beta, *beta_prev_inv, *beta_inv, *p_coeff, *gscale;
     double *alpha_bar, *phi_bar, *c_bar, *s_bar, *zeta_bar, *zeta0, *a_norm_sq, *abs_tol;
     double *prev_rho, *prev_rho_bar, *t_x, *t_hbar, *t_h, *inv_alpha, *bnorm_scale;
+    double *den_norm; // ||D'W y|| = beta_1 ||p~_1||, the denominator of the true residual
     int32_t *active, *upd, *brk, *stop, *iters, *err;
 };
-constexpr int N_DOUBLE_FIELDS = 21;
+constexpr int N_DOUBLE_FIELDS = 22;
 constexpr int N_INT_FIELDS = 6;
 
 __device__ inline double term_load(const DesignArgs &d, int32_t li, uint64_t i) {
@@ -50,15 +51,61 @@ __device__ inline double block_reduce_cols(double v, uint32_t kp, double *sh) {
     return sh[threadIdx.x];
 }
 
+// (D x)[row] for R rows of one column j, as in crates/within/src/operator/design/gather.rs:10-78:
+// terms summed in order from 0.0, each term folded from -0.0 over its coefficient columns.
+// TT > 0 fixes the number of terms at compile time; PLAIN says every term is one intercept
+// column, so the fold reduces to the value itself (-0.0 + x*1.0 == x bitwise) and all R*TT
+// code loads are issued before any x load.
+template <int R, int TT, bool PLAIN>
+__device__ inline void gather_rows(const DesignArgs &d, const double *__restrict__ x, const uint64_t (&rows)[R],
+                                   const bool (&ok)[R], uint32_t k, uint32_t j, double (&acc)[R]) {
+    if constexpr (TT > 0 && PLAIN) {
+        uint32_t code[TT][R];
+#pragma unroll
+        for (int t = 0; t < TT; ++t)
+#pragma unroll
+            for (int q = 0; q < R; ++q) code[t][q] = ok[q] ? d.codes[(uint64_t)t * d.n + rows[q]] : 0u;
+        double val[TT][R];
+#pragma unroll
+        for (int t = 0; t < TT; ++t)
+#pragma unroll
+            for (int q = 0; q < R; ++q) val[t][q] = x[(d.terms[t].offset + code[t][q]) * k + j];
+#pragma unroll
+        for (int q = 0; q < R; ++q) {
+            double a = 0.0;
+#pragma unroll
+            for (int t = 0; t < TT; ++t) a += val[t][q];
+            acc[q] = a;
+        }
+    } else {
+#pragma unroll
+        for (int q = 0; q < R; ++q) acc[q] = 0.0;
+        for (uint32_t t = 0; t < d.T; ++t) {
+            const TermInfo &ti = d.terms[t];
+            uint64_t code[R];
+#pragma unroll
+            for (int q = 0; q < R; ++q) code[q] = ok[q] ? d.codes[(uint64_t)t * d.n + rows[q]] : 0;
+#pragma unroll
+            for (int q = 0; q < R; ++q) {
+                double tv = -0.0;
+                for (uint32_t c = 0; c < ti.ncols; ++c)
+                    tv = tv + x[(ti.offset + c * ti.n_levels + code[q]) * k + j] *
+                                  (ok[q] ? term_load(d, ti.load[c], rows[q]) : 0.0);
+                acc[q] += tv;
+            }
+        }
+    }
+}
+
 // ---------------------------------------------------------------------------
-// D v gather fused with the u update:  u <- sqrt(w) .* (D v) + gscale_j * u,  ||u||^2 partials.
-// The per-row sum follows crates/within/src/operator/design/gather.rs:10-78 (terms summed in
-// order from 0.0, each term folded from -0.0, weight applied after the terms) and the update
-// axpy_with_sq_norm crates/schwarz-precond/src/lsmr/bidiag.rs:24-43.
+// D v gather fused with the u update:  u <- sqrt(w) .* (D v) + gscale_j * u_old,  ||u||^2
+// partials (axpy_with_sq_norm, crates/schwarz-precond/src/lsmr/bidiag.rs:24-43).  u is kept
+// unnormalised (u_k = beta_k * u_k-hat) as in ModifiedGolubKahan::step (bidiag.rs:275-309).
+// FIRST: u_old is the right-hand side b = sqrt(w) .* y, formed from y_int on the fly.
 // ---------------------------------------------------------------------------
-template <bool WEIGHTED>
+template <bool WEIGHTED, bool FIRST, int TT, bool PLAIN>
 __global__ __launch_bounds__(BLK) void k_gather_axpy(DesignArgs d, const double *__restrict__ sw,
-                                                     const double *__restrict__ v, double *__restrict__ u,
+                                                     const double *__restrict__ v, const double *uin, double *u,
                                                      ColState cs, double *__restrict__ part, uint32_t k,
                                                      uint32_t kshift) {
     __shared__ double sh[BLK];
@@ -74,37 +121,81 @@ __global__ __launch_bounds__(BLK) void k_gather_axpy(DesignArgs d, const double 
         for (uint64_t i0 = ((uint64_t)blockIdx.x * BLK + threadIdx.x) >> kshift; i0 < d.n; i0 += R * rstride) {
             uint64_t rows[R];
             bool ok[R];
-            double acc[R], uo[R];
+            double uo[R], wq[R], acc[R];
 #pragma unroll
             for (int q = 0; q < R; ++q) {
                 rows[q] = i0 + q * rstride;
                 ok[q] = rows[q] < d.n;
-                acc[q] = 0.0;
-                uo[q] = ok[q] ? u[rows[q] * k + j] : 0.0;
+                uo[q] = ok[q] ? uin[rows[q] * k + j] : 0.0;
+                wq[q] = WEIGHTED && ok[q] ? sw[rows[q]] : 1.0;
             }
-            for (uint32_t t = 0; t < d.T; ++t) {
-                const TermInfo &ti = d.terms[t];
-                uint64_t code[R];
-#pragma unroll
-                for (int q = 0; q < R; ++q) code[q] = ok[q] ? d.codes[(uint64_t)t * d.n + rows[q]] : 0;
-#pragma unroll
-                for (int q = 0; q < R; ++q) {
-                    double tv = -0.0;
-                    for (uint32_t c = 0; c < ti.ncols; ++c)
-                        tv = tv + v[(ti.offset + c * ti.n_levels + code[q]) * k + j] *
-                                      (ok[q] ? term_load(d, ti.load[c], rows[q]) : 0.0);
-                    acc[q] += tv;
-                }
-            }
+            gather_rows<R, TT, PLAIN>(d, v, rows, ok, k, j, acc);
 #pragma unroll
             for (int q = 0; q < R; ++q) {
                 if (!ok[q]) continue;
                 double a = acc[q];
-                if (WEIGHTED) a *= sw[rows[q]];
-                const double nu = a + scale * uo[q];
+                if (WEIGHTED) a *= wq[q];
+                const double b = (FIRST && WEIGHTED) ? wq[q] * uo[q] : uo[q];
+                const double nu = a + scale * b;
                 u[rows[q] * k + j] = nu;
                 sq += nu * nu;
             }
+        }
+    }
+    double tot = block_reduce_cols(sq, kp, sh);
+    if (threadIdx.x < kp) part[(uint64_t)blockIdx.x * kp + threadIdx.x] = tot;
+}
+
+// The same update walked along the D' work items of a sorted single-column term (the term
+// the observations are ordered by), so the term's reduction D_t' sqrt(w) u comes out of the
+// same pass: a row group owns one item (<= 256 consecutive rows of one level), forms u for
+// its rows four at a time, and sums sqrt(w_i) u_i in row order -- the order k_segreduce uses,
+// so the result is bitwise the same as the separate reduction.
+template <bool WEIGHTED, bool FIRST, int TT, bool PLAIN>
+__global__ __launch_bounds__(BLK) void k_gather_seg(DesignArgs d, TermInfo ts, const SegItem *__restrict__ items,
+                                                    uint32_t n_items, const double *__restrict__ sw,
+                                                    const double *__restrict__ v, const double *uin, double *u,
+                                                    double *__restrict__ atu, double *__restrict__ partials,
+                                                    ColState cs, double *__restrict__ part, uint32_t k,
+                                                    uint32_t kshift) {
+    __shared__ double sh[BLK];
+    const uint32_t kp = 1u << kshift;
+    const uint32_t j = threadIdx.x & (kp - 1);
+    double sq = 0.0;
+    const bool act = j < k && cs.active[j];
+    if (act) {
+        const double scale = cs.gscale[j];
+        constexpr int R = 4;
+        const uint32_t groups = (gridDim.x * BLK) >> kshift;
+        for (uint32_t it = (blockIdx.x * BLK + threadIdx.x) >> kshift; it < n_items; it += groups) {
+            const SegItem item = items[it];
+            double adj = 0.0;
+            for (uint32_t p0 = item.begin; p0 < item.end; p0 += R) {
+                uint64_t rows[R];
+                bool ok[R];
+                double uo[R], wq[R], acc[R];
+#pragma unroll
+                for (int q = 0; q < R; ++q) {
+                    rows[q] = p0 + q;
+                    ok[q] = p0 + q < item.end;
+                    uo[q] = ok[q] ? uin[rows[q] * k + j] : 0.0;
+                    wq[q] = WEIGHTED && ok[q] ? sw[rows[q]] : 1.0;
+                }
+                gather_rows<R, TT, PLAIN>(d, v, rows, ok, k, j, acc);
+#pragma unroll
+                for (int q = 0; q < R; ++q) {
+                    if (!ok[q]) continue;
+                    double a = acc[q];
+                    if (WEIGHTED) a *= wq[q];
+                    const double b = (FIRST && WEIGHTED) ? wq[q] * uo[q] : uo[q];
+                    const double nu = a + scale * b;
+                    u[rows[q] * k + j] = nu;
+                    sq += nu * nu;
+                    adj += WEIGHTED ? wq[q] * nu : nu;
+                }
+            }
+            if (item.dest >= 0) atu[(ts.offset + item.level) * k + j] = adj;
+            else partials[(uint64_t)(-item.dest - 1) * k + j] = adj;
         }
     }
     double tot = block_reduce_cols(sq, kp, sh);
@@ -133,21 +224,22 @@ __global__ __launch_bounds__(BLK) void k_segreduce(DesignArgs d, TermInfo ti, co
         double acc[NC];
 #pragma unroll
         for (int c = 0; c < NC; ++c) acc[c] = 0.0;
-        // rows are read four at a time (independent loads in flight) but summed strictly in order
+        // rows are read eight at a time (independent loads in flight) but summed strictly in order
+        constexpr uint32_t U = 8;
         uint32_t p = item.begin;
-        for (; p + 4 <= item.end; p += 4) {
-            uint64_t row[4];
-            double b[4];
+        for (; p + U <= item.end; p += U) {
+            uint64_t row[U];
+            double b[U];
 #pragma unroll
-            for (int q = 0; q < 4; ++q) row[q] = perm ? perm[p + q] : p + q;
+            for (int q = 0; q < (int)U; ++q) row[q] = perm ? perm[p + q] : p + q;
 #pragma unroll
-            for (int q = 0; q < 4; ++q) {
+            for (int q = 0; q < (int)U; ++q) {
                 b[q] = src[row[q] * k + j];
                 if (SCALE == 1) b[q] = scale[row[q]] * b[q];
                 if (SCALE == 2) b[q] = (scale[row[q]] * scale[row[q]]) * b[q];
             }
 #pragma unroll
-            for (int q = 0; q < 4; ++q)
+            for (int q = 0; q < (int)U; ++q)
 #pragma unroll
                 for (int c = 0; c < NC; ++c)
                     acc[c] += (ti.load[c] < 0 ? b[q] : term_load(d, ti.load[c], row[q]) * b[q]);
@@ -201,58 +293,62 @@ __global__ void k_segfix(TermInfo ti, const SegFix *__restrict__ fixes, uint32_t
 // ---------------------------------------------------------------------------
 // Right-hand side loading and vec_norm (crates/schwarz-precond/src/lsmr.rs:24-36)
 // ---------------------------------------------------------------------------
-// Caller-order column-major Y -> internal-order interleaved u = sqrt(w) .* y and y_int = y,
-// through a 64-row LDS tile: Y is read along rows (coalesced) and each internal row's k
-// values are written as one contiguous segment.  Non-finite entries are counted.
+// Caller-order column-major Y -> internal-order interleaved y_int through 64-row LDS tiles
+// (k+1 doubles per tile row): Y is read along rows (coalesced) and each internal row's k
+// values are written as one contiguous segment.  Non-finite entries are counted, and every
+// block leaves per-column partials of sum(b^2) and max|b| for b = sqrt(w) .* y, so beta_1
+// needs no further pass over the observations.
 template <bool WEIGHTED>
 __global__ __launch_bounds__(BLK) void k_load_rhs(uint64_t n, const uint32_t *__restrict__ inv_perm,
                                                   const double *__restrict__ sw, const double *__restrict__ Y,
-                                                  uint64_t ldy, double *__restrict__ u, double *__restrict__ y_int,
-                                                  int32_t *__restrict__ bad, uint32_t k) {
-    __shared__ double tile[64][65];
-    const uint64_t r0 = (uint64_t)blockIdx.x * 64;
+                                                  uint64_t ldy, double *__restrict__ y_int, double *__restrict__ part,
+                                                  int32_t *__restrict__ bad, uint32_t k, uint32_t kshift) {
+    extern __shared__ double tile[]; // 64 x (k + 1)
+    __shared__ double sh[BLK];
+    const uint32_t ld = k + 1;
+    const uint32_t kp = 1u << kshift, j = threadIdx.x & (kp - 1), grp = threadIdx.x >> kshift;
+    const uint32_t ngrp = BLK >> kshift;
+    const uint64_t n_tiles = (n + 63) / 64;
+    double mx = 0.0, sq = 0.0;
     int32_t nbad = 0;
-    for (uint32_t q = threadIdx.x; q < 64 * k; q += BLK) {
-        const uint32_t c = q >> 6, rr = q & 63;
-        const uint64_t r = r0 + rr;
-        if (r < n) {
-            const double y = Y[r + c * ldy];
-            nbad += !isfinite(y);
-            tile[rr][c] = y;
+    for (uint64_t tb = blockIdx.x; tb < n_tiles; tb += gridDim.x) {
+        const uint64_t r0 = tb * 64;
+        for (uint32_t q = threadIdx.x; q < 64 * k; q += BLK) {
+            const uint32_t c = q >> 6, rr = q & 63;
+            const uint64_t r = r0 + rr;
+            if (r < n) {
+                const double y = Y[r + c * ldy];
+                nbad += !isfinite(y);
+                tile[rr * ld + c] = y;
+            }
         }
+        __syncthreads();
+        if (j < k)
+            for (uint32_t rr = grp; rr < 64; rr += ngrp) {
+                const uint64_t r = r0 + rr;
+                if (r >= n) break;
+                const uint64_t i = inv_perm ? inv_perm[r] : r;
+                const double y = tile[rr * ld + j];
+                y_int[i * k + j] = y;
+                const double b = WEIGHTED ? sw[i] * y : y;
+                mx = fmax(mx, fabs(b));
+                sq += b * b;
+            }
+        __syncthreads();
     }
     if (nbad) atomicAdd(bad, nbad);
+    const double tsq = block_reduce_cols(sq, kp, sh);
     __syncthreads();
-    for (uint32_t q = threadIdx.x; q < 64 * k; q += BLK) {
-        const uint32_t rr = q / k, c = q - rr * k;
-        const uint64_t r = r0 + rr;
-        if (r < n) {
-            const uint64_t i = inv_perm ? inv_perm[r] : r;
-            const double y = tile[rr][c];
-            y_int[i * k + c] = y;
-            u[i * k + c] = WEIGHTED ? sw[i] * y : y;
-        }
-    }
-}
-
-// Per-block max|u| partials (for vec_norm's scale).
-__global__ __launch_bounds__(BLK) void k_absmax(uint64_t n, const double *__restrict__ u, double *__restrict__ part,
-                                                uint32_t k, uint32_t kshift) {
-    __shared__ double sh[BLK];
-    const uint32_t kp = 1u << kshift;
-    const uint32_t j = threadIdx.x & (kp - 1);
-    const uint64_t rstride = ((uint64_t)gridDim.x * BLK) >> kshift;
-    double mx = 0.0;
-    if (j < k)
-        for (uint64_t i = ((uint64_t)blockIdx.x * BLK + threadIdx.x) >> kshift; i < n; i += rstride)
-            mx = fmax(mx, fabs(u[i * k + j]));
     sh[threadIdx.x] = mx;
     __syncthreads();
     for (uint32_t s = BLK / 2; s >= kp && s > 0; s >>= 1) {
         if (threadIdx.x < s) sh[threadIdx.x] = fmax(sh[threadIdx.x], sh[threadIdx.x + s]);
         __syncthreads();
     }
-    if (threadIdx.x < kp) part[(uint64_t)blockIdx.x * kp + threadIdx.x] = sh[threadIdx.x];
+    if (threadIdx.x < kp) {
+        part[((uint64_t)blockIdx.x * kp + threadIdx.x) * 2 + 0] = tsq;
+        part[((uint64_t)blockIdx.x * kp + threadIdx.x) * 2 + 1] = sh[threadIdx.x];
+    }
 }
 
 // Per-column reduction of block partials (one block per column, fixed tree order).
@@ -275,15 +371,53 @@ __device__ inline double reduce_partials(const double *part, uint32_t nb, uint32
     return r;
 }
 
-__global__ __launch_bounds__(BLK) void k_colmax(const double *part, uint32_t nb, uint32_t kp, ColState cs) {
-    __shared__ double sh[BLK];
-    const uint32_t j = blockIdx.x;
-    double m = reduce_partials(part, nb, kp, 1, 0, j, sh, true);
-    if (threadIdx.x == 0) cs.bnorm_scale[j] = m;
+// beta_1 of a column and the stop test for a zero right-hand side (lsmr.rs:266-282).
+__device__ inline void set_beta1(ColState &cs, uint32_t j, double beta, double tol) {
+    cs.beta[j] = beta;
+    cs.abs_tol[j] = tol * beta;
+    cs.iters[j] = 0;
+    cs.err[j] = 0;
+    cs.brk[j] = 0;
+    cs.upd[j] = 0;
+    cs.den_norm[j] = 0.0;
+    if (beta == 0.0) {
+        cs.active[j] = 0;
+        cs.stop[j] = WX_STOP_ZERO_RHS;
+        cs.beta_inv[j] = 0.0;
+    } else {
+        cs.active[j] = 1;
+        cs.stop[j] = -1;
+        cs.beta_inv[j] = 1.0 / beta;
+    }
 }
 
-// sum((u/scale)^2) partials for vec_norm
-__global__ __launch_bounds__(BLK) void k_scaled_sumsq(uint64_t n, const double *__restrict__ u, ColState cs,
+// vec_norm (crates/schwarz-precond/src/lsmr.rs:24-36) scales by max|b| to avoid overflow and
+// underflow of the squares.  When max|b| lies in [1e-140, 1e140] no square can overflow and
+// the largest is normal, so sqrt(sum b^2) from the load partials is used directly; other
+// columns raise `need_scaled` and get the scaled two-pass form (k_scaled_sumsq).
+constexpr double NORM_SAFE_LO = 1e-140, NORM_SAFE_HI = 1e140;
+
+__global__ __launch_bounds__(BLK) void k_beta1(const double *part, uint32_t nb, uint32_t kp, ColState cs, double tol,
+                                               int32_t *need_scaled) {
+    __shared__ double sh[BLK];
+    const uint32_t j = blockIdx.x;
+    const double s = reduce_partials(part, nb, kp, 2, 0, j, sh, false);
+    const double m = reduce_partials(part, nb, kp, 2, 1, j, sh, true);
+    if (threadIdx.x == 0) {
+        cs.bnorm_scale[j] = m;
+        double beta = sqrt(s);
+        if (m != 0.0 && !(m >= NORM_SAFE_LO && m <= NORM_SAFE_HI)) {
+            beta = m; // provisional; replaced by the scaled pass
+            need_scaled[0] = 1;
+        }
+        set_beta1(cs, j, m == 0.0 ? 0.0 : beta, tol);
+    }
+}
+
+// sum((b/scale)^2) partials, b = sqrt(w) .* y_int, for the columns outside the safe range
+template <bool WEIGHTED>
+__global__ __launch_bounds__(BLK) void k_scaled_sumsq(uint64_t n, const double *__restrict__ sw,
+                                                      const double *__restrict__ y_int, ColState cs,
                                                       double *__restrict__ part, uint32_t k, uint32_t kshift) {
     __shared__ double sh[BLK];
     const uint32_t kp = 1u << kshift;
@@ -293,7 +427,8 @@ __global__ __launch_bounds__(BLK) void k_scaled_sumsq(uint64_t n, const double *
     if (j < k && cs.bnorm_scale[j] > 0.0) {
         const double sc = cs.bnorm_scale[j];
         for (uint64_t i = ((uint64_t)blockIdx.x * BLK + threadIdx.x) >> kshift; i < n; i += rstride) {
-            double q = u[i * k + j] / sc;
+            const double b = WEIGHTED ? sw[i] * y_int[i * k + j] : y_int[i * k + j];
+            const double q = b / sc;
             sq += q * q;
         }
     }
@@ -301,47 +436,22 @@ __global__ __launch_bounds__(BLK) void k_scaled_sumsq(uint64_t n, const double *
     if (threadIdx.x < kp) part[(uint64_t)blockIdx.x * kp + threadIdx.x] = tot;
 }
 
-// beta_1 = vec_norm(b); zero rhs -> stop ZeroRhs (lsmr.rs:266-282).
-__global__ __launch_bounds__(BLK) void k_beta1(const double *part, uint32_t nb, uint32_t kp, ColState cs, double tol,
-                                               uint32_t k) {
+__global__ __launch_bounds__(BLK) void k_beta1_scaled(const double *part, uint32_t nb, uint32_t kp, ColState cs,
+                                                      double tol) {
     __shared__ double sh[BLK];
     const uint32_t j = blockIdx.x;
-    double s = reduce_partials(part, nb, kp, 1, 0, j, sh, false);
+    const double s = reduce_partials(part, nb, kp, 1, 0, j, sh, false);
     if (threadIdx.x == 0) {
-        double sc = cs.bnorm_scale[j];
-        double beta = sc == 0.0 ? 0.0 : sc * sqrt(s);
-        cs.beta[j] = beta;
-        cs.abs_tol[j] = tol * beta;
-        cs.iters[j] = 0;
-        cs.err[j] = 0;
-        cs.brk[j] = 0;
-        cs.upd[j] = 0;
-        if (beta == 0.0) {
-            cs.active[j] = 0;
-            cs.stop[j] = WX_STOP_ZERO_RHS;
-            cs.beta_inv[j] = 0.0;
-        } else {
-            cs.active[j] = 1;
-            cs.stop[j] = -1;
-            cs.beta_inv[j] = 1.0 / beta;
-        }
+        const double m = cs.bnorm_scale[j];
+        if (m != 0.0 && !(m >= NORM_SAFE_LO && m <= NORM_SAFE_HI)) set_beta1(cs, j, m * sqrt(s), tol);
     }
-}
-
-__global__ __launch_bounds__(BLK) void k_scale_obs(uint64_t n, double *__restrict__ u, ColState cs, uint32_t k,
-                                                   uint32_t kshift) {
-    const uint32_t kp = 1u << kshift;
-    const uint32_t j = threadIdx.x & (kp - 1);
-    if (j >= k || !cs.active[j]) return;
-    const double s = cs.beta_inv[j];
-    const uint64_t rstride = ((uint64_t)gridDim.x * BLK) >> kshift;
-    for (uint64_t i = ((uint64_t)blockIdx.x * BLK + threadIdx.x) >> kshift; i < n; i += rstride) u[i * k + j] *= s;
 }
 
 // ---------------------------------------------------------------------------
 // DOF-space kernels
 // ---------------------------------------------------------------------------
-// PREC: 0 identity, 1 diagonal, 2 external (only p~ is written).  INIT: p~ already holds A'u.
+// PREC: 0 identity, 1 diagonal, 2 external (only p~ is written).  INIT: p~ holds A'b and
+// becomes A'b / beta_1 = A'u_1.
 // p~ <- A'u * (1/beta) - (beta/alpha) p~   (bidiag.rs:475-485), v = M^{-1} p~, and partial
 // sums of <v,p~>, <v,v>, <p~,p~> for alpha_from_vp (bidiag.rs:99-111).
 template <int PREC, bool INIT>
@@ -355,15 +465,15 @@ __global__ __launch_bounds__(BLK) void k_ptilde(uint64_t N, const double *__rest
     double vp = 0.0, vv = 0.0, pp = 0.0;
     if (j < k && cs.active[j]) {
         const bool brk = !INIT && cs.brk[j];
-        const double bi = INIT ? 1.0 : cs.beta_inv[j];
+        const double bi = cs.beta_inv[j];
         const double pc = INIT ? 0.0 : cs.p_coeff[j];
         for (uint64_t dof = ((uint64_t)blockIdx.x * BLK + threadIdx.x) >> kshift; dof < N; dof += rstride) {
             const uint64_t q = dof * k + j;
             double p;
             if (brk) p = 0.0;
-            else if (INIT) p = pt[q];
+            else if (INIT) p = bi * pt[q];
             else p = bi * atu[q] + (-pc) * pt[q];
-            if (!INIT) pt[q] = p;
+            pt[q] = p;
             if (PREC != 2) {
                 const double z = brk ? 0.0 : (PREC == 1 ? inv_diag[dof] * p : p);
                 v[q] = z;
@@ -468,6 +578,8 @@ __global__ __launch_bounds__(BLK) void k_alpha1(const double *part, uint32_t nb,
     double vv = reduce_partials(part, nb, kp, 3, 1, j, sh, false);
     double pp = reduce_partials(part, nb, kp, 3, 2, j, sh, false);
     if (threadIdx.x == 0 && cs.active[j]) {
+        double beta = cs.beta[j];
+        cs.den_norm[j] = beta * sqrt(pp);
         bool indef;
         double alpha = alpha_from_parts(vp, vv, pp, indef);
         if (indef) {
@@ -475,7 +587,6 @@ __global__ __launch_bounds__(BLK) void k_alpha1(const double *part, uint32_t nb,
             cs.active[j] = 0;
             return;
         }
-        double beta = cs.beta[j];
         cs.alpha[j] = alpha;
         cs.inv_alpha[j] = alpha > 0.0 ? 1.0 / alpha : 0.0;
         cs.beta_prev_inv[j] = 1.0;
@@ -497,7 +608,7 @@ __global__ __launch_bounds__(BLK) void k_alpha1(const double *part, uint32_t nb,
         cs.a_norm_sq[j] = alpha * alpha;
         cs.prev_rho[j] = 1.0;
         cs.prev_rho_bar[j] = 1.0;
-        cs.gscale[j] = -(alpha * 1.0);
+        cs.gscale[j] = -(alpha * cs.beta_inv[j]); // u_1 is kept as b = beta_1 * u_1-hat
     }
 }
 
@@ -633,6 +744,7 @@ __global__ __launch_bounds__(BLK) void k_update_solution(uint64_t N, double *__r
 
 // dm_int[i, j] = y_int[i, j] - (D x)[i, j] in internal order (solver.rs:469-473; the gather
 // is unweighted and folds each term from -0.0 like the operator's).
+template <int TT, bool PLAIN>
 __global__ __launch_bounds__(BLK) void k_demean(DesignArgs d, const double *__restrict__ x,
                                                 const double *__restrict__ y_int, double *__restrict__ dm_int,
                                                 uint32_t k, uint32_t kshift) {
@@ -640,37 +752,49 @@ __global__ __launch_bounds__(BLK) void k_demean(DesignArgs d, const double *__re
     const uint32_t j = threadIdx.x & (kp - 1);
     if (j >= k) return;
     const uint64_t rstride = ((uint64_t)gridDim.x * BLK) >> kshift;
-    for (uint64_t i = ((uint64_t)blockIdx.x * BLK + threadIdx.x) >> kshift; i < d.n; i += rstride) {
-        double acc = 0.0;
-        for (uint32_t t = 0; t < d.T; ++t) {
-            const TermInfo &ti = d.terms[t];
-            const uint64_t code = d.codes[(uint64_t)t * d.n + i];
-            double tv = -0.0;
-            for (uint32_t c = 0; c < ti.ncols; ++c)
-                tv = tv + x[(ti.offset + c * ti.n_levels + code) * k + j] * term_load(d, ti.load[c], i);
-            acc += tv;
+    constexpr int R = 4;
+    for (uint64_t i0 = ((uint64_t)blockIdx.x * BLK + threadIdx.x) >> kshift; i0 < d.n; i0 += R * rstride) {
+        uint64_t rows[R];
+        bool ok[R];
+        double yv[R], acc[R];
+#pragma unroll
+        for (int q = 0; q < R; ++q) {
+            rows[q] = i0 + q * rstride;
+            ok[q] = rows[q] < d.n;
+            yv[q] = ok[q] ? y_int[rows[q] * k + j] : 0.0;
         }
-        dm_int[i * k + j] = y_int[i * k + j] - acc;
+        gather_rows<R, TT, PLAIN>(d, x, rows, ok, k, j, acc);
+#pragma unroll
+        for (int q = 0; q < R; ++q)
+            if (ok[q]) dm_int[rows[q] * k + j] = yv[q] - acc[q];
     }
 }
 
-// Internal-order interleaved dm_int -> caller-order column-major Dm (solver.rs:483), through a
-// 64-row LDS tile so the column-major writes are coalesced.
+// Internal-order interleaved dm_int -> caller-order column-major Dm (solver.rs:483), through
+// 64-row LDS tiles (k+1 doubles per tile row) so the column-major writes are coalesced.
 __global__ __launch_bounds__(BLK) void k_store_caller(uint64_t n, const uint32_t *__restrict__ inv_perm,
                                                       const double *__restrict__ dm_int, double *__restrict__ Dm,
-                                                      uint64_t ldd, uint32_t k) {
-    __shared__ double tile[64][65];
-    const uint64_t r0 = (uint64_t)blockIdx.x * 64;
-    for (uint32_t q = threadIdx.x; q < 64 * k; q += BLK) {
-        const uint32_t rr = q / k, c = q - rr * k;
-        const uint64_t r = r0 + rr;
-        if (r < n) tile[rr][c] = dm_int[(inv_perm ? inv_perm[r] : r) * k + c];
-    }
-    __syncthreads();
-    for (uint32_t q = threadIdx.x; q < 64 * k; q += BLK) {
-        const uint32_t c = q >> 6, rr = q & 63;
-        const uint64_t r = r0 + rr;
-        if (r < n) Dm[r + c * ldd] = tile[rr][c];
+                                                      uint64_t ldd, uint32_t k, uint32_t kshift) {
+    extern __shared__ double tile[]; // 64 x (k + 1)
+    const uint32_t ld = k + 1;
+    const uint32_t kp = 1u << kshift, j = threadIdx.x & (kp - 1), grp = threadIdx.x >> kshift;
+    const uint32_t ngrp = BLK >> kshift;
+    const uint64_t n_tiles = (n + 63) / 64;
+    for (uint64_t tb = blockIdx.x; tb < n_tiles; tb += gridDim.x) {
+        const uint64_t r0 = tb * 64;
+        if (j < k)
+            for (uint32_t rr = grp; rr < 64; rr += ngrp) {
+                const uint64_t r = r0 + rr;
+                if (r >= n) break;
+                tile[rr * ld + j] = dm_int[(inv_perm ? inv_perm[r] : r) * k + j];
+            }
+        __syncthreads();
+        for (uint32_t q = threadIdx.x; q < 64 * k; q += BLK) {
+            const uint32_t c = q >> 6, rr = q & 63;
+            const uint64_t r = r0 + rr;
+            if (r < n) Dm[r + c * ldd] = tile[rr * ld + c];
+        }
+        __syncthreads();
     }
 }
 
@@ -755,6 +879,8 @@ void upload_design(Solver &s, const std::vector<double> &weights_internal) {
     dv.terms.clear();
     dv.terms.resize(d.terms.size());
     dv.max_partials = 0;
+    dv.plain = true;
+    for (const Term &tm : d.terms) dv.plain = dv.plain && tm.ncols() == 1 && tm.loads[0] < 0;
     for (size_t t = 0; t < d.terms.size(); ++t) {
         const Term &tm = d.terms[t];
         DevTerm &dt = dv.terms[t];
@@ -800,15 +926,37 @@ void upload_design(Solver &s, const std::vector<double> &weights_internal) {
         dt.n_partials = partials;
         dv.max_partials = std::max(dv.max_partials, partials * dt.info.ncols);
     }
+    // the sorted single-intercept term with the most work items gets its D' fused into the gather
+    dv.fuse_term = -1;
+    for (size_t t = 0; t < d.terms.size(); ++t) {
+        const Term &tm = d.terms[t];
+        if (!tm.sorted || tm.ncols() != 1 || tm.loads[0] >= 0) continue;
+        if (dv.fuse_term < 0 || dv.terms[t].n_items > dv.terms[dv.fuse_term].n_items) dv.fuse_term = (int32_t)t;
+    }
 }
 
 namespace {
 
-// Launches the D' reduction of every term: out (N x k) = D' (scale .* src).
+// Combines the split levels of one term's D' reduction.
+void launch_segfix(const DevTerm &dt, const double *partials, double *out, const int32_t *active, uint32_t k,
+                   uint32_t kshift, hipStream_t st) {
+    if (!dt.n_fixes) return;
+    const uint32_t fgrid = dt.n_fixes;
+    switch (dt.info.ncols) {
+    case 1: hipLaunchKernelGGL((k_segfix<1>), dim3(fgrid), dim3(BLK), 0, st, dt.info, dt.fixes.as<SegFix>(), dt.n_fixes, partials, out, active, k, kshift); break;
+    case 2: hipLaunchKernelGGL((k_segfix<2>), dim3(fgrid), dim3(BLK), 0, st, dt.info, dt.fixes.as<SegFix>(), dt.n_fixes, partials, out, active, k, kshift); break;
+    case 3: hipLaunchKernelGGL((k_segfix<3>), dim3(fgrid), dim3(BLK), 0, st, dt.info, dt.fixes.as<SegFix>(), dt.n_fixes, partials, out, active, k, kshift); break;
+    default: hipLaunchKernelGGL((k_segfix<4>), dim3(fgrid), dim3(BLK), 0, st, dt.info, dt.fixes.as<SegFix>(), dt.n_fixes, partials, out, active, k, kshift); break;
+    }
+}
+
+// Launches the D' reduction of every term (but `skip`): out (N x k) = D' (scale .* src).
 void launch_adjoint(const DevDesign &dv, const DesignArgs &da, const double *scale, int scale_mode,
                     const double *src, double *out, double *partials, const int32_t *active, uint32_t k,
-                    uint32_t kshift, hipStream_t st) {
-    for (const DevTerm &dt : dv.terms) {
+                    uint32_t kshift, hipStream_t st, int32_t skip = -1) {
+    for (size_t t = 0; t < dv.terms.size(); ++t) {
+        if ((int32_t)t == skip) continue;
+        const DevTerm &dt = dv.terms[t];
         const uint32_t groups_per_block = BLK >> kshift;
         uint32_t grid = (dt.n_items + groups_per_block - 1) / groups_per_block;
         grid = std::max<uint32_t>(1, std::min<uint32_t>(grid, 65535u * 8));
@@ -828,17 +976,90 @@ void launch_adjoint(const DevDesign &dv, const DesignArgs &da, const double *sca
         else { WX_SEG_NC(2) }
 #undef WX_SEG_NC
 #undef WX_SEG
-        if (dt.n_fixes) {
-            const uint32_t fgrid = dt.n_fixes;
-            switch (dt.info.ncols) {
-            case 1: hipLaunchKernelGGL((k_segfix<1>), dim3(fgrid), dim3(BLK), 0, st, dt.info, dt.fixes.as<SegFix>(), dt.n_fixes, partials, out, active, k, kshift); break;
-            case 2: hipLaunchKernelGGL((k_segfix<2>), dim3(fgrid), dim3(BLK), 0, st, dt.info, dt.fixes.as<SegFix>(), dt.n_fixes, partials, out, active, k, kshift); break;
-            case 3: hipLaunchKernelGGL((k_segfix<3>), dim3(fgrid), dim3(BLK), 0, st, dt.info, dt.fixes.as<SegFix>(), dt.n_fixes, partials, out, active, k, kshift); break;
-            default: hipLaunchKernelGGL((k_segfix<4>), dim3(fgrid), dim3(BLK), 0, st, dt.info, dt.fixes.as<SegFix>(), dt.n_fixes, partials, out, active, k, kshift); break;
-            }
-        }
+        launch_segfix(dt, partials, out, active, k, kshift, st);
     }
     WX_HIP(hipGetLastError());
+}
+
+// D v gather launch, specialised on the term count when every term is one intercept column.
+template <bool W, bool F>
+void launch_gather_t(const DevDesign &dv, const DesignArgs &da, uint32_t grid, const double *sw, const double *v,
+                     const double *uin, double *u, const ColState &cs, double *part, uint32_t k, uint32_t kshift,
+                     hipStream_t st) {
+#define WX_G(TT, PL)                                                                                             \
+    hipLaunchKernelGGL((k_gather_axpy<W, F, TT, PL>), dim3(grid), dim3(BLK), 0, st, da, sw, v, uin, u, cs, part, \
+                       k, kshift)
+    switch (dv.plain ? da.T : 0) {
+    case 1: WX_G(1, true); break;
+    case 2: WX_G(2, true); break;
+    case 3: WX_G(3, true); break;
+    case 4: WX_G(4, true); break;
+    default: WX_G(0, false); break;
+    }
+#undef WX_G
+}
+
+void launch_gather(const DevDesign &dv, const DesignArgs &da, bool first, uint32_t grid, const double *v,
+                   const double *uin, double *u, const ColState &cs, double *part, uint32_t k, uint32_t kshift,
+                   hipStream_t st) {
+    const double *sw = dv.sw.p ? dv.sw.as<double>() : nullptr;
+    if (sw) {
+        if (first) launch_gather_t<true, true>(dv, da, grid, sw, v, uin, u, cs, part, k, kshift, st);
+        else launch_gather_t<true, false>(dv, da, grid, sw, v, uin, u, cs, part, k, kshift, st);
+    } else {
+        if (first) launch_gather_t<false, true>(dv, da, grid, sw, v, uin, u, cs, part, k, kshift, st);
+        else launch_gather_t<false, false>(dv, da, grid, sw, v, uin, u, cs, part, k, kshift, st);
+    }
+}
+
+// Gather fused with the D' of the sorted term dv.fuse_term (k_gather_seg); returns the grid,
+// i.e. the number of norm partials written.
+template <bool W, bool F>
+uint32_t launch_gather_seg_t(const DevDesign &dv, const DesignArgs &da, const double *sw, const double *v,
+                             const double *uin, double *u, double *atu, double *partials, const ColState &cs,
+                             double *part, uint32_t k, uint32_t kshift, hipStream_t st) {
+    const DevTerm &dt = dv.terms[dv.fuse_term];
+    const uint32_t groups_per_block = BLK >> kshift;
+    const uint32_t grid = std::max<uint32_t>(
+        1, std::min<uint32_t>((dt.n_items + groups_per_block - 1) / groups_per_block, MAX_PART_BLOCKS));
+    const SegItem *items = dt.items.as<SegItem>();
+#define WX_GS(TT, PL)                                                                                            \
+    hipLaunchKernelGGL((k_gather_seg<W, F, TT, PL>), dim3(grid), dim3(BLK), 0, st, da, dt.info, items, dt.n_items, \
+                       sw, v, uin, u, atu, partials, cs, part, k, kshift)
+    switch (dv.plain ? da.T : 0) {
+    case 1: WX_GS(1, true); break;
+    case 2: WX_GS(2, true); break;
+    case 3: WX_GS(3, true); break;
+    case 4: WX_GS(4, true); break;
+    default: WX_GS(0, false); break;
+    }
+#undef WX_GS
+    return grid;
+}
+
+uint32_t launch_gather_seg(const DevDesign &dv, const DesignArgs &da, bool first, const double *v, const double *uin,
+                           double *u, double *atu, double *partials, const ColState &cs, double *part, uint32_t k,
+                           uint32_t kshift, hipStream_t st) {
+    const double *sw = dv.sw.p ? dv.sw.as<double>() : nullptr;
+    if (sw)
+        return first ? launch_gather_seg_t<true, true>(dv, da, sw, v, uin, u, atu, partials, cs, part, k, kshift, st)
+                     : launch_gather_seg_t<true, false>(dv, da, sw, v, uin, u, atu, partials, cs, part, k, kshift, st);
+    return first ? launch_gather_seg_t<false, true>(dv, da, sw, v, uin, u, atu, partials, cs, part, k, kshift, st)
+                 : launch_gather_seg_t<false, false>(dv, da, sw, v, uin, u, atu, partials, cs, part, k, kshift, st);
+}
+
+void launch_demean(const DevDesign &dv, const DesignArgs &da, uint32_t grid, const double *x, const double *y_int,
+                   double *dm_int, uint32_t k, uint32_t kshift, hipStream_t st) {
+#define WX_DM(TT, PL)                                                                                            \
+    hipLaunchKernelGGL((k_demean<TT, PL>), dim3(grid), dim3(BLK), 0, st, da, x, y_int, dm_int, k, kshift)
+    switch (dv.plain ? da.T : 0) {
+    case 1: WX_DM(1, true); break;
+    case 2: WX_DM(2, true); break;
+    case 3: WX_DM(3, true); break;
+    case 4: WX_DM(4, true); break;
+    default: WX_DM(0, false); break;
+    }
+#undef WX_DM
 }
 
 // HIP-event timer of the solve phases (enabled by wx_lsmr_options::timing).
@@ -904,7 +1125,7 @@ void setup_colstate(Workspace &ws) {
                                         &ws.cs.c_bar, &ws.cs.s_bar, &ws.cs.zeta_bar, &ws.cs.zeta0,
                                         &ws.cs.a_norm_sq, &ws.cs.abs_tol, &ws.cs.prev_rho, &ws.cs.prev_rho_bar,
                                         &ws.cs.t_x, &ws.cs.t_hbar, &ws.cs.t_h, &ws.cs.inv_alpha,
-                                        &ws.cs.bnorm_scale};
+                                        &ws.cs.bnorm_scale, &ws.cs.den_norm};
     for (int f = 0; f < N_DOUBLE_FIELDS; ++f) *fields[f] = dp + f * MAX_BATCH;
     int32_t *ip = ws.ints.as<int32_t>();
     int32_t **ifields[N_INT_FIELDS] = {&ws.cs.active, &ws.cs.upd, &ws.cs.brk, &ws.cs.stop, &ws.cs.iters, &ws.cs.err};
@@ -921,7 +1142,7 @@ std::shared_ptr<Workspace> new_workspace(const DevDesign &dv, uint32_t k) {
     for (DevBuf *b : {&w->v, &w->pt, &w->atu, &w->h, &w->hb, &w->x}) b->alloc(N * k * 8);
     w->part.alloc((size_t)MAX_PART_BLOCKS * kp * 3 * 8);
     w->segpart.alloc((size_t)std::max<uint32_t>(dv.max_partials, 1) * k * 8);
-    w->bad.alloc(4);
+    w->bad.alloc(8);
     w->red.alloc(2 * MAX_BATCH * 8);
     WX_HIP(hipHostMalloc((void **)&w->hflags, 2 * MAX_BATCH * sizeof(int32_t), hipHostMallocDefault));
     for (auto &e : w->flag_ev) WX_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
@@ -993,30 +1214,39 @@ void solve_batch_device(const Solver &s, const double *Y, uint64_t ldy, uint64_t
         double *part = ws.part.as<double>();
         const double *Yc = Y + c0 * ldy;
 
-        // --- right-hand side: b = sqrt(W) y (internal order), beta_1 = vec_norm(b) ---
-        WX_HIP(hipMemsetAsync(ws.bad.p, 0, 4, st));
-        const uint32_t gt = (uint32_t)((n + 63) / 64);
+        // --- right-hand side: y_int (internal order), beta_1 = vec_norm(sqrt(W) y) ---
+        WX_HIP(hipMemsetAsync(ws.bad.p, 0, 8, st));
+        const uint32_t gt = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>((n + 63) / 64, MAX_PART_BLOCKS));
+        const size_t tile_bytes = (size_t)64 * (k + 1) * sizeof(double);
+        int32_t *badp = ws.bad.as<int32_t>();
         if (weighted)
-            hipLaunchKernelGGL((k_load_rhs<true>), dim3(gt), dim3(BLK), 0, st, n, inv_perm, dv.sw.as<double>(), Yc, ldy,
-                               ws.u.as<double>(), ws.yint.as<double>(), ws.bad.as<int32_t>(), k);
+            hipLaunchKernelGGL((k_load_rhs<true>), dim3(gt), dim3(BLK), tile_bytes, st, n, inv_perm,
+                               dv.sw.as<double>(), Yc, ldy, ws.yint.as<double>(), part, badp, k, kshift);
         else
-            hipLaunchKernelGGL((k_load_rhs<false>), dim3(gt), dim3(BLK), 0, st, n, inv_perm, nullptr, Yc, ldy,
-                               ws.u.as<double>(), ws.yint.as<double>(), ws.bad.as<int32_t>(), k);
-        hipLaunchKernelGGL(k_absmax, dim3(gn), dim3(BLK), 0, st, n, ws.u.as<double>(), part, k, kshift);
-        hipLaunchKernelGGL(k_colmax, dim3(k), dim3(BLK), 0, st, part, gn, kp, cs);
-        hipLaunchKernelGGL(k_scaled_sumsq, dim3(gn), dim3(BLK), 0, st, n, ws.u.as<double>(), cs, part, k, kshift);
-        hipLaunchKernelGGL(k_beta1, dim3(k), dim3(BLK), 0, st, part, gn, kp, cs, opts.tol, k);
-        int32_t nbad = 0;
-        WX_HIP(hipMemcpyAsync(&nbad, ws.bad.p, 4, hipMemcpyDeviceToHost, st));
+            hipLaunchKernelGGL((k_load_rhs<false>), dim3(gt), dim3(BLK), tile_bytes, st, n, inv_perm, nullptr, Yc,
+                               ldy, ws.yint.as<double>(), part, badp, k, kshift);
+        hipLaunchKernelGGL(k_beta1, dim3(k), dim3(BLK), 0, st, part, gt, kp, cs, opts.tol, badp + 1);
+        int32_t hbad[2] = {0, 0};
+        WX_HIP(hipMemcpyAsync(hbad, ws.bad.p, 8, hipMemcpyDeviceToHost, st));
         WX_HIP(hipStreamSynchronize(st));
-        if (nbad) fail(WX_ERR_INVALID_INPUT, "Solver::solve: response contains non-finite values");
-        hipLaunchKernelGGL(k_scale_obs, dim3(gn), dim3(BLK), 0, st, n, ws.u.as<double>(), cs, k, kshift);
+        if (hbad[0]) fail(WX_ERR_INVALID_INPUT, "Solver::solve: response contains non-finite values");
+        if (hbad[1]) { // a column outside the safe range of the unscaled sum of squares
+            if (weighted)
+                hipLaunchKernelGGL((k_scaled_sumsq<true>), dim3(gn), dim3(BLK), 0, st, n, dv.sw.as<double>(),
+                                   ws.yint.as<double>(), cs, part, k, kshift);
+            else
+                hipLaunchKernelGGL((k_scaled_sumsq<false>), dim3(gn), dim3(BLK), 0, st, n, nullptr,
+                                   ws.yint.as<double>(), cs, part, k, kshift);
+            hipLaunchKernelGGL(k_beta1_scaled, dim3(k), dim3(BLK), 0, st, part, gn, kp, cs, opts.tol);
+        }
 
-        // --- init: p~ = A'u, v = M^{-1} p~, alpha_1 ---
-        launch_adjoint(dv, da, weighted ? dv.sw.as<double>() : nullptr, weighted ? 1 : 0, ws.u.as<double>(),
+        // --- init: p~ = A'b / beta_1, v = M^{-1} p~, alpha_1 ---
+        launch_adjoint(dv, da, weighted ? dv.w.as<double>() : nullptr, weighted ? 1 : 0, ws.yint.as<double>(),
                        ws.pt.as<double>(), ws.segpart.as<double>(), cs.active, k, kshift, st);
         if (prec_kind == 2) {
-            schwarz_run->apply(ws.pt.as<double>(), ws.v.as<double>(), k, st);
+            hipLaunchKernelGGL((k_ptilde<2, true>), dim3(gN), dim3(BLK), 0, st, N, nullptr, ws.pt.as<double>(),
+                               ws.v.as<double>(), nullptr, cs, part, k, kshift);
+            schwarz_run->apply(ws.pt.as<double>(), ws.v.as<double>(), k, st, cs.active);
             hipLaunchKernelGGL(k_dots, dim3(gN), dim3(BLK), 0, st, N, ws.pt.as<double>(), ws.v.as<double>(), cs, part, k, kshift);
         } else if (prec_kind == 1) {
             hipLaunchKernelGGL((k_ptilde<1, true>), dim3(gN), dim3(BLK), 0, st, N, nullptr, ws.pt.as<double>(),
@@ -1037,23 +1267,28 @@ void solve_batch_device(const Solver &s, const double *Y, uint64_t ldy, uint64_t
         for (int itn = 1; itn <= maxiter; ++itn) {
             timer.begin(3);
             timer.begin(0);
-            if (weighted)
-                hipLaunchKernelGGL((k_gather_axpy<true>), dim3(gn), dim3(BLK), 0, st, da, dv.sw.as<double>(),
-                                   ws.v.as<double>(), ws.u.as<double>(), cs, part, k, kshift);
+            const double *uin = itn == 1 ? ws.yint.as<double>() : ws.u.as<double>();
+            uint32_t gpart = gn;
+            if (dv.fuse_term >= 0)
+                gpart = launch_gather_seg(dv, da, itn == 1, ws.v.as<double>(), uin, ws.u.as<double>(),
+                                          ws.atu.as<double>(), ws.segpart.as<double>(), cs, part, k, kshift, st);
             else
-                hipLaunchKernelGGL((k_gather_axpy<false>), dim3(gn), dim3(BLK), 0, st, da, nullptr, ws.v.as<double>(),
-                                   ws.u.as<double>(), cs, part, k, kshift);
+                launch_gather(dv, da, itn == 1, gn, ws.v.as<double>(), uin, ws.u.as<double>(), cs, part, k, kshift, st);
             timer.end(0);
-            hipLaunchKernelGGL(k_beta, dim3(k), dim3(BLK), 0, st, part, gn, kp, cs);
+            hipLaunchKernelGGL(k_beta, dim3(k), dim3(BLK), 0, st, part, gpart, kp, cs);
             timer.begin(1);
+            // split levels of the fused term first: the other terms reuse the partial buffer
+            if (dv.fuse_term >= 0)
+                launch_segfix(dv.terms[dv.fuse_term], ws.segpart.as<double>(), ws.atu.as<double>(), cs.active, k,
+                              kshift, st);
             launch_adjoint(dv, da, weighted ? dv.sw.as<double>() : nullptr, weighted ? 1 : 0, ws.u.as<double>(),
-                           ws.atu.as<double>(), ws.segpart.as<double>(), cs.active, k, kshift, st);
+                           ws.atu.as<double>(), ws.segpart.as<double>(), cs.active, k, kshift, st, dv.fuse_term);
             timer.end(1);
             if (prec_kind == 2) {
                 hipLaunchKernelGGL((k_ptilde<2, false>), dim3(gN), dim3(BLK), 0, st, N, ws.atu.as<double>(),
                                    ws.pt.as<double>(), ws.v.as<double>(), nullptr, cs, part, k, kshift);
                 timer.begin(2);
-                schwarz_run->apply(ws.pt.as<double>(), ws.v.as<double>(), k, st);
+                schwarz_run->apply(ws.pt.as<double>(), ws.v.as<double>(), k, st, cs.active);
                 timer.end(2);
                 hipLaunchKernelGGL(k_dots, dim3(gN), dim3(BLK), 0, st, N, ws.pt.as<double>(), ws.v.as<double>(), cs,
                                    part, k, kshift);
@@ -1095,33 +1330,29 @@ void solve_batch_device(const Solver &s, const double *Y, uint64_t ldy, uint64_t
 
         // --- outputs: x, demeaned (caller order), FP64 true residual ---
         double *dm_int = ws.u.as<double>();
-        double *y_int = ws.yint.as<double>();
         const uint32_t gd = grid_for(n, kshift, 65535);
-        hipLaunchKernelGGL(k_demean, dim3(gd), dim3(BLK), 0, st, da, ws.x.as<double>(), y_int, dm_int, k, kshift);
+        launch_demean(dv, da, gd, ws.x.as<double>(), ws.yint.as<double>(), dm_int, k, kshift, st);
         if (Dm)
-            hipLaunchKernelGGL(k_store_caller, dim3(gt), dim3(BLK), 0, st, n, inv_perm, dm_int, Dm + c0 * n, n, k);
+            hipLaunchKernelGGL(k_store_caller, dim3(gt), dim3(BLK), tile_bytes, st, n, inv_perm, dm_int, Dm + c0 * n,
+                               n, k, kshift);
         if (X && !s.reparam) {
             uint64_t tot = N * k;
             hipLaunchKernelGGL(k_store_x, dim3((tot + BLK - 1) / BLK), dim3(BLK), 0, st, N, ws.x.as<double>(),
                                X + c0 * N, N, k);
         }
-        // residual numerator D'W(z - Dx) and denominator D'W z
+        // residual numerator ||D'W(y - Dx)||; the denominator ||D'W y|| = beta_1 ||p~_1|| was
+        // kept at initialisation
         const double *wv = weighted ? dv.w.as<double>() : nullptr;
         launch_adjoint(dv, da, wv, weighted ? 1 : 0, dm_int, ws.h.as<double>(), ws.segpart.as<double>(), nullptr, k,
                        kshift, st);
-        launch_adjoint(dv, da, wv, weighted ? 1 : 0, y_int, ws.hb.as<double>(), ws.segpart.as<double>(), nullptr, k,
-                       kshift, st);
-        double num[MAX_BATCH], den[MAX_BATCH];
+        double num[MAX_BATCH];
         DevBuf &red = ws.red;
         hipLaunchKernelGGL(k_sumsq, dim3(gN), dim3(BLK), 0, st, N, ws.h.as<double>(), part, k, kshift);
         hipLaunchKernelGGL(k_reduce_to, dim3(k), dim3(BLK), 0, st, part, gN, kp, red.as<double>());
-        hipLaunchKernelGGL(k_sumsq, dim3(gN), dim3(BLK), 0, st, N, ws.hb.as<double>(), part, k, kshift);
-        hipLaunchKernelGGL(k_reduce_to, dim3(k), dim3(BLK), 0, st, part, gN, kp, red.as<double>() + MAX_BATCH);
         WX_HIP(hipGetLastError());
         double host_d[N_DOUBLE_FIELDS * MAX_BATCH];
         int32_t host_i[N_INT_FIELDS * MAX_BATCH];
         WX_HIP(hipMemcpyAsync(num, red.as<double>(), MAX_BATCH * 8, hipMemcpyDeviceToHost, st));
-        WX_HIP(hipMemcpyAsync(den, red.as<double>() + MAX_BATCH, MAX_BATCH * 8, hipMemcpyDeviceToHost, st));
         WX_HIP(hipMemcpyAsync(host_d, ws.cols.p, sizeof(host_d), hipMemcpyDeviceToHost, st));
         WX_HIP(hipMemcpyAsync(host_i, ws.ints.p, sizeof(host_i), hipMemcpyDeviceToHost, st));
         WX_HIP(hipStreamSynchronize(st));
@@ -1137,7 +1368,7 @@ void solve_batch_device(const Solver &s, const double *Y, uint64_t ldy, uint64_t
         auto t1 = std::chrono::steady_clock::now();
         const double secs = std::chrono::duration<double>(t1 - t0).count();
         const double *phi_bar = host_d + 7 * MAX_BATCH, *zeta_bar = host_d + 10 * MAX_BATCH,
-                     *zeta0 = host_d + 11 * MAX_BATCH;
+                     *zeta0 = host_d + 11 * MAX_BATCH, *den_norm = host_d + 21 * MAX_BATCH;
         const int32_t *active = host_i, *stop = host_i + 3 * MAX_BATCH, *iters = host_i + 4 * MAX_BATCH,
                       *err = host_i + 5 * MAX_BATCH;
         for (uint32_t j = 0; j < k; ++j) {
@@ -1164,7 +1395,7 @@ void solve_batch_device(const Solver &s, const double *Y, uint64_t ldy, uint64_t
                 o.residual = std::fabs(zeta_bar[j]) / zeta0[j];
                 o.residual_norm = std::fabs(phi_bar[j]);
             }
-            o.true_residual = den[j] > 0.0 ? std::sqrt(num[j]) / std::sqrt(den[j]) : 0.0;
+            o.true_residual = den_norm[j] > 0.0 ? std::sqrt(num[j]) / den_norm[j] : 0.0;
             o.time_solve = secs / k;
         }
     }

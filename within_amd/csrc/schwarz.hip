@@ -30,7 +30,7 @@ template <typename T> inline T *dp(const DevBuf &b) { return static_cast<T *>(b.
 
 constexpr int BLK = 256;
 constexpr uint32_t ROW_CHUNK = 256;   // CSR entries per work item before a row is split
-constexpr uint32_t MEAN_CHUNK = 256;  // slots per partial-sum chunk
+constexpr uint32_t MEAN_CHUNK = 64;   // slots per partial-sum chunk
 constexpr uint32_t DENSE_TILED_MIN = 256; // kept blocks at least this large use the tiled product
 
 struct RowItem {
@@ -93,13 +93,23 @@ struct SpMeta {
     uint64_t piv_off;   // first pivot
 };
 
+// A solve passes its per-column "active" flags: when no column is active (the iteration
+// launched after the last column converged) every kernel returns at once.
+__device__ inline bool gate_closed(const int32_t *gate, uint32_t k) {
+    if (!gate) return false;
+    int32_t a = 0;
+    for (uint32_t j = 0; j < k; ++j) a |= gate[j];
+    return a == 0;
+}
+
 // ---------------------------------------------------------------------------
 // kernels (kp lanes per row; lane j = RHS column)
 // ---------------------------------------------------------------------------
 template <typename CT>
 __global__ __launch_bounds__(BLK) void sk_restrict(uint64_t ns, const uint32_t *__restrict__ glob,
                                                    const CT *__restrict__ sgnpw, const double *__restrict__ r,
-                                                   double *__restrict__ loc, uint32_t k, uint32_t kshift) {
+                                                   double *__restrict__ loc, uint32_t k, uint32_t kshift, const int32_t *__restrict__ gate) {
+    if (gate_closed(gate, k)) return;
     const uint32_t kp = 1u << kshift, j = threadIdx.x & (kp - 1);
     if (j >= k) return;
     const uint64_t stride = ((uint64_t)gridDim.x * BLK) >> kshift;
@@ -110,14 +120,25 @@ __global__ __launch_bounds__(BLK) void sk_restrict(uint64_t ns, const uint32_t *
 // partial sums of src over chunks of slots
 __global__ __launch_bounds__(BLK) void sk_chunk_sum(const Chunk *__restrict__ chunks, uint32_t n_chunks,
                                                     const double *__restrict__ src, double *__restrict__ part,
-                                                    uint32_t k, uint32_t kshift) {
+                                                    uint32_t k, uint32_t kshift, const int32_t *__restrict__ gate) {
+    if (gate_closed(gate, k)) return;
     const uint32_t kp = 1u << kshift, j = threadIdx.x & (kp - 1);
     if (j >= k) return;
     const uint32_t g = (blockIdx.x * BLK + threadIdx.x) >> kshift;
     if (g >= n_chunks) return;
     const Chunk ch = chunks[g];
     double acc = 0.0;
-    for (uint32_t s = ch.begin; s < ch.end; ++s) acc += src[(uint64_t)s * k + j];
+    uint32_t s = ch.begin;
+    for (; s + 4 <= ch.end; s += 4) { // four loads in flight, summed in slot order
+        double t[4];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) t[q] = src[(uint64_t)(s + q) * k + j];
+        acc += t[0];
+        acc += t[1];
+        acc += t[2];
+        acc += t[3];
+    }
+    for (; s < ch.end; ++s) acc += src[(uint64_t)s * k + j];
     part[(uint64_t)g * k + j] = acc;
 }
 
@@ -125,13 +146,27 @@ __global__ __launch_bounds__(BLK) void sk_chunk_sum(const Chunk *__restrict__ ch
 // row groups stride over the chunks, then a fixed-order LDS tree across the groups
 __global__ __launch_bounds__(BLK) void sk_comp_mean(const uint32_t *__restrict__ cptr, const double *__restrict__ part,
                                                     const uint32_t *__restrict__ count, double *__restrict__ mu,
-                                                    uint32_t k, uint32_t kshift) {
+                                                    uint32_t k, uint32_t kshift, const int32_t *__restrict__ gate) {
+    if (gate_closed(gate, k)) return;
     __shared__ double sh[BLK];
     const uint32_t kp = 1u << kshift, j = threadIdx.x & (kp - 1), grp = threadIdx.x >> kshift;
     const uint32_t c = blockIdx.x;
-    double acc = 0.0;
-    if (j < k)
-        for (uint32_t q = cptr[c] + grp; q < cptr[c + 1]; q += BLK >> kshift) acc += part[(uint64_t)q * k + j];
+    // eight accumulators (partial q goes to q / ngrp mod 8) keep eight loads in flight; they
+    // are combined in a fixed order
+    const uint32_t ngrp = BLK >> kshift;
+    double a8[8] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+    if (j < k) {
+        const uint32_t e = cptr[c + 1];
+        uint32_t q = cptr[c] + grp;
+        for (; q + 7 * ngrp < e; q += 8 * ngrp) {
+#pragma unroll
+            for (int m = 0; m < 8; ++m) a8[m] += part[(uint64_t)(q + m * ngrp) * k + j];
+        }
+#pragma unroll
+        for (int m = 0; m < 7; ++m)
+            if (q + m * ngrp < e) a8[m] += part[(uint64_t)(q + m * ngrp) * k + j];
+    }
+    const double acc = ((a8[0] + a8[1]) + (a8[2] + a8[3])) + ((a8[4] + a8[5]) + (a8[6] + a8[7]));
     sh[threadIdx.x] = acc;
     __syncthreads();
     for (uint32_t s = BLK / 2; s >= kp && s > 0; s >>= 1) {
@@ -171,7 +206,8 @@ __global__ __launch_bounds__(BLK) void sk_rows(const RowItem *__restrict__ items
                                                const uint32_t *__restrict__ slot_comp, const double *__restrict__ loc,
                                                const double *__restrict__ xs, const CT *__restrict__ inv_de,
                                                const double *__restrict__ mu, double *__restrict__ out,
-                                               double *__restrict__ part, uint32_t k, uint32_t kshift) {
+                                               double *__restrict__ part, uint32_t k, uint32_t kshift, const int32_t *__restrict__ gate) {
+    if (gate_closed(gate, k)) return;
     const uint32_t kp = 1u << kshift, j = threadIdx.x & (kp - 1);
     if (j >= k) return;
     const uint32_t groups = (gridDim.x * BLK) >> kshift;
@@ -200,7 +236,8 @@ __global__ __launch_bounds__(BLK) void sk_rows_fix(const RowFix *__restrict__ fi
                                                    const uint32_t *__restrict__ slot_comp,
                                                    const double *__restrict__ loc, const CT *__restrict__ inv_de,
                                                    const double *__restrict__ mu, const double *__restrict__ part,
-                                                   double *__restrict__ out, uint32_t k, uint32_t kshift) {
+                                                   double *__restrict__ out, uint32_t k, uint32_t kshift, const int32_t *__restrict__ gate) {
+    if (gate_closed(gate, k)) return;
     const uint32_t kp = 1u << kshift, j = threadIdx.x & (kp - 1);
     if (j >= k) return;
     const uint32_t g = (blockIdx.x * BLK + threadIdx.x) >> kshift;
@@ -232,7 +269,8 @@ struct DenseComp {
 template <typename CT, int KC>
 __global__ __launch_bounds__(64) void sk_dense_sym(const DenseTile *__restrict__ tiles, const CT *__restrict__ mat,
                                                    const double *__restrict__ red, double *__restrict__ part,
-                                                   uint32_t kstride, uint32_t c0) {
+                                                   uint32_t kstride, uint32_t c0, const int32_t *__restrict__ gate) {
+    if (gate_closed(gate, kstride)) return;
     const DenseTile T = tiles[blockIdx.x];
     const uint32_t row = T.row0 + threadIdx.x;
     const bool on = row < T.nk;
@@ -270,7 +308,8 @@ __global__ __launch_bounds__(64) void sk_dense_sym(const DenseTile *__restrict__
 __global__ __launch_bounds__(BLK) void sk_dense_reduce(const DenseComp *__restrict__ comps, uint32_t n_comps,
                                                        const uint64_t *__restrict__ row_start,
                                                        const double *__restrict__ part, double *__restrict__ xs,
-                                                       uint32_t k, uint32_t kshift) {
+                                                       uint32_t k, uint32_t kshift, const int32_t *__restrict__ gate) {
+    if (gate_closed(gate, k)) return;
     const uint32_t kp = 1u << kshift, j = threadIdx.x & (kp - 1);
     if (j >= k) return;
     const uint64_t total = row_start[n_comps];
@@ -297,7 +336,8 @@ __global__ __launch_bounds__(BLK) void sk_dense(uint32_t n_rows, const uint64_t 
                                                 const uint32_t *__restrict__ row_nk,
                                                 const uint32_t *__restrict__ row_slot, const CT *__restrict__ mat,
                                                 const double *__restrict__ red, double *__restrict__ xs, uint32_t k,
-                                                uint32_t kshift) {
+                                                uint32_t kshift, const int32_t *__restrict__ gate) {
+    if (gate_closed(gate, k)) return;
     const uint32_t kp = 1u << kshift, j = threadIdx.x & (kp - 1);
     if (j >= k) return;
     const uint32_t groups = (gridDim.x * BLK) >> kshift;
@@ -325,7 +365,8 @@ __global__ __launch_bounds__(BLK) void sk_sparse(const SpMeta *__restrict__ meta
                                                  const uint32_t *__restrict__ lt_idx, const CT *__restrict__ lt_val,
                                                  const CT *__restrict__ piv, const double *__restrict__ red,
                                                  const double *__restrict__ mu2, double *xs, uint32_t k,
-                                                 uint32_t kshift) {
+                                                 uint32_t kshift, const int32_t *__restrict__ gate) {
+    if (gate_closed(gate, k)) return;
     const uint32_t kp = 1u << kshift, j = threadIdx.x & (kp - 1);
     const uint32_t lane_row = threadIdx.x >> kshift, rows_per_pass = BLK >> kshift;
     const SpMeta m = meta[blockIdx.x];
@@ -374,7 +415,8 @@ __global__ __launch_bounds__(BLK) void sk_prolong(uint64_t N, const uint64_t *__
                                                   const uint32_t *__restrict__ d_slot, const CT *__restrict__ sgnpw,
                                                   const uint32_t *__restrict__ slot_comp,
                                                   const double *__restrict__ xs, const double *__restrict__ mu3,
-                                                  double *__restrict__ z, uint32_t k, uint32_t kshift) {
+                                                  double *__restrict__ z, uint32_t k, uint32_t kshift, const int32_t *__restrict__ gate) {
+    if (gate_closed(gate, k)) return;
     const uint32_t kp = 1u << kshift, j = threadIdx.x & (kp - 1);
     if (j >= k) return;
     const uint64_t stride = ((uint64_t)gridDim.x * BLK) >> kshift;
@@ -724,38 +766,38 @@ struct SchwarzRunImpl : SchwarzRun {
     }
 
     void mean(const DevBuf &chunks, uint32_t n_chunks, const DevBuf &cptr, const DevBuf &count, const double *src,
-              double *mu, uint32_t k, uint32_t kshift, hipStream_t st) {
+              double *mu, uint32_t k, uint32_t kshift, hipStream_t st, const int32_t *gate) {
         hipLaunchKernelGGL(sk_chunk_sum, dim3(grid_rows(n_chunks, kshift)), dim3(BLK), 0, st, dp<Chunk>(chunks),
-                           n_chunks, src, dp<double>(part), k, kshift);
+                           n_chunks, src, dp<double>(part), k, kshift, gate);
         hipLaunchKernelGGL(sk_comp_mean, dim3((uint32_t)D->nc), dim3(BLK), 0, st, dp<uint32_t>(cptr), dp<double>(part),
-                           dp<uint32_t>(count), mu, k, kshift);
+                           dp<uint32_t>(count), mu, k, kshift, gate);
     }
 
-    void apply(const double *r, double *z, uint32_t k, hipStream_t st) override {
+    void apply(const double *r, double *z, uint32_t k, hipStream_t st, const int32_t *gate) override {
         ensure(k);
         const uint32_t kshift = pow2_shift(k);
         const SchwarzDev<CT> &d = *D;
         double *locp = dp<double>(loc), *redp = dp<double>(red), *xsp = dp<double>(xs);
         hipLaunchKernelGGL((sk_restrict<CT>), dim3(grid_rows(d.ns, kshift)), dim3(BLK), 0, st, d.ns,
-                           dp<uint32_t>(d.glob), dp<CT>(d.sgnpw), r, locp, k, kshift);
-        mean(d.all_chunks, d.n_all_chunks, d.all_cptr, d.comp_count, locp, dp<double>(mu1), k, kshift, st);
+                           dp<uint32_t>(d.glob), dp<CT>(d.sgnpw), r, locp, k, kshift, gate);
+        mean(d.all_chunks, d.n_all_chunks, d.all_cptr, d.comp_count, locp, dp<double>(mu1), k, kshift, st, gate);
         // reduced right-hand side on kept rows
         hipLaunchKernelGGL((sk_rows<CT, 0>), dim3(grid_rows(d.n_ct_items, kshift)), dim3(BLK), 0, st,
                            dp<RowItem>(d.ct_items), d.n_ct_items, dp<uint32_t>(d.ct_idx), dp<CT>(d.ct_val),
                            dp<uint32_t>(d.slot_comp), locp, xsp, dp<CT>(d.inv_de), dp<double>(mu1), redp,
-                           dp<double>(rowpart), k, kshift);
+                           dp<double>(rowpart), k, kshift, gate);
         if (d.n_ct_fix)
             hipLaunchKernelGGL((sk_rows_fix<CT, 0>), dim3(grid_rows(d.n_ct_fix, kshift)), dim3(BLK), 0, st,
                                dp<RowFix>(d.ct_fix), d.n_ct_fix, dp<uint32_t>(d.slot_comp), locp, dp<CT>(d.inv_de),
-                               dp<double>(mu1), dp<double>(rowpart), redp, k, kshift);
+                               dp<double>(mu1), dp<double>(rowpart), redp, k, kshift, gate);
         // kept-block solves
         if (d.n_sparse) {
-            mean(d.kept_chunks, d.n_kept_chunks, d.kept_cptr, d.comp_kcount, redp, dp<double>(mu2), k, kshift, st);
+            mean(d.kept_chunks, d.n_kept_chunks, d.kept_cptr, d.comp_kcount, redp, dp<double>(mu2), k, kshift, st, gate);
             hipLaunchKernelGGL((sk_sparse<CT>), dim3(d.n_sparse), dim3(BLK), 0, st, dp<SpMeta>(d.sp_meta),
                                dp<uint64_t>(d.sp_lvl_f_ptr), dp<uint32_t>(d.sp_lvl_f_rows),
                                dp<uint64_t>(d.sp_lvl_b_ptr), dp<uint32_t>(d.sp_lvl_b_rows), dp<uint64_t>(d.l_ptr),
                                dp<uint32_t>(d.l_idx), dp<CT>(d.l_val), dp<uint64_t>(d.lt_ptr), dp<uint32_t>(d.lt_idx),
-                               dp<CT>(d.lt_val), dp<CT>(d.piv), redp, dp<double>(mu2), xsp, k, kshift);
+                               dp<CT>(d.lt_val), dp<CT>(d.piv), redp, dp<double>(mu2), xsp, k, kshift, gate);
         }
         if (d.n_dtiles) {
             // column windows of exact power-of-two widths (<= 16): the wave-uniform red loads
@@ -765,7 +807,7 @@ struct SchwarzRunImpl : SchwarzRun {
                 while (kc > k - c0) kc >>= 1;
 #define WX_DS(N)                                                                                                   \
     hipLaunchKernelGGL((sk_dense_sym<CT, N>), dim3(d.n_dtiles), dim3(64), 0, st, dp<DenseTile>(d.dtiles),         \
-                       dp<CT>(d.dmat), redp, dp<double>(dpart), k, c0)
+                       dp<CT>(d.dmat), redp, dp<double>(dpart), k, c0, gate)
                 if (kc == 16) WX_DS(16);
                 else if (kc == 8) WX_DS(8);
                 else if (kc == 4) WX_DS(4);
@@ -776,25 +818,25 @@ struct SchwarzRunImpl : SchwarzRun {
             }
             hipLaunchKernelGGL(sk_dense_reduce, dim3(grid_rows(d.dense_part_rows, kshift)), dim3(BLK), 0, st,
                                dp<DenseComp>(d.dcomps), d.n_dcomps, dp<uint64_t>(d.drow_start), dp<double>(dpart), xsp,
-                               k, kshift);
+                               k, kshift, gate);
         }
         if (d.n_drows)
             hipLaunchKernelGGL((sk_dense<CT>), dim3(grid_rows(d.n_drows, kshift)), dim3(BLK), 0, st, d.n_drows,
                                dp<uint64_t>(d.drow_off), dp<uint32_t>(d.drow_base), dp<uint32_t>(d.drow_nk),
-                               dp<uint32_t>(d.drow_slot), dp<CT>(d.dmat), redp, xsp, k, kshift);
+                               dp<uint32_t>(d.drow_slot), dp<CT>(d.dmat), redp, xsp, k, kshift, gate);
         // back-substitution on eliminated rows
         hipLaunchKernelGGL((sk_rows<CT, 1>), dim3(grid_rows(d.n_c_items, kshift)), dim3(BLK), 0, st,
                            dp<RowItem>(d.c_items), d.n_c_items, dp<uint32_t>(d.c_idx), dp<CT>(d.c_val),
                            dp<uint32_t>(d.slot_comp), locp, xsp, dp<CT>(d.inv_de), dp<double>(mu1), xsp,
-                           dp<double>(rowpart), k, kshift);
+                           dp<double>(rowpart), k, kshift, gate);
         if (d.n_c_fix)
             hipLaunchKernelGGL((sk_rows_fix<CT, 1>), dim3(grid_rows(d.n_c_fix, kshift)), dim3(BLK), 0, st,
                                dp<RowFix>(d.c_fix), d.n_c_fix, dp<uint32_t>(d.slot_comp), locp, dp<CT>(d.inv_de),
-                               dp<double>(mu1), dp<double>(rowpart), xsp, k, kshift);
-        mean(d.all_chunks, d.n_all_chunks, d.all_cptr, d.comp_count, xsp, dp<double>(mu3), k, kshift, st);
+                               dp<double>(mu1), dp<double>(rowpart), xsp, k, kshift, gate);
+        mean(d.all_chunks, d.n_all_chunks, d.all_cptr, d.comp_count, xsp, dp<double>(mu3), k, kshift, st, gate);
         hipLaunchKernelGGL((sk_prolong<CT>), dim3(grid_rows(d.N, kshift)), dim3(BLK), 0, st, d.N,
                            dp<uint64_t>(d.d_ptr), dp<uint32_t>(d.d_slot), dp<CT>(d.sgnpw),
-                           dp<uint32_t>(d.slot_comp), xsp, dp<double>(mu3), z, k, kshift);
+                           dp<uint32_t>(d.slot_comp), xsp, dp<double>(mu3), z, k, kshift, gate);
         WX_HIP(hipGetLastError());
     }
 };
@@ -857,7 +899,7 @@ void precond_apply_device(const Precond &p, const double *x, uint64_t k, double 
         const uint64_t tot = n * kk;
         hipLaunchKernelGGL(sk_to_interleaved, dim3((tot + BLK - 1) / BLK), dim3(BLK), 0, st, n, x + c0 * n,
                            dp<double>(xi), kk);
-        run->apply(dp<double>(xi), dp<double>(yi), kk, st);
+        run->apply(dp<double>(xi), dp<double>(yi), kk, st, nullptr);
         hipLaunchKernelGGL(sk_from_interleaved, dim3((tot + BLK - 1) / BLK), dim3(BLK), 0, st, n, dp<double>(yi),
                            y + c0 * n, kk);
         WX_HIP(hipStreamSynchronize(st));

@@ -58,8 +58,9 @@ struct Precond::Schwarz {
 // Device apply of an additive Schwarz preconditioner for one batch width.
 struct SchwarzRun {
     virtual ~SchwarzRun() = default;
-    // v = M^{-1} pt for N x k interleaved FP64 arrays, enqueued on `st`.
-    virtual void apply(const double *pt, double *v, uint32_t k, hipStream_t st) = 0;
+    // v = M^{-1} pt for N x k interleaved FP64 arrays, enqueued on `st`.  `gate` (k flags on
+    // the device, or null) lets every kernel return at once when no flag is set.
+    virtual void apply(const double *pt, double *v, uint32_t k, hipStream_t st, const int32_t *gate) = 0;
 };
 
 std::unique_ptr<SchwarzRun> make_schwarz_run(const Precond &p, int32_t precision, hipStream_t st);

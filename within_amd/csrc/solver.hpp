@@ -62,6 +62,8 @@ struct DevDesign {
     DevBuf codes, slopes, sw, w, obs_perm, inv_perm, inv_diag;
     std::vector<DevTerm> terms;
     uint32_t max_partials = 0;
+    bool plain = false; // every term is a single intercept column
+    int32_t fuse_term = -1; // sorted single-intercept term whose D' is fused into the gather
     DesignArgs args() const;
 };
 
