@@ -159,6 +159,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="also run a plain 1 GiB device-to-device copy (calibrates HBM byte counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only: --gpus {args.gpus} is not supported by this build "
@@ -177,6 +179,15 @@ def main():
     n, k, N = args.n, args.k, solver.n_dofs
     log(f"[bench] setup (design + preconditioner build + upload) {setup:.2f} s, n_dofs={N}")
     Y_dev = torch.from_numpy(np.ascontiguousarray(Y.T)).cuda()  # (k, n) row-major == (n, k) column-major
+
+    if args.copy_probe:
+        src = torch.empty(1 << 27, dtype=torch.float64, device="cuda").fill_(1.0)
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        log("[bench] copy probe: 1 GiB read + 1 GiB written")
+        del src, dst
 
     res = {}
     for precision in (["mixed"] if args.no_fp64 else ["mixed", "fp64"]):

@@ -22,8 +22,10 @@ namespace wx {
 namespace {
 
 constexpr int BLK = 256;
-constexpr uint32_t SEG_CHUNK = 256; // rows per D' work item before a level is split
-constexpr uint32_t MAX_PART_BLOCKS = 4096;
+constexpr uint32_t SEG_CHUNK = 64;  // rows per D' work item before a level is split
+constexpr uint32_t FIX_CHUNK = 64;  // partials per combine item
+constexpr uint32_t MAX_PART_BLOCKS = 4096;   // grid cap of the grid-stride kernels
+constexpr uint32_t MAX_FUSED_BLOCKS = 32768; // grid cap of the fused gather (one item per row group)
 
 // Per-column LSMR state (structure of arrays, each of length MAX_BATCH).
 struct ColState {
@@ -64,7 +66,8 @@ __device__ inline void gather_rows(const DesignArgs &d, const double *__restrict
 #pragma unroll
         for (int t = 0; t < TT; ++t)
 #pragma unroll
-            for (int q = 0; q < R; ++q) code[t][q] = ok[q] ? d.codes[(uint64_t)t * d.n + rows[q]] : 0u;
+            for (int q = 0; q < R; ++q)
+                code[t][q] = ok[q] ? d.codes[(uint64_t)t * d.n + rows[q]] : 0u;
         double val[TT][R];
 #pragma unroll
         for (int t = 0; t < TT; ++t)
@@ -209,10 +212,13 @@ __global__ __launch_bounds__(BLK) void k_gather_seg(DesignArgs d, TermInfo ts, c
 // crates/within/src/operator/design/scatter.rs:14-242 (sequential / fold / CAS atomics /
 // run-coalesced atomics) with one atomic-free code-sorted segmented sum.
 // ---------------------------------------------------------------------------
-template <int NC, int SCALE> // SCALE: 0 none, 1 sqrt-weights, 2 weights (= scale^2)
+// SCALE: multiply each row by a per-row factor (sqrt-weights or weights).  The factor array is
+// indexed by position in the term's sorted order (a copy permuted at upload), so it is read
+// sequentially; only the k-wide rows of src are gathered through the permutation.
+template <int NC, bool SCALE>
 __global__ __launch_bounds__(BLK) void k_segreduce(DesignArgs d, TermInfo ti, const uint32_t *__restrict__ perm,
                                                    const SegItem *__restrict__ items, uint32_t n_items,
-                                                   const double *__restrict__ scale, const double *__restrict__ src,
+                                                   const double *__restrict__ scale_pos, const double *__restrict__ src,
                                                    double *__restrict__ out, double *__restrict__ partials,
                                                    const int32_t *__restrict__ active, uint32_t k, uint32_t kshift) {
     const uint32_t kp = 1u << kshift;
@@ -229,14 +235,16 @@ __global__ __launch_bounds__(BLK) void k_segreduce(DesignArgs d, TermInfo ti, co
         uint32_t p = item.begin;
         for (; p + U <= item.end; p += U) {
             uint64_t row[U];
-            double b[U];
+            double b[U], sc[U];
 #pragma unroll
-            for (int q = 0; q < (int)U; ++q) row[q] = perm ? perm[p + q] : p + q;
+            for (int q = 0; q < (int)U; ++q) {
+                row[q] = perm ? perm[p + q] : p + q;
+                sc[q] = SCALE ? scale_pos[p + q] : 1.0;
+            }
 #pragma unroll
             for (int q = 0; q < (int)U; ++q) {
                 b[q] = src[row[q] * k + j];
-                if (SCALE == 1) b[q] = scale[row[q]] * b[q];
-                if (SCALE == 2) b[q] = (scale[row[q]] * scale[row[q]]) * b[q];
+                if (SCALE) b[q] = sc[q] * b[q];
             }
 #pragma unroll
             for (int q = 0; q < (int)U; ++q)
@@ -247,8 +255,7 @@ __global__ __launch_bounds__(BLK) void k_segreduce(DesignArgs d, TermInfo ti, co
         for (; p < item.end; ++p) {
             const uint64_t row = perm ? perm[p] : p;
             double b = src[row * k + j];
-            if (SCALE == 1) b = scale[row] * b;
-            if (SCALE == 2) b = (scale[row] * scale[row]) * b;
+            if (SCALE) b = scale_pos[p] * b;
 #pragma unroll
             for (int c = 0; c < NC; ++c) acc[c] += (ti.load[c] < 0 ? b : term_load(d, ti.load[c], row) * b);
         }
@@ -300,7 +307,7 @@ __global__ void k_segfix(TermInfo ti, const SegFix *__restrict__ fixes, uint32_t
 // needs no further pass over the observations.
 template <bool WEIGHTED>
 __global__ __launch_bounds__(BLK) void k_load_rhs(uint64_t n, const uint32_t *__restrict__ inv_perm,
-                                                  const double *__restrict__ sw, const double *__restrict__ Y,
+                                                  const double *__restrict__ sw_caller, const double *__restrict__ Y,
                                                   uint64_t ldy, double *__restrict__ y_int, double *__restrict__ part,
                                                   int32_t *__restrict__ bad, uint32_t k, uint32_t kshift) {
     extern __shared__ double tile[]; // 64 x (k + 1)
@@ -330,7 +337,7 @@ __global__ __launch_bounds__(BLK) void k_load_rhs(uint64_t n, const uint32_t *__
                 const uint64_t i = inv_perm ? inv_perm[r] : r;
                 const double y = tile[rr * ld + j];
                 y_int[i * k + j] = y;
-                const double b = WEIGHTED ? sw[i] * y : y;
+                const double b = WEIGHTED ? sw_caller[r] * y : y;
                 mx = fmax(mx, fabs(b));
                 sq += b * b;
             }
@@ -873,6 +880,11 @@ void upload_design(Solver &s, const std::vector<double> &weights_internal) {
         std::vector<uint32_t> inv(d.obs_perm.size());
         for (uint64_t i = 0; i < inv.size(); ++i) inv[d.obs_perm[i]] = (uint32_t)i;
         dv.inv_perm.upload(inv.data(), inv.size());
+        if (!weights_internal.empty()) { // sqrt-weights in caller order for the right-hand-side load
+            std::vector<double> swc(inv.size());
+            for (uint64_t r = 0; r < inv.size(); ++r) swc[r] = std::sqrt(weights_internal[inv[r]]);
+            dv.sw_caller.upload(swc.data(), swc.size());
+        }
     }
     if (s.precond && s.precond->kind == Precond::DIAGONAL)
         dv.inv_diag.upload(s.precond->inv_diag.data(), s.precond->inv_diag.size());
@@ -898,6 +910,15 @@ void upload_design(Solver &s, const std::vector<double> &weights_internal) {
         if (!tm.sorted) {
             std::vector<uint32_t> perm = stable_argsort(col, d.n_obs, tm.n_levels);
             dt.perm.upload(perm.data(), perm.size());
+            if (!weights_internal.empty()) { // weights in this term's sorted order (sequential reads)
+                std::vector<double> wp(perm.size()), swp(perm.size());
+                for (size_t q = 0; q < perm.size(); ++q) {
+                    wp[q] = weights_internal[perm[q]];
+                    swp[q] = std::sqrt(weights_internal[perm[q]]);
+                }
+                dt.w_pos.upload(wp.data(), wp.size());
+                dt.sw_pos.upload(swp.data(), swp.size());
+            }
         }
         std::vector<SegItem> items;
         std::vector<SegFix> fixes;
@@ -924,7 +945,45 @@ void upload_design(Solver &s, const std::vector<double> &weights_internal) {
         dt.fixes.upload(fixes.data(), fixes.size());
         dt.n_fixes = (uint32_t)fixes.size();
         dt.n_partials = partials;
-        dv.max_partials = std::max(dv.max_partials, partials * dt.info.ncols);
+        uint64_t slots = (uint64_t)partials * dt.info.ncols;
+        dt.stages.clear();
+        if (dt.info.ncols == 1) {
+            // combine stages: a split level with <= FIX_CHUNK partials is summed into its output,
+            // a larger one into ceil(count / FIX_CHUNK) partials of the next region
+            std::vector<SegFix> cur = fixes;
+            uint64_t in_base = 0, out_base = partials;
+            while (!cur.empty()) {
+                std::vector<SegItem> st_items;
+                std::vector<SegFix> next;
+                uint32_t np = 0;
+                for (const SegFix &f : cur) {
+                    if (f.count <= FIX_CHUNK) {
+                        st_items.push_back({f.level, f.first, f.first + f.count, (int32_t)f.level});
+                        continue;
+                    }
+                    const uint32_t first = np;
+                    uint32_t cnt = 0;
+                    for (uint32_t q = 0; q < f.count; q += FIX_CHUNK) {
+                        st_items.push_back({f.level, f.first + q, f.first + std::min(f.count, q + FIX_CHUNK),
+                                            -(int32_t)(np + 1)});
+                        ++np;
+                        ++cnt;
+                    }
+                    next.push_back({f.level, first, cnt, 0});
+                }
+                DevTerm::FixStage stage;
+                stage.items.upload(st_items.data(), st_items.size());
+                stage.n_items = (uint32_t)st_items.size();
+                stage.src_base = in_base;
+                stage.dst_base = out_base;
+                dt.stages.push_back(std::move(stage));
+                in_base = out_base;
+                out_base += np;
+                cur.swap(next);
+            }
+            slots = out_base;
+        }
+        dv.max_partials = std::max<uint32_t>(dv.max_partials, (uint32_t)slots);
     }
     // the sorted single-intercept term with the most work items gets its D' fused into the gather
     dv.fuse_term = -1;
@@ -938,9 +997,21 @@ void upload_design(Solver &s, const std::vector<double> &weights_internal) {
 namespace {
 
 // Combines the split levels of one term's D' reduction.
-void launch_segfix(const DevTerm &dt, const double *partials, double *out, const int32_t *active, uint32_t k,
+void launch_segfix(const DevTerm &dt, double *partials, double *out, const int32_t *active, uint32_t k,
                    uint32_t kshift, hipStream_t st) {
     if (!dt.n_fixes) return;
+    if (!dt.stages.empty()) {
+        const uint32_t groups_per_block = BLK >> kshift;
+        TermInfo fi = dt.info; // partials are already weighted by the column's load: plain sums
+        fi.load[0] = -1;
+        for (const DevTerm::FixStage &sg : dt.stages) {
+            const uint32_t grid = std::max<uint32_t>(1, (sg.n_items + groups_per_block - 1) / groups_per_block);
+            hipLaunchKernelGGL((k_segreduce<1, false>), dim3(grid), dim3(BLK), 0, st, DesignArgs{}, fi, nullptr,
+                               sg.items.as<SegItem>(), sg.n_items, nullptr, partials + sg.src_base * k, out,
+                               partials + sg.dst_base * k, active, k, kshift);
+        }
+        return;
+    }
     const uint32_t fgrid = dt.n_fixes;
     switch (dt.info.ncols) {
     case 1: hipLaunchKernelGGL((k_segfix<1>), dim3(fgrid), dim3(BLK), 0, st, dt.info, dt.fixes.as<SegFix>(), dt.n_fixes, partials, out, active, k, kshift); break;
@@ -950,10 +1021,11 @@ void launch_segfix(const DevTerm &dt, const double *partials, double *out, const
     }
 }
 
-// Launches the D' reduction of every term (but `skip`): out (N x k) = D' (scale .* src).
-void launch_adjoint(const DevDesign &dv, const DesignArgs &da, const double *scale, int scale_mode,
-                    const double *src, double *out, double *partials, const int32_t *active, uint32_t k,
-                    uint32_t kshift, hipStream_t st, int32_t skip = -1) {
+// Launches the D' reduction of every term (but `skip`): out (N x k) = D' (f .* src) with
+// f = 1 (which = 0), sqrt(w) (which = 1) or w (which = 2).
+void launch_adjoint(const DevDesign &dv, const DesignArgs &da, int which, const double *src, double *out,
+                    double *partials, const int32_t *active, uint32_t k, uint32_t kshift, hipStream_t st,
+                    int32_t skip = -1) {
     for (size_t t = 0; t < dv.terms.size(); ++t) {
         if ((int32_t)t == skip) continue;
         const DevTerm &dt = dv.terms[t];
@@ -961,19 +1033,22 @@ void launch_adjoint(const DevDesign &dv, const DesignArgs &da, const double *sca
         uint32_t grid = (dt.n_items + groups_per_block - 1) / groups_per_block;
         grid = std::max<uint32_t>(1, std::min<uint32_t>(grid, 65535u * 8));
         const uint32_t *perm = dt.perm.p ? dt.perm.as<uint32_t>() : nullptr;
-#define WX_SEG(NC, SM)                                                                                           \
-    hipLaunchKernelGGL((k_segreduce<NC, SM>), dim3(grid), dim3(BLK), 0, st, da, dt.info, perm,                   \
-                       dt.items.as<SegItem>(), dt.n_items, scale, src, out, partials, active, k, kshift)
-#define WX_SEG_NC(SM)                                                                                            \
+        // the factor in the term's sorted order: the plain arrays for the sorted term
+        const double *f = nullptr;
+        if (which == 1) f = perm ? dt.sw_pos.as<double>() : dv.sw.as<double>();
+        if (which == 2) f = perm ? dt.w_pos.as<double>() : dv.w.as<double>();
+#define WX_SEG(NC, SC)                                                                                           \
+    hipLaunchKernelGGL((k_segreduce<NC, SC>), dim3(grid), dim3(BLK), 0, st, da, dt.info, perm,                   \
+                       dt.items.as<SegItem>(), dt.n_items, f, src, out, partials, active, k, kshift)
+#define WX_SEG_NC(SC)                                                                                            \
     switch (dt.info.ncols) {                                                                                     \
-    case 1: WX_SEG(1, SM); break;                                                                                \
-    case 2: WX_SEG(2, SM); break;                                                                                \
-    case 3: WX_SEG(3, SM); break;                                                                                \
-    default: WX_SEG(4, SM); break;                                                                               \
+    case 1: WX_SEG(1, SC); break;                                                                                \
+    case 2: WX_SEG(2, SC); break;                                                                                \
+    case 3: WX_SEG(3, SC); break;                                                                                \
+    default: WX_SEG(4, SC); break;                                                                               \
     }
-        if (scale_mode == 0 || !scale) { WX_SEG_NC(0) }
-        else if (scale_mode == 1) { WX_SEG_NC(1) }
-        else { WX_SEG_NC(2) }
+        if (f) { WX_SEG_NC(true) }
+        else { WX_SEG_NC(false) }
 #undef WX_SEG_NC
 #undef WX_SEG
         launch_segfix(dt, partials, out, active, k, kshift, st);
@@ -1021,7 +1096,7 @@ uint32_t launch_gather_seg_t(const DevDesign &dv, const DesignArgs &da, const do
     const DevTerm &dt = dv.terms[dv.fuse_term];
     const uint32_t groups_per_block = BLK >> kshift;
     const uint32_t grid = std::max<uint32_t>(
-        1, std::min<uint32_t>((dt.n_items + groups_per_block - 1) / groups_per_block, MAX_PART_BLOCKS));
+        1, std::min<uint32_t>((dt.n_items + groups_per_block - 1) / groups_per_block, MAX_FUSED_BLOCKS));
     const SegItem *items = dt.items.as<SegItem>();
 #define WX_GS(TT, PL)                                                                                            \
     hipLaunchKernelGGL((k_gather_seg<W, F, TT, PL>), dim3(grid), dim3(BLK), 0, st, da, dt.info, items, dt.n_items, \
@@ -1140,7 +1215,7 @@ std::shared_ptr<Workspace> new_workspace(const DevDesign &dv, uint32_t k) {
     w->u.alloc(n * k * 8);
     w->yint.alloc(n * k * 8);
     for (DevBuf *b : {&w->v, &w->pt, &w->atu, &w->h, &w->hb, &w->x}) b->alloc(N * k * 8);
-    w->part.alloc((size_t)MAX_PART_BLOCKS * kp * 3 * 8);
+    w->part.alloc((size_t)std::max(MAX_PART_BLOCKS, MAX_FUSED_BLOCKS) * kp * 3 * 8);
     w->segpart.alloc((size_t)std::max<uint32_t>(dv.max_partials, 1) * k * 8);
     w->bad.alloc(8);
     w->red.alloc(2 * MAX_BATCH * 8);
@@ -1221,7 +1296,8 @@ void solve_batch_device(const Solver &s, const double *Y, uint64_t ldy, uint64_t
         int32_t *badp = ws.bad.as<int32_t>();
         if (weighted)
             hipLaunchKernelGGL((k_load_rhs<true>), dim3(gt), dim3(BLK), tile_bytes, st, n, inv_perm,
-                               dv.sw.as<double>(), Yc, ldy, ws.yint.as<double>(), part, badp, k, kshift);
+                               inv_perm ? dv.sw_caller.as<double>() : dv.sw.as<double>(), Yc, ldy,
+                               ws.yint.as<double>(), part, badp, k, kshift);
         else
             hipLaunchKernelGGL((k_load_rhs<false>), dim3(gt), dim3(BLK), tile_bytes, st, n, inv_perm, nullptr, Yc,
                                ldy, ws.yint.as<double>(), part, badp, k, kshift);
@@ -1241,7 +1317,7 @@ void solve_batch_device(const Solver &s, const double *Y, uint64_t ldy, uint64_t
         }
 
         // --- init: p~ = A'b / beta_1, v = M^{-1} p~, alpha_1 ---
-        launch_adjoint(dv, da, weighted ? dv.w.as<double>() : nullptr, weighted ? 1 : 0, ws.yint.as<double>(),
+        launch_adjoint(dv, da, weighted ? 2 : 0, ws.yint.as<double>(),
                        ws.pt.as<double>(), ws.segpart.as<double>(), cs.active, k, kshift, st);
         if (prec_kind == 2) {
             hipLaunchKernelGGL((k_ptilde<2, true>), dim3(gN), dim3(BLK), 0, st, N, nullptr, ws.pt.as<double>(),
@@ -1281,7 +1357,7 @@ void solve_batch_device(const Solver &s, const double *Y, uint64_t ldy, uint64_t
             if (dv.fuse_term >= 0)
                 launch_segfix(dv.terms[dv.fuse_term], ws.segpart.as<double>(), ws.atu.as<double>(), cs.active, k,
                               kshift, st);
-            launch_adjoint(dv, da, weighted ? dv.sw.as<double>() : nullptr, weighted ? 1 : 0, ws.u.as<double>(),
+            launch_adjoint(dv, da, weighted ? 1 : 0, ws.u.as<double>(),
                            ws.atu.as<double>(), ws.segpart.as<double>(), cs.active, k, kshift, st, dv.fuse_term);
             timer.end(1);
             if (prec_kind == 2) {
@@ -1342,8 +1418,7 @@ void solve_batch_device(const Solver &s, const double *Y, uint64_t ldy, uint64_t
         }
         // residual numerator ||D'W(y - Dx)||; the denominator ||D'W y|| = beta_1 ||p~_1|| was
         // kept at initialisation
-        const double *wv = weighted ? dv.w.as<double>() : nullptr;
-        launch_adjoint(dv, da, wv, weighted ? 1 : 0, dm_int, ws.h.as<double>(), ws.segpart.as<double>(), nullptr, k,
+        launch_adjoint(dv, da, weighted ? 2 : 0, dm_int, ws.h.as<double>(), ws.segpart.as<double>(), nullptr, k,
                        kshift, st);
         double num[MAX_BATCH];
         DevBuf &red = ws.red;
@@ -1485,7 +1560,7 @@ void design_apply_device(const Solver &s, bool adjoint, const double *X, uint64_
         } else {
             hipLaunchKernelGGL(k_interleave, dim3((n * k + BLK - 1) / BLK), dim3(BLK), 0, st, n, X + c0 * n, n,
                                obs_perm, in.as<double>(), k);
-            launch_adjoint(dv, da, weighted ? dv.sw.as<double>() : nullptr, weighted ? 1 : 0, in.as<double>(),
+            launch_adjoint(dv, da, weighted ? 1 : 0, in.as<double>(),
                            out.as<double>(), part.as<double>(), nullptr, k, kshift, st);
             hipLaunchKernelGGL(k_deinterleave, dim3((N * k + BLK - 1) / BLK), dim3(BLK), 0, st, N, out.as<double>(),
                                Y + c0 * N, N, nullptr, k);

@@ -50,16 +50,26 @@ struct SegFix {
 struct DevTerm {
     TermInfo info;
     DevBuf perm;   // position -> internal row, empty when the term is sorted
+    DevBuf sw_pos, w_pos; // sqrt-weights and weights by position (unsorted weighted terms)
     DevBuf items;  // SegItem[n_items]
     uint32_t n_items = 0;
-    DevBuf fixes;  // SegFix[n_fixes]
+    DevBuf fixes;  // SegFix[n_fixes] (one-stage combine, terms with several columns)
     uint32_t n_fixes = 0;
     uint32_t n_partials = 0;
+    // single-column terms: the split levels' partials are combined in stages, each stage a
+    // segmented sum over the previous stage's partial slots (<= 64 per item, in slot order)
+    struct FixStage {
+        DevBuf items;    // SegItem over partial slots of region src_base
+        uint32_t n_items = 0;
+        uint64_t src_base = 0, dst_base = 0; // partial-slot offsets of the input / output regions
+    };
+    std::vector<FixStage> stages;
 };
 
 struct DevDesign {
     uint64_t n = 0, N = 0;
     DevBuf codes, slopes, sw, w, obs_perm, inv_perm, inv_diag;
+    DevBuf sw_caller; // sqrt-weights in caller row order (when the rows are permuted)
     std::vector<DevTerm> terms;
     uint32_t max_partials = 0;
     bool plain = false; // every term is a single intercept column
