@@ -27,6 +27,11 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
 
+# HBM bytes per k_gather_seg dispatch at the default workload (n = 5,000,000, 3 factors,
+# k = 16), measured with rocprofv3 FETCH_SIZE and WRITE_SIZE passes scaled by a 1 GiB copy of the
+# same run (profiles/k_gather_seg_n5M_k16_hbm_traffic.txt); None for other workloads.
+GATHER_TRAFFIC_BYTES = {(5_000_000, 3, 16): 1.349079e9 + 0.675656e9}
+
 
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
@@ -230,7 +235,7 @@ def main():
                                "C 50 uniform; k=16; weights U(0.5,2); tol 1e-8",
                    "n": n, "k": k, "n_dofs": N, "precision": "mixed"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": kern,
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": GATHER_TRAFFIC_BYTES.get((n, F, k)), "kernel": kern,
                      "algorithmic_bytes_per_launch": bytes_per, "avg_launch_ms": dur * 1e3},
         "roofline_adjoint_rest": {"achieved": b_rest / adj_dur / 1e9, "unit": "GB/s",
                                   "frac": b_rest / adj_dur / 1e9 / HBM_PEAK_GBS,
