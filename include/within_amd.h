@@ -111,7 +111,9 @@ typedef struct wx_rhs_stats {
     int64_t iterations;
     double residual;           /* LSMR estimate |zeta_k| / |zeta_0| (M-metric when preconditioned) */
     double residual_norm;      /* |phi_bar| */
-    double true_residual;      /* ||D'W(z - D x)|| / ||D'W z|| computed in FP64 on the device */
+    double true_residual;      /* ||D'W(z - D x)|| / ||D'W z|| computed in FP64 on the device; with
+                                  slope covariates D holds the per-level reparametrised slope
+                                  columns the solver works in (the residual z - D x is the same) */
     double time_solve;         /* seconds */
 } wx_rhs_stats;
 

@@ -84,6 +84,10 @@ class _Schwarz(C.Structure):
         ("fac_idx", C.c_void_p),
         ("piv_off", C.c_void_p),
         ("piv", C.c_void_p),
+        ("fold", C.c_void_p),
+        ("gauge", C.c_void_p),
+        ("cover_off", C.c_void_p),
+        ("cover_perm", C.c_void_p),
     ]
 
 
@@ -257,7 +261,12 @@ class OracleSchwarz:
         dtypes = dict(slot_start=np.uint64, ne=np.uint64, nk=np.uint64, glob=np.uint32, pw=np.float64,
                       inv_de=np.float64, c_ptr=np.uint64, c_idx=np.uint32, c_val=np.float64,
                       fac_kind=np.int32, fac_off=np.uint64, fac_val=np.float64, fac_ptr=np.uint64,
-                      fac_idx=np.uint32, piv_off=np.uint64, piv=np.float64)
+                      fac_idx=np.uint32, piv_off=np.uint64, piv=np.float64, fold=np.float64, gauge=np.int32,
+                      cover_off=np.uint64, cover_perm=np.uint32)
+        if arrays.get("fold") is None:
+            raise ValueError("OracleSchwarz needs the per-slot coordinate factors `fold`")
+        if arrays.get("gauge") is None:
+            raise ValueError("OracleSchwarz needs the per-component `gauge`")
         for key, dt in dtypes.items():
             a = arrays.get(key)
             if a is None or len(a) == 0:

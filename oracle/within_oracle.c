@@ -534,8 +534,19 @@ int wo_lsmr_dense(const double *a, uint64_t m, uint64_t n, const double *b, int3
  *              diagonal, with pivots d[] (the last pivot of a floating Laplacian is 0
  *              and is pseudo-inverted to 0), S = L diag(d) L'
  *     sparse : fac_kind 1, L strictly-lower in CSR by rows (row i lists j < i), pivots d[]
- *     inverse: fac_kind 2, an explicit symmetric nk x nk operator (the pseudo-inverse S^+)
- *              at fac_off, applied as a matrix-vector product.
+ *     inverse: fac_kind 2, an explicit symmetric nk x nk operator (the pseudo-inverse S^+
+ *              of a floating Laplacian, S^-1 of a definite block) at fac_off, applied as a
+ *              matrix-vector product.
+ *     cover  : fac_kind 3, a sparse factor (as kind 1) of the Gremban cover of a signed kept
+ *              block: 2 nk rows in elimination order, row t holding cover vertex
+ *              cover_perm[cover_off + t] (vertex v < nk = kept level v, else level v - nk on
+ *              the second sheet); solved on the antisymmetric embed [b, -b] and read back as
+ *              (y+ - y-)/2 (block_elim/factor.rs:93-118).
+ *   fold[slot]   coordinate map into SDDM form (sddm.rs:53-82): +1 / -1 on the two blocks of
+ *                a plain pair, sign * dominance scale of a general one; applied on restriction
+ *                and again on prolongation.
+ *   gauge[comp]  0 floating: the component mean is removed before and after the solve and the
+ *                reduced rhs mean before the reduced solve; 1 none (grounded or cover).
  * The local solve follows crates/within/src/block_elim/solver.rs:350-466 (fold, mean,
  * eliminate, reduced solve, back-substitution, mean, unfold) and the subdomain wrapper
  * crates/schwarz-precond/src/local_solve.rs:106-147 with domain.rs:135-188 (weighted
@@ -557,6 +568,10 @@ typedef struct {
     const uint32_t *fac_idx;    /* sparse column indices (local) */
     const uint64_t *piv_off;    /* per component: into piv */
     const double *piv;          /* pivots d */
+    const double *fold;         /* per slot */
+    const int32_t *gauge;       /* per component */
+    const uint64_t *cover_off;  /* per component: into cover_perm */
+    const uint32_t *cover_perm;
 } wo_schwarz;
 
 static double compensated_sum(const double *v, uint64_t n) {
@@ -575,9 +590,37 @@ static void subtract_mean(double *v, uint64_t n) {
     for (uint64_t i = 0; i < n; ++i) v[i] -= mean;
 }
 
+/* Sparse unit-lower LDL' solve in place over n rows (a zero pivot, the floating gauge, gives 0). */
+static void sparse_ldl_solve(const wo_schwarz *s, uint64_t c, double *x, uint64_t n) {
+    const double *d = s->piv + s->piv_off[c];
+    const uint64_t *ptr = s->fac_ptr + s->fac_off[c];
+    for (uint64_t i = 0; i < n; ++i) {
+        double acc = x[i];
+        for (uint64_t p = ptr[i]; p < ptr[i + 1]; ++p) acc -= s->fac_val[p] * x[s->fac_idx[p]];
+        x[i] = acc;
+    }
+    for (uint64_t i = 0; i < n; ++i) x[i] = d[i] != 0.0 ? x[i] / d[i] : 0.0;
+    for (uint64_t ii = n; ii-- > 0;) {
+        double xi = x[ii];
+        for (uint64_t p = ptr[ii]; p < ptr[ii + 1]; ++p) x[s->fac_idx[p]] -= s->fac_val[p] * xi;
+    }
+}
+
 /* LDL' solve of the reduced kept-block system, x overwritten in place. */
 static void reduced_solve(const wo_schwarz *s, uint64_t c, double *x, uint64_t nk) {
     const double *d = s->piv + s->piv_off[c];
+    if (s->fac_kind[c] == 3) { /* Gremban cover: embed [x, -x] in elimination order, solve, read back */
+        const uint32_t *perm = s->cover_perm + s->cover_off[c];
+        double *y = malloc((2 * nk ? 2 * nk : 1) * sizeof(double));
+        for (uint64_t t = 0; t < 2 * nk; ++t) y[t] = perm[t] < nk ? x[perm[t]] : -x[perm[t] - nk];
+        sparse_ldl_solve(s, c, y, 2 * nk);
+        for (uint64_t t = 0; t < 2 * nk; ++t)
+            if (perm[t] < nk) x[perm[t]] = 0.5 * y[t];
+        for (uint64_t t = 0; t < 2 * nk; ++t)
+            if (perm[t] >= nk) x[perm[t] - nk] -= 0.5 * y[t];
+        free(y);
+        return;
+    }
     if (s->fac_kind[c] == 2) { /* explicit symmetric operator S^+ (nk x nk row-major) */
         const double *P = s->fac_val + s->fac_off[c];
         double *t = malloc((nk ? nk : 1) * sizeof(double));
@@ -600,17 +643,7 @@ static void reduced_solve(const wo_schwarz *s, uint64_t c, double *x, uint64_t n
             x[ii] = acc;
         }
     } else {
-        const uint64_t *ptr = s->fac_ptr + s->fac_off[c];
-        for (uint64_t i = 0; i < nk; ++i) {
-            double acc = x[i];
-            for (uint64_t p = ptr[i]; p < ptr[i + 1]; ++p) acc -= s->fac_val[p] * x[s->fac_idx[p]];
-            x[i] = acc;
-        }
-        for (uint64_t i = 0; i < nk; ++i) x[i] = d[i] != 0.0 ? x[i] / d[i] : 0.0;
-        for (uint64_t ii = nk; ii-- > 0;) {
-            double xi = x[ii];
-            for (uint64_t p = ptr[ii]; p < ptr[ii + 1]; ++p) x[s->fac_idx[p]] -= s->fac_val[p] * xi;
-        }
+        sparse_ldl_solve(s, c, x, nk);
     }
 }
 
@@ -625,11 +658,12 @@ int wo_schwarz_apply(const wo_schwarz *s, const double *r, double *z) {
     double *rl = malloc((maxn + 1) * sizeof(double)), *sl = malloc((maxn + 1) * sizeof(double));
     for (uint64_t c = 0; c < s->n_comp; ++c) {
         const uint64_t base = s->slot_start[c], ne = s->ne[c], nk = s->nk[c], nl = ne + nk;
+        const int floating = s->gauge[c] == 0;
         /* restrict_weighted (domain.rs:135-152) */
         for (uint64_t i = 0; i < nl; ++i) rl[i] = r[s->glob[base + i]] * s->pw[base + i];
-        /* fold: Canonical negates the kept block (sddm.rs:62-82) */
-        for (uint64_t i = ne; i < nl; ++i) rl[i] = -rl[i];
-        subtract_mean(rl, nl);
+        /* fold into SDDM coordinates (sddm.rs:62-82) */
+        for (uint64_t i = 0; i < nl; ++i) rl[i] *= s->fold[base + i];
+        if (floating) subtract_mean(rl, nl);
         /* eliminate_and_recover (block_elim/solver.rs:350-423) */
         for (uint64_t i = 0; i < ne; ++i) rl[i] *= s->inv_de[base + i];
         double *red = sl + ne; /* reduced rhs built in place in the kept half of sol */
@@ -638,7 +672,7 @@ int wo_schwarz_apply(const wo_schwarz *s, const double *r, double *z) {
             for (uint64_t p = s->c_ptr[base + i]; p < s->c_ptr[base + i + 1]; ++p)
                 red[s->c_idx[p]] += s->c_val[p] * rl[i];
         }
-        subtract_mean(red, nk);
+        if (floating) subtract_mean(red, nk);
         reduced_solve(s, c, red, nk);
         for (uint64_t i = 0; i < ne; ++i) { /* x_e = b_e + D_e^{-1} (C x_k) */
             double acc = 0.0;
@@ -646,8 +680,8 @@ int wo_schwarz_apply(const wo_schwarz *s, const double *r, double *z) {
                 acc += s->c_val[p] * red[s->c_idx[p]];
             sl[i] = rl[i] + s->inv_de[base + i] * acc;
         }
-        subtract_mean(sl, nl);
-        for (uint64_t i = ne; i < nl; ++i) sl[i] = -sl[i]; /* unfold */
+        if (floating) subtract_mean(sl, nl);
+        for (uint64_t i = 0; i < nl; ++i) sl[i] *= s->fold[base + i]; /* unfold */
         for (uint64_t i = 0; i < nl; ++i) z[s->glob[base + i]] += sl[i] * s->pw[base + i];
     }
     free(rl);

@@ -8,7 +8,7 @@ import pytest
 import within_amd as wa
 from within_amd import _lib
 from oracle.oracle import exact_demean, normal_eq_residual, sparse_design
-from tests._helpers import oracle_schwarz_from_bytes, parse_precond
+from tests._helpers import oracle_schwarz_from_bytes, parse_precond, whiten
 from tests.conftest import cfg1_data
 
 pytestmark = pytest.mark.gpu
@@ -283,3 +283,80 @@ def test_sparse_factor_apply_matches_oracle(gpu, oracle, case):
         r = s.solve_batch(y, wa.LsmrOptions(tol=1e-10, precision=precision))
         assert all(r.converged)
         np.testing.assert_allclose(r.demeaned, SMALL[f"{case}__demeaned"], atol=1e-8)
+
+
+# ---------------------------------------------------------------------------
+# additive Schwarz over slope (signed) channel pairs
+# ---------------------------------------------------------------------------
+
+def _slope_design(seed, n=3000, la=80, lb=30, lc=4):
+    rng = np.random.default_rng(seed)
+    a = rng.integers(0, la, n).astype(np.uint32)
+    b = rng.integers(0, lb, n).astype(np.uint32)
+    c = rng.integers(0, lc, n).astype(np.uint32)
+    z = rng.standard_normal(n)
+    w = rng.uniform(0.5, 2.0, n)
+    design = [wa.Effect(a, True), wa.Effect(b, True, [z]), wa.Effect(c, True)]
+    D = sparse_design(np.column_stack([a, b, c]), effects=[(True, []), (True, [z]), (True, [])])
+    # the solver's columns: the slope whitened per level (its normal-equation residual basis)
+    Dw = sparse_design(np.column_stack([a, b, c]), effects=[(True, []), (True, [whiten(b, z, w)]), (True, [])])
+    return rng, design, D, Dw, w
+
+
+@pytest.mark.parametrize("dense_threshold", [None, 0])
+def test_slope_schwarz_apply_matches_oracle(gpu, oracle, dense_threshold):
+    rng, design, _, _, w = _slope_design(51)
+    cfg = wa.config.AdditiveSchwarz(wa.config.LocalSolverConfig(dense_threshold=dense_threshold))
+    s = wa.Solver(design, weights=w, preconditioner=cfg)
+    p = s.preconditioner
+    kinds = set(parse_precond(p.to_bytes())["fac_kind"].tolist())
+    assert kinds == ({0} if dense_threshold is None else {1, 2})
+    sch = oracle_schwarz_from_bytes(oracle, p.to_bytes())
+    for _ in range(3):
+        x = rng.standard_normal(s.n_dofs)
+        np.testing.assert_allclose(p.apply(x), sch.apply(x), rtol=1e-10, atol=1e-11)
+
+
+@pytest.mark.parametrize("dense_threshold", [None, 0])
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_slope_additive_solve_matches_exact(gpu, dense_threshold, precision):
+    rng, design, D, Dw, w = _slope_design(52)
+    Y = np.asfortranarray(rng.standard_normal((D.shape[0], 3)))
+    cfg = wa.config.AdditiveSchwarz(wa.config.LocalSolverConfig(dense_threshold=dense_threshold))
+    s = wa.Solver(design, weights=w, preconditioner=cfg)
+    r = s.solve_batch(Y, wa.LsmrOptions(tol=1e-10, precision=precision))
+    assert all(r.converged)
+    ex = exact_demean(D, Y, w)
+    for j in range(Y.shape[1]):
+        assert rel(r.demeaned[:, j], ex[:, j]) <= 1e-8, j
+    np.testing.assert_allclose(r.true_residual, normal_eq_residual(Dw, Y, r.demeaned, w), rtol=1e-6, atol=1e-13)
+    # the preconditioner does its job: far fewer iterations than the unpreconditioned solve
+    off = wa.Solver(design, weights=w, preconditioner=wa.PreconditionerConfig.Off)
+    r0 = off.solve_batch(Y, wa.LsmrOptions(tol=1e-10, maxiter=20000))
+    assert max(r.iterations) * 3 < max(r0.iterations)
+
+
+def test_worker_firm_panel_with_firm_slope(gpu):
+    # a miniature of the bipartite worker-firm panel: workers x 5 periods, Zipf initial firm,
+    # moves w.p. 0.05, year effect, firm intercept + slope; tol 1e-10 against the exact projection
+    rng = np.random.default_rng(3)
+    nw, T, nf = 3000, 5, 400
+    p = (np.arange(nf) + 1.0) ** -1.2
+    p /= p.sum()
+    firm = np.empty((nw, T), np.int64)
+    firm[:, 0] = rng.choice(nf, nw, p=p)
+    for t in range(1, T):
+        move = rng.uniform(size=nw) < 0.05
+        firm[:, t] = np.where(move, rng.integers(0, nf, nw), firm[:, t - 1])
+    worker = np.repeat(np.arange(nw), T).astype(np.uint32)
+    firm = firm.ravel().astype(np.uint32)
+    year = np.tile(np.arange(T), nw).astype(np.uint32)
+    z = rng.standard_normal(nw * T)
+    design = [wa.Effect(worker, True), wa.Effect(firm, True, [z]), wa.Effect(year, True)]
+    D = sparse_design(np.column_stack([worker, firm, year]), effects=[(True, []), (True, [z]), (True, [])])
+    Y = np.asfortranarray(rng.standard_normal((nw * T, 4)))
+    r = wa.solve_batch(design, Y, options=wa.LsmrOptions(tol=1e-10))
+    assert all(r.converged)
+    ex = exact_demean(D, Y)
+    for j in range(4):
+        assert rel(r.demeaned[:, j], ex[:, j]) <= 1e-8, j

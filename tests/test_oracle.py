@@ -217,7 +217,7 @@ def _export_exact_schwarz(oracle, codes, w):
     for ii in comps:
         count[ii] += 1
     A = {k: [] for k in ["slot_start", "ne", "nk", "glob", "pw", "inv_de", "c_ptr", "c_idx", "c_val", "fac_kind",
-                         "fac_off", "fac_val", "piv_off", "piv"]}
+                         "fac_off", "fac_val", "piv_off", "piv", "fold", "gauge"]}
     slot, fo, po, cnnz = 0, 0, 0, 0
     A["slot_start"].append(0)
     n_levels0 = int(codes[:, 0].max()) + 1
@@ -239,6 +239,8 @@ def _export_exact_schwarz(oracle, codes, w):
         A["nk"].append(nk)
         A["glob"] += E + K
         A["pw"] += [1 / np.sqrt(count[g]) for g in E + K]
+        A["fold"] += [1.0] * ne + [-1.0] * nk  # plain pair: the kept block is negated
+        A["gauge"].append(0)
         A["inv_de"] += [1 / G[g, g] for g in E] + [0.0] * nk
         for e in E:
             A["c_ptr"].append(cnnz)
@@ -292,3 +294,119 @@ def test_oracle_schwarz_solve_matches_golden(oracle):
     X, Dm, res = oracle.solve_batch(d, y, w, prec="schwarz", schwarz=sch, tol=1e-12)
     assert all(r.converged for r in res)
     np.testing.assert_allclose(Dm, SMALL["three_factor_weighted__demeaned"], atol=1e-8)
+
+
+# ---------------------------------------------------------------------------
+# general (signed) components: fold, gauge none, explicit inverse and Gremban cover
+# ---------------------------------------------------------------------------
+
+def _one_component_schwarz(oracle, M, ne, fold, kind, fac):
+    """Oracle layout of a single component whose SDDM-coordinate matrix is M (eliminated block
+    first, both diagonal blocks diagonal), folded by `fold`, with the kept-block factor `fac`."""
+    nk = M.shape[0] - ne
+    C = -M[:ne, ne:]
+    A = dict(slot_start=[0, ne + nk], ne=[ne], nk=[nk], glob=list(range(ne + nk)), pw=[1.0] * (ne + nk),
+             inv_de=[1 / M[e, e] for e in range(ne)] + [0.0] * nk, fold=list(fold), gauge=[1],
+             c_ptr=[], c_idx=[], c_val=[], fac_kind=[kind], fac_off=[0], piv_off=[0])
+    nnz = 0
+    for e in range(ne):
+        A["c_ptr"].append(nnz)
+        for k in range(nk):
+            if C[e, k] != 0:
+                A["c_idx"].append(k)
+                A["c_val"].append(C[e, k])
+                nnz += 1
+    A["c_ptr"] += [nnz] * (nk + 1)
+    A.update(fac)
+    return OracleSchwarz(oracle, ne + nk, {k: np.array(v) for k, v in A.items()})
+
+
+def _sparse_ldl_rows(S, order):
+    """LDL' of S permuted to `order` (no pivoting) as strictly-lower CSR rows; a pivot that is
+    round-off of a singular (floating) system becomes 0."""
+    P = S[np.ix_(order, order)].copy()
+    n = len(order)
+    L = np.eye(n)
+    d = np.zeros(n)
+    scale = np.abs(np.diag(S)).max()
+    for q in range(n):
+        d[q] = P[q, q] - L[q, :q] ** 2 @ d[:q]
+        if abs(d[q]) < 1e-12 * scale:
+            d[q] = 0.0
+        for i in range(q + 1, n):
+            v = P[i, q] - (L[i, :q] * L[q, :q]) @ d[:q]
+            L[i, q] = v / d[q] if d[q] != 0 else 0.0
+    ptr, idx, val = [0], [], []
+    for i in range(n):
+        for j in range(i):
+            if L[i, j] != 0:
+                idx.append(j)
+                val.append(L[i, j])
+        ptr.append(len(idx))
+    return ptr, idx, val, d
+
+
+def test_oracle_general_component_explicit_inverse(oracle):
+    # slope channel pair (term 0 intercept, term 1 slope) of a small weighted design: for any
+    # coordinate map F, fold -> exact kept inverse of M = F G F -> unfold gives G^-1
+    rng = np.random.default_rng(11)
+    n = 400
+    a, b = rng.integers(0, 12, n), rng.integers(0, 5, n)
+    z, w = rng.standard_normal(n), rng.uniform(0.5, 2.0, n)
+    ne, nk = 12, 5
+    G = np.zeros((ne + nk, ne + nk))
+    np.add.at(G, (a, a), w)
+    np.add.at(G, (ne + b, ne + b), w * z * z)
+    np.add.at(G, (a, ne + b), w * z)
+    np.add.at(G, (ne + b, a), w * z)
+    assert np.linalg.eigvalsh(G).min() > 0
+    F = rng.choice([-1.0, 1.0], ne + nk) * rng.uniform(0.5, 2.0, ne + nk)
+    M = F[:, None] * G * F[None, :]
+    De, C = np.diag(M)[:ne], -M[:ne, ne:]
+    S = np.diag(np.diag(M)[ne:]) - C.T @ np.diag(1 / De) @ C
+    sch = _one_component_schwarz(oracle, M, ne, F, 2, dict(fac_val=np.linalg.inv(S).ravel(), piv=[0.0]))
+    Gi = np.linalg.inv(G)
+    for _ in range(3):
+        r = rng.standard_normal(ne + nk)
+        np.testing.assert_allclose(sch.apply(r), Gi @ r, rtol=1e-10, atol=1e-12)
+
+
+@pytest.mark.parametrize("surplus", [True, False])
+def test_oracle_cover_solve_equals_inverse(oracle, surplus):
+    # frustrated SDD bipartite component; the reduced solve goes through an LDL' of the
+    # Gremban cover's Schur complement in a random elimination order (block_elim/solver.rs:221-277,
+    # factor.rs:93-118).  Without surplus the cover is a singular Laplacian (zero last pivot).
+    rng = np.random.default_rng(12 + surplus)
+    ne, nk = 9, 4
+    C = rng.uniform(0.2, 1.0, (ne, nk)) * rng.choice([-1.0, 1.0], (ne, nk))
+    C[rng.uniform(size=(ne, nk)) < 0.3] = 0.0
+    for e in range(ne):  # keep every eliminated level connected
+        if not C[e].any():
+            C[e, rng.integers(nk)] = 0.7
+    sur = rng.uniform(0.1, 0.5, ne + nk) if surplus else np.zeros(ne + nk)
+    d = np.concatenate([np.abs(C).sum(1), np.abs(C).sum(0)]) + sur
+    M = np.diag(d)
+    M[:ne, ne:] = -C
+    M[ne:, :ne] = -C.T
+    assert np.linalg.eigvalsh(M).min() > 1e-9  # frustrated: definite even without surplus
+    # cover: positive entries join equal sheets, negative ones opposite sheets
+    Cc = np.zeros((2 * ne, 2 * nk))
+    for e in range(ne):
+        for k in range(nk):
+            v = C[e, k]
+            if v > 0:
+                Cc[e, k] = Cc[ne + e, nk + k] = v
+            elif v < 0:
+                Cc[e, nk + k] = Cc[ne + e, k] = -v
+    Dc = np.concatenate([d[:ne], d[:ne], d[ne:], d[ne:]])
+    Sc = np.diag(Dc[2 * ne:]) - Cc.T @ np.diag(1 / Dc[:2 * ne]) @ Cc
+    order = rng.permutation(2 * nk)
+    ptr, idx, val, piv = _sparse_ldl_rows(Sc, order)
+    F = rng.choice([-1.0, 1.0], ne + nk) * rng.uniform(0.5, 2.0, ne + nk)
+    G = M / F[:, None] / F[None, :]  # original coordinates: M = F G F
+    fac = dict(fac_val=val, fac_ptr=ptr, fac_idx=idx, piv=piv, cover_off=[0], cover_perm=order)
+    sch = _one_component_schwarz(oracle, M, ne, F, 3, fac)
+    Gi = np.linalg.inv(G)
+    for _ in range(3):
+        r = rng.standard_normal(ne + nk)
+        np.testing.assert_allclose(sch.apply(r), Gi @ r, rtol=1e-9, atol=1e-11)

@@ -6,8 +6,9 @@ names, defaults, error types and output layouts.  Every solve runs on the GPU th
 ``libwithin_amd.so`` (ctypes releases the GIL for the duration of each native call).
 
 Additive fields beyond the reference: ``LsmrOptions.precision`` ("fp64" or "mixed": FP32
-preconditioner coefficients, FP64 Krylov), and per-RHS ``true_residual`` /
-``stop_reason`` on the results.
+preconditioner coefficients, FP64 Krylov), and per-RHS ``true_residual`` (the FP64
+normal-equation residual ||D'W(z - Dx)|| / ||D'Wz||; with slope covariates, D holds the
+per-level reparametrised slope columns the solver works in) / ``stop_reason`` on the results.
 """
 from __future__ import annotations
 

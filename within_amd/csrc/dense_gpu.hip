@@ -64,12 +64,12 @@ bool gpu_available() {
 }
 
 // S: n x n row-major symmetric Laplacian (connected); overwritten with S^+.
-bool dense_laplacian_pinv_gpu(std::vector<double> &S, uint32_t n) {
-    if (n <= 1) {
+bool dense_laplacian_pinv_gpu(std::vector<double> &S, uint32_t n, bool floating) {
+    if (floating && n <= 1) {
         std::fill(S.begin(), S.end(), 0.0);
         return true;
     }
-    const int64_t m = (int64_t)n - 1;
+    const int64_t m = floating ? (int64_t)n - 1 : (int64_t)n;
     Handle H;
     DevBuf A, X;
     A.alloc((size_t)m * m * 8);
@@ -106,6 +106,11 @@ bool dense_laplacian_pinv_gpu(std::vector<double> &S, uint32_t n) {
                         X.as<double>(), (rocblas_int)m, &zero, a, (rocblas_int)m));
     std::vector<double> G((size_t)m * m);
     WX_HIP(hipMemcpy(G.data(), a, (size_t)m * m * 8, hipMemcpyDeviceToHost));
+    if (!floating) { // positive definite block: the inverse itself, symmetrised
+        for (int64_t i = 0; i < m; ++i)
+            for (int64_t j = 0; j < m; ++j) S[(size_t)i * n + j] = i >= j ? G[i + j * m] : G[j + i * m];
+        return true;
+    }
     // symmetrise (column-major lower: G[i + j*m], i >= j), pad and project with P on both sides
     std::vector<double> rowmean(n, 0.0);
     double total = 0.0;

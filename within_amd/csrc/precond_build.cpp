@@ -12,8 +12,10 @@
 //     factor in elimination order, ready for level-scheduled triangular solves.
 #include <algorithm>
 #include <cmath>
+#include <cstdio>
 #include <cstring>
 #include <numeric>
+#include <string>
 #ifdef _OPENMP
 #include <omp.h>
 #endif
@@ -24,7 +26,7 @@ namespace wx {
 
 // dense_gpu.hip
 bool gpu_available();
-bool dense_laplacian_pinv_gpu(std::vector<double> &S, uint32_t n);
+bool dense_laplacian_pinv_gpu(std::vector<double> &S, uint32_t n, bool floating);
 
 std::shared_ptr<Precond> build_diagonal(const Design &d, const std::vector<double> &w) {
     // crates/within/src/operator/schwarz.rs:185-229
@@ -137,7 +139,9 @@ void sort_and_merge(std::vector<Edge> &e) {
 // Approximate Cholesky of a connected Laplacian given by its off-diagonal edges, as
 // L D L' with L unit-lower in elimination order.  Vertices are eliminated in a seeded random
 // order; each star's clique is replaced by `split` sampled clique trees.  The last vertex
-// has an empty star and pivot 0 (the floating gauge, pseudo-inverted to 0).
+// has an empty star and pivot 0 (the floating gauge, pseudo-inverted to 0).  `last` >= 0
+// forces that vertex to the end of the order: with the ground vertex of a grounded system
+// there, dropping it leaves a factor of the grounded (positive definite) kept block.
 struct AcFactor {
     std::vector<uint32_t> order;    // order[t] = vertex eliminated t-th
     std::vector<uint64_t> rptr;     // L by rows (relabelled), strictly lower
@@ -146,7 +150,7 @@ struct AcFactor {
     std::vector<double> d;          // pivots (relabelled)
 };
 
-AcFactor approx_cholesky(uint32_t n, const std::vector<Edge> &edges, uint64_t seed, int split) {
+AcFactor approx_cholesky(uint32_t n, const std::vector<Edge> &edges, uint64_t seed, int split, int64_t last = -1) {
     AcFactor f;
     std::vector<std::vector<std::pair<uint32_t, double>>> adj(n);
     for (const Edge &e : edges) {
@@ -157,6 +161,10 @@ AcFactor approx_cholesky(uint32_t n, const std::vector<Edge> &edges, uint64_t se
     std::iota(f.order.begin(), f.order.end(), 0u);
     Rng ord(mix_seed(seed, n, 0xAC0));
     for (uint32_t i = n; i > 1; --i) std::swap(f.order[i - 1], f.order[ord.next() % i]);
+    if (last >= 0 && n > 0) { // move `last` to the end, keeping the others' relative order
+        auto it = std::find(f.order.begin(), f.order.end(), (uint32_t)last);
+        std::rotate(it, it + 1, f.order.end());
+    }
     std::vector<uint32_t> pos(n);
     for (uint32_t t = 0; t < n; ++t) pos[f.order[t]] = t;
     std::vector<uint8_t> gone(n, 0);
@@ -209,15 +217,16 @@ AcFactor approx_cholesky(uint32_t n, const std::vector<Edge> &edges, uint64_t se
     return f;
 }
 
-// Dense pseudo-inverse of a connected Laplacian S (n x n row-major, in place):
-// ground the last vertex, invert the SPD minor through Cholesky, pad, and project with
-// P = I - 11'/n on both sides, so the result is the Moore-Penrose inverse S^+.
-bool dense_laplacian_pinv(std::vector<double> &S, uint32_t n) {
-    if (n <= 1) {
+// Dense inverse of a kept block S (n x n row-major, in place).  Floating (a connected
+// Laplacian): ground the last vertex, invert the SPD minor through Cholesky, pad, and project
+// with P = I - 11'/n on both sides, so the result is the Moore-Penrose inverse S^+.
+// Otherwise S is positive definite (grounded or signed) and is inverted whole.
+bool dense_laplacian_pinv(std::vector<double> &S, uint32_t n, bool floating = true) {
+    if (floating && n <= 1) {
         std::fill(S.begin(), S.end(), 0.0);
         return true;
     }
-    const uint32_t m = n - 1;
+    const uint32_t m = floating ? n - 1 : n;
     std::vector<double> A((size_t)m * m);
     for (uint32_t i = 0; i < m; ++i) std::memcpy(&A[(size_t)i * m], &S[(size_t)i * n], m * sizeof(double));
     // Cholesky A = L L' (lower, in place), right-looking with the trailing update in parallel
@@ -257,6 +266,10 @@ bool dense_laplacian_pinv(std::vector<double> &S, uint32_t n) {
             G[(size_t)i * m + j] = acc;
             G[(size_t)j * m + i] = acc;
         }
+    if (!floating) {
+        std::copy(G.begin(), G.end(), S.begin());
+        return true;
+    }
     // pad and project
     std::vector<double> rowmean(n, 0.0);
     double total = 0.0;
@@ -280,23 +293,181 @@ struct Channel {
 };
 
 struct Component {
-    // eliminated-major after orientation
+    // eliminated-major after orientation, in SDDM coordinates
     std::vector<uint32_t> glob;  // ne + nk
-    std::vector<double> diag;    // ne + nk (row sums adopted)
+    std::vector<double> diag;    // ne + nk
     uint32_t ne = 0, nk = 0;
-    // C: ne x nk CSR (local kept index)
+    // C: ne x nk CSR (local kept index); the matrix is [[D_e, -C], [-C', D_k]]
     std::vector<uint64_t> cptr;
     std::vector<uint32_t> cidx;
     std::vector<double> cval;
+    std::vector<double> fold;    // coordinate map: original = fold .* SDDM coordinates
+    std::vector<double> surplus; // ground-edge weight, diag - sum |C| (zero when floating)
+    bool grounded = false;       // surplus retained: no gauge, the kept block is definite
+    bool signed_form = false;    // frustrated: C keeps both signs (solved through a cover)
 };
 
-// Cross-tab of one channel pair and its components (cross_tab.rs, accumulate.rs, factor_pairs.rs:95-152).
+// Magnitude-weighted neighbours of a component node in the bipartite graph: node i < ne is
+// an eliminated level (row i of C), node ne + k the kept level k (column k, rows ascending).
+struct Adjacency {
+    std::vector<uint64_t> tp;
+    std::vector<uint32_t> te;
+    std::vector<double> tv;
+    explicit Adjacency(const Component &cm) : tp(cm.nk + 1, 0), te(cm.cidx.size()), tv(cm.cidx.size()) {
+        for (uint32_t k : cm.cidx) tp[k + 1]++;
+        for (uint32_t k = 0; k < cm.nk; ++k) tp[k + 1] += tp[k];
+        std::vector<uint64_t> cur(tp.begin(), tp.end() - 1);
+        for (uint32_t e = 0; e < cm.ne; ++e)
+            for (uint64_t p = cm.cptr[e]; p < cm.cptr[e + 1]; ++p) {
+                te[cur[cm.cidx[p]]] = e;
+                tv[cur[cm.cidx[p]]] = cm.cval[p];
+                cur[cm.cidx[p]]++;
+            }
+    }
+    template <typename F> void each(const Component &cm, uint32_t i, F &&fn) const {
+        if (i < cm.ne) {
+            for (uint64_t p = cm.cptr[i]; p < cm.cptr[i + 1]; ++p) fn(cm.ne + cm.cidx[p], cm.cval[p]);
+        } else {
+            const uint32_t k = i - cm.ne;
+            for (uint64_t q = tp[k]; q < tp[k + 1]; ++q) fn(te[q], tv[q]);
+        }
+    }
+};
+
+// Conversion of a general (slope-channel) component to SDDM form, following
+// crates/within/src/domain/factor_pairs/sddm.rs:199-463: a folding signature when the signed
+// graph is balanced (no cycle with an odd number of negative entries), else the block sign
+// flip of the frustrated form; the dominance scaling of the diagonal (sweeps that raise mu_i
+// to its neighbours' scaled weight, renormalised by the peak); the folded block
+// -f_i f_j C_ij; and the surplus diag - sum |C| kept as ground-edge weight (grounded), or
+// dropped when it is round-off (floating).  Returns false when no SDDM form exists.
+bool convert_general(Component &cm, const LocalSolverSettings &cfg, int64_t &sweeps_out, double &violation_out) {
+    const uint32_t ne = cm.ne, n = cm.ne + cm.nk;
+    const double tol = cfg.scaling_tolerance;
+    Adjacency adj(cm);
+    // folding signature (sddm.rs:355-380): neighbours take the opposite sign of a positive entry
+    std::vector<double> sg(n, 0.0);
+    bool balanced = true;
+    std::vector<uint32_t> stack;
+    for (uint32_t root = 0; root < n && balanced; ++root) {
+        if (sg[root] != 0.0) continue;
+        sg[root] = 1.0;
+        stack.assign(1, root);
+        while (!stack.empty() && balanced) {
+            const uint32_t i = stack.back();
+            stack.pop_back();
+            adj.each(cm, i, [&](uint32_t j, double v) {
+                if (!balanced) return;
+                const double expected = -sg[i] * (std::signbit(v) ? -1.0 : 1.0);
+                if (sg[j] == 0.0) {
+                    sg[j] = expected;
+                    stack.push_back(j);
+                } else if (sg[j] != expected) {
+                    balanced = false;
+                }
+            });
+        }
+    }
+    // dominance scaling (sddm.rs:389-463)
+    for (double dv : cm.diag)
+        if (!std::isfinite(dv) || !(dv > 0.0)) return false;
+    std::vector<double> scales(n, 1.0);
+    double violation = 0.0;
+    int64_t sweeps = 0;
+    bool dominant = true;
+    for (uint32_t i = 0; i < n && dominant; ++i) {
+        double rs = 0.0;
+        adj.each(cm, i, [&](uint32_t, double v) { rs += std::fabs(v); });
+        dominant = rs <= cm.diag[i] * (1.0 + tol);
+    }
+    if (!dominant) {
+        std::vector<double> isq(n), mu(n, 1.0);
+        for (uint32_t i = 0; i < n; ++i) isq[i] = 1.0 / std::sqrt(cm.diag[i]);
+        auto target = [&](uint32_t i) {
+            double t = 0.0;
+            adj.each(cm, i, [&](uint32_t j, double v) { t += std::fabs(v) * isq[i] * isq[j] * mu[j]; });
+            return t;
+        };
+        auto violation_of = [&]() {
+            double m = 0.0;
+            for (uint32_t i = 0; i < n; ++i) m = std::fmax(m, target(i) / mu[i] - 1.0);
+            return m;
+        };
+        violation = violation_of();
+        while (violation > tol && sweeps < cfg.scaling_max_sweeps) {
+            for (uint32_t i = 0; i < n; ++i) {
+                const double t = target(i);
+                if (t > mu[i]) mu[i] = t;
+            }
+            double peak = 0.0;
+            for (double v : mu) peak = std::fmax(peak, v);
+            if (!std::isfinite(peak)) return false;
+            for (double &v : mu) v /= peak;
+            violation = violation_of();
+            ++sweeps;
+        }
+        for (uint32_t i = 0; i < n; ++i) scales[i] = mu[i] * isq[i];
+        for (double v : scales)
+            if (!std::isfinite(v) || !(v > 0.0)) return false;
+        violation = std::fmax(violation, 0.0);
+    }
+    if (violation > tol && cfg.scaling_error) return false;
+    std::vector<double> f(n);
+    for (uint32_t i = 0; i < n; ++i) f[i] = balanced ? sg[i] * scales[i] : (i < ne ? scales[i] : -scales[i]);
+    // folded block; a balanced component must fold to a Z-matrix (sddm.rs:230-278)
+    for (uint32_t e = 0; e < ne; ++e)
+        for (uint64_t p = cm.cptr[e]; p < cm.cptr[e + 1]; ++p) {
+            const double folded = -f[e] * f[ne + cm.cidx[p]] * cm.cval[p];
+            if (!std::isfinite(folded) || (balanced && folded < 0.0)) return false;
+            cm.cval[p] = folded;
+        }
+    for (uint32_t i = 0; i < n; ++i) cm.diag[i] = cm.diag[i] * f[i] * f[i];
+    // surplus and gauge (sddm.rs:281-336)
+    const Adjacency folded_adj(cm);
+    std::vector<double> rsum(n, 0.0), sur(n, 0.0);
+    for (uint32_t i = 0; i < n; ++i) folded_adj.each(cm, i, [&](uint32_t, double v) { rsum[i] += std::fabs(v); });
+    double total_d = 0.0, total_s = 0.0, clamped = 0.0;
+    for (uint32_t i = 0; i < n; ++i) {
+        double &dg = cm.diag[i];
+        if (!std::isfinite(dg) || !(dg > 0.0)) return false;
+        const double deficit = (rsum[i] - dg) / dg;
+        if (deficit > tol) {
+            if (cfg.scaling_error) return false;
+            clamped = std::fmax(clamped, deficit);
+        }
+        dg = std::fmax(dg, rsum[i]);
+        sur[i] = std::fmax(dg - rsum[i], 0.0);
+        total_d += dg;
+        total_s += sur[i];
+    }
+    const bool floats = total_s <= 4.0 * 2.220446049250313e-16 * std::sqrt((double)std::max<uint32_t>(n, 1)) * total_d;
+    if (floats) {
+        cm.diag = rsum;
+        std::fill(sur.begin(), sur.end(), 0.0);
+    }
+    cm.fold = std::move(f);
+    cm.surplus = std::move(sur);
+    cm.grounded = !floats;
+    cm.signed_form = !balanced;
+    sweeps_out = sweeps;
+    violation_out = std::fmax(violation, clamped);
+    return true;
+}
+
+std::string pair_name(Channel a, Channel b) {
+    return "term " + std::to_string(a.term) + " column " + std::to_string(a.column) + " and term " +
+           std::to_string(b.term) + " column " + std::to_string(b.column);
+}
+
+// Cross-tab of one channel pair and its components (cross_tab.rs, accumulate.rs, factor_pairs.rs:95-152),
+// each converted to SDDM form (sddm.rs).  Runs inside a parallel loop, so a failure is returned
+// as a message in `error` rather than thrown.
 void build_pair_components(const Design &d, const std::vector<double> &w, Channel rc, Channel cc,
-                           const std::vector<std::vector<uint8_t>> &active, std::vector<Component> &out,
-                           bool &general) {
+                           const std::vector<std::vector<uint8_t>> &active, const LocalSolverSettings &cfg,
+                           std::vector<Component> &out, std::vector<std::string> &warnings, std::string &error) {
     const Term &tr = d.terms[rc.term], &tc = d.terms[cc.term];
     const int64_t lr = tr.loads[rc.column], lc = tc.loads[cc.column];
-    general = lr >= 0 || lc >= 0;
+    const bool general = lr >= 0 || lc >= 0;
     // compact active levels
     std::vector<uint32_t> rmap(tr.n_levels, UINT32_MAX), cmap(tc.n_levels, UINT32_MAX);
     uint32_t nr = 0, nc = 0;
@@ -473,18 +644,138 @@ void build_pair_components(const Design &d, const std::vector<double> &w, Channe
             total_diag += dg;
             mismatch += std::fabs(dg - rowsum_k[ki]);
         }
+        for (uint32_t k : K) (swap ? rremap[k] : cremap[k]) = UINT32_MAX;
         if (!general) {
+            // plain pair: validate the Laplacian claim and adopt the row sums (sddm.rs:173-197);
+            // the coordinate map negates the kept block
             const double tol = 64.0 * 2.220446049250313e-16 * std::sqrt((double)std::max<size_t>(1, E.size() + K.size())) * total_diag;
-            if (mismatch > tol) fail(WX_ERR_BUILD, "factor-pair component failed Laplacian validation");
+            if (mismatch > tol) {
+                error = "signed component between " + pair_name(rc, cc) + " is not diagonally scalable to SDDM form";
+                return;
+            }
             comp.diag = rowsum_e;
             comp.diag.insert(comp.diag.end(), rowsum_k.begin(), rowsum_k.end());
+            comp.fold.assign(E.size() + K.size(), 1.0);
+            std::fill(comp.fold.begin() + E.size(), comp.fold.end(), -1.0);
+            comp.surplus.assign(E.size() + K.size(), 0.0);
         } else {
             for (size_t ei = 0; ei < E.size(); ++ei) comp.diag.push_back(swap ? cdiag[E[ei]] : rdiag[E[ei]]);
             for (size_t ki = 0; ki < K.size(); ++ki) comp.diag.push_back(swap ? rdiag[K[ki]] : cdiag[K[ki]]);
+            int64_t sweeps = 0;
+            double violation = 0.0;
+            if (!convert_general(comp, cfg, sweeps, violation)) {
+                error = "signed component between " + pair_name(rc, cc) + " is not diagonally scalable to SDDM form";
+                return;
+            }
+            if (violation > cfg.scaling_tolerance) {
+                char buf[64];
+                std::snprintf(buf, sizeof buf, "%.2e", violation);
+                warnings.push_back("signed component between " + pair_name(rc, cc) +
+                                   ": dominance scaling uncertified after " + std::to_string(sweeps) +
+                                   " sweeps (max relative violation " + buf +
+                                   "); deficits clamped, preconditioner quality may degrade");
+            }
         }
-        for (uint32_t k : K) (swap ? rremap[k] : cremap[k]) = UINT32_MAX;
         out.push_back(std::move(comp));
     }
+}
+
+// Dense reduced kept block of a component (row-major nk x nk, rows in parallel): the exact
+// Schur complement's off-diagonals -sum_e (C_ei C_ej) / D_e (schur.rs:96-113) and its diagonal
+// rebuilt from the adjacency plus the reduced surplus (schur.rs:197-268), so a floating
+// Laplacian's rows sum to zero exactly.  For a signed component this is the single-sized
+// operator whose action on antisymmetric vectors equals the Gremban cover's complement; its
+// diagonal is the cover row's adjacency, sum_e |C_ei| (sum_j |C_ej| - |C_ei|) / D_e.
+void dense_schur(const Component &cm, std::vector<double> &Sm, bool parallel) {
+    const uint32_t ne = cm.ne, nk = cm.nk;
+    const Adjacency adj(cm);
+    std::vector<double> rowabs(ne, 0.0);
+    for (uint32_t e = 0; e < ne; ++e)
+        for (uint64_t p = cm.cptr[e]; p < cm.cptr[e + 1]; ++p) rowabs[e] += std::fabs(cm.cval[p]);
+    Sm.assign((size_t)nk * nk, 0.0);
+#pragma omp parallel for schedule(dynamic, 8) if (parallel)
+    for (int64_t i = 0; i < (int64_t)nk; ++i) {
+        double *row = &Sm[(size_t)i * nk];
+        double surplus = cm.grounded ? cm.surplus[ne + i] : 0.0, mag = 0.0;
+        for (uint64_t q = adj.tp[i]; q < adj.tp[i + 1]; ++q) {
+            const uint32_t e = adj.te[q];
+            const double wi = adj.tv[q], inv = 1.0 / cm.diag[e];
+            for (uint64_t p = cm.cptr[e]; p < cm.cptr[e + 1]; ++p) row[cm.cidx[p]] -= (wi * cm.cval[p]) * inv;
+            if (cm.grounded) surplus += std::fabs(wi) * (inv * cm.surplus[e]);
+            if (cm.signed_form) mag += std::fabs(wi) * (rowabs[e] - std::fabs(wi)) * inv;
+        }
+        double adjacency = 0.0;
+        if (cm.signed_form) adjacency = mag;
+        else
+            for (uint32_t j = 0; j < nk; ++j)
+                if (j != (uint32_t)i) adjacency -= row[j];
+        row[i] = adjacency + surplus;
+    }
+}
+
+// Schur-complement edges of a Laplacian-form component over its kept vertices 0..nk-1 plus,
+// when grounded, the ground vertex nk (schur.rs:64-68, 337-372): per eliminated star the
+// sampled clique trees (or the exact clique), a grounded star's surplus joining it as one
+// more neighbour (its capacity is then its diagonal), and the kept surplus as ground edges.
+void schur_edges(const Component &cm, const LocalSolverSettings &cfg, uint64_t stream, std::vector<Edge> &edges) {
+    const uint32_t ne = cm.ne, nk = cm.nk, ground = nk;
+    std::vector<std::pair<uint32_t, double>> nb;
+    for (uint32_t e = 0; e < ne; ++e) {
+        nb.clear();
+        for (uint64_t p = cm.cptr[e]; p < cm.cptr[e + 1]; ++p) nb.push_back({cm.cidx[p], cm.cval[p]});
+        if (cm.grounded && cm.surplus[e] > 0.0) nb.push_back({ground, cm.surplus[e]});
+        if (nb.size() < 2) continue;
+        const double De = cm.diag[e];
+        if (cfg.schur_exact) {
+            for (size_t p = 0; p < nb.size(); ++p)
+                for (size_t q = p + 1; q < nb.size(); ++q)
+                    edges.push_back({std::min(nb[p].first, nb[q].first), std::max(nb[p].first, nb[q].first),
+                                     (nb[p].second * nb[q].second) / De});
+        } else {
+            Rng rng(mix_seed(cfg.schur_seed, e + (stream << 32), 0x5C4));
+            sample_clique_tree(nb, De, std::max(1, cfg.schur_split), rng, edges);
+        }
+    }
+    if (cm.grounded)
+        for (uint32_t k = 0; k < nk; ++k)
+            if (cm.surplus[ne + k] > 0.0) edges.push_back({k, ground, cm.surplus[ne + k]});
+    sort_and_merge(edges);
+}
+
+// Gremban cover of a signed component (block_elim/solver.rs:221-277): two sheets of every
+// vertex; a positive entry joins equal sheets and a negative one opposite sheets, each with
+// its magnitude.  The cover is a Laplacian-form SDDM of twice the size (surplus and gauge
+// carried over) acting on antisymmetric vectors [z, -z] as the signed matrix does.
+Component make_cover(const Component &cm) {
+    Component cv;
+    const uint32_t ne = cm.ne, nk = cm.nk;
+    cv.ne = 2 * ne;
+    cv.nk = 2 * nk;
+    cv.grounded = cm.grounded;
+    cv.cptr.push_back(0);
+    for (int copy = 0; copy < 2; ++copy)
+        for (uint32_t e = 0; e < ne; ++e) {
+            for (int shifted = 0; shifted < 2; ++shifted) {
+                const bool negative = shifted != copy;
+                for (uint64_t p = cm.cptr[e]; p < cm.cptr[e + 1]; ++p) {
+                    if ((cm.cval[p] < 0.0) != negative) continue;
+                    cv.cidx.push_back(cm.cidx[p] + (shifted ? nk : 0));
+                    cv.cval.push_back(std::fabs(cm.cval[p]));
+                }
+            }
+            cv.cptr.push_back(cv.cidx.size());
+        }
+    auto doubled = [&](const std::vector<double> &v) {
+        std::vector<double> o;
+        o.insert(o.end(), v.begin(), v.begin() + ne);
+        o.insert(o.end(), v.begin(), v.begin() + ne);
+        o.insert(o.end(), v.begin() + ne, v.end());
+        o.insert(o.end(), v.begin() + ne, v.end());
+        return o;
+    };
+    cv.diag = doubled(cm.diag);
+    cv.surplus = doubled(cm.surplus);
+    return cv;
 }
 
 } // namespace
@@ -503,20 +794,19 @@ std::shared_ptr<Precond> build_additive(const Design &d, const std::vector<doubl
     for (auto &c : channels) any_slope |= d.terms[c.term].loads[c.column] >= 0;
     const auto active = find_active_levels(d);
     std::vector<std::vector<Component>> per_pair(pairs.size());
-    std::vector<uint8_t> pair_general(pairs.size(), 0);
+    std::vector<std::vector<std::string>> pair_warn(pairs.size());
+    std::vector<std::string> pair_err(pairs.size());
 #pragma omp parallel for schedule(dynamic, 1)
-    for (int64_t p = 0; p < (int64_t)pairs.size(); ++p) {
-        bool general = false;
-        build_pair_components(d, w, pairs[p].first, pairs[p].second, active, per_pair[p], general);
-        pair_general[p] = general;
+    for (int64_t p = 0; p < (int64_t)pairs.size(); ++p)
+        build_pair_components(d, w, pairs[p].first, pairs[p].second, active, cfg, per_pair[p], pair_warn[p],
+                              pair_err[p]);
+    for (size_t p = 0; p < pairs.size(); ++p) {
+        if (!pair_err[p].empty()) fail(WX_ERR_BUILD, pair_err[p]);
+        warnings.insert(warnings.end(), pair_warn[p].begin(), pair_warn[p].end());
     }
     std::vector<Component> comps;
     for (auto &v : per_pair)
         for (auto &c : v) comps.push_back(std::move(c));
-    for (size_t p = 0; p < pairs.size(); ++p)
-        if (pair_general[p] && !per_pair[p].empty())
-            fail(WX_ERR_BUILD, "additive Schwarz over slope (signed) channel pairs is not supported by this build; "
-                               "use PreconditionerConfig.Diagonal for varying-slope designs");
     if (comps.empty()) return nullptr;
 
     auto pre = std::make_shared<Precond>();
@@ -533,24 +823,29 @@ std::shared_ptr<Precond> build_additive(const Design &d, const std::vector<doubl
     S->ne.resize(nc);
     S->nk.resize(nc);
     S->fac_kind.resize(nc);
+    S->gauge.resize(nc);
     S->dense_off.assign(nc, 0);
     S->l_row_off.assign(nc, 0);
     S->piv_off.assign(nc, 0);
+    S->cover_off.assign(nc, 0);
     for (uint64_t c = 0; c < nc; ++c) {
         S->ne[c] = comps[c].ne;
         S->nk[c] = comps[c].nk;
         S->slot_start[c + 1] = S->slot_start[c] + comps[c].ne + comps[c].nk;
+        S->gauge[c] = (comps[c].grounded || comps[c].signed_form) ? 1 : 0;
     }
     const uint64_t ns = S->slot_start[nc];
     S->glob.resize(ns);
     S->pw.resize(ns);
+    S->fold.resize(ns);
     S->inv_de.assign(ns, 0.0);
     S->c_ptr.assign(ns + 1, 0);
     // factor choice per component.  Device cost model (dense_threshold < 0): small kept blocks
-    // and dense-ish reduced graphs get the exact dense pseudo-inverse -- one parallel
-    // symmetric product per apply -- while large sparse reduced graphs get approximate
-    // Cholesky, whose triangular solves run level by level.  The density estimate is the
-    // sampled Schur complement's edge count sum_e (deg(e) - 1) over nk^2 / 2.
+    // and dense-ish reduced graphs get the exact dense inverse -- one parallel symmetric
+    // product per apply -- while large sparse reduced graphs get approximate Cholesky (of the
+    // Gremban cover for a signed block), whose triangular solves run level by level.  The
+    // density estimate is the sampled Schur complement's edge count sum_e (deg(e) - 1) over
+    // nk^2 / 2.
     const bool have_gpu = gpu_available();
     std::vector<uint8_t> dense(nc), big(nc, 0);
     for (uint64_t c = 0; c < nc; ++c) {
@@ -566,7 +861,7 @@ std::shared_ptr<Precond> build_additive(const Design &d, const std::vector<doubl
             dense[c] = 2.0 * m / ((double)nk * nk) >= 0.005;
         } else dense[c] = 0;
         big[c] = dense[c] && nk > 1024 && have_gpu;
-        S->fac_kind[c] = dense[c] ? 0 : 1;
+        S->fac_kind[c] = dense[c] ? 0 : (comps[c].signed_form ? 2 : 1);
     }
     // per-component factorisation (parallel)
     std::vector<std::vector<double>> dense_mats(nc);
@@ -575,124 +870,89 @@ std::shared_ptr<Precond> build_additive(const Design &d, const std::vector<doubl
 #pragma omp parallel for schedule(dynamic, 1)
     for (int64_t c = 0; c < (int64_t)nc; ++c) {
         const Component &cm = comps[c];
-        const uint32_t ne = cm.ne, nk = cm.nk;
-        for (uint32_t e = 0; e < ne; ++e)
+        for (uint32_t e = 0; e < cm.ne; ++e)
             if (!(cm.diag[e] > 0.0)) failed[c] = 1;
         if (failed[c] || big[c]) continue;
+        const bool floating = S->gauge[c] == 0;
         if (dense[c]) {
-            // exact Schur complement S = D_k - C' D_e^{-1} C (schur.rs:29-61)
-            std::vector<double> Sm((size_t)nk * nk, 0.0);
-            for (uint32_t k = 0; k < nk; ++k) Sm[(size_t)k * nk + k] = cm.diag[ne + k];
-            for (uint32_t e = 0; e < ne; ++e) {
-                const double inv = 1.0 / cm.diag[e];
-                for (uint64_t p = cm.cptr[e]; p < cm.cptr[e + 1]; ++p) {
-                    const uint32_t i = cm.cidx[p];
-                    const double wi = cm.cval[p];
-                    for (uint64_t q = cm.cptr[e]; q < cm.cptr[e + 1]; ++q)
-                        Sm[(size_t)i * nk + cm.cidx[q]] -= (wi * cm.cval[q]) * inv;
-                }
-            }
-            if (!dense_laplacian_pinv(Sm, nk)) failed[c] = 2;
+            std::vector<double> Sm;
+            dense_schur(cm, Sm, false);
+            if (!dense_laplacian_pinv(Sm, cm.nk, floating)) failed[c] = 2;
             dense_mats[c] = std::move(Sm);
         } else {
-            // Schur complement edges: sampled clique trees per eliminated star, or exact
+            // Laplacian-form system to factor: the component itself, or its Gremban cover
+            const Component cover = cm.signed_form ? make_cover(cm) : Component{};
+            const Component &lf = cm.signed_form ? cover : cm;
             std::vector<Edge> edges;
-            for (uint32_t e = 0; e < ne; ++e) {
-                const uint64_t b = cm.cptr[e], en = cm.cptr[e + 1];
-                if (en - b < 2) continue;
-                const double De = cm.diag[e];
-                if (cfg.schur_exact) {
-                    for (uint64_t p = b; p < en; ++p)
-                        for (uint64_t q = p + 1; q < en; ++q)
-                            edges.push_back({std::min(cm.cidx[p], cm.cidx[q]), std::max(cm.cidx[p], cm.cidx[q]),
-                                             (cm.cval[p] * cm.cval[q]) / De});
-                } else {
-                    std::vector<std::pair<uint32_t, double>> nb;
-                    for (uint64_t p = b; p < en; ++p) nb.push_back({cm.cidx[p], cm.cval[p]});
-                    Rng rng(mix_seed(cfg.schur_seed, e + ((uint64_t)c << 32), 0x5C4));
-                    sample_clique_tree(nb, De, std::max(1, cfg.schur_split), rng, edges);
-                }
+            schur_edges(lf, cfg, (uint64_t)c, edges);
+            const uint32_t n_ac = lf.nk + (lf.grounded ? 1 : 0);
+            acs[c] = approx_cholesky(n_ac, edges, mix_seed(cfg.approx_chol_seed, c, 0xC401),
+                                     cfg.split_merge > 0 ? cfg.split_merge : 1, lf.grounded ? (int64_t)lf.nk : -1);
+            if (lf.grounded) { // drop the ground vertex, eliminated last
+                AcFactor &f = acs[c];
+                f.order.resize(lf.nk);
+                f.d.resize(lf.nk);
+                f.rptr.resize(lf.nk + 1);
+                f.ridx.resize(f.rptr[lf.nk]);
+                f.rval.resize(f.rptr[lf.nk]);
             }
-            sort_and_merge(edges);
-            acs[c] = approx_cholesky(nk, edges, mix_seed(cfg.approx_chol_seed, c, 0xC401),
-                                     cfg.split_merge > 0 ? cfg.split_merge : 1);
         }
     }
-    // large dense kept blocks: exact Schur complement accumulated row-parallel on the host
-    // (each thread owns kept rows, entries ordered (w*v)*inv as in schur.rs:112-113), then the
-    // pseudo-inverse on the GPU
+    // large dense kept blocks: the reduced block accumulated row-parallel on the host, then its
+    // inverse on the GPU
     for (uint64_t c = 0; c < nc; ++c) {
         if (!big[c] || failed[c]) continue;
-        const Component &cm = comps[c];
-        const uint32_t ne = cm.ne, nk = cm.nk;
-        std::vector<uint64_t> tp(nk + 1, 0);
-        for (uint32_t k : cm.cidx) tp[k + 1]++;
-        for (uint32_t k = 0; k < nk; ++k) tp[k + 1] += tp[k];
-        std::vector<uint32_t> te(cm.cidx.size());
-        std::vector<double> tv(cm.cidx.size());
-        {
-            std::vector<uint64_t> cur(tp.begin(), tp.end() - 1);
-            for (uint32_t e = 0; e < ne; ++e)
-                for (uint64_t p = cm.cptr[e]; p < cm.cptr[e + 1]; ++p) {
-                    te[cur[cm.cidx[p]]] = e;
-                    tv[cur[cm.cidx[p]]] = cm.cval[p];
-                    cur[cm.cidx[p]]++;
-                }
-        }
-        std::vector<double> Sm((size_t)nk * nk, 0.0);
-#pragma omp parallel for schedule(dynamic, 8)
-        for (int64_t i = 0; i < (int64_t)nk; ++i) {
-            double *row = &Sm[(size_t)i * nk];
-            row[i] = cm.diag[ne + i];
-            for (uint64_t q = tp[i]; q < tp[i + 1]; ++q) {
-                const uint32_t e = te[q];
-                const double wi = tv[q], inv = 1.0 / cm.diag[e];
-                for (uint64_t p = cm.cptr[e]; p < cm.cptr[e + 1]; ++p) row[cm.cidx[p]] -= (wi * cm.cval[p]) * inv;
-            }
-        }
-        if (!dense_laplacian_pinv_gpu(Sm, nk)) failed[c] = 2;
+        std::vector<double> Sm;
+        dense_schur(comps[c], Sm, true);
+        if (!dense_laplacian_pinv_gpu(Sm, comps[c].nk, S->gauge[c] == 0)) failed[c] = 2;
         dense_mats[c] = std::move(Sm);
     }
     for (uint64_t c = 0; c < nc; ++c) {
         if (failed[c] == 1) fail(WX_ERR_BUILD, "zero diagonal in preconditioner at index " + std::to_string(c));
         if (failed[c] == 2) fail(WX_ERR_BUILD, "local solver build failed: Schur complement is not positive definite");
     }
-    // assemble slots (kept block of sparse components relabelled into elimination order)
-    uint64_t dense_total = 0, l_total = 0, piv_total = 0;
+    // assemble slots (kept block of sparse components relabelled into elimination order;
+    // cover components keep their kept slots and carry the cover's elimination order)
+    uint64_t dense_total = 0, l_total = 0, piv_total = 0, rows_total = 0, cover_total = 0;
     for (uint64_t c = 0; c < nc; ++c) {
-        if (dense[c]) dense_total += (uint64_t)comps[c].nk * comps[c].nk;
-        else {
+        if (dense[c]) {
+            dense_total += (uint64_t)comps[c].nk * comps[c].nk;
+        } else {
+            const uint64_t nf = acs[c].d.size();
             l_total += acs[c].ridx.size();
-            piv_total += comps[c].nk;
+            piv_total += nf;
+            rows_total += nf + 1;
+            if (S->fac_kind[c] == 2) cover_total += nf;
         }
     }
     S->dense.resize(dense_total);
     S->l_idx.resize(l_total);
     S->l_val.resize(l_total);
     S->piv.resize(piv_total);
-    uint64_t doff = 0, loff = 0, poff = 0, rows_total = 0;
-    for (uint64_t c = 0; c < nc; ++c)
-        if (!dense[c]) rows_total += comps[c].nk + 1;
     S->l_ptr.resize(rows_total);
-    uint64_t rptr_off = 0;
+    S->cover_perm.resize(cover_total);
     uint64_t cnnz = 0;
     for (auto &cm : comps) cnnz += cm.cidx.size();
     S->c_idx.resize(cnnz);
     S->c_val.resize(cnnz);
-    uint64_t cpos = 0;
+    uint64_t doff = 0, loff = 0, poff = 0, rptr_off = 0, coff = 0, cpos = 0;
     for (uint64_t c = 0; c < nc; ++c) {
         const Component &cm = comps[c];
         const uint64_t base = S->slot_start[c];
         const uint32_t ne = cm.ne, nk = cm.nk;
         std::vector<uint32_t> relabel(nk); // old kept index -> new
-        if (dense[c]) std::iota(relabel.begin(), relabel.end(), 0u);
-        else
+        if (S->fac_kind[c] == 1)
             for (uint32_t t = 0; t < nk; ++t) relabel[acs[c].order[t]] = t;
+        else std::iota(relabel.begin(), relabel.end(), 0u);
         for (uint32_t e = 0; e < ne; ++e) {
             S->glob[base + e] = cm.glob[e];
+            S->fold[base + e] = cm.fold[e];
             S->inv_de[base + e] = 1.0 / cm.diag[e];
         }
-        for (uint32_t k = 0; k < nk; ++k) S->glob[base + ne + relabel[k]] = cm.glob[ne + k];
+        for (uint32_t k = 0; k < nk; ++k) {
+            S->glob[base + ne + relabel[k]] = cm.glob[ne + k];
+            S->fold[base + ne + relabel[k]] = cm.fold[ne + k];
+        }
         for (uint32_t s = 0; s < ne + nk; ++s) {
             const uint32_t g = S->glob[base + s];
             S->pw[base + s] = (!any_slope && counts[g] > 1) ? 1.0 / std::sqrt((double)counts[g]) : 1.0;
@@ -721,20 +981,25 @@ std::shared_ptr<Precond> build_additive(const Design &d, const std::vector<doubl
             std::vector<double>().swap(dense_mats[c]);
         } else {
             const AcFactor &f = acs[c];
+            const uint64_t nf = f.d.size();
             S->l_row_off[c] = rptr_off;
             S->piv_off[c] = poff;
-            for (uint32_t i = 0; i <= nk; ++i) S->l_ptr[rptr_off + i] = loff + f.rptr[i];
+            for (uint64_t i = 0; i <= nf; ++i) S->l_ptr[rptr_off + i] = loff + f.rptr[i];
             std::copy(f.ridx.begin(), f.ridx.end(), S->l_idx.begin() + loff);
             std::copy(f.rval.begin(), f.rval.end(), S->l_val.begin() + loff);
             std::copy(f.d.begin(), f.d.end(), S->piv.begin() + poff);
             loff += f.ridx.size();
-            rptr_off += nk + 1;
-            poff += nk;
+            rptr_off += nf + 1;
+            poff += nf;
+            if (S->fac_kind[c] == 2) {
+                S->cover_off[c] = coff;
+                std::copy(f.order.begin(), f.order.end(), S->cover_perm.begin() + coff);
+                coff += nf;
+            }
         }
     }
     S->c_ptr[ns] = cpos;
     pre->schwarz = S;
-    (void)warnings;
     return pre;
 }
 

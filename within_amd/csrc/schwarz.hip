@@ -74,6 +74,9 @@ struct SchwarzDev {
     DevBuf sp_comp, sp_lvl_f_ptr, sp_lvl_b_ptr, sp_lvl_f_rows, sp_lvl_b_rows, sp_meta;
     DevBuf l_ptr, l_idx, l_val, lt_ptr, lt_idx, lt_val, piv;
     uint32_t n_sparse = 0;
+    // auxiliary slots [ns, ns + n_aux) of the covers: real kept slot | sheet bit (second sheet)
+    DevBuf aux_src;
+    uint64_t n_aux = 0;
     // prolongation: DOF -> slots
     DevBuf d_ptr, d_slot;
     bool has_sparse = false;
@@ -82,11 +85,11 @@ struct SchwarzDev {
 // per-sparse-component metadata
 struct SpMeta {
     uint32_t comp;
-    uint32_t kept_base; // first kept slot
-    uint32_t nk;
+    uint32_t kept_base; // first factor row's slot: a kept slot, or an auxiliary slot of a cover
+    uint32_t nk;        // factor rows (twice the kept block for a cover)
     uint32_t nlev_f;
     uint32_t nlev_b;
-    uint32_t pad;
+    uint32_t cover;     // 1: the factor is a Gremban cover's, on auxiliary slots
     uint64_t lvl_f_off; // into lvl_f_ptr
     uint64_t lvl_b_off;
     uint64_t row_off;   // first row pointer of the component in l_ptr / lt_ptr
@@ -364,7 +367,8 @@ __global__ __launch_bounds__(BLK) void sk_sparse(const SpMeta *__restrict__ meta
                                                  const CT *__restrict__ l_val, const uint64_t *__restrict__ lt_ptr,
                                                  const uint32_t *__restrict__ lt_idx, const CT *__restrict__ lt_val,
                                                  const CT *__restrict__ piv, const double *__restrict__ red,
-                                                 const double *__restrict__ mu2, double *xs, uint32_t k,
+                                                 const double *__restrict__ mu2, const uint32_t *__restrict__ aux_src,
+                                                 uint64_t ns, double *xs, uint32_t k,
                                                  uint32_t kshift, const int32_t *__restrict__ gate) {
     if (gate_closed(gate, k)) return;
     const uint32_t kp = 1u << kshift, j = threadIdx.x & (kp - 1);
@@ -379,7 +383,14 @@ __global__ __launch_bounds__(BLK) void sk_sparse(const SpMeta *__restrict__ meta
             for (uint64_t q = b + lane_row; q < e; q += rows_per_pass) {
                 const uint32_t row = lvl_f_rows[q];
                 const uint64_t lr = (uint64_t)(row - m.kept_base) + m.row_off;
-                double acc = red[(uint64_t)row * k + j] - mu;
+                double acc;
+                if (m.cover) { // antisymmetric embed [b, -b] of the kept block's reduced rhs
+                    const uint32_t a = aux_src[row - ns];
+                    const double b = red[(uint64_t)(a & 0x7FFFFFFFu) * k + j];
+                    acc = (a >> 31) ? -b : b;
+                } else {
+                    acc = red[(uint64_t)row * k + j] - mu;
+                }
                 for (uint64_t p = l_ptr[lr]; p < l_ptr[lr + 1]; ++p) acc -= (double)l_val[p] * xs[(uint64_t)l_idx[p] * k + j];
                 xs[(uint64_t)row * k + j] = acc;
             }
@@ -406,6 +417,19 @@ __global__ __launch_bounds__(BLK) void sk_sparse(const SpMeta *__restrict__ meta
                 xs[(uint64_t)row * k + j] = acc;
             }
         __syncthreads();
+    }
+    if (m.cover) { // read back x = (y+ - y-) / 2 into the real kept slots, one sheet at a time
+        for (int sheet = 0; sheet < 2; ++sheet) {
+            if (on)
+                for (uint32_t r = lane_row; r < m.nk; r += rows_per_pass) {
+                    const uint32_t a = aux_src[m.kept_base + r - ns];
+                    if ((a >> 31) != (uint32_t)sheet) continue;
+                    const double y = 0.5 * xs[(uint64_t)(m.kept_base + r) * k + j];
+                    double &dst = xs[(uint64_t)(a & 0x7FFFFFFFu) * k + j];
+                    dst = sheet ? dst - y : y;
+                }
+            __syncthreads();
+        }
     }
 }
 
@@ -476,8 +500,7 @@ std::shared_ptr<SchwarzDev<CT>> build_dev(const Precond::Schwarz &S) {
     for (uint64_t c = 0; c < nc; ++c)
         for (uint64_t s = S.slot_start[c]; s < S.slot_start[c + 1]; ++s) {
             slot_comp[s] = (uint32_t)c;
-            const bool kept = s >= S.slot_start[c] + S.ne[c];
-            sgnpw[s] = (CT)(kept ? -S.pw[s] : S.pw[s]);
+            sgnpw[s] = (CT)(S.fold[s] * S.pw[s]);
         }
     D->glob.upload(S.glob.data(), ns);
     D->sgnpw.upload(sgnpw.data(), ns);
@@ -561,12 +584,14 @@ std::shared_ptr<SchwarzDev<CT>> build_dev(const Precond::Schwarz &S) {
         const uint32_t b = (uint32_t)S.slot_start[c], e = (uint32_t)S.slot_start[c + 1];
         for (uint32_t s = b; s < e; s += MEAN_CHUNK) all.push_back({(uint32_t)c, s, std::min(e, s + MEAN_CHUNK), 0});
         all_cptr[c + 1] = (uint32_t)all.size();
-        cnt[c] = e - b;
+        // a count of 0 makes the mean 0: only floating components are gauged by their means
+        const bool floating = S.gauge[c] == 0;
+        cnt[c] = floating ? e - b : 0;
         const uint32_t kb = b + S.ne[c];
-        if (S.fac_kind[c] == 1)
+        if (S.fac_kind[c] == 1 && floating)
             for (uint32_t s = kb; s < e; s += MEAN_CHUNK) kept.push_back({(uint32_t)c, s, std::min(e, s + MEAN_CHUNK), 0});
         kept_cptr[c + 1] = (uint32_t)kept.size();
-        kcnt[c] = e - kb;
+        kcnt[c] = floating ? e - kb : 0;
     }
     D->all_chunks.upload(all.data(), all.size());
     D->n_all_chunks = (uint32_t)all.size();
@@ -641,7 +666,7 @@ std::shared_ptr<SchwarzDev<CT>> build_dev(const Precond::Schwarz &S) {
     }
     D->d_ptr.upload(d_ptr.data(), d_ptr.size());
     D->d_slot.upload(d_slot.data(), d_slot.size());
-    for (uint64_t c = 0; c < nc; ++c) D->has_sparse |= S.fac_kind[c] == 1;
+    for (uint64_t c = 0; c < nc; ++c) D->has_sparse |= S.fac_kind[c] == 1 || S.fac_kind[c] == 2;
     if (D->has_sparse) {
         // level schedules of L (forward) and L' (backward) per sparse component
         std::vector<SpMeta> metas;
@@ -649,15 +674,26 @@ std::shared_ptr<SchwarzDev<CT>> build_dev(const Precond::Schwarz &S) {
         std::vector<uint32_t> lidx, ltidx, lvf_rows, lvb_rows;
         std::vector<CT> lval, ltval, piv;
         uint64_t max_depth = 0, total_rows = 0;
+        std::vector<uint32_t> aux_src;
         for (uint64_t c = 0; c < nc; ++c) {
-            if (S.fac_kind[c] != 1) continue;
-            const uint32_t nk = S.nk[c];
-            const uint32_t kb = (uint32_t)(S.slot_start[c] + S.ne[c]);
+            if (S.fac_kind[c] != 1 && S.fac_kind[c] != 2) continue;
+            const bool cover = S.fac_kind[c] == 2;
+            // factor rows: the kept slots (in elimination order), or auxiliary slots holding
+            // the cover's vertices in its elimination order
+            const uint32_t nk = cover ? 2 * S.nk[c] : S.nk[c];
+            const uint32_t real_kb = (uint32_t)(S.slot_start[c] + S.ne[c]);
+            const uint32_t kb = cover ? (uint32_t)(ns + aux_src.size()) : real_kb;
+            if (cover)
+                for (uint32_t t = 0; t < nk; ++t) {
+                    const uint32_t v = S.cover_perm[S.cover_off[c] + t];
+                    aux_src.push_back(v < S.nk[c] ? real_kb + v : ((real_kb + v - S.nk[c]) | 0x80000000u));
+                }
             const uint64_t ro = S.l_row_off[c];
             SpMeta m{};
             m.comp = (uint32_t)c;
             m.kept_base = kb;
             m.nk = nk;
+            m.cover = cover ? 1u : 0u;
             m.row_off = lptr.size();
             m.piv_off = piv.size();
             // L rows
@@ -726,6 +762,9 @@ std::shared_ptr<SchwarzDev<CT>> build_dev(const Precond::Schwarz &S) {
         }
         D->n_sparse = (uint32_t)metas.size();
         D->sp_meta.upload(metas.data(), metas.size());
+        D->n_aux = aux_src.size();
+        if (ns + D->n_aux >= 0x7FFFFFFFull) fail(WX_ERR_INVALID_INPUT, "too many Schwarz slots for the device path");
+        D->aux_src.upload(aux_src.data(), aux_src.size());
         D->l_ptr.upload(lptr.data(), lptr.size());
         D->l_idx.upload(lidx.data(), lidx.size());
         D->l_val.upload(lval.data(), lval.size());
@@ -756,7 +795,7 @@ struct SchwarzRunImpl : SchwarzRun {
         const uint64_t ns = D->ns;
         loc.alloc(ns * k * 8);
         red.alloc(ns * k * 8);
-        xs.alloc(ns * k * 8);
+        xs.alloc((ns + D->n_aux) * k * 8);
         part.alloc((uint64_t)std::max(D->n_all_chunks, D->n_kept_chunks) * k * 8 + 8);
         rowpart.alloc((uint64_t)std::max(D->n_c_part, D->n_ct_part) * k * 8 + 8);
         mu1.alloc(D->nc * k * 8);
@@ -797,7 +836,8 @@ struct SchwarzRunImpl : SchwarzRun {
                                dp<uint64_t>(d.sp_lvl_f_ptr), dp<uint32_t>(d.sp_lvl_f_rows),
                                dp<uint64_t>(d.sp_lvl_b_ptr), dp<uint32_t>(d.sp_lvl_b_rows), dp<uint64_t>(d.l_ptr),
                                dp<uint32_t>(d.l_idx), dp<CT>(d.l_val), dp<uint64_t>(d.lt_ptr), dp<uint32_t>(d.lt_idx),
-                               dp<CT>(d.lt_val), dp<CT>(d.piv), redp, dp<double>(mu2), xsp, k, kshift, gate);
+                               dp<CT>(d.lt_val), dp<CT>(d.piv), redp, dp<double>(mu2), dp<uint32_t>(d.aux_src), d.ns,
+                               xsp, k, kshift, gate);
         }
         if (d.n_dtiles) {
             // column windows of exact power-of-two widths (<= 16): the wave-uniform red loads

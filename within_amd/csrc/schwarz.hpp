@@ -25,16 +25,24 @@ struct LocalSolverSettings {
 
 // Host form of a one-level additive Schwarz preconditioner over factor-pair components.
 // Every component is stored eliminated-block first: slots [slot_start[c], +ne[c]) are the
-// eliminated levels, the next nk[c] slots the kept levels.
+// eliminated levels, the next nk[c] slots the kept levels.  A component's matrix is stored
+// in its SDDM coordinates (crates/within/src/domain/factor_pairs/sddm.rs): the restriction
+// multiplies by fold[s] (the coordinate map: +1 / -1 on the two blocks of a plain pair, sign
+// times dominance scale of a general one), the prolongation by fold[s] again.
 struct Precond::Schwarz {
     uint64_t n_dofs = 0;
     // components
     std::vector<uint64_t> slot_start; // n_comp + 1
     std::vector<uint32_t> ne, nk;
-    std::vector<uint8_t> fac_kind;    // 0 dense pseudo-inverse of the kept block, 1 sparse LDL'
+    // 0: dense kept-block operator (S^+ of a floating Laplacian, S^-1 otherwise);
+    // 1: sparse LDL' over the kept slots (in elimination order);
+    // 2: sparse LDL' of the Gremban cover of a signed kept block (2 nk vertices, cover_perm)
+    std::vector<uint8_t> fac_kind;
+    std::vector<uint8_t> gauge;       // 0 floating (component means removed), 1 none
     // slots
     std::vector<uint32_t> glob;       // global DOF
     std::vector<double> pw;           // partition weight
+    std::vector<double> fold;         // coordinate-map factor
     std::vector<double> inv_de;       // 1/D_elim (0 on kept slots)
     // C (eliminated x kept, local kept index) as CSR over slots; kept slots have empty rows
     std::vector<uint64_t> c_ptr;      // n_slots + 1
@@ -50,6 +58,10 @@ struct Precond::Schwarz {
     std::vector<double> l_val;
     std::vector<uint64_t> piv_off;    // n_comp
     std::vector<double> piv;
+    // Gremban covers (fac_kind 2): cover vertex eliminated t-th is cover_perm[cover_off[c] + t];
+    // vertex v < nk is kept level v on the first sheet, v >= nk kept level v - nk on the second
+    std::vector<uint64_t> cover_off;  // n_comp (unused unless fac_kind 2)
+    std::vector<uint32_t> cover_perm;
 
     uint64_t n_comp() const { return ne.size(); }
     uint64_t n_slots() const { return glob.size(); }
