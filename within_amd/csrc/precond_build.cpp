@@ -15,7 +15,9 @@
 #include <cstdio>
 #include <cstring>
 #include <numeric>
+#include <queue>
 #include <string>
+#include <tuple>
 #ifdef _OPENMP
 #include <omp.h>
 #endif
@@ -137,11 +139,14 @@ void sort_and_merge(std::vector<Edge> &e) {
 }
 
 // Approximate Cholesky of a connected Laplacian given by its off-diagonal edges, as
-// L D L' with L unit-lower in elimination order.  Vertices are eliminated in a seeded random
-// order; each star's clique is replaced by `split` sampled clique trees.  The last vertex
-// has an empty star and pivot 0 (the floating gauge, pseudo-inverted to 0).  `last` >= 0
-// forces that vertex to the end of the order: with the ground vertex of a grounded system
-// there, dropping it leaves a factor of the grounded (positive definite) kept block.
+// L D L' with L unit-lower in elimination order.  Vertices are eliminated in approximate
+// minimum-degree order (the count of live edge instances, a seeded hash breaking ties), as
+// approximate Cholesky is usually run: low-degree vertices first keeps the sampled fill and
+// the triangular solves' level depth small on hub-heavy graphs.  Each star's clique is
+// replaced by `split` sampled clique trees.  The last vertex has an empty star and pivot 0
+// (the floating gauge, pseudo-inverted to 0).  `last` >= 0 holds that vertex back to the
+// end: with the ground vertex of a grounded system there, dropping it leaves a factor of
+// the grounded (positive definite) kept block.
 struct AcFactor {
     std::vector<uint32_t> order;    // order[t] = vertex eliminated t-th
     std::vector<uint64_t> rptr;     // L by rows (relabelled), strictly lower
@@ -157,29 +162,35 @@ AcFactor approx_cholesky(uint32_t n, const std::vector<Edge> &edges, uint64_t se
         adj[e.lo].push_back({e.hi, e.w});
         adj[e.hi].push_back({e.lo, e.w});
     }
-    f.order.resize(n);
-    std::iota(f.order.begin(), f.order.end(), 0u);
-    Rng ord(mix_seed(seed, n, 0xAC0));
-    for (uint32_t i = n; i > 1; --i) std::swap(f.order[i - 1], f.order[ord.next() % i]);
-    if (last >= 0 && n > 0) { // move `last` to the end, keeping the others' relative order
-        auto it = std::find(f.order.begin(), f.order.end(), (uint32_t)last);
-        std::rotate(it, it + 1, f.order.end());
+    std::vector<uint64_t> cnt(n), tie(n);
+    Rng tr(mix_seed(seed, n, 0xAC0));
+    for (uint32_t v = 0; v < n; ++v) {
+        cnt[v] = adj[v].size();
+        tie[v] = tr.next();
     }
-    std::vector<uint32_t> pos(n);
-    for (uint32_t t = 0; t < n; ++t) pos[f.order[t]] = t;
+    using Key = std::tuple<uint64_t, uint64_t, uint32_t>; // (degree, tie, vertex), smallest first
+    std::priority_queue<Key, std::vector<Key>, std::greater<Key>> pq;
+    for (uint32_t v = 0; v < n; ++v)
+        if ((int64_t)v != last) pq.push({cnt[v], tie[v], v});
     std::vector<uint8_t> gone(n, 0);
-    // columns of L in elimination order: (row position, value)
-    std::vector<std::vector<std::pair<uint32_t, double>>> cols(n);
-    f.d.assign(n, 0.0);
+    // columns of L in elimination order: (row vertex, value)
+    std::vector<std::vector<std::pair<uint32_t, double>>> cols;
+    cols.reserve(n);
+    f.order.reserve(n);
+    f.d.reserve(n);
     std::vector<std::pair<uint32_t, double>> nb;
     std::vector<Edge> fill;
-    for (uint32_t t = 0; t < n; ++t) {
-        const uint32_t v = f.order[t];
+    auto eliminate = [&](uint32_t v) {
         nb.clear();
         for (auto &e : adj[v])
-            if (!gone[e.first]) nb.push_back(e);
+            if (!gone[e.first]) {
+                nb.push_back(e);
+                --cnt[e.first];
+                pq.push({cnt[e.first], tie[e.first], e.first});
+            }
         std::vector<std::pair<uint32_t, double>>().swap(adj[v]);
         gone[v] = 1;
+        f.order.push_back(v);
         std::sort(nb.begin(), nb.end());
         size_t wr = 0;
         for (size_t r = 0; r < nb.size(); ++r) {
@@ -189,30 +200,46 @@ AcFactor approx_cholesky(uint32_t n, const std::vector<Edge> &edges, uint64_t se
         nb.resize(wr);
         double W = 0.0;
         for (auto &e : nb) W += e.second;
-        f.d[t] = W;
-        if (W <= 0.0) continue;
-        for (auto &e : nb) cols[t].push_back({pos[e.first], -e.second / W});
+        f.d.push_back(W);
+        cols.emplace_back();
+        if (W <= 0.0) return;
+        for (auto &e : nb) cols.back().push_back({e.first, -e.second / W});
         fill.clear();
         Rng rng(mix_seed(seed, v, 0x57A));
         sample_clique_tree(nb, W, split, rng, fill);
         for (const Edge &e : fill) {
             adj[e.lo].push_back({e.hi, e.w});
             adj[e.hi].push_back({e.lo, e.w});
+            for (uint32_t u : {e.lo, e.hi}) {
+                ++cnt[u];
+                if ((int64_t)u != last) pq.push({cnt[u], tie[u], u});
+            }
         }
+    };
+    // every degree change pushes a fresh entry, so an entry whose degree disagrees is stale
+    while (!pq.empty()) {
+        const auto [c, tb, v] = pq.top();
+        pq.pop();
+        if (gone[v] || c != cnt[v] || (int64_t)v == last) continue;
+        eliminate(v);
     }
+    if (last >= 0 && n > 0) eliminate((uint32_t)last);
+    std::vector<uint32_t> pos(n);
+    for (uint32_t t = 0; t < n; ++t) pos[f.order[t]] = t;
     // transpose columns into rows (row = later vertex, col = earlier vertex)
     f.rptr.assign(n + 1, 0);
     for (uint32_t t = 0; t < n; ++t)
-        for (auto &e : cols[t]) f.rptr[e.first + 1]++;
+        for (auto &e : cols[t]) f.rptr[pos[e.first] + 1]++;
     for (uint32_t i = 0; i < n; ++i) f.rptr[i + 1] += f.rptr[i];
     f.ridx.resize(f.rptr[n]);
     f.rval.resize(f.rptr[n]);
     std::vector<uint64_t> cur(f.rptr.begin(), f.rptr.end() - 1);
     for (uint32_t t = 0; t < n; ++t)
         for (auto &e : cols[t]) {
-            f.ridx[cur[e.first]] = t;
-            f.rval[cur[e.first]] = e.second;
-            cur[e.first]++;
+            const uint32_t row = pos[e.first];
+            f.ridx[cur[row]] = t;
+            f.rval[cur[row]] = e.second;
+            cur[row]++;
         }
     return f;
 }

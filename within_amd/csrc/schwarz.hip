@@ -32,6 +32,7 @@ constexpr int BLK = 256;
 constexpr uint32_t ROW_CHUNK = 256;   // CSR entries per work item before a row is split
 constexpr uint32_t MEAN_CHUNK = 64;   // slots per partial-sum chunk
 constexpr uint32_t DENSE_TILED_MIN = 256; // kept blocks at least this large use the tiled product
+constexpr uint64_t LONG_ROW = 64;         // factor rows with more entries are summed by a whole workgroup
 
 struct RowItem {
     uint32_t row;   // slot id of the output row
@@ -72,6 +73,7 @@ struct SchwarzDev {
     uint64_t dense_part_rows = 0;
     // sparse kept blocks (one workgroup per component): levels and factor rows
     DevBuf sp_comp, sp_lvl_f_ptr, sp_lvl_b_ptr, sp_lvl_f_rows, sp_lvl_b_rows, sp_meta;
+    DevBuf sp_lvl_f_long, sp_lvl_b_long; // per level: first long row
     DevBuf l_ptr, l_idx, l_val, lt_ptr, lt_idx, lt_val, piv;
     uint32_t n_sparse = 0;
     // auxiliary slots [ns, ns + n_aux) of the covers: real kept slot | sheet bit (second sheet)
@@ -359,9 +361,69 @@ __global__ __launch_bounds__(BLK) void sk_dense(uint32_t n_rows, const uint64_t 
 // level.  Rows within a level are independent; __syncthreads() orders the levels.  Row and
 // column ids are global slot ids; y and x live in xs (in place).
 template <typename CT>
+__device__ inline double sparse_rhs(const SpMeta &m, uint32_t row, const double *red, const uint32_t *aux_src,
+                                    uint64_t ns, double mu, uint32_t k, uint32_t j) {
+    if (m.cover) { // antisymmetric embed [b, -b] of the kept block's reduced rhs
+        const uint32_t a = aux_src[row - ns];
+        const double b = red[(uint64_t)(a & 0x7FFFFFFFu) * k + j];
+        return (a >> 31) ? -b : b;
+    }
+    return red[(uint64_t)row * k + j] - mu;
+}
+
+// One level of a triangular solve.  Rows [b, el) are short: one row group each, entries
+// summed in order.  Rows [el, e) are long (more than LONG_ROW entries, typically the hubs
+// eliminated last): the whole workgroup splits each one's entries over its row groups and
+// adds the partial sums in a fixed LDS tree.  FORWARD: x = rhs - L x; else x = x - L' x.
+template <typename CT, bool FORWARD>
+__device__ inline void sparse_level(const SpMeta &m, uint64_t b, uint64_t el, uint64_t e, const uint32_t *rows,
+                                    const uint64_t *ptr, const uint32_t *idx, const CT *val, const double *red,
+                                    const uint32_t *aux_src, uint64_t ns, double mu, double *xs, double *sh,
+                                    uint32_t k, uint32_t kshift) {
+    const uint32_t kp = 1u << kshift, j = threadIdx.x & (kp - 1);
+    const uint32_t grp = threadIdx.x >> kshift, ngrp = BLK >> kshift;
+    const bool on = j < k;
+    if (on)
+        for (uint64_t q = b + grp; q < el; q += ngrp) {
+            const uint32_t row = rows[q];
+            const uint64_t lr = (uint64_t)(row - m.kept_base) + m.row_off;
+            double acc = FORWARD ? sparse_rhs<CT>(m, row, red, aux_src, ns, mu, k, j) : xs[(uint64_t)row * k + j];
+            for (uint64_t p = ptr[lr]; p < ptr[lr + 1]; ++p) acc -= (double)val[p] * xs[(uint64_t)idx[p] * k + j];
+            xs[(uint64_t)row * k + j] = acc;
+        }
+    for (uint64_t q = el; q < e; ++q) {
+        const uint32_t row = rows[q];
+        const uint64_t lr = (uint64_t)(row - m.kept_base) + m.row_off;
+        double part = 0.0;
+        if (on)
+            for (uint64_t p = ptr[lr] + grp; p < ptr[lr + 1]; p += ngrp)
+                part += (double)val[p] * xs[(uint64_t)idx[p] * k + j];
+        sh[threadIdx.x] = part;
+        __syncthreads();
+        for (uint32_t s = BLK / 2; s >= kp && s > 0; s >>= 1) {
+            if (threadIdx.x < s) sh[threadIdx.x] += sh[threadIdx.x + s];
+            __syncthreads();
+        }
+        if (threadIdx.x < kp && on) {
+            const double base = FORWARD ? sparse_rhs<CT>(m, row, red, aux_src, ns, mu, k, j) : xs[(uint64_t)row * k + j];
+            xs[(uint64_t)row * k + j] = base - sh[threadIdx.x];
+        }
+        __syncthreads();
+    }
+}
+
+// Sparse kept-block solve, one workgroup per component: forward L y = red - mu2 level by
+// level, y /= d (a zero pivot -- the floating gauge -- gives 0), backward L' x = y level by
+// level.  Rows within a level are independent; __syncthreads() orders the levels.  Row and
+// column ids are global slot ids; y and x live in xs (in place).  A Gremban cover's factor
+// runs on auxiliary slots: its right-hand side is the embed [b, -b] and the kept slots
+// receive (y+ - y-) / 2 at the end.
+template <typename CT>
 __global__ __launch_bounds__(BLK) void sk_sparse(const SpMeta *__restrict__ meta, const uint64_t *__restrict__ lvl_f_ptr,
+                                                 const uint64_t *__restrict__ lvl_f_long,
                                                  const uint32_t *__restrict__ lvl_f_rows,
                                                  const uint64_t *__restrict__ lvl_b_ptr,
+                                                 const uint64_t *__restrict__ lvl_b_long,
                                                  const uint32_t *__restrict__ lvl_b_rows,
                                                  const uint64_t *__restrict__ l_ptr, const uint32_t *__restrict__ l_idx,
                                                  const CT *__restrict__ l_val, const uint64_t *__restrict__ lt_ptr,
@@ -371,6 +433,7 @@ __global__ __launch_bounds__(BLK) void sk_sparse(const SpMeta *__restrict__ meta
                                                  uint64_t ns, double *xs, uint32_t k,
                                                  uint32_t kshift, const int32_t *__restrict__ gate) {
     if (gate_closed(gate, k)) return;
+    __shared__ double sh[BLK];
     const uint32_t kp = 1u << kshift, j = threadIdx.x & (kp - 1);
     const uint32_t lane_row = threadIdx.x >> kshift, rows_per_pass = BLK >> kshift;
     const SpMeta m = meta[blockIdx.x];
@@ -378,22 +441,9 @@ __global__ __launch_bounds__(BLK) void sk_sparse(const SpMeta *__restrict__ meta
     const double mu = on ? mu2[(uint64_t)m.comp * k + j] : 0.0;
     // forward substitution; l_ptr / piv are indexed by (row slot - kept_base) + row offset
     for (uint32_t lv = 0; lv < m.nlev_f; ++lv) {
-        const uint64_t b = lvl_f_ptr[m.lvl_f_off + lv], e = lvl_f_ptr[m.lvl_f_off + lv + 1];
-        if (on)
-            for (uint64_t q = b + lane_row; q < e; q += rows_per_pass) {
-                const uint32_t row = lvl_f_rows[q];
-                const uint64_t lr = (uint64_t)(row - m.kept_base) + m.row_off;
-                double acc;
-                if (m.cover) { // antisymmetric embed [b, -b] of the kept block's reduced rhs
-                    const uint32_t a = aux_src[row - ns];
-                    const double b = red[(uint64_t)(a & 0x7FFFFFFFu) * k + j];
-                    acc = (a >> 31) ? -b : b;
-                } else {
-                    acc = red[(uint64_t)row * k + j] - mu;
-                }
-                for (uint64_t p = l_ptr[lr]; p < l_ptr[lr + 1]; ++p) acc -= (double)l_val[p] * xs[(uint64_t)l_idx[p] * k + j];
-                xs[(uint64_t)row * k + j] = acc;
-            }
+        const uint64_t o = m.lvl_f_off + lv;
+        sparse_level<CT, true>(m, lvl_f_ptr[o], lvl_f_long[o], lvl_f_ptr[o + 1], lvl_f_rows, l_ptr, l_idx, l_val, red,
+                               aux_src, ns, mu, xs, sh, k, kshift);
         __syncthreads();
     }
     // diagonal
@@ -406,16 +456,9 @@ __global__ __launch_bounds__(BLK) void sk_sparse(const SpMeta *__restrict__ meta
     __syncthreads();
     // backward substitution with L' (rows of L' = columns of L)
     for (uint32_t lv = 0; lv < m.nlev_b; ++lv) {
-        const uint64_t b = lvl_b_ptr[m.lvl_b_off + lv], e = lvl_b_ptr[m.lvl_b_off + lv + 1];
-        if (on)
-            for (uint64_t q = b + lane_row; q < e; q += rows_per_pass) {
-                const uint32_t row = lvl_b_rows[q];
-                const uint64_t lr = (uint64_t)(row - m.kept_base) + m.row_off;
-                double acc = xs[(uint64_t)row * k + j];
-                for (uint64_t p = lt_ptr[lr]; p < lt_ptr[lr + 1]; ++p)
-                    acc -= (double)lt_val[p] * xs[(uint64_t)lt_idx[p] * k + j];
-                xs[(uint64_t)row * k + j] = acc;
-            }
+        const uint64_t o = m.lvl_b_off + lv;
+        sparse_level<CT, false>(m, lvl_b_ptr[o], lvl_b_long[o], lvl_b_ptr[o + 1], lvl_b_rows, lt_ptr, lt_idx, lt_val,
+                                red, aux_src, ns, mu, xs, sh, k, kshift);
         __syncthreads();
     }
     if (m.cover) { // read back x = (y+ - y-) / 2 into the real kept slots, one sheet at a time
@@ -670,7 +713,7 @@ std::shared_ptr<SchwarzDev<CT>> build_dev(const Precond::Schwarz &S) {
     if (D->has_sparse) {
         // level schedules of L (forward) and L' (backward) per sparse component
         std::vector<SpMeta> metas;
-        std::vector<uint64_t> lptr, ltptr, lvf_ptr, lvb_ptr;
+        std::vector<uint64_t> lptr, ltptr, lvf_ptr, lvb_ptr, lvf_long, lvb_long;
         std::vector<uint32_t> lidx, ltidx, lvf_rows, lvb_rows;
         std::vector<CT> lval, ltval, piv;
         uint64_t max_depth = 0, total_rows = 0;
@@ -739,23 +782,43 @@ std::shared_ptr<SchwarzDev<CT>> build_dev(const Precond::Schwarz &S) {
                 lb[i] = v;
                 db = std::max(db, v + 1);
             }
-            auto bucket = [&](const std::vector<uint32_t> &lvl, uint32_t depth, std::vector<uint64_t> &ptr,
-                              std::vector<uint32_t> &rows) {
+            // rows bucketed by level, short rows first and rows with more than LONG_ROW entries
+            // last; `lng` (aligned with `ptr`) marks where a level's long rows start
+            auto bucket = [&](const std::vector<uint32_t> &lvl, uint32_t depth, const std::vector<uint64_t> &rnnz,
+                              std::vector<uint64_t> &ptr, std::vector<uint64_t> &lng, std::vector<uint32_t> &rows) {
                 const uint64_t off = ptr.size();
-                std::vector<uint64_t> cnt(depth + 1, 0);
-                for (uint32_t i = 0; i < nk; ++i) cnt[lvl[i] + 1]++;
+                std::vector<uint64_t> cnt(depth + 1, 0), nlong(depth + 1, 0);
+                for (uint32_t i = 0; i < nk; ++i) {
+                    cnt[lvl[i] + 1]++;
+                    if (rnnz[i] > LONG_ROW) nlong[lvl[i]]++;
+                }
                 for (uint32_t l = 0; l < depth; ++l) cnt[l + 1] += cnt[l];
                 const uint64_t rbase = rows.size();
                 rows.resize(rbase + nk);
-                std::vector<uint64_t> cur(cnt.begin(), cnt.end() - 1);
-                for (uint32_t i = 0; i < nk; ++i) rows[rbase + cur[lvl[i]]++] = kb + i;
-                for (uint32_t l = 0; l <= depth; ++l) ptr.push_back(rbase + cnt[l]);
+                std::vector<uint64_t> cs(depth), cl(depth);
+                for (uint32_t l = 0; l < depth; ++l) {
+                    cs[l] = cnt[l];
+                    cl[l] = cnt[l + 1] - nlong[l];
+                }
+                for (uint32_t i = 0; i < nk; ++i) {
+                    const uint32_t l = lvl[i];
+                    rows[rbase + (rnnz[i] > LONG_ROW ? cl[l]++ : cs[l]++)] = kb + i;
+                }
+                for (uint32_t l = 0; l <= depth; ++l) {
+                    ptr.push_back(rbase + cnt[l]);
+                    lng.push_back(rbase + (l < depth ? cnt[l + 1] - nlong[l] : cnt[depth]));
+                }
                 return off;
             };
+            std::vector<uint64_t> nnz_f(nk), nnz_b(nk);
+            for (uint32_t i = 0; i < nk; ++i) {
+                nnz_f[i] = S.l_ptr[ro + i + 1] - S.l_ptr[ro + i];
+                nnz_b[i] = tp[i + 1] - tp[i];
+            }
             m.nlev_f = df;
             m.nlev_b = db;
-            m.lvl_f_off = bucket(lf, df, lvf_ptr, lvf_rows);
-            m.lvl_b_off = bucket(lb, db, lvb_ptr, lvb_rows);
+            m.lvl_f_off = bucket(lf, df, nnz_f, lvf_ptr, lvf_long, lvf_rows);
+            m.lvl_b_off = bucket(lb, db, nnz_b, lvb_ptr, lvb_long, lvb_rows);
             max_depth = std::max<uint64_t>(max_depth, std::max(df, db));
             total_rows += nk;
             metas.push_back(m);
@@ -773,6 +836,8 @@ std::shared_ptr<SchwarzDev<CT>> build_dev(const Precond::Schwarz &S) {
         D->lt_val.upload(ltval.data(), ltval.size());
         D->piv.upload(piv.data(), piv.size());
         D->sp_lvl_f_ptr.upload(lvf_ptr.data(), lvf_ptr.size());
+        D->sp_lvl_f_long.upload(lvf_long.data(), lvf_long.size());
+        D->sp_lvl_b_long.upload(lvb_long.data(), lvb_long.size());
         D->sp_lvl_f_rows.upload(lvf_rows.data(), lvf_rows.size());
         D->sp_lvl_b_ptr.upload(lvb_ptr.data(), lvb_ptr.size());
         D->sp_lvl_b_rows.upload(lvb_rows.data(), lvb_rows.size());
@@ -833,8 +898,9 @@ struct SchwarzRunImpl : SchwarzRun {
         if (d.n_sparse) {
             mean(d.kept_chunks, d.n_kept_chunks, d.kept_cptr, d.comp_kcount, redp, dp<double>(mu2), k, kshift, st, gate);
             hipLaunchKernelGGL((sk_sparse<CT>), dim3(d.n_sparse), dim3(BLK), 0, st, dp<SpMeta>(d.sp_meta),
-                               dp<uint64_t>(d.sp_lvl_f_ptr), dp<uint32_t>(d.sp_lvl_f_rows),
-                               dp<uint64_t>(d.sp_lvl_b_ptr), dp<uint32_t>(d.sp_lvl_b_rows), dp<uint64_t>(d.l_ptr),
+                               dp<uint64_t>(d.sp_lvl_f_ptr), dp<uint64_t>(d.sp_lvl_f_long),
+                               dp<uint32_t>(d.sp_lvl_f_rows), dp<uint64_t>(d.sp_lvl_b_ptr),
+                               dp<uint64_t>(d.sp_lvl_b_long), dp<uint32_t>(d.sp_lvl_b_rows), dp<uint64_t>(d.l_ptr),
                                dp<uint32_t>(d.l_idx), dp<CT>(d.l_val), dp<uint64_t>(d.lt_ptr), dp<uint32_t>(d.lt_idx),
                                dp<CT>(d.lt_val), dp<CT>(d.piv), redp, dp<double>(mu2), dp<uint32_t>(d.aux_src), d.ns,
                                xsp, k, kshift, gate);
