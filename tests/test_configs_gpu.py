@@ -1,0 +1,50 @@
+"""Full-size workloads on the GPU, checked through properties that do not need an exact
+solve at this size: convergence, the FP64 normal-equation residual, FP64 against mixed
+precision, and (config 2) the CPU oracle solving a few columns with the same preconditioner."""
+import numpy as np
+import pytest
+
+import within_amd as wa
+from tests._configs import cfg2_data, cfg3_data
+from tests._helpers import oracle_schwarz_from_bytes
+
+pytestmark = pytest.mark.gpu
+
+
+def rel(a, b):
+    return np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300)
+
+
+def test_config2_weighted_batch_full_size(gpu, oracle):
+    codes, w, Y = cfg2_data()
+    s = wa.Solver(codes, weights=w)
+    rm = s.solve_batch(Y, wa.LsmrOptions(tol=1e-8, precision="mixed"))
+    rf = s.solve_batch(Y, wa.LsmrOptions(tol=1e-8))
+    assert all(rm.converged) and all(rf.converged)
+    assert max(rm.true_residual) < 1e-6 and max(rf.true_residual) < 1e-6
+    d_mf = max(rel(rm.demeaned[:, j], rf.demeaned[:, j]) for j in range(Y.shape[1]))
+    # the CPU oracle: the same algorithm and preconditioner data, FP64, two columns
+    sch = oracle_schwarz_from_bytes(oracle, s.preconditioner.to_bytes())
+    d = oracle.design(codes)
+    _, Dm, res = oracle.solve_batch(d, np.asfortranarray(Y[:, :2]), w, prec="schwarz", schwarz=sch, tol=1e-8,
+                                   threads=2)
+    assert all(r.converged for r in res)
+    d_or = max(rel(rf.demeaned[:, j], Dm[:, j]) for j in range(2))
+    print(f"config 2: mixed vs fp64 {d_mf:.2e}, fp64 vs oracle {d_or:.2e}, iterations {rf.iterations[:2]} vs "
+          f"{[r.iterations for r in res]}, true residual {max(rf.true_residual):.2e}")
+    assert d_mf <= 1e-8
+    assert d_or <= 1e-8
+
+
+def test_config3_worker_firm_panel_full_size(gpu):
+    worker, firm, year, z, Y = cfg3_data()
+    design = [wa.Effect(worker, True), wa.Effect(firm, True, [z]), wa.Effect(year, True)]
+    s = wa.Solver(design)
+    rf = s.solve_batch(Y, wa.LsmrOptions(tol=1e-10))
+    rm = s.solve_batch(Y, wa.LsmrOptions(tol=1e-10, precision="mixed"))
+    assert all(rf.converged) and all(rm.converged)
+    assert max(rf.true_residual) < 1e-6 and max(rm.true_residual) < 1e-6
+    d_mf = max(rel(rm.demeaned[:, j], rf.demeaned[:, j]) for j in range(Y.shape[1]))
+    print(f"config 3: mixed vs fp64 {d_mf:.2e}, iterations {rf.iterations} / {rm.iterations}, "
+          f"true residual {max(rf.true_residual):.2e}")
+    assert d_mf <= 1e-8
