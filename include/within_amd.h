@@ -189,6 +189,17 @@ wx_status wx_solver_solve_batch_device(const wx_solver *s, const double *Y_devic
                                        const wx_lsmr_options *opts, double *X_device, double *demeaned_device,
                                        wx_rhs_stats *stats, void *stream);
 
+/* Adjoint of the weighted projection demeaned = z - D x(z, w) for reverse-mode
+ * differentiation.  No counterpart in the reference (its bindings are not differentiable);
+ * it serves torch.autograd callers of within_amd.autograd.  Given the upstream gradient
+ * G = dL/d(demeaned) and the forward's output R (both n_obs x k column-major, device, caller
+ * row order), one adjoint solve D'WD lambda = D'G reuses the solver's preconditioner and a
+ * fused gather forms grad_z = G - w .* (D lambda) (n_obs x k) and, when grad_w is not NULL,
+ * grad_w = -sum_j R_j .* (D lambda)_j (n_obs).  Needs strictly positive weights. */
+wx_status wx_solver_projection_grad_device(const wx_solver *s, const double *G, uint64_t ldg, const double *R,
+                                           uint64_t ldr, uint64_t k, const wx_lsmr_options *opts, double *grad_z,
+                                           double *grad_w, wx_rhs_stats *stats, void *stream);
+
 /* Replaces `Operator::apply` / `Operator::apply_adjoint` of the weighted design operator
  * A = W^{1/2} D (crates/schwarz-precond/src/lib.rs:48-57 implemented by
  * crates/within/src/operator/design.rs:92-117), in the caller's row order:

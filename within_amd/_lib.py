@@ -89,6 +89,9 @@ EXPORTS = {
     "wx_solver_solve_batch_device": ([C.c_void_p, C.c_void_p, C.c_uint64, C.c_uint64, C.POINTER(LsmrOpts),
                                       C.c_void_p, C.c_void_p, C.POINTER(RhsStats), C.c_void_p], C.c_int),
     "wx_solver_apply_operator": ([C.c_void_p, C.c_int32, C.c_void_p, C.c_uint64, C.c_void_p], C.c_int),
+    "wx_solver_projection_grad_device": ([C.c_void_p, C.c_void_p, C.c_uint64, C.c_void_p, C.c_uint64, C.c_uint64,
+                                          C.POINTER(LsmrOpts), C.c_void_p, C.c_void_p, C.POINTER(RhsStats),
+                                          C.c_void_p], C.c_int),
     "wx_solver_preconditioner": ([C.c_void_p, C.POINTER(C.c_void_p)], C.c_int),
     "wx_solver_n_terms": ([C.c_void_p], C.c_uint64),
     "wx_solver_layout": ([C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p], None),

@@ -209,6 +209,7 @@ wx_status wx_solver_new(const wx_design *design, const double *weights, const wx
             permute_in(s.design, weights, w.data());
         }
         s.weighted = weights != nullptr;
+        if (weights) s.min_weight = *std::min_element(w.begin(), w.end());
         if (s.design.has_slopes()) s.reparam = build_reparam(s.design, weights ? w.data() : nullptr);
         if (prebuilt) {
             if (prebuilt->p->n != s.design.n_dofs)
@@ -281,6 +282,21 @@ wx_status wx_solver_solve_batch_device(const wx_solver *s, const double *Y, uint
         Stream st(stream);
         solve_batch_device(s->s, Y, ldy, k, o, X, Dm, stats, st.s);
         WX_HIP(hipStreamSynchronize(st.s));
+    });
+}
+
+wx_status wx_solver_projection_grad_device(const wx_solver *s, const double *G, uint64_t ldg, const double *R,
+                                           uint64_t ldr, uint64_t k, const wx_lsmr_options *opts, double *grad_z,
+                                           double *grad_w, wx_rhs_stats *stats, void *stream) {
+    return guard([&] {
+        if (!s || !stats || (k && (!G || !R || !grad_z))) fail(WX_ERR_INVALID_INPUT, "null solver, array or stats pointer");
+        wx_lsmr_options o;
+        if (opts) o = *opts;
+        else wx_lsmr_options_default(&o);
+        if (o.local_size > 0)
+            fail(WX_ERR_INVALID_INPUT, "local_size (windowed reorthogonalization) is not supported on the device");
+        Stream st(stream);
+        projection_grad_device(s->s, G, ldg, R, ldr, k, o, grad_z, grad_w, stats, st.s);
     });
 }
 

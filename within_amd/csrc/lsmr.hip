@@ -1256,7 +1256,7 @@ struct PooledWorkspace {
 // Batched solve (Solver::solve_batch, crates/within/src/solver.rs:423-577)
 // ---------------------------------------------------------------------------
 void solve_batch_device(const Solver &s, const double *Y, uint64_t ldy, uint64_t k_total, const wx_lsmr_options &opts,
-                        double *X, double *Dm, wx_rhs_stats *stats, hipStream_t st) {
+                        double *X, double *Dm, wx_rhs_stats *stats, hipStream_t st, bool x_solver_basis) {
     const DevDesign &dv = s.dev;
     const uint64_t n = dv.n, N = dv.N;
     if (!std::isfinite(opts.tol) || opts.tol < 0.0)
@@ -1411,7 +1411,7 @@ void solve_batch_device(const Solver &s, const double *Y, uint64_t ldy, uint64_t
         if (Dm)
             hipLaunchKernelGGL(k_store_caller, dim3(gt), dim3(BLK), tile_bytes, st, n, inv_perm, dm_int, Dm + c0 * n,
                                n, k, kshift);
-        if (X && !s.reparam) {
+        if (X && (!s.reparam || x_solver_basis)) {
             uint64_t tot = N * k;
             hipLaunchKernelGGL(k_store_x, dim3((tot + BLK - 1) / BLK), dim3(BLK), 0, st, N, ws.x.as<double>(),
                                X + c0 * N, N, k);
@@ -1431,7 +1431,7 @@ void solve_batch_device(const Solver &s, const double *Y, uint64_t ldy, uint64_t
         WX_HIP(hipMemcpyAsync(host_d, ws.cols.p, sizeof(host_d), hipMemcpyDeviceToHost, st));
         WX_HIP(hipMemcpyAsync(host_i, ws.ints.p, sizeof(host_i), hipMemcpyDeviceToHost, st));
         WX_HIP(hipStreamSynchronize(st));
-        if (X && s.reparam) {
+        if (X && s.reparam && !x_solver_basis) {
             std::vector<double> xi(N * k), xo(N);
             WX_HIP(hipMemcpy(xi.data(), ws.x.p, N * k * 8, hipMemcpyDeviceToHost));
             for (uint32_t j = 0; j < k; ++j) {
@@ -1483,6 +1483,79 @@ void solve_batch_device(const Solver &s, const double *Y, uint64_t ldy, uint64_t
 std::vector<double> &last_timing() {
     static thread_local std::vector<double> t;
     return t;
+}
+
+// ---------------------------------------------------------------------------
+// Adjoint of the projection (reverse-mode differentiation of R = Z - D x(Z, w))
+// ---------------------------------------------------------------------------
+namespace {
+// Y[r + j*n] = G[r + j*ldg] / w_r   (caller order; w is stored in internal order)
+__global__ void k_div_weights(uint64_t n, const uint32_t *__restrict__ inv_perm, const double *__restrict__ w,
+                              const double *__restrict__ G, uint64_t ldg, double *__restrict__ Y, uint32_t k) {
+    const uint64_t idx = (uint64_t)blockIdx.x * BLK + threadIdx.x;
+    if (idx >= n * k) return;
+    const uint64_t j = idx / n, r = idx - j * n;
+    const double wr = w ? w[inv_perm ? inv_perm[r] : r] : 1.0;
+    Y[r + j * n] = G[r + j * ldg] / wr;
+}
+
+// One fused pass per caller row r: (D lambda)_r for every column (the gather of the
+// operator, unweighted, column-major lambda), then grad_z = G - w .* (D lambda) and
+// grad_w = -sum_j R_j .* (D lambda)_j.
+__global__ __launch_bounds__(BLK) void k_projection_grad(DesignArgs d, const uint32_t *__restrict__ inv_perm,
+                                                          const double *__restrict__ w,
+                                                          const double *__restrict__ lam, uint64_t N,
+                                                          const double *__restrict__ G, uint64_t ldg,
+                                                          const double *__restrict__ R, uint64_t ldr,
+                                                          double *__restrict__ gz, double *__restrict__ gw,
+                                                          uint32_t k) {
+    const uint64_t stride = (uint64_t)gridDim.x * BLK;
+    for (uint64_t r = (uint64_t)blockIdx.x * BLK + threadIdx.x; r < d.n; r += stride) {
+        const uint64_t i = inv_perm ? inv_perm[r] : r;
+        const double wi = w ? w[i] : 1.0;
+        double gacc = 0.0;
+        for (uint32_t j = 0; j < k; ++j) {
+            double dl = 0.0;
+            for (uint32_t t = 0; t < d.T; ++t) {
+                const TermInfo &ti = d.terms[t];
+                const uint64_t code = d.codes[(uint64_t)t * d.n + i];
+                double tv = -0.0;
+                for (uint32_t c = 0; c < ti.ncols; ++c)
+                    tv = tv + lam[ti.offset + c * ti.n_levels + code + (uint64_t)j * N] * term_load(d, ti.load[c], i);
+                dl += tv;
+            }
+            gz[r + (uint64_t)j * d.n] = G[r + (uint64_t)j * ldg] - wi * dl;
+            gacc -= R[r + (uint64_t)j * ldr] * dl;
+        }
+        if (gw) gw[r] = gacc;
+    }
+}
+} // namespace
+
+void projection_grad_device(const Solver &s, const double *G, uint64_t ldg, const double *R, uint64_t ldr, uint64_t k,
+                            const wx_lsmr_options &opts, double *grad_z, double *grad_w, wx_rhs_stats *stats,
+                            hipStream_t st) {
+    const DevDesign &dv = s.dev;
+    const uint64_t n = dv.n, N = dv.N;
+    if (ldg < n || ldr < n) fail(WX_ERR_INVALID_INPUT, "leading dimension smaller than n_obs");
+    if (!(s.min_weight > 0.0))
+        fail(WX_ERR_INVALID_INPUT, "the projection's adjoint needs strictly positive weights");
+    if (k == 0) return;
+    const uint32_t *inv_perm = dv.inv_perm.p ? dv.inv_perm.as<uint32_t>() : nullptr;
+    const double *w = dv.w.p ? dv.w.as<double>() : nullptr;
+    // adjoint solve: D'WD lambda = D'G is the normal equation of the projection of G ./ w
+    DevBuf Y, lam;
+    Y.alloc(n * k * 8);
+    lam.alloc(N * k * 8);
+    hipLaunchKernelGGL(k_div_weights, dim3((uint32_t)((n * k + BLK - 1) / BLK)), dim3(BLK), 0, st, n, inv_perm, w, G,
+                       ldg, Y.as<double>(), (uint32_t)k);
+    WX_HIP(hipGetLastError());
+    solve_batch_device(s, Y.as<double>(), n, k, opts, lam.as<double>(), nullptr, stats, st, true);
+    const uint32_t grid = (uint32_t)std::min<uint64_t>((n + BLK - 1) / BLK, 65535);
+    hipLaunchKernelGGL(k_projection_grad, dim3(std::max<uint32_t>(grid, 1)), dim3(BLK), 0, st, dv.args(), inv_perm, w,
+                       lam.as<double>(), N, G, ldg, R, ldr, grad_z, grad_w, (uint32_t)k);
+    WX_HIP(hipGetLastError());
+    WX_HIP(hipStreamSynchronize(st));
 }
 
 // ---------------------------------------------------------------------------

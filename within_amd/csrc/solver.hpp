@@ -94,6 +94,7 @@ struct Solver {
     Design design;
     std::unique_ptr<SlopeReparam> reparam;
     bool weighted = false;
+    double min_weight = 1.0; // smallest observation weight (1 when unweighted)
     std::shared_ptr<Precond> precond;
     std::vector<std::string> warnings;
     double time_setup = 0.0;
@@ -107,9 +108,17 @@ struct Solver {
 // Uploads the design (codes, slopes, weights, D' segment plans) to the current device.
 void upload_design(Solver &s, const std::vector<double> &weights_internal);
 
-// Runs the batched solve with device-resident inputs/outputs on `stream`.
+// Runs the batched solve with device-resident inputs/outputs on `stream`.  x_solver_basis:
+// X is left in the solver's (reparametrised) coefficient basis instead of the caller's.
 void solve_batch_device(const Solver &s, const double *Y, uint64_t ldy, uint64_t k, const wx_lsmr_options &opts,
-                        double *X, double *Dm, wx_rhs_stats *stats, hipStream_t stream);
+                        double *X, double *Dm, wx_rhs_stats *stats, hipStream_t stream, bool x_solver_basis = false);
+
+// Adjoint of the weighted projection R = Z - D x(Z, w) for reverse-mode differentiation:
+// grad_z = G - w .* (D lambda), grad_w = -sum_j R_j .* (D lambda)_j with D'WD lambda = D'G.
+// Column-major device arrays in caller row order; grad_w may be null.
+void projection_grad_device(const Solver &s, const double *G, uint64_t ldg, const double *R, uint64_t ldr, uint64_t k,
+                            const wx_lsmr_options &opts, double *grad_z, double *grad_w, wx_rhs_stats *stats,
+                            hipStream_t stream);
 
 // Per-phase (ms, launches) pairs of this thread's last timed solve (see wx_last_timing).
 std::vector<double> &last_timing();
