@@ -360,3 +360,50 @@ def test_worker_firm_panel_with_firm_slope(gpu):
     ex = exact_demean(D, Y)
     for j in range(4):
         assert rel(r.demeaned[:, j], ex[:, j]) <= 1e-8, j
+
+
+# ---------------------------------------------------------------------------
+# windowed reorthogonalisation (local_size) on the device
+# ---------------------------------------------------------------------------
+
+@pytest.mark.parametrize("prec", ["off", "diagonal", "schwarz"])
+@pytest.mark.parametrize("local_size", [1, 5, 40])
+def test_local_size_matches_oracle(gpu, oracle, prec, local_size):
+    codes, y, w = small("three_factor_weighted")
+    opts = wa.LsmrOptions(tol=1e-10, maxiter=5000, local_size=local_size)
+    d = oracle.design(codes)
+    if prec == "schwarz":
+        s = wa.Solver(codes, weights=w)
+        r = s.solve_batch(y, opts)
+        sch = oracle_schwarz_from_bytes(oracle, s.preconditioner.to_bytes())
+        _, Dm, res = oracle.solve_batch(d, y, w, prec="schwarz", schwarz=sch, tol=1e-10, local_size=local_size)
+    else:
+        pc = wa.PreconditionerConfig.Off if prec == "off" else wa.PreconditionerConfig.Diagonal
+        r = wa.solve_batch(codes, y, weights=w, options=opts, preconditioner=pc)
+        _, Dm, res = oracle.solve_batch(d, y, w, prec=prec, tol=1e-10, maxiter=5000, local_size=local_size)
+    assert all(r.converged) and all(q.converged for q in res)
+    ex = SMALL["three_factor_weighted__demeaned"]
+    for j in range(y.shape[1]):
+        assert rel(r.demeaned[:, j], Dm[:, j]) <= 1e-8, j
+        assert rel(r.demeaned[:, j], ex[:, j]) <= 1e-8, j
+    assert max(abs(a - b.iterations) for a, b in zip(r.iterations, res)) <= 2
+
+
+def test_local_size_mixed_precision_and_batch(gpu):
+    rng = np.random.default_rng(71)
+    n = 6000
+    codes = np.asfortranarray(np.column_stack([rng.integers(0, 300, n), rng.integers(0, 40, n),
+                                               rng.integers(0, 6, n)]).astype(np.uint32))
+    Y = np.asfortranarray(rng.standard_normal((n, 5)))
+    s = wa.Solver(codes)
+    ex = exact_demean(sparse_design(codes), Y)
+    for precision in ("fp64", "mixed"):
+        r = s.solve_batch(Y, wa.LsmrOptions(tol=1e-10, local_size=8, precision=precision))
+        assert all(r.converged)
+        for j in range(Y.shape[1]):
+            assert rel(r.demeaned[:, j], ex[:, j]) <= 1e-8
+    # batched columns equal column-by-column solves with the same window
+    rb = s.solve_batch(Y, wa.LsmrOptions(tol=1e-10, local_size=8))
+    for j in range(2):
+        r1 = s.solve_batch(np.asfortranarray(Y[:, j:j + 1]), wa.LsmrOptions(tol=1e-10, local_size=8))
+        np.testing.assert_allclose(r1.demeaned[:, 0], rb.demeaned[:, j], rtol=0, atol=1e-12)

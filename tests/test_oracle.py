@@ -410,3 +410,53 @@ def test_oracle_cover_solve_equals_inverse(oracle, surplus):
     for _ in range(3):
         r = rng.standard_normal(ne + nk)
         np.testing.assert_allclose(sch.apply(r), Gi @ r, rtol=1e-9, atol=1e-11)
+
+
+# ---------------------------------------------------------------------------
+# windowed reorthogonalisation (crates/schwarz-precond/src/lsmr/tests/solve.rs:330-432)
+# ---------------------------------------------------------------------------
+
+def _vandermonde(rows=30, cols=12):
+    x = np.arange(rows) / max(rows - 1, 1)
+    A = np.vander(x, cols, increasing=True)
+    b = np.log1p(x)
+    return A, b
+
+
+def _normal_residual(A, x, b):
+    return np.linalg.norm(A.T @ (b - A @ x))
+
+
+def test_oracle_reorth_unpreconditioned_converges(oracle):
+    A, b = _vandermonde()
+    rc0, x0, r0 = oracle.lsmr_dense(A, b, tol=1e-9, maxiter=30)
+    rc, x, r = oracle.lsmr_dense(A, b, tol=1e-9, maxiter=30, local_size=10)
+    assert rc == 0 and r.converged
+    assert not r0.converged or r.iterations <= r0.iterations
+    assert _normal_residual(A, x, b) < 1e-6
+
+
+def test_oracle_reorth_preconditioned_converges(oracle):
+    A, b = _vandermonde()
+    jac = 1.0 / (A * A).sum(0)
+    rc, x, r = oracle.lsmr_dense(A, b, tol=1e-9, maxiter=30, prec="diag", prec_data=jac, local_size=10)
+    assert rc == 0 and r.converged
+    assert _normal_residual(A, x, b) < 1e-6
+
+
+@pytest.mark.parametrize("window", [1, 12, 13])
+def test_oracle_reorth_window_boundaries(oracle, window):
+    A, b = _vandermonde()
+    rc, x, r = oracle.lsmr_dense(A, b, tol=1e-9, maxiter=200, local_size=window)
+    assert rc == 0 and r.converged
+    assert _normal_residual(A, x, b) < 1e-6
+
+
+def test_oracle_reorth_identity_preconditioner_matches_unpreconditioned(oracle):
+    # the M-weighted ring with M = I is the Euclidean one (lsmr.rs: lsmr == mlsmr(M = I))
+    A, b = _vandermonde()
+    _, x1, r1 = oracle.lsmr_dense(A, b, tol=1e-9, maxiter=30, local_size=10)
+    _, x2, r2 = oracle.lsmr_dense(A, b, tol=1e-9, maxiter=30, prec="identity", local_size=10)
+    assert r1.iterations == r2.iterations
+    # cond(A) ~ 1e10 leaves x ill-determined; the fitted values are not
+    np.testing.assert_allclose(A @ x1, A @ x2, rtol=1e-9, atol=1e-9)

@@ -277,8 +277,6 @@ wx_status wx_solver_solve_batch_device(const wx_solver *s, const double *Y, uint
         wx_lsmr_options o;
         if (opts) o = *opts;
         else wx_lsmr_options_default(&o);
-        if (o.local_size > 0)
-            fail(WX_ERR_INVALID_INPUT, "local_size (windowed reorthogonalization) is not supported on the device");
         Stream st(stream);
         solve_batch_device(s->s, Y, ldy, k, o, X, Dm, stats, st.s);
         WX_HIP(hipStreamSynchronize(st.s));
@@ -293,8 +291,6 @@ wx_status wx_solver_projection_grad_device(const wx_solver *s, const double *G, 
         wx_lsmr_options o;
         if (opts) o = *opts;
         else wx_lsmr_options_default(&o);
-        if (o.local_size > 0)
-            fail(WX_ERR_INVALID_INPUT, "local_size (windowed reorthogonalization) is not supported on the device");
         Stream st(stream);
         projection_grad_device(s->s, G, ldg, R, ldr, k, o, grad_z, grad_w, stats, st.s);
     });
@@ -318,8 +314,6 @@ wx_status wx_solver_solve_batch(const wx_solver *s, const double *Y, uint64_t ld
         wx_lsmr_options o;
         if (opts) o = *opts;
         else wx_lsmr_options_default(&o);
-        if (o.local_size > 0)
-            fail(WX_ERR_INVALID_INPUT, "local_size (windowed reorthogonalization) is not supported on the device");
         Stream st(nullptr);
         DevBuf dY, dX, dD;
         dY.alloc(n * k * 8);

@@ -534,6 +534,40 @@ __global__ __launch_bounds__(BLK) void k_dots(uint64_t N, const double *__restri
     }
 }
 
+// Windowed reorthogonalisation (WindowRing<2>, crates/schwarz-precond/src/lsmr/bidiag.rs:114-217):
+// the ring holds the last `cap` normalised v and M v = p~ / alpha of every column.  One MGS
+// sweep over the stored slots, oldest first: c = <v, P_slot>, v -= c V_slot, p~ -= c P_slot.
+__global__ __launch_bounds__(BLK) void k_ring_dot(uint64_t N, const double *__restrict__ v,
+                                                  const double *__restrict__ P, ColState cs,
+                                                  double *__restrict__ part, uint32_t k, uint32_t kshift) {
+    __shared__ double sh[BLK];
+    const uint32_t kp = 1u << kshift;
+    const uint32_t j = threadIdx.x & (kp - 1);
+    const uint64_t rstride = ((uint64_t)gridDim.x * BLK) >> kshift;
+    double acc = 0.0;
+    if (j < k && cs.active[j])
+        for (uint64_t dof = ((uint64_t)blockIdx.x * BLK + threadIdx.x) >> kshift; dof < N; dof += rstride)
+            acc += v[dof * k + j] * P[dof * k + j];
+    const double tot = block_reduce_cols(acc, kp, sh);
+    if (threadIdx.x < kp) part[(uint64_t)blockIdx.x * kp + threadIdx.x] = tot;
+}
+
+__global__ __launch_bounds__(BLK) void k_ring_axpy(uint64_t N, double *__restrict__ v, double *__restrict__ pt,
+                                                   const double *__restrict__ V, const double *__restrict__ P,
+                                                   const double *__restrict__ coef, ColState cs, uint32_t k,
+                                                   uint32_t kshift) {
+    const uint32_t kp = 1u << kshift;
+    const uint32_t j = threadIdx.x & (kp - 1);
+    if (j >= k || !cs.active[j]) return;
+    const double c = coef[j];
+    const uint64_t rstride = ((uint64_t)gridDim.x * BLK) >> kshift;
+    for (uint64_t dof = ((uint64_t)blockIdx.x * BLK + threadIdx.x) >> kshift; dof < N; dof += rstride) {
+        const uint64_t q = dof * k + j;
+        v[q] = -c * V[q] + 1.0 * v[q];
+        pt[q] = -c * P[q] + 1.0 * pt[q];
+    }
+}
+
 __device__ inline void givens(double a, double b, double &c, double &s, double &r) {
     // crates/schwarz-precond/src/lsmr/recurrence.rs:20-27
     r = hypot(a, b);
@@ -622,7 +656,8 @@ __global__ __launch_bounds__(BLK) void k_alpha1(const double *part, uint32_t nb,
 // v <- v / alpha_1, h = v, x = 0, h_bar = 0 (SolutionState::init, recurrence.rs:134-141).
 __global__ __launch_bounds__(BLK) void k_init_solution(uint64_t N, double *__restrict__ v, double *__restrict__ h,
                                                        double *__restrict__ hb, double *__restrict__ x, ColState cs,
-                                                       uint32_t k, uint32_t kshift) {
+                                                       uint32_t k, uint32_t kshift, const double *__restrict__ pt,
+                                                       double *__restrict__ Vpush, double *__restrict__ Ppush) {
     const uint32_t kp = 1u << kshift;
     const uint32_t j = threadIdx.x & (kp - 1);
     if (j >= k) return;
@@ -637,6 +672,10 @@ __global__ __launch_bounds__(BLK) void k_init_solution(uint64_t N, double *__res
             double z = v[q] * ia;
             v[q] = z;
             h[q] = z;
+            if (Vpush) { // ring push of v and p~ / alpha (bidiag.rs:459-462)
+                Vpush[q] = z;
+                Ppush[q] = pt[q] * ia;
+            }
         } else {
             h[q] = 0.0;
         }
@@ -727,7 +766,8 @@ __global__ __launch_bounds__(BLK) void k_alpha_step(const double *part, uint32_t
 // (bidiag.rs:505-507).
 __global__ __launch_bounds__(BLK) void k_update_solution(uint64_t N, double *__restrict__ v, double *__restrict__ h,
                                                          double *__restrict__ hb, double *__restrict__ x, ColState cs,
-                                                         uint32_t k, uint32_t kshift) {
+                                                         uint32_t k, uint32_t kshift, const double *__restrict__ pt,
+                                                         double *__restrict__ Vpush, double *__restrict__ Ppush) {
     const uint32_t kp = 1u << kshift;
     const uint32_t j = threadIdx.x & (kp - 1);
     if (j >= k || !cs.upd[j]) return;
@@ -740,6 +780,10 @@ __global__ __launch_bounds__(BLK) void k_update_solution(uint64_t N, double *__r
         if (scale) {
             vq *= ia;
             v[q] = vq;
+        }
+        if (Vpush) { // ring push of the normalised v and p~ / alpha (bidiag.rs:501-510)
+            Vpush[q] = vq;
+            Ppush[q] = pt[q] * ia;
         }
         const double h_old = h[q];
         const double hbq = h_old - t_hbar * hb[q];
@@ -1332,8 +1376,42 @@ void solve_batch_device(const Solver &s, const double *Y, uint64_t ldy, uint64_t
                                ws.v.as<double>(), nullptr, cs, part, k, kshift);
         }
         hipLaunchKernelGGL(k_alpha1, dim3(k), dim3(BLK), 0, st, part, gN, kp, cs);
-        hipLaunchKernelGGL(k_init_solution, dim3(gN), dim3(BLK), 0, st, N, ws.v.as<double>(), ws.h.as<double>(),
-                           ws.hb.as<double>(), ws.x.as<double>(), cs, k, kshift);
+        // windowed reorthogonalisation ring: capacity min(local_size, n_obs, n_dofs)
+        // (bidiag.rs:121-133); the write index is the same for every column
+        const uint64_t cap = opts.local_size > 0 ? std::min<uint64_t>((uint64_t)opts.local_size, std::min(n, N)) : 0;
+        DevBuf ringV, ringP, ringc;
+        uint64_t ring_next = 0, ring_count = 0;
+        if (cap) {
+            ringV.alloc(cap * N * k * 8);
+            ringP.alloc(cap * N * k * 8);
+            ringc.alloc(MAX_BATCH * 8);
+        }
+        auto ring_v = [&](uint64_t slot) { return ringV.as<double>() + slot * N * k; };
+        auto ring_p = [&](uint64_t slot) { return ringP.as<double>() + slot * N * k; };
+        auto ring_advance = [&]() {
+            const uint64_t slot = ring_next;
+            ring_next = (ring_next + 1) % cap;
+            if (ring_count < cap) ++ring_count;
+            return slot;
+        };
+        // one MGS sweep of v (and p~ in lockstep) over the stored slots, oldest first
+        auto ring_mgs = [&]() {
+            const uint64_t start = ring_count < cap ? 0 : ring_next;
+            for (uint64_t q = 0; q < ring_count; ++q) {
+                const uint64_t slot = (start + q) % cap;
+                hipLaunchKernelGGL(k_ring_dot, dim3(gN), dim3(BLK), 0, st, N, ws.v.as<double>(), ring_p(slot), cs,
+                                   part, k, kshift);
+                hipLaunchKernelGGL(k_reduce_to, dim3(k), dim3(BLK), 0, st, part, gN, kp, ringc.as<double>());
+                hipLaunchKernelGGL(k_ring_axpy, dim3(gN), dim3(BLK), 0, st, N, ws.v.as<double>(), ws.pt.as<double>(),
+                                   ring_v(slot), ring_p(slot), ringc.as<double>(), cs, k, kshift);
+            }
+        };
+        {
+            const uint64_t push = cap ? ring_advance() : 0;
+            hipLaunchKernelGGL(k_init_solution, dim3(gN), dim3(BLK), 0, st, N, ws.v.as<double>(), ws.h.as<double>(),
+                               ws.hb.as<double>(), ws.x.as<double>(), cs, k, kshift, ws.pt.as<double>(),
+                               cap ? ring_v(push) : nullptr, cap ? ring_p(push) : nullptr);
+        }
         WX_HIP(hipGetLastError());
 
         // --- iterations ---
@@ -1366,18 +1444,29 @@ void solve_batch_device(const Solver &s, const double *Y, uint64_t ldy, uint64_t
                 timer.begin(2);
                 schwarz_run->apply(ws.pt.as<double>(), ws.v.as<double>(), k, st, cs.active);
                 timer.end(2);
+                if (cap) ring_mgs();
                 hipLaunchKernelGGL(k_dots, dim3(gN), dim3(BLK), 0, st, N, ws.pt.as<double>(), ws.v.as<double>(), cs,
                                    part, k, kshift);
-            } else if (prec_kind == 1) {
-                hipLaunchKernelGGL((k_ptilde<1, false>), dim3(gN), dim3(BLK), 0, st, N, ws.atu.as<double>(),
-                                   ws.pt.as<double>(), ws.v.as<double>(), inv_diag, cs, part, k, kshift);
             } else {
-                hipLaunchKernelGGL((k_ptilde<0, false>), dim3(gN), dim3(BLK), 0, st, N, ws.atu.as<double>(),
-                                   ws.pt.as<double>(), ws.v.as<double>(), nullptr, cs, part, k, kshift);
+                if (prec_kind == 1)
+                    hipLaunchKernelGGL((k_ptilde<1, false>), dim3(gN), dim3(BLK), 0, st, N, ws.atu.as<double>(),
+                                       ws.pt.as<double>(), ws.v.as<double>(), inv_diag, cs, part, k, kshift);
+                else
+                    hipLaunchKernelGGL((k_ptilde<0, false>), dim3(gN), dim3(BLK), 0, st, N, ws.atu.as<double>(),
+                                       ws.pt.as<double>(), ws.v.as<double>(), nullptr, cs, part, k, kshift);
+                if (cap) { // reorthogonalise, then the alpha dot products of the updated pair
+                    ring_mgs();
+                    hipLaunchKernelGGL(k_dots, dim3(gN), dim3(BLK), 0, st, N, ws.pt.as<double>(), ws.v.as<double>(),
+                                       cs, part, k, kshift);
+                }
             }
             hipLaunchKernelGGL(k_alpha_step, dim3(k), dim3(BLK), 0, st, part, gN, kp, cs, opts.tol, itn);
-            hipLaunchKernelGGL(k_update_solution, dim3(gN), dim3(BLK), 0, st, N, ws.v.as<double>(), ws.h.as<double>(),
-                               ws.hb.as<double>(), ws.x.as<double>(), cs, k, kshift);
+            {
+                const uint64_t push = cap ? ring_advance() : 0;
+                hipLaunchKernelGGL(k_update_solution, dim3(gN), dim3(BLK), 0, st, N, ws.v.as<double>(),
+                                   ws.h.as<double>(), ws.hb.as<double>(), ws.x.as<double>(), cs, k, kshift,
+                                   ws.pt.as<double>(), cap ? ring_v(push) : nullptr, cap ? ring_p(push) : nullptr);
+            }
             timer.end(3);
             itn_done = itn;
             if (check_every == 1) {
