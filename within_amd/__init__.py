@@ -14,8 +14,11 @@ from ._within import (
     SolveResult,
     Solver,
     UnidentifiedDirection,
+    last_many_shape,
+    set_many_shape,
     solve,
     solve_batch,
+    solve_many,
 )
 from . import config  # noqa: F401
 
@@ -31,5 +34,8 @@ __all__ = [
     "UnidentifiedDirection",
     "solve",
     "solve_batch",
+    "solve_many",
+    "last_many_shape",
+    "set_many_shape",
     "config",
 ]

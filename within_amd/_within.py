@@ -677,3 +677,66 @@ def solve_batch(design, Y: np.ndarray, weights: Optional[np.ndarray] = None, opt
     r.time_setup += setup
     r.time_total = time.perf_counter() - t0
     return r
+
+
+def solve_many(solvers, Ys, options: Optional[LsmrOptions] = None) -> list:
+    """Solves many independent problems at once: ``solvers[p].solve_batch(Ys[p], options)``
+    for every p, with the problems iterating concurrently on the GPU (several streams,
+    captured iteration graphs, per-problem convergence; the launch shape is chosen by an
+    autotuner on the first call for a problem-count class).  Returns the list of
+    BatchSolveResult.  No counterpart in the reference, whose solves run one Solver at a time."""
+    t0 = time.perf_counter()
+    options = _check_options(options)
+    solvers = list(solvers)
+    Ys = [np.asfortranarray(_f64_2d("Y", Y)) for Y in Ys]
+    if len(solvers) != len(Ys):
+        raise ValueError(f"{len(solvers)} solvers but {len(Ys)} right-hand-side blocks")
+    P = len(solvers)
+    for p, (s, Y) in enumerate(zip(solvers, Ys)):
+        if not isinstance(s, Solver):
+            raise TypeError("solve_many takes Solver objects")
+        if Y.shape[0] != s.n_obs:
+            raise ValueError(f"problem {p}: Y has {Y.shape[0]} rows, expected n_obs={s.n_obs}")
+    Xs = [np.empty((s.n_dofs, Y.shape[1]), order="F") for s, Y in zip(solvers, Ys)]
+    Ds = [np.empty(Y.shape, order="F") for Y in Ys]
+    stats = [(_lib.RhsStats * max(1, Y.shape[1]))() for Y in Ys]
+    P_ = max(P, 1)
+    hs = (C.c_void_p * P_)(*[s._h for s in solvers])
+    yp = (C.c_void_p * P_)(*[Y.ctypes.data for Y in Ys])
+    ld = (C.c_uint64 * P_)(*[Y.shape[0] for Y in Ys])
+    kk = (C.c_uint64 * P_)(*[Y.shape[1] for Y in Ys])
+    xp = (C.c_void_p * P_)(*[X.ctypes.data for X in Xs])
+    dp = (C.c_void_p * P_)(*[D.ctypes.data for D in Ds])
+    sp = (C.c_void_p * P_)(*[C.cast(st, C.c_void_p) for st in stats])
+    _lib.check(_lib.lib().wx_solve_many(hs, P, yp, ld, kk, C.byref(options._native()), xp, dp, sp))
+    out = []
+    total = time.perf_counter() - t0
+    for s, X, D, st, Y in zip(solvers, Xs, Ds, stats, Ys):
+        sl = [st[j] for j in range(Y.shape[1])]
+        r = BatchSolveResult()
+        r.x = X
+        r.demeaned = D
+        r.unidentified = list(s._unidentified)
+        r.layout = s._layout
+        r.converged = [bool(q.converged) for q in sl]
+        r.iterations = [int(q.iterations) for q in sl]
+        r.residual = [float(q.residual) for q in sl]
+        r.true_residual = [float(q.true_residual) for q in sl]
+        r.stop_reason = [_lib.STOP_REASONS.get(int(q.stop_reason), str(q.stop_reason)) for q in sl]
+        r.time_solve = [float(q.time_solve) for q in sl]
+        r.time_setup = 0.0
+        r.time_total = total
+        out.append(r)
+    return out
+
+
+def last_many_shape() -> dict:
+    """The launch shape the autotuner picked for the last solve_many call."""
+    s, g, gr = C.c_int32(), C.c_int32(), C.c_int32()
+    _lib.lib().wx_last_many_shape(C.byref(s), C.byref(g), C.byref(gr))
+    return {"streams": s.value, "iterations_per_graph": g.value, "graph": bool(gr.value)}
+
+
+def set_many_shape(streams: int = 0, iterations_per_graph: int = 1, graph: bool = True) -> None:
+    """Fixes the launch shape of later solve_many calls; streams=0 returns it to the autotuner."""
+    _lib.lib().wx_set_many_shape(int(streams), int(iterations_per_graph), 1 if graph else 0)

@@ -329,6 +329,86 @@ wx_status wx_solver_solve_batch(const wx_solver *s, const double *Y, uint64_t ld
     });
 }
 
+wx_status wx_solve_many_device(const wx_solver *const *solvers, uint64_t n_problems, const double *const *Y,
+                               const uint64_t *ldy, const uint64_t *k, const wx_lsmr_options *opts, double *const *X,
+                               double *const *demeaned, wx_rhs_stats *const *stats, void *stream) {
+    return guard([&] {
+        if (n_problems && (!solvers || !Y || !ldy || !k || !stats)) fail(WX_ERR_INVALID_INPUT, "null table pointer");
+        std::vector<const Solver *> sv(n_problems);
+        std::vector<const double *> yv(n_problems);
+        std::vector<uint64_t> lv(n_problems), kv(n_problems);
+        std::vector<double *> xv(n_problems), dv(n_problems);
+        std::vector<wx_rhs_stats *> stv(n_problems);
+        for (uint64_t p = 0; p < n_problems; ++p) {
+            if (!solvers[p] || (!Y[p] && k[p]) || !stats[p])
+                fail(WX_ERR_INVALID_INPUT, "null solver, response or stats pointer of problem " + std::to_string(p));
+            sv[p] = &solvers[p]->s;
+            yv[p] = Y[p];
+            lv[p] = ldy[p];
+            kv[p] = k[p];
+            xv[p] = X ? X[p] : nullptr;
+            dv[p] = demeaned ? demeaned[p] : nullptr;
+            stv[p] = stats[p];
+        }
+        wx_lsmr_options o;
+        if (opts) o = *opts;
+        else wx_lsmr_options_default(&o);
+        Stream st(stream);
+        solve_many_device(sv, yv, lv, kv, o, xv, dv, stv, st.s);
+    });
+}
+
+wx_status wx_solve_many(const wx_solver *const *solvers, uint64_t n_problems, const double *const *Y,
+                        const uint64_t *ldy, const uint64_t *k, const wx_lsmr_options *opts, double *const *X,
+                        double *const *demeaned, wx_rhs_stats *const *stats) {
+    return guard([&] {
+        if (n_problems && (!solvers || !Y || !ldy || !k || !stats)) fail(WX_ERR_INVALID_INPUT, "null table pointer");
+        std::vector<DevBuf> dY(n_problems), dX(n_problems), dD(n_problems);
+        std::vector<const wx_solver *> sv(n_problems);
+        std::vector<const double *> yv(n_problems);
+        std::vector<uint64_t> lv(n_problems);
+        std::vector<double *> xv(n_problems, nullptr), dv(n_problems, nullptr);
+        Stream st(nullptr);
+        for (uint64_t p = 0; p < n_problems; ++p) {
+            const wx_solver *s = solvers[p];
+            if (!s || (!Y[p] && k[p]) || !stats[p])
+                fail(WX_ERR_INVALID_INPUT, "null solver, response or stats pointer of problem " + std::to_string(p));
+            const uint64_t n = s->s.design.n_obs, N = s->s.design.n_dofs;
+            if (ldy[p] < n) fail(WX_ERR_INVALID_INPUT, "leading dimension of Y is smaller than n_obs");
+            for (uint64_t j = 0; j < k[p]; ++j)
+                for (uint64_t i = 0; i < n; ++i)
+                    if (!std::isfinite(Y[p][j * ldy[p] + i]))
+                        fail(WX_ERR_INVALID_INPUT, "Solver::solve: response at index " + std::to_string(i) +
+                                                       " must be finite, got " + std::to_string(Y[p][j * ldy[p] + i]));
+            dY[p].alloc(n * std::max<uint64_t>(k[p], 1) * 8);
+            for (uint64_t j = 0; j < k[p]; ++j)
+                WX_HIP(hipMemcpyAsync(dY[p].as<double>() + j * n, Y[p] + j * ldy[p], n * 8, hipMemcpyHostToDevice, st.s));
+            if (X && X[p]) dX[p].alloc(N * std::max<uint64_t>(k[p], 1) * 8), xv[p] = dX[p].as<double>();
+            if (demeaned && demeaned[p]) dD[p].alloc(n * std::max<uint64_t>(k[p], 1) * 8), dv[p] = dD[p].as<double>();
+            sv[p] = s;
+            yv[p] = dY[p].as<double>();
+            lv[p] = n;
+        }
+        const wx_status rc = wx_solve_many_device(sv.data(), n_problems, yv.data(), lv.data(), k, opts, xv.data(),
+                                                  dv.data(), stats, st.s);
+        if (rc != WX_OK) fail(rc, wx_last_error());
+        for (uint64_t p = 0; p < n_problems; ++p) {
+            const uint64_t n = solvers[p]->s.design.n_obs, N = solvers[p]->s.design.n_dofs;
+            if (xv[p]) WX_HIP(hipMemcpyAsync(X[p], xv[p], N * k[p] * 8, hipMemcpyDeviceToHost, st.s));
+            if (dv[p]) WX_HIP(hipMemcpyAsync(demeaned[p], dv[p], n * k[p] * 8, hipMemcpyDeviceToHost, st.s));
+        }
+        WX_HIP(hipStreamSynchronize(st.s));
+    });
+}
+
+void wx_set_many_shape(int32_t streams, int32_t iterations_per_graph, int32_t graph) {
+    force_many_shape(streams, iterations_per_graph, graph);
+}
+
+void wx_last_many_shape(int32_t *streams, int32_t *iterations_per_graph, int32_t *graph) {
+    last_many_shape(streams, iterations_per_graph, graph);
+}
+
 wx_status wx_solver_apply_operator(const wx_solver *s, int32_t adjoint, const double *X, uint64_t k, double *Y) {
     return guard([&] {
         if (!s || (k && (!X || !Y))) fail(WX_ERR_INVALID_INPUT, "null pointer");

@@ -13,6 +13,7 @@
 #include <cmath>
 #include <cstring>
 #include <numeric>
+#include <thread>
 
 #include "schwarz.hpp"
 #include "solver.hpp"
@@ -686,8 +687,10 @@ __global__ __launch_bounds__(BLK) void k_init_solution(uint64_t N, double *__res
 // coefficients and the dual stopping test.  Follows lsmr_from_bidiag (lsmr.rs:323-361),
 // LsmrRecurrenceState::step (recurrence.rs:81-107), SolutionState::update's coefficients
 // (recurrence.rs:143-161) and ConvergenceState::{observe, check} (recurrence.rs:239-255).
+// The iteration count is kept per column on the device, so the launch is identical every
+// iteration and a captured graph can replay it.
 __global__ __launch_bounds__(BLK) void k_alpha_step(const double *part, uint32_t nb, uint32_t kp, ColState cs,
-                                                    double rel_tol, int32_t itn) {
+                                                    double rel_tol) {
     __shared__ double sh[BLK];
     const uint32_t j = blockIdx.x;
     double vp = reduce_partials(part, nb, kp, 3, 0, j, sh, false);
@@ -708,7 +711,7 @@ __global__ __launch_bounds__(BLK) void k_alpha_step(const double *part, uint32_t
             cs.err[j] = 1;
             cs.active[j] = 0;
             cs.upd[j] = 0;
-            cs.iters[j] = itn;
+            cs.iters[j] += 1; // steps taken (reset by set_beta1)
             return;
         }
     }
@@ -755,7 +758,7 @@ __global__ __launch_bounds__(BLK) void k_alpha_step(const double *part, uint32_t
         if (normar / (a_norm * fmax(residual, 2.2250738585072014e-308)) <= rel_tol) stop = WX_STOP_NORMAL_EQ_TOLERANCE;
     }
     cs.upd[j] = 1;
-    cs.iters[j] = itn;
+    cs.iters[j] += 1; // steps taken (reset by set_beta1)
     if (stop >= 0) {
         cs.stop[j] = stop;
         cs.active[j] = 0;
@@ -1224,11 +1227,23 @@ struct Workspace {
     DevBuf u, v, pt, atu, h, hb, x, yint, part, segpart, cols, ints, bad, red;
     ColState cs{};
     std::unique_ptr<SchwarzRun> runs[2];
-    // pinned copies of the per-column "active" flags of the last two iterations, and the
-    // events that mark them complete (the host polls one iteration behind the device)
-    int32_t *hflags = nullptr;
+    // pinned copies of the per-column "active" flags of the last two iteration groups, and
+    // the events that mark them complete (the host polls one group behind the device);
+    // hbad: the right-hand-side load's (non-finite count, needs-scaled-norm) flags
+    int32_t *hflags = nullptr, *hbad = nullptr;
     hipEvent_t flag_ev[2] = {nullptr, nullptr};
+    // captured iteration group (replayed by later solves on this workspace), its key
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t graph_exec = nullptr;
+    uint64_t graph_key = 0;
+    void drop_graph() {
+        if (graph_exec) (void)hipGraphExecDestroy(graph_exec);
+        if (graph) (void)hipGraphDestroy(graph);
+        graph_exec = nullptr;
+        graph = nullptr;
+    }
     ~Workspace() {
+        drop_graph();
         if (hflags) (void)hipHostFree(hflags);
         for (auto &e : flag_ev)
             if (e) (void)hipEventDestroy(e);
@@ -1263,7 +1278,8 @@ std::shared_ptr<Workspace> new_workspace(const DevDesign &dv, uint32_t k) {
     w->segpart.alloc((size_t)std::max<uint32_t>(dv.max_partials, 1) * k * 8);
     w->bad.alloc(8);
     w->red.alloc(2 * MAX_BATCH * 8);
-    WX_HIP(hipHostMalloc((void **)&w->hflags, 2 * MAX_BATCH * sizeof(int32_t), hipHostMallocDefault));
+    WX_HIP(hipHostMalloc((void **)&w->hflags, (2 * MAX_BATCH + 2) * sizeof(int32_t), hipHostMallocDefault));
+    w->hbad = w->hflags + 2 * MAX_BATCH;
     for (auto &e : w->flag_ev) WX_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
     return w;
 }
@@ -1299,251 +1315,202 @@ struct PooledWorkspace {
 // ---------------------------------------------------------------------------
 // Batched solve (Solver::solve_batch, crates/within/src/solver.rs:423-577)
 // ---------------------------------------------------------------------------
-void solve_batch_device(const Solver &s, const double *Y, uint64_t ldy, uint64_t k_total, const wx_lsmr_options &opts,
-                        double *X, double *Dm, wx_rhs_stats *stats, hipStream_t st, bool x_solver_basis) {
-    const DevDesign &dv = s.dev;
-    const uint64_t n = dv.n, N = dv.N;
-    if (!std::isfinite(opts.tol) || opts.tol < 0.0)
-        fail(WX_ERR_INVALID_INPUT, "lsmr: tolerance must be finite and nonnegative, got " + std::to_string(opts.tol));
-    if (ldy < n) fail(WX_ERR_INVALID_INPUT, "leading dimension of Y is smaller than n_obs");
-    if (k_total == 0) return;
-    const DesignArgs da = dv.args();
-    const bool weighted = dv.sw.p != nullptr;
-    const uint32_t *inv_perm = dv.inv_perm.p ? dv.inv_perm.as<uint32_t>() : nullptr;
-    const int maxiter = (int)std::max<int64_t>(0, opts.maxiter);
-    const Precond *pc = s.precond.get();
-    const int prec_kind = pc == nullptr ? 0 : (pc->kind == Precond::DIAGONAL ? 1 : 2);
-    const double *inv_diag = prec_kind == 1 ? dv.inv_diag.as<double>() : nullptr;
-    const int prec_slot = opts.precision == WX_PRECISION_MIXED ? 1 : 0;
-    PhaseTimer timer;
-    timer.on = opts.timing != 0;
-    timer.st = st;
+namespace {
 
-    for (uint64_t c0 = 0; c0 < k_total; c0 += MAX_BATCH) {
-        const uint32_t k = (uint32_t)std::min<uint64_t>(MAX_BATCH, k_total - c0);
-        const uint32_t kshift = pow2_shift(k);
-        const uint32_t kp = 1u << kshift;
-        auto t0 = std::chrono::steady_clock::now();
-        PooledWorkspace pw(s, k);
-        Workspace &ws = *pw.ws;
-        if (prec_kind == 2 && !ws.runs[prec_slot]) ws.runs[prec_slot] = make_schwarz_run(*pc, opts.precision, st);
-        SchwarzRun *schwarz_run = prec_kind == 2 ? ws.runs[prec_slot].get() : nullptr;
-        ColState cs = ws.cs;
-        const uint32_t gn = grid_for(n, kshift), gN = grid_for(N, kshift);
-        double *part = ws.part.as<double>();
-        const double *Yc = Y + c0 * ldy;
+// One chunk (<= MAX_BATCH right-hand sides) of a batched solve as a resumable run, so a
+// driver can interleave many independent runs on several streams (solve_many_device):
+//   launch_load()  right-hand side load and beta_1 partials (asynchronous)
+//   finish_init()  waits for the load's validation flags, then the bidiagonalisation init
+//   launch_group() one group of `group` iterations: the first iteration directly, later
+//                  groups as one replay of a captured hipGraph; the per-column active flags
+//                  of the group are copied to pinned memory behind an event
+//   poll(wait)     the flags of the group before the last one (lag one): 1 when no column
+//                  is active any more, 0 when one is, -1 when not yet known
+//   finish()       demeaned output, x, FP64 true residual, statistics (synchronous)
+class ChunkRun {
+  public:
+    ChunkRun(const Solver &s, const double *Y, uint64_t ldy, uint64_t c0, uint32_t k, const wx_lsmr_options &opts,
+             double *X, double *Dm, wx_rhs_stats *stats, hipStream_t st, bool x_solver_basis, PhaseTimer *timer,
+             int group, bool use_graph)
+        : s_(s), dv_(s.dev), da_(s.dev.args()), Y_(Y + c0 * ldy), ldy_(ldy), c0_(c0), k_(k), opts_(opts), X_(X),
+          Dm_(Dm), stats_(stats), st_(st), x_solver_basis_(x_solver_basis), timer_(timer), pw_(s, k) {
+        n_ = dv_.n;
+        N_ = dv_.N;
+        weighted_ = dv_.sw.p != nullptr;
+        inv_perm_ = dv_.inv_perm.p ? dv_.inv_perm.as<uint32_t>() : nullptr;
+        maxiter_ = (int)std::max<int64_t>(0, opts.maxiter);
+        const Precond *pc = s.precond.get();
+        prec_kind_ = pc == nullptr ? 0 : (pc->kind == Precond::DIAGONAL ? 1 : 2);
+        inv_diag_ = prec_kind_ == 1 ? dv_.inv_diag.as<double>() : nullptr;
+        const int prec_slot = opts.precision == WX_PRECISION_MIXED ? 1 : 0;
+        kshift_ = pow2_shift(k);
+        kp_ = 1u << kshift_;
+        t0_ = std::chrono::steady_clock::now();
+        ws_ = pw_.ws.get();
+        if (prec_kind_ == 2 && !ws_->runs[prec_slot]) ws_->runs[prec_slot] = make_schwarz_run(*pc, opts.precision, st);
+        schwarz_ = prec_kind_ == 2 ? ws_->runs[prec_slot].get() : nullptr;
+        cs_ = ws_->cs;
+        gn_ = grid_for(n_, kshift_);
+        gN_ = grid_for(N_, kshift_);
+        part_ = ws_->part.as<double>();
+        gt_ = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>((n_ + 63) / 64, MAX_PART_BLOCKS));
+        tile_bytes_ = (size_t)64 * (k + 1) * sizeof(double);
+        cap_ = opts.local_size > 0 ? std::min<uint64_t>((uint64_t)opts.local_size, std::min(n_, N_)) : 0;
+        group_ = std::max(1, group);
+        // a graph replays fixed launch arguments: not with the reorthogonalisation ring (its
+        // slot changes every iteration) nor with the event timer
+        use_graph_ = use_graph && cap_ == 0 && !(timer_ && timer_->on);
+        graph_key_ = (uint64_t)group_ * 4 + (uint64_t)prec_slot;
+    }
 
-        // --- right-hand side: y_int (internal order), beta_1 = vec_norm(sqrt(W) y) ---
-        WX_HIP(hipMemsetAsync(ws.bad.p, 0, 8, st));
-        const uint32_t gt = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>((n + 63) / 64, MAX_PART_BLOCKS));
-        const size_t tile_bytes = (size_t)64 * (k + 1) * sizeof(double);
-        int32_t *badp = ws.bad.as<int32_t>();
-        if (weighted)
-            hipLaunchKernelGGL((k_load_rhs<true>), dim3(gt), dim3(BLK), tile_bytes, st, n, inv_perm,
-                               inv_perm ? dv.sw_caller.as<double>() : dv.sw.as<double>(), Yc, ldy,
-                               ws.yint.as<double>(), part, badp, k, kshift);
+    void launch_load() {
+        WX_HIP(hipMemsetAsync(ws_->bad.p, 0, 8, st_));
+        int32_t *badp = ws_->bad.as<int32_t>();
+        if (weighted_)
+            hipLaunchKernelGGL((k_load_rhs<true>), dim3(gt_), dim3(BLK), tile_bytes_, st_, n_, inv_perm_,
+                               inv_perm_ ? dv_.sw_caller.as<double>() : dv_.sw.as<double>(), Y_, ldy_,
+                               ws_->yint.as<double>(), part_, badp, k_, kshift_);
         else
-            hipLaunchKernelGGL((k_load_rhs<false>), dim3(gt), dim3(BLK), tile_bytes, st, n, inv_perm, nullptr, Yc,
-                               ldy, ws.yint.as<double>(), part, badp, k, kshift);
-        hipLaunchKernelGGL(k_beta1, dim3(k), dim3(BLK), 0, st, part, gt, kp, cs, opts.tol, badp + 1);
-        int32_t hbad[2] = {0, 0};
-        WX_HIP(hipMemcpyAsync(hbad, ws.bad.p, 8, hipMemcpyDeviceToHost, st));
-        WX_HIP(hipStreamSynchronize(st));
-        if (hbad[0]) fail(WX_ERR_INVALID_INPUT, "Solver::solve: response contains non-finite values");
-        if (hbad[1]) { // a column outside the safe range of the unscaled sum of squares
-            if (weighted)
-                hipLaunchKernelGGL((k_scaled_sumsq<true>), dim3(gn), dim3(BLK), 0, st, n, dv.sw.as<double>(),
-                                   ws.yint.as<double>(), cs, part, k, kshift);
-            else
-                hipLaunchKernelGGL((k_scaled_sumsq<false>), dim3(gn), dim3(BLK), 0, st, n, nullptr,
-                                   ws.yint.as<double>(), cs, part, k, kshift);
-            hipLaunchKernelGGL(k_beta1_scaled, dim3(k), dim3(BLK), 0, st, part, gn, kp, cs, opts.tol);
-        }
+            hipLaunchKernelGGL((k_load_rhs<false>), dim3(gt_), dim3(BLK), tile_bytes_, st_, n_, inv_perm_, nullptr, Y_,
+                               ldy_, ws_->yint.as<double>(), part_, badp, k_, kshift_);
+        hipLaunchKernelGGL(k_beta1, dim3(k_), dim3(BLK), 0, st_, part_, gt_, kp_, cs_, opts_.tol, badp + 1);
+        WX_HIP(hipMemcpyAsync(ws_->hbad, ws_->bad.p, 8, hipMemcpyDeviceToHost, st_));
+        WX_HIP(hipGetLastError());
+    }
 
-        // --- init: p~ = A'b / beta_1, v = M^{-1} p~, alpha_1 ---
-        launch_adjoint(dv, da, weighted ? 2 : 0, ws.yint.as<double>(),
-                       ws.pt.as<double>(), ws.segpart.as<double>(), cs.active, k, kshift, st);
-        if (prec_kind == 2) {
-            hipLaunchKernelGGL((k_ptilde<2, true>), dim3(gN), dim3(BLK), 0, st, N, nullptr, ws.pt.as<double>(),
-                               ws.v.as<double>(), nullptr, cs, part, k, kshift);
-            schwarz_run->apply(ws.pt.as<double>(), ws.v.as<double>(), k, st, cs.active);
-            hipLaunchKernelGGL(k_dots, dim3(gN), dim3(BLK), 0, st, N, ws.pt.as<double>(), ws.v.as<double>(), cs, part, k, kshift);
-        } else if (prec_kind == 1) {
-            hipLaunchKernelGGL((k_ptilde<1, true>), dim3(gN), dim3(BLK), 0, st, N, nullptr, ws.pt.as<double>(),
-                               ws.v.as<double>(), inv_diag, cs, part, k, kshift);
-        } else {
-            hipLaunchKernelGGL((k_ptilde<0, true>), dim3(gN), dim3(BLK), 0, st, N, nullptr, ws.pt.as<double>(),
-                               ws.v.as<double>(), nullptr, cs, part, k, kshift);
+    void finish_init() {
+        WX_HIP(hipStreamSynchronize(st_));
+        if (ws_->hbad[0]) fail(WX_ERR_INVALID_INPUT, "Solver::solve: response contains non-finite values");
+        if (ws_->hbad[1]) { // a column outside the safe range of the unscaled sum of squares
+            if (weighted_)
+                hipLaunchKernelGGL((k_scaled_sumsq<true>), dim3(gn_), dim3(BLK), 0, st_, n_, dv_.sw.as<double>(),
+                                   ws_->yint.as<double>(), cs_, part_, k_, kshift_);
+            else
+                hipLaunchKernelGGL((k_scaled_sumsq<false>), dim3(gn_), dim3(BLK), 0, st_, n_, nullptr,
+                                   ws_->yint.as<double>(), cs_, part_, k_, kshift_);
+            hipLaunchKernelGGL(k_beta1_scaled, dim3(k_), dim3(BLK), 0, st_, part_, gn_, kp_, cs_, opts_.tol);
         }
-        hipLaunchKernelGGL(k_alpha1, dim3(k), dim3(BLK), 0, st, part, gN, kp, cs);
+        // init: p~ = A'b / beta_1, v = M^{-1} p~, alpha_1
+        launch_adjoint(dv_, da_, weighted_ ? 2 : 0, ws_->yint.as<double>(), ws_->pt.as<double>(),
+                       ws_->segpart.as<double>(), cs_.active, k_, kshift_, st_);
+        const uint64_t N = N_;
+        if (prec_kind_ == 2) {
+            hipLaunchKernelGGL((k_ptilde<2, true>), dim3(gN_), dim3(BLK), 0, st_, N, nullptr, ws_->pt.as<double>(),
+                               ws_->v.as<double>(), nullptr, cs_, part_, k_, kshift_);
+            schwarz_->apply(ws_->pt.as<double>(), ws_->v.as<double>(), k_, st_, cs_.active);
+            hipLaunchKernelGGL(k_dots, dim3(gN_), dim3(BLK), 0, st_, N, ws_->pt.as<double>(), ws_->v.as<double>(), cs_,
+                               part_, k_, kshift_);
+        } else if (prec_kind_ == 1) {
+            hipLaunchKernelGGL((k_ptilde<1, true>), dim3(gN_), dim3(BLK), 0, st_, N, nullptr, ws_->pt.as<double>(),
+                               ws_->v.as<double>(), inv_diag_, cs_, part_, k_, kshift_);
+        } else {
+            hipLaunchKernelGGL((k_ptilde<0, true>), dim3(gN_), dim3(BLK), 0, st_, N, nullptr, ws_->pt.as<double>(),
+                               ws_->v.as<double>(), nullptr, cs_, part_, k_, kshift_);
+        }
+        hipLaunchKernelGGL(k_alpha1, dim3(k_), dim3(BLK), 0, st_, part_, gN_, kp_, cs_);
         // windowed reorthogonalisation ring: capacity min(local_size, n_obs, n_dofs)
         // (bidiag.rs:121-133); the write index is the same for every column
-        const uint64_t cap = opts.local_size > 0 ? std::min<uint64_t>((uint64_t)opts.local_size, std::min(n, N)) : 0;
-        DevBuf ringV, ringP, ringc;
-        uint64_t ring_next = 0, ring_count = 0;
-        if (cap) {
-            ringV.alloc(cap * N * k * 8);
-            ringP.alloc(cap * N * k * 8);
-            ringc.alloc(MAX_BATCH * 8);
+        if (cap_) {
+            ringV_.alloc(cap_ * N_ * k_ * 8);
+            ringP_.alloc(cap_ * N_ * k_ * 8);
+            ringc_.alloc(MAX_BATCH * 8);
         }
-        auto ring_v = [&](uint64_t slot) { return ringV.as<double>() + slot * N * k; };
-        auto ring_p = [&](uint64_t slot) { return ringP.as<double>() + slot * N * k; };
-        auto ring_advance = [&]() {
-            const uint64_t slot = ring_next;
-            ring_next = (ring_next + 1) % cap;
-            if (ring_count < cap) ++ring_count;
-            return slot;
-        };
-        // one MGS sweep of v (and p~ in lockstep) over the stored slots, oldest first
-        auto ring_mgs = [&]() {
-            const uint64_t start = ring_count < cap ? 0 : ring_next;
-            for (uint64_t q = 0; q < ring_count; ++q) {
-                const uint64_t slot = (start + q) % cap;
-                hipLaunchKernelGGL(k_ring_dot, dim3(gN), dim3(BLK), 0, st, N, ws.v.as<double>(), ring_p(slot), cs,
-                                   part, k, kshift);
-                hipLaunchKernelGGL(k_reduce_to, dim3(k), dim3(BLK), 0, st, part, gN, kp, ringc.as<double>());
-                hipLaunchKernelGGL(k_ring_axpy, dim3(gN), dim3(BLK), 0, st, N, ws.v.as<double>(), ws.pt.as<double>(),
-                                   ring_v(slot), ring_p(slot), ringc.as<double>(), cs, k, kshift);
-            }
-        };
-        {
-            const uint64_t push = cap ? ring_advance() : 0;
-            hipLaunchKernelGGL(k_init_solution, dim3(gN), dim3(BLK), 0, st, N, ws.v.as<double>(), ws.h.as<double>(),
-                               ws.hb.as<double>(), ws.x.as<double>(), cs, k, kshift, ws.pt.as<double>(),
-                               cap ? ring_v(push) : nullptr, cap ? ring_p(push) : nullptr);
-        }
+        const uint64_t push = cap_ ? ring_advance() : 0;
+        hipLaunchKernelGGL(k_init_solution, dim3(gN_), dim3(BLK), 0, st_, N, ws_->v.as<double>(), ws_->h.as<double>(),
+                           ws_->hb.as<double>(), ws_->x.as<double>(), cs_, k_, kshift_, ws_->pt.as<double>(),
+                           cap_ ? ring_v(push) : nullptr, cap_ ? ring_p(push) : nullptr);
         WX_HIP(hipGetLastError());
+    }
 
-        // --- iterations ---
-        int32_t host_active[MAX_BATCH];
-        const int check_every = opts.check_every > 0 ? opts.check_every : 1;
-        int itn_done = 0;
-        for (int itn = 1; itn <= maxiter; ++itn) {
-            timer.begin(3);
-            timer.begin(0);
-            const double *uin = itn == 1 ? ws.yint.as<double>() : ws.u.as<double>();
-            uint32_t gpart = gn;
-            if (dv.fuse_term >= 0)
-                gpart = launch_gather_seg(dv, da, itn == 1, ws.v.as<double>(), uin, ws.u.as<double>(),
-                                          ws.atu.as<double>(), ws.segpart.as<double>(), cs, part, k, kshift, st);
-            else
-                launch_gather(dv, da, itn == 1, gn, ws.v.as<double>(), uin, ws.u.as<double>(), cs, part, k, kshift, st);
-            timer.end(0);
-            hipLaunchKernelGGL(k_beta, dim3(k), dim3(BLK), 0, st, part, gpart, kp, cs);
-            timer.begin(1);
-            // split levels of the fused term first: the other terms reuse the partial buffer
-            if (dv.fuse_term >= 0)
-                launch_segfix(dv.terms[dv.fuse_term], ws.segpart.as<double>(), ws.atu.as<double>(), cs.active, k,
-                              kshift, st);
-            launch_adjoint(dv, da, weighted ? 1 : 0, ws.u.as<double>(),
-                           ws.atu.as<double>(), ws.segpart.as<double>(), cs.active, k, kshift, st, dv.fuse_term);
-            timer.end(1);
-            if (prec_kind == 2) {
-                hipLaunchKernelGGL((k_ptilde<2, false>), dim3(gN), dim3(BLK), 0, st, N, ws.atu.as<double>(),
-                                   ws.pt.as<double>(), ws.v.as<double>(), nullptr, cs, part, k, kshift);
-                timer.begin(2);
-                schwarz_run->apply(ws.pt.as<double>(), ws.v.as<double>(), k, st, cs.active);
-                timer.end(2);
-                if (cap) ring_mgs();
-                hipLaunchKernelGGL(k_dots, dim3(gN), dim3(BLK), 0, st, N, ws.pt.as<double>(), ws.v.as<double>(), cs,
-                                   part, k, kshift);
-            } else {
-                if (prec_kind == 1)
-                    hipLaunchKernelGGL((k_ptilde<1, false>), dim3(gN), dim3(BLK), 0, st, N, ws.atu.as<double>(),
-                                       ws.pt.as<double>(), ws.v.as<double>(), inv_diag, cs, part, k, kshift);
-                else
-                    hipLaunchKernelGGL((k_ptilde<0, false>), dim3(gN), dim3(BLK), 0, st, N, ws.atu.as<double>(),
-                                       ws.pt.as<double>(), ws.v.as<double>(), nullptr, cs, part, k, kshift);
-                if (cap) { // reorthogonalise, then the alpha dot products of the updated pair
-                    ring_mgs();
-                    hipLaunchKernelGGL(k_dots, dim3(gN), dim3(BLK), 0, st, N, ws.pt.as<double>(), ws.v.as<double>(),
-                                       cs, part, k, kshift);
-                }
-            }
-            hipLaunchKernelGGL(k_alpha_step, dim3(k), dim3(BLK), 0, st, part, gN, kp, cs, opts.tol, itn);
-            {
-                const uint64_t push = cap ? ring_advance() : 0;
-                hipLaunchKernelGGL(k_update_solution, dim3(gN), dim3(BLK), 0, st, N, ws.v.as<double>(),
-                                   ws.h.as<double>(), ws.hb.as<double>(), ws.x.as<double>(), cs, k, kshift,
-                                   ws.pt.as<double>(), cap ? ring_v(push) : nullptr, cap ? ring_p(push) : nullptr);
-            }
-            timer.end(3);
-            itn_done = itn;
-            if (check_every == 1) {
-                // lag-one polling: record this iteration's flags, then wait for the previous
-                // iteration's while the device already runs this one
-                const int slot = itn & 1;
-                WX_HIP(hipMemcpyAsync(ws.hflags + slot * MAX_BATCH, cs.active, k * 4, hipMemcpyDeviceToHost, st));
-                WX_HIP(hipEventRecord(ws.flag_ev[slot], st));
-                if (itn >= 2) {
-                    WX_HIP(hipEventSynchronize(ws.flag_ev[slot ^ 1]));
-                    bool any = false;
-                    for (uint32_t j = 0; j < k; ++j) any |= ws.hflags[(slot ^ 1) * MAX_BATCH + j] != 0;
-                    if (!any) break;
-                }
-            } else if (itn % check_every == 0 || itn == maxiter) {
-                WX_HIP(hipGetLastError());
-                WX_HIP(hipMemcpyAsync(host_active, cs.active, k * 4, hipMemcpyDeviceToHost, st));
-                WX_HIP(hipStreamSynchronize(st));
-                bool any = false;
-                for (uint32_t j = 0; j < k; ++j) any |= host_active[j] != 0;
-                if (!any) break;
-            }
+    bool exhausted() const { return itn_ >= maxiter_; }
+    int launches() const { return launches_; }
+
+    void launch_group() {
+        if (itn_ == 0) {
+            iteration(true);
+            ++itn_;
+        } else if (use_graph_ && itn_ + group_ <= maxiter_) {
+            replay_graph();
+            itn_ += group_;
+        } else {
+            const int g = std::min(group_, maxiter_ - itn_);
+            for (int q = 0; q < g; ++q) iteration(false);
+            itn_ += g;
         }
+        const int slot = launches_ & 1;
+        WX_HIP(hipMemcpyAsync(ws_->hflags + slot * MAX_BATCH, cs_.active, k_ * 4, hipMemcpyDeviceToHost, st_));
+        WX_HIP(hipEventRecord(ws_->flag_ev[slot], st_));
+        ++launches_;
         WX_HIP(hipGetLastError());
-        (void)itn_done;
+    }
 
-        // --- outputs: x, demeaned (caller order), FP64 true residual ---
+    int poll(bool wait) {
+        if (launches_ < 2) return 0;
+        const int slot = (launches_ - 2) & 1;
+        if (wait) WX_HIP(hipEventSynchronize(ws_->flag_ev[slot]));
+        else {
+            const hipError_t q = hipEventQuery(ws_->flag_ev[slot]);
+            if (q == hipErrorNotReady) return -1;
+            WX_HIP(q);
+        }
+        for (uint32_t j = 0; j < k_; ++j)
+            if (ws_->hflags[slot * MAX_BATCH + j]) return 0;
+        return 1;
+    }
+
+    void finish() {
+        const DesignArgs &da = da_;
+        const uint64_t n = n_, N = N_;
+        const uint32_t k = k_, kshift = kshift_, kp = kp_;
+        hipStream_t st = st_;
+        Workspace &ws = *ws_;
         double *dm_int = ws.u.as<double>();
         const uint32_t gd = grid_for(n, kshift, 65535);
-        launch_demean(dv, da, gd, ws.x.as<double>(), ws.yint.as<double>(), dm_int, k, kshift, st);
-        if (Dm)
-            hipLaunchKernelGGL(k_store_caller, dim3(gt), dim3(BLK), tile_bytes, st, n, inv_perm, dm_int, Dm + c0 * n,
-                               n, k, kshift);
-        if (X && (!s.reparam || x_solver_basis)) {
-            uint64_t tot = N * k;
+        launch_demean(dv_, da, gd, ws.x.as<double>(), ws.yint.as<double>(), dm_int, k, kshift, st);
+        if (Dm_)
+            hipLaunchKernelGGL(k_store_caller, dim3(gt_), dim3(BLK), tile_bytes_, st, n, inv_perm_, dm_int,
+                               Dm_ + c0_ * n, n, k, kshift);
+        if (X_ && (!s_.reparam || x_solver_basis_)) {
+            const uint64_t tot = N * k;
             hipLaunchKernelGGL(k_store_x, dim3((tot + BLK - 1) / BLK), dim3(BLK), 0, st, N, ws.x.as<double>(),
-                               X + c0 * N, N, k);
+                               X_ + c0_ * N, N, k);
         }
         // residual numerator ||D'W(y - Dx)||; the denominator ||D'W y|| = beta_1 ||p~_1|| was
         // kept at initialisation
-        launch_adjoint(dv, da, weighted ? 2 : 0, dm_int, ws.h.as<double>(), ws.segpart.as<double>(), nullptr, k,
+        launch_adjoint(dv_, da, weighted_ ? 2 : 0, dm_int, ws.h.as<double>(), ws.segpart.as<double>(), nullptr, k,
                        kshift, st);
         double num[MAX_BATCH];
-        DevBuf &red = ws.red;
-        hipLaunchKernelGGL(k_sumsq, dim3(gN), dim3(BLK), 0, st, N, ws.h.as<double>(), part, k, kshift);
-        hipLaunchKernelGGL(k_reduce_to, dim3(k), dim3(BLK), 0, st, part, gN, kp, red.as<double>());
+        hipLaunchKernelGGL(k_sumsq, dim3(gN_), dim3(BLK), 0, st, N, ws.h.as<double>(), part_, k, kshift);
+        hipLaunchKernelGGL(k_reduce_to, dim3(k), dim3(BLK), 0, st, part_, gN_, kp, ws.red.as<double>());
         WX_HIP(hipGetLastError());
         double host_d[N_DOUBLE_FIELDS * MAX_BATCH];
         int32_t host_i[N_INT_FIELDS * MAX_BATCH];
-        WX_HIP(hipMemcpyAsync(num, red.as<double>(), MAX_BATCH * 8, hipMemcpyDeviceToHost, st));
+        WX_HIP(hipMemcpyAsync(num, ws.red.as<double>(), MAX_BATCH * 8, hipMemcpyDeviceToHost, st));
         WX_HIP(hipMemcpyAsync(host_d, ws.cols.p, sizeof(host_d), hipMemcpyDeviceToHost, st));
         WX_HIP(hipMemcpyAsync(host_i, ws.ints.p, sizeof(host_i), hipMemcpyDeviceToHost, st));
         WX_HIP(hipStreamSynchronize(st));
-        if (X && s.reparam && !x_solver_basis) {
+        if (X_ && s_.reparam && !x_solver_basis_) {
             std::vector<double> xi(N * k), xo(N);
             WX_HIP(hipMemcpy(xi.data(), ws.x.p, N * k * 8, hipMemcpyDeviceToHost));
             for (uint32_t j = 0; j < k; ++j) {
                 for (uint64_t q = 0; q < N; ++q) xo[q] = xi[q * k + j];
-                s.reparam->back_transform(s.design, xo.data());
-                WX_HIP(hipMemcpy(X + (c0 + j) * N, xo.data(), N * 8, hipMemcpyHostToDevice));
+                s_.reparam->back_transform(s_.design, xo.data());
+                WX_HIP(hipMemcpy(X_ + (c0_ + j) * N, xo.data(), N * 8, hipMemcpyHostToDevice));
             }
         }
-        auto t1 = std::chrono::steady_clock::now();
-        const double secs = std::chrono::duration<double>(t1 - t0).count();
+        const double secs = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0_).count();
         const double *phi_bar = host_d + 7 * MAX_BATCH, *zeta_bar = host_d + 10 * MAX_BATCH,
                      *zeta0 = host_d + 11 * MAX_BATCH, *den_norm = host_d + 21 * MAX_BATCH;
         const int32_t *active = host_i, *stop = host_i + 3 * MAX_BATCH, *iters = host_i + 4 * MAX_BATCH,
                       *err = host_i + 5 * MAX_BATCH;
         for (uint32_t j = 0; j < k; ++j) {
-            if (err[j])
-                fail(WX_ERR_NOT_POSITIVE_DEF, "mlsmr: preconditioner not positive definite (<v, Mv> < 0)");
-            wx_rhs_stats &o = stats[c0 + j];
+            if (err[j]) fail(WX_ERR_NOT_POSITIVE_DEF, "mlsmr: preconditioner not positive definite (<v, Mv> < 0)");
+            wx_rhs_stats &o = stats_[c0_ + j];
             int32_t sr = stop[j];
             if (active[j] || sr < 0) {
                 sr = WX_STOP_MAX_ITERATIONS;
                 o.converged = 0;
-                o.iterations = maxiter;
+                o.iterations = maxiter_;
             } else {
                 o.converged = 1;
                 o.iterations = iters[j];
@@ -1563,10 +1530,311 @@ void solve_batch_device(const Solver &s, const double *Y, uint64_t ldy, uint64_t
             o.time_solve = secs / k;
         }
     }
+
+  private:
+    // one LSMR iteration (u update, beta, D', p~, M^{-1}, reorthogonalisation, alpha, x update)
+    void iteration(bool first) {
+        Workspace &ws = *ws_;
+        const uint64_t N = N_;
+        const uint32_t k = k_, kshift = kshift_;
+        hipStream_t st = st_;
+        const bool timed = timer_ && timer_->on;
+        if (timed) timer_->begin(3), timer_->begin(0);
+        const double *uin = first ? ws.yint.as<double>() : ws.u.as<double>();
+        uint32_t gpart = gn_;
+        if (dv_.fuse_term >= 0)
+            gpart = launch_gather_seg(dv_, da_, first, ws.v.as<double>(), uin, ws.u.as<double>(), ws.atu.as<double>(),
+                                      ws.segpart.as<double>(), cs_, part_, k, kshift, st);
+        else
+            launch_gather(dv_, da_, first, gn_, ws.v.as<double>(), uin, ws.u.as<double>(), cs_, part_, k, kshift, st);
+        if (timed) timer_->end(0);
+        hipLaunchKernelGGL(k_beta, dim3(k), dim3(BLK), 0, st, part_, gpart, kp_, cs_);
+        if (timed) timer_->begin(1);
+        // split levels of the fused term first: the other terms reuse the partial buffer
+        if (dv_.fuse_term >= 0)
+            launch_segfix(dv_.terms[dv_.fuse_term], ws.segpart.as<double>(), ws.atu.as<double>(), cs_.active, k, kshift,
+                          st);
+        launch_adjoint(dv_, da_, weighted_ ? 1 : 0, ws.u.as<double>(), ws.atu.as<double>(), ws.segpart.as<double>(),
+                       cs_.active, k, kshift, st, dv_.fuse_term);
+        if (timed) timer_->end(1);
+        if (prec_kind_ == 2) {
+            hipLaunchKernelGGL((k_ptilde<2, false>), dim3(gN_), dim3(BLK), 0, st, N, ws.atu.as<double>(),
+                               ws.pt.as<double>(), ws.v.as<double>(), nullptr, cs_, part_, k, kshift);
+            if (timed) timer_->begin(2);
+            schwarz_->apply(ws.pt.as<double>(), ws.v.as<double>(), k, st, cs_.active);
+            if (timed) timer_->end(2);
+            if (cap_) ring_mgs();
+            hipLaunchKernelGGL(k_dots, dim3(gN_), dim3(BLK), 0, st, N, ws.pt.as<double>(), ws.v.as<double>(), cs_,
+                               part_, k, kshift);
+        } else {
+            if (prec_kind_ == 1)
+                hipLaunchKernelGGL((k_ptilde<1, false>), dim3(gN_), dim3(BLK), 0, st, N, ws.atu.as<double>(),
+                                   ws.pt.as<double>(), ws.v.as<double>(), inv_diag_, cs_, part_, k, kshift);
+            else
+                hipLaunchKernelGGL((k_ptilde<0, false>), dim3(gN_), dim3(BLK), 0, st, N, ws.atu.as<double>(),
+                                   ws.pt.as<double>(), ws.v.as<double>(), nullptr, cs_, part_, k, kshift);
+            if (cap_) { // reorthogonalise, then the alpha dot products of the updated pair
+                ring_mgs();
+                hipLaunchKernelGGL(k_dots, dim3(gN_), dim3(BLK), 0, st, N, ws.pt.as<double>(), ws.v.as<double>(), cs_,
+                                   part_, k, kshift);
+            }
+        }
+        hipLaunchKernelGGL(k_alpha_step, dim3(k), dim3(BLK), 0, st, part_, gN_, kp_, cs_, opts_.tol);
+        const uint64_t push = cap_ ? ring_advance() : 0;
+        hipLaunchKernelGGL(k_update_solution, dim3(gN_), dim3(BLK), 0, st, N, ws.v.as<double>(), ws.h.as<double>(),
+                           ws.hb.as<double>(), ws.x.as<double>(), cs_, k, kshift, ws.pt.as<double>(),
+                           cap_ ? ring_v(push) : nullptr, cap_ ? ring_p(push) : nullptr);
+        if (timed) timer_->end(3);
+    }
+
+    // `group_` iterations captured once per workspace into a graph and replayed
+    void replay_graph() {
+        Workspace &ws = *ws_;
+        if (!ws.graph_exec || ws.graph_key != graph_key_) {
+            ws.drop_graph();
+            hipGraph_t g = nullptr;
+            WX_HIP(hipStreamBeginCapture(st_, hipStreamCaptureModeRelaxed));
+            for (int q = 0; q < group_; ++q) iteration(false);
+            WX_HIP(hipStreamEndCapture(st_, &g));
+            ws.graph = g;
+            WX_HIP(hipGraphInstantiate(&ws.graph_exec, g, nullptr, nullptr, 0));
+            ws.graph_key = graph_key_;
+        }
+        WX_HIP(hipGraphLaunch(ws.graph_exec, st_));
+    }
+
+    double *ring_v(uint64_t slot) { return ringV_.as<double>() + slot * N_ * k_; }
+    double *ring_p(uint64_t slot) { return ringP_.as<double>() + slot * N_ * k_; }
+    uint64_t ring_advance() {
+        const uint64_t slot = ring_next_;
+        ring_next_ = (ring_next_ + 1) % cap_;
+        if (ring_count_ < cap_) ++ring_count_;
+        return slot;
+    }
+    // one MGS sweep of v (and p~ in lockstep) over the stored slots, oldest first
+    void ring_mgs() {
+        const uint64_t start = ring_count_ < cap_ ? 0 : ring_next_;
+        for (uint64_t q = 0; q < ring_count_; ++q) {
+            const uint64_t slot = (start + q) % cap_;
+            hipLaunchKernelGGL(k_ring_dot, dim3(gN_), dim3(BLK), 0, st_, N_, ws_->v.as<double>(), ring_p(slot), cs_,
+                               part_, k_, kshift_);
+            hipLaunchKernelGGL(k_reduce_to, dim3(k_), dim3(BLK), 0, st_, part_, gN_, kp_, ringc_.as<double>());
+            hipLaunchKernelGGL(k_ring_axpy, dim3(gN_), dim3(BLK), 0, st_, N_, ws_->v.as<double>(), ws_->pt.as<double>(),
+                               ring_v(slot), ring_p(slot), ringc_.as<double>(), cs_, k_, kshift_);
+        }
+    }
+
+    const Solver &s_;
+    const DevDesign &dv_;
+    const DesignArgs da_;
+    const double *Y_;
+    uint64_t ldy_, c0_;
+    uint32_t k_, kshift_ = 0, kp_ = 1;
+    wx_lsmr_options opts_;
+    double *X_, *Dm_;
+    wx_rhs_stats *stats_;
+    hipStream_t st_;
+    bool x_solver_basis_;
+    PhaseTimer *timer_;
+    PooledWorkspace pw_;
+    Workspace *ws_ = nullptr;
+    SchwarzRun *schwarz_ = nullptr;
+    ColState cs_{};
+    uint64_t n_ = 0, N_ = 0;
+    bool weighted_ = false;
+    const uint32_t *inv_perm_ = nullptr;
+    int maxiter_ = 0, prec_kind_ = 0;
+    const double *inv_diag_ = nullptr;
+    uint32_t gn_ = 1, gN_ = 1, gt_ = 1;
+    size_t tile_bytes_ = 0;
+    double *part_ = nullptr;
+    std::chrono::steady_clock::time_point t0_;
+    uint64_t cap_ = 0, ring_next_ = 0, ring_count_ = 0;
+    DevBuf ringV_, ringP_, ringc_;
+    int group_ = 1, itn_ = 0, launches_ = 0;
+    bool use_graph_ = false;
+    uint64_t graph_key_ = 0;
+};
+
+void check_solve_args(const Solver &s, uint64_t ldy, const wx_lsmr_options &opts) {
+    if (!std::isfinite(opts.tol) || opts.tol < 0.0)
+        fail(WX_ERR_INVALID_INPUT, "lsmr: tolerance must be finite and nonnegative, got " + std::to_string(opts.tol));
+    if (ldy < s.dev.n) fail(WX_ERR_INVALID_INPUT, "leading dimension of Y is smaller than n_obs");
+}
+
+} // namespace
+
+void solve_batch_device(const Solver &s, const double *Y, uint64_t ldy, uint64_t k_total, const wx_lsmr_options &opts,
+                        double *X, double *Dm, wx_rhs_stats *stats, hipStream_t st, bool x_solver_basis) {
+    check_solve_args(s, ldy, opts);
+    if (k_total == 0) return;
+    PhaseTimer timer;
+    timer.on = opts.timing != 0;
+    timer.st = st;
+    // check_every > 1 launches that many iterations back to back between polls
+    const int group = opts.check_every > 0 ? opts.check_every : 1;
+    for (uint64_t c0 = 0; c0 < k_total; c0 += MAX_BATCH) {
+        const uint32_t k = (uint32_t)std::min<uint64_t>(MAX_BATCH, k_total - c0);
+        ChunkRun run(s, Y, ldy, c0, k, opts, X, Dm, stats, st, x_solver_basis, &timer, group, false);
+        run.launch_load();
+        run.finish_init();
+        while (!run.exhausted()) {
+            run.launch_group();
+            if (run.poll(true) == 1) break;
+        }
+        run.finish();
+    }
     if (timer.on) {
         WX_HIP(hipStreamSynchronize(st));
         timer.collect(last_timing());
     }
+}
+
+// ---------------------------------------------------------------------------
+// Many independent problems at once.  Every (problem, chunk) run is resumable; the driver
+// spreads the runs over `streams` default-priority streams, replays each run's iterations
+// as captured graphs of `group` iterations, and polls every run's convergence flags without
+// blocking, so converged problems stop launching while the others continue.  A run whose
+// columns have all converged produces its outputs at once.  The (streams, group) shape is
+// chosen by a small autotuner on the first call for a problem-count class and then cached.
+// ---------------------------------------------------------------------------
+namespace {
+struct ManyShape {
+    int streams = 1, group = 1;
+    bool graph = false;
+};
+std::mutex g_tune_mu;
+std::vector<std::pair<uint64_t, ManyShape>> g_tuned; // problem-count class -> shape
+ManyShape g_last_shape;
+ManyShape g_forced;      // streams == 0: the autotuner decides
+
+std::vector<hipStream_t> &stream_pool() {
+    static std::vector<hipStream_t> pool;
+    return pool;
+}
+
+hipStream_t pool_stream(int i) {
+    auto &pool = stream_pool();
+    while ((int)pool.size() <= i) {
+        hipStream_t s;
+        WX_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+        pool.push_back(s);
+    }
+    return pool[i];
+}
+
+void run_many(const std::vector<const Solver *> &solvers, const std::vector<const double *> &Y,
+              const std::vector<uint64_t> &ldy, const std::vector<uint64_t> &k, const wx_lsmr_options &opts,
+              const std::vector<double *> &X, const std::vector<double *> &Dm, const std::vector<wx_rhs_stats *> &stats,
+              const ManyShape &shape, hipStream_t caller) {
+    // the caller's stream work comes first
+    hipEvent_t ready;
+    WX_HIP(hipEventCreateWithFlags(&ready, hipEventDisableTiming));
+    WX_HIP(hipEventRecord(ready, caller));
+    std::vector<std::unique_ptr<ChunkRun>> runs;
+    int next_stream = 0;
+    for (size_t p = 0; p < solvers.size(); ++p)
+        for (uint64_t c0 = 0; c0 < k[p]; c0 += MAX_BATCH) {
+            const uint32_t kk = (uint32_t)std::min<uint64_t>(MAX_BATCH, k[p] - c0);
+            hipStream_t st = pool_stream(next_stream);
+            next_stream = (next_stream + 1) % shape.streams;
+            WX_HIP(hipStreamWaitEvent(st, ready, 0));
+            runs.push_back(std::make_unique<ChunkRun>(*solvers[p], Y[p], ldy[p], c0, kk, opts, X[p], Dm[p], stats[p], st,
+                                                      false, nullptr, shape.group, shape.graph));
+        }
+    for (auto &r : runs) r->launch_load();
+    for (auto &r : runs) r->finish_init();
+    std::vector<ChunkRun *> active;
+    for (auto &r : runs) active.push_back(r.get());
+    while (!active.empty()) {
+        bool progressed = false;
+        for (size_t i = 0; i < active.size();) {
+            ChunkRun &r = *active[i];
+            const int state = r.poll(false);
+            if (state == 1 || (r.exhausted() && state != -1)) {
+                r.finish();
+                active.erase(active.begin() + i);
+                progressed = true;
+                continue;
+            }
+            if (state == 0 && !r.exhausted()) {
+                r.launch_group();
+                progressed = true;
+            }
+            ++i;
+        }
+        if (!progressed) std::this_thread::yield();
+    }
+    WX_HIP(hipEventDestroy(ready));
+}
+} // namespace
+
+void solve_many_device(const std::vector<const Solver *> &solvers, const std::vector<const double *> &Y,
+                       const std::vector<uint64_t> &ldy, const std::vector<uint64_t> &k, const wx_lsmr_options &opts,
+                       const std::vector<double *> &X, const std::vector<double *> &Dm,
+                       const std::vector<wx_rhs_stats *> &stats, hipStream_t caller) {
+    for (size_t p = 0; p < solvers.size(); ++p) check_solve_args(*solvers[p], ldy[p], opts);
+    uint64_t n_runs = 0;
+    for (uint64_t kp : k) n_runs += (kp + MAX_BATCH - 1) / MAX_BATCH;
+    if (n_runs == 0) return;
+    uint64_t cls = 1;
+    while (cls < n_runs) cls <<= 1;
+    ManyShape shape;
+    bool tuned = false;
+    {
+        std::lock_guard<std::mutex> lk(g_tune_mu);
+        if (g_forced.streams > 0) {
+            shape = g_forced;
+            tuned = true;
+        } else {
+            for (auto &e : g_tuned)
+                if (e.first == cls) {
+                    shape = e.second;
+                    tuned = true;
+                    break;
+                }
+        }
+    }
+    if (!tuned) {
+        // autotuner: time each candidate shape on the real problems with the iteration count
+        // capped (the final run below overwrites every output)
+        static const ManyShape cand[] = {{1, 1, false}, {4, 1, true}, {8, 1, true}, {8, 2, true}, {16, 2, true},
+                                         {16, 4, true}};
+        wx_lsmr_options probe = opts;
+        probe.maxiter = std::min<int64_t>(opts.maxiter, 24);
+        probe.timing = 0;
+        double best = 1e300;
+        for (const ManyShape &c : cand) {
+            WX_HIP(hipStreamSynchronize(caller));
+            const auto t0 = std::chrono::steady_clock::now();
+            run_many(solvers, Y, ldy, k, probe, X, Dm, stats, c, caller);
+            const double dt = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+            if (dt < best) {
+                best = dt;
+                shape = c;
+            }
+        }
+        std::lock_guard<std::mutex> lk(g_tune_mu);
+        g_tuned.push_back({cls, shape});
+    }
+    {
+        std::lock_guard<std::mutex> lk(g_tune_mu);
+        g_last_shape = shape;
+    }
+    run_many(solvers, Y, ldy, k, opts, X, Dm, stats, shape, caller);
+}
+
+void force_many_shape(int32_t streams, int32_t group, int32_t graph) {
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    g_forced.streams = std::max(0, std::min(streams, 64));
+    g_forced.group = std::max(1, std::min(group, 64));
+    g_forced.graph = graph != 0;
+}
+
+void last_many_shape(int32_t *streams, int32_t *group, int32_t *graph) {
+    if (streams) *streams = g_last_shape.streams;
+    if (group) *group = g_last_shape.group;
+    if (graph) *graph = g_last_shape.graph ? 1 : 0;
 }
 
 std::vector<double> &last_timing() {

@@ -113,6 +113,19 @@ void upload_design(Solver &s, const std::vector<double> &weights_internal);
 void solve_batch_device(const Solver &s, const double *Y, uint64_t ldy, uint64_t k, const wx_lsmr_options &opts,
                         double *X, double *Dm, wx_rhs_stats *stats, hipStream_t stream, bool x_solver_basis = false);
 
+// Many independent batched solves at once (solvers[p] on Y[p], n_p x k[p] column-major,
+// device arrays); outputs and statistics as for solve_batch_device, per problem.
+void solve_many_device(const std::vector<const Solver *> &solvers, const std::vector<const double *> &Y,
+                       const std::vector<uint64_t> &ldy, const std::vector<uint64_t> &k, const wx_lsmr_options &opts,
+                       const std::vector<double *> &X, const std::vector<double *> &Dm,
+                       const std::vector<wx_rhs_stats *> &stats, hipStream_t caller);
+
+// Fixes the shape of later solve_many_device calls (streams 0: back to the autotuner).
+void force_many_shape(int32_t streams, int32_t group, int32_t graph);
+
+// The (streams, iterations per graph, graph on) shape the last solve_many_device used.
+void last_many_shape(int32_t *streams, int32_t *group, int32_t *graph);
+
 // Adjoint of the weighted projection R = Z - D x(Z, w) for reverse-mode differentiation:
 // grad_z = G - w .* (D lambda), grad_w = -sum_j R_j .* (D lambda)_j with D'WD lambda = D'G.
 // Column-major device arrays in caller row order; grad_w may be null.
