@@ -153,6 +153,99 @@ def cpu_baseline(solver, codes, w, Y, seconds_budget=30.0):
                       f"{max(r.iterations for r in res)} max iterations, {dt:.1f} s wall"}
 
 
+def bench_many(args):
+    """Uneven batch of independent problems: 64 problems with n log-uniform in [10^4, 10^6],
+    2 or 3 factors (n/20, n/200, 30 uniform levels), 1-8 RHS each, tol 1e-8, default additive
+    Schwarz.  A step is one solve_many call over all problems from inputs resident in HBM;
+    beside it, the same problems solved one solve_batch call after another on the same GPU."""
+    import torch
+    import within_amd as wa
+    from within_amd import _lib
+    from tests._configs import cfg4_problems
+    L = _lib.lib()
+    probs = cfg4_problems(seed=4, n_problems=args.problems)
+    t0 = time.perf_counter()
+    solvers = [wa.Solver(codes) for codes, _ in probs]
+    setup = time.perf_counter() - t0
+    P = len(probs)
+    ks = [Y.shape[1] for _, Y in probs]
+    ns = [Y.shape[0] for _, Y in probs]
+    log(f"[bench] {P} problems, n {min(ns)}..{max(ns)} (total {sum(ns)}), {sum(ks)} RHS, setup {setup:.1f} s")
+    Yd = [torch.from_numpy(np.ascontiguousarray(Y.T)).cuda() for _, Y in probs]
+    Xd = [torch.empty((k, s.n_dofs), dtype=torch.float64, device="cuda") for k, s in zip(ks, solvers)]
+    Dd = [torch.empty((k, n), dtype=torch.float64, device="cuda") for k, n in zip(ks, ns)]
+    stats = [(_lib.RhsStats * k)() for k in ks]
+    opts = _lib.LsmrOpts()
+    L.wx_lsmr_options_default(C.byref(opts))
+    opts.tol = args.tol
+    opts.precision = 1
+    stream = torch.cuda.current_stream().cuda_stream
+    hs = (C.c_void_p * P)(*[s._h for s in solvers])
+    yp = (C.c_void_p * P)(*[t.data_ptr() for t in Yd])
+    ld = (C.c_uint64 * P)(*ns)
+    kk = (C.c_uint64 * P)(*ks)
+    xp = (C.c_void_p * P)(*[t.data_ptr() for t in Xd])
+    dp = (C.c_void_p * P)(*[t.data_ptr() for t in Dd])
+    sp = (C.c_void_p * P)(*[C.cast(st, C.c_void_p) for st in stats])
+
+    def many():
+        _lib.check(L.wx_solve_many_device(hs, P, yp, ld, kk, C.byref(opts), xp, dp, sp, C.c_void_p(stream)))
+
+    def sequential():
+        for p in range(P):
+            _lib.check(L.wx_solver_solve_batch_device(solvers[p]._h, yp[p], ns[p], ks[p], C.byref(opts), xp[p], dp[p],
+                                                      stats[p], C.c_void_p(stream)))
+
+    def timed(fn):
+        for _ in range(args.warmup):
+            fn()
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        for _ in range(args.steps):
+            fn()
+        torch.cuda.synchronize()
+        return (time.perf_counter() - t) / args.steps
+
+    t_seq = timed(sequential)
+    its_seq = [[stats[p][j].iterations for j in range(ks[p])] for p in range(P)]
+    t_many = timed(many)  # the first warm-up call runs the autotuner
+    its = [[stats[p][j].iterations for j in range(ks[p])] for p in range(P)]
+    shape = wa.last_many_shape()
+    # per-problem latency of the last call: time from the call's start to the problem's outputs
+    lat = np.array([sum(stats[p][j].time_solve for j in range(ks[p])) for p in range(P)]) * 1e3
+    conv = all(stats[p][j].converged for p in range(P) for j in range(ks[p]))
+    tr = max(stats[p][j].true_residual for p in range(P) for j in range(ks[p]))
+    log(f"[bench] solve_many {t_many * 1e3:.1f} ms, sequential {t_seq * 1e3:.1f} ms, shape {shape}")
+    out = {
+        "metric": "RHS solved/s to tol 1e-8, many independent problems in one call",
+        "value": sum(ks) / t_many,
+        "unit": "RHS/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": t_many * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64 Krylov + f32 preconditioner coefficients",
+        "data": "synthetic (seeded generator of the uneven many-problem workload)",
+        "config": {"workload": f"{P} independent problems, n log-uniform in [1e4, 1e6], 2-3 factors "
+                               "(n/20, n/200, 30 uniform levels), 1-8 RHS each, tol 1e-8",
+                   "problems": P, "total_rhs": sum(ks), "total_obs": sum(ns), "precision": "mixed"},
+        "sequential_solve_batch_value": sum(ks) / t_seq,
+        "many_over_sequential": t_seq / t_many,
+        "launch_shape": shape,
+        "latency_ms": {"p50": float(np.percentile(lat, 50)), "p95": float(np.percentile(lat, 95)),
+                       "max": float(lat.max()), "min": float(lat.min())},
+        "converged": conv,
+        "true_residual_max": tr,
+        "iterations_equal_sequential": its == its_seq,
+        "max_iterations": max(max(i) for i in its),
+        "setup_s": setup,
+    }
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -164,6 +257,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--workload", choices=["solve_batch", "many"], default="solve_batch",
+                    help="solve_batch: the weighted 5M-row batch (default); many: 64 uneven independent problems")
+    ap.add_argument("--problems", type=int, default=64)
     ap.add_argument("--copy-probe", action="store_true",
                     help="also run a plain 1 GiB device-to-device copy (calibrates HBM byte counters)")
     args = ap.parse_args()
@@ -171,6 +267,10 @@ def main():
         print(f"bench.py runs on one GPU only: --gpus {args.gpus} is not supported by this build "
               "(the solve batches are independent; see the sharding note in DESIGN.md)", file=sys.stderr)
         sys.exit(2)
+
+    if args.workload == "many":
+        bench_many(args)
+        return
 
     import torch
     import within_amd as wa
