@@ -208,6 +208,14 @@ def bench_many(args):
 
     t_seq = timed(sequential)
     its_seq = [[stats[p][j].iterations for j in range(ks[p])] for p in range(P)]
+    sweep = {}
+    if args.sweep:
+        for shp in [(1, 1, False, False), (8, 2, True, False), (1, 1, False, True), (2, 1, True, True),
+                    (4, 1, True, True), (4, 2, True, True), (8, 2, True, True), (4, 4, True, True)]:
+            wa.set_many_shape(*shp)
+            sweep[str(shp)] = timed(many) * 1e3
+            log(f"[bench] shape {shp}: {sweep[str(shp)]:.1f} ms")
+        wa.set_many_shape(0)
     t_many = timed(many)  # the first warm-up call runs the autotuner
     its = [[stats[p][j].iterations for j in range(ks[p])] for p in range(P)]
     shape = wa.last_many_shape()
@@ -235,6 +243,7 @@ def bench_many(args):
         "sequential_solve_batch_value": sum(ks) / t_seq,
         "many_over_sequential": t_seq / t_many,
         "launch_shape": shape,
+        "shape_sweep_ms": sweep or None,
         "latency_ms": {"p50": float(np.percentile(lat, 50)), "p95": float(np.percentile(lat, 95)),
                        "max": float(lat.max()), "min": float(lat.min())},
         "converged": conv,
@@ -260,6 +269,7 @@ def main():
     ap.add_argument("--workload", choices=["solve_batch", "many"], default="solve_batch",
                     help="solve_batch: the weighted 5M-row batch (default); many: 64 uneven independent problems")
     ap.add_argument("--problems", type=int, default=64)
+    ap.add_argument("--sweep", action="store_true", help="many: also time every launch shape")
     ap.add_argument("--copy-probe", action="store_true",
                     help="also run a plain 1 GiB device-to-device copy (calibrates HBM byte counters)")
     args = ap.parse_args()

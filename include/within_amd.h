@@ -192,10 +192,12 @@ wx_status wx_solver_solve_batch_device(const wx_solver *s, const double *Y_devic
 /* Many independent problems in one call: problem p solves Y[p] (n_obs(p) x k[p] column-major,
  * leading dimension ldy[p]) with solvers[p] and writes X[p], demeaned[p] (either table or
  * entry may be NULL) and stats[p][0..k[p]).  Replaces a loop of `Solver::solve_batch` calls
- * (crates/within/src/solver.rs:530-577) over independent solvers; the problems iterate
- * concurrently (several streams, captured iteration graphs, per-problem convergence), with a
- * (streams, iterations per graph) shape chosen by an autotuner on the first call for a
- * problem-count class.  _device: every array in device memory, work ordered after `stream`;
+ * (crates/within/src/solver.rs:530-577) over independent solvers.  Problems of one shape
+ * (same terms, weighting and preconditioner kind, no slopes, width class) are packed into one
+ * block-diagonal run whose every iteration kernel covers all of them, with per-problem LSMR
+ * state and convergence; the runs iterate concurrently on several streams as captured
+ * iteration graphs.  The launch shape (packing, streams, iterations per graph, graphs) is
+ * chosen by an autotuner on the first call for a problem-count class.  _device: every array in device memory, work ordered after `stream`;
  * the host form copies. */
 wx_status wx_solve_many_device(const wx_solver *const *solvers, uint64_t n_problems, const double *const *Y,
                                const uint64_t *ldy, const uint64_t *k, const wx_lsmr_options *opts, double *const *X,
@@ -205,12 +207,14 @@ wx_status wx_solve_many(const wx_solver *const *solvers, uint64_t n_problems, co
                         double *const *demeaned, wx_rhs_stats *const *stats);
 
 /* The shape the autotuner chose for the last many-problem call: streams, iterations per
- * captured graph, and whether graphs were used (1) or iterations launched one by one (0). */
-void wx_last_many_shape(int32_t *streams, int32_t *iterations_per_graph, int32_t *graph);
+ * captured graph, whether graphs were used (1) or iterations launched one by one (0), and
+ * whether problems of one shape were packed (1) or run one by one (0). */
+void wx_last_many_shape(int32_t *streams, int32_t *iterations_per_graph, int32_t *graph, int32_t *pack);
 
 /* Fixes the shape of later many-problem calls instead of the autotuner's choice (streams 1-64,
- * iterations per graph 1-64, graph 0/1); streams = 0 returns the choice to the autotuner. */
-void wx_set_many_shape(int32_t streams, int32_t iterations_per_graph, int32_t graph);
+ * iterations per graph 1-64, graph 0/1, pack 0/1); streams = 0 returns the choice to the
+ * autotuner. */
+void wx_set_many_shape(int32_t streams, int32_t iterations_per_graph, int32_t graph, int32_t pack);
 
 /* Adjoint of the weighted projection demeaned = z - D x(z, w) for reverse-mode
  * differentiation.  No counterpart in the reference (its bindings are not differentiable);

@@ -1,8 +1,9 @@
-"""Many independent problems in one call (solve_many): every problem's output must equal the
-same Solver's own solve_batch, whatever launch shape runs them (one stream without graphs,
-several streams with captured iteration graphs of 1-4 iterations, the autotuner's choice).
-The per-run computation is the same sequence of kernels in every shape; the extra iterations
-a group or the lag-one poll runs past convergence are gated off, so the comparison is tight."""
+"""Many independent problems in one call (solve_many).  Unpacked shapes (one run per problem on
+one or several streams, with or without captured iteration graphs) run the same sequence of
+kernels per problem as the Solver's own solve_batch, and the extra iterations a group or the
+lag-one poll runs past convergence are gated off, so their output is bitwise equal.  Packed
+shapes run problems of one shape as one block-diagonal batch: the same algorithm per problem
+with the norm and dot-product partial sums grouped differently, so they agree to rounding."""
 import numpy as np
 import pytest
 
@@ -31,6 +32,15 @@ def problems():
     a, b = rng.integers(0, 800, n).astype(np.uint32), rng.integers(0, 60, n).astype(np.uint32)
     z = rng.standard_normal(n)
     out.append((wa.Solver([wa.Effect(a, True), wa.Effect(b, True, [z])]), np.asfortranarray(rng.standard_normal((n, 3)))))
+    # pairs that pack together: weighted (Schwarz), Diagonal, Off; k 3 and 4 share a width class
+    for pc in (None, wa.PreconditionerConfig.Diagonal, wa.PreconditionerConfig.Off):
+        for q, k in enumerate((3, 4)):
+            n = 8_000 + 3_000 * q
+            codes = np.asfortranarray(np.column_stack([rng.integers(0, n // 20, n), rng.integers(0, 40, n),
+                                                       rng.integers(0, 7, n)]).astype(np.uint32))
+            w = np.exp(rng.normal(0, 0.4, n)) if pc is None else None
+            s = wa.Solver(codes, weights=w) if pc is None else wa.Solver(codes, preconditioner=pc)
+            out.append((s, np.asfortranarray(rng.standard_normal((n, k)))))
     return out
 
 
@@ -40,25 +50,59 @@ def singles(problems):
     return [s.solve_batch(Y, opts) for s, Y in problems]
 
 
-@pytest.mark.parametrize("shape", [(1, 1, False), (4, 1, True), (8, 2, True), (16, 4, True)])
+SHAPES = [(1, 1, False, False), (4, 1, True, False), (8, 2, True, False), (16, 4, True, False),
+          (1, 1, False, True), (4, 1, True, True), (4, 4, True, True)]
+
+
+@pytest.mark.parametrize("shape", SHAPES)
 def test_many_matches_individual_solves(gpu, problems, singles, shape):
+    print(f"[many] shape {shape}", flush=True)
     wa.set_many_shape(*shape)
     try:
         res = wa.solve_many([s for s, _ in problems], [Y for _, Y in problems], wa.LsmrOptions(tol=1e-8))
     finally:
         wa.set_many_shape(0)
     got = wa.last_many_shape()
-    assert (got["streams"], got["iterations_per_graph"], got["graph"]) == shape
-    worst_d = worst_x = 0.0
+    assert (got["streams"], got["iterations_per_graph"], got["graph"], got["pack"]) == shape
+    packed = shape[3]
+    worst_d = worst_x = worst_off = worst_plain = 0.0
+    diter = n_off = n_cols = 0
     for p, (r, ref) in enumerate(zip(res, singles)):
-        assert r.converged == ref.converged, p
-        assert r.iterations == ref.iterations, p
+        # plain LSMR (no preconditioner, ~40 steps without reorthogonalisation) amplifies the
+        # rounding of differently grouped partial sums; preconditioned runs stay at rounding
+        plain = problems[p][0].preconditioner is None
         assert all(r.converged), p
-        worst_d = max(worst_d, max(rel(r.demeaned[:, j], ref.demeaned[:, j]) for j in range(r.demeaned.shape[1])))
-        worst_x = max(worst_x, max(rel(r.x[:, j], ref.x[:, j]) for j in range(r.x.shape[1])))
-        np.testing.assert_allclose(r.true_residual, ref.true_residual, rtol=1e-9, atol=1e-15)
-    print(f"shape {shape}: demeaned {worst_d:.2e}, x {worst_x:.2e} vs solve_batch over {len(res)} problems")
-    assert worst_d <= 1e-13 and worst_x <= 1e-12
+        assert r.demeaned.shape == ref.demeaned.shape and r.x.shape == ref.x.shape
+        if not packed:
+            assert r.iterations == ref.iterations, p
+            np.testing.assert_allclose(r.true_residual, ref.true_residual, rtol=1e-9, atol=1e-15)
+        for j in range(r.demeaned.shape[1]):
+            n_cols += 1
+            dd = rel(r.demeaned[:, j], ref.demeaned[:, j])
+            if r.iterations[j] == ref.iterations[j] and plain:
+                worst_plain = max(worst_plain, dd)
+            elif r.iterations[j] == ref.iterations[j]:
+                worst_d = max(worst_d, dd)
+                worst_x = max(worst_x, rel(r.x[:, j], ref.x[:, j]))
+            else:
+                # a borderline stopping test decided one step apart: both are converged solutions
+                n_off += 1
+                diter = max(diter, abs(r.iterations[j] - ref.iterations[j]))
+                worst_off = max(worst_off, dd)
+        assert max(r.true_residual) < 1e-5, p
+        pd = max(rel(r.demeaned[:, j], ref.demeaned[:, j]) for j in range(r.demeaned.shape[1]))
+        if pd > 1e-11:
+            print(f"  problem {p}: n={r.demeaned.shape[0]} k={r.demeaned.shape[1]} "
+                  f"precond={repr(problems[p][0].preconditioner)[:30]} "
+                  f"iterations {r.iterations} vs {ref.iterations}, demeaned {pd:.2e}")
+    print(f"shape {shape}: same-iteration columns demeaned {worst_d:.2e}, x {worst_x:.2e} (unpreconditioned "
+          f"{worst_plain:.2e}); {n_off} of {n_cols} columns stop {diter} step(s) apart, demeaned {worst_off:.2e}")
+    if packed:
+        # the same iterates to rounding (partial sums grouped differently)
+        assert worst_d <= 1e-10 and worst_x <= 1e-9 and worst_plain <= 1e-7
+        assert diter <= 1 and n_off <= max(2, n_cols // 10) and worst_off <= 1e-6
+    else:
+        assert n_off == 0 and max(worst_d, worst_plain) <= 1e-13 and worst_x <= 1e-12
 
 
 def test_many_autotuned_mixed_precision(gpu, problems):
@@ -68,13 +112,21 @@ def test_many_autotuned_mixed_precision(gpu, problems):
     # a second call reuses the cached shape
     res2 = wa.solve_many([s for s, _ in problems], [Y for _, Y in problems], opts)
     assert wa.last_many_shape() == shape
+    # packed runs agree with solve_batch to rounding (to the tolerance where a borderline
+    # stopping test decides one step apart), unpacked ones bitwise
     for p, ((s, Y), r, r2) in enumerate(zip(problems, res, res2)):
         ref = s.solve_batch(Y, opts)
-        assert all(r.converged), p
-        assert r.iterations == ref.iterations, p
+        assert all(r.converged) and all(r2.converged), p
         for j in range(Y.shape[1]):
-            assert rel(r.demeaned[:, j], ref.demeaned[:, j]) <= 1e-13
-            assert rel(r2.demeaned[:, j], ref.demeaned[:, j]) <= 1e-13
+            if not shape["pack"]:
+                assert r.iterations[j] == ref.iterations[j]
+                bound = 1e-13
+            else:
+                assert abs(r.iterations[j] - ref.iterations[j]) <= 1
+                plain = s.preconditioner is None
+                bound = 1e-6 if r.iterations[j] != ref.iterations[j] else (1e-7 if plain else 1e-10)
+            assert rel(r.demeaned[:, j], ref.demeaned[:, j]) <= bound
+            assert rel(r2.demeaned[:, j], r.demeaned[:, j]) <= 1e-13 # repeatable
     print(f"autotuned shape {shape}")
 
 
@@ -82,12 +134,12 @@ def test_many_maxiter_and_zero_rhs(gpu, problems):
     s, Y = problems[0]
     Z = np.zeros_like(Y)
     opts = wa.LsmrOptions(tol=1e-14, maxiter=5)
-    wa.set_many_shape(4, 2, True)
+    wa.set_many_shape(4, 2, True, True)
     try:
         r_cap, r_zero = wa.solve_many([s, s], [Y, Z], opts)
     finally:
         wa.set_many_shape(0)
     ref = s.solve_batch(Y, opts)
     assert not any(r_cap.converged) and r_cap.iterations == ref.iterations
-    assert max(rel(r_cap.demeaned[:, j], ref.demeaned[:, j]) for j in range(Y.shape[1])) <= 1e-13
+    assert max(rel(r_cap.demeaned[:, j], ref.demeaned[:, j]) for j in range(Y.shape[1])) <= 1e-10
     assert all(r_zero.converged) and np.all(r_zero.demeaned == 0.0)

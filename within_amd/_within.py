@@ -732,11 +732,12 @@ def solve_many(solvers, Ys, options: Optional[LsmrOptions] = None) -> list:
 
 def last_many_shape() -> dict:
     """The launch shape the autotuner picked for the last solve_many call."""
-    s, g, gr = C.c_int32(), C.c_int32(), C.c_int32()
-    _lib.lib().wx_last_many_shape(C.byref(s), C.byref(g), C.byref(gr))
-    return {"streams": s.value, "iterations_per_graph": g.value, "graph": bool(gr.value)}
+    s, g, gr, pk = C.c_int32(), C.c_int32(), C.c_int32(), C.c_int32()
+    _lib.lib().wx_last_many_shape(C.byref(s), C.byref(g), C.byref(gr), C.byref(pk))
+    return {"streams": s.value, "iterations_per_graph": g.value, "graph": bool(gr.value), "pack": bool(pk.value)}
 
 
-def set_many_shape(streams: int = 0, iterations_per_graph: int = 1, graph: bool = True) -> None:
-    """Fixes the launch shape of later solve_many calls; streams=0 returns it to the autotuner."""
-    _lib.lib().wx_set_many_shape(int(streams), int(iterations_per_graph), 1 if graph else 0)
+def set_many_shape(streams: int = 0, iterations_per_graph: int = 1, graph: bool = True, pack: bool = False) -> None:
+    """Fixes the launch shape of later solve_many calls (pack: problems of one shape run as one
+    block-diagonal batch); streams=0 returns the choice to the autotuner."""
+    _lib.lib().wx_set_many_shape(int(streams), int(iterations_per_graph), 1 if graph else 0, 1 if pack else 0)

@@ -226,6 +226,8 @@ wx_status wx_solver_new(const wx_design *design, const double *weights, const wx
             else s.precond = build_additive(s.design, w, settings_from(c), s.warnings);
         }
         upload_design(s, w);
+        s.uid = next_solver_uid();
+        s.w_internal = std::move(w);
         s.time_setup = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
         *out = h.release();
     });
@@ -401,12 +403,12 @@ wx_status wx_solve_many(const wx_solver *const *solvers, uint64_t n_problems, co
     });
 }
 
-void wx_set_many_shape(int32_t streams, int32_t iterations_per_graph, int32_t graph) {
-    force_many_shape(streams, iterations_per_graph, graph);
+void wx_set_many_shape(int32_t streams, int32_t iterations_per_graph, int32_t graph, int32_t pack) {
+    force_many_shape(streams, iterations_per_graph, graph, pack);
 }
 
-void wx_last_many_shape(int32_t *streams, int32_t *iterations_per_graph, int32_t *graph) {
-    last_many_shape(streams, iterations_per_graph, graph);
+void wx_last_many_shape(int32_t *streams, int32_t *iterations_per_graph, int32_t *graph, int32_t *pack) {
+    last_many_shape(streams, iterations_per_graph, graph, pack);
 }
 
 wx_status wx_solver_apply_operator(const wx_solver *s, int32_t adjoint, const double *X, uint64_t k, double *Y) {

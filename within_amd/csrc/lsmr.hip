@@ -12,7 +12,10 @@
 #include <chrono>
 #include <cmath>
 #include <cstring>
+#include <map>
+#include <mutex>
 #include <numeric>
+#include <string>
 #include <thread>
 
 #include "schwarz.hpp"
@@ -22,10 +25,9 @@ namespace wx {
 
 namespace {
 
-constexpr int BLK = 256;
+constexpr int BLK = (int)KERNEL_BLOCK;
 constexpr uint32_t SEG_CHUNK = 64;  // rows per D' work item before a level is split
 constexpr uint32_t FIX_CHUNK = 64;  // partials per combine item
-constexpr uint32_t MAX_PART_BLOCKS = 4096;   // grid cap of the grid-stride kernels
 constexpr uint32_t MAX_FUSED_BLOCKS = 32768; // grid cap of the fused gather (one item per row group)
 
 // Per-column LSMR state (structure of arrays, each of length MAX_BATCH).
@@ -161,14 +163,15 @@ __global__ __launch_bounds__(BLK) void k_gather_seg(DesignArgs d, TermInfo ts, c
                                                     const double *__restrict__ v, const double *uin, double *u,
                                                     double *__restrict__ atu, double *__restrict__ partials,
                                                     ColState cs, double *__restrict__ part, uint32_t k,
-                                                    uint32_t kshift) {
+                                                    uint32_t kshift, const uint32_t *__restrict__ blk_col) {
     __shared__ double sh[BLK];
     const uint32_t kp = 1u << kshift;
     const uint32_t j = threadIdx.x & (kp - 1);
+    const uint32_t c = (blk_col ? blk_col[blockIdx.x] : 0u) + j; // column-state index
     double sq = 0.0;
-    const bool act = j < k && cs.active[j];
+    const bool act = j < k && cs.active[c];
     if (act) {
-        const double scale = cs.gscale[j];
+        const double scale = cs.gscale[c];
         constexpr int R = 4;
         const uint32_t groups = (gridDim.x * BLK) >> kshift;
         for (uint32_t it = (blockIdx.x * BLK + threadIdx.x) >> kshift; it < n_items; it += groups) {
@@ -199,7 +202,7 @@ __global__ __launch_bounds__(BLK) void k_gather_seg(DesignArgs d, TermInfo ts, c
                 }
             }
             if (item.dest >= 0) atu[(ts.offset + item.level) * k + j] = adj;
-            else partials[(uint64_t)(-item.dest - 1) * k + j] = adj;
+            else if (item.dest != SEG_SKIP) partials[(uint64_t)(-item.dest - 1) * k + j] = adj;
         }
     }
     double tot = block_reduce_cols(sq, kp, sh);
@@ -263,7 +266,7 @@ __global__ __launch_bounds__(BLK) void k_segreduce(DesignArgs d, TermInfo ti, co
         if (item.dest >= 0) {
 #pragma unroll
             for (int c = 0; c < NC; ++c) out[(ti.offset + c * ti.n_levels + item.level) * k + j] = acc[c];
-        } else {
+        } else if (item.dest != SEG_SKIP) {
             const uint64_t slot = (uint64_t)(-item.dest - 1);
 #pragma unroll
             for (int c = 0; c < NC; ++c) partials[(slot * NC + c) * k + j] = acc[c];
@@ -310,7 +313,9 @@ template <bool WEIGHTED>
 __global__ __launch_bounds__(BLK) void k_load_rhs(uint64_t n, const uint32_t *__restrict__ inv_perm,
                                                   const double *__restrict__ sw_caller, const double *__restrict__ Y,
                                                   uint64_t ldy, double *__restrict__ y_int, double *__restrict__ part,
-                                                  int32_t *__restrict__ bad, uint32_t k, uint32_t kshift) {
+                                                  int32_t *__restrict__ bad, uint32_t k, uint32_t kshift, uint32_t kv) {
+    // kv <= k: Y holds kv columns; columns kv .. k of y_int are zero (a packed member
+    // narrower than its pack)
     extern __shared__ double tile[]; // 64 x (k + 1)
     __shared__ double sh[BLK];
     const uint32_t ld = k + 1;
@@ -325,7 +330,7 @@ __global__ __launch_bounds__(BLK) void k_load_rhs(uint64_t n, const uint32_t *__
             const uint32_t c = q >> 6, rr = q & 63;
             const uint64_t r = r0 + rr;
             if (r < n) {
-                const double y = Y[r + c * ldy];
+                const double y = c < kv ? Y[r + c * ldy] : 0.0;
                 nbad += !isfinite(y);
                 tile[rr * ld + c] = y;
             }
@@ -359,13 +364,27 @@ __global__ __launch_bounds__(BLK) void k_load_rhs(uint64_t n, const uint32_t *__
     }
 }
 
-// Per-column reduction of block partials (one block per column, fixed tree order).
+// Partial blocks of one packed member: the ranges lh[2r], lh[2r+1] for r in
+// [ptr[m], ptr[m+1]).  A null ptr means one problem and the blocks [0, nb).
+struct Ranges {
+    const uint32_t *ptr;
+    const uint32_t *lh;
+};
+
+// Per-column reduction of block partials (one block per column, fixed tree order).  The
+// reduction kernels run one workgroup per column-state index c = m * k + j: column j of
+// packed member m (m = 0 without packing).
 __device__ inline double reduce_partials(const double *part, uint32_t nb, uint32_t kp, uint32_t stride_fields,
-                                         uint32_t field, uint32_t j, double *sh, bool is_max) {
+                                         uint32_t field, uint32_t j, double *sh, bool is_max, Ranges rg = {},
+                                         uint32_t m = 0) {
     double acc = 0.0;
-    for (uint32_t b = threadIdx.x; b < nb; b += BLK) {
-        double v = part[((uint64_t)b * kp + j) * stride_fields + field];
-        acc = is_max ? fmax(acc, v) : acc + v;
+    const uint32_t r0 = rg.ptr ? rg.ptr[m] : 0u, r1 = rg.ptr ? rg.ptr[m + 1] : 1u;
+    for (uint32_t r = r0; r < r1; ++r) {
+        const uint32_t lo = rg.ptr ? rg.lh[2 * r] : 0u, hi = rg.ptr ? rg.lh[2 * r + 1] : nb;
+        for (uint32_t b = lo + threadIdx.x; b < hi; b += BLK) {
+            double v = part[((uint64_t)b * kp + j) * stride_fields + field];
+            acc = is_max ? fmax(acc, v) : acc + v;
+        }
     }
     sh[threadIdx.x] = acc;
     __syncthreads();
@@ -406,19 +425,19 @@ __device__ inline void set_beta1(ColState &cs, uint32_t j, double beta, double t
 constexpr double NORM_SAFE_LO = 1e-140, NORM_SAFE_HI = 1e140;
 
 __global__ __launch_bounds__(BLK) void k_beta1(const double *part, uint32_t nb, uint32_t kp, ColState cs, double tol,
-                                               int32_t *need_scaled) {
+                                               int32_t *need_scaled, uint32_t k, Ranges rg) {
     __shared__ double sh[BLK];
-    const uint32_t j = blockIdx.x;
-    const double s = reduce_partials(part, nb, kp, 2, 0, j, sh, false);
-    const double m = reduce_partials(part, nb, kp, 2, 1, j, sh, true);
+    const uint32_t c = blockIdx.x, j = c % k;
+    const double s = reduce_partials(part, nb, kp, 2, 0, j, sh, false, rg, c / k);
+    const double m = reduce_partials(part, nb, kp, 2, 1, j, sh, true, rg, c / k);
     if (threadIdx.x == 0) {
-        cs.bnorm_scale[j] = m;
+        cs.bnorm_scale[c] = m;
         double beta = sqrt(s);
         if (m != 0.0 && !(m >= NORM_SAFE_LO && m <= NORM_SAFE_HI)) {
             beta = m; // provisional; replaced by the scaled pass
             need_scaled[0] = 1;
         }
-        set_beta1(cs, j, m == 0.0 ? 0.0 : beta, tol);
+        set_beta1(cs, c, m == 0.0 ? 0.0 : beta, tol);
     }
 }
 
@@ -465,16 +484,18 @@ __global__ __launch_bounds__(BLK) void k_beta1_scaled(const double *part, uint32
 template <int PREC, bool INIT>
 __global__ __launch_bounds__(BLK) void k_ptilde(uint64_t N, const double *__restrict__ atu, double *__restrict__ pt,
                                                 double *__restrict__ v, const double *__restrict__ inv_diag,
-                                                ColState cs, double *__restrict__ part, uint32_t k, uint32_t kshift) {
+                                                ColState cs, double *__restrict__ part, uint32_t k, uint32_t kshift,
+                                                const uint32_t *__restrict__ blk_col) {
     __shared__ double sh[BLK];
     const uint32_t kp = 1u << kshift;
     const uint32_t j = threadIdx.x & (kp - 1);
+    const uint32_t c = (blk_col ? blk_col[blockIdx.x] : 0u) + j;
     const uint64_t rstride = ((uint64_t)gridDim.x * BLK) >> kshift;
     double vp = 0.0, vv = 0.0, pp = 0.0;
-    if (j < k && cs.active[j]) {
-        const bool brk = !INIT && cs.brk[j];
-        const double bi = cs.beta_inv[j];
-        const double pc = INIT ? 0.0 : cs.p_coeff[j];
+    if (j < k && cs.active[c]) {
+        const bool brk = !INIT && cs.brk[c];
+        const double bi = cs.beta_inv[c];
+        const double pc = INIT ? 0.0 : cs.p_coeff[c];
         for (uint64_t dof = ((uint64_t)blockIdx.x * BLK + threadIdx.x) >> kshift; dof < N; dof += rstride) {
             const uint64_t q = dof * k + j;
             double p;
@@ -497,24 +518,26 @@ __global__ __launch_bounds__(BLK) void k_ptilde(uint64_t N, const double *__rest
     __syncthreads();
     double b = block_reduce_cols(vv, kp, sh);
     __syncthreads();
-    double c = block_reduce_cols(pp, kp, sh);
+    double cc = block_reduce_cols(pp, kp, sh);
     if (threadIdx.x < kp) {
         double *o = part + ((uint64_t)blockIdx.x * kp + threadIdx.x) * 3;
         o[0] = a;
         o[1] = b;
-        o[2] = c;
+        o[2] = cc;
     }
 }
 
 // <v,p~>, <v,v>, <p~,p~> partials after an external preconditioner apply.
 __global__ __launch_bounds__(BLK) void k_dots(uint64_t N, const double *__restrict__ pt, const double *__restrict__ v,
-                                              ColState cs, double *__restrict__ part, uint32_t k, uint32_t kshift) {
+                                              ColState cs, double *__restrict__ part, uint32_t k, uint32_t kshift,
+                                              const uint32_t *__restrict__ blk_col) {
     __shared__ double sh[BLK];
     const uint32_t kp = 1u << kshift;
     const uint32_t j = threadIdx.x & (kp - 1);
+    const uint32_t c = (blk_col ? blk_col[blockIdx.x] : 0u) + j;
     const uint64_t rstride = ((uint64_t)gridDim.x * BLK) >> kshift;
     double vp = 0.0, vv = 0.0, pp = 0.0;
-    if (j < k && cs.active[j]) {
+    if (j < k && cs.active[c]) {
         for (uint64_t dof = ((uint64_t)blockIdx.x * BLK + threadIdx.x) >> kshift; dof < N; dof += rstride) {
             const double z = v[dof * k + j], p = pt[dof * k + j];
             vp += z * p;
@@ -526,12 +549,12 @@ __global__ __launch_bounds__(BLK) void k_dots(uint64_t N, const double *__restri
     __syncthreads();
     double b = block_reduce_cols(vv, kp, sh);
     __syncthreads();
-    double c = block_reduce_cols(pp, kp, sh);
+    double cc = block_reduce_cols(pp, kp, sh);
     if (threadIdx.x < kp) {
         double *o = part + ((uint64_t)blockIdx.x * kp + threadIdx.x) * 3;
         o[0] = a;
         o[1] = b;
-        o[2] = c;
+        o[2] = cc;
     }
 }
 
@@ -592,65 +615,66 @@ __device__ inline double alpha_from_parts(double vp, double vv, double pp, bool 
 }
 
 // beta = ||u|| after the gather; breakdown flags and the p~ update coefficients.
-__global__ __launch_bounds__(BLK) void k_beta(const double *part, uint32_t nb, uint32_t kp, ColState cs) {
+__global__ __launch_bounds__(BLK) void k_beta(const double *part, uint32_t nb, uint32_t kp, ColState cs, uint32_t k,
+                                              Ranges rg) {
     __shared__ double sh[BLK];
-    const uint32_t j = blockIdx.x;
-    double s = reduce_partials(part, nb, kp, 1, 0, j, sh, false);
-    if (threadIdx.x == 0 && cs.active[j]) {
+    const uint32_t c = blockIdx.x, j = c % k;
+    double s = reduce_partials(part, nb, kp, 1, 0, j, sh, false, rg, c / k);
+    if (threadIdx.x == 0 && cs.active[c]) {
         double beta = sqrt(s);
-        cs.beta[j] = beta;
+        cs.beta[c] = beta;
         if (beta == 0.0) { // lucky breakdown (bidiag.rs:282-291)
-            cs.brk[j] = 1;
-            cs.beta_inv[j] = 0.0;
-            cs.p_coeff[j] = 0.0;
+            cs.brk[c] = 1;
+            cs.beta_inv[c] = 0.0;
+            cs.p_coeff[c] = 0.0;
         } else {
-            cs.brk[j] = 0;
-            cs.beta_inv[j] = 1.0 / beta;
-            cs.p_coeff[j] = beta / cs.alpha[j];
+            cs.brk[c] = 0;
+            cs.beta_inv[c] = 1.0 / beta;
+            cs.p_coeff[c] = beta / cs.alpha[c];
         }
     }
 }
 
 // First alpha (ModifiedGolubKahan::init, bidiag.rs:428-472) and the recurrence seed
 // (LsmrRecurrenceState::init, recurrence.rs:68-79; lsmr_from_bidiag early exit lsmr.rs:306-316).
-__global__ __launch_bounds__(BLK) void k_alpha1(const double *part, uint32_t nb, uint32_t kp, ColState cs) {
+__global__ __launch_bounds__(BLK) void k_alpha1(const double *part, uint32_t nb, uint32_t kp, ColState cs, uint32_t k, Ranges rg) {
     __shared__ double sh[BLK];
-    const uint32_t j = blockIdx.x;
-    double vp = reduce_partials(part, nb, kp, 3, 0, j, sh, false);
-    double vv = reduce_partials(part, nb, kp, 3, 1, j, sh, false);
-    double pp = reduce_partials(part, nb, kp, 3, 2, j, sh, false);
-    if (threadIdx.x == 0 && cs.active[j]) {
-        double beta = cs.beta[j];
-        cs.den_norm[j] = beta * sqrt(pp);
+    const uint32_t c = blockIdx.x, j = c % k;
+    double vp = reduce_partials(part, nb, kp, 3, 0, j, sh, false, rg, c / k);
+    double vv = reduce_partials(part, nb, kp, 3, 1, j, sh, false, rg, c / k);
+    double pp = reduce_partials(part, nb, kp, 3, 2, j, sh, false, rg, c / k);
+    if (threadIdx.x == 0 && cs.active[c]) {
+        double beta = cs.beta[c];
+        cs.den_norm[c] = beta * sqrt(pp);
         bool indef;
         double alpha = alpha_from_parts(vp, vv, pp, indef);
         if (indef) {
-            cs.err[j] = 1;
-            cs.active[j] = 0;
+            cs.err[c] = 1;
+            cs.active[c] = 0;
             return;
         }
-        cs.alpha[j] = alpha;
-        cs.inv_alpha[j] = alpha > 0.0 ? 1.0 / alpha : 0.0;
-        cs.beta_prev_inv[j] = 1.0;
+        cs.alpha[c] = alpha;
+        cs.inv_alpha[c] = alpha > 0.0 ? 1.0 / alpha : 0.0;
+        cs.beta_prev_inv[c] = 1.0;
         if (alpha == 0.0) {
-            cs.stop[j] = WX_STOP_INITIAL_NORMAL_EQ_ZERO;
-            cs.active[j] = 0;
-            cs.phi_bar[j] = beta;
-            cs.zeta_bar[j] = 0.0;
-            cs.zeta0[j] = 1.0;
+            cs.stop[c] = WX_STOP_INITIAL_NORMAL_EQ_ZERO;
+            cs.active[c] = 0;
+            cs.phi_bar[c] = beta;
+            cs.zeta_bar[c] = 0.0;
+            cs.zeta0[c] = 1.0;
             return;
         }
-        cs.alpha_bar[j] = alpha;
-        cs.phi_bar[j] = beta;
-        cs.c_bar[j] = 1.0;
-        cs.s_bar[j] = 0.0;
+        cs.alpha_bar[c] = alpha;
+        cs.phi_bar[c] = beta;
+        cs.c_bar[c] = 1.0;
+        cs.s_bar[c] = 0.0;
         double zb = alpha * beta;
-        cs.zeta_bar[j] = zb;
-        cs.zeta0[j] = fmax(fabs(zb), 2.2250738585072014e-308);
-        cs.a_norm_sq[j] = alpha * alpha;
-        cs.prev_rho[j] = 1.0;
-        cs.prev_rho_bar[j] = 1.0;
-        cs.gscale[j] = -(alpha * cs.beta_inv[j]); // u_1 is kept as b = beta_1 * u_1-hat
+        cs.zeta_bar[c] = zb;
+        cs.zeta0[c] = fmax(fabs(zb), 2.2250738585072014e-308);
+        cs.a_norm_sq[c] = alpha * alpha;
+        cs.prev_rho[c] = 1.0;
+        cs.prev_rho_bar[c] = 1.0;
+        cs.gscale[c] = -(alpha * cs.beta_inv[c]); // u_1 is kept as b = beta_1 * u_1-hat
     }
 }
 
@@ -658,12 +682,14 @@ __global__ __launch_bounds__(BLK) void k_alpha1(const double *part, uint32_t nb,
 __global__ __launch_bounds__(BLK) void k_init_solution(uint64_t N, double *__restrict__ v, double *__restrict__ h,
                                                        double *__restrict__ hb, double *__restrict__ x, ColState cs,
                                                        uint32_t k, uint32_t kshift, const double *__restrict__ pt,
-                                                       double *__restrict__ Vpush, double *__restrict__ Ppush) {
+                                                       double *__restrict__ Vpush, double *__restrict__ Ppush,
+                                                       const uint32_t *__restrict__ blk_col) {
     const uint32_t kp = 1u << kshift;
     const uint32_t j = threadIdx.x & (kp - 1);
     if (j >= k) return;
-    const bool act = cs.active[j];
-    const double ia = cs.inv_alpha[j];
+    const uint32_t c = (blk_col ? blk_col[blockIdx.x] : 0u) + j;
+    const bool act = cs.active[c];
+    const double ia = cs.inv_alpha[c];
     const uint64_t rstride = ((uint64_t)gridDim.x * BLK) >> kshift;
     for (uint64_t dof = ((uint64_t)blockIdx.x * BLK + threadIdx.x) >> kshift; dof < N; dof += rstride) {
         const uint64_t q = dof * k + j;
@@ -690,78 +716,78 @@ __global__ __launch_bounds__(BLK) void k_init_solution(uint64_t N, double *__res
 // The iteration count is kept per column on the device, so the launch is identical every
 // iteration and a captured graph can replay it.
 __global__ __launch_bounds__(BLK) void k_alpha_step(const double *part, uint32_t nb, uint32_t kp, ColState cs,
-                                                    double rel_tol) {
+                                                    double rel_tol, uint32_t k, Ranges rg) {
     __shared__ double sh[BLK];
-    const uint32_t j = blockIdx.x;
-    double vp = reduce_partials(part, nb, kp, 3, 0, j, sh, false);
-    double vv = reduce_partials(part, nb, kp, 3, 1, j, sh, false);
-    double pp = reduce_partials(part, nb, kp, 3, 2, j, sh, false);
+    const uint32_t c = blockIdx.x, j = c % k;
+    double vp = reduce_partials(part, nb, kp, 3, 0, j, sh, false, rg, c / k);
+    double vv = reduce_partials(part, nb, kp, 3, 1, j, sh, false, rg, c / k);
+    double pp = reduce_partials(part, nb, kp, 3, 2, j, sh, false, rg, c / k);
     if (threadIdx.x != 0) return;
-    if (!cs.active[j]) {
-        cs.upd[j] = 0;
+    if (!cs.active[c]) {
+        cs.upd[c] = 0;
         return;
     }
-    double alpha, beta = cs.beta[j];
-    if (cs.brk[j]) {
+    double alpha, beta = cs.beta[c];
+    if (cs.brk[c]) {
         alpha = 0.0;
     } else {
         bool indef;
         alpha = alpha_from_parts(vp, vv, pp, indef);
         if (indef) {
-            cs.err[j] = 1;
-            cs.active[j] = 0;
-            cs.upd[j] = 0;
-            cs.iters[j] += 1; // steps taken (reset by set_beta1)
+            cs.err[c] = 1;
+            cs.active[c] = 0;
+            cs.upd[c] = 0;
+            cs.iters[c] += 1; // steps taken (reset by set_beta1)
             return;
         }
     }
-    cs.inv_alpha[j] = alpha > 0.0 ? 1.0 / alpha : 0.0;
+    cs.inv_alpha[c] = alpha > 0.0 ? 1.0 / alpha : 0.0;
     // convergence.observe
-    double a_norm_sq = cs.a_norm_sq[j] + alpha * alpha + beta * beta;
-    cs.a_norm_sq[j] = a_norm_sq;
+    double a_norm_sq = cs.a_norm_sq[c] + alpha * alpha + beta * beta;
+    cs.a_norm_sq[c] = a_norm_sq;
     // recurrence.step
     double ph_c, ph_s, ph_r;
-    givens(cs.alpha_bar[j], beta, ph_c, ph_s, ph_r);
+    givens(cs.alpha_bar[c], beta, ph_c, ph_s, ph_r);
     const double theta_new = ph_s * alpha;
     const double alpha_bar_new = -ph_c * alpha;
-    const double phi_bar_new = ph_s * cs.phi_bar[j];
-    const double theta_bar = cs.s_bar[j] * ph_r;
+    const double phi_bar_new = ph_s * cs.phi_bar[c];
+    const double theta_bar = cs.s_bar[c] * ph_r;
     double pb_c, pb_s, pb_r;
-    givens(cs.c_bar[j] * ph_r, theta_new, pb_c, pb_s, pb_r);
-    const double zeta_bar = cs.zeta_bar[j];
+    givens(cs.c_bar[c] * ph_r, theta_new, pb_c, pb_s, pb_r);
+    const double zeta_bar = cs.zeta_bar[c];
     const double zeta = pb_c * zeta_bar;
     const double zeta_bar_new = -pb_s * zeta_bar;
-    cs.alpha_bar[j] = alpha_bar_new;
-    cs.phi_bar[j] = phi_bar_new;
-    cs.c_bar[j] = pb_c;
-    cs.s_bar[j] = pb_s;
-    cs.zeta_bar[j] = zeta_bar_new;
+    cs.alpha_bar[c] = alpha_bar_new;
+    cs.phi_bar[c] = phi_bar_new;
+    cs.c_bar[c] = pb_c;
+    cs.s_bar[c] = pb_s;
+    cs.zeta_bar[c] = zeta_bar_new;
     // solution.update coefficients
     const double eps = 2.220446049250313e-16;
     const double t_x_denom = ph_r * pb_r;
-    cs.t_x[j] = fabs(t_x_denom) > eps ? zeta / t_x_denom : 0.0;
-    const double t_hbar_denom = cs.prev_rho[j] * cs.prev_rho_bar[j];
-    cs.t_hbar[j] = fabs(t_hbar_denom) > eps ? theta_bar * ph_r / t_hbar_denom : 0.0;
-    cs.t_h[j] = fabs(ph_r) > eps ? theta_new / ph_r : 0.0;
-    cs.prev_rho[j] = ph_r;
-    cs.prev_rho_bar[j] = pb_r;
+    cs.t_x[c] = fabs(t_x_denom) > eps ? zeta / t_x_denom : 0.0;
+    const double t_hbar_denom = cs.prev_rho[c] * cs.prev_rho_bar[c];
+    cs.t_hbar[c] = fabs(t_hbar_denom) > eps ? theta_bar * ph_r / t_hbar_denom : 0.0;
+    cs.t_h[c] = fabs(ph_r) > eps ? theta_new / ph_r : 0.0;
+    cs.prev_rho[c] = ph_r;
+    cs.prev_rho_bar[c] = pb_r;
     // bidiagonal state for the next step
-    cs.gscale[j] = -(alpha * cs.beta_inv[j]); // -(alpha_k / beta_k); 0 after a breakdown
-    cs.alpha[j] = alpha;
+    cs.gscale[c] = -(alpha * cs.beta_inv[c]); // -(alpha_k / beta_k); 0 after a breakdown
+    cs.alpha[c] = alpha;
     // convergence.check
     const double residual = fabs(phi_bar_new);
     int32_t stop = -1;
-    if (residual <= cs.abs_tol[j]) stop = WX_STOP_RESIDUAL_TOLERANCE;
+    if (residual <= cs.abs_tol[c]) stop = WX_STOP_RESIDUAL_TOLERANCE;
     else {
         const double a_norm = fmax(sqrt(a_norm_sq), 2.2250738585072014e-308);
         const double normar = fabs(zeta_bar_new);
         if (normar / (a_norm * fmax(residual, 2.2250738585072014e-308)) <= rel_tol) stop = WX_STOP_NORMAL_EQ_TOLERANCE;
     }
-    cs.upd[j] = 1;
-    cs.iters[j] += 1; // steps taken (reset by set_beta1)
+    cs.upd[c] = 1;
+    cs.iters[c] += 1; // steps taken (reset by set_beta1)
     if (stop >= 0) {
-        cs.stop[j] = stop;
-        cs.active[j] = 0;
+        cs.stop[c] = stop;
+        cs.active[c] = 0;
     }
 }
 
@@ -770,11 +796,13 @@ __global__ __launch_bounds__(BLK) void k_alpha_step(const double *part, uint32_t
 __global__ __launch_bounds__(BLK) void k_update_solution(uint64_t N, double *__restrict__ v, double *__restrict__ h,
                                                          double *__restrict__ hb, double *__restrict__ x, ColState cs,
                                                          uint32_t k, uint32_t kshift, const double *__restrict__ pt,
-                                                         double *__restrict__ Vpush, double *__restrict__ Ppush) {
+                                                         double *__restrict__ Vpush, double *__restrict__ Ppush,
+                                                         const uint32_t *__restrict__ blk_col) {
     const uint32_t kp = 1u << kshift;
     const uint32_t j = threadIdx.x & (kp - 1);
-    if (j >= k || !cs.upd[j]) return;
-    const double ia = cs.inv_alpha[j], t_x = cs.t_x[j], t_hbar = cs.t_hbar[j], t_h = cs.t_h[j];
+    const uint32_t c = (blk_col ? blk_col[blockIdx.x] : 0u) + j;
+    if (j >= k || !cs.upd[c]) return;
+    const double ia = cs.inv_alpha[c], t_x = cs.t_x[c], t_hbar = cs.t_hbar[c], t_h = cs.t_h[c];
     const bool scale = ia > 0.0;
     const uint64_t rstride = ((uint64_t)gridDim.x * BLK) >> kshift;
     for (uint64_t dof = ((uint64_t)blockIdx.x * BLK + threadIdx.x) >> kshift; dof < N; dof += rstride) {
@@ -828,7 +856,7 @@ __global__ __launch_bounds__(BLK) void k_demean(DesignArgs d, const double *__re
 // 64-row LDS tiles (k+1 doubles per tile row) so the column-major writes are coalesced.
 __global__ __launch_bounds__(BLK) void k_store_caller(uint64_t n, const uint32_t *__restrict__ inv_perm,
                                                       const double *__restrict__ dm_int, double *__restrict__ Dm,
-                                                      uint64_t ldd, uint32_t k, uint32_t kshift) {
+                                                      uint64_t ldd, uint32_t k, uint32_t kshift, uint32_t kv) {
     extern __shared__ double tile[]; // 64 x (k + 1)
     const uint32_t ld = k + 1;
     const uint32_t kp = 1u << kshift, j = threadIdx.x & (kp - 1), grp = threadIdx.x >> kshift;
@@ -846,7 +874,7 @@ __global__ __launch_bounds__(BLK) void k_store_caller(uint64_t n, const uint32_t
         for (uint32_t q = threadIdx.x; q < 64 * k; q += BLK) {
             const uint32_t c = q >> 6, rr = q & 63;
             const uint64_t r = r0 + rr;
-            if (r < n) Dm[r + c * ldd] = tile[rr * ld + c];
+            if (r < n && c < kv) Dm[r + c * ldd] = tile[rr * ld + c];
         }
         __syncthreads();
     }
@@ -858,6 +886,23 @@ __global__ void k_store_x(uint64_t N, const double *__restrict__ x, double *__re
     if (idx >= N * k) return;
     const uint64_t dof = idx / k, j = idx % k;
     X[dof + j * ldx] = x[idx];
+}
+
+// Packed x -> each member's X (column-major, its own DOF order, leading dimension own_ld[m]);
+// padding DOFs (own index -1) are skipped.
+__global__ void k_store_x_packed(uint64_t N, const double *__restrict__ x, uint32_t k,
+                                 const uint32_t *__restrict__ dof_member, const int32_t *__restrict__ dof_own,
+                                 double *const *__restrict__ X, const uint64_t *__restrict__ own_ld,
+                                 const uint32_t *__restrict__ own_kv) {
+    const uint64_t idx = (uint64_t)blockIdx.x * BLK + threadIdx.x;
+    if (idx >= N * k) return;
+    const uint64_t dof = idx / k, j = idx % k;
+    const int32_t own = dof_own[dof];
+    if (own < 0) return;
+    const uint32_t m = dof_member[dof];
+    if (j >= own_kv[m]) return;
+    double *Xm = X[m];
+    if (Xm) Xm[(uint64_t)own + j * own_ld[m]] = x[idx];
 }
 
 // sum of squares per column of an N x k interleaved array (block partials)
@@ -877,11 +922,12 @@ __global__ __launch_bounds__(BLK) void k_sumsq(uint64_t N, const double *__restr
     if (threadIdx.x < kp) part[(uint64_t)blockIdx.x * kp + threadIdx.x] = tot;
 }
 
-__global__ __launch_bounds__(BLK) void k_reduce_to(const double *part, uint32_t nb, uint32_t kp, double *outv) {
+__global__ __launch_bounds__(BLK) void k_reduce_to(const double *part, uint32_t nb, uint32_t kp, double *outv,
+                                                   uint32_t k = 1, Ranges rg = {}) {
     __shared__ double sh[BLK];
-    const uint32_t j = blockIdx.x;
-    double s = reduce_partials(part, nb, kp, 1, 0, j, sh, false);
-    if (threadIdx.x == 0) outv[j] = s;
+    const uint32_t c = blockIdx.x, j = rg.ptr ? c % k : c;
+    double s = reduce_partials(part, nb, kp, 1, 0, j, sh, false, rg, rg.ptr ? c / k : 0u);
+    if (threadIdx.x == 0) outv[c] = s;
 }
 
 uint32_t grid_for(uint64_t rows, uint32_t kshift, uint32_t cap = MAX_PART_BLOCKS) {
@@ -1136,18 +1182,23 @@ void launch_gather(const DevDesign &dv, const DesignArgs &da, bool first, uint32
 
 // Gather fused with the D' of the sorted term dv.fuse_term (k_gather_seg); returns the grid,
 // i.e. the number of norm partials written.
+// A packed design (pk) launches exactly one item per row group, so that every workgroup's
+// items belong to one member, and names the member's column-state base per workgroup.
 template <bool W, bool F>
 uint32_t launch_gather_seg_t(const DevDesign &dv, const DesignArgs &da, const double *sw, const double *v,
                              const double *uin, double *u, double *atu, double *partials, const ColState &cs,
-                             double *part, uint32_t k, uint32_t kshift, hipStream_t st) {
+                             double *part, uint32_t k, uint32_t kshift, hipStream_t st, const Pack *pk) {
     const DevTerm &dt = dv.terms[dv.fuse_term];
     const uint32_t groups_per_block = BLK >> kshift;
-    const uint32_t grid = std::max<uint32_t>(
-        1, std::min<uint32_t>((dt.n_items + groups_per_block - 1) / groups_per_block, MAX_FUSED_BLOCKS));
+    const uint32_t grid = pk ? pk->n_item_blocks
+                             : std::max<uint32_t>(1, std::min<uint32_t>((dt.n_items + groups_per_block - 1) /
+                                                                            groups_per_block,
+                                                                        MAX_FUSED_BLOCKS));
+    const uint32_t *blk_col = pk ? pk->item_col.as<uint32_t>() : nullptr;
     const SegItem *items = dt.items.as<SegItem>();
 #define WX_GS(TT, PL)                                                                                            \
     hipLaunchKernelGGL((k_gather_seg<W, F, TT, PL>), dim3(grid), dim3(BLK), 0, st, da, dt.info, items, dt.n_items, \
-                       sw, v, uin, u, atu, partials, cs, part, k, kshift)
+                       sw, v, uin, u, atu, partials, cs, part, k, kshift, blk_col)
     switch (dv.plain ? da.T : 0) {
     case 1: WX_GS(1, true); break;
     case 2: WX_GS(2, true); break;
@@ -1161,13 +1212,14 @@ uint32_t launch_gather_seg_t(const DevDesign &dv, const DesignArgs &da, const do
 
 uint32_t launch_gather_seg(const DevDesign &dv, const DesignArgs &da, bool first, const double *v, const double *uin,
                            double *u, double *atu, double *partials, const ColState &cs, double *part, uint32_t k,
-                           uint32_t kshift, hipStream_t st) {
+                           uint32_t kshift, hipStream_t st, const Pack *pk = nullptr) {
     const double *sw = dv.sw.p ? dv.sw.as<double>() : nullptr;
     if (sw)
-        return first ? launch_gather_seg_t<true, true>(dv, da, sw, v, uin, u, atu, partials, cs, part, k, kshift, st)
-                     : launch_gather_seg_t<true, false>(dv, da, sw, v, uin, u, atu, partials, cs, part, k, kshift, st);
-    return first ? launch_gather_seg_t<false, true>(dv, da, sw, v, uin, u, atu, partials, cs, part, k, kshift, st)
-                 : launch_gather_seg_t<false, false>(dv, da, sw, v, uin, u, atu, partials, cs, part, k, kshift, st);
+        return first ? launch_gather_seg_t<true, true>(dv, da, sw, v, uin, u, atu, partials, cs, part, k, kshift, st, pk)
+                     : launch_gather_seg_t<true, false>(dv, da, sw, v, uin, u, atu, partials, cs, part, k, kshift, st,
+                                                        pk);
+    return first ? launch_gather_seg_t<false, true>(dv, da, sw, v, uin, u, atu, partials, cs, part, k, kshift, st, pk)
+                 : launch_gather_seg_t<false, false>(dv, da, sw, v, uin, u, atu, partials, cs, part, k, kshift, st, pk);
 }
 
 void launch_demean(const DevDesign &dv, const DesignArgs &da, uint32_t grid, const double *x, const double *y_int,
@@ -1224,6 +1276,7 @@ struct PhaseTimer {
 // width; a solve takes one out and returns it, so concurrent solves never share one).
 struct Workspace {
     uint32_t k = 0;
+    uint32_t cap = MAX_BATCH; // column-state entries (packed members x k)
     DevBuf u, v, pt, atu, h, hb, x, yint, part, segpart, cols, ints, bad, red;
     ColState cs{};
     std::unique_ptr<SchwarzRun> runs[2];
@@ -1251,8 +1304,9 @@ struct Workspace {
 };
 
 void setup_colstate(Workspace &ws) {
-    ws.cols.alloc(sizeof(double) * MAX_BATCH * N_DOUBLE_FIELDS);
-    ws.ints.alloc(sizeof(int32_t) * MAX_BATCH * N_INT_FIELDS);
+    const uint32_t cap = ws.cap;
+    ws.cols.alloc(sizeof(double) * cap * N_DOUBLE_FIELDS);
+    ws.ints.alloc(sizeof(int32_t) * cap * N_INT_FIELDS);
     double *dp = ws.cols.as<double>();
     double **fields[N_DOUBLE_FIELDS] = {&ws.cs.alpha, &ws.cs.beta, &ws.cs.beta_prev_inv, &ws.cs.beta_inv,
                                         &ws.cs.p_coeff, &ws.cs.gscale, &ws.cs.alpha_bar, &ws.cs.phi_bar,
@@ -1260,26 +1314,43 @@ void setup_colstate(Workspace &ws) {
                                         &ws.cs.a_norm_sq, &ws.cs.abs_tol, &ws.cs.prev_rho, &ws.cs.prev_rho_bar,
                                         &ws.cs.t_x, &ws.cs.t_hbar, &ws.cs.t_h, &ws.cs.inv_alpha,
                                         &ws.cs.bnorm_scale, &ws.cs.den_norm};
-    for (int f = 0; f < N_DOUBLE_FIELDS; ++f) *fields[f] = dp + f * MAX_BATCH;
+    for (int f = 0; f < N_DOUBLE_FIELDS; ++f) *fields[f] = dp + (uint64_t)f * cap;
     int32_t *ip = ws.ints.as<int32_t>();
     int32_t **ifields[N_INT_FIELDS] = {&ws.cs.active, &ws.cs.upd, &ws.cs.brk, &ws.cs.stop, &ws.cs.iters, &ws.cs.err};
-    for (int f = 0; f < N_INT_FIELDS; ++f) *ifields[f] = ip + f * MAX_BATCH;
+    for (int f = 0; f < N_INT_FIELDS; ++f) *ifields[f] = ip + (uint64_t)f * cap;
 }
 
-std::shared_ptr<Workspace> new_workspace(const DevDesign &dv, uint32_t k) {
+// The column state of packed member m (entries m * k ...) seen as a state of k columns.
+ColState colstate_view(const ColState &cs, uint32_t off) {
+    ColState v = cs;
+    double **d[N_DOUBLE_FIELDS] = {&v.alpha, &v.beta, &v.beta_prev_inv, &v.beta_inv, &v.p_coeff, &v.gscale,
+                                   &v.alpha_bar, &v.phi_bar, &v.c_bar, &v.s_bar, &v.zeta_bar, &v.zeta0,
+                                   &v.a_norm_sq, &v.abs_tol, &v.prev_rho, &v.prev_rho_bar, &v.t_x, &v.t_hbar,
+                                   &v.t_h, &v.inv_alpha, &v.bnorm_scale, &v.den_norm};
+    for (auto *f : d) *f += off;
+    int32_t **i[N_INT_FIELDS] = {&v.active, &v.upd, &v.brk, &v.stop, &v.iters, &v.err};
+    for (auto *f : i) *f += off;
+    return v;
+}
+
+// cap: column-state entries (k, or members x k of a pack); part_blocks: partial blocks of the
+// largest exact-grid launch of a pack (0: the grid caps of the unpacked kernels)
+std::shared_ptr<Workspace> new_workspace(const DevDesign &dv, uint32_t k, uint32_t cap = MAX_BATCH,
+                                         uint64_t part_blocks = 0) {
     auto w = std::make_shared<Workspace>();
     const uint64_t n = dv.n, N = dv.N, kp = 1u << pow2_shift(k);
     w->k = k;
+    w->cap = std::max(cap, MAX_BATCH);
     setup_colstate(*w);
     w->u.alloc(n * k * 8);
     w->yint.alloc(n * k * 8);
     for (DevBuf *b : {&w->v, &w->pt, &w->atu, &w->h, &w->hb, &w->x}) b->alloc(N * k * 8);
-    w->part.alloc((size_t)std::max(MAX_PART_BLOCKS, MAX_FUSED_BLOCKS) * kp * 3 * 8);
+    w->part.alloc((size_t)std::max<uint64_t>(std::max(MAX_PART_BLOCKS, MAX_FUSED_BLOCKS), part_blocks) * kp * 3 * 8);
     w->segpart.alloc((size_t)std::max<uint32_t>(dv.max_partials, 1) * k * 8);
     w->bad.alloc(8);
-    w->red.alloc(2 * MAX_BATCH * 8);
-    WX_HIP(hipHostMalloc((void **)&w->hflags, (2 * MAX_BATCH + 2) * sizeof(int32_t), hipHostMallocDefault));
-    w->hbad = w->hflags + 2 * MAX_BATCH;
+    w->red.alloc(2 * (size_t)w->cap * 8);
+    WX_HIP(hipHostMalloc((void **)&w->hflags, (2 * (size_t)w->cap + 2) * sizeof(int32_t), hipHostMallocDefault));
+    w->hbad = w->hflags + 2 * w->cap;
     for (auto &e : w->flag_ev) WX_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
     return w;
 }
@@ -1288,7 +1359,8 @@ std::shared_ptr<Workspace> new_workspace(const DevDesign &dv, uint32_t k) {
 struct PooledWorkspace {
     const Solver &s;
     std::shared_ptr<Workspace> ws;
-    PooledWorkspace(const Solver &solver, uint32_t k) : s(solver) {
+    PooledWorkspace(const Solver &solver, uint32_t k, uint32_t cap = MAX_BATCH, uint64_t part_blocks = 0)
+        : s(solver) {
         {
             std::lock_guard<std::mutex> lk(s.pool_mu);
             for (size_t i = 0; i < s.pool.size(); ++i)
@@ -1299,7 +1371,7 @@ struct PooledWorkspace {
                     break;
                 }
         }
-        if (!ws) ws = new_workspace(s.dev, k);
+        if (!ws) ws = new_workspace(s.dev, k, cap, part_blocks);
     }
     ~PooledWorkspace() {
         std::lock_guard<std::mutex> lk(s.pool_mu);
@@ -1329,51 +1401,65 @@ namespace {
 //   finish()       demeaned output, x, FP64 true residual, statistics (synchronous)
 class ChunkRun {
   public:
+    // one chunk (columns c0 .. c0 + k) of one problem
     ChunkRun(const Solver &s, const double *Y, uint64_t ldy, uint64_t c0, uint32_t k, const wx_lsmr_options &opts,
              double *X, double *Dm, wx_rhs_stats *stats, hipStream_t st, bool x_solver_basis, PhaseTimer *timer,
              int group, bool use_graph)
-        : s_(s), dv_(s.dev), da_(s.dev.args()), Y_(Y + c0 * ldy), ldy_(ldy), c0_(c0), k_(k), opts_(opts), X_(X),
-          Dm_(Dm), stats_(stats), st_(st), x_solver_basis_(x_solver_basis), timer_(timer), pw_(s, k) {
-        n_ = dv_.n;
-        N_ = dv_.N;
-        weighted_ = dv_.sw.p != nullptr;
-        inv_perm_ = dv_.inv_perm.p ? dv_.inv_perm.as<uint32_t>() : nullptr;
-        maxiter_ = (int)std::max<int64_t>(0, opts.maxiter);
-        const Precond *pc = s.precond.get();
-        prec_kind_ = pc == nullptr ? 0 : (pc->kind == Precond::DIAGONAL ? 1 : 2);
-        inv_diag_ = prec_kind_ == 1 ? dv_.inv_diag.as<double>() : nullptr;
-        const int prec_slot = opts.precision == WX_PRECISION_MIXED ? 1 : 0;
-        kshift_ = pow2_shift(k);
-        kp_ = 1u << kshift_;
-        t0_ = std::chrono::steady_clock::now();
-        ws_ = pw_.ws.get();
-        if (prec_kind_ == 2 && !ws_->runs[prec_slot]) ws_->runs[prec_slot] = make_schwarz_run(*pc, opts.precision, st);
-        schwarz_ = prec_kind_ == 2 ? ws_->runs[prec_slot].get() : nullptr;
-        cs_ = ws_->cs;
-        gn_ = grid_for(n_, kshift_);
-        gN_ = grid_for(N_, kshift_);
-        part_ = ws_->part.as<double>();
-        gt_ = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>((n_ + 63) / 64, MAX_PART_BLOCKS));
-        tile_bytes_ = (size_t)64 * (k + 1) * sizeof(double);
-        cap_ = opts.local_size > 0 ? std::min<uint64_t>((uint64_t)opts.local_size, std::min(n_, N_)) : 0;
-        group_ = std::max(1, group);
-        // a graph replays fixed launch arguments: not with the reorthogonalisation ring (its
-        // slot changes every iteration) nor with the event timer
-        use_graph_ = use_graph && cap_ == 0 && !(timer_ && timer_->on);
-        graph_key_ = (uint64_t)group_ * 4 + (uint64_t)prec_slot;
+        : ChunkRun(s, nullptr, k, opts, st, timer, group, use_graph) {
+        Y_ = Y + c0 * ldy;
+        ldy_ = ldy;
+        c0_ = c0;
+        X_ = X;
+        Dm_ = Dm;
+        stats_ = stats;
+        x_solver_basis_ = x_solver_basis;
+    }
+    // a pack: member m solves mY[m] (n_m x k column-major, leading dimension mld[m]) and
+    // writes mX[m], mDm[m] (either may be null) and mStats[m][0 .. k)
+    ChunkRun(const Pack &pk, std::vector<const double *> mY, std::vector<uint64_t> mld, const wx_lsmr_options &opts,
+             std::vector<double *> mX, std::vector<double *> mDm, std::vector<wx_rhs_stats *> mStats, hipStream_t st,
+             int group, bool use_graph)
+        : ChunkRun(*pk.merged, &pk, pk.k, opts, st, nullptr, group, use_graph) {
+        mY_ = std::move(mY);
+        mld_ = std::move(mld);
+        mX_ = std::move(mX);
+        mDm_ = std::move(mDm);
+        mStats_ = std::move(mStats);
     }
 
     void launch_load() {
         WX_HIP(hipMemsetAsync(ws_->bad.p, 0, 8, st_));
         int32_t *badp = ws_->bad.as<int32_t>();
-        if (weighted_)
-            hipLaunchKernelGGL((k_load_rhs<true>), dim3(gt_), dim3(BLK), tile_bytes_, st_, n_, inv_perm_,
-                               inv_perm_ ? dv_.sw_caller.as<double>() : dv_.sw.as<double>(), Y_, ldy_,
-                               ws_->yint.as<double>(), part_, badp, k_, kshift_);
-        else
-            hipLaunchKernelGGL((k_load_rhs<false>), dim3(gt_), dim3(BLK), tile_bytes_, st_, n_, inv_perm_, nullptr, Y_,
-                               ldy_, ws_->yint.as<double>(), part_, badp, k_, kshift_);
-        hipLaunchKernelGGL(k_beta1, dim3(k_), dim3(BLK), 0, st_, part_, gt_, kp_, cs_, opts_.tol, badp + 1);
+        if (pk_) {
+            // each member's rows from its own caller order into its slice of the merged rows;
+            // its load partials land in its own block range
+            for (size_t m = 0; m < pk_->members.size(); ++m) {
+                const PackMember &mb = pk_->members[m];
+                const DevDesign &md = mb.s->dev;
+                const uint32_t *ip = md.inv_perm.p ? md.inv_perm.as<uint32_t>() : nullptr;
+                double *yint = ws_->yint.as<double>() + mb.row_off * k_;
+                double *pp = part_ + (uint64_t)mb.tile_off * kp_ * 2;
+                if (weighted_)
+                    hipLaunchKernelGGL((k_load_rhs<true>), dim3(mb.n_tiles), dim3(BLK), tile_bytes_, st_, mb.n, ip,
+                                       ip ? md.sw_caller.as<double>() : md.sw.as<double>(), mY_[m], mld_[m], yint, pp,
+                                       badp, k_, kshift_, mb.kv);
+                else
+                    hipLaunchKernelGGL((k_load_rhs<false>), dim3(mb.n_tiles), dim3(BLK), tile_bytes_, st_, mb.n, ip,
+                                       nullptr, mY_[m], mld_[m], yint, pp, badp, k_, kshift_, mb.kv);
+            }
+            hipLaunchKernelGGL(k_beta1, dim3(ncol_), dim3(BLK), 0, st_, part_, pk_->n_tiles, kp_, cs_, opts_.tol,
+                               badp + 1, k_, rg_tile_);
+        } else {
+            if (weighted_)
+                hipLaunchKernelGGL((k_load_rhs<true>), dim3(gt_), dim3(BLK), tile_bytes_, st_, n_, inv_perm_,
+                                   inv_perm_ ? dv_.sw_caller.as<double>() : dv_.sw.as<double>(), Y_, ldy_,
+                                   ws_->yint.as<double>(), part_, badp, k_, kshift_, k_);
+            else
+                hipLaunchKernelGGL((k_load_rhs<false>), dim3(gt_), dim3(BLK), tile_bytes_, st_, n_, inv_perm_, nullptr,
+                                   Y_, ldy_, ws_->yint.as<double>(), part_, badp, k_, kshift_, k_);
+            hipLaunchKernelGGL(k_beta1, dim3(k_), dim3(BLK), 0, st_, part_, gt_, kp_, cs_, opts_.tol, badp + 1, k_,
+                               Ranges{});
+        }
         WX_HIP(hipMemcpyAsync(ws_->hbad, ws_->bad.p, 8, hipMemcpyDeviceToHost, st_));
         WX_HIP(hipGetLastError());
     }
@@ -1382,32 +1468,39 @@ class ChunkRun {
         WX_HIP(hipStreamSynchronize(st_));
         if (ws_->hbad[0]) fail(WX_ERR_INVALID_INPUT, "Solver::solve: response contains non-finite values");
         if (ws_->hbad[1]) { // a column outside the safe range of the unscaled sum of squares
-            if (weighted_)
-                hipLaunchKernelGGL((k_scaled_sumsq<true>), dim3(gn_), dim3(BLK), 0, st_, n_, dv_.sw.as<double>(),
-                                   ws_->yint.as<double>(), cs_, part_, k_, kshift_);
-            else
-                hipLaunchKernelGGL((k_scaled_sumsq<false>), dim3(gn_), dim3(BLK), 0, st_, n_, nullptr,
-                                   ws_->yint.as<double>(), cs_, part_, k_, kshift_);
-            hipLaunchKernelGGL(k_beta1_scaled, dim3(k_), dim3(BLK), 0, st_, part_, gn_, kp_, cs_, opts_.tol);
+            const size_t nm = pk_ ? pk_->members.size() : 1;
+            for (size_t m = 0; m < nm; ++m) {
+                const uint64_t r0 = pk_ ? pk_->members[m].row_off : 0, nr = pk_ ? pk_->members[m].n : n_;
+                const ColState v = colstate_view(cs_, (uint32_t)(m * k_));
+                const uint32_t g = grid_for(nr, kshift_);
+                const double *y = ws_->yint.as<double>() + r0 * k_;
+                if (weighted_)
+                    hipLaunchKernelGGL((k_scaled_sumsq<true>), dim3(g), dim3(BLK), 0, st_, nr,
+                                       dv_.sw.as<double>() + r0, y, v, part_, k_, kshift_);
+                else
+                    hipLaunchKernelGGL((k_scaled_sumsq<false>), dim3(g), dim3(BLK), 0, st_, nr, nullptr, y, v, part_,
+                                       k_, kshift_);
+                hipLaunchKernelGGL(k_beta1_scaled, dim3(k_), dim3(BLK), 0, st_, part_, g, kp_, v, opts_.tol);
+            }
         }
         // init: p~ = A'b / beta_1, v = M^{-1} p~, alpha_1
         launch_adjoint(dv_, da_, weighted_ ? 2 : 0, ws_->yint.as<double>(), ws_->pt.as<double>(),
-                       ws_->segpart.as<double>(), cs_.active, k_, kshift_, st_);
+                       ws_->segpart.as<double>(), gate(), k_, kshift_, st_);
         const uint64_t N = N_;
         if (prec_kind_ == 2) {
             hipLaunchKernelGGL((k_ptilde<2, true>), dim3(gN_), dim3(BLK), 0, st_, N, nullptr, ws_->pt.as<double>(),
-                               ws_->v.as<double>(), nullptr, cs_, part_, k_, kshift_);
-            schwarz_->apply(ws_->pt.as<double>(), ws_->v.as<double>(), k_, st_, cs_.active);
+                               ws_->v.as<double>(), nullptr, cs_, part_, k_, kshift_, dof_col_);
+            schwarz_->apply(ws_->pt.as<double>(), ws_->v.as<double>(), k_, st_, gate());
             hipLaunchKernelGGL(k_dots, dim3(gN_), dim3(BLK), 0, st_, N, ws_->pt.as<double>(), ws_->v.as<double>(), cs_,
-                               part_, k_, kshift_);
+                               part_, k_, kshift_, dof_col_);
         } else if (prec_kind_ == 1) {
             hipLaunchKernelGGL((k_ptilde<1, true>), dim3(gN_), dim3(BLK), 0, st_, N, nullptr, ws_->pt.as<double>(),
-                               ws_->v.as<double>(), inv_diag_, cs_, part_, k_, kshift_);
+                               ws_->v.as<double>(), inv_diag_, cs_, part_, k_, kshift_, dof_col_);
         } else {
             hipLaunchKernelGGL((k_ptilde<0, true>), dim3(gN_), dim3(BLK), 0, st_, N, nullptr, ws_->pt.as<double>(),
-                               ws_->v.as<double>(), nullptr, cs_, part_, k_, kshift_);
+                               ws_->v.as<double>(), nullptr, cs_, part_, k_, kshift_, dof_col_);
         }
-        hipLaunchKernelGGL(k_alpha1, dim3(k_), dim3(BLK), 0, st_, part_, gN_, kp_, cs_);
+        hipLaunchKernelGGL(k_alpha1, dim3(ncol_), dim3(BLK), 0, st_, part_, gN_, kp_, cs_, k_, rg_dof_);
         // windowed reorthogonalisation ring: capacity min(local_size, n_obs, n_dofs)
         // (bidiag.rs:121-133); the write index is the same for every column
         if (cap_) {
@@ -1418,7 +1511,7 @@ class ChunkRun {
         const uint64_t push = cap_ ? ring_advance() : 0;
         hipLaunchKernelGGL(k_init_solution, dim3(gN_), dim3(BLK), 0, st_, N, ws_->v.as<double>(), ws_->h.as<double>(),
                            ws_->hb.as<double>(), ws_->x.as<double>(), cs_, k_, kshift_, ws_->pt.as<double>(),
-                           cap_ ? ring_v(push) : nullptr, cap_ ? ring_p(push) : nullptr);
+                           cap_ ? ring_v(push) : nullptr, cap_ ? ring_p(push) : nullptr, dof_col_);
         WX_HIP(hipGetLastError());
     }
 
@@ -1438,7 +1531,8 @@ class ChunkRun {
             itn_ += g;
         }
         const int slot = launches_ & 1;
-        WX_HIP(hipMemcpyAsync(ws_->hflags + slot * MAX_BATCH, cs_.active, k_ * 4, hipMemcpyDeviceToHost, st_));
+        WX_HIP(hipMemcpyAsync(ws_->hflags + (size_t)slot * ws_->cap, cs_.active, (size_t)ncol_ * 4,
+                              hipMemcpyDeviceToHost, st_));
         WX_HIP(hipEventRecord(ws_->flag_ev[slot], st_));
         ++launches_;
         WX_HIP(hipGetLastError());
@@ -1453,8 +1547,9 @@ class ChunkRun {
             if (q == hipErrorNotReady) return -1;
             WX_HIP(q);
         }
-        for (uint32_t j = 0; j < k_; ++j)
-            if (ws_->hflags[slot * MAX_BATCH + j]) return 0;
+        const int32_t *f = ws_->hflags + (size_t)slot * ws_->cap;
+        for (uint32_t c = 0; c < ncol_; ++c)
+            if (f[c]) return 0;
         return 1;
     }
 
@@ -1467,29 +1562,50 @@ class ChunkRun {
         double *dm_int = ws.u.as<double>();
         const uint32_t gd = grid_for(n, kshift, 65535);
         launch_demean(dv_, da, gd, ws.x.as<double>(), ws.yint.as<double>(), dm_int, k, kshift, st);
-        if (Dm_)
-            hipLaunchKernelGGL(k_store_caller, dim3(gt_), dim3(BLK), tile_bytes_, st, n, inv_perm_, dm_int,
-                               Dm_ + c0_ * n, n, k, kshift);
-        if (X_ && (!s_.reparam || x_solver_basis_)) {
-            const uint64_t tot = N * k;
-            hipLaunchKernelGGL(k_store_x, dim3((tot + BLK - 1) / BLK), dim3(BLK), 0, st, N, ws.x.as<double>(),
-                               X_ + c0_ * N, N, k);
+        DevBuf xtab;
+        if (pk_) {
+            bool any_x = false;
+            for (size_t m = 0; m < pk_->members.size(); ++m) {
+                const PackMember &mb = pk_->members[m];
+                const DevDesign &md = mb.s->dev;
+                if (mDm_[m])
+                    hipLaunchKernelGGL(k_store_caller, dim3(mb.n_tiles), dim3(BLK), tile_bytes_, st, mb.n,
+                                       md.inv_perm.p ? md.inv_perm.as<uint32_t>() : nullptr, dm_int + mb.row_off * k,
+                                       mDm_[m], mb.n, k, kshift, mb.kv);
+                any_x = any_x || mX_[m];
+            }
+            if (any_x) {
+                xtab.upload(mX_.data(), mX_.size(), st);
+                const uint64_t tot = N * k;
+                hipLaunchKernelGGL(k_store_x_packed, dim3((tot + BLK - 1) / BLK), dim3(BLK), 0, st, N,
+                                   ws.x.as<double>(), k, pk_->dof_member.as<uint32_t>(), pk_->dof_own.as<int32_t>(),
+                                   xtab.as<double *>(), pk_->own_ld.as<uint64_t>(), pk_->own_kv.as<uint32_t>());
+            }
+        } else {
+            if (Dm_)
+                hipLaunchKernelGGL(k_store_caller, dim3(gt_), dim3(BLK), tile_bytes_, st, n, inv_perm_, dm_int,
+                                   Dm_ + c0_ * n, n, k, kshift, k);
+            if (X_ && (!s_.reparam || x_solver_basis_)) {
+                const uint64_t tot = N * k;
+                hipLaunchKernelGGL(k_store_x, dim3((tot + BLK - 1) / BLK), dim3(BLK), 0, st, N, ws.x.as<double>(),
+                                   X_ + c0_ * N, N, k);
+            }
         }
         // residual numerator ||D'W(y - Dx)||; the denominator ||D'W y|| = beta_1 ||p~_1|| was
         // kept at initialisation
         launch_adjoint(dv_, da, weighted_ ? 2 : 0, dm_int, ws.h.as<double>(), ws.segpart.as<double>(), nullptr, k,
                        kshift, st);
-        double num[MAX_BATCH];
         hipLaunchKernelGGL(k_sumsq, dim3(gN_), dim3(BLK), 0, st, N, ws.h.as<double>(), part_, k, kshift);
-        hipLaunchKernelGGL(k_reduce_to, dim3(k), dim3(BLK), 0, st, part_, gN_, kp, ws.red.as<double>());
+        hipLaunchKernelGGL(k_reduce_to, dim3(ncol_), dim3(BLK), 0, st, part_, gN_, kp, ws.red.as<double>(), k, rg_dof_);
         WX_HIP(hipGetLastError());
-        double host_d[N_DOUBLE_FIELDS * MAX_BATCH];
-        int32_t host_i[N_INT_FIELDS * MAX_BATCH];
-        WX_HIP(hipMemcpyAsync(num, ws.red.as<double>(), MAX_BATCH * 8, hipMemcpyDeviceToHost, st));
-        WX_HIP(hipMemcpyAsync(host_d, ws.cols.p, sizeof(host_d), hipMemcpyDeviceToHost, st));
-        WX_HIP(hipMemcpyAsync(host_i, ws.ints.p, sizeof(host_i), hipMemcpyDeviceToHost, st));
+        const size_t cap = ws.cap;
+        std::vector<double> num(ncol_), host_d(N_DOUBLE_FIELDS * cap);
+        std::vector<int32_t> host_i(N_INT_FIELDS * cap);
+        WX_HIP(hipMemcpyAsync(num.data(), ws.red.as<double>(), ncol_ * 8, hipMemcpyDeviceToHost, st));
+        WX_HIP(hipMemcpyAsync(host_d.data(), ws.cols.p, host_d.size() * 8, hipMemcpyDeviceToHost, st));
+        WX_HIP(hipMemcpyAsync(host_i.data(), ws.ints.p, host_i.size() * 4, hipMemcpyDeviceToHost, st));
         WX_HIP(hipStreamSynchronize(st));
-        if (X_ && s_.reparam && !x_solver_basis_) {
+        if (!pk_ && X_ && s_.reparam && !x_solver_basis_) {
             std::vector<double> xi(N * k), xo(N);
             WX_HIP(hipMemcpy(xi.data(), ws.x.p, N * k * 8, hipMemcpyDeviceToHost));
             for (uint32_t j = 0; j < k; ++j) {
@@ -1499,21 +1615,23 @@ class ChunkRun {
             }
         }
         const double secs = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0_).count();
-        const double *phi_bar = host_d + 7 * MAX_BATCH, *zeta_bar = host_d + 10 * MAX_BATCH,
-                     *zeta0 = host_d + 11 * MAX_BATCH, *den_norm = host_d + 21 * MAX_BATCH;
-        const int32_t *active = host_i, *stop = host_i + 3 * MAX_BATCH, *iters = host_i + 4 * MAX_BATCH,
-                      *err = host_i + 5 * MAX_BATCH;
-        for (uint32_t j = 0; j < k; ++j) {
-            if (err[j]) fail(WX_ERR_NOT_POSITIVE_DEF, "mlsmr: preconditioner not positive definite (<v, Mv> < 0)");
-            wx_rhs_stats &o = stats_[c0_ + j];
-            int32_t sr = stop[j];
-            if (active[j] || sr < 0) {
+        const double *phi_bar = host_d.data() + 7 * cap, *zeta_bar = host_d.data() + 10 * cap,
+                     *zeta0 = host_d.data() + 11 * cap, *den_norm = host_d.data() + 21 * cap;
+        const int32_t *active = host_i.data(), *stop = host_i.data() + 3 * cap, *iters = host_i.data() + 4 * cap,
+                      *err = host_i.data() + 5 * cap;
+        for (uint32_t c = 0; c < ncol_; ++c) {
+            if (err[c]) fail(WX_ERR_NOT_POSITIVE_DEF, "mlsmr: preconditioner not positive definite (<v, Mv> < 0)");
+            const uint32_t j = c % k;
+            if (pk_ && j >= pk_->members[c / k].kv) continue; // a narrower member's padding column
+            wx_rhs_stats &o = pk_ ? mStats_[c / k][j] : stats_[c0_ + j];
+            int32_t sr = stop[c];
+            if (active[c] || sr < 0) {
                 sr = WX_STOP_MAX_ITERATIONS;
                 o.converged = 0;
                 o.iterations = maxiter_;
             } else {
                 o.converged = 1;
-                o.iterations = iters[j];
+                o.iterations = iters[c];
             }
             o.stop_reason = sr;
             if (sr == WX_STOP_ZERO_RHS) {
@@ -1521,17 +1639,66 @@ class ChunkRun {
                 o.residual_norm = 0.0;
             } else if (sr == WX_STOP_INITIAL_NORMAL_EQ_ZERO) {
                 o.residual = 0.0;
-                o.residual_norm = phi_bar[j];
+                o.residual_norm = phi_bar[c];
             } else {
-                o.residual = std::fabs(zeta_bar[j]) / zeta0[j];
-                o.residual_norm = std::fabs(phi_bar[j]);
+                o.residual = std::fabs(zeta_bar[c]) / zeta0[c];
+                o.residual_norm = std::fabs(phi_bar[c]);
             }
-            o.true_residual = den_norm[j] > 0.0 ? std::sqrt(num[j]) / den_norm[j] : 0.0;
+            o.true_residual = den_norm[c] > 0.0 ? std::sqrt(num[c]) / den_norm[c] : 0.0;
             o.time_solve = secs / k;
         }
     }
 
   private:
+    ChunkRun(const Solver &s, const Pack *pk, uint32_t k, const wx_lsmr_options &opts, hipStream_t st,
+             PhaseTimer *timer, int group, bool use_graph)
+        : s_(s), pk_(pk), dv_(s.dev), da_(s.dev.args()), k_(k), opts_(opts), st_(st), timer_(timer),
+          pw_(s, k, pk ? (uint32_t)(pk->members.size() * k) : MAX_BATCH, pk ? pk->part_blocks : 0) {
+        n_ = dv_.n;
+        N_ = dv_.N;
+        weighted_ = dv_.sw.p != nullptr;
+        inv_perm_ = dv_.inv_perm.p ? dv_.inv_perm.as<uint32_t>() : nullptr;
+        maxiter_ = (int)std::max<int64_t>(0, opts.maxiter);
+        const Precond *pc = s.precond.get();
+        prec_kind_ = pc == nullptr ? 0 : (pc->kind == Precond::DIAGONAL ? 1 : 2);
+        inv_diag_ = prec_kind_ == 1 ? dv_.inv_diag.as<double>() : nullptr;
+        const int prec_slot = opts.precision == WX_PRECISION_MIXED ? 1 : 0;
+        kshift_ = pow2_shift(k);
+        kp_ = 1u << kshift_;
+        t0_ = std::chrono::steady_clock::now();
+        ws_ = pw_.ws.get();
+        if (prec_kind_ == 2 && !ws_->runs[prec_slot]) ws_->runs[prec_slot] = make_schwarz_run(*pc, opts.precision, st);
+        schwarz_ = prec_kind_ == 2 ? ws_->runs[prec_slot].get() : nullptr;
+        cs_ = ws_->cs;
+        gn_ = grid_for(n_, kshift_);
+        part_ = ws_->part.as<double>();
+        gt_ = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>((n_ + 63) / 64, MAX_PART_BLOCKS));
+        tile_bytes_ = (size_t)64 * (k + 1) * sizeof(double);
+        group_ = std::max(1, group);
+        if (pk) {
+            // exact grids over the padded DOF blocks; reductions over each member's ranges
+            ncol_ = (uint32_t)(pk->members.size() * k);
+            gN_ = pk->n_dof_blocks;
+            dof_col_ = pk->dof_col.as<uint32_t>();
+            rg_dof_ = Ranges{pk->dof_rng_ptr.as<uint32_t>(), pk->dof_rng.as<uint32_t>()};
+            rg_item_ = Ranges{pk->item_rng_ptr.as<uint32_t>(), pk->item_rng.as<uint32_t>()};
+            rg_tile_ = Ranges{pk->tile_rng_ptr.as<uint32_t>(), pk->tile_rng.as<uint32_t>()};
+            cap_ = 0;
+        } else {
+            ncol_ = k;
+            gN_ = grid_for(N_, kshift_);
+            cap_ = opts.local_size > 0 ? std::min<uint64_t>((uint64_t)opts.local_size, std::min(n_, N_)) : 0;
+        }
+        // a graph replays fixed launch arguments: not with the reorthogonalisation ring (its
+        // slot changes every iteration) nor with the event timer
+        use_graph_ = use_graph && cap_ == 0 && !(timer_ && timer_->on);
+        graph_key_ = (uint64_t)group_ * 4 + (uint64_t)prec_slot;
+    }
+
+    // convergence gate of the kernels: the run's active flags (a pack's members converge
+    // separately, so its kernels run ungated and the per-column state decides)
+    const int32_t *gate() const { return pk_ ? nullptr : cs_.active; }
+
     // one LSMR iteration (u update, beta, D', p~, M^{-1}, reorthogonalisation, alpha, x update)
     void iteration(bool first) {
         Workspace &ws = *ws_;
@@ -1544,46 +1711,46 @@ class ChunkRun {
         uint32_t gpart = gn_;
         if (dv_.fuse_term >= 0)
             gpart = launch_gather_seg(dv_, da_, first, ws.v.as<double>(), uin, ws.u.as<double>(), ws.atu.as<double>(),
-                                      ws.segpart.as<double>(), cs_, part_, k, kshift, st);
+                                      ws.segpart.as<double>(), cs_, part_, k, kshift, st, pk_);
         else
             launch_gather(dv_, da_, first, gn_, ws.v.as<double>(), uin, ws.u.as<double>(), cs_, part_, k, kshift, st);
         if (timed) timer_->end(0);
-        hipLaunchKernelGGL(k_beta, dim3(k), dim3(BLK), 0, st, part_, gpart, kp_, cs_);
+        hipLaunchKernelGGL(k_beta, dim3(ncol_), dim3(BLK), 0, st, part_, gpart, kp_, cs_, k, rg_item_);
         if (timed) timer_->begin(1);
         // split levels of the fused term first: the other terms reuse the partial buffer
         if (dv_.fuse_term >= 0)
-            launch_segfix(dv_.terms[dv_.fuse_term], ws.segpart.as<double>(), ws.atu.as<double>(), cs_.active, k, kshift,
+            launch_segfix(dv_.terms[dv_.fuse_term], ws.segpart.as<double>(), ws.atu.as<double>(), gate(), k, kshift,
                           st);
         launch_adjoint(dv_, da_, weighted_ ? 1 : 0, ws.u.as<double>(), ws.atu.as<double>(), ws.segpart.as<double>(),
-                       cs_.active, k, kshift, st, dv_.fuse_term);
+                       gate(), k, kshift, st, dv_.fuse_term);
         if (timed) timer_->end(1);
         if (prec_kind_ == 2) {
             hipLaunchKernelGGL((k_ptilde<2, false>), dim3(gN_), dim3(BLK), 0, st, N, ws.atu.as<double>(),
-                               ws.pt.as<double>(), ws.v.as<double>(), nullptr, cs_, part_, k, kshift);
+                               ws.pt.as<double>(), ws.v.as<double>(), nullptr, cs_, part_, k, kshift, dof_col_);
             if (timed) timer_->begin(2);
-            schwarz_->apply(ws.pt.as<double>(), ws.v.as<double>(), k, st, cs_.active);
+            schwarz_->apply(ws.pt.as<double>(), ws.v.as<double>(), k, st, gate());
             if (timed) timer_->end(2);
             if (cap_) ring_mgs();
             hipLaunchKernelGGL(k_dots, dim3(gN_), dim3(BLK), 0, st, N, ws.pt.as<double>(), ws.v.as<double>(), cs_,
-                               part_, k, kshift);
+                               part_, k, kshift, dof_col_);
         } else {
             if (prec_kind_ == 1)
                 hipLaunchKernelGGL((k_ptilde<1, false>), dim3(gN_), dim3(BLK), 0, st, N, ws.atu.as<double>(),
-                                   ws.pt.as<double>(), ws.v.as<double>(), inv_diag_, cs_, part_, k, kshift);
+                                   ws.pt.as<double>(), ws.v.as<double>(), inv_diag_, cs_, part_, k, kshift, dof_col_);
             else
                 hipLaunchKernelGGL((k_ptilde<0, false>), dim3(gN_), dim3(BLK), 0, st, N, ws.atu.as<double>(),
-                                   ws.pt.as<double>(), ws.v.as<double>(), nullptr, cs_, part_, k, kshift);
+                                   ws.pt.as<double>(), ws.v.as<double>(), nullptr, cs_, part_, k, kshift, dof_col_);
             if (cap_) { // reorthogonalise, then the alpha dot products of the updated pair
                 ring_mgs();
                 hipLaunchKernelGGL(k_dots, dim3(gN_), dim3(BLK), 0, st, N, ws.pt.as<double>(), ws.v.as<double>(), cs_,
-                                   part_, k, kshift);
+                                   part_, k, kshift, dof_col_);
             }
         }
-        hipLaunchKernelGGL(k_alpha_step, dim3(k), dim3(BLK), 0, st, part_, gN_, kp_, cs_, opts_.tol);
+        hipLaunchKernelGGL(k_alpha_step, dim3(ncol_), dim3(BLK), 0, st, part_, gN_, kp_, cs_, opts_.tol, k, rg_dof_);
         const uint64_t push = cap_ ? ring_advance() : 0;
         hipLaunchKernelGGL(k_update_solution, dim3(gN_), dim3(BLK), 0, st, N, ws.v.as<double>(), ws.h.as<double>(),
                            ws.hb.as<double>(), ws.x.as<double>(), cs_, k, kshift, ws.pt.as<double>(),
-                           cap_ ? ring_v(push) : nullptr, cap_ ? ring_p(push) : nullptr);
+                           cap_ ? ring_v(push) : nullptr, cap_ ? ring_p(push) : nullptr, dof_col_);
         if (timed) timer_->end(3);
     }
 
@@ -1618,23 +1785,29 @@ class ChunkRun {
             const uint64_t slot = (start + q) % cap_;
             hipLaunchKernelGGL(k_ring_dot, dim3(gN_), dim3(BLK), 0, st_, N_, ws_->v.as<double>(), ring_p(slot), cs_,
                                part_, k_, kshift_);
-            hipLaunchKernelGGL(k_reduce_to, dim3(k_), dim3(BLK), 0, st_, part_, gN_, kp_, ringc_.as<double>());
+            hipLaunchKernelGGL(k_reduce_to, dim3(k_), dim3(BLK), 0, st_, part_, gN_, kp_, ringc_.as<double>(), 1u,
+                               Ranges{});
             hipLaunchKernelGGL(k_ring_axpy, dim3(gN_), dim3(BLK), 0, st_, N_, ws_->v.as<double>(), ws_->pt.as<double>(),
                                ring_v(slot), ring_p(slot), ringc_.as<double>(), cs_, k_, kshift_);
         }
     }
 
     const Solver &s_;
+    const Pack *pk_ = nullptr;
     const DevDesign &dv_;
     const DesignArgs da_;
-    const double *Y_;
-    uint64_t ldy_, c0_;
+    const double *Y_ = nullptr;
+    uint64_t ldy_ = 0, c0_ = 0;
     uint32_t k_, kshift_ = 0, kp_ = 1;
     wx_lsmr_options opts_;
-    double *X_, *Dm_;
-    wx_rhs_stats *stats_;
+    double *X_ = nullptr, *Dm_ = nullptr;
+    wx_rhs_stats *stats_ = nullptr;
+    std::vector<const double *> mY_;
+    std::vector<uint64_t> mld_;
+    std::vector<double *> mX_, mDm_;
+    std::vector<wx_rhs_stats *> mStats_;
     hipStream_t st_;
-    bool x_solver_basis_;
+    bool x_solver_basis_ = false;
     PhaseTimer *timer_;
     PooledWorkspace pw_;
     Workspace *ws_ = nullptr;
@@ -1645,7 +1818,9 @@ class ChunkRun {
     const uint32_t *inv_perm_ = nullptr;
     int maxiter_ = 0, prec_kind_ = 0;
     const double *inv_diag_ = nullptr;
-    uint32_t gn_ = 1, gN_ = 1, gt_ = 1;
+    uint32_t gn_ = 1, gN_ = 1, gt_ = 1, ncol_ = 1;
+    const uint32_t *dof_col_ = nullptr;
+    Ranges rg_dof_{}, rg_item_{}, rg_tile_{};
     size_t tile_bytes_ = 0;
     double *part_ = nullptr;
     std::chrono::steady_clock::time_point t0_;
@@ -1702,6 +1877,7 @@ namespace {
 struct ManyShape {
     int streams = 1, group = 1;
     bool graph = false;
+    bool pack = false; // problems of one shape packed into one block-diagonal run
 };
 std::mutex g_tune_mu;
 std::vector<std::pair<uint64_t, ManyShape>> g_tuned; // problem-count class -> shape
@@ -1732,9 +1908,47 @@ void run_many(const std::vector<const Solver *> &solvers, const std::vector<cons
     WX_HIP(hipEventCreateWithFlags(&ready, hipEventDisableTiming));
     WX_HIP(hipEventRecord(ready, caller));
     std::vector<std::unique_ptr<ChunkRun>> runs;
+    std::vector<std::shared_ptr<Pack>> packs; // kept alive for the runs
+    std::vector<char> packed(solvers.size(), 0);
     int next_stream = 0;
+    if (shape.pack && opts.local_size <= 0) {
+        // problems of one signature and one power-of-two width class share a pack run at the
+        // class's widest member; narrower members carry zero padding columns
+        std::map<std::string, std::vector<size_t>> groups;
+        for (size_t p = 0; p < solvers.size(); ++p) {
+            std::string sig;
+            if (k[p] < 1 || k[p] > MAX_BATCH || !pack_signature(*solvers[p], sig)) continue;
+            groups[sig + "/" + std::to_string(pow2_shift((uint32_t)k[p]))].push_back(p);
+        }
+        for (auto &g : groups) {
+            const std::vector<size_t> &idx = g.second;
+            if (idx.size() < 2) continue;
+            std::vector<const Solver *> mem;
+            std::vector<uint32_t> kv;
+            std::vector<const double *> mY;
+            std::vector<uint64_t> mld;
+            std::vector<double *> mX, mDm;
+            std::vector<wx_rhs_stats *> mSt;
+            for (size_t p : idx) {
+                mem.push_back(solvers[p]);
+                kv.push_back((uint32_t)k[p]);
+                mY.push_back(Y[p]);
+                mld.push_back(ldy[p]);
+                mX.push_back(X[p]);
+                mDm.push_back(Dm[p]);
+                mSt.push_back(stats[p]);
+                packed[p] = 1;
+            }
+            packs.push_back(get_pack(mem, kv));
+            hipStream_t st = pool_stream(next_stream);
+            next_stream = (next_stream + 1) % shape.streams;
+            WX_HIP(hipStreamWaitEvent(st, ready, 0));
+            runs.push_back(std::make_unique<ChunkRun>(*packs.back(), mY, mld, opts, mX, mDm, mSt, st, shape.group,
+                                                      shape.graph));
+        }
+    }
     for (size_t p = 0; p < solvers.size(); ++p)
-        for (uint64_t c0 = 0; c0 < k[p]; c0 += MAX_BATCH) {
+        for (uint64_t c0 = 0; c0 < k[p] && !packed[p]; c0 += MAX_BATCH) {
             const uint32_t kk = (uint32_t)std::min<uint64_t>(MAX_BATCH, k[p] - c0);
             hipStream_t st = pool_stream(next_stream);
             next_stream = (next_stream + 1) % shape.streams;
@@ -1798,13 +2012,17 @@ void solve_many_device(const std::vector<const Solver *> &solvers, const std::ve
     if (!tuned) {
         // autotuner: time each candidate shape on the real problems with the iteration count
         // capped (the final run below overwrites every output)
-        static const ManyShape cand[] = {{1, 1, false}, {4, 1, true}, {8, 1, true}, {8, 2, true}, {16, 2, true},
-                                         {16, 4, true}};
+        static const ManyShape cand[] = {{1, 1, false, false}, {4, 1, true, false}, {8, 2, true, false},
+                                         {16, 4, true, false}, {1, 1, false, true},  {4, 1, false, true},
+                                         {4, 1, true, true},   {4, 2, true, true},   {8, 2, true, true}};
         wx_lsmr_options probe = opts;
         probe.maxiter = std::min<int64_t>(opts.maxiter, 24);
         probe.timing = 0;
         double best = 1e300;
         for (const ManyShape &c : cand) {
+            // the first run of a shape builds what later calls reuse (packs, captured graphs,
+            // workspaces); the second is timed
+            run_many(solvers, Y, ldy, k, probe, X, Dm, stats, c, caller);
             WX_HIP(hipStreamSynchronize(caller));
             const auto t0 = std::chrono::steady_clock::now();
             run_many(solvers, Y, ldy, k, probe, X, Dm, stats, c, caller);
@@ -1824,17 +2042,20 @@ void solve_many_device(const std::vector<const Solver *> &solvers, const std::ve
     run_many(solvers, Y, ldy, k, opts, X, Dm, stats, shape, caller);
 }
 
-void force_many_shape(int32_t streams, int32_t group, int32_t graph) {
+void force_many_shape(int32_t streams, int32_t group, int32_t graph, int32_t pack) {
     std::lock_guard<std::mutex> lk(g_tune_mu);
     g_forced.streams = std::max(0, std::min(streams, 64));
     g_forced.group = std::max(1, std::min(group, 64));
     g_forced.graph = graph != 0;
+    g_forced.pack = pack != 0;
 }
 
-void last_many_shape(int32_t *streams, int32_t *group, int32_t *graph) {
+void last_many_shape(int32_t *streams, int32_t *group, int32_t *graph, int32_t *pack) {
+    std::lock_guard<std::mutex> lk(g_tune_mu);
     if (streams) *streams = g_last_shape.streams;
     if (group) *group = g_last_shape.group;
     if (graph) *graph = g_last_shape.graph ? 1 : 0;
+    if (pack) *pack = g_last_shape.pack ? 1 : 0;
 }
 
 std::vector<double> &last_timing() {

@@ -14,6 +14,10 @@ namespace wx {
 constexpr uint32_t MAX_TERMS = 8;
 constexpr uint32_t MAX_COLS = 4;   // coefficient columns per term on the device path
 constexpr uint32_t MAX_BATCH = 64; // RHS columns interleaved per device pass
+constexpr uint32_t KERNEL_BLOCK = 256;        // threads per workgroup of the solver kernels
+constexpr uint32_t MAX_PART_BLOCKS = 4096;    // grid cap of the grid-stride kernels
+constexpr uint32_t PACK_ALIGN = 256;          // DOF / work-item alignment of a packed problem
+constexpr int32_t SEG_SKIP = INT32_MIN;       // SegItem::dest of a padding item (no rows, no output)
 
 // Per-term description passed by value to kernels.
 struct TermInfo {
@@ -91,10 +95,12 @@ struct Precond {
 };
 
 struct Solver {
+    uint64_t uid = 0; // process-unique id (cache key of packed groups)
     Design design;
     std::unique_ptr<SlopeReparam> reparam;
     bool weighted = false;
     double min_weight = 1.0; // smallest observation weight (1 when unweighted)
+    std::vector<double> w_internal; // observation weights in internal order (empty: unweighted)
     std::shared_ptr<Precond> precond;
     std::vector<std::string> warnings;
     double time_setup = 0.0;
@@ -107,6 +113,44 @@ struct Solver {
 
 // Uploads the design (codes, slopes, weights, D' segment plans) to the current device.
 void upload_design(Solver &s, const std::vector<double> &weights_internal);
+
+uint64_t next_solver_uid();
+
+// Independent problems of one shape packed into one block-diagonal design, so that every
+// iteration kernel runs once for all of them.  Member m's rows are the merged rows
+// [row_off, row_off + n) in the member's own internal order; its term-t levels are the
+// merged levels [lvl_off[t], lvl_off[t] + L[t]) of term t, padded to PACK_ALIGN so that no
+// workgroup of a DOF-space kernel straddles two members; the fused term's work items are
+// padded the same way.  LSMR state of member m, column j lives at index m * k + j.
+struct PackMember {
+    const Solver *s = nullptr;
+    uint64_t row_off = 0, n = 0, n_dofs = 0;
+    uint32_t tile_off = 0, n_tiles = 0; // right-hand-side load partial blocks
+    uint32_t kv = 1;                    // the member's own columns (<= the pack's k)
+    uint64_t lvl_off[MAX_TERMS] = {}, L[MAX_TERMS] = {}, own_off[MAX_TERMS] = {};
+};
+struct Pack {
+    std::vector<PackMember> members;
+    uint32_t k = 1, kshift = 0;
+    std::unique_ptr<Solver> merged;
+    uint32_t n_dof_blocks = 0, n_item_blocks = 0, n_tiles = 0;
+    DevBuf dof_col, item_col;         // u32 per DOF block / fused-item block: member * k
+    DevBuf dof_rng_ptr, dof_rng;      // per member: CSR of [lo, hi) DOF-block ranges
+    DevBuf item_rng_ptr, item_rng;    // per member: its fused-item blocks
+    DevBuf tile_rng_ptr, tile_rng;    // per member: its load partial blocks
+    DevBuf dof_member, dof_own;       // per merged DOF: member, own DOF index (-1: padding)
+    DevBuf own_ld;                    // per member: own n_dofs (u64)
+    DevBuf own_kv;                    // per member: own columns (u32)
+    uint64_t part_blocks = 0;         // partial blocks the largest packed launch writes
+};
+
+// Whether the solvers can share one pack (same term count and sorted term, plain design,
+// same weighting and preconditioner kind) and their key.
+bool pack_signature(const Solver &s, std::string &sig);
+
+// Builds (or returns from a small cache) the pack of `members` with kv[m] columns each, run at
+// batch width k = max kv.
+std::shared_ptr<Pack> get_pack(const std::vector<const Solver *> &members, const std::vector<uint32_t> &kv);
 
 // Runs the batched solve with device-resident inputs/outputs on `stream`.  x_solver_basis:
 // X is left in the solver's (reparametrised) coefficient basis instead of the caller's.
@@ -121,10 +165,10 @@ void solve_many_device(const std::vector<const Solver *> &solvers, const std::ve
                        const std::vector<wx_rhs_stats *> &stats, hipStream_t caller);
 
 // Fixes the shape of later solve_many_device calls (streams 0: back to the autotuner).
-void force_many_shape(int32_t streams, int32_t group, int32_t graph);
+void force_many_shape(int32_t streams, int32_t group, int32_t graph, int32_t pack);
 
-// The (streams, iterations per graph, graph on) shape the last solve_many_device used.
-void last_many_shape(int32_t *streams, int32_t *group, int32_t *graph);
+// The (streams, iterations per graph, graph on, packed) shape the last solve_many_device used.
+void last_many_shape(int32_t *streams, int32_t *group, int32_t *graph, int32_t *pack);
 
 // Adjoint of the weighted projection R = Z - D x(Z, w) for reverse-mode differentiation:
 // grad_z = G - w .* (D lambda), grad_w = -sum_j R_j .* (D lambda)_j with D'WD lambda = D'G.
