@@ -206,8 +206,10 @@ def bench_many(args):
         torch.cuda.synchronize()
         return (time.perf_counter() - t) / args.steps
 
-    t_seq = timed(sequential)
-    its_seq = [[stats[p][j].iterations for j in range(ks[p])] for p in range(P)]
+    if args.shape:
+        wa.set_many_shape(*[int(v) for v in args.shape.split(",")])
+    t_seq = timed(sequential) if not args.no_seq else float("nan")
+    its_seq = [[stats[p][j].iterations for j in range(ks[p])] for p in range(P)] if not args.no_seq else None
     sweep = {}
     if args.sweep:
         for shp in [(1, 1, False, False), (8, 2, True, False), (1, 1, False, True), (2, 1, True, True),
@@ -240,15 +242,15 @@ def bench_many(args):
         "config": {"workload": f"{P} independent problems, n log-uniform in [1e4, 1e6], 2-3 factors "
                                "(n/20, n/200, 30 uniform levels), 1-8 RHS each, tol 1e-8",
                    "problems": P, "total_rhs": sum(ks), "total_obs": sum(ns), "precision": "mixed"},
-        "sequential_solve_batch_value": sum(ks) / t_seq,
-        "many_over_sequential": t_seq / t_many,
+        "sequential_solve_batch_value": None if args.no_seq else sum(ks) / t_seq,
+        "many_over_sequential": None if args.no_seq else t_seq / t_many,
         "launch_shape": shape,
         "shape_sweep_ms": sweep or None,
         "latency_ms": {"p50": float(np.percentile(lat, 50)), "p95": float(np.percentile(lat, 95)),
                        "max": float(lat.max()), "min": float(lat.min())},
         "converged": conv,
         "true_residual_max": tr,
-        "iterations_equal_sequential": its == its_seq,
+        "iterations_equal_sequential": (its == its_seq) if its_seq is not None else None,
         "max_iterations": max(max(i) for i in its),
         "setup_s": setup,
     }
@@ -270,6 +272,8 @@ def main():
                     help="solve_batch: the weighted 5M-row batch (default); many: 64 uneven independent problems")
     ap.add_argument("--problems", type=int, default=64)
     ap.add_argument("--sweep", action="store_true", help="many: also time every launch shape")
+    ap.add_argument("--shape", default="", help="many: fixed launch shape 'streams,iterations_per_graph,graph,pack'")
+    ap.add_argument("--no-seq", action="store_true", help="many: skip the one-after-another comparison")
     ap.add_argument("--copy-probe", action="store_true",
                     help="also run a plain 1 GiB device-to-device copy (calibrates HBM byte counters)")
     args = ap.parse_args()
