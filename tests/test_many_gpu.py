@@ -40,8 +40,27 @@ def problems():
                                                        rng.integers(0, 7, n)]).astype(np.uint32))
             w = np.exp(rng.normal(0, 0.4, n)) if pc is None else None
             s = wa.Solver(codes, weights=w) if pc is None else wa.Solver(codes, preconditioner=pc)
-            out.append((s, np.asfortranarray(rng.standard_normal((n, k)))))
+            Y = np.asfortranarray(rng.standard_normal((n, k)))
+            if pc is wa.PreconditionerConfig.Off:
+                EXACT[len(out)] = exact_demeaned(codes, Y)
+            out.append((s, Y))
     return out
+
+
+EXACT = {}  # problem index -> exact projection (the unpreconditioned problems)
+
+
+def exact_demeaned(codes, Y):
+    """Y - D x with x = argmin ||Y - D x|| by dense least squares (small designs)."""
+    cols = []
+    for t in range(codes.shape[1]):
+        L = int(codes[:, t].max()) + 1
+        Dt = np.zeros((codes.shape[0], L))
+        Dt[np.arange(codes.shape[0]), codes[:, t]] = 1.0
+        cols.append(Dt)
+    D = np.hstack(cols)
+    x = np.linalg.lstsq(D, Y, rcond=None)[0]
+    return Y - D @ x
 
 
 @pytest.fixture(scope="module")
@@ -68,8 +87,6 @@ def test_many_matches_individual_solves(gpu, problems, singles, shape):
     worst_d = worst_x = worst_off = worst_plain = 0.0
     diter = n_off = n_cols = 0
     for p, (r, ref) in enumerate(zip(res, singles)):
-        # plain LSMR (no preconditioner, ~40 steps without reorthogonalisation) amplifies the
-        # rounding of differently grouped partial sums; preconditioned runs stay at rounding
         plain = problems[p][0].preconditioner is None
         assert all(r.converged), p
         assert r.demeaned.shape == ref.demeaned.shape and r.x.shape == ref.x.shape
@@ -79,8 +96,13 @@ def test_many_matches_individual_solves(gpu, problems, singles, shape):
         for j in range(r.demeaned.shape[1]):
             n_cols += 1
             dd = rel(r.demeaned[:, j], ref.demeaned[:, j])
-            if r.iterations[j] == ref.iterations[j] and plain:
-                worst_plain = max(worst_plain, dd)
+            if not packed:
+                assert dd <= 1e-13, (p, j)
+            if plain:
+                # plain LSMR (~40 steps, no reorthogonalisation) amplifies rounding
+                # differences chaotically: both answers are held to the exact projection
+                ex = EXACT[p][:, j]
+                worst_plain = max(worst_plain, rel(r.demeaned[:, j], ex), rel(ref.demeaned[:, j], ex))
             elif r.iterations[j] == ref.iterations[j]:
                 worst_d = max(worst_d, dd)
                 worst_x = max(worst_x, rel(r.x[:, j], ref.x[:, j]))
@@ -99,10 +121,10 @@ def test_many_matches_individual_solves(gpu, problems, singles, shape):
           f"{worst_plain:.2e}); {n_off} of {n_cols} columns stop {diter} step(s) apart, demeaned {worst_off:.2e}")
     if packed:
         # the same iterates to rounding (partial sums grouped differently)
-        assert worst_d <= 1e-10 and worst_x <= 1e-9 and worst_plain <= 1e-7
+        assert worst_d <= 1e-10 and worst_x <= 1e-9 and worst_plain <= 1e-6
         assert diter <= 1 and n_off <= max(2, n_cols // 10) and worst_off <= 1e-6
     else:
-        assert n_off == 0 and max(worst_d, worst_plain) <= 1e-13 and worst_x <= 1e-12
+        assert n_off == 0 and worst_d <= 1e-13 and worst_x <= 1e-12 and worst_plain <= 1e-6
 
 
 def test_many_autotuned_mixed_precision(gpu, problems):
@@ -124,7 +146,7 @@ def test_many_autotuned_mixed_precision(gpu, problems):
             else:
                 assert abs(r.iterations[j] - ref.iterations[j]) <= 1
                 plain = s.preconditioner is None
-                bound = 1e-6 if r.iterations[j] != ref.iterations[j] else (1e-7 if plain else 1e-10)
+                bound = 1e-6 if (plain or r.iterations[j] != ref.iterations[j]) else 1e-10
             assert rel(r.demeaned[:, j], ref.demeaned[:, j]) <= bound
             assert rel(r2.demeaned[:, j], r.demeaned[:, j]) <= 1e-13 # repeatable
     print(f"autotuned shape {shape}")

@@ -29,6 +29,7 @@ constexpr int BLK = (int)KERNEL_BLOCK;
 constexpr uint32_t SEG_CHUNK = 64;  // rows per D' work item before a level is split
 constexpr uint32_t FIX_CHUNK = 64;  // partials per combine item
 constexpr uint32_t MAX_FUSED_BLOCKS = 32768; // grid cap of the fused gather (one item per row group)
+constexpr size_t STAGE_BYTES = 16384;        // LDS per workgroup for small terms' coefficient rows
 
 // Per-column LSMR state (structure of arrays, each of length MAX_BATCH).
 struct ColState {
@@ -60,22 +61,50 @@ __device__ inline double block_reduce_cols(double v, uint32_t kp, double *sh) {
 // terms summed in order from 0.0, each term folded from -0.0 over its coefficient columns.
 // TT > 0 fixes the number of terms at compile time; PLAIN says every term is one intercept
 // column, so the fold reduces to the value itself (-0.0 + x*1.0 == x bitwise) and all R*TT
-// code loads are issued before any x load.
+// code loads are issued before any x load.  Every row must be valid: callers clamp rows past
+// their range to a valid one and discard those results, so no load sits behind a branch.
+// Terms whose coefficient rows a workgroup keeps in LDS (bit t of mask: term t's rows
+// x[offset_t + l], all k columns, at LDS row off[t] + l).  Scattered row gathers are bound by
+// the vector L1's line throughput even when they hit; a small term read from LDS costs none.
+struct Staged {
+    uint32_t mask;
+    uint32_t off[MAX_TERMS];
+};
+
+// Copies the staged terms' rows of x into LDS (all threads of the workgroup take part).
+__device__ inline void stage_terms(const DesignArgs &d, const Staged &sg, const double *__restrict__ x, double *xl,
+                                   uint32_t k) {
+    for (uint32_t t = 0; t < d.T; ++t) {
+        if (!((sg.mask >> t) & 1u)) continue;
+        const double *src = x + d.terms[t].offset * k;
+        double *dst = xl + (uint64_t)sg.off[t] * k;
+        const uint32_t cnt = (uint32_t)d.terms[t].n_levels * k;
+        for (uint32_t q = threadIdx.x; q < cnt; q += BLK) dst[q] = src[q];
+    }
+    __syncthreads();
+}
+
 template <int R, int TT, bool PLAIN>
-__device__ inline void gather_rows(const DesignArgs &d, const double *__restrict__ x, const uint64_t (&rows)[R],
-                                   const bool (&ok)[R], uint32_t k, uint32_t j, double (&acc)[R]) {
+__device__ inline void gather_rows_staged(const DesignArgs &d, const double *__restrict__ x,
+                                          const uint64_t (&rows)[R], uint32_t k, uint32_t j, double (&acc)[R],
+                                          Staged sg, const double *xl) {
     if constexpr (TT > 0 && PLAIN) {
         uint32_t code[TT][R];
 #pragma unroll
         for (int t = 0; t < TT; ++t)
 #pragma unroll
-            for (int q = 0; q < R; ++q)
-                code[t][q] = ok[q] ? d.codes[(uint64_t)t * d.n + rows[q]] : 0u;
+            for (int q = 0; q < R; ++q) code[t][q] = d.codes[(uint64_t)t * d.n + rows[q]];
         double val[TT][R];
 #pragma unroll
-        for (int t = 0; t < TT; ++t)
+        for (int t = 0; t < TT; ++t) {
+            if ((sg.mask >> t) & 1u) {
 #pragma unroll
-            for (int q = 0; q < R; ++q) val[t][q] = x[(d.terms[t].offset + code[t][q]) * k + j];
+                for (int q = 0; q < R; ++q) val[t][q] = xl[(sg.off[t] + code[t][q]) * k + j];
+            } else {
+#pragma unroll
+                for (int q = 0; q < R; ++q) val[t][q] = x[(d.terms[t].offset + code[t][q]) * k + j];
+            }
+        }
 #pragma unroll
         for (int q = 0; q < R; ++q) {
             double a = 0.0;
@@ -90,17 +119,24 @@ __device__ inline void gather_rows(const DesignArgs &d, const double *__restrict
             const TermInfo &ti = d.terms[t];
             uint64_t code[R];
 #pragma unroll
-            for (int q = 0; q < R; ++q) code[q] = ok[q] ? d.codes[(uint64_t)t * d.n + rows[q]] : 0;
+            for (int q = 0; q < R; ++q) code[q] = d.codes[(uint64_t)t * d.n + rows[q]];
 #pragma unroll
             for (int q = 0; q < R; ++q) {
                 double tv = -0.0;
                 for (uint32_t c = 0; c < ti.ncols; ++c)
-                    tv = tv + x[(ti.offset + c * ti.n_levels + code[q]) * k + j] *
-                                  (ok[q] ? term_load(d, ti.load[c], rows[q]) : 0.0);
+                    tv = tv + x[(ti.offset + c * ti.n_levels + code[q]) * k + j] * term_load(d, ti.load[c], rows[q]);
                 acc[q] += tv;
             }
         }
     }
+}
+
+template <int R, int TT, bool PLAIN>
+__device__ inline void gather_rows(const DesignArgs &d, const double *__restrict__ x, const uint64_t (&rows)[R],
+                                   uint32_t k, uint32_t j, double (&acc)[R]) {
+    Staged none;
+    none.mask = 0u;
+    gather_rows_staged<R, TT, PLAIN>(d, x, rows, k, j, acc, none, nullptr);
 }
 
 // ---------------------------------------------------------------------------
@@ -130,12 +166,12 @@ __global__ __launch_bounds__(BLK) void k_gather_axpy(DesignArgs d, const double 
             double uo[R], wq[R], acc[R];
 #pragma unroll
             for (int q = 0; q < R; ++q) {
-                rows[q] = i0 + q * rstride;
-                ok[q] = rows[q] < d.n;
-                uo[q] = ok[q] ? uin[rows[q] * k + j] : 0.0;
-                wq[q] = WEIGHTED && ok[q] ? sw[rows[q]] : 1.0;
+                ok[q] = i0 + q * rstride < d.n;
+                rows[q] = ok[q] ? i0 + q * rstride : i0; // clamped: loads stay unconditional
+                uo[q] = uin[rows[q] * k + j];
+                wq[q] = WEIGHTED ? sw[rows[q]] : 1.0;
             }
-            gather_rows<R, TT, PLAIN>(d, v, rows, ok, k, j, acc);
+            gather_rows<R, TT, PLAIN>(d, v, rows, k, j, acc);
 #pragma unroll
             for (int q = 0; q < R; ++q) {
                 if (!ok[q]) continue;
@@ -158,13 +194,18 @@ __global__ __launch_bounds__(BLK) void k_gather_axpy(DesignArgs d, const double 
 // its rows four at a time, and sums sqrt(w_i) u_i in row order -- the order k_segreduce uses,
 // so the result is bitwise the same as the separate reduction.
 template <bool WEIGHTED, bool FIRST, int TT, bool PLAIN>
-__global__ __launch_bounds__(BLK) void k_gather_seg(DesignArgs d, TermInfo ts, const SegItem *__restrict__ items,
+__global__ __launch_bounds__(BLK) void k_gather_seg(DesignArgs d, TermInfo ts, uint32_t fterm,
+                                                    const uint32_t *__restrict__ fcodes,
+                                                    const SegItem *__restrict__ items,
                                                     uint32_t n_items, const double *__restrict__ sw,
                                                     const double *__restrict__ v, const double *uin, double *u,
                                                     double *__restrict__ atu, double *__restrict__ partials,
                                                     ColState cs, double *__restrict__ part, uint32_t k,
-                                                    uint32_t kshift, const uint32_t *__restrict__ blk_col) {
+                                                    uint32_t kshift, const uint32_t *__restrict__ blk_col,
+                                                    Staged sg) {
     __shared__ double sh[BLK];
+    extern __shared__ double vl[]; // the staged terms' rows of v
+    if (sg.mask) stage_terms(d, sg, v, vl, k);
     const uint32_t kp = 1u << kshift;
     const uint32_t j = threadIdx.x & (kp - 1);
     const uint32_t c = (blk_col ? blk_col[blockIdx.x] : 0u) + j; // column-state index
@@ -176,19 +217,24 @@ __global__ __launch_bounds__(BLK) void k_gather_seg(DesignArgs d, TermInfo ts, c
         const uint32_t groups = (gridDim.x * BLK) >> kshift;
         for (uint32_t it = (blockIdx.x * BLK + threadIdx.x) >> kshift; it < n_items; it += groups) {
             const SegItem item = items[it];
+            // an item of several complete levels writes each level's sum when the level
+            // changes, and zero for the levels in its range that have no rows
+            uint32_t cur = item.level;
             double adj = 0.0;
             for (uint32_t p0 = item.begin; p0 < item.end; p0 += R) {
                 uint64_t rows[R];
                 bool ok[R];
                 double uo[R], wq[R], acc[R];
+                uint32_t lv[R];
 #pragma unroll
                 for (int q = 0; q < R; ++q) {
-                    rows[q] = p0 + q;
                     ok[q] = p0 + q < item.end;
-                    uo[q] = ok[q] ? uin[rows[q] * k + j] : 0.0;
-                    wq[q] = WEIGHTED && ok[q] ? sw[rows[q]] : 1.0;
+                    rows[q] = ok[q] ? p0 + q : p0; // clamped: loads stay unconditional
+                    uo[q] = __builtin_nontemporal_load(&uin[rows[q] * k + j]);
+                    wq[q] = WEIGHTED ? sw[rows[q]] : 1.0;
+                    lv[q] = fcodes[rows[q]]; // equals cur throughout a single-level item
                 }
-                gather_rows<R, TT, PLAIN>(d, v, rows, ok, k, j, acc);
+                gather_rows_staged<R, TT, PLAIN>(d, v, rows, k, j, acc, sg, vl);
 #pragma unroll
                 for (int q = 0; q < R; ++q) {
                     if (!ok[q]) continue;
@@ -196,17 +242,135 @@ __global__ __launch_bounds__(BLK) void k_gather_seg(DesignArgs d, TermInfo ts, c
                     if (WEIGHTED) a *= wq[q];
                     const double b = (FIRST && WEIGHTED) ? wq[q] * uo[q] : uo[q];
                     const double nu = a + scale * b;
-                    u[rows[q] * k + j] = nu;
+                    // u streams through once per iteration: non-temporal, so the gathered
+                    // coefficient rows keep their place in L2
+                    __builtin_nontemporal_store(nu, &u[rows[q] * k + j]);
                     sq += nu * nu;
+                    if (lv[q] != cur) {
+                        atu[(ts.offset + cur) * k + j] = adj;
+                        for (uint32_t l = cur + 1; l < lv[q]; ++l) atu[(ts.offset + l) * k + j] = 0.0;
+                        adj = 0.0;
+                        cur = lv[q];
+                    }
                     adj += WEIGHTED ? wq[q] * nu : nu;
                 }
             }
-            if (item.dest >= 0) atu[(ts.offset + item.level) * k + j] = adj;
-            else if (item.dest != SEG_SKIP) partials[(uint64_t)(-item.dest - 1) * k + j] = adj;
+            if (item.dest >= 0) {
+                atu[(ts.offset + cur) * k + j] = adj;
+                for (uint32_t l = cur + 1; l <= (uint32_t)item.dest; ++l) atu[(ts.offset + l) * k + j] = 0.0;
+            } else if (item.dest != SEG_SKIP) {
+                partials[(uint64_t)(-item.dest - 1) * k + j] = adj;
+            }
         }
     }
     double tot = block_reduce_cols(sq, kp, sh);
     if (threadIdx.x < kp) part[(uint64_t)blockIdx.x * kp + threadIdx.x] = tot;
+}
+
+// The same fused pass with two columns per lane (16-byte loads and stores, half the lanes per
+// row) for plain designs and an even number of columns: the scattered coefficient-row loads
+// are limited by the vector memory instruction rate, so moving twice the bytes per
+// instruction shortens the pass.  Per row and column the arithmetic is exactly that of
+// k_gather_seg.  A lane computes both of its columns when either is active (an inactive
+// column's u, norm partial and level sums are never read again).
+typedef double dbl2 __attribute__((ext_vector_type(2)));
+
+template <bool WEIGHTED, bool FIRST, int TT>
+__global__ __launch_bounds__(BLK) __attribute__((amdgpu_waves_per_eu(4, 8))) void k_gather_seg2(DesignArgs d, TermInfo ts, const uint32_t *__restrict__ fcodes,
+                                                     const SegItem *__restrict__ items, uint32_t n_items,
+                                                     const double *__restrict__ sw, const double *__restrict__ v,
+                                                     const double *uin, double *u, double *__restrict__ atu,
+                                                     double *__restrict__ partials, ColState cs,
+                                                     double *__restrict__ part, uint32_t k, uint32_t kshift) {
+    __shared__ double sh[BLK];
+    const uint32_t kp = 1u << kshift, kv = kp >> 1; // lanes per row
+    const uint32_t jv = threadIdx.x & (kv - 1), j0 = 2 * jv;
+    double sq0 = 0.0, sq1 = 0.0;
+    const bool act = j0 < k && (cs.active[j0] || cs.active[j0 + 1]);
+    if (act) {
+        const double sc0 = cs.gscale[j0], sc1 = cs.gscale[j0 + 1];
+        constexpr int R = 4;
+        const uint32_t groups = (gridDim.x * BLK) / kv;
+        for (uint32_t it = (blockIdx.x * BLK + threadIdx.x) / kv; it < n_items; it += groups) {
+            const SegItem item = items[it];
+            uint32_t cur = item.level;
+            double adj0 = 0.0, adj1 = 0.0;
+            for (uint32_t p0 = item.begin; p0 < item.end; p0 += R) {
+                uint32_t rows[R]; // row indices fit 32 bits (n_obs < 2^32)
+                bool ok[R];
+                dbl2 uo[R];
+                double wq[R];
+                uint32_t lv[R], code[TT][R];
+#pragma unroll
+                for (int q = 0; q < R; ++q) {
+                    ok[q] = p0 + q < item.end;
+                    rows[q] = ok[q] ? p0 + q : p0; // clamped: loads stay unconditional
+                    uo[q] = __builtin_nontemporal_load((const dbl2 *)&uin[(uint64_t)rows[q] * k + j0]);
+                    wq[q] = WEIGHTED ? sw[rows[q]] : 1.0;
+                    lv[q] = fcodes[rows[q]];
+#pragma unroll
+                    for (int t = 0; t < TT; ++t) code[t][q] = d.codes[(uint64_t)t * d.n + rows[q]];
+                }
+                dbl2 val[TT][R];
+#pragma unroll
+                for (int t = 0; t < TT; ++t)
+#pragma unroll
+                    for (int q = 0; q < R; ++q)
+                        val[t][q] = *(const dbl2 *)&v[(d.terms[t].offset + code[t][q]) * k + j0];
+#pragma unroll
+                for (int q = 0; q < R; ++q) {
+                    if (!ok[q]) continue;
+                    double a0 = 0.0, a1 = 0.0;
+#pragma unroll
+                    for (int t = 0; t < TT; ++t) {
+                        a0 += val[t][q].x;
+                        a1 += val[t][q].y;
+                    }
+                    if (WEIGHTED) {
+                        a0 *= wq[q];
+                        a1 *= wq[q];
+                    }
+                    const double b0 = (FIRST && WEIGHTED) ? wq[q] * uo[q].x : uo[q].x;
+                    const double b1 = (FIRST && WEIGHTED) ? wq[q] * uo[q].y : uo[q].y;
+                    dbl2 nu;
+                    nu.x = a0 + sc0 * b0;
+                    nu.y = a1 + sc1 * b1;
+                    __builtin_nontemporal_store(nu, (dbl2 *)&u[(uint64_t)rows[q] * k + j0]);
+                    sq0 += nu.x * nu.x;
+                    sq1 += nu.y * nu.y;
+                    if (lv[q] != cur) {
+                        dbl2 av;
+                        av.x = adj0;
+                        av.y = adj1;
+                        *(dbl2 *)&atu[(ts.offset + cur) * k + j0] = av;
+                        const dbl2 zero = {0.0, 0.0};
+                        for (uint32_t l = cur + 1; l < lv[q]; ++l) *(dbl2 *)&atu[(ts.offset + l) * k + j0] = zero;
+                        adj0 = adj1 = 0.0;
+                        cur = lv[q];
+                    }
+                    adj0 += WEIGHTED ? wq[q] * nu.x : nu.x;
+                    adj1 += WEIGHTED ? wq[q] * nu.y : nu.y;
+                }
+            }
+            dbl2 av;
+            av.x = adj0;
+            av.y = adj1;
+            if (item.dest >= 0) {
+                *(dbl2 *)&atu[(ts.offset + cur) * k + j0] = av;
+                const dbl2 zero = {0.0, 0.0};
+                for (uint32_t l = cur + 1; l <= (uint32_t)item.dest; ++l) *(dbl2 *)&atu[(ts.offset + l) * k + j0] = zero;
+            } else if (item.dest != SEG_SKIP) {
+                *(dbl2 *)&partials[(uint64_t)(-item.dest - 1) * k + j0] = av;
+            }
+        }
+    }
+    // column j0 sums over the threads with the same (tid & (kv-1)), then column j0 + 1
+    const double t0 = block_reduce_cols(sq0, kv, sh);
+    __syncthreads();
+    if (threadIdx.x < kv) part[(uint64_t)blockIdx.x * kp + 2 * threadIdx.x] = t0;
+    __syncthreads();
+    const double t1 = block_reduce_cols(sq1, kv, sh);
+    if (threadIdx.x < kv) part[(uint64_t)blockIdx.x * kp + 2 * threadIdx.x + 1] = t1;
 }
 
 // ---------------------------------------------------------------------------
@@ -271,6 +435,56 @@ __global__ __launch_bounds__(BLK) void k_segreduce(DesignArgs d, TermInfo ti, co
 #pragma unroll
             for (int c = 0; c < NC; ++c) partials[(slot * NC + c) * k + j] = acc[c];
         }
+    }
+}
+
+// The same reduction for a single intercept column with two columns per lane (16-byte row
+// loads, half the lanes per row) for an even number of columns; per column the sums are
+// exactly those of k_segreduce<1, SCALE>.
+template <bool SCALE>
+__global__ __launch_bounds__(BLK) void k_segreduce2(TermInfo ti, const uint32_t *__restrict__ perm,
+                                                    const SegItem *__restrict__ items, uint32_t n_items,
+                                                    const double *__restrict__ scale_pos,
+                                                    const double *__restrict__ src, double *__restrict__ out,
+                                                    double *__restrict__ partials, const int32_t *__restrict__ active,
+                                                    uint32_t k, uint32_t kshift) {
+    const uint32_t kv = (1u << kshift) >> 1;
+    const uint32_t j0 = 2 * (threadIdx.x & (kv - 1));
+    if (j0 >= k || (active && !active[j0] && !active[j0 + 1])) return;
+    const uint32_t groups = (gridDim.x * BLK) / kv;
+    for (uint32_t it = (blockIdx.x * BLK + threadIdx.x) / kv; it < n_items; it += groups) {
+        const SegItem item = items[it];
+        double a0 = 0.0, a1 = 0.0;
+        constexpr uint32_t U = 8;
+        uint32_t p = item.begin;
+        for (; p + U <= item.end; p += U) {
+            uint32_t row[U];
+            double sc[U];
+            dbl2 b[U];
+#pragma unroll
+            for (int q = 0; q < (int)U; ++q) {
+                row[q] = perm ? perm[p + q] : p + q;
+                sc[q] = SCALE ? scale_pos[p + q] : 1.0;
+            }
+#pragma unroll
+            for (int q = 0; q < (int)U; ++q) b[q] = *(const dbl2 *)&src[(uint64_t)row[q] * k + j0];
+#pragma unroll
+            for (int q = 0; q < (int)U; ++q) {
+                a0 += SCALE ? sc[q] * b[q].x : b[q].x;
+                a1 += SCALE ? sc[q] * b[q].y : b[q].y;
+            }
+        }
+        for (; p < item.end; ++p) {
+            const uint64_t row = perm ? perm[p] : p;
+            const dbl2 b = *(const dbl2 *)&src[row * k + j0];
+            a0 += SCALE ? scale_pos[p] * b.x : b.x;
+            a1 += SCALE ? scale_pos[p] * b.y : b.y;
+        }
+        dbl2 av;
+        av.x = a0;
+        av.y = a1;
+        if (item.dest >= 0) *(dbl2 *)&out[(ti.offset + item.level) * k + j0] = av;
+        else if (item.dest != SEG_SKIP) *(dbl2 *)&partials[(uint64_t)(-item.dest - 1) * k + j0] = av;
     }
 }
 
@@ -841,11 +1055,11 @@ __global__ __launch_bounds__(BLK) void k_demean(DesignArgs d, const double *__re
         double yv[R], acc[R];
 #pragma unroll
         for (int q = 0; q < R; ++q) {
-            rows[q] = i0 + q * rstride;
-            ok[q] = rows[q] < d.n;
-            yv[q] = ok[q] ? y_int[rows[q] * k + j] : 0.0;
+            ok[q] = i0 + q * rstride < d.n;
+            rows[q] = ok[q] ? i0 + q * rstride : i0; // clamped: loads stay unconditional
+            yv[q] = y_int[rows[q] * k + j];
         }
-        gather_rows<R, TT, PLAIN>(d, x, rows, ok, k, j, acc);
+        gather_rows<R, TT, PLAIN>(d, x, rows, k, j, acc);
 #pragma unroll
         for (int q = 0; q < R; ++q)
             if (ok[q]) dm_int[rows[q] * k + j] = yv[q] - acc[q];
@@ -984,6 +1198,7 @@ void upload_design(Solver &s, const std::vector<double> &weights_internal) {
     dv.terms.clear();
     dv.terms.resize(d.terms.size());
     dv.max_partials = 0;
+    std::vector<std::vector<SegItem>> host_items(d.terms.size()); // of the fusable terms
     dv.plain = true;
     for (const Term &tm : d.terms) dv.plain = dv.plain && tm.ncols() == 1 && tm.loads[0] < 0;
     for (size_t t = 0; t < d.terms.size(); ++t) {
@@ -1035,6 +1250,7 @@ void upload_design(Solver &s, const std::vector<double> &weights_internal) {
         }
         dt.items.upload(items.data(), items.size());
         dt.n_items = (uint32_t)items.size();
+        if (tm.sorted && tm.ncols() == 1 && tm.loads[0] < 0) host_items[t] = items;
         dt.fixes.upload(fixes.data(), fixes.size());
         dt.n_fixes = (uint32_t)fixes.size();
         dt.n_partials = partials;
@@ -1085,6 +1301,30 @@ void upload_design(Solver &s, const std::vector<double> &weights_internal) {
         if (!tm.sorted || tm.ncols() != 1 || tm.loads[0] >= 0) continue;
         if (dv.fuse_term < 0 || dv.terms[t].n_items > dv.terms[dv.fuse_term].n_items) dv.fuse_term = (int32_t)t;
     }
+    if (dv.fuse_term >= 0) {
+        // the fused gather's items: runs of consecutive complete levels (empty ones included)
+        // merged up to SEG_CHUNK rows and GATHER_SPAN levels; chunks of split levels keep their
+        // partial slots
+        constexpr uint32_t GATHER_SPAN = 256;
+        const std::vector<SegItem> &src = host_items[dv.fuse_term];
+        std::vector<SegItem> g;
+        g.reserve(src.size());
+        for (const SegItem &it : src) {
+            if (!g.empty() && it.dest >= 0) {
+                SegItem &m = g.back();
+                if (m.dest >= 0 && it.level == (uint32_t)m.dest + 1 && it.begin == m.end &&
+                    (it.end - m.begin) <= SEG_CHUNK && (it.level - m.level) < GATHER_SPAN) {
+                    m.end = it.end;
+                    m.dest = it.dest;
+                    continue;
+                }
+            }
+            g.push_back(it);
+        }
+        DevTerm &ft = dv.terms[dv.fuse_term];
+        ft.gitems.upload(g.data(), g.size());
+        ft.n_gitems = (uint32_t)g.size();
+    }
 }
 
 namespace {
@@ -1099,9 +1339,17 @@ void launch_segfix(const DevTerm &dt, double *partials, double *out, const int32
         fi.load[0] = -1;
         for (const DevTerm::FixStage &sg : dt.stages) {
             const uint32_t grid = std::max<uint32_t>(1, (sg.n_items + groups_per_block - 1) / groups_per_block);
-            hipLaunchKernelGGL((k_segreduce<1, false>), dim3(grid), dim3(BLK), 0, st, DesignArgs{}, fi, nullptr,
-                               sg.items.as<SegItem>(), sg.n_items, nullptr, partials + sg.src_base * k, out,
-                               partials + sg.dst_base * k, active, k, kshift);
+            if (k >= 2 && (k & 1) == 0) {
+                const uint32_t grid2 = std::max<uint32_t>(1, (sg.n_items + 2 * groups_per_block - 1) /
+                                                                 (2 * groups_per_block));
+                hipLaunchKernelGGL((k_segreduce2<false>), dim3(grid2), dim3(BLK), 0, st, fi, nullptr,
+                                   sg.items.as<SegItem>(), sg.n_items, nullptr, partials + sg.src_base * k, out,
+                                   partials + sg.dst_base * k, active, k, kshift);
+            } else {
+                hipLaunchKernelGGL((k_segreduce<1, false>), dim3(grid), dim3(BLK), 0, st, DesignArgs{}, fi, nullptr,
+                                   sg.items.as<SegItem>(), sg.n_items, nullptr, partials + sg.src_base * k, out,
+                                   partials + sg.dst_base * k, active, k, kshift);
+            }
         }
         return;
     }
@@ -1140,7 +1388,17 @@ void launch_adjoint(const DevDesign &dv, const DesignArgs &da, int which, const 
     case 3: WX_SEG(3, SC); break;                                                                                \
     default: WX_SEG(4, SC); break;                                                                               \
     }
-        if (f) { WX_SEG_NC(true) }
+        if (dt.info.ncols == 1 && dt.info.load[0] < 0 && k >= 2 && (k & 1) == 0) {
+            // single intercept column, even width: two columns per lane
+            const uint32_t grid2 = std::max<uint32_t>(
+                1, std::min<uint32_t>((dt.n_items + 2 * groups_per_block - 1) / (2 * groups_per_block), 65535u * 8));
+            if (f)
+                hipLaunchKernelGGL((k_segreduce2<true>), dim3(grid2), dim3(BLK), 0, st, dt.info, perm,
+                                   dt.items.as<SegItem>(), dt.n_items, f, src, out, partials, active, k, kshift);
+            else
+                hipLaunchKernelGGL((k_segreduce2<false>), dim3(grid2), dim3(BLK), 0, st, dt.info, perm,
+                                   dt.items.as<SegItem>(), dt.n_items, f, src, out, partials, active, k, kshift);
+        } else if (f) { WX_SEG_NC(true) }
         else { WX_SEG_NC(false) }
 #undef WX_SEG_NC
 #undef WX_SEG
@@ -1191,14 +1449,45 @@ uint32_t launch_gather_seg_t(const DevDesign &dv, const DesignArgs &da, const do
     const DevTerm &dt = dv.terms[dv.fuse_term];
     const uint32_t groups_per_block = BLK >> kshift;
     const uint32_t grid = pk ? pk->n_item_blocks
-                             : std::max<uint32_t>(1, std::min<uint32_t>((dt.n_items + groups_per_block - 1) /
+                             : std::max<uint32_t>(1, std::min<uint32_t>((dt.n_gitems + groups_per_block - 1) /
                                                                             groups_per_block,
                                                                         MAX_FUSED_BLOCKS));
     const uint32_t *blk_col = pk ? pk->item_col.as<uint32_t>() : nullptr;
-    const SegItem *items = dt.items.as<SegItem>();
+    const SegItem *items = dt.gitems.as<SegItem>();
+    const uint32_t *fcodes = da.codes + (uint64_t)dv.fuse_term * da.n;
+    // small single-intercept terms other than the fused one are read from LDS
+    Staged sg{};
+    uint32_t lds_rows = 0;
+    if (dv.plain)
+        for (uint32_t t = 0; t < da.T; ++t) {
+            const uint64_t L = da.terms[t].n_levels;
+            if ((int32_t)t == dv.fuse_term || (lds_rows + L) * k * sizeof(double) > STAGE_BYTES) continue;
+            sg.mask |= 1u << t;
+            sg.off[t] = lds_rows;
+            lds_rows += (uint32_t)L;
+        }
+    const size_t lds_bytes = (size_t)lds_rows * k * sizeof(double);
+    if (!pk && dv.plain && k >= 2 && (k & 1) == 0 && da.T >= 1 && da.T <= 4) {
+        // two columns per lane: half the lanes per row, twice the row groups per workgroup
+        const uint32_t gpb = 2 * groups_per_block;
+        const uint32_t grid2 = std::max<uint32_t>(
+            1, std::min<uint32_t>((dt.n_gitems + gpb - 1) / gpb, MAX_FUSED_BLOCKS));
+#define WX_GS2(TT)                                                                                               \
+    hipLaunchKernelGGL((k_gather_seg2<W, F, TT>), dim3(grid2), dim3(BLK), 0, st, da, dt.info, fcodes, items,     \
+                       dt.n_gitems, sw, v, uin, u, atu, partials, cs, part, k, kshift)
+        switch (da.T) {
+        case 1: WX_GS2(1); break;
+        case 2: WX_GS2(2); break;
+        case 3: WX_GS2(3); break;
+        default: WX_GS2(4); break;
+        }
+#undef WX_GS2
+        return grid2;
+    }
 #define WX_GS(TT, PL)                                                                                            \
-    hipLaunchKernelGGL((k_gather_seg<W, F, TT, PL>), dim3(grid), dim3(BLK), 0, st, da, dt.info, items, dt.n_items, \
-                       sw, v, uin, u, atu, partials, cs, part, k, kshift, blk_col)
+    hipLaunchKernelGGL((k_gather_seg<W, F, TT, PL>), dim3(grid), dim3(BLK), lds_bytes, st, da, dt.info,          \
+                       (uint32_t)dv.fuse_term, fcodes, items,                                                      \
+                       dt.n_gitems, sw, v, uin, u, atu, partials, cs, part, k, kshift, blk_col, sg)
     switch (dv.plain ? da.T : 0) {
     case 1: WX_GS(1, true); break;
     case 2: WX_GS(2, true); break;

@@ -185,7 +185,8 @@ std::shared_ptr<Pack> build_pack(const std::vector<const Solver *> &ms, const st
     upload_design(*sv, w);
     DevDesign &dv = sv->dev;
     if (dv.fuse_term < 0) fail(WX_ERR_INVALID_INPUT, "packed design has no sorted term to fuse");
-    // fused term: each member's items padded to PACK_ALIGN (every workgroup inside one member)
+    // fused term: the gather walks one level (or level chunk) per item, each member's items
+    // padded to PACK_ALIGN (every workgroup inside one member)
     DevTerm &ft = dv.terms[dv.fuse_term];
     const uint32_t fterm = (uint32_t)dv.fuse_term;
     std::vector<SegItem> items(ft.n_items), padded;
@@ -203,8 +204,8 @@ std::shared_ptr<Pack> build_pack(const std::vector<const Solver *> &ms, const st
         item_ptr.push_back((uint32_t)(m + 1));
     }
     if (q != items.size()) fail(WX_ERR_INVALID_INPUT, "packed design: work items out of member order");
-    ft.items.upload(padded.data(), padded.size());
-    ft.n_items = (uint32_t)padded.size();
+    ft.gitems.upload(padded.data(), padded.size());
+    ft.n_gitems = (uint32_t)padded.size();
     pk->n_item_blocks = (uint32_t)(padded.size() / g);
     std::vector<uint32_t> item_col(pk->n_item_blocks);
     for (size_t m = 0; m < M; ++m)

@@ -36,8 +36,9 @@ struct DesignArgs {
 };
 
 // Work item of the deterministic D' segmented reduction: rows at positions [begin, end)
-// of the term's code-sorted order sum into `dest` (>= 0: the level itself; < 0: partial
-// slot -dest-1, combined afterwards in a fixed order).
+// of the term's code-sorted order sum into `dest` (>= 0: the levels level .. dest, complete,
+// usually one; < 0: partial slot -dest-1 of `level`, combined afterwards in a fixed order;
+// SEG_SKIP: a padding item with no rows).
 struct SegItem {
     uint32_t level;
     uint32_t begin;
@@ -55,8 +56,12 @@ struct DevTerm {
     TermInfo info;
     DevBuf perm;   // position -> internal row, empty when the term is sorted
     DevBuf sw_pos, w_pos; // sqrt-weights and weights by position (unsorted weighted terms)
-    DevBuf items;  // SegItem[n_items]
+    DevBuf items;  // SegItem[n_items], one level (or one chunk of a level) each
     uint32_t n_items = 0;
+    // fused gather's items: consecutive complete levels merged up to SEG_CHUNK rows, so every
+    // row group gets a similar amount of work (the fused term only)
+    DevBuf gitems;
+    uint32_t n_gitems = 0;
     DevBuf fixes;  // SegFix[n_fixes] (one-stage combine, terms with several columns)
     uint32_t n_fixes = 0;
     uint32_t n_partials = 0;
