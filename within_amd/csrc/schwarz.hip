@@ -236,6 +236,84 @@ __global__ __launch_bounds__(BLK) void sk_rows(const RowItem *__restrict__ items
     }
 }
 
+// sk_rows with two columns per lane (16-byte loads of the gathered vector rows, half the lanes
+// per row) for an even number of columns; per column the arithmetic and the summation order
+// are those of sk_rows.
+typedef double dbl2 __attribute__((ext_vector_type(2)));
+
+template <typename CT, int MODE>
+__device__ inline dbl2 row_term2(const uint32_t *idx, const CT *val, uint64_t p, const double *loc, const double *xs,
+                                 const CT *inv_de, dbl2 mu, uint32_t k, uint32_t j0) {
+    const uint64_t e = idx[p];
+    const double v = (double)val[p];
+    dbl2 r;
+    if (MODE == 0) {
+        const double id = (double)inv_de[e];
+        const dbl2 l = *(const dbl2 *)&loc[e * k + j0];
+        r.x = v * (id * (l.x - mu.x));
+        r.y = v * (id * (l.y - mu.y));
+    } else {
+        const dbl2 x = *(const dbl2 *)&xs[e * k + j0];
+        r.x = v * x.x;
+        r.y = v * x.y;
+    }
+    return r;
+}
+
+template <typename CT, int MODE>
+__global__ __launch_bounds__(BLK) void sk_rows2(const RowItem *__restrict__ items, uint32_t n_items,
+                                                const uint32_t *__restrict__ idx, const CT *__restrict__ val,
+                                                const uint32_t *__restrict__ slot_comp, const double *__restrict__ loc,
+                                                const double *__restrict__ xs, const CT *__restrict__ inv_de,
+                                                const double *__restrict__ mu, double *__restrict__ out,
+                                                double *__restrict__ part, uint32_t k, uint32_t kshift,
+                                                const int32_t *__restrict__ gate) {
+    if (gate_closed(gate, k)) return;
+    const uint32_t kv = (1u << kshift) >> 1, j0 = 2 * (threadIdx.x & (kv - 1));
+    if (j0 >= k) return;
+    const uint32_t groups = (gridDim.x * BLK) / kv;
+    for (uint32_t it = (blockIdx.x * BLK + threadIdx.x) / kv; it < n_items; it += groups) {
+        const RowItem item = items[it];
+        const double *mu_c = mu + (uint64_t)slot_comp[item.row] * k;
+        const dbl2 m = *(const dbl2 *)&mu_c[j0];
+        double a0 = 0.0, a1 = 0.0;
+        uint64_t p = item.begin;
+        for (; p + 8 <= item.end; p += 8) { // eight independent gathers in flight, summed in order
+            dbl2 t[8];
+#pragma unroll
+            for (int q = 0; q < 8; ++q) t[q] = row_term2<CT, MODE>(idx, val, p + q, loc, xs, inv_de, m, k, j0);
+#pragma unroll
+            for (int q = 0; q < 8; ++q) {
+                a0 += t[q].x;
+                a1 += t[q].y;
+            }
+        }
+        for (; p < item.end; ++p) {
+            const dbl2 t = row_term2<CT, MODE>(idx, val, p, loc, xs, inv_de, m, k, j0);
+            a0 += t.x;
+            a1 += t.y;
+        }
+        dbl2 o;
+        if (item.dest >= 0) {
+            const dbl2 b = *(const dbl2 *)&loc[(uint64_t)item.row * k + j0];
+            const double b0 = b.x - m.x, b1 = b.y - m.y;
+            if (MODE == 0) {
+                o.x = b0 + a0;
+                o.y = b1 + a1;
+            } else {
+                const double id = (double)inv_de[item.row];
+                o.x = id * b0 + id * a0;
+                o.y = id * b1 + id * a1;
+            }
+            *(dbl2 *)&out[(uint64_t)item.row * k + j0] = o;
+        } else {
+            o.x = a0;
+            o.y = a1;
+            *(dbl2 *)&part[(uint64_t)(-item.dest - 1) * k + j0] = o;
+        }
+    }
+}
+
 template <typename CT, int MODE>
 __global__ __launch_bounds__(BLK) void sk_rows_fix(const RowFix *__restrict__ fixes, uint32_t n_fix,
                                                    const uint32_t *__restrict__ slot_comp,
@@ -869,6 +947,26 @@ struct SchwarzRunImpl : SchwarzRun {
         dpart.alloc(D->dense_part_rows * k * 8 + 8);
     }
 
+    // the elimination SpMVs (C' rows into the reduced right-hand side, C rows in the
+    // back-substitution); two columns per lane for an even width
+    template <int MODE>
+    void rows_spmv(const DevBuf &items, uint32_t n_items, const DevBuf &idx, const DevBuf &val, double *locp,
+                   double *xsp, double *out, uint32_t k, uint32_t kshift, hipStream_t st, const int32_t *gate) {
+        const SchwarzDev<CT> &d = *D;
+        if (k >= 2 && (k & 1) == 0) {
+            const uint64_t per = 2 * (uint64_t)(BLK >> kshift);
+            const uint32_t g = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>((n_items + per - 1) / per, 65535));
+            hipLaunchKernelGGL((sk_rows2<CT, MODE>), dim3(g), dim3(BLK), 0, st, dp<RowItem>(items), n_items,
+                               dp<uint32_t>(idx), dp<CT>(val), dp<uint32_t>(d.slot_comp), locp, xsp,
+                               dp<CT>(d.inv_de), dp<double>(mu1), out, dp<double>(rowpart), k, kshift, gate);
+        } else {
+            hipLaunchKernelGGL((sk_rows<CT, MODE>), dim3(grid_rows(n_items, kshift)), dim3(BLK), 0, st,
+                               dp<RowItem>(items), n_items, dp<uint32_t>(idx), dp<CT>(val), dp<uint32_t>(d.slot_comp),
+                               locp, xsp, dp<CT>(d.inv_de), dp<double>(mu1), out, dp<double>(rowpart), k, kshift,
+                               gate);
+        }
+    }
+
     void mean(const DevBuf &chunks, uint32_t n_chunks, const DevBuf &cptr, const DevBuf &count, const double *src,
               double *mu, uint32_t k, uint32_t kshift, hipStream_t st, const int32_t *gate) {
         hipLaunchKernelGGL(sk_chunk_sum, dim3(grid_rows(n_chunks, kshift)), dim3(BLK), 0, st, dp<Chunk>(chunks),
@@ -886,10 +984,7 @@ struct SchwarzRunImpl : SchwarzRun {
                            dp<uint32_t>(d.glob), dp<CT>(d.sgnpw), r, locp, k, kshift, gate);
         mean(d.all_chunks, d.n_all_chunks, d.all_cptr, d.comp_count, locp, dp<double>(mu1), k, kshift, st, gate);
         // reduced right-hand side on kept rows
-        hipLaunchKernelGGL((sk_rows<CT, 0>), dim3(grid_rows(d.n_ct_items, kshift)), dim3(BLK), 0, st,
-                           dp<RowItem>(d.ct_items), d.n_ct_items, dp<uint32_t>(d.ct_idx), dp<CT>(d.ct_val),
-                           dp<uint32_t>(d.slot_comp), locp, xsp, dp<CT>(d.inv_de), dp<double>(mu1), redp,
-                           dp<double>(rowpart), k, kshift, gate);
+        rows_spmv<0>(d.ct_items, d.n_ct_items, d.ct_idx, d.ct_val, locp, xsp, redp, k, kshift, st, gate);
         if (d.n_ct_fix)
             hipLaunchKernelGGL((sk_rows_fix<CT, 0>), dim3(grid_rows(d.n_ct_fix, kshift)), dim3(BLK), 0, st,
                                dp<RowFix>(d.ct_fix), d.n_ct_fix, dp<uint32_t>(d.slot_comp), locp, dp<CT>(d.inv_de),
@@ -931,10 +1026,7 @@ struct SchwarzRunImpl : SchwarzRun {
                                dp<uint64_t>(d.drow_off), dp<uint32_t>(d.drow_base), dp<uint32_t>(d.drow_nk),
                                dp<uint32_t>(d.drow_slot), dp<CT>(d.dmat), redp, xsp, k, kshift, gate);
         // back-substitution on eliminated rows
-        hipLaunchKernelGGL((sk_rows<CT, 1>), dim3(grid_rows(d.n_c_items, kshift)), dim3(BLK), 0, st,
-                           dp<RowItem>(d.c_items), d.n_c_items, dp<uint32_t>(d.c_idx), dp<CT>(d.c_val),
-                           dp<uint32_t>(d.slot_comp), locp, xsp, dp<CT>(d.inv_de), dp<double>(mu1), xsp,
-                           dp<double>(rowpart), k, kshift, gate);
+        rows_spmv<1>(d.c_items, d.n_c_items, d.c_idx, d.c_val, locp, xsp, xsp, k, kshift, st, gate);
         if (d.n_c_fix)
             hipLaunchKernelGGL((sk_rows_fix<CT, 1>), dim3(grid_rows(d.n_c_fix, kshift)), dim3(BLK), 0, st,
                                dp<RowFix>(d.c_fix), d.n_c_fix, dp<uint32_t>(d.slot_comp), locp, dp<CT>(d.inv_de),
