@@ -1066,6 +1066,51 @@ __global__ __launch_bounds__(BLK) void k_demean(DesignArgs d, const double *__re
     }
 }
 
+// k_demean with two columns per lane (plain designs, even width); per column the arithmetic is
+// that of k_demean.
+template <int TT>
+__global__ __launch_bounds__(BLK) void k_demean2(DesignArgs d, const double *__restrict__ x,
+                                                 const double *__restrict__ y_int, double *__restrict__ dm_int,
+                                                 uint32_t k, uint32_t kshift) {
+    const uint32_t kv = (1u << kshift) >> 1, j0 = 2 * (threadIdx.x & (kv - 1));
+    if (j0 >= k) return;
+    const uint32_t rstride = (uint32_t)(((uint64_t)gridDim.x * BLK) / kv);
+    constexpr int R = 4;
+    for (uint32_t i0 = (blockIdx.x * BLK + threadIdx.x) / kv; i0 < d.n; i0 += R * rstride) {
+        uint32_t rows[R];
+        bool ok[R];
+        dbl2 yv[R];
+        uint32_t code[TT][R];
+#pragma unroll
+        for (int q = 0; q < R; ++q) {
+            ok[q] = i0 + q * rstride < d.n;
+            rows[q] = ok[q] ? i0 + q * rstride : i0; // clamped: loads stay unconditional
+            yv[q] = *(const dbl2 *)&y_int[(uint64_t)rows[q] * k + j0];
+#pragma unroll
+            for (int t = 0; t < TT; ++t) code[t][q] = d.codes[(uint64_t)t * d.n + rows[q]];
+        }
+        dbl2 val[TT][R];
+#pragma unroll
+        for (int t = 0; t < TT; ++t)
+#pragma unroll
+            for (int q = 0; q < R; ++q) val[t][q] = *(const dbl2 *)&x[(d.terms[t].offset + code[t][q]) * k + j0];
+#pragma unroll
+        for (int q = 0; q < R; ++q) {
+            if (!ok[q]) continue;
+            double a0 = 0.0, a1 = 0.0;
+#pragma unroll
+            for (int t = 0; t < TT; ++t) {
+                a0 += val[t][q].x;
+                a1 += val[t][q].y;
+            }
+            dbl2 o;
+            o.x = yv[q].x - a0;
+            o.y = yv[q].y - a1;
+            *(dbl2 *)&dm_int[(uint64_t)rows[q] * k + j0] = o;
+        }
+    }
+}
+
 // Internal-order interleaved dm_int -> caller-order column-major Dm (solver.rs:483), through
 // 64-row LDS tiles (k+1 doubles per tile row) so the column-major writes are coalesced.
 __global__ __launch_bounds__(BLK) void k_store_caller(uint64_t n, const uint32_t *__restrict__ inv_perm,
@@ -1513,6 +1558,19 @@ uint32_t launch_gather_seg(const DevDesign &dv, const DesignArgs &da, bool first
 
 void launch_demean(const DevDesign &dv, const DesignArgs &da, uint32_t grid, const double *x, const double *y_int,
                    double *dm_int, uint32_t k, uint32_t kshift, hipStream_t st) {
+    if (dv.plain && k >= 2 && (k & 1) == 0 && da.T >= 1 && da.T <= 4 && da.n < (1ull << 31)) {
+        const uint32_t g2 = std::max<uint32_t>(1, grid / 2);
+#define WX_DM2(TT)                                                                                               \
+    hipLaunchKernelGGL((k_demean2<TT>), dim3(g2), dim3(BLK), 0, st, da, x, y_int, dm_int, k, kshift)
+        switch (da.T) {
+        case 1: WX_DM2(1); break;
+        case 2: WX_DM2(2); break;
+        case 3: WX_DM2(3); break;
+        default: WX_DM2(4); break;
+        }
+#undef WX_DM2
+        return;
+    }
 #define WX_DM(TT, PL)                                                                                            \
     hipLaunchKernelGGL((k_demean<TT, PL>), dim3(grid), dim3(BLK), 0, st, da, x, y_int, dm_int, k, kshift)
     switch (dv.plain ? da.T : 0) {
