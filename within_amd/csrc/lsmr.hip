@@ -29,7 +29,6 @@ constexpr int BLK = (int)KERNEL_BLOCK;
 constexpr uint32_t SEG_CHUNK = 64;  // rows per D' work item before a level is split
 constexpr uint32_t FIX_CHUNK = 64;  // partials per combine item
 constexpr uint32_t MAX_FUSED_BLOCKS = 32768; // grid cap of the fused gather (one item per row group)
-constexpr size_t STAGE_BYTES = 16384;        // LDS per workgroup for small terms' coefficient rows
 
 // Per-column LSMR state (structure of arrays, each of length MAX_BATCH).
 struct ColState {
@@ -63,31 +62,9 @@ __device__ inline double block_reduce_cols(double v, uint32_t kp, double *sh) {
 // column, so the fold reduces to the value itself (-0.0 + x*1.0 == x bitwise) and all R*TT
 // code loads are issued before any x load.  Every row must be valid: callers clamp rows past
 // their range to a valid one and discard those results, so no load sits behind a branch.
-// Terms whose coefficient rows a workgroup keeps in LDS (bit t of mask: term t's rows
-// x[offset_t + l], all k columns, at LDS row off[t] + l).  Scattered row gathers are bound by
-// the vector L1's line throughput even when they hit; a small term read from LDS costs none.
-struct Staged {
-    uint32_t mask;
-    uint32_t off[MAX_TERMS];
-};
-
-// Copies the staged terms' rows of x into LDS (all threads of the workgroup take part).
-__device__ inline void stage_terms(const DesignArgs &d, const Staged &sg, const double *__restrict__ x, double *xl,
-                                   uint32_t k) {
-    for (uint32_t t = 0; t < d.T; ++t) {
-        if (!((sg.mask >> t) & 1u)) continue;
-        const double *src = x + d.terms[t].offset * k;
-        double *dst = xl + (uint64_t)sg.off[t] * k;
-        const uint32_t cnt = (uint32_t)d.terms[t].n_levels * k;
-        for (uint32_t q = threadIdx.x; q < cnt; q += BLK) dst[q] = src[q];
-    }
-    __syncthreads();
-}
-
 template <int R, int TT, bool PLAIN>
-__device__ inline void gather_rows_staged(const DesignArgs &d, const double *__restrict__ x,
-                                          const uint64_t (&rows)[R], uint32_t k, uint32_t j, double (&acc)[R],
-                                          Staged sg, const double *xl) {
+__device__ inline void gather_rows(const DesignArgs &d, const double *__restrict__ x, const uint64_t (&rows)[R],
+                                   uint32_t k, uint32_t j, double (&acc)[R]) {
     if constexpr (TT > 0 && PLAIN) {
         uint32_t code[TT][R];
 #pragma unroll
@@ -96,15 +73,9 @@ __device__ inline void gather_rows_staged(const DesignArgs &d, const double *__r
             for (int q = 0; q < R; ++q) code[t][q] = d.codes[(uint64_t)t * d.n + rows[q]];
         double val[TT][R];
 #pragma unroll
-        for (int t = 0; t < TT; ++t) {
-            if ((sg.mask >> t) & 1u) {
+        for (int t = 0; t < TT; ++t)
 #pragma unroll
-                for (int q = 0; q < R; ++q) val[t][q] = xl[(sg.off[t] + code[t][q]) * k + j];
-            } else {
-#pragma unroll
-                for (int q = 0; q < R; ++q) val[t][q] = x[(d.terms[t].offset + code[t][q]) * k + j];
-            }
-        }
+            for (int q = 0; q < R; ++q) val[t][q] = x[(d.terms[t].offset + code[t][q]) * k + j];
 #pragma unroll
         for (int q = 0; q < R; ++q) {
             double a = 0.0;
@@ -131,13 +102,6 @@ __device__ inline void gather_rows_staged(const DesignArgs &d, const double *__r
     }
 }
 
-template <int R, int TT, bool PLAIN>
-__device__ inline void gather_rows(const DesignArgs &d, const double *__restrict__ x, const uint64_t (&rows)[R],
-                                   uint32_t k, uint32_t j, double (&acc)[R]) {
-    Staged none;
-    none.mask = 0u;
-    gather_rows_staged<R, TT, PLAIN>(d, x, rows, k, j, acc, none, nullptr);
-}
 
 // ---------------------------------------------------------------------------
 // D v gather fused with the u update:  u <- sqrt(w) .* (D v) + gscale_j * u_old,  ||u||^2
@@ -194,18 +158,14 @@ __global__ __launch_bounds__(BLK) void k_gather_axpy(DesignArgs d, const double 
 // its rows four at a time, and sums sqrt(w_i) u_i in row order -- the order k_segreduce uses,
 // so the result is bitwise the same as the separate reduction.
 template <bool WEIGHTED, bool FIRST, int TT, bool PLAIN>
-__global__ __launch_bounds__(BLK) void k_gather_seg(DesignArgs d, TermInfo ts, uint32_t fterm,
-                                                    const uint32_t *__restrict__ fcodes,
+__global__ __launch_bounds__(BLK) void k_gather_seg(DesignArgs d, TermInfo ts, const uint32_t *__restrict__ fcodes,
                                                     const SegItem *__restrict__ items,
                                                     uint32_t n_items, const double *__restrict__ sw,
                                                     const double *__restrict__ v, const double *uin, double *u,
                                                     double *__restrict__ atu, double *__restrict__ partials,
                                                     ColState cs, double *__restrict__ part, uint32_t k,
-                                                    uint32_t kshift, const uint32_t *__restrict__ blk_col,
-                                                    Staged sg) {
+                                                    uint32_t kshift, const uint32_t *__restrict__ blk_col) {
     __shared__ double sh[BLK];
-    extern __shared__ double vl[]; // the staged terms' rows of v
-    if (sg.mask) stage_terms(d, sg, v, vl, k);
     const uint32_t kp = 1u << kshift;
     const uint32_t j = threadIdx.x & (kp - 1);
     const uint32_t c = (blk_col ? blk_col[blockIdx.x] : 0u) + j; // column-state index
@@ -234,7 +194,7 @@ __global__ __launch_bounds__(BLK) void k_gather_seg(DesignArgs d, TermInfo ts, u
                     wq[q] = WEIGHTED ? sw[rows[q]] : 1.0;
                     lv[q] = fcodes[rows[q]]; // equals cur throughout a single-level item
                 }
-                gather_rows_staged<R, TT, PLAIN>(d, v, rows, k, j, acc, sg, vl);
+                gather_rows<R, TT, PLAIN>(d, v, rows, k, j, acc);
 #pragma unroll
                 for (int q = 0; q < R; ++q) {
                     if (!ok[q]) continue;
@@ -1500,18 +1460,6 @@ uint32_t launch_gather_seg_t(const DevDesign &dv, const DesignArgs &da, const do
     const uint32_t *blk_col = pk ? pk->item_col.as<uint32_t>() : nullptr;
     const SegItem *items = dt.gitems.as<SegItem>();
     const uint32_t *fcodes = da.codes + (uint64_t)dv.fuse_term * da.n;
-    // small single-intercept terms other than the fused one are read from LDS
-    Staged sg{};
-    uint32_t lds_rows = 0;
-    if (dv.plain)
-        for (uint32_t t = 0; t < da.T; ++t) {
-            const uint64_t L = da.terms[t].n_levels;
-            if ((int32_t)t == dv.fuse_term || (lds_rows + L) * k * sizeof(double) > STAGE_BYTES) continue;
-            sg.mask |= 1u << t;
-            sg.off[t] = lds_rows;
-            lds_rows += (uint32_t)L;
-        }
-    const size_t lds_bytes = (size_t)lds_rows * k * sizeof(double);
     if (!pk && dv.plain && k >= 2 && (k & 1) == 0 && da.T >= 1 && da.T <= 4) {
         // two columns per lane: half the lanes per row, twice the row groups per workgroup
         const uint32_t gpb = 2 * groups_per_block;
@@ -1530,9 +1478,8 @@ uint32_t launch_gather_seg_t(const DevDesign &dv, const DesignArgs &da, const do
         return grid2;
     }
 #define WX_GS(TT, PL)                                                                                            \
-    hipLaunchKernelGGL((k_gather_seg<W, F, TT, PL>), dim3(grid), dim3(BLK), lds_bytes, st, da, dt.info,          \
-                       (uint32_t)dv.fuse_term, fcodes, items,                                                      \
-                       dt.n_gitems, sw, v, uin, u, atu, partials, cs, part, k, kshift, blk_col, sg)
+    hipLaunchKernelGGL((k_gather_seg<W, F, TT, PL>), dim3(grid), dim3(BLK), 0, st, da, dt.info, fcodes, items,    \
+                       dt.n_gitems, sw, v, uin, u, atu, partials, cs, part, k, kshift, blk_col)
     switch (dv.plain ? da.T : 0) {
     case 1: WX_GS(1, true); break;
     case 2: WX_GS(2, true); break;

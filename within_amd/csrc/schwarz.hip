@@ -32,6 +32,7 @@ constexpr int BLK = 256;
 constexpr uint32_t ROW_CHUNK = 256;   // CSR entries per work item before a row is split
 constexpr uint32_t MEAN_CHUNK = 64;   // slots per partial-sum chunk
 constexpr uint32_t DENSE_TILED_MIN = 256; // kept blocks at least this large use the tiled product
+constexpr uint64_t DENSE_TILE_TARGET = 8192; // (row tile, column segment) pairs wanted in flight
 constexpr uint64_t LONG_ROW = 64;         // factor rows with more entries are summed by a whole workgroup
 
 struct RowItem {
@@ -363,12 +364,12 @@ __global__ __launch_bounds__(64) void sk_dense_sym(const DenseTile *__restrict__
 #pragma unroll
     for (int c = 0; c < KC; ++c) acc[c] = 0.0;
     uint32_t l = T.l0;
-    for (; l + 4 <= T.l1; l += 4) {
-        double m[4];
+    for (; l + 8 <= T.l1; l += 8) { // eight coefficient loads in flight per lane
+        double m[8];
 #pragma unroll
-        for (int q = 0; q < 4; ++q) m[q] = (double)M[(uint64_t)(l + q) * T.nk];
+        for (int q = 0; q < 8; ++q) m[q] = (double)M[(uint64_t)(l + q) * T.nk];
 #pragma unroll
-        for (int q = 0; q < 4; ++q) {
+        for (int q = 0; q < 8; ++q) {
             const double *rl = R + (uint64_t)(l + q) * kstride;
 #pragma unroll
             for (int c = 0; c < KC; ++c) acc[c] += m[q] * rl[c];
@@ -739,7 +740,7 @@ std::shared_ptr<SchwarzDev<CT>> build_dev(const Precond::Schwarz &S) {
         if (nk >= DENSE_TILED_MIN) {
             // enough (row tile, column segment) pairs to fill the device, segments >= 256 columns
             const uint32_t max_seg = (nk + 255) / 256;
-            const uint32_t want = (uint32_t)std::max<uint64_t>(1, (4096 + row_tiles - 1) / row_tiles);
+            const uint32_t want = (uint32_t)std::max<uint64_t>(1, (DENSE_TILE_TARGET + row_tiles - 1) / row_tiles);
             const uint32_t n_seg = std::min(max_seg, want);
             const uint32_t seg_len = ((nk + n_seg - 1) / n_seg + 63) / 64 * 64;
             DenseComp dc{nk, kb, 0, 0, part_rows};
