@@ -236,7 +236,7 @@ __global__ __launch_bounds__(BLK) void k_gather_seg(DesignArgs d, TermInfo ts, c
 typedef double dbl2 __attribute__((ext_vector_type(2)));
 
 template <bool WEIGHTED, bool FIRST, int TT>
-__global__ __launch_bounds__(BLK) __attribute__((amdgpu_waves_per_eu(4, 8))) void k_gather_seg2(DesignArgs d, TermInfo ts, const uint32_t *__restrict__ fcodes,
+__global__ __launch_bounds__(BLK) __attribute__((amdgpu_waves_per_eu(4, 8))) void k_gather_seg2(DesignArgs d, TermInfo ts, const uint4 *__restrict__ codes4, uint32_t fterm,
                                                      const SegItem *__restrict__ items, uint32_t n_items,
                                                      const double *__restrict__ sw, const double *__restrict__ v,
                                                      const double *uin, double *u, double *__restrict__ atu,
@@ -267,9 +267,11 @@ __global__ __launch_bounds__(BLK) __attribute__((amdgpu_waves_per_eu(4, 8))) voi
                     rows[q] = ok[q] ? p0 + q : p0; // clamped: loads stay unconditional
                     uo[q] = __builtin_nontemporal_load((const dbl2 *)&uin[(uint64_t)rows[q] * k + j0]);
                     wq[q] = WEIGHTED ? sw[rows[q]] : 1.0;
-                    lv[q] = fcodes[rows[q]];
+                    const uint4 c4 = codes4[rows[q]];
+                    const uint32_t cc[4] = {c4.x, c4.y, c4.z, c4.w};
 #pragma unroll
-                    for (int t = 0; t < TT; ++t) code[t][q] = d.codes[(uint64_t)t * d.n + rows[q]];
+                    for (int t = 0; t < TT; ++t) code[t][q] = cc[t];
+                    lv[q] = fterm == 0 ? c4.x : fterm == 1 ? c4.y : fterm == 2 ? c4.z : c4.w;
                 }
                 dbl2 val[TT][R];
 #pragma unroll
@@ -1029,7 +1031,7 @@ __global__ __launch_bounds__(BLK) void k_demean(DesignArgs d, const double *__re
 // k_demean with two columns per lane (plain designs, even width); per column the arithmetic is
 // that of k_demean.
 template <int TT>
-__global__ __launch_bounds__(BLK) void k_demean2(DesignArgs d, const double *__restrict__ x,
+__global__ __launch_bounds__(BLK) void k_demean2(DesignArgs d, const uint4 *__restrict__ codes4, const double *__restrict__ x,
                                                  const double *__restrict__ y_int, double *__restrict__ dm_int,
                                                  uint32_t k, uint32_t kshift) {
     const uint32_t kv = (1u << kshift) >> 1, j0 = 2 * (threadIdx.x & (kv - 1));
@@ -1046,8 +1048,10 @@ __global__ __launch_bounds__(BLK) void k_demean2(DesignArgs d, const double *__r
             ok[q] = i0 + q * rstride < d.n;
             rows[q] = ok[q] ? i0 + q * rstride : i0; // clamped: loads stay unconditional
             yv[q] = *(const dbl2 *)&y_int[(uint64_t)rows[q] * k + j0];
+            const uint4 c4 = codes4[rows[q]];
+            const uint32_t cc[4] = {c4.x, c4.y, c4.z, c4.w};
 #pragma unroll
-            for (int t = 0; t < TT; ++t) code[t][q] = d.codes[(uint64_t)t * d.n + rows[q]];
+            for (int t = 0; t < TT; ++t) code[t][q] = cc[t];
         }
         dbl2 val[TT][R];
 #pragma unroll
@@ -1299,6 +1303,17 @@ void upload_design(Solver &s, const std::vector<double> &weights_internal) {
         }
         dv.max_partials = std::max<uint32_t>(dv.max_partials, (uint32_t)slots);
     }
+    // a row's codes as one aligned 16-byte record: the fused gather and the demean read them with
+    // one load instead of one per term
+    dv.codes4.release();
+    if (dv.plain && d.terms.size() <= 4 && !d.terms.empty()) {
+        std::vector<uint32_t> c4(d.n_obs * 4, 0u);
+        for (size_t t = 0; t < d.terms.size(); ++t) {
+            const uint32_t *col = d.level_column(t);
+            for (uint64_t i = 0; i < d.n_obs; ++i) c4[i * 4 + t] = col[i];
+        }
+        dv.codes4.upload(c4.data(), c4.size());
+    }
     // the sorted single-intercept term with the most work items gets its D' fused into the gather
     dv.fuse_term = -1;
     for (size_t t = 0; t < d.terms.size(); ++t) {
@@ -1460,14 +1475,15 @@ uint32_t launch_gather_seg_t(const DevDesign &dv, const DesignArgs &da, const do
     const uint32_t *blk_col = pk ? pk->item_col.as<uint32_t>() : nullptr;
     const SegItem *items = dt.gitems.as<SegItem>();
     const uint32_t *fcodes = da.codes + (uint64_t)dv.fuse_term * da.n;
-    if (!pk && dv.plain && k >= 2 && (k & 1) == 0 && da.T >= 1 && da.T <= 4) {
+    if (!pk && dv.codes4.p && k >= 2 && (k & 1) == 0) {
         // two columns per lane: half the lanes per row, twice the row groups per workgroup
         const uint32_t gpb = 2 * groups_per_block;
         const uint32_t grid2 = std::max<uint32_t>(
             1, std::min<uint32_t>((dt.n_gitems + gpb - 1) / gpb, MAX_FUSED_BLOCKS));
 #define WX_GS2(TT)                                                                                               \
-    hipLaunchKernelGGL((k_gather_seg2<W, F, TT>), dim3(grid2), dim3(BLK), 0, st, da, dt.info, fcodes, items,     \
-                       dt.n_gitems, sw, v, uin, u, atu, partials, cs, part, k, kshift)
+    hipLaunchKernelGGL((k_gather_seg2<W, F, TT>), dim3(grid2), dim3(BLK), 0, st, da, dt.info,                     \
+                       dv.codes4.as<uint4>(), (uint32_t)dv.fuse_term, items, dt.n_gitems, sw, v, uin, u, atu, partials, \
+                       cs, part, k, kshift)
         switch (da.T) {
         case 1: WX_GS2(1); break;
         case 2: WX_GS2(2); break;
@@ -1505,10 +1521,11 @@ uint32_t launch_gather_seg(const DevDesign &dv, const DesignArgs &da, bool first
 
 void launch_demean(const DevDesign &dv, const DesignArgs &da, uint32_t grid, const double *x, const double *y_int,
                    double *dm_int, uint32_t k, uint32_t kshift, hipStream_t st) {
-    if (dv.plain && k >= 2 && (k & 1) == 0 && da.T >= 1 && da.T <= 4 && da.n < (1ull << 31)) {
+    if (dv.codes4.p && k >= 2 && (k & 1) == 0 && da.n < (1ull << 31)) {
         const uint32_t g2 = std::max<uint32_t>(1, grid / 2);
 #define WX_DM2(TT)                                                                                               \
-    hipLaunchKernelGGL((k_demean2<TT>), dim3(g2), dim3(BLK), 0, st, da, x, y_int, dm_int, k, kshift)
+    hipLaunchKernelGGL((k_demean2<TT>), dim3(g2), dim3(BLK), 0, st, da, dv.codes4.as<uint4>(), x, y_int, dm_int, k, \
+                       kshift)
         switch (da.T) {
         case 1: WX_DM2(1); break;
         case 2: WX_DM2(2); break;

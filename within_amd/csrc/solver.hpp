@@ -79,6 +79,7 @@ struct DevDesign {
     uint64_t n = 0, N = 0;
     DevBuf codes, slopes, sw, w, obs_perm, inv_perm, inv_diag;
     DevBuf sw_caller; // sqrt-weights in caller row order (when the rows are permuted)
+    DevBuf codes4;    // plain designs with at most 4 terms: the row's codes as one 16-byte record
     std::vector<DevTerm> terms;
     uint32_t max_partials = 0;
     bool plain = false; // every term is a single intercept column
