@@ -157,7 +157,7 @@ __global__ __launch_bounds__(BLK) void k_gather_axpy(DesignArgs d, const double 
 // same pass: a row group owns one item (<= 256 consecutive rows of one level), forms u for
 // its rows four at a time, and sums sqrt(w_i) u_i in row order -- the order k_segreduce uses,
 // so the result is bitwise the same as the separate reduction.
-template <bool WEIGHTED, bool FIRST, int TT, bool PLAIN>
+template <bool WEIGHTED, bool FIRST, int TT, bool PLAIN, bool NT>
 __global__ __launch_bounds__(BLK) void k_gather_seg(DesignArgs d, TermInfo ts, const uint32_t *__restrict__ fcodes,
                                                     const SegItem *__restrict__ items,
                                                     uint32_t n_items, const double *__restrict__ sw,
@@ -190,7 +190,7 @@ __global__ __launch_bounds__(BLK) void k_gather_seg(DesignArgs d, TermInfo ts, c
                 for (int q = 0; q < R; ++q) {
                     ok[q] = p0 + q < item.end;
                     rows[q] = ok[q] ? p0 + q : p0; // clamped: loads stay unconditional
-                    uo[q] = __builtin_nontemporal_load(&uin[rows[q] * k + j]);
+                    uo[q] = NT ? __builtin_nontemporal_load(&uin[rows[q] * k + j]) : uin[rows[q] * k + j];
                     wq[q] = WEIGHTED ? sw[rows[q]] : 1.0;
                     lv[q] = fcodes[rows[q]]; // equals cur throughout a single-level item
                 }
@@ -202,9 +202,10 @@ __global__ __launch_bounds__(BLK) void k_gather_seg(DesignArgs d, TermInfo ts, c
                     if (WEIGHTED) a *= wq[q];
                     const double b = (FIRST && WEIGHTED) ? wq[q] * uo[q] : uo[q];
                     const double nu = a + scale * b;
-                    // u streams through once per iteration: non-temporal, so the gathered
-                    // coefficient rows keep their place in L2
-                    __builtin_nontemporal_store(nu, &u[rows[q] * k + j]);
+                    // u streams through once per iteration: non-temporal (wide rows), so the
+                    // gathered coefficient rows keep their place in L2
+                    if (NT) __builtin_nontemporal_store(nu, &u[rows[q] * k + j]);
+                    else u[rows[q] * k + j] = nu;
                     sq += nu * nu;
                     if (lv[q] != cur) {
                         atu[(ts.offset + cur) * k + j] = adj;
@@ -1468,21 +1469,25 @@ uint32_t launch_gather_seg_t(const DevDesign &dv, const DesignArgs &da, const do
                              double *part, uint32_t k, uint32_t kshift, hipStream_t st, const Pack *pk) {
     const DevTerm &dt = dv.terms[dv.fuse_term];
     const uint32_t groups_per_block = BLK >> kshift;
+    // narrow rows (fewer than 8 lanes): one level per item, so there are enough row groups to
+    // fill the device, and ordinary loads (non-temporal ones re-fetch lines for 8-byte rows)
+    const bool wide = kshift >= 3;
+    const bool use_g = pk || wide; // a pack's padded item list is always the gather's
+    const uint32_t n_it = use_g ? dt.n_gitems : dt.n_items;
     const uint32_t grid = pk ? pk->n_item_blocks
-                             : std::max<uint32_t>(1, std::min<uint32_t>((dt.n_gitems + groups_per_block - 1) /
+                             : std::max<uint32_t>(1, std::min<uint32_t>((n_it + groups_per_block - 1) /
                                                                             groups_per_block,
                                                                         MAX_FUSED_BLOCKS));
     const uint32_t *blk_col = pk ? pk->item_col.as<uint32_t>() : nullptr;
-    const SegItem *items = dt.gitems.as<SegItem>();
+    const SegItem *items = use_g ? dt.gitems.as<SegItem>() : dt.items.as<SegItem>();
     const uint32_t *fcodes = da.codes + (uint64_t)dv.fuse_term * da.n;
-    if (!pk && dv.codes4.p && k >= 2 && (k & 1) == 0) {
+    if (!pk && wide && dv.codes4.p && (k & 1) == 0) {
         // two columns per lane: half the lanes per row, twice the row groups per workgroup
         const uint32_t gpb = 2 * groups_per_block;
-        const uint32_t grid2 = std::max<uint32_t>(
-            1, std::min<uint32_t>((dt.n_gitems + gpb - 1) / gpb, MAX_FUSED_BLOCKS));
+        const uint32_t grid2 = std::max<uint32_t>(1, std::min<uint32_t>((n_it + gpb - 1) / gpb, MAX_FUSED_BLOCKS));
 #define WX_GS2(TT)                                                                                               \
     hipLaunchKernelGGL((k_gather_seg2<W, F, TT>), dim3(grid2), dim3(BLK), 0, st, da, dt.info,                     \
-                       dv.codes4.as<uint4>(), (uint32_t)dv.fuse_term, items, dt.n_gitems, sw, v, uin, u, atu, partials, \
+                       dv.codes4.as<uint4>(), (uint32_t)dv.fuse_term, items, n_it, sw, v, uin, u, atu, partials,      \
                        cs, part, k, kshift)
         switch (da.T) {
         case 1: WX_GS2(1); break;
@@ -1494,8 +1499,12 @@ uint32_t launch_gather_seg_t(const DevDesign &dv, const DesignArgs &da, const do
         return grid2;
     }
 #define WX_GS(TT, PL)                                                                                            \
-    hipLaunchKernelGGL((k_gather_seg<W, F, TT, PL>), dim3(grid), dim3(BLK), 0, st, da, dt.info, fcodes, items,    \
-                       dt.n_gitems, sw, v, uin, u, atu, partials, cs, part, k, kshift, blk_col)
+    if (wide)                                                                                                    \
+        hipLaunchKernelGGL((k_gather_seg<W, F, TT, PL, true>), dim3(grid), dim3(BLK), 0, st, da, dt.info, fcodes, \
+                           items, n_it, sw, v, uin, u, atu, partials, cs, part, k, kshift, blk_col);              \
+    else                                                                                                         \
+        hipLaunchKernelGGL((k_gather_seg<W, F, TT, PL, false>), dim3(grid), dim3(BLK), 0, st, da, dt.info,        \
+                           fcodes, items, n_it, sw, v, uin, u, atu, partials, cs, part, k, kshift, blk_col)
     switch (dv.plain ? da.T : 0) {
     case 1: WX_GS(1, true); break;
     case 2: WX_GS(2, true); break;
