@@ -34,6 +34,7 @@ constexpr uint32_t MEAN_CHUNK = 64;   // slots per partial-sum chunk
 constexpr uint32_t DENSE_TILED_MIN = 256; // kept blocks at least this large use the tiled product
 constexpr uint64_t DENSE_TILE_TARGET = 8192; // (row tile, column segment) pairs wanted in flight
 constexpr uint64_t LONG_ROW = 64;         // factor rows with more entries are summed by a whole workgroup
+constexpr uint64_t BIG_LEVEL = 2048;      // factor levels with at least this many rows use grid-wide launches
 
 struct RowItem {
     uint32_t row;   // slot id of the output row
@@ -75,6 +76,10 @@ struct SchwarzDev {
     // sparse kept blocks (one workgroup per component): levels and factor rows
     DevBuf sp_comp, sp_lvl_f_ptr, sp_lvl_b_ptr, sp_lvl_f_rows, sp_lvl_b_rows, sp_meta;
     DevBuf sp_lvl_f_long, sp_lvl_b_long; // per level: first long row
+    DevBuf grid_f_rows, grid_b_rows;     // GridRow lists of the grid-wide steps
+    std::vector<uint64_t> grid_f_ptr, grid_b_ptr; // step s: rows [ptr[s], ptr[s+1])
+    DevBuf cover_rows;                   // auxiliary slots of the Gremban covers
+    uint64_t n_cover_rows = 0;
     DevBuf l_ptr, l_idx, l_val, lt_ptr, lt_idx, lt_val, piv;
     uint32_t n_sparse = 0;
     // auxiliary slots [ns, ns + n_aux) of the covers: real kept slot | sheet bit (second sheet)
@@ -97,6 +102,14 @@ struct SpMeta {
     uint64_t lvl_b_off;
     uint64_t row_off;   // first row pointer of the component in l_ptr / lt_ptr
     uint64_t piv_off;   // first pivot
+    uint32_t nbig_f;    // leading forward levels solved by grid-wide launches (sk_sparse_grid)
+    uint32_t nbig_b;    // trailing backward levels solved by grid-wide launches
+};
+
+// A row of a large level, solved by a grid-wide launch: the component's SpMeta and the row slot.
+struct GridRow {
+    uint32_t meta;
+    uint32_t row;
 };
 
 // A solve passes its per-column "active" flags: when no column is active (the iteration
@@ -519,7 +532,8 @@ __global__ __launch_bounds__(BLK) void sk_sparse(const SpMeta *__restrict__ meta
     const bool on = j < k;
     const double mu = on ? mu2[(uint64_t)m.comp * k + j] : 0.0;
     // forward substitution; l_ptr / piv are indexed by (row slot - kept_base) + row offset
-    for (uint32_t lv = 0; lv < m.nlev_f; ++lv) {
+    // the first nbig_f levels were solved by grid-wide launches before this kernel
+    for (uint32_t lv = m.nbig_f; lv < m.nlev_f; ++lv) {
         const uint64_t o = m.lvl_f_off + lv;
         sparse_level<CT, true>(m, lvl_f_ptr[o], lvl_f_long[o], lvl_f_ptr[o + 1], lvl_f_rows, l_ptr, l_idx, l_val, red,
                                aux_src, ns, mu, xs, sh, k, kshift);
@@ -534,24 +548,56 @@ __global__ __launch_bounds__(BLK) void sk_sparse(const SpMeta *__restrict__ meta
         }
     __syncthreads();
     // backward substitution with L' (rows of L' = columns of L)
-    for (uint32_t lv = 0; lv < m.nlev_b; ++lv) {
+    // the last nbig_b levels are solved by grid-wide launches after this kernel
+    for (uint32_t lv = 0; lv + m.nbig_b < m.nlev_b; ++lv) {
         const uint64_t o = m.lvl_b_off + lv;
         sparse_level<CT, false>(m, lvl_b_ptr[o], lvl_b_long[o], lvl_b_ptr[o + 1], lvl_b_rows, lt_ptr, lt_idx, lt_val,
                                 red, aux_src, ns, mu, xs, sh, k, kshift);
         __syncthreads();
     }
-    if (m.cover) { // read back x = (y+ - y-) / 2 into the real kept slots, one sheet at a time
-        for (int sheet = 0; sheet < 2; ++sheet) {
-            if (on)
-                for (uint32_t r = lane_row; r < m.nk; r += rows_per_pass) {
-                    const uint32_t a = aux_src[m.kept_base + r - ns];
-                    if ((a >> 31) != (uint32_t)sheet) continue;
-                    const double y = 0.5 * xs[(uint64_t)(m.kept_base + r) * k + j];
-                    double &dst = xs[(uint64_t)(a & 0x7FFFFFFFu) * k + j];
-                    dst = sheet ? dst - y : y;
-                }
-            __syncthreads();
-        }
+}
+
+// One large level of every component that has it, all rows in parallel across the grid (one
+// row group per row, entries summed in order).  FORWARD: x = rhs - L x; else x = x - L' x.
+template <typename CT, bool FORWARD>
+__global__ __launch_bounds__(BLK) void sk_sparse_grid(const GridRow *__restrict__ rows, uint64_t n_rows,
+                                                      const SpMeta *__restrict__ meta, const uint64_t *__restrict__ ptr,
+                                                      const uint32_t *__restrict__ idx, const CT *__restrict__ val,
+                                                      const double *__restrict__ red, const double *__restrict__ mu2,
+                                                      const uint32_t *__restrict__ aux_src, uint64_t ns, double *xs,
+                                                      uint32_t k, uint32_t kshift, const int32_t *__restrict__ gate) {
+    if (gate_closed(gate, k)) return;
+    const uint32_t kp = 1u << kshift, j = threadIdx.x & (kp - 1);
+    if (j >= k) return;
+    const uint64_t stride = ((uint64_t)gridDim.x * BLK) >> kshift;
+    for (uint64_t q = ((uint64_t)blockIdx.x * BLK + threadIdx.x) >> kshift; q < n_rows; q += stride) {
+        const GridRow gr = rows[q];
+        const SpMeta &m = meta[gr.meta];
+        const uint64_t lr = (uint64_t)(gr.row - m.kept_base) + m.row_off;
+        double acc = FORWARD ? sparse_rhs<CT>(m, gr.row, red, aux_src, ns, mu2[(uint64_t)m.comp * k + j], k, j)
+                             : xs[(uint64_t)gr.row * k + j];
+        for (uint64_t p = ptr[lr]; p < ptr[lr + 1]; ++p) acc -= (double)val[p] * xs[(uint64_t)idx[p] * k + j];
+        xs[(uint64_t)gr.row * k + j] = acc;
+    }
+}
+
+// Gremban covers: read back x = (y+ - y-) / 2 into the real kept slots, the first sheet's
+// rows assigning and then the second sheet's subtracting (two launches, sheet 0 then 1).
+__global__ __launch_bounds__(BLK) void sk_cover_readback(const uint32_t *__restrict__ cover_rows, uint64_t n,
+                                                         const uint32_t *__restrict__ aux_src, uint64_t ns, double *xs,
+                                                         uint32_t sheet, uint32_t k, uint32_t kshift,
+                                                         const int32_t *__restrict__ gate) {
+    if (gate_closed(gate, k)) return;
+    const uint32_t kp = 1u << kshift, j = threadIdx.x & (kp - 1);
+    if (j >= k) return;
+    const uint64_t stride = ((uint64_t)gridDim.x * BLK) >> kshift;
+    for (uint64_t q = ((uint64_t)blockIdx.x * BLK + threadIdx.x) >> kshift; q < n; q += stride) {
+        const uint32_t r = cover_rows[q];
+        const uint32_t a = aux_src[r - ns];
+        if ((a >> 31) != sheet) continue;
+        const double y = 0.5 * xs[(uint64_t)r * k + j];
+        double &dst = xs[(uint64_t)(a & 0x7FFFFFFFu) * k + j];
+        dst = sheet ? dst - y : y;
     }
 }
 
@@ -846,21 +892,19 @@ std::shared_ptr<SchwarzDev<CT>> build_dev(const Precond::Schwarz &S) {
             for (uint32_t i = 0; i <= nk; ++i) ltptr.push_back(tbase + tp[i]);
             ltidx.insert(ltidx.end(), ti.begin(), ti.end());
             ltval.insert(ltval.end(), tv.begin(), tv.end());
-            // levels: forward lvl(i) = 1 + max lvl(j), j in row i of L; backward over L' in reverse
+            // levels: forward lvl(i) = 1 + max lvl(j), j in row i of L; the backward schedule is
+            // its mirror, depth - 1 - lvl(i) (a row's dependents in L' sit on higher forward
+            // levels), so the large leading forward levels become the trailing backward ones
             std::vector<uint32_t> lf(nk, 0), lb(nk, 0);
-            uint32_t df = 0, db = 0;
+            uint32_t df = 0;
             for (uint32_t i = 0; i < nk; ++i) {
                 uint32_t v = 0;
                 for (uint64_t p = S.l_ptr[ro + i]; p < S.l_ptr[ro + i + 1]; ++p) v = std::max(v, lf[S.l_idx[p]] + 1);
                 lf[i] = v;
                 df = std::max(df, v + 1);
             }
-            for (uint32_t i = nk; i-- > 0;) {
-                uint32_t v = 0;
-                for (uint64_t p = tp[i]; p < tp[i + 1]; ++p) v = std::max(v, lb[ti[p] - kb] + 1);
-                lb[i] = v;
-                db = std::max(db, v + 1);
-            }
+            const uint32_t db = df;
+            for (uint32_t i = 0; i < nk; ++i) lb[i] = df - 1 - lf[i];
             // rows bucketed by level, short rows first and rows with more than LONG_ROW entries
             // last; `lng` (aligned with `ptr`) marks where a level's long rows start
             auto bucket = [&](const std::vector<uint32_t> &lvl, uint32_t depth, const std::vector<uint64_t> &rnnz,
@@ -898,12 +942,57 @@ std::shared_ptr<SchwarzDev<CT>> build_dev(const Precond::Schwarz &S) {
             m.nlev_b = db;
             m.lvl_f_off = bucket(lf, df, nnz_f, lvf_ptr, lvf_long, lvf_rows);
             m.lvl_b_off = bucket(lb, db, nnz_b, lvb_ptr, lvb_long, lvb_rows);
+            // leading forward and trailing backward levels with at least BIG_LEVEL rows are solved
+            // by grid-wide launches; the workgroup of the component takes the rest
+            auto level_rows = [&](const std::vector<uint64_t> &ptr, uint64_t off, uint32_t l) {
+                return ptr[off + l + 1] - ptr[off + l];
+            };
+            m.nbig_f = 0;
+            while (m.nbig_f < df && level_rows(lvf_ptr, m.lvl_f_off, m.nbig_f) >= BIG_LEVEL) ++m.nbig_f;
+            m.nbig_b = 0;
+            while (m.nbig_b < db && level_rows(lvb_ptr, m.lvl_b_off, db - 1 - m.nbig_b) >= BIG_LEVEL) ++m.nbig_b;
             max_depth = std::max<uint64_t>(max_depth, std::max(df, db));
             total_rows += nk;
             metas.push_back(m);
         }
         D->n_sparse = (uint32_t)metas.size();
         D->sp_meta.upload(metas.data(), metas.size());
+        // grid-wide steps: forward step g solves level g of every component with nbig_f > g;
+        // backward steps are aligned on the last level, step h solving level nlev_b - maxB + h
+        uint32_t max_f = 0, max_b = 0;
+        for (const SpMeta &m : metas) {
+            max_f = std::max(max_f, m.nbig_f);
+            max_b = std::max(max_b, m.nbig_b);
+        }
+        std::vector<GridRow> gf, gb;
+        D->grid_f_ptr.assign(1, 0);
+        D->grid_b_ptr.assign(1, 0);
+        for (uint32_t g = 0; g < max_f; ++g) {
+            for (uint32_t mi = 0; mi < metas.size(); ++mi) {
+                const SpMeta &m = metas[mi];
+                if (g >= m.nbig_f) continue;
+                const uint64_t o = m.lvl_f_off + g;
+                for (uint64_t q = lvf_ptr[o]; q < lvf_ptr[o + 1]; ++q) gf.push_back({mi, lvf_rows[q]});
+            }
+            D->grid_f_ptr.push_back(gf.size());
+        }
+        for (uint32_t h = 0; h < max_b; ++h) {
+            for (uint32_t mi = 0; mi < metas.size(); ++mi) {
+                const SpMeta &m = metas[mi];
+                if (h + m.nbig_b < max_b) continue;
+                const uint64_t o = m.lvl_b_off + (m.nlev_b - max_b + h);
+                for (uint64_t q = lvb_ptr[o]; q < lvb_ptr[o + 1]; ++q) gb.push_back({mi, lvb_rows[q]});
+            }
+            D->grid_b_ptr.push_back(gb.size());
+        }
+        D->grid_f_rows.upload(gf.data(), gf.size());
+        D->grid_b_rows.upload(gb.data(), gb.size());
+        std::vector<uint32_t> crow;
+        for (const SpMeta &m : metas)
+            if (m.cover)
+                for (uint32_t r = 0; r < m.nk; ++r) crow.push_back(m.kept_base + r);
+        D->n_cover_rows = crow.size();
+        D->cover_rows.upload(crow.data(), crow.size());
         D->n_aux = aux_src.size();
         if (ns + D->n_aux >= 0x7FFFFFFFull) fail(WX_ERR_INVALID_INPUT, "too many Schwarz slots for the device path");
         D->aux_src.upload(aux_src.data(), aux_src.size());
@@ -993,6 +1082,16 @@ struct SchwarzRunImpl : SchwarzRun {
         // kept-block solves
         if (d.n_sparse) {
             mean(d.kept_chunks, d.n_kept_chunks, d.kept_cptr, d.comp_kcount, redp, dp<double>(mu2), k, kshift, st, gate);
+            // large leading forward levels across the grid, then each component's workgroup
+            // (remaining forward levels, pivots, leading backward levels), then the large
+            // trailing backward levels across the grid, then the covers' read-back
+            for (size_t g = 0; g + 1 < d.grid_f_ptr.size(); ++g) {
+                const uint64_t b = d.grid_f_ptr[g], n = d.grid_f_ptr[g + 1] - b;
+                hipLaunchKernelGGL((sk_sparse_grid<CT, true>), dim3(grid_rows(n, kshift)), dim3(BLK), 0, st,
+                                   dp<GridRow>(d.grid_f_rows) + b, n, dp<SpMeta>(d.sp_meta), dp<uint64_t>(d.l_ptr),
+                                   dp<uint32_t>(d.l_idx), dp<CT>(d.l_val), redp, dp<double>(mu2),
+                                   dp<uint32_t>(d.aux_src), d.ns, xsp, k, kshift, gate);
+            }
             hipLaunchKernelGGL((sk_sparse<CT>), dim3(d.n_sparse), dim3(BLK), 0, st, dp<SpMeta>(d.sp_meta),
                                dp<uint64_t>(d.sp_lvl_f_ptr), dp<uint64_t>(d.sp_lvl_f_long),
                                dp<uint32_t>(d.sp_lvl_f_rows), dp<uint64_t>(d.sp_lvl_b_ptr),
@@ -1000,6 +1099,18 @@ struct SchwarzRunImpl : SchwarzRun {
                                dp<uint32_t>(d.l_idx), dp<CT>(d.l_val), dp<uint64_t>(d.lt_ptr), dp<uint32_t>(d.lt_idx),
                                dp<CT>(d.lt_val), dp<CT>(d.piv), redp, dp<double>(mu2), dp<uint32_t>(d.aux_src), d.ns,
                                xsp, k, kshift, gate);
+            for (size_t h = 0; h + 1 < d.grid_b_ptr.size(); ++h) {
+                const uint64_t b = d.grid_b_ptr[h], n = d.grid_b_ptr[h + 1] - b;
+                hipLaunchKernelGGL((sk_sparse_grid<CT, false>), dim3(grid_rows(n, kshift)), dim3(BLK), 0, st,
+                                   dp<GridRow>(d.grid_b_rows) + b, n, dp<SpMeta>(d.sp_meta), dp<uint64_t>(d.lt_ptr),
+                                   dp<uint32_t>(d.lt_idx), dp<CT>(d.lt_val), redp, dp<double>(mu2),
+                                   dp<uint32_t>(d.aux_src), d.ns, xsp, k, kshift, gate);
+            }
+            if (d.n_cover_rows)
+                for (uint32_t sheet = 0; sheet < 2; ++sheet)
+                    hipLaunchKernelGGL(sk_cover_readback, dim3(grid_rows(d.n_cover_rows, kshift)), dim3(BLK), 0, st,
+                                       dp<uint32_t>(d.cover_rows), d.n_cover_rows, dp<uint32_t>(d.aux_src), d.ns,
+                                       xsp, sheet, k, kshift, gate);
         }
         if (d.n_dtiles) {
             // column windows of exact power-of-two widths (<= 16): the wave-uniform red loads
