@@ -80,6 +80,8 @@ struct SchwarzDev {
     std::vector<uint64_t> grid_f_ptr, grid_b_ptr; // step s: rows [ptr[s], ptr[s+1])
     DevBuf cover_rows;                   // auxiliary slots of the Gremban covers
     uint64_t n_cover_rows = 0;
+    DevBuf diag_rows;                    // every factor row (GridRow), for the pivots
+    uint64_t n_diag_rows = 0;
     DevBuf l_ptr, l_idx, l_val, lt_ptr, lt_idx, lt_val, piv;
     uint32_t n_sparse = 0;
     // auxiliary slots [ns, ns + n_aux) of the covers: real kept slot | sheet bit (second sheet)
@@ -334,15 +336,27 @@ __global__ __launch_bounds__(BLK) void sk_rows_fix(const RowFix *__restrict__ fi
                                                    const double *__restrict__ loc, const CT *__restrict__ inv_de,
                                                    const double *__restrict__ mu, const double *__restrict__ part,
                                                    double *__restrict__ out, uint32_t k, uint32_t kshift, const int32_t *__restrict__ gate) {
+    // one workgroup per split row: its row groups stride over the partials (a hub row can have
+    // thousands), then a fixed LDS tree adds the groups' sums
     if (gate_closed(gate, k)) return;
+    __shared__ double sh[BLK];
     const uint32_t kp = 1u << kshift, j = threadIdx.x & (kp - 1);
-    if (j >= k) return;
-    const uint32_t g = (blockIdx.x * BLK + threadIdx.x) >> kshift;
-    if (g >= n_fix) return;
-    const RowFix f = fixes[g];
-    double acc = 0.0;
-    for (uint32_t q = 0; q < f.count; ++q) acc += part[(f.first + q) * k + j];
-    row_finish<CT, MODE>(f.row, acc, loc, out, inv_de, mu + (uint64_t)slot_comp[f.row] * k, k, j);
+    const uint32_t grp = threadIdx.x >> kshift, ngrp = BLK >> kshift;
+    for (uint32_t fi = blockIdx.x; fi < n_fix; fi += gridDim.x) {
+        const RowFix f = fixes[fi];
+        double acc = 0.0;
+        if (j < k)
+            for (uint32_t q = grp; q < f.count; q += ngrp) acc += part[(uint64_t)(f.first + q) * k + j];
+        sh[threadIdx.x] = acc;
+        __syncthreads();
+        for (uint32_t s = BLK / 2; s >= kp && s > 0; s >>= 1) {
+            if (threadIdx.x < s) sh[threadIdx.x] += sh[threadIdx.x + s];
+            __syncthreads();
+        }
+        if (threadIdx.x < kp && j < k)
+            row_finish<CT, MODE>(f.row, sh[threadIdx.x], loc, out, inv_de, mu + (uint64_t)slot_comp[f.row] * k, k, j);
+        __syncthreads();
+    }
 }
 
 // Large dense kept blocks: xs = S^+ red as a split-K product that streams S^+ once.  S^+ is
@@ -523,32 +537,26 @@ __global__ __launch_bounds__(BLK) void sk_sparse(const SpMeta *__restrict__ meta
                                                  const CT *__restrict__ piv, const double *__restrict__ red,
                                                  const double *__restrict__ mu2, const uint32_t *__restrict__ aux_src,
                                                  uint64_t ns, double *xs, uint32_t k,
-                                                 uint32_t kshift, const int32_t *__restrict__ gate) {
+                                                 uint32_t kshift, const int32_t *__restrict__ gate, uint32_t phase) {
     if (gate_closed(gate, k)) return;
     __shared__ double sh[BLK];
     const uint32_t kp = 1u << kshift, j = threadIdx.x & (kp - 1);
-    const uint32_t lane_row = threadIdx.x >> kshift, rows_per_pass = BLK >> kshift;
     const SpMeta m = meta[blockIdx.x];
     const bool on = j < k;
     const double mu = on ? mu2[(uint64_t)m.comp * k + j] : 0.0;
-    // forward substitution; l_ptr / piv are indexed by (row slot - kept_base) + row offset
-    // the first nbig_f levels were solved by grid-wide launches before this kernel
-    for (uint32_t lv = m.nbig_f; lv < m.nlev_f; ++lv) {
-        const uint64_t o = m.lvl_f_off + lv;
-        sparse_level<CT, true>(m, lvl_f_ptr[o], lvl_f_long[o], lvl_f_ptr[o + 1], lvl_f_rows, l_ptr, l_idx, l_val, red,
-                               aux_src, ns, mu, xs, sh, k, kshift);
-        __syncthreads();
-    }
-    // diagonal
-    if (on)
-        for (uint32_t r = lane_row; r < m.nk; r += rows_per_pass) {
-            const double dv = (double)piv[m.piv_off + r];
-            const uint64_t q = (uint64_t)(m.kept_base + r) * k + j;
-            xs[q] = dv != 0.0 ? xs[q] / dv : 0.0;
+    if (phase == 0) {
+        // forward substitution; l_ptr / piv are indexed by (row slot - kept_base) + row offset;
+        // the first nbig_f levels were solved by grid-wide launches before this kernel
+        for (uint32_t lv = m.nbig_f; lv < m.nlev_f; ++lv) {
+            const uint64_t o = m.lvl_f_off + lv;
+            sparse_level<CT, true>(m, lvl_f_ptr[o], lvl_f_long[o], lvl_f_ptr[o + 1], lvl_f_rows, l_ptr, l_idx, l_val,
+                                   red, aux_src, ns, mu, xs, sh, k, kshift);
+            __syncthreads();
         }
-    __syncthreads();
-    // backward substitution with L' (rows of L' = columns of L)
-    // the last nbig_b levels are solved by grid-wide launches after this kernel
+        return;
+    }
+    // (pivots applied by sk_sparse_diag in between) backward substitution with L' (rows of L' =
+    // columns of L); the last nbig_b levels are solved by grid-wide launches after this kernel
     for (uint32_t lv = 0; lv + m.nbig_b < m.nlev_b; ++lv) {
         const uint64_t o = m.lvl_b_off + lv;
         sparse_level<CT, false>(m, lvl_b_ptr[o], lvl_b_long[o], lvl_b_ptr[o + 1], lvl_b_rows, lt_ptr, lt_idx, lt_val,
@@ -578,6 +586,26 @@ __global__ __launch_bounds__(BLK) void sk_sparse_grid(const GridRow *__restrict_
                              : xs[(uint64_t)gr.row * k + j];
         for (uint64_t p = ptr[lr]; p < ptr[lr + 1]; ++p) acc -= (double)val[p] * xs[(uint64_t)idx[p] * k + j];
         xs[(uint64_t)gr.row * k + j] = acc;
+    }
+}
+
+// y /= d on every factor row (a zero pivot -- the floating gauge -- gives 0), all components
+// across the grid, between the forward and the backward substitutions.
+template <typename CT>
+__global__ __launch_bounds__(BLK) void sk_sparse_diag(const GridRow *__restrict__ rows, uint64_t n_rows,
+                                                      const SpMeta *__restrict__ meta, const CT *__restrict__ piv,
+                                                      double *xs, uint32_t k, uint32_t kshift,
+                                                      const int32_t *__restrict__ gate) {
+    if (gate_closed(gate, k)) return;
+    const uint32_t kp = 1u << kshift, j = threadIdx.x & (kp - 1);
+    if (j >= k) return;
+    const uint64_t stride = ((uint64_t)gridDim.x * BLK) >> kshift;
+    for (uint64_t q = ((uint64_t)blockIdx.x * BLK + threadIdx.x) >> kshift; q < n_rows; q += stride) {
+        const GridRow gr = rows[q];
+        const SpMeta &m = meta[gr.meta];
+        const double dv = (double)piv[m.piv_off + (gr.row - m.kept_base)];
+        const uint64_t x = (uint64_t)gr.row * k + j;
+        xs[x] = dv != 0.0 ? xs[x] / dv : 0.0;
     }
 }
 
@@ -988,9 +1016,15 @@ std::shared_ptr<SchwarzDev<CT>> build_dev(const Precond::Schwarz &S) {
         D->grid_f_rows.upload(gf.data(), gf.size());
         D->grid_b_rows.upload(gb.data(), gb.size());
         std::vector<uint32_t> crow;
-        for (const SpMeta &m : metas)
+        std::vector<GridRow> drow;
+        for (uint32_t mi = 0; mi < metas.size(); ++mi) {
+            const SpMeta &m = metas[mi];
+            for (uint32_t r = 0; r < m.nk; ++r) drow.push_back({mi, m.kept_base + r});
             if (m.cover)
                 for (uint32_t r = 0; r < m.nk; ++r) crow.push_back(m.kept_base + r);
+        }
+        D->n_diag_rows = drow.size();
+        D->diag_rows.upload(drow.data(), drow.size());
         D->n_cover_rows = crow.size();
         D->cover_rows.upload(crow.data(), crow.size());
         D->n_aux = aux_src.size();
@@ -1076,7 +1110,7 @@ struct SchwarzRunImpl : SchwarzRun {
         // reduced right-hand side on kept rows
         rows_spmv<0>(d.ct_items, d.n_ct_items, d.ct_idx, d.ct_val, locp, xsp, redp, k, kshift, st, gate);
         if (d.n_ct_fix)
-            hipLaunchKernelGGL((sk_rows_fix<CT, 0>), dim3(grid_rows(d.n_ct_fix, kshift)), dim3(BLK), 0, st,
+            hipLaunchKernelGGL((sk_rows_fix<CT, 0>), dim3(std::min<uint32_t>(d.n_ct_fix, 65535)), dim3(BLK), 0, st,
                                dp<RowFix>(d.ct_fix), d.n_ct_fix, dp<uint32_t>(d.slot_comp), locp, dp<CT>(d.inv_de),
                                dp<double>(mu1), dp<double>(rowpart), redp, k, kshift, gate);
         // kept-block solves
@@ -1092,13 +1126,19 @@ struct SchwarzRunImpl : SchwarzRun {
                                    dp<uint32_t>(d.l_idx), dp<CT>(d.l_val), redp, dp<double>(mu2),
                                    dp<uint32_t>(d.aux_src), d.ns, xsp, k, kshift, gate);
             }
-            hipLaunchKernelGGL((sk_sparse<CT>), dim3(d.n_sparse), dim3(BLK), 0, st, dp<SpMeta>(d.sp_meta),
-                               dp<uint64_t>(d.sp_lvl_f_ptr), dp<uint64_t>(d.sp_lvl_f_long),
-                               dp<uint32_t>(d.sp_lvl_f_rows), dp<uint64_t>(d.sp_lvl_b_ptr),
-                               dp<uint64_t>(d.sp_lvl_b_long), dp<uint32_t>(d.sp_lvl_b_rows), dp<uint64_t>(d.l_ptr),
-                               dp<uint32_t>(d.l_idx), dp<CT>(d.l_val), dp<uint64_t>(d.lt_ptr), dp<uint32_t>(d.lt_idx),
-                               dp<CT>(d.lt_val), dp<CT>(d.piv), redp, dp<double>(mu2), dp<uint32_t>(d.aux_src), d.ns,
-                               xsp, k, kshift, gate);
+            for (uint32_t phase = 0; phase < 2; ++phase) {
+                hipLaunchKernelGGL((sk_sparse<CT>), dim3(d.n_sparse), dim3(BLK), 0, st, dp<SpMeta>(d.sp_meta),
+                                   dp<uint64_t>(d.sp_lvl_f_ptr), dp<uint64_t>(d.sp_lvl_f_long),
+                                   dp<uint32_t>(d.sp_lvl_f_rows), dp<uint64_t>(d.sp_lvl_b_ptr),
+                                   dp<uint64_t>(d.sp_lvl_b_long), dp<uint32_t>(d.sp_lvl_b_rows),
+                                   dp<uint64_t>(d.l_ptr), dp<uint32_t>(d.l_idx), dp<CT>(d.l_val),
+                                   dp<uint64_t>(d.lt_ptr), dp<uint32_t>(d.lt_idx), dp<CT>(d.lt_val), dp<CT>(d.piv),
+                                   redp, dp<double>(mu2), dp<uint32_t>(d.aux_src), d.ns, xsp, k, kshift, gate, phase);
+                if (phase == 0)
+                    hipLaunchKernelGGL((sk_sparse_diag<CT>), dim3(grid_rows(d.n_diag_rows, kshift)), dim3(BLK), 0,
+                                       st, dp<GridRow>(d.diag_rows), d.n_diag_rows, dp<SpMeta>(d.sp_meta),
+                                       dp<CT>(d.piv), xsp, k, kshift, gate);
+            }
             for (size_t h = 0; h + 1 < d.grid_b_ptr.size(); ++h) {
                 const uint64_t b = d.grid_b_ptr[h], n = d.grid_b_ptr[h + 1] - b;
                 hipLaunchKernelGGL((sk_sparse_grid<CT, false>), dim3(grid_rows(n, kshift)), dim3(BLK), 0, st,
@@ -1140,7 +1180,7 @@ struct SchwarzRunImpl : SchwarzRun {
         // back-substitution on eliminated rows
         rows_spmv<1>(d.c_items, d.n_c_items, d.c_idx, d.c_val, locp, xsp, xsp, k, kshift, st, gate);
         if (d.n_c_fix)
-            hipLaunchKernelGGL((sk_rows_fix<CT, 1>), dim3(grid_rows(d.n_c_fix, kshift)), dim3(BLK), 0, st,
+            hipLaunchKernelGGL((sk_rows_fix<CT, 1>), dim3(std::min<uint32_t>(d.n_c_fix, 65535)), dim3(BLK), 0, st,
                                dp<RowFix>(d.c_fix), d.n_c_fix, dp<uint32_t>(d.slot_comp), locp, dp<CT>(d.inv_de),
                                dp<double>(mu1), dp<double>(rowpart), xsp, k, kshift, gate);
         mean(d.all_chunks, d.n_all_chunks, d.all_cptr, d.comp_count, xsp, dp<double>(mu3), k, kshift, st, gate);
