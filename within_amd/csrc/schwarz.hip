@@ -477,6 +477,38 @@ __device__ inline double sparse_rhs(const SpMeta &m, uint32_t row, const double 
     return red[(uint64_t)row * k + j] - mu;
 }
 
+// sum(val[p] * xs[idx[p]]) over [p0, p1) for one column, accumulated into acc in entry order
+// (acc -= each term when SUB, else acc += each), with eight entries' loads in flight at a time:
+// the chain index -> gathered value is a memory round trip, so one entry at a time would
+// leave a hub row of thousands of entries latency-bound.
+template <typename CT, bool SUB>
+__device__ inline double dot_entries(double acc, uint64_t p0, uint64_t p1, uint64_t step, const uint32_t *idx,
+                                     const CT *val, const double *xs, uint32_t k, uint32_t j) {
+    constexpr int U = 8;
+    uint64_t p = p0;
+    for (; p + (U - 1) * step < p1; p += U * step) {
+        uint32_t e[U];
+        double v[U], x[U];
+#pragma unroll
+        for (int q = 0; q < U; ++q) {
+            e[q] = idx[p + q * step];
+            v[q] = (double)val[p + q * step];
+        }
+#pragma unroll
+        for (int q = 0; q < U; ++q) x[q] = xs[(uint64_t)e[q] * k + j];
+#pragma unroll
+        for (int q = 0; q < U; ++q) {
+            if (SUB) acc -= v[q] * x[q];
+            else acc += v[q] * x[q];
+        }
+    }
+    for (; p < p1; p += step) {
+        if (SUB) acc -= (double)val[p] * xs[(uint64_t)idx[p] * k + j];
+        else acc += (double)val[p] * xs[(uint64_t)idx[p] * k + j];
+    }
+    return acc;
+}
+
 // One level of a triangular solve.  Rows [b, el) are short: one row group each, entries
 // summed in order.  Rows [el, e) are long (more than LONG_ROW entries, typically the hubs
 // eliminated last): the whole workgroup splits each one's entries over its row groups and
@@ -494,16 +526,14 @@ __device__ inline void sparse_level(const SpMeta &m, uint64_t b, uint64_t el, ui
             const uint32_t row = rows[q];
             const uint64_t lr = (uint64_t)(row - m.kept_base) + m.row_off;
             double acc = FORWARD ? sparse_rhs<CT>(m, row, red, aux_src, ns, mu, k, j) : xs[(uint64_t)row * k + j];
-            for (uint64_t p = ptr[lr]; p < ptr[lr + 1]; ++p) acc -= (double)val[p] * xs[(uint64_t)idx[p] * k + j];
+            acc = dot_entries<CT, true>(acc, ptr[lr], ptr[lr + 1], 1, idx, val, xs, k, j);
             xs[(uint64_t)row * k + j] = acc;
         }
     for (uint64_t q = el; q < e; ++q) {
         const uint32_t row = rows[q];
         const uint64_t lr = (uint64_t)(row - m.kept_base) + m.row_off;
         double part = 0.0;
-        if (on)
-            for (uint64_t p = ptr[lr] + grp; p < ptr[lr + 1]; p += ngrp)
-                part += (double)val[p] * xs[(uint64_t)idx[p] * k + j];
+        if (on) part = dot_entries<CT, false>(0.0, ptr[lr] + grp, ptr[lr + 1], ngrp, idx, val, xs, k, j);
         sh[threadIdx.x] = part;
         __syncthreads();
         for (uint32_t s = BLK / 2; s >= kp && s > 0; s >>= 1) {
@@ -584,7 +614,7 @@ __global__ __launch_bounds__(BLK) void sk_sparse_grid(const GridRow *__restrict_
         const uint64_t lr = (uint64_t)(gr.row - m.kept_base) + m.row_off;
         double acc = FORWARD ? sparse_rhs<CT>(m, gr.row, red, aux_src, ns, mu2[(uint64_t)m.comp * k + j], k, j)
                              : xs[(uint64_t)gr.row * k + j];
-        for (uint64_t p = ptr[lr]; p < ptr[lr + 1]; ++p) acc -= (double)val[p] * xs[(uint64_t)idx[p] * k + j];
+        acc = dot_entries<CT, true>(acc, ptr[lr], ptr[lr + 1], 1, idx, val, xs, k, j);
         xs[(uint64_t)gr.row * k + j] = acc;
     }
 }
