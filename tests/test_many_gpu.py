@@ -165,3 +165,20 @@ def test_many_maxiter_and_zero_rhs(gpu, problems):
     assert not any(r_cap.converged) and r_cap.iterations == ref.iterations
     assert max(rel(r_cap.demeaned[:, j], ref.demeaned[:, j]) for j in range(Y.shape[1])) <= 1e-10
     assert all(r_zero.converged) and np.all(r_zero.demeaned == 0.0)
+
+
+def test_many_fresh_process_runs_the_autotuner(gpu):
+    """A process that never pinned a shape lets the autotuner choose (regression: the pinned
+    shape used to start out as one stream without packing)."""
+    import subprocess
+    import sys
+    code = ("import numpy as np, within_amd as wa\n"
+            "from tests._configs import cfg4_problems\n"
+            "pr = cfg4_problems(seed=5, n_problems=6, lo=3.3, hi=3.8)\n"
+            "r = wa.solve_many([wa.Solver(c) for c, _ in pr], [Y for _, Y in pr])\n"
+            "assert all(all(x.converged) for x in r)\n"
+            "print('shape', wa.last_many_shape())\n")
+    out = subprocess.run([sys.executable, "-c", code], capture_output=True, text=True, timeout=240,
+                         env={**__import__("os").environ, "WITHIN_AMD_VERBOSE": "1"})
+    assert out.returncode == 0, out.stderr[-2000:]
+    assert "autotune" in out.stderr

@@ -11,6 +11,8 @@
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <map>
 #include <mutex>
@@ -2202,7 +2204,7 @@ struct ManyShape {
 std::mutex g_tune_mu;
 std::vector<std::pair<uint64_t, ManyShape>> g_tuned; // problem-count class -> shape
 ManyShape g_last_shape;
-ManyShape g_forced;      // streams == 0: the autotuner decides
+ManyShape g_forced{0, 1, false, false}; // streams == 0: the autotuner decides
 
 std::vector<hipStream_t> &stream_pool() {
     static std::vector<hipStream_t> pool;
@@ -2347,6 +2349,9 @@ void solve_many_device(const std::vector<const Solver *> &solvers, const std::ve
             const auto t0 = std::chrono::steady_clock::now();
             run_many(solvers, Y, ldy, k, probe, X, Dm, stats, c, caller);
             const double dt = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+            if (std::getenv("WITHIN_AMD_VERBOSE"))
+                fprintf(stderr, "[within_amd] autotune streams %d group %d graph %d pack %d: %.2f ms\n", c.streams,
+                        c.group, (int)c.graph, (int)c.pack, dt * 1e3);
             if (dt < best) {
                 best = dt;
                 shape = c;
