@@ -166,6 +166,17 @@ wx_status wx_solver_new(const wx_design *design, const double *weights, const wx
 
 void wx_solver_release(wx_solver *s);
 
+/* A Solver for the same design with new observation weights (NULL: unweighted), as
+ * wx_solver_new(design, weights, cfg, prebuilt) would build it, reusing `base`'s design and
+ * device index structures (copied on the device) so that only the weights and the
+ * preconditioner are formed anew; with `prebuilt` (e.g. the base's preconditioner, still an SPD
+ * preconditioner of the reweighted system) nothing is built on the host.  No counterpart in the
+ * reference, which builds every Solver from its design; for callers that re-solve one design
+ * under changing weights (the differentiable projection's training loops).  Designs with slope
+ * columns are rejected: their whitened columns depend on the weights. */
+wx_status wx_solver_reweight(const wx_solver *base, const double *weights, const wx_precond_config *cfg,
+                             const wx_precond *prebuilt, wx_solver **out);
+
 /* Replaces `Solver::n_obs / n_dofs` (crates/within/src/solver.rs:586-596). */
 uint64_t wx_solver_n_obs(const wx_solver *s);
 uint64_t wx_solver_n_dofs(const wx_solver *s);

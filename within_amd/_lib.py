@@ -80,6 +80,7 @@ EXPORTS = {
                                 C.POINTER(C.c_void_p)], C.c_int),
     "wx_design_release": ([C.c_void_p], None),
     "wx_solver_new": ([C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.POINTER(C.c_void_p)], C.c_int),
+    "wx_solver_reweight": ([C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.POINTER(C.c_void_p)], C.c_int),
     "wx_solver_release": ([C.c_void_p], None),
     "wx_solver_n_obs": ([C.c_void_p], C.c_uint64),
     "wx_solver_n_dofs": ([C.c_void_p], C.c_uint64),

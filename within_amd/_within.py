@@ -553,8 +553,12 @@ class Solver:
                                        None if prebuilt is None else prebuilt._h, C.byref(out)))
         finally:
             L.wx_design_release(d)
+        self._adopt(out, prebuilt is not None, t0)
+
+    def _adopt(self, out, prebuilt: bool, t0: float):
+        L = _lib.lib()
         self._h = out
-        self._prebuilt = prebuilt is not None
+        self._prebuilt = prebuilt
         self._time_setup = time.perf_counter() - t0
         nt = int(L.wx_solver_n_terms(out))
         off = np.zeros(nt, np.uint64)
@@ -569,7 +573,30 @@ class Solver:
         self._unidentified = [UnidentifiedDirection(*tr[3 * i:3 * i + 3]) for i in range(cnt.value)]
         self._warnings = [L.wx_solver_warning(out, i).decode() for i in range(int(L.wx_solver_n_warnings(out)))]
         for w in self._warnings:
-            warnings.warn(w, UserWarning, stacklevel=2)
+            warnings.warn(w, UserWarning, stacklevel=3)
+
+    def reweight(self, weights: Optional[np.ndarray], preconditioner=None) -> "Solver":
+        """A Solver for this design with new observation weights (None: unweighted), equal to
+        ``Solver(design, weights, preconditioner)``, that reuses this Solver's device design
+        (copied on the device) instead of rebuilding it from the codes.  ``preconditioner`` is
+        a config (built anew for the new weights) or a prebuilt Preconditioner, e.g. this
+        Solver's own, which stays an SPD preconditioner of the reweighted normal equations.
+        Designs with slope columns are rejected (their whitened columns depend on the weights)."""
+        t0 = time.perf_counter()
+        L = _lib.lib()
+        if weights is not None:
+            weights = np.ascontiguousarray(_f64_1d("weights", weights))
+            if weights.shape[0] != self.n_obs:
+                raise ValueError(f"weights has length {weights.shape[0]}, expected {self.n_obs}")
+        prebuilt = preconditioner if isinstance(preconditioner, Preconditioner) else None
+        cfg = None if prebuilt is not None else _native_config(preconditioner)
+        out = C.c_void_p()
+        _lib.check(L.wx_solver_reweight(self._h, None if weights is None else weights.ctypes.data,
+                                        None if cfg is None else C.byref(cfg),
+                                        None if prebuilt is None else prebuilt._h, C.byref(out)))
+        r = Solver.__new__(Solver)
+        r._adopt(out, prebuilt is not None, t0)
+        return r
 
     def __del__(self):
         h = getattr(self, "_h", None)

@@ -12,7 +12,12 @@ The backward pass is hand-written: for an upstream gradient ``G``,
 
 where the adjoint solve is the library's solve of ``G / w`` (its normal equations are exactly
 ``D'WD lambda = D'G`` for positive weights) and one fused gather kernel forms both gradients
-(``wx_solver_projection_grad_device``).  PyTorch is used only here, at the binding: tensors
+(``wx_solver_projection_grad_device``).
+
+``design`` may be a Solver of the same design: the forward then reweights it
+(``Solver.reweight``: the device design is copied on the device rather than rebuilt from the
+codes), which is what a training loop that re-solves one design under changing weights wants;
+with a prebuilt Preconditioner as well, nothing is built on the host at all.  PyTorch is used only here, at the binding: tensors
 are float64 CUDA tensors shaped ``(n, k)`` (or ``(n,)``), passed to the C ABI by pointer.
 """
 from __future__ import annotations
@@ -35,9 +40,9 @@ def _colmajor(t: torch.Tensor) -> torch.Tensor:
 
 
 class WeightedProjection(torch.autograd.Function):
-    """R = Z - D x(Z, w).  Forward builds a Solver for the weights (host preconditioner build,
-    design upload) and runs the batched device solve; backward runs the adjoint solve and the
-    fused gradient kernel with the same Solver."""
+    """R = Z - D x(Z, w).  Forward builds a Solver for the weights (from the design, or by
+    reweighting a given Solver) and runs the batched device solve; backward runs the adjoint
+    solve and the fused gradient kernel with the same Solver."""
 
     @staticmethod
     def forward(ctx, Z: torch.Tensor, w: torch.Tensor, design, options: Optional[LsmrOptions], preconditioner):
@@ -51,7 +56,11 @@ class WeightedProjection(torch.autograd.Function):
         if w.shape != (n,):
             raise ValueError(f"w must have shape ({n},), got {tuple(w.shape)}")
         opts = options or LsmrOptions()
-        solver = Solver(design, weights=w.detach().cpu().numpy(), preconditioner=preconditioner)
+        wh = w.detach().cpu().numpy()
+        if isinstance(design, Solver):
+            solver = design.reweight(wh, preconditioner)
+        else:
+            solver = Solver(design, weights=wh, preconditioner=preconditioner)
         L = _lib.lib()
         Zc = _colmajor(Z2.detach())
         Rc = torch.empty_like(Zc)
@@ -85,5 +94,6 @@ class WeightedProjection(torch.autograd.Function):
 
 def project(Z: torch.Tensor, w: torch.Tensor, design, options: Optional[LsmrOptions] = None,
             preconditioner=None) -> torch.Tensor:
-    """Differentiable demeaned output ``Z - D x`` of the weighted fixed-effects projection."""
+    """Differentiable demeaned output ``Z - D x`` of the weighted fixed-effects projection.
+    ``design`` is a design (codes or effects) or a Solver of that design to reweight."""
     return WeightedProjection.apply(Z, w, design, options, preconditioner)

@@ -233,6 +233,49 @@ wx_status wx_solver_new(const wx_design *design, const double *weights, const wx
     });
 }
 
+wx_status wx_solver_reweight(const wx_solver *base, const double *weights, const wx_precond_config *cfg,
+                             const wx_precond *prebuilt, wx_solver **out) {
+    return guard([&] {
+        if (!base || !out) fail(WX_ERR_INVALID_INPUT, "null solver or output pointer");
+        if (cfg && prebuilt) fail(WX_ERR_INVALID_INPUT, "pass either a preconditioner config or a prebuilt one");
+        const Solver &b = base->s;
+        if (b.design.has_slopes())
+            fail(WX_ERR_INVALID_INPUT, "reweighting keeps the design's whitened slope columns, which depend on the "
+                                       "weights: build a new Solver for a design with slopes");
+        require_device();
+        auto t0 = std::chrono::steady_clock::now();
+        auto h = std::make_unique<wx_solver>();
+        Solver &s = h->s;
+        s.design = b.design;
+        std::vector<double> w;
+        if (weights) {
+            validate_weights(s.design, weights, s.design.n_obs);
+            w.resize(s.design.n_obs);
+            permute_in(s.design, weights, w.data());
+        }
+        s.weighted = weights != nullptr;
+        if (weights) s.min_weight = *std::min_element(w.begin(), w.end());
+        if (prebuilt) {
+            if (prebuilt->p->n != s.design.n_dofs)
+                fail(WX_ERR_BUILD, "preconditioner dimension mismatch: expected " + std::to_string(s.design.n_dofs) +
+                                       ", got " + std::to_string(prebuilt->p->n));
+            s.precond = prebuilt->p;
+        } else {
+            wx_precond_config c;
+            if (cfg) c = *cfg;
+            else wx_precond_config_default(&c);
+            if (c.kind == WX_PRECOND_OFF) s.precond = nullptr;
+            else if (c.kind == WX_PRECOND_DIAGONAL) s.precond = build_diagonal(s.design, w);
+            else s.precond = build_additive(s.design, w, settings_from(c), s.warnings);
+        }
+        reweight_design(s, b, w);
+        s.uid = next_solver_uid();
+        s.w_internal = std::move(w);
+        s.time_setup = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+        *out = h.release();
+    });
+}
+
 wx_status wx_precond_build(const wx_design *design, const double *weights, const wx_precond_config *cfg,
                            wx_precond **out) {
     return guard([&] {

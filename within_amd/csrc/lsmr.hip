@@ -1351,6 +1351,86 @@ void upload_design(Solver &s, const std::vector<double> &weights_internal) {
 }
 
 namespace {
+void clone_buf(DevBuf &dst, const DevBuf &src) {
+    if (!src.p) {
+        dst.release();
+        return;
+    }
+    dst.alloc(src.bytes);
+    WX_HIP(hipMemcpy(dst.p, src.p, src.bytes, hipMemcpyDeviceToDevice));
+}
+} // namespace
+
+// The device design of `base` with new observation weights (internal order; empty:
+// unweighted): every index structure is copied on the device, only the weight arrays (and the
+// diagonal preconditioner) are formed anew.
+void reweight_design(Solver &s, const Solver &base, const std::vector<double> &w) {
+    const DevDesign &b = base.dev;
+    DevDesign &dv = s.dev;
+    dv.n = b.n;
+    dv.N = b.N;
+    clone_buf(dv.codes, b.codes);
+    clone_buf(dv.slopes, b.slopes);
+    clone_buf(dv.obs_perm, b.obs_perm);
+    clone_buf(dv.inv_perm, b.inv_perm);
+    clone_buf(dv.codes4, b.codes4);
+    dv.plain = b.plain;
+    dv.fuse_term = b.fuse_term;
+    dv.max_partials = b.max_partials;
+    dv.sw.release();
+    dv.w.release();
+    dv.sw_caller.release();
+    std::vector<double> sw(w.size());
+    for (size_t i = 0; i < w.size(); ++i) sw[i] = std::sqrt(w[i]);
+    if (!w.empty()) {
+        dv.sw.upload(sw.data(), sw.size());
+        dv.w.upload(w.data(), w.size());
+        if (!s.design.obs_perm.empty()) {
+            std::vector<double> swc(w.size());
+            for (uint64_t i = 0; i < w.size(); ++i) swc[s.design.obs_perm[i]] = sw[i];
+            dv.sw_caller.upload(swc.data(), swc.size());
+        }
+    }
+    dv.inv_diag.release();
+    if (s.precond && s.precond->kind == Precond::DIAGONAL)
+        dv.inv_diag.upload(s.precond->inv_diag.data(), s.precond->inv_diag.size());
+    dv.terms.clear();
+    dv.terms.resize(b.terms.size());
+    std::vector<uint32_t> perm;
+    for (size_t t = 0; t < b.terms.size(); ++t) {
+        const DevTerm &bt = b.terms[t];
+        DevTerm &dt = dv.terms[t];
+        dt.info = bt.info;
+        clone_buf(dt.perm, bt.perm);
+        clone_buf(dt.items, bt.items);
+        dt.n_items = bt.n_items;
+        clone_buf(dt.fixes, bt.fixes);
+        dt.n_fixes = bt.n_fixes;
+        dt.n_partials = bt.n_partials;
+        clone_buf(dt.gitems, bt.gitems);
+        dt.n_gitems = bt.n_gitems;
+        dt.stages.resize(bt.stages.size());
+        for (size_t q = 0; q < bt.stages.size(); ++q) {
+            clone_buf(dt.stages[q].items, bt.stages[q].items);
+            dt.stages[q].n_items = bt.stages[q].n_items;
+            dt.stages[q].src_base = bt.stages[q].src_base;
+            dt.stages[q].dst_base = bt.stages[q].dst_base;
+        }
+        if (bt.perm.p && !w.empty()) { // weights in this term's sorted order
+            perm.resize(b.n);
+            WX_HIP(hipMemcpy(perm.data(), bt.perm.p, b.n * sizeof(uint32_t), hipMemcpyDeviceToHost));
+            std::vector<double> wp(b.n), swp(b.n);
+            for (uint64_t q = 0; q < b.n; ++q) {
+                wp[q] = w[perm[q]];
+                swp[q] = sw[perm[q]];
+            }
+            dt.w_pos.upload(wp.data(), wp.size());
+            dt.sw_pos.upload(swp.data(), swp.size());
+        }
+    }
+}
+
+namespace {
 
 // Combines the split levels of one term's D' reduction.
 void launch_segfix(const DevTerm &dt, double *partials, double *out, const int32_t *active, uint32_t k,

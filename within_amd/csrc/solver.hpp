@@ -120,6 +120,10 @@ struct Solver {
 // Uploads the design (codes, slopes, weights, D' segment plans) to the current device.
 void upload_design(Solver &s, const std::vector<double> &weights_internal);
 
+// s.dev = base's device design with the observation weights w (internal order; empty:
+// unweighted), the index structures copied on the device (s.design and s.precond set).
+void reweight_design(Solver &s, const Solver &base, const std::vector<double> &w);
+
 uint64_t next_solver_uid();
 
 // Independent problems of one shape packed into one block-diagonal design, so that every
