@@ -27,10 +27,10 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
 
-# HBM bytes per k_gather_seg2 dispatch at the default workload (n = 5,000,000, 3 factors,
+# HBM bytes per k_gather_wave dispatch at the default workload (n = 5,000,000, 3 factors,
 # k = 16), measured with rocprofv3 FETCH_SIZE and WRITE_SIZE passes scaled by a 1 GiB copy of the
-# same run (profiles/k_gather_seg2_n5M_k16_hbm_traffic.txt); None for other workloads.
-GATHER_TRAFFIC_BYTES = {(5_000_000, 3, 16): 0.933753e9 + 0.673890e9}
+# same run (profiles/k_gather_wave_n5M_k16_hbm_traffic.txt); None for other workloads.
+GATHER_TRAFFIC_BYTES = {(5_000_000, 3, 16): 0.813631e9 + 0.674607e9}
 
 
 def log(*a):
@@ -452,8 +452,8 @@ def main():
     g_ms, _ = ph["gather"]
     a_ms, _ = ph["adjoint"]
     n_sorted = sorted_term_levels(codes)
-    # k_gather_seg2: the gather with the u update (B_D) plus the sorted term's D' output
-    kern = "k_gather_seg2 (D v + u update + D' of the sorted term, two columns per lane)"
+    # k_gather_wave: the gather with the u update (B_D) plus the sorted term's D' output
+    kern = "k_gather_wave (D v + u update + D' of the sorted term; a wavefront per item, two columns per lane)"
     bytes_per = b_d + 8 * k * n_sorted
     dur = g_ms / iters * 1e-3
     achieved = bytes_per / dur / 1e9

@@ -31,6 +31,8 @@ constexpr int BLK = (int)KERNEL_BLOCK;
 constexpr uint32_t SEG_CHUNK = 64;  // rows per D' work item before a level is split
 constexpr uint32_t FIX_CHUNK = 64;  // partials per combine item
 constexpr uint32_t MAX_FUSED_BLOCKS = 32768; // grid cap of the fused gather (one item per row group)
+constexpr int GATHER_WAVE_STEPS = 2;             // row steps per pass of k_gather_wave
+constexpr uint32_t GATHER_WAVE_BLOCKS = 8192;    // its grid cap (4 waves, each looping over items)
 
 // Per-column LSMR state (structure of arrays, each of length MAX_BATCH).
 struct ColState {
@@ -230,44 +232,72 @@ __global__ __launch_bounds__(BLK) void k_gather_seg(DesignArgs d, TermInfo ts, c
     if (threadIdx.x < kp) part[(uint64_t)blockIdx.x * kp + threadIdx.x] = tot;
 }
 
-// The same fused pass with two columns per lane (16-byte loads and stores, half the lanes per
-// row) for plain designs and an even number of columns: the scattered coefficient-row loads
-// are limited by the vector memory instruction rate, so moving twice the bytes per
-// instruction shortens the pass.  Per row and column the arithmetic is exactly that of
-// k_gather_seg.  A lane computes both of its columns when either is active (an inactive
-// column's u, norm partial and level sums are never read again).
 typedef double dbl2 __attribute__((ext_vector_type(2)));
 
-template <bool WEIGHTED, bool FIRST, int TT>
-__global__ __launch_bounds__(BLK) __attribute__((amdgpu_waves_per_eu(4, 8))) void k_gather_seg2(DesignArgs d, TermInfo ts, const uint4 *__restrict__ codes4, uint32_t fterm,
-                                                     const SegItem *__restrict__ items, uint32_t n_items,
-                                                     const double *__restrict__ sw, const double *__restrict__ v,
-                                                     const double *uin, double *u, double *__restrict__ atu,
-                                                     double *__restrict__ partials, ColState cs,
-                                                     double *__restrict__ part, uint32_t k, uint32_t kshift) {
+// Orders a wavefront's LDS accesses: what one lane wrote before is visible to the others after.
+__device__ inline void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// Sum of a lane's value over the wave's row groups (lanes jv, jv + kv, ...), in a fixed tree
+// order; every lane of the column pair gets the total.
+__device__ inline dbl2 sum_over_groups(dbl2 a, uint32_t kv) {
+    for (uint32_t off = kv; off < 64; off <<= 1) {
+        a.x += __shfl_xor(a.x, (int)off);
+        a.y += __shfl_xor(a.y, (int)off);
+    }
+    return a;
+}
+
+// The fused gather pass (k_gather_seg's arithmetic) for plain designs and an even number of
+// columns, two columns per lane (16-byte loads and stores), one wavefront per work item: the
+// wave's row groups take
+// consecutive rows (row group g of step s owns row base + s*G + g, G = 64 / lanes per row), so
+// each load instruction covers G adjacent rows.  Their code records, sqrt-weights and u rows are
+// contiguous, and rows that share a level of the sorted term -- or of a small factor, which the
+// observation order keeps together within a level of the sorted term -- share coefficient lines,
+// so an instruction touches few distinct lines instead of one per row group.
+// The sorted term's level sums: while a pass stays in the current level (most rows: big levels
+// are split into chunks of one level), each lane adds its own rows and the row groups' sums are
+// combined when the level ends; a pass that crosses levels goes through LDS, where the first row
+// group adds its rows in order.  The order is fixed by the work items, so the sums are
+// deterministic (they differ from k_segreduce's row-order sums by rounding only).
+template <bool WEIGHTED, bool FIRST, int TT, int S>
+__global__ __launch_bounds__(BLK) __attribute__((amdgpu_waves_per_eu(4, 8))) void k_gather_wave(
+    DesignArgs d, TermInfo ts, const uint4 *__restrict__ codes4, uint32_t fterm, const SegItem *__restrict__ items,
+    uint32_t n_items, const double *__restrict__ sw, const double *__restrict__ v, const double *uin, double *u,
+    double *__restrict__ atu, double *__restrict__ partials, ColState cs, double *__restrict__ part, uint32_t k,
+    uint32_t kshift) {
+    constexpr uint32_t WAVE = 64, NW = BLK / WAVE;
     __shared__ double sh[BLK];
+    __shared__ dbl2 cbuf[NW][S * WAVE];   // per wave: row contributions of a pass, slot = step * 64 + lane
+    __shared__ uint32_t lbuf[NW][S * 16]; // sorted-term level per row of the pass (G <= 16)
     const uint32_t kp = 1u << kshift, kv = kp >> 1; // lanes per row
-    const uint32_t jv = threadIdx.x & (kv - 1), j0 = 2 * jv;
+    const uint32_t lane = threadIdx.x & (WAVE - 1), wv = threadIdx.x / WAVE;
+    const uint32_t G = WAVE / kv;                   // rows per step
+    const uint32_t jv = lane & (kv - 1), g = lane / kv, j0 = 2 * jv;
     double sq0 = 0.0, sq1 = 0.0;
     const bool act = j0 < k && (cs.active[j0] || cs.active[j0 + 1]);
     if (act) {
         const double sc0 = cs.gscale[j0], sc1 = cs.gscale[j0 + 1];
-        constexpr int R = 4;
-        const uint32_t groups = (gridDim.x * BLK) / kv;
-        for (uint32_t it = (blockIdx.x * BLK + threadIdx.x) / kv; it < n_items; it += groups) {
+        const uint32_t waves = gridDim.x * NW;
+        for (uint32_t it = blockIdx.x * NW + wv; it < n_items; it += waves) {
             const SegItem item = items[it];
             uint32_t cur = item.level;
-            double adj0 = 0.0, adj1 = 0.0;
-            for (uint32_t p0 = item.begin; p0 < item.end; p0 += R) {
-                uint32_t rows[R]; // row indices fit 32 bits (n_obs < 2^32)
-                bool ok[R];
-                dbl2 uo[R];
-                double wq[R];
-                uint32_t lv[R], code[TT][R];
+            dbl2 adj = {0.0, 0.0}; // this lane's part of level cur's sum
+            for (uint32_t base = item.begin; base < item.end; base += S * G) {
+                uint32_t rows[S];
+                bool ok[S];
+                dbl2 uo[S];
+                double wq[S];
+                uint32_t lv[S], code[TT][S];
 #pragma unroll
-                for (int q = 0; q < R; ++q) {
-                    ok[q] = p0 + q < item.end;
-                    rows[q] = ok[q] ? p0 + q : p0; // clamped: loads stay unconditional
+                for (int q = 0; q < S; ++q) {
+                    const uint32_t r = base + q * G + g;
+                    ok[q] = r < item.end;
+                    rows[q] = ok[q] ? r : item.begin; // clamped: loads stay unconditional
                     uo[q] = __builtin_nontemporal_load((const dbl2 *)&uin[(uint64_t)rows[q] * k + j0]);
                     wq[q] = WEIGHTED ? sw[rows[q]] : 1.0;
                     const uint4 c4 = codes4[rows[q]];
@@ -276,15 +306,16 @@ __global__ __launch_bounds__(BLK) __attribute__((amdgpu_waves_per_eu(4, 8))) voi
                     for (int t = 0; t < TT; ++t) code[t][q] = cc[t];
                     lv[q] = fterm == 0 ? c4.x : fterm == 1 ? c4.y : fterm == 2 ? c4.z : c4.w;
                 }
-                dbl2 val[TT][R];
+                dbl2 val[TT][S];
 #pragma unroll
                 for (int t = 0; t < TT; ++t)
 #pragma unroll
-                    for (int q = 0; q < R; ++q)
+                    for (int q = 0; q < S; ++q)
                         val[t][q] = *(const dbl2 *)&v[(d.terms[t].offset + code[t][q]) * k + j0];
+                dbl2 cq[S];
+                bool cross = false;
 #pragma unroll
-                for (int q = 0; q < R; ++q) {
-                    if (!ok[q]) continue;
+                for (int q = 0; q < S; ++q) {
                     double a0 = 0.0, a1 = 0.0;
 #pragma unroll
                     for (int t = 0; t < TT; ++t) {
@@ -300,36 +331,66 @@ __global__ __launch_bounds__(BLK) __attribute__((amdgpu_waves_per_eu(4, 8))) voi
                     dbl2 nu;
                     nu.x = a0 + sc0 * b0;
                     nu.y = a1 + sc1 * b1;
-                    __builtin_nontemporal_store(nu, (dbl2 *)&u[(uint64_t)rows[q] * k + j0]);
-                    sq0 += nu.x * nu.x;
-                    sq1 += nu.y * nu.y;
-                    if (lv[q] != cur) {
-                        dbl2 av;
-                        av.x = adj0;
-                        av.y = adj1;
-                        *(dbl2 *)&atu[(ts.offset + cur) * k + j0] = av;
-                        const dbl2 zero = {0.0, 0.0};
-                        for (uint32_t l = cur + 1; l < lv[q]; ++l) *(dbl2 *)&atu[(ts.offset + l) * k + j0] = zero;
-                        adj0 = adj1 = 0.0;
-                        cur = lv[q];
+                    cq[q].x = ok[q] ? (WEIGHTED ? wq[q] * nu.x : nu.x) : 0.0;
+                    cq[q].y = ok[q] ? (WEIGHTED ? wq[q] * nu.y : nu.y) : 0.0;
+                    if (ok[q]) {
+                        __builtin_nontemporal_store(nu, (dbl2 *)&u[(uint64_t)rows[q] * k + j0]);
+                        sq0 += nu.x * nu.x;
+                        sq1 += nu.y * nu.y;
+                        cross |= lv[q] != cur;
                     }
-                    adj0 += WEIGHTED ? wq[q] * nu.x : nu.x;
-                    adj1 += WEIGHTED ? wq[q] * nu.y : nu.y;
                 }
+                if (!__any(cross)) {
+#pragma unroll
+                    for (int q = 0; q < S; ++q) {
+                        adj.x += cq[q].x;
+                        adj.y += cq[q].y;
+                    }
+                    continue;
+                }
+                // the pass crosses levels: the first row group carries level cur's sum so far
+                // and adds the pass's rows in order, writing each level that ends
+                adj = sum_over_groups(adj, kv);
+#pragma unroll
+                for (int q = 0; q < S; ++q) {
+                    cbuf[wv][q * WAVE + lane] = cq[q];
+                    if (jv == 0) lbuf[wv][q * G + g] = lv[q];
+                }
+                wave_sync();
+                if (g == 0) {
+                    const uint32_t nrow = min(S * G, item.end - base); // row i: slot i * kv + jv
+                    for (uint32_t i = 0; i < nrow; ++i) {
+                        const uint32_t l = lbuf[wv][i];
+                        const dbl2 c = cbuf[wv][i * kv + jv];
+                        if (l != cur) {
+                            *(dbl2 *)&atu[(ts.offset + cur) * k + j0] = adj;
+                            const dbl2 zero = {0.0, 0.0};
+                            for (uint32_t m = cur + 1; m < l; ++m) *(dbl2 *)&atu[(ts.offset + m) * k + j0] = zero;
+                            adj = zero;
+                            cur = l;
+                        }
+                        adj.x += c.x;
+                        adj.y += c.y;
+                    }
+                } else {
+                    adj.x = adj.y = 0.0;
+                }
+                cur = __shfl(cur, (int)jv); // the first row group's level, for every group
+                wave_sync(); // the next pass overwrites the buffers
             }
-            dbl2 av;
-            av.x = adj0;
-            av.y = adj1;
-            if (item.dest >= 0) {
-                *(dbl2 *)&atu[(ts.offset + cur) * k + j0] = av;
-                const dbl2 zero = {0.0, 0.0};
-                for (uint32_t l = cur + 1; l <= (uint32_t)item.dest; ++l) *(dbl2 *)&atu[(ts.offset + l) * k + j0] = zero;
-            } else if (item.dest != SEG_SKIP) {
-                *(dbl2 *)&partials[(uint64_t)(-item.dest - 1) * k + j0] = av;
+            adj = sum_over_groups(adj, kv);
+            if (g == 0) {
+                if (item.dest >= 0) {
+                    *(dbl2 *)&atu[(ts.offset + cur) * k + j0] = adj;
+                    const dbl2 zero = {0.0, 0.0};
+                    for (uint32_t m = cur + 1; m <= (uint32_t)item.dest; ++m)
+                        *(dbl2 *)&atu[(ts.offset + m) * k + j0] = zero;
+                } else if (item.dest != SEG_SKIP) {
+                    *(dbl2 *)&partials[(uint64_t)(-item.dest - 1) * k + j0] = adj;
+                }
             }
         }
     }
-    // column j0 sums over the threads with the same (tid & (kv-1)), then column j0 + 1
     const double t0 = block_reduce_cols(sq0, kv, sh);
     __syncthreads();
     if (threadIdx.x < kv) part[(uint64_t)blockIdx.x * kp + 2 * threadIdx.x] = t0;
@@ -1564,21 +1625,21 @@ uint32_t launch_gather_seg_t(const DevDesign &dv, const DesignArgs &da, const do
     const SegItem *items = use_g ? dt.gitems.as<SegItem>() : dt.items.as<SegItem>();
     const uint32_t *fcodes = da.codes + (uint64_t)dv.fuse_term * da.n;
     if (!pk && wide && dv.codes4.p && (k & 1) == 0) {
-        // two columns per lane: half the lanes per row, twice the row groups per workgroup
-        const uint32_t gpb = 2 * groups_per_block;
-        const uint32_t grid2 = std::max<uint32_t>(1, std::min<uint32_t>((n_it + gpb - 1) / gpb, MAX_FUSED_BLOCKS));
-#define WX_GS2(TT)                                                                                               \
-    hipLaunchKernelGGL((k_gather_seg2<W, F, TT>), dim3(grid2), dim3(BLK), 0, st, da, dt.info,                     \
+        // one wavefront per item, two columns per lane
+        const uint32_t wpb = BLK / 64;
+        const uint32_t gridw = std::max<uint32_t>(1, std::min<uint32_t>((n_it + wpb - 1) / wpb, GATHER_WAVE_BLOCKS));
+#define WX_GW(TT)                                                                                                \
+    hipLaunchKernelGGL((k_gather_wave<W, F, TT, GATHER_WAVE_STEPS>), dim3(gridw), dim3(BLK), 0, st, da, dt.info,  \
                        dv.codes4.as<uint4>(), (uint32_t)dv.fuse_term, items, n_it, sw, v, uin, u, atu, partials,      \
                        cs, part, k, kshift)
         switch (da.T) {
-        case 1: WX_GS2(1); break;
-        case 2: WX_GS2(2); break;
-        case 3: WX_GS2(3); break;
-        default: WX_GS2(4); break;
+        case 1: WX_GW(1); break;
+        case 2: WX_GW(2); break;
+        case 3: WX_GW(3); break;
+        default: WX_GW(4); break;
         }
-#undef WX_GS2
-        return grid2;
+#undef WX_GW
+        return gridw;
     }
 #define WX_GS(TT, PL)                                                                                            \
     if (wide)                                                                                                    \
