@@ -366,6 +366,30 @@ def test_worker_firm_panel_with_firm_slope(gpu):
 # windowed reorthogonalisation (local_size) on the device
 # ---------------------------------------------------------------------------
 
+@pytest.mark.parametrize("k", [8, 16, 64])
+def test_wide_batch_with_small_factor_matches_oracle(gpu, oracle, k):
+    """Wide even batches run the wavefront gather (4, 8 and 32 lanes per row: 16, 8 and 2 rows
+    per wave step) over a Zipf-sorted term with split and merged levels."""
+    rng = np.random.default_rng(80 + k)
+    n = 60_000
+    p = (np.arange(3000) + 1.0) ** -1.1
+    p /= p.sum()
+    codes = np.asfortranarray(np.column_stack([rng.choice(3000, n, p=p), rng.integers(0, 400, n),
+                                               rng.integers(0, 12, n)]).astype(np.uint32))
+    w = rng.uniform(0.5, 2.0, n)
+    Y = np.asfortranarray(rng.standard_normal((n, k)))
+    print(f"[wide batch] k={k}", flush=True)
+    r = wa.solve_batch(codes, Y, weights=w, options=wa.LsmrOptions(tol=1e-10, maxiter=5000),
+                       preconditioner=wa.PreconditionerConfig.Diagonal)
+    _, Dm, res = oracle.solve_batch(oracle.design(codes), Y, w, prec="diagonal", tol=1e-10, maxiter=5000, threads=8)
+    assert all(r.converged) and all(q.converged for q in res)
+    ex = exact_demean(sparse_design(codes), Y, w)
+    for j in range(k):
+        assert rel(r.demeaned[:, j], Dm[:, j]) <= 1e-8, j
+        assert rel(r.demeaned[:, j], ex[:, j]) <= 1e-8, j
+    assert max(abs(a - b.iterations) for a, b in zip(r.iterations, res)) <= 2
+
+
 @pytest.mark.parametrize("prec", ["off", "diagonal", "schwarz"])
 @pytest.mark.parametrize("local_size", [1, 5, 40])
 def test_local_size_matches_oracle(gpu, oracle, prec, local_size):
