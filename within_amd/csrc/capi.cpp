@@ -3,6 +3,8 @@
 
 #include <chrono>
 #include <cmath>
+#include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <new>
 
@@ -191,6 +193,16 @@ void wx_design_release(wx_design *d) {
     if (d && --d->refs == 0) delete d;
 }
 
+namespace {
+// WITHIN_AMD_VERBOSE: cumulative wall time of a Solver's set-up phases on stderr
+void setup_phase(const char *name, std::chrono::steady_clock::time_point t0) {
+    static const bool verbose = std::getenv("WITHIN_AMD_VERBOSE") != nullptr;
+    if (!verbose) return;
+    std::fprintf(stderr, "[within_amd] solver set-up: %s done at %.1f ms\n", name,
+                 std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count());
+}
+} // namespace
+
 wx_status wx_solver_new(const wx_design *design, const double *weights, const wx_precond_config *cfg,
                         const wx_precond *prebuilt, wx_solver **out) {
     return guard([&] {
@@ -202,6 +214,7 @@ wx_status wx_solver_new(const wx_design *design, const double *weights, const wx
         Solver &s = h->s;
         // crates/within/src/solver.rs:361-411
         s.design = build_design(design->n_obs, design->columns, design->structure, design->slopes, true);
+        setup_phase("design", t0);
         std::vector<double> w;
         if (weights) {
             validate_weights(s.design, weights, s.design.n_obs);
@@ -225,7 +238,9 @@ wx_status wx_solver_new(const wx_design *design, const double *weights, const wx
             else if (c.kind == WX_PRECOND_DIAGONAL) s.precond = build_diagonal(s.design, w);
             else s.precond = build_additive(s.design, w, settings_from(c), s.warnings);
         }
+        setup_phase("preconditioner", t0);
         upload_design(s, w);
+        setup_phase("device design", t0);
         s.uid = next_solver_uid();
         s.w_internal = std::move(w);
         s.time_setup = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();

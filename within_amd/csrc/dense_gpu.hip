@@ -1,19 +1,23 @@
-// dense_gpu.cpp -- setup-time pseudo-inverse of large dense kept-block Laplacians on the GPU.
+// dense_gpu.hip -- setup-time reduced blocks of large dense kept blocks on the GPU.
 //
-// For a connected Laplacian S (n x n) the kept-block operator is S^+ = P G P with G the
+// A component's kept-block Schur complement S = D_k - C' D_e^{-1} C is assembled on the device
+// (one workgroup per kept row, the row in LDS; the eliminated rows adjacent to it are added one
+// after another, so every entry sums its terms in adjacency order), then its pseudo-inverse is
+// formed there too.  For a connected Laplacian S (n x n) the operator is S^+ = P G P with G the
 // inverse of the grounded minor S[0:n-1, 0:n-1] (padded with a zero row/column) and
-// P = I - 11'/n.  The minor is factored by a blocked right-looking Cholesky: the small
-// diagonal blocks on the host, the panel solves and trailing updates with rocBLAS level-3
-// routines (dtrsm, dsyrk); then L^{-1} = dtrsm(L, I) and G = L^{-T} L^{-1} = dsyrk.
-// This runs once per preconditioner build, off the solve path.
+// P = I - 11'/n; for a definite block (grounded or signed form) it is S^{-1}.  The inverse comes
+// from rocSOLVER's Cholesky factorisation and inverse (dpotrf, dpotri).  This runs once per
+// preconditioner build, off the solve path.
 #include <hip/hip_runtime.h>
 #include <rocblas/rocblas.h>
+#include <rocsolver/rocsolver.h>
 
 #include <cmath>
 #include <cstring>
 #include <vector>
 
 #include "device.hpp"
+#include "schwarz.hpp"
 
 namespace wx {
 
@@ -33,28 +37,116 @@ struct Handle {
     }
 };
 
-// host Cholesky of a small column-major block (lower), in place; false on a non-positive pivot
-bool potrf_small(double *a, int nb) {
-    for (int k = 0; k < nb; ++k) {
-        double d = a[k + (size_t)k * nb];
-        for (int q = 0; q < k; ++q) d -= a[k + (size_t)q * nb] * a[k + (size_t)q * nb];
-        if (!(d > 0.0)) return false;
-        d = std::sqrt(d);
-        a[k + (size_t)k * nb] = d;
-        for (int i = k + 1; i < nb; ++i) {
-            double s = a[i + (size_t)k * nb];
-            for (int q = 0; q < k; ++q) s -= a[i + (size_t)q * nb] * a[k + (size_t)q * nb];
-            a[i + (size_t)k * nb] = s / d;
+constexpr int DB = 256;             // workgroup size of the set-up kernels
+constexpr uint32_t ROW_CHUNK = 8192; // kept columns per assembly workgroup (64 KB of LDS)
+
+// Off-diagonal part of kept row i, columns [c0, c0 + chunk): -sum over the eliminated rows e
+// adjacent to i, in adjacency order, of (w_ie * C_ej) / D_e.  Entries of one eliminated row
+// have distinct columns, so the threads of a workgroup update disjoint slots between barriers.
+__global__ __launch_bounds__(DB) void k_schur_rows(uint32_t nk, const uint64_t *__restrict__ cptr,
+                                                   const uint32_t *__restrict__ cidx, const double *__restrict__ cval,
+                                                   const double *__restrict__ diag, const uint64_t *__restrict__ tp,
+                                                   const uint32_t *__restrict__ te, const double *__restrict__ tv,
+                                                   double *__restrict__ S) {
+    extern __shared__ double row[];
+    const uint32_t i = blockIdx.x, c0 = blockIdx.y * ROW_CHUNK, c1 = min(nk, c0 + ROW_CHUNK), w = c1 - c0;
+    for (uint32_t j = threadIdx.x; j < w; j += DB) row[j] = 0.0;
+    __syncthreads();
+    for (uint64_t q = tp[i]; q < tp[i + 1]; ++q) {
+        const uint32_t e = te[q];
+        const double wi = tv[q], inv = 1.0 / diag[e];
+        for (uint64_t p = cptr[e] + threadIdx.x; p < cptr[e + 1]; p += DB) {
+            const uint32_t j = cidx[p];
+            if (j >= c0 && j < c1) row[j - c0] -= (wi * cval[p]) * inv;
         }
-        for (int i = 0; i < k; ++i) a[i + (size_t)k * nb] = 0.0; // clear the upper part
+        __syncthreads();
     }
-    return true;
+    for (uint32_t j = threadIdx.x; j < w; j += DB) S[(uint64_t)i * nk + c0 + j] = row[j];
 }
 
-__global__ void k_set_identity(double *x, int64_t m) {
-    const int64_t idx = (int64_t)blockIdx.x * 256 + threadIdx.x;
-    if (idx >= m * m) return;
-    x[idx] = (idx % m == idx / m) ? 1.0 : 0.0;
+// Diagonal of kept row i: minus the off-diagonal row sum (a fixed-order tree; the magnitude
+// of the signed form instead), plus the surplus carried to the kept block when grounded.
+__global__ __launch_bounds__(DB) void k_schur_diag(uint32_t ne, uint32_t nk, const double *__restrict__ diag,
+                                                   const double *__restrict__ surplus,
+                                                   const double *__restrict__ rowabs, const uint64_t *__restrict__ tp,
+                                                   const uint32_t *__restrict__ te, const double *__restrict__ tv,
+                                                   int signed_form, double *__restrict__ S) {
+    __shared__ double sh[DB];
+    const uint32_t i = blockIdx.x;
+    double a = 0.0;
+    for (uint32_t j = threadIdx.x; j < nk; j += DB)
+        if (j != i) a += S[(uint64_t)i * nk + j];
+    sh[threadIdx.x] = a;
+    __syncthreads();
+    for (int s = DB / 2; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s) sh[threadIdx.x] += sh[threadIdx.x + s];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        double sur = surplus ? surplus[ne + i] : 0.0, mag = 0.0;
+        for (uint64_t q = tp[i]; q < tp[i + 1]; ++q) {
+            const uint32_t e = te[q];
+            const double wi = tv[q], inv = 1.0 / diag[e];
+            if (surplus) sur += fabs(wi) * (inv * surplus[e]);
+            if (signed_form) mag += fabs(wi) * (rowabs[e] - fabs(wi)) * inv;
+        }
+        S[(uint64_t)i * nk + i] = (signed_form ? mag : -sh[0]) + sur;
+    }
+}
+
+// Full symmetric G from dpotri's lower triangle (column-major, leading dimension ld) on the
+// leading m x m block; row and column m (the floating gauge's dropped vertex) set to zero when
+// pad, i.e. n = m + 1.
+__global__ __launch_bounds__(DB) void k_symmetrise(double *__restrict__ G, uint32_t m, uint32_t n, uint64_t ld) {
+    const uint64_t idx = (uint64_t)blockIdx.x * DB + threadIdx.x;
+    if (idx >= (uint64_t)n * n) return;
+    const uint32_t r = (uint32_t)(idx % n), c = (uint32_t)(idx / n); // element (r, c) at r + c * ld
+    if (r >= m || c >= m) G[r + c * ld] = 0.0;
+    else if (r < c) G[r + c * ld] = G[c + r * ld];
+}
+
+// rmean[c] = (sum_r G[r, c]) / n, a fixed-order tree per column
+__global__ __launch_bounds__(DB) void k_col_means(const double *__restrict__ G, uint32_t n, uint64_t ld,
+                                                  double *__restrict__ rmean) {
+    __shared__ double sh[DB];
+    const uint32_t c = blockIdx.x;
+    double a = 0.0;
+    for (uint32_t r = threadIdx.x; r < n; r += DB) a += G[r + c * ld];
+    sh[threadIdx.x] = a;
+    __syncthreads();
+    for (int s = DB / 2; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s) sh[threadIdx.x] += sh[threadIdx.x + s];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) rmean[c] = sh[0] / n;
+}
+
+// total = mean of the column means (one workgroup, fixed order); written to rmean[n]
+__global__ __launch_bounds__(DB) void k_total_mean(double *__restrict__ rmean, uint32_t n) {
+    __shared__ double sh[DB];
+    double a = 0.0;
+    for (uint32_t c = threadIdx.x; c < n; c += DB) a += rmean[c];
+    sh[threadIdx.x] = a;
+    __syncthreads();
+    for (int s = DB / 2; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s) sh[threadIdx.x] += sh[threadIdx.x + s];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) rmean[n] = sh[0] / n;
+}
+
+// S^+ = P G P: G[r, c] - mean_r - mean_c + total
+__global__ __launch_bounds__(DB) void k_project(double *__restrict__ G, uint32_t n, uint64_t ld,
+                                                const double *__restrict__ rmean) {
+    const uint64_t idx = (uint64_t)blockIdx.x * DB + threadIdx.x;
+    if (idx >= (uint64_t)n * n) return;
+    const uint32_t r = (uint32_t)(idx % n), c = (uint32_t)(idx / n);
+    G[r + c * ld] = G[r + c * ld] - rmean[r] - rmean[c] + rmean[n];
+}
+
+template <typename T> void upload_vec(DevBuf &b, const T *p, size_t n) {
+    b.alloc(std::max<size_t>(n, 1) * sizeof(T));
+    if (n) WX_HIP(hipMemcpy(b.p, p, n * sizeof(T), hipMemcpyHostToDevice));
 }
 } // namespace
 
@@ -63,74 +155,53 @@ bool gpu_available() {
     return hipGetDeviceCount(&n) == hipSuccess && n > 0;
 }
 
-// S: n x n row-major symmetric Laplacian (connected); overwritten with S^+.
-bool dense_laplacian_pinv_gpu(std::vector<double> &S, uint32_t n, bool floating) {
-    if (floating && n <= 1) {
-        std::fill(S.begin(), S.end(), 0.0);
-        return true;
-    }
-    const int64_t m = floating ? (int64_t)n - 1 : (int64_t)n;
-    Handle H;
-    DevBuf A, X;
-    A.alloc((size_t)m * m * 8);
-    X.alloc((size_t)m * m * 8);
-    // the minor is symmetric: row-major == column-major
-    WX_HIP(hipMemcpy2D(A.p, m * 8, S.data(), (size_t)n * 8, m * 8, m, hipMemcpyHostToDevice));
-    double *a = A.as<double>();
-    const int64_t b = 256;
-    std::vector<double> blk(b * b);
-    const double one = 1.0, mone = -1.0, zero = 0.0;
-    for (int64_t p = 0; p < m; p += b) {
-        const int64_t nb = std::min<int64_t>(b, m - p);
-        double *app = a + p + p * m;
-        WX_HIP(hipMemcpy2D(blk.data(), nb * 8, app, m * 8, nb * 8, nb, hipMemcpyDeviceToHost));
-        if (!potrf_small(blk.data(), (int)nb)) return false;
-        WX_HIP(hipMemcpy2D(app, m * 8, blk.data(), nb * 8, nb * 8, nb, hipMemcpyHostToDevice));
-        const int64_t rest = m - p - nb;
-        if (rest > 0) {
-            double *a21 = a + (p + nb) + p * m, *a22 = a + (p + nb) + (p + nb) * m;
-            WX_RB(rocblas_dtrsm(H.h, rocblas_side_right, rocblas_fill_lower, rocblas_operation_transpose,
-                                rocblas_diagonal_non_unit, (rocblas_int)rest, (rocblas_int)nb, &one, app, (rocblas_int)m,
-                                a21, (rocblas_int)m));
-            WX_RB(rocblas_dsyrk(H.h, rocblas_fill_lower, rocblas_operation_none, (rocblas_int)rest, (rocblas_int)nb,
-                                &mone, a21, (rocblas_int)m, &one, a22, (rocblas_int)m));
-        }
-    }
-    // X = L^{-1}
-    hipLaunchKernelGGL(k_set_identity, dim3((unsigned)((m * m + 255) / 256)), dim3(256), 0, 0, X.as<double>(), m);
+bool dense_schur_pinv_gpu(const DenseSchurArgs &a, bool floating, std::vector<double> &out) {
+    const uint32_t n = a.nk;
+    out.assign((size_t)n * n, 0.0);
+    if (floating && n <= 1) return true;
+    DevBuf cptr, cidx, cval, diag, sur, rabs, tp, te, tv, S, mean, info;
+    upload_vec(cptr, a.cptr, (size_t)a.ne + 1);
+    upload_vec(cidx, a.cidx, a.nnz);
+    upload_vec(cval, a.cval, a.nnz);
+    upload_vec(diag, a.diag, (size_t)a.ne + a.nk);
+    if (a.surplus) upload_vec(sur, a.surplus, (size_t)a.ne + a.nk);
+    if (a.rowabs) upload_vec(rabs, a.rowabs, a.ne);
+    upload_vec(tp, a.tp, (size_t)n + 1);
+    upload_vec(te, a.te, a.nnz);
+    upload_vec(tv, a.tv, a.nnz);
+    S.alloc((size_t)n * n * 8);
+    double *s = S.as<double>();
+    const uint32_t chunks = (n + ROW_CHUNK - 1) / ROW_CHUNK;
+    const size_t lds = (size_t)std::min(n, ROW_CHUNK) * 8;
+    hipLaunchKernelGGL(k_schur_rows, dim3(n, chunks), dim3(DB), lds, 0, n, cptr.as<uint64_t>(), cidx.as<uint32_t>(),
+                       cval.as<double>(), diag.as<double>(), tp.as<uint64_t>(), te.as<uint32_t>(), tv.as<double>(), s);
     WX_HIP(hipGetLastError());
-    WX_RB(rocblas_dtrsm(H.h, rocblas_side_left, rocblas_fill_lower, rocblas_operation_none, rocblas_diagonal_non_unit,
-                        (rocblas_int)m, (rocblas_int)m, &one, a, (rocblas_int)m, X.as<double>(), (rocblas_int)m));
-    // G = X' X (lower triangle) into A
-    WX_RB(rocblas_dsyrk(H.h, rocblas_fill_lower, rocblas_operation_transpose, (rocblas_int)m, (rocblas_int)m, &one,
-                        X.as<double>(), (rocblas_int)m, &zero, a, (rocblas_int)m));
-    std::vector<double> G((size_t)m * m);
-    WX_HIP(hipMemcpy(G.data(), a, (size_t)m * m * 8, hipMemcpyDeviceToHost));
-    if (!floating) { // positive definite block: the inverse itself, symmetrised
-        for (int64_t i = 0; i < m; ++i)
-            for (int64_t j = 0; j < m; ++j) S[(size_t)i * n + j] = i >= j ? G[i + j * m] : G[j + i * m];
-        return true;
+    hipLaunchKernelGGL(k_schur_diag, dim3(n), dim3(DB), 0, 0, a.ne, n, diag.as<double>(),
+                       a.surplus ? sur.as<double>() : nullptr, a.rowabs ? rabs.as<double>() : nullptr,
+                       tp.as<uint64_t>(), te.as<uint32_t>(), tv.as<double>(), a.signed_form ? 1 : 0, s);
+    WX_HIP(hipGetLastError());
+    // Cholesky and inverse of the (grounded) minor in place: row-major == column-major
+    const uint32_t m = floating ? n - 1 : n;
+    Handle H;
+    info.alloc(sizeof(rocblas_int));
+    rocblas_int hinfo = 0;
+    WX_RB(rocsolver_dpotrf(H.h, rocblas_fill_lower, (rocblas_int)m, s, (rocblas_int)n, info.as<rocblas_int>()));
+    WX_HIP(hipMemcpy(&hinfo, info.p, sizeof(hinfo), hipMemcpyDeviceToHost));
+    if (hinfo != 0) return false;
+    WX_RB(rocsolver_dpotri(H.h, rocblas_fill_lower, (rocblas_int)m, s, (rocblas_int)n, info.as<rocblas_int>()));
+    WX_HIP(hipMemcpy(&hinfo, info.p, sizeof(hinfo), hipMemcpyDeviceToHost));
+    if (hinfo != 0) return false;
+    const uint64_t nn = (uint64_t)n * n;
+    const uint32_t eg = (uint32_t)((nn + DB - 1) / DB);
+    hipLaunchKernelGGL(k_symmetrise, dim3(eg), dim3(DB), 0, 0, s, m, n, (uint64_t)n);
+    if (floating) {
+        mean.alloc(((size_t)n + 1) * 8);
+        hipLaunchKernelGGL(k_col_means, dim3(n), dim3(DB), 0, 0, s, n, (uint64_t)n, mean.as<double>());
+        hipLaunchKernelGGL(k_total_mean, dim3(1), dim3(DB), 0, 0, mean.as<double>(), n);
+        hipLaunchKernelGGL(k_project, dim3(eg), dim3(DB), 0, 0, s, n, (uint64_t)n, mean.as<double>());
     }
-    // symmetrise (column-major lower: G[i + j*m], i >= j), pad and project with P on both sides
-    std::vector<double> rowmean(n, 0.0);
-    double total = 0.0;
-    for (int64_t j = 0; j < m; ++j)
-        for (int64_t i = j; i < m; ++i) {
-            const double g = G[i + j * m];
-            rowmean[i] += g;
-            if (i != j) rowmean[j] += g;
-        }
-    for (uint32_t i = 0; i < n; ++i) {
-        total += rowmean[i];
-        rowmean[i] /= n;
-    }
-    total /= (double)n * n;
-    for (int64_t i = 0; i < (int64_t)n; ++i)
-        for (int64_t j = 0; j < (int64_t)n; ++j) {
-            double g = 0.0;
-            if (i < m && j < m) g = i >= j ? G[i + j * m] : G[j + i * m];
-            S[(size_t)i * n + j] = g - rowmean[i] - rowmean[j] + total;
-        }
+    WX_HIP(hipGetLastError());
+    WX_HIP(hipMemcpy(out.data(), s, nn * 8, hipMemcpyDeviceToHost));
     return true;
 }
 

@@ -13,12 +13,15 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <numeric>
 #include <queue>
 #include <string>
 #include <tuple>
 #ifdef _OPENMP
+#include <chrono>
+
 #include <omp.h>
 #endif
 
@@ -27,8 +30,6 @@
 namespace wx {
 
 // dense_gpu.hip
-bool gpu_available();
-bool dense_laplacian_pinv_gpu(std::vector<double> &S, uint32_t n, bool floating);
 
 std::shared_ptr<Precond> build_diagonal(const Design &d, const std::vector<double> &w) {
     // crates/within/src/operator/schwarz.rs:185-229
@@ -809,6 +810,16 @@ Component make_cover(const Component &cm) {
 
 std::shared_ptr<Precond> build_additive(const Design &d, const std::vector<double> &w, const LocalSolverSettings &cfg,
                                         std::vector<std::string> &warnings) {
+    // WITHIN_AMD_VERBOSE: per-phase wall times of the build on stderr
+    const bool verbose = std::getenv("WITHIN_AMD_VERBOSE") != nullptr;
+    auto t_last = std::chrono::steady_clock::now();
+    auto phase = [&](const char *name) {
+        if (!verbose) return;
+        const auto now = std::chrono::steady_clock::now();
+        std::fprintf(stderr, "[within_amd] preconditioner build: %s %.1f ms\n", name,
+                     std::chrono::duration<double, std::milli>(now - t_last).count());
+        t_last = now;
+    };
     // channels and cross-term pairs (factor_pairs.rs:36-56)
     std::vector<Channel> channels;
     for (uint32_t t = 0; t < d.terms.size(); ++t)
@@ -827,6 +838,7 @@ std::shared_ptr<Precond> build_additive(const Design &d, const std::vector<doubl
     for (int64_t p = 0; p < (int64_t)pairs.size(); ++p)
         build_pair_components(d, w, pairs[p].first, pairs[p].second, active, cfg, per_pair[p], pair_warn[p],
                               pair_err[p]);
+    phase("cross-tabs and components");
     for (size_t p = 0; p < pairs.size(); ++p) {
         if (!pair_err[p].empty()) fail(WX_ERR_BUILD, pair_err[p]);
         warnings.insert(warnings.end(), pair_warn[p].begin(), pair_warn[p].end());
@@ -925,15 +937,27 @@ std::shared_ptr<Precond> build_additive(const Design &d, const std::vector<doubl
             }
         }
     }
-    // large dense kept blocks: the reduced block accumulated row-parallel on the host, then its
-    // inverse on the GPU
+    phase("local factorisations");
+    // large dense kept blocks: the reduced block and its pseudo-inverse formed on the GPU
     for (uint64_t c = 0; c < nc; ++c) {
         if (!big[c] || failed[c]) continue;
-        std::vector<double> Sm;
-        dense_schur(comps[c], Sm, true);
-        if (!dense_laplacian_pinv_gpu(Sm, comps[c].nk, S->gauge[c] == 0)) failed[c] = 2;
-        dense_mats[c] = std::move(Sm);
+        const Component &cm = comps[c];
+        const Adjacency adj(cm);
+        std::vector<double> rowabs;
+        if (cm.signed_form) {
+            rowabs.assign(cm.ne, 0.0);
+            for (uint32_t e = 0; e < cm.ne; ++e)
+                for (uint64_t p = cm.cptr[e]; p < cm.cptr[e + 1]; ++p) rowabs[e] += std::fabs(cm.cval[p]);
+        }
+        const DenseSchurArgs a{cm.ne,          cm.nk,          cm.cptr.data(),
+                               cm.cidx.data(), cm.cval.data(), (uint64_t)cm.cidx.size(),
+                               cm.diag.data(), cm.grounded ? cm.surplus.data() : nullptr,
+                               cm.signed_form ? rowabs.data() : nullptr,
+                               adj.tp.data(),  adj.te.data(),  adj.tv.data(),
+                               cm.signed_form};
+        if (!dense_schur_pinv_gpu(a, S->gauge[c] == 0, dense_mats[c])) failed[c] = 2;
     }
+    phase("large dense blocks");
     for (uint64_t c = 0; c < nc; ++c) {
         if (failed[c] == 1) fail(WX_ERR_BUILD, "zero diagonal in preconditioner at index " + std::to_string(c));
         if (failed[c] == 2) fail(WX_ERR_BUILD, "local solver build failed: Schur complement is not positive definite");
@@ -1027,6 +1051,7 @@ std::shared_ptr<Precond> build_additive(const Design &d, const std::vector<doubl
     }
     S->c_ptr[ns] = cpos;
     pre->schwarz = S;
+    phase("assembly");
     return pre;
 }
 

@@ -82,6 +82,28 @@ std::unique_ptr<SchwarzRun> make_schwarz_run(const Precond &p, int32_t precision
 std::shared_ptr<Precond> build_additive(const Design &d, const std::vector<double> &weights,
                                         const LocalSolverSettings &cfg, std::vector<std::string> &warnings);
 
+// A component's kept-block reduction on the GPU (set-up time): S = D_k - C' D_e^{-1} C with its
+// diagonal formed as the host assembly forms it (Laplacian row sums, or the signed form's
+// magnitudes, plus the grounded surplus), then S^+ (floating Laplacian: P G P over the grounded
+// minor's inverse G) or S^{-1}, returned row-major n_kept x n_kept.  False when the Cholesky
+// factorisation meets a non-positive pivot.
+struct DenseSchurArgs {
+    uint32_t ne, nk;
+    const uint64_t *cptr; // C: ne x nk CSR, columns ascending per row
+    const uint32_t *cidx;
+    const double *cval;
+    uint64_t nnz;
+    const double *diag;    // ne + nk
+    const double *surplus; // ne + nk, or null (not grounded)
+    const double *rowabs;  // ne row sums of |C| (signed form), or null
+    const uint64_t *tp;    // C' CSR: kept row k's eliminated neighbours te/tv[tp[k] .. tp[k+1])
+    const uint32_t *te;
+    const double *tv;
+    bool signed_form;
+};
+bool gpu_available();
+bool dense_schur_pinv_gpu(const DenseSchurArgs &a, bool floating, std::vector<double> &out);
+
 // Diagonal preconditioner (operator/schwarz.rs:185-229).
 std::shared_ptr<Precond> build_diagonal(const Design &d, const std::vector<double> &weights);
 
