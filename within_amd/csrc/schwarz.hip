@@ -29,6 +29,7 @@ namespace {
 template <typename T> inline T *dp(const DevBuf &b) { return static_cast<T *>(b.p); }
 
 constexpr int BLK = 256;
+constexpr int SP_BLK = 1024; // threads of the per-component sparse triangular solve (one workgroup per component)
 constexpr uint32_t ROW_CHUNK = 256;   // CSR entries per work item before a row is split
 constexpr uint32_t MEAN_CHUNK = 64;   // slots per partial-sum chunk
 constexpr uint32_t DENSE_TILED_MIN = 256; // kept blocks at least this large use the tiled product
@@ -462,10 +463,8 @@ __global__ __launch_bounds__(BLK) void sk_dense(uint32_t n_rows, const uint64_t 
     }
 }
 
-// Sparse kept-block solve, one workgroup per component: forward L y = red - mu2 level by
-// level, y /= d (a zero pivot -- the floating gauge -- gives 0), backward L' x = y level by
-// level.  Rows within a level are independent; __syncthreads() orders the levels.  Row and
-// column ids are global slot ids; y and x live in xs (in place).
+// Right-hand side of a forward-substitution row: the reduced rhs minus the component mean, or
+// for a Gremban cover's auxiliary slot the antisymmetric embed of its kept slot's.
 template <typename CT>
 __device__ inline double sparse_rhs(const SpMeta &m, uint32_t row, const double *red, const uint32_t *aux_src,
                                     uint64_t ns, double mu, uint32_t k, uint32_t j) {
@@ -519,7 +518,7 @@ __device__ inline void sparse_level(const SpMeta &m, uint64_t b, uint64_t el, ui
                                     const uint32_t *aux_src, uint64_t ns, double mu, double *xs, double *sh,
                                     uint32_t k, uint32_t kshift) {
     const uint32_t kp = 1u << kshift, j = threadIdx.x & (kp - 1);
-    const uint32_t grp = threadIdx.x >> kshift, ngrp = BLK >> kshift;
+    const uint32_t grp = threadIdx.x >> kshift, ngrp = SP_BLK >> kshift;
     const bool on = j < k;
     if (on)
         for (uint64_t q = b + grp; q < el; q += ngrp) {
@@ -536,7 +535,7 @@ __device__ inline void sparse_level(const SpMeta &m, uint64_t b, uint64_t el, ui
         if (on) part = dot_entries<CT, false>(0.0, ptr[lr] + grp, ptr[lr + 1], ngrp, idx, val, xs, k, j);
         sh[threadIdx.x] = part;
         __syncthreads();
-        for (uint32_t s = BLK / 2; s >= kp && s > 0; s >>= 1) {
+        for (uint32_t s = SP_BLK / 2; s >= kp && s > 0; s >>= 1) {
             if (threadIdx.x < s) sh[threadIdx.x] += sh[threadIdx.x + s];
             __syncthreads();
         }
@@ -555,7 +554,7 @@ __device__ inline void sparse_level(const SpMeta &m, uint64_t b, uint64_t el, ui
 // runs on auxiliary slots: its right-hand side is the embed [b, -b] and the kept slots
 // receive (y+ - y-) / 2 at the end.
 template <typename CT>
-__global__ __launch_bounds__(BLK) void sk_sparse(const SpMeta *__restrict__ meta, const uint64_t *__restrict__ lvl_f_ptr,
+__global__ __launch_bounds__(SP_BLK) void sk_sparse(const SpMeta *__restrict__ meta, const uint64_t *__restrict__ lvl_f_ptr,
                                                  const uint64_t *__restrict__ lvl_f_long,
                                                  const uint32_t *__restrict__ lvl_f_rows,
                                                  const uint64_t *__restrict__ lvl_b_ptr,
@@ -569,7 +568,7 @@ __global__ __launch_bounds__(BLK) void sk_sparse(const SpMeta *__restrict__ meta
                                                  uint64_t ns, double *xs, uint32_t k,
                                                  uint32_t kshift, const int32_t *__restrict__ gate, uint32_t phase) {
     if (gate_closed(gate, k)) return;
-    __shared__ double sh[BLK];
+    __shared__ double sh[SP_BLK];
     const uint32_t kp = 1u << kshift, j = threadIdx.x & (kp - 1);
     const SpMeta m = meta[blockIdx.x];
     const bool on = j < k;
@@ -1157,7 +1156,7 @@ struct SchwarzRunImpl : SchwarzRun {
                                    dp<uint32_t>(d.aux_src), d.ns, xsp, k, kshift, gate);
             }
             for (uint32_t phase = 0; phase < 2; ++phase) {
-                hipLaunchKernelGGL((sk_sparse<CT>), dim3(d.n_sparse), dim3(BLK), 0, st, dp<SpMeta>(d.sp_meta),
+                hipLaunchKernelGGL((sk_sparse<CT>), dim3(d.n_sparse), dim3(SP_BLK), 0, st, dp<SpMeta>(d.sp_meta),
                                    dp<uint64_t>(d.sp_lvl_f_ptr), dp<uint64_t>(d.sp_lvl_f_long),
                                    dp<uint32_t>(d.sp_lvl_f_rows), dp<uint64_t>(d.sp_lvl_b_ptr),
                                    dp<uint64_t>(d.sp_lvl_b_long), dp<uint32_t>(d.sp_lvl_b_rows),
