@@ -285,6 +285,38 @@ def test_sparse_factor_apply_matches_oracle(gpu, oracle, case):
         np.testing.assert_allclose(r.demeaned, SMALL[f"{case}__demeaned"], atol=1e-8)
 
 
+def test_sparse_factor_dense_tail_matches_oracle(gpu, oracle, capfd, monkeypatch):
+    """A worker-firm panel (Zipf firms) eliminates the hub firms last, in a chain of one-row
+    levels; the device solves that tail as one dense product M_T = (I+L_TT')^-1 D_T^+ (I+L_TT)^-1.
+    The apply must match the oracle's level-by-level restatement of the same factor."""
+    monkeypatch.setenv("WITHIN_AMD_VERBOSE", "1")
+    rng = np.random.default_rng(31)
+    nw, T, nf = 20_000, 5, 3_000
+    p = (np.arange(nf) + 1.0) ** -1.2
+    p /= p.sum()
+    firm = np.empty((nw, T), np.int64)
+    firm[:, 0] = rng.choice(nf, nw, p=p)
+    for t in range(1, T):
+        move = rng.uniform(size=nw) < 0.05
+        firm[:, t] = np.where(move, rng.integers(0, nf, nw), firm[:, t - 1])
+    codes = np.asfortranarray(np.column_stack([np.repeat(np.arange(nw), T), firm.ravel(),
+                                               np.tile(np.arange(T), nw)]).astype(np.uint32))
+    cfg = wa.config.AdditiveSchwarz(wa.config.LocalSolverConfig(dense_threshold=0))
+    s = wa.Solver(codes, preconditioner=cfg)
+    pc = s.preconditioner
+    x = rng.standard_normal(s.n_dofs)
+    got = pc.apply(x)
+    assert "dense tail" in capfd.readouterr().err, "the factor has no dense tail: the test does not cover it"
+    sch = oracle_schwarz_from_bytes(oracle, pc.to_bytes())
+    np.testing.assert_allclose(got, sch.apply(x), rtol=1e-10, atol=1e-11)
+    y = rng.standard_normal((codes.shape[0], 2))
+    ex = exact_demean(sparse_design(codes), y)
+    for precision in ["fp64", "mixed"]:
+        r = s.solve_batch(np.asfortranarray(y), wa.LsmrOptions(tol=1e-10, precision=precision))
+        assert all(r.converged)
+        assert rel(r.demeaned, ex) <= 1e-8
+
+
 # ---------------------------------------------------------------------------
 # additive Schwarz over slope (signed) channel pairs
 # ---------------------------------------------------------------------------

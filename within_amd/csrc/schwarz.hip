@@ -36,6 +36,10 @@ constexpr uint32_t DENSE_TILED_MIN = 256; // kept blocks at least this large use
 constexpr uint64_t DENSE_TILE_TARGET = 8192; // (row tile, column segment) pairs wanted in flight
 constexpr uint64_t LONG_ROW = 64;         // factor rows with more entries are summed by a whole workgroup
 constexpr uint64_t BIG_LEVEL = 2048;      // factor levels with at least this many rows use grid-wide launches
+constexpr uint32_t TAIL_LEVEL_ROWS = 32;  // factor levels this small at the end of the schedule ...
+constexpr uint32_t TAIL_MAX_ROWS = 512;   // ... up to this many rows ...
+constexpr uint32_t TAIL_MIN_LEVELS = 8;   // ... and at least this many levels form the dense tail
+constexpr uint32_t TAIL_CHUNK = 32;       // tail-row entries per partial sum
 
 struct RowItem {
     uint32_t row;   // slot id of the output row
@@ -84,6 +88,8 @@ struct SchwarzDev {
     DevBuf diag_rows;                    // every factor row (GridRow), for the pivots
     uint64_t n_diag_rows = 0;
     DevBuf l_ptr, l_idx, l_val, lt_ptr, lt_idx, lt_val, piv;
+    DevBuf tail_rows, tail_cptr, tail_chunks, tail_idx, tail_val, tail_mat; // dense tails
+    uint64_t n_tail_rows = 0, n_tail_chunks = 0;
     uint32_t n_sparse = 0;
     // auxiliary slots [ns, ns + n_aux) of the covers: real kept slot | sheet bit (second sheet)
     DevBuf aux_src;
@@ -107,6 +113,32 @@ struct SpMeta {
     uint64_t piv_off;   // first pivot
     uint32_t nbig_f;    // leading forward levels solved by grid-wide launches (sk_sparse_grid)
     uint32_t nbig_b;    // trailing backward levels solved by grid-wide launches
+    // dense tail: the last-eliminated nt factor rows (the last levels, a chain of hubs) are
+    // solved as one dense product x_T = M_T (rhs_T - L_TF y_F); nt = 0: none
+    uint32_t nt;
+    uint32_t n_tchunks; // chunks of the tail rows' entries on earlier rows
+    uint64_t tail_off;  // into tail_rows / tail_cptr (rows) and the tail scratch
+    uint64_t tchunk_off; // into tail_chunks and the chunk scratch
+    uint64_t tmat_off;  // M_T, nt x nt row-major
+};
+
+// A run of at most TAIL_CHUNK entries of one tail row (its entries on rows before the tail)
+struct TailChunk {
+    uint32_t row;     // tail-local row
+    uint32_t pad;
+    uint64_t p0, p1;  // into tail_idx / tail_val
+};
+
+// Dense tails of the sparse factors (device pointers) and their scratch: per tail row the
+// reduced right-hand side (tsc), per chunk a partial sum (csc)
+template <typename CT> struct TailArgs {
+    const uint32_t *rows;  // slot of each tail row
+    const uint64_t *cptr;  // first chunk of each tail row (one more entry at the end)
+    const TailChunk *chunks;
+    const uint32_t *idx;
+    const CT *val;
+    const CT *mat;         // M_T of each component, row-major
+    double *tsc, *csc;
 };
 
 // A row of a large level, solved by a grid-wide launch: the component's SpMeta and the row slot.
@@ -566,7 +598,8 @@ __global__ __launch_bounds__(SP_BLK) void sk_sparse(const SpMeta *__restrict__ m
                                                  const CT *__restrict__ piv, const double *__restrict__ red,
                                                  const double *__restrict__ mu2, const uint32_t *__restrict__ aux_src,
                                                  uint64_t ns, double *xs, uint32_t k,
-                                                 uint32_t kshift, const int32_t *__restrict__ gate, uint32_t phase) {
+                                                 uint32_t kshift, const int32_t *__restrict__ gate, uint32_t phase,
+                                                 TailArgs<CT> ta) {
     if (gate_closed(gate, k)) return;
     __shared__ double sh[SP_BLK];
     const uint32_t kp = 1u << kshift, j = threadIdx.x & (kp - 1);
@@ -581,6 +614,31 @@ __global__ __launch_bounds__(SP_BLK) void sk_sparse(const SpMeta *__restrict__ m
             sparse_level<CT, true>(m, lvl_f_ptr[o], lvl_f_long[o], lvl_f_ptr[o + 1], lvl_f_rows, l_ptr, l_idx, l_val,
                                    red, aux_src, ns, mu, xs, sh, k, kshift);
             __syncthreads();
+        }
+        if (m.nt) { // the dense tail: x_T = M_T (rhs_T - L_TF y_F), final (no pivot step)
+            const uint32_t grp = threadIdx.x >> kshift, ngrp = SP_BLK >> kshift;
+            if (on) // partial sums of the tail rows' entries on earlier rows, per chunk
+                for (uint32_t c = grp; c < m.n_tchunks; c += ngrp) {
+                    const TailChunk ch = ta.chunks[m.tchunk_off + c];
+                    ta.csc[(m.tchunk_off + c) * k + j] =
+                        dot_entries<CT, false>(0.0, ch.p0, ch.p1, 1, ta.idx, ta.val, xs, k, j);
+                }
+            __syncthreads();
+            if (on) // reduced right-hand sides: rhs minus the row's chunks in order
+                for (uint32_t a = grp; a < m.nt; a += ngrp) {
+                    const uint64_t ra = m.tail_off + a;
+                    double r = sparse_rhs<CT>(m, ta.rows[ra], red, aux_src, ns, mu, k, j);
+                    for (uint64_t c = ta.cptr[ra]; c < ta.cptr[ra + 1]; ++c) r -= ta.csc[c * k + j];
+                    ta.tsc[ra * k + j] = r;
+                }
+            __syncthreads();
+            if (on)
+                for (uint32_t a = grp; a < m.nt; a += ngrp) {
+                    const CT *Mr = ta.mat + m.tmat_off + (uint64_t)a * m.nt;
+                    double acc = 0.0;
+                    for (uint32_t b = 0; b < m.nt; ++b) acc += (double)Mr[b] * ta.tsc[(m.tail_off + b) * k + j];
+                    xs[(uint64_t)ta.rows[m.tail_off + a] * k + j] = acc;
+                }
         }
         return;
     }
@@ -897,6 +955,12 @@ std::shared_ptr<SchwarzDev<CT>> build_dev(const Precond::Schwarz &S) {
         std::vector<SpMeta> metas;
         std::vector<uint64_t> lptr, ltptr, lvf_ptr, lvb_ptr, lvf_long, lvb_long;
         std::vector<uint32_t> lidx, ltidx, lvf_rows, lvb_rows;
+        // dense tails: rows (slots), per-row first chunk, chunks, entries on earlier rows, M_T
+        std::vector<uint32_t> t_rows, t_idx;
+        std::vector<uint64_t> t_cptr;
+        std::vector<TailChunk> t_chunks;
+        std::vector<CT> t_val, t_mat;
+        std::vector<std::vector<uint32_t>> tail_local; // per sparse component: its tail rows (local)
         std::vector<CT> lval, ltval, piv;
         uint64_t max_depth = 0, total_rows = 0;
         std::vector<uint32_t> aux_src;
@@ -960,8 +1024,91 @@ std::shared_ptr<SchwarzDev<CT>> build_dev(const Precond::Schwarz &S) {
                 lf[i] = v;
                 df = std::max(df, v + 1);
             }
+            // dense tail: the trailing levels with at most TAIL_LEVEL_ROWS rows each (a chain of
+            // hubs eliminated last), at most TAIL_MAX_ROWS rows, when they span TAIL_MIN_LEVELS
+            // levels or more; the schedules then cover the rows before the cut only
+            uint32_t cut = df;
+            {
+                std::vector<uint32_t> per(df, 0);
+                for (uint32_t i = 0; i < nk; ++i) per[lf[i]]++;
+                uint32_t rows_t = 0;
+                while (cut > 1 && per[cut - 1] <= TAIL_LEVEL_ROWS && rows_t + per[cut - 1] <= TAIL_MAX_ROWS)
+                    rows_t += per[--cut];
+                if (df - cut < TAIL_MIN_LEVELS) cut = df;
+            }
+            constexpr uint32_t SKIP = UINT32_MAX;
+            const uint32_t df_all = df;
+            df = cut;
             const uint32_t db = df;
-            for (uint32_t i = 0; i < nk; ++i) lb[i] = df - 1 - lf[i];
+            for (uint32_t i = 0; i < nk; ++i) {
+                if (lf[i] >= cut) lf[i] = lb[i] = SKIP;
+                else lb[i] = df - 1 - lf[i];
+            }
+            m.nt = 0;
+            m.n_tchunks = 0;
+            m.tail_off = t_rows.size();
+            m.tchunk_off = t_chunks.size();
+            m.tmat_off = t_mat.size();
+            tail_local.emplace_back();
+            if (cut < df_all) {
+                std::vector<uint32_t> &trow = tail_local.back();
+                std::vector<int32_t> tix(nk, -1);
+                for (uint32_t i = 0; i < nk; ++i)
+                    if (lf[i] == SKIP) {
+                        tix[i] = (int32_t)trow.size();
+                        trow.push_back(i);
+                    }
+                const uint32_t nt = (uint32_t)trow.size();
+                m.nt = nt;
+                // entries of the tail rows on rows before the tail, in entry order, chunked
+                for (uint32_t a = 0; a < nt; ++a) {
+                    const uint32_t i = trow[a];
+                    t_rows.push_back(kb + i);
+                    t_cptr.push_back(t_chunks.size());
+                    uint64_t p0 = t_idx.size();
+                    for (uint64_t p = S.l_ptr[ro + i]; p < S.l_ptr[ro + i + 1]; ++p)
+                        if (tix[S.l_idx[p]] < 0) {
+                            t_idx.push_back(kb + S.l_idx[p]);
+                            t_val.push_back((CT)S.l_val[p]);
+                        }
+                    for (; p0 < t_idx.size(); p0 += TAIL_CHUNK)
+                        t_chunks.push_back({a, 0u, p0, std::min<uint64_t>(t_idx.size(), p0 + TAIL_CHUNK)});
+                }
+                m.n_tchunks = (uint32_t)(t_chunks.size() - m.tchunk_off);
+                // M_T = (I + L_TT')^{-1} D_T^+ (I + L_TT)^{-1}, one column per unit vector: the
+                // tail's forward substitution, the pivots (zero: the gauge, giving 0) and the
+                // backward substitution, exactly the operator the sparse solve applies
+                std::vector<double> M((size_t)nt * nt), y(nt);
+                for (uint32_t sidx = 0; sidx < nt; ++sidx) {
+                    std::fill(y.begin(), y.end(), 0.0);
+                    y[sidx] = 1.0;
+                    for (uint32_t a = 0; a < nt; ++a) {
+                        const uint32_t i = trow[a];
+                        double v = y[a];
+                        for (uint64_t p = S.l_ptr[ro + i]; p < S.l_ptr[ro + i + 1]; ++p) {
+                            const int32_t b = tix[S.l_idx[p]];
+                            if (b >= 0) v -= S.l_val[p] * y[b];
+                        }
+                        y[a] = v;
+                    }
+                    for (uint32_t a = 0; a < nt; ++a) {
+                        const double dv = S.piv[S.piv_off[c] + trow[a]];
+                        y[a] = dv != 0.0 ? y[a] / dv : 0.0;
+                    }
+                    for (uint32_t a = nt; a-- > 0;) { // column-oriented: row a is final here
+                        const uint32_t i = trow[a];
+                        for (uint64_t p = S.l_ptr[ro + i]; p < S.l_ptr[ro + i + 1]; ++p) {
+                            const int32_t b = tix[S.l_idx[p]];
+                            if (b >= 0) y[b] -= S.l_val[p] * y[a];
+                        }
+                    }
+                    for (uint32_t a = 0; a < nt; ++a) M[(size_t)a * nt + sidx] = y[a];
+                }
+                for (double v : M) t_mat.push_back((CT)v);
+                if (std::getenv("WITHIN_AMD_VERBOSE"))
+                    std::fprintf(stderr, "[within_amd] sparse factor of component %u: dense tail of %u rows (%u of %u levels)\n",
+                                 (unsigned)c, nt, df_all - cut, df_all);
+            }
             // rows bucketed by level, short rows first and rows with more than LONG_ROW entries
             // last; `lng` (aligned with `ptr`) marks where a level's long rows start
             auto bucket = [&](const std::vector<uint32_t> &lvl, uint32_t depth, const std::vector<uint64_t> &rnnz,
@@ -969,12 +1116,13 @@ std::shared_ptr<SchwarzDev<CT>> build_dev(const Precond::Schwarz &S) {
                 const uint64_t off = ptr.size();
                 std::vector<uint64_t> cnt(depth + 1, 0), nlong(depth + 1, 0);
                 for (uint32_t i = 0; i < nk; ++i) {
+                    if (lvl[i] == SKIP) continue; // a dense-tail row
                     cnt[lvl[i] + 1]++;
                     if (rnnz[i] > LONG_ROW) nlong[lvl[i]]++;
                 }
                 for (uint32_t l = 0; l < depth; ++l) cnt[l + 1] += cnt[l];
                 const uint64_t rbase = rows.size();
-                rows.resize(rbase + nk);
+                rows.resize(rbase + cnt[depth]);
                 std::vector<uint64_t> cs(depth), cl(depth);
                 for (uint32_t l = 0; l < depth; ++l) {
                     cs[l] = cnt[l];
@@ -982,6 +1130,7 @@ std::shared_ptr<SchwarzDev<CT>> build_dev(const Precond::Schwarz &S) {
                 }
                 for (uint32_t i = 0; i < nk; ++i) {
                     const uint32_t l = lvl[i];
+                    if (l == SKIP) continue;
                     rows[rbase + (rnnz[i] > LONG_ROW ? cl[l]++ : cs[l]++)] = kb + i;
                 }
                 for (uint32_t l = 0; l <= depth; ++l) {
@@ -1048,7 +1197,10 @@ std::shared_ptr<SchwarzDev<CT>> build_dev(const Precond::Schwarz &S) {
         std::vector<GridRow> drow;
         for (uint32_t mi = 0; mi < metas.size(); ++mi) {
             const SpMeta &m = metas[mi];
-            for (uint32_t r = 0; r < m.nk; ++r) drow.push_back({mi, m.kept_base + r});
+            std::vector<uint8_t> in_tail(m.nk, 0); // tail rows are final after the dense product
+            for (uint32_t i : tail_local[mi]) in_tail[i] = 1;
+            for (uint32_t r = 0; r < m.nk; ++r)
+                if (!in_tail[r]) drow.push_back({mi, m.kept_base + r});
             if (m.cover)
                 for (uint32_t r = 0; r < m.nk; ++r) crow.push_back(m.kept_base + r);
         }
@@ -1066,6 +1218,15 @@ std::shared_ptr<SchwarzDev<CT>> build_dev(const Precond::Schwarz &S) {
         D->lt_idx.upload(ltidx.data(), ltidx.size());
         D->lt_val.upload(ltval.data(), ltval.size());
         D->piv.upload(piv.data(), piv.size());
+        t_cptr.push_back(t_chunks.size());
+        D->n_tail_rows = t_rows.size();
+        D->n_tail_chunks = t_chunks.size();
+        D->tail_rows.upload(t_rows.data(), t_rows.size());
+        D->tail_cptr.upload(t_cptr.data(), t_cptr.size());
+        D->tail_chunks.upload(t_chunks.data(), t_chunks.size());
+        D->tail_idx.upload(t_idx.data(), t_idx.size());
+        D->tail_val.upload(t_val.data(), t_val.size());
+        D->tail_mat.upload(t_mat.data(), t_mat.size());
         D->sp_lvl_f_ptr.upload(lvf_ptr.data(), lvf_ptr.size());
         D->sp_lvl_f_long.upload(lvf_long.data(), lvf_long.size());
         D->sp_lvl_b_long.upload(lvb_long.data(), lvb_long.size());
@@ -1083,7 +1244,7 @@ template <typename CT>
 struct SchwarzRunImpl : SchwarzRun {
     std::shared_ptr<SchwarzDev<CT>> D;
     uint32_t kcap = 0;
-    DevBuf loc, red, xs, part, rowpart, mu1, mu2, mu3, dpart;
+    DevBuf loc, red, xs, part, rowpart, mu1, mu2, mu3, dpart, tsc, csc;
 
     void ensure(uint32_t k) {
         if (k <= kcap) return;
@@ -1098,6 +1259,8 @@ struct SchwarzRunImpl : SchwarzRun {
         mu2.alloc(D->nc * k * 8);
         mu3.alloc(D->nc * k * 8);
         dpart.alloc(D->dense_part_rows * k * 8 + 8);
+        tsc.alloc(D->n_tail_rows * k * 8 + 8);
+        csc.alloc(D->n_tail_chunks * k * 8 + 8);
     }
 
     // the elimination SpMVs (C' rows into the reduced right-hand side, C rows in the
@@ -1162,8 +1325,12 @@ struct SchwarzRunImpl : SchwarzRun {
                                    dp<uint64_t>(d.sp_lvl_b_long), dp<uint32_t>(d.sp_lvl_b_rows),
                                    dp<uint64_t>(d.l_ptr), dp<uint32_t>(d.l_idx), dp<CT>(d.l_val),
                                    dp<uint64_t>(d.lt_ptr), dp<uint32_t>(d.lt_idx), dp<CT>(d.lt_val), dp<CT>(d.piv),
-                                   redp, dp<double>(mu2), dp<uint32_t>(d.aux_src), d.ns, xsp, k, kshift, gate, phase);
-                if (phase == 0)
+                                   redp, dp<double>(mu2), dp<uint32_t>(d.aux_src), d.ns, xsp, k, kshift, gate, phase,
+                                   TailArgs<CT>{dp<uint32_t>(d.tail_rows), dp<uint64_t>(d.tail_cptr),
+                                                dp<TailChunk>(d.tail_chunks), dp<uint32_t>(d.tail_idx),
+                                                dp<CT>(d.tail_val), dp<CT>(d.tail_mat), dp<double>(tsc),
+                                                dp<double>(csc)});
+                if (phase == 0 && d.n_diag_rows)
                     hipLaunchKernelGGL((sk_sparse_diag<CT>), dim3(grid_rows(d.n_diag_rows, kshift)), dim3(BLK), 0,
                                        st, dp<GridRow>(d.diag_rows), d.n_diag_rows, dp<SpMeta>(d.sp_meta),
                                        dp<CT>(d.piv), xsp, k, kshift, gate);
