@@ -628,15 +628,36 @@ __global__ __launch_bounds__(SP_BLK) void sk_sparse(const SpMeta *__restrict__ m
                 for (uint32_t a = grp; a < m.nt; a += ngrp) {
                     const uint64_t ra = m.tail_off + a;
                     double r = sparse_rhs<CT>(m, ta.rows[ra], red, aux_src, ns, mu, k, j);
-                    for (uint64_t c = ta.cptr[ra]; c < ta.cptr[ra + 1]; ++c) r -= ta.csc[c * k + j];
+                    const uint64_t c1 = ta.cptr[ra + 1];
+                    uint64_t c = ta.cptr[ra];
+                    for (; c + 8 <= c1; c += 8) { // eight loads in flight, subtracted in order
+                        double v[8];
+#pragma unroll
+                        for (int q = 0; q < 8; ++q) v[q] = ta.csc[(c + q) * k + j];
+#pragma unroll
+                        for (int q = 0; q < 8; ++q) r -= v[q];
+                    }
+                    for (; c < c1; ++c) r -= ta.csc[c * k + j];
                     ta.tsc[ra * k + j] = r;
                 }
             __syncthreads();
             if (on)
                 for (uint32_t a = grp; a < m.nt; a += ngrp) {
                     const CT *Mr = ta.mat + m.tmat_off + (uint64_t)a * m.nt;
+                    const double *r = ta.tsc + m.tail_off * k + j;
                     double acc = 0.0;
-                    for (uint32_t b = 0; b < m.nt; ++b) acc += (double)Mr[b] * ta.tsc[(m.tail_off + b) * k + j];
+                    uint32_t b = 0;
+                    for (; b + 8 <= m.nt; b += 8) { // eight products' loads in flight, summed in order
+                        double mv[8], rv[8];
+#pragma unroll
+                        for (int q = 0; q < 8; ++q) {
+                            mv[q] = (double)Mr[b + q];
+                            rv[q] = r[(uint64_t)(b + q) * k];
+                        }
+#pragma unroll
+                        for (int q = 0; q < 8; ++q) acc += mv[q] * rv[q];
+                    }
+                    for (; b < m.nt; ++b) acc += (double)Mr[b] * r[(uint64_t)b * k];
                     xs[(uint64_t)ta.rows[m.tail_off + a] * k + j] = acc;
                 }
         }
