@@ -155,10 +155,12 @@ bool gpu_available() {
     return hipGetDeviceCount(&n) == hipSuccess && n > 0;
 }
 
-bool dense_schur_pinv_gpu(const DenseSchurArgs &a, bool floating, std::vector<double> &out) {
+bool dense_schur_pinv_gpu(const DenseSchurArgs &a, bool floating, double *out) {
     const uint32_t n = a.nk;
-    out.assign((size_t)n * n, 0.0);
-    if (floating && n <= 1) return true;
+    if (floating && n <= 1) {
+        std::fill(out, out + (size_t)n * n, 0.0);
+        return true;
+    }
     DevBuf cptr, cidx, cval, diag, sur, rabs, tp, te, tv, S, mean, info;
     upload_vec(cptr, a.cptr, (size_t)a.ne + 1);
     upload_vec(cidx, a.cidx, a.nnz);
@@ -201,7 +203,7 @@ bool dense_schur_pinv_gpu(const DenseSchurArgs &a, bool floating, std::vector<do
         hipLaunchKernelGGL(k_project, dim3(eg), dim3(DB), 0, 0, s, n, (uint64_t)n, mean.as<double>());
     }
     WX_HIP(hipGetLastError());
-    WX_HIP(hipMemcpy(out.data(), s, nn * 8, hipMemcpyDeviceToHost));
+    WX_HIP(hipMemcpy(out, s, nn * 8, hipMemcpyDeviceToHost));
     return true;
 }
 

@@ -902,6 +902,14 @@ std::shared_ptr<Precond> build_additive(const Design &d, const std::vector<doubl
         big[c] = dense[c] && nk > 1024 && have_gpu;
         S->fac_kind[c] = dense[c] ? 0 : (comps[c].signed_form ? 2 : 1);
     }
+    // dense blocks' place in the concatenated storage; the GPU path writes its blocks there
+    uint64_t dense_total = 0;
+    for (uint64_t c = 0; c < nc; ++c)
+        if (dense[c]) {
+            S->dense_off[c] = dense_total;
+            dense_total += (uint64_t)comps[c].nk * comps[c].nk;
+        }
+    S->dense.resize(dense_total);
     // per-component factorisation (parallel)
     std::vector<std::vector<double>> dense_mats(nc);
     std::vector<AcFactor> acs(nc);
@@ -955,7 +963,7 @@ std::shared_ptr<Precond> build_additive(const Design &d, const std::vector<doubl
                                cm.signed_form ? rowabs.data() : nullptr,
                                adj.tp.data(),  adj.te.data(),  adj.tv.data(),
                                cm.signed_form};
-        if (!dense_schur_pinv_gpu(a, S->gauge[c] == 0, dense_mats[c])) failed[c] = 2;
+        if (!dense_schur_pinv_gpu(a, S->gauge[c] == 0, S->dense.data() + S->dense_off[c])) failed[c] = 2;
     }
     phase("large dense blocks");
     for (uint64_t c = 0; c < nc; ++c) {
@@ -964,11 +972,9 @@ std::shared_ptr<Precond> build_additive(const Design &d, const std::vector<doubl
     }
     // assemble slots (kept block of sparse components relabelled into elimination order;
     // cover components keep their kept slots and carry the cover's elimination order)
-    uint64_t dense_total = 0, l_total = 0, piv_total = 0, rows_total = 0, cover_total = 0;
+    uint64_t l_total = 0, piv_total = 0, rows_total = 0, cover_total = 0;
     for (uint64_t c = 0; c < nc; ++c) {
-        if (dense[c]) {
-            dense_total += (uint64_t)comps[c].nk * comps[c].nk;
-        } else {
+        if (!dense[c]) {
             const uint64_t nf = acs[c].d.size();
             l_total += acs[c].ridx.size();
             piv_total += nf;
@@ -976,7 +982,6 @@ std::shared_ptr<Precond> build_additive(const Design &d, const std::vector<doubl
             if (S->fac_kind[c] == 2) cover_total += nf;
         }
     }
-    S->dense.resize(dense_total);
     S->l_idx.resize(l_total);
     S->l_val.resize(l_total);
     S->piv.resize(piv_total);
@@ -986,7 +991,7 @@ std::shared_ptr<Precond> build_additive(const Design &d, const std::vector<doubl
     for (auto &cm : comps) cnnz += cm.cidx.size();
     S->c_idx.resize(cnnz);
     S->c_val.resize(cnnz);
-    uint64_t doff = 0, loff = 0, poff = 0, rptr_off = 0, coff = 0, cpos = 0;
+    uint64_t loff = 0, poff = 0, rptr_off = 0, coff = 0, cpos = 0;
     for (uint64_t c = 0; c < nc; ++c) {
         const Component &cm = comps[c];
         const uint64_t base = S->slot_start[c];
@@ -1025,10 +1030,8 @@ std::shared_ptr<Precond> build_additive(const Design &d, const std::vector<doubl
             }
         }
         for (uint32_t k = 0; k < nk; ++k) S->c_ptr[base + ne + k] = cpos;
-        if (dense[c]) {
-            S->dense_off[c] = doff;
-            std::copy(dense_mats[c].begin(), dense_mats[c].end(), S->dense.begin() + doff);
-            doff += (uint64_t)nk * nk;
+        if (dense[c]) { // the large blocks are in place already
+            if (!big[c]) std::copy(dense_mats[c].begin(), dense_mats[c].end(), S->dense.begin() + S->dense_off[c]);
             std::vector<double>().swap(dense_mats[c]);
         } else {
             const AcFactor &f = acs[c];

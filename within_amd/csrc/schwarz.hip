@@ -19,6 +19,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <numeric>
+#include <type_traits>
 
 #include "schwarz.hpp"
 
@@ -766,7 +767,8 @@ uint32_t grid_rows(uint64_t rows, uint32_t kshift, uint32_t cap = 65535) {
 
 template <typename T, typename S> std::vector<T> cast_vec(const std::vector<S> &v) {
     std::vector<T> o(v.size());
-    for (size_t i = 0; i < v.size(); ++i) o[i] = (T)v[i];
+#pragma omp parallel for schedule(static) if (v.size() > (1u << 20))
+    for (int64_t i = 0; i < (int64_t)v.size(); ++i) o[i] = (T)v[i];
     return o;
 }
 
@@ -955,7 +957,9 @@ std::shared_ptr<SchwarzDev<CT>> build_dev(const Precond::Schwarz &S) {
     D->drow_base.upload(drow_base.data(), drow_base.size());
     D->drow_nk.upload(drow_nk.data(), drow_nk.size());
     D->drow_slot.upload(drow_slot.data(), drow_slot.size());
-    {
+    if constexpr (std::is_same<CT, double>::value) {
+        D->dmat.upload(S.dense.data(), S.dense.size());
+    } else {
         auto v = cast_vec<CT>(S.dense);
         D->dmat.upload(v.data(), v.size());
     }

@@ -85,8 +85,8 @@ std::shared_ptr<Precond> build_additive(const Design &d, const std::vector<doubl
 // A component's kept-block reduction on the GPU (set-up time): S = D_k - C' D_e^{-1} C with its
 // diagonal formed as the host assembly forms it (Laplacian row sums, or the signed form's
 // magnitudes, plus the grounded surplus), then S^+ (floating Laplacian: P G P over the grounded
-// minor's inverse G) or S^{-1}, returned row-major n_kept x n_kept.  False when the Cholesky
-// factorisation meets a non-positive pivot.
+// minor's inverse G) or S^{-1}, written row-major n_kept x n_kept to `out`.  False when the
+// Cholesky factorisation meets a non-positive pivot.
 struct DenseSchurArgs {
     uint32_t ne, nk;
     const uint64_t *cptr; // C: ne x nk CSR, columns ascending per row
@@ -102,7 +102,7 @@ struct DenseSchurArgs {
     bool signed_form;
 };
 bool gpu_available();
-bool dense_schur_pinv_gpu(const DenseSchurArgs &a, bool floating, std::vector<double> &out);
+bool dense_schur_pinv_gpu(const DenseSchurArgs &a, bool floating, double *out);
 
 // Diagonal preconditioner (operator/schwarz.rs:185-229).
 std::shared_ptr<Precond> build_diagonal(const Design &d, const std::vector<double> &weights);
