@@ -525,11 +525,20 @@ __global__ void k_segfix(TermInfo ti, const SegFix *__restrict__ fixes, uint32_t
     const uint32_t j = threadIdx.x & (kp - 1), grp = threadIdx.x >> kshift;
     const bool on = j < k && !(active && !active[j]);
     const SegFix f = fixes[blockIdx.x];
+    const uint32_t step = BLK >> kshift;
     for (int c = 0; c < NC; ++c) {
         double acc = 0.0;
-        if (on)
-            for (uint32_t q = grp; q < f.count; q += BLK >> kshift)
-                acc += partials[((uint64_t)(f.first + q) * NC + c) * k + j];
+        if (on) { // eight loads in flight, added in slot order
+            uint32_t q = grp;
+            for (; q + 7 * step < f.count; q += 8 * step) {
+                double v[8];
+#pragma unroll
+                for (int u = 0; u < 8; ++u) v[u] = partials[((uint64_t)(f.first + q + u * step) * NC + c) * k + j];
+#pragma unroll
+                for (int u = 0; u < 8; ++u) acc += v[u];
+            }
+            for (; q < f.count; q += step) acc += partials[((uint64_t)(f.first + q) * NC + c) * k + j];
+        }
         sh[threadIdx.x] = acc;
         __syncthreads();
         for (uint32_t s = BLK / 2; s >= kp && s > 0; s >>= 1) {
