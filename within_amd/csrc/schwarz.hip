@@ -488,10 +488,21 @@ __global__ __launch_bounds__(BLK) void sk_dense(uint32_t n_rows, const uint64_t 
     const uint32_t groups = (gridDim.x * BLK) >> kshift;
     for (uint32_t r = (blockIdx.x * BLK + threadIdx.x) >> kshift; r < n_rows; r += groups) {
         const CT *m = mat + row_off[r];
-        const uint64_t base = row_base[r];
+        const double *rd = red + row_base[r] * k + j;
         const uint32_t nk = row_nk[r];
         double acc = 0.0;
-        for (uint32_t l = 0; l < nk; ++l) acc += (double)m[l] * red[(base + l) * k + j];
+        uint32_t l = 0;
+        for (; l + 8 <= nk; l += 8) { // eight products' loads in flight, summed in order
+            double mv[8], rv[8];
+#pragma unroll
+            for (int q = 0; q < 8; ++q) {
+                mv[q] = (double)m[l + q];
+                rv[q] = rd[(uint64_t)(l + q) * k];
+            }
+#pragma unroll
+            for (int q = 0; q < 8; ++q) acc += mv[q] * rv[q];
+        }
+        for (; l < nk; ++l) acc += (double)m[l] * rd[(uint64_t)l * k];
         xs[(uint64_t)row_slot[r] * k + j] = acc;
     }
 }
