@@ -314,6 +314,49 @@ def bench_panel(args):
     }), flush=True)
 
 
+def bench_fwl(args):
+    """The Frisch-Waugh-Lovell step after demeaning: G = A'A of a demeaned block A = [y~, X~]
+    of n = 5,000,000 rows and 16 columns (a response and 15 regressors), column-major in HBM.
+    A step is one wx_gram_device call (one pass over A plus the fixed-order combine of the
+    workgroups' partial triangles, synchronised)."""
+    import torch
+    from within_amd import _lib
+    L = _lib.lib()
+    n, m = 5_000_000, 16
+    g = torch.Generator(device="cuda")
+    g.manual_seed(6)
+    A = torch.randn((m, n), dtype=torch.float64, device="cuda", generator=g)  # column-major n x m
+    G = torch.empty((m, m), dtype=torch.float64, device="cuda")
+    stream = torch.cuda.current_stream().cuda_stream
+
+    def step():
+        _lib.check(L.wx_gram_device(A.data_ptr(), n, m, n, G.data_ptr(), C.c_void_p(stream)))
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        step()
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t0) / args.steps
+    ref = (A @ A.t()).cpu().numpy()
+    err = float(np.abs(G.cpu().numpy() - ref).max() / np.abs(ref).max())
+    bytes_per = 8.0 * n * m
+    log(f"[bench] fwl gram: {dt * 1e6:.1f} us per call, {bytes_per / dt / 1e9:.0f} GB/s, max rel diff vs torch {err:.1e}")
+    print(json.dumps({
+        "metric": "Gram matrix A'A of the demeaned block (FWL step) per second", "value": 1.0 / dt, "unit": "calls/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": dt * 1e3, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic (seeded N(0,1) block)",
+        "config": {"workload": "FWL Gram: n=5,000,000 rows x 16 columns, column-major in HBM", "n": n, "m": m},
+        "roofline": {"bound": "hbm", "achieved": bytes_per / dt / 1e9, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": bytes_per / dt / 1e9 / HBM_PEAK_GBS, "traffic": None, "kernel": "k_gram + k_gram_reduce",
+                     "algorithmic_bytes_per_launch": bytes_per, "avg_launch_ms": dt * 1e3,
+                     "note": "per call including the synchronisation; A is read once"},
+        "max_rel_diff_vs_torch": err,
+    }), flush=True)
+
+
 def bench_grad(args):
     """Differentiable weighted projection: n = 1,000,000, factors of 50,000 Zipf(1.1) and 2,000
     uniform levels, k = 8, weights exp(N(0, 0.5^2)), loss sum(mask * demeaned^2).  A step is one
@@ -389,9 +432,10 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
-    ap.add_argument("--workload", choices=["solve_batch", "many", "panel", "grad"], default="solve_batch",
+    ap.add_argument("--workload", choices=["solve_batch", "many", "panel", "grad", "fwl"], default="solve_batch",
                     help="solve_batch: the weighted 5M-row batch (default); many: 64 uneven independent problems; "
-                         "panel: worker-firm panel with a firm slope; grad: differentiable projection")
+                         "panel: worker-firm panel with a firm slope; grad: differentiable projection; "
+                         "fwl: the Gram matrix of a demeaned block")
     ap.add_argument("--problems", type=int, default=64)
     ap.add_argument("--sweep", action="store_true", help="many: also time every launch shape")
     ap.add_argument("--shape", default="", help="many: fixed launch shape 'streams,iterations_per_graph,graph,pack'")
@@ -412,6 +456,9 @@ def main():
         return
     if args.workload == "grad":
         bench_grad(args)
+        return
+    if args.workload == "fwl":
+        bench_fwl(args)
         return
 
     import torch

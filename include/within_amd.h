@@ -166,6 +166,15 @@ wx_status wx_solver_new(const wx_design *design, const double *weights, const wx
 
 void wx_solver_release(wx_solver *s);
 
+/* The Frisch-Waugh-Lovell step after demeaning (the reference's README.md:55-66, where numpy's
+ * lstsq does it on the host): G = A'A (m x m, column-major) for the column-major n x m matrix
+ * A = [y~, X~] of demeaned columns, so that X~'X~ and X~'y~ are G's blocks.  One pass over A,
+ * deterministic (partial sums combined in a fixed order); 1 <= m <= 64, lda >= n.
+ * wx_gram_device: A and G in device memory, enqueued on `stream` (a hipStream_t, NULL: the
+ * default stream) and synchronised before returning.  wx_gram: A and G in host memory. */
+wx_status wx_gram_device(const double *A, uint64_t n, uint64_t m, uint64_t lda, double *G, void *stream);
+wx_status wx_gram(const double *A, uint64_t n, uint64_t m, uint64_t lda, double *G);
+
 /* A Solver for the same design with new observation weights (NULL: unweighted), as
  * wx_solver_new(design, weights, cfg, prebuilt) would build it, reusing `base`'s design and
  * device index structures (copied on the device) so that only the weights and the

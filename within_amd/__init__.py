@@ -14,6 +14,8 @@ from ._within import (
     SolveResult,
     Solver,
     UnidentifiedDirection,
+    fwl,
+    gram,
     last_many_shape,
     set_many_shape,
     solve,
@@ -35,6 +37,8 @@ __all__ = [
     "solve",
     "solve_batch",
     "solve_many",
+    "gram",
+    "fwl",
     "last_many_shape",
     "set_many_shape",
     "config",

@@ -13,6 +13,7 @@ per-level reparametrised slope columns the solver works in) / ``stop_reason`` on
 from __future__ import annotations
 
 import ctypes as C
+import sys
 import time
 import warnings
 from typing import Optional, Sequence
@@ -376,6 +377,42 @@ def _f64_2d(name: str, obj) -> np.ndarray:
     if not isinstance(obj, np.ndarray) or obj.dtype != np.float64 or obj.ndim != 2:
         raise TypeError(f"{name} must be a 2-D float64 array{_dtype_name(obj)}")
     return obj
+
+
+def gram(A) -> np.ndarray:
+    """A'A (m x m) of an n x m float64 matrix, formed on the GPU in one pass (1 <= m <= 64;
+    deterministic).  A is a numpy array (copied to the device) or a float64 CUDA tensor (read in
+    HBM; made column-major first if it is not)."""
+    L = _lib.lib()
+    torch = sys.modules.get("torch")
+    if torch is not None and isinstance(A, torch.Tensor):
+        if A.dtype != torch.float64 or A.dim() != 2:
+            raise TypeError("A must be a 2-D float64 tensor")
+        if not A.is_cuda:
+            raise ValueError("A must be a CUDA (HIP) tensor")
+        n, m = A.shape
+        Ac = A.t().contiguous()  # (m, n) row-major: column-major n x m
+        G = torch.empty((m, m), dtype=torch.float64, device=A.device)
+        _lib.check(L.wx_gram_device(Ac.data_ptr(), n, m, n, G.data_ptr(),
+                                    C.c_void_p(torch.cuda.current_stream().cuda_stream)))
+        return G.cpu().numpy()
+    A = np.asfortranarray(_f64_2d("A", A))
+    n, m = A.shape
+    G = np.empty((m, m), order="F")
+    _lib.check(L.wx_gram(A.ctypes.data, n, m, n, G.ctypes.data))
+    return G
+
+
+def fwl(demeaned, rcond=None) -> np.ndarray:
+    """Frisch-Waugh-Lovell coefficients after demeaning ``[y, X]`` together (README.md:55-66 of
+    the reference): beta = argmin ||y~ - X~ beta|| with y~ = demeaned[:, 0] and X~ = demeaned[:, 1:].
+    Formed from the normal equations X~'X~ beta = X~'y~: G = gram(demeaned) on the GPU, then the
+    p x p least-squares solve on the host (numpy lstsq with ``rcond``; the minimum-norm solution
+    when X~ is rank deficient)."""
+    G = gram(demeaned)
+    if G.shape[0] < 2:
+        raise ValueError("demeaned needs the response and at least one regressor column")
+    return np.linalg.lstsq(G[1:, 1:], G[1:, 0], rcond=rcond)[0]
 
 
 class Effect:

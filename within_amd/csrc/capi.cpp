@@ -291,6 +291,48 @@ wx_status wx_solver_reweight(const wx_solver *base, const double *weights, const
     });
 }
 
+wx_status wx_gram_device(const double *A, uint64_t n, uint64_t m, uint64_t lda, double *G, void *stream) {
+    return guard([&] {
+        if (!A || !G) fail(WX_ERR_INVALID_INPUT, "null matrix pointer");
+        if (m == 0 || m > gram_max_cols())
+            fail(WX_ERR_INVALID_INPUT, "the Gram matrix takes 1 to " + std::to_string(gram_max_cols()) + " columns");
+        if (lda < n) fail(WX_ERR_INVALID_INPUT, "leading dimension smaller than the number of rows");
+        require_device();
+        hipStream_t st = (hipStream_t)stream;
+        // per-thread partial-sum scratch, grown on demand and kept until the process ends (the
+        // call synchronises, so a thread's calls never overlap on it)
+        static thread_local DevBuf *part = new DevBuf();
+        static thread_local int part_dev = -1;
+        const uint64_t bytes = gram_scratch_doubles(n, (uint32_t)m) * 8;
+        int dev = 0;
+        WX_HIP(hipGetDevice(&dev));
+        if (part->bytes < bytes || part_dev != dev) {
+            if (part_dev != dev) part->p = nullptr; // another device's buffer: left to the process end
+            part->alloc(bytes);
+            part_dev = dev;
+        }
+        gram_device(A, n, (uint32_t)m, lda, G, part->as<double>(), st);
+        WX_HIP(hipStreamSynchronize(st));
+    });
+}
+
+wx_status wx_gram(const double *A, uint64_t n, uint64_t m, uint64_t lda, double *G) {
+    return guard([&] {
+        if (!A || !G) fail(WX_ERR_INVALID_INPUT, "null matrix pointer");
+        if (m == 0 || m > gram_max_cols())
+            fail(WX_ERR_INVALID_INPUT, "the Gram matrix takes 1 to " + std::to_string(gram_max_cols()) + " columns");
+        if (lda < n) fail(WX_ERR_INVALID_INPUT, "leading dimension smaller than the number of rows");
+        require_device();
+        DevBuf a, g, part;
+        a.alloc(std::max<uint64_t>(n, 1) * m * 8);
+        if (n) WX_HIP(hipMemcpy2D(a.p, n * 8, A, lda * 8, n * 8, m, hipMemcpyHostToDevice));
+        g.alloc(m * m * 8);
+        part.alloc(gram_scratch_doubles(n, (uint32_t)m) * 8);
+        gram_device(a.as<double>(), n, (uint32_t)m, n, g.as<double>(), part.as<double>(), nullptr);
+        WX_HIP(hipMemcpy(G, g.p, m * m * 8, hipMemcpyDeviceToHost));
+    });
+}
+
 wx_status wx_precond_build(const wx_design *design, const double *weights, const wx_precond_config *cfg,
                            wx_precond **out) {
     return guard([&] {

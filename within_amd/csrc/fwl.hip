@@ -1,0 +1,159 @@
+// fwl.hip -- the Frisch-Waugh-Lovell step after demeaning: G = A'A for the demeaned block
+// A = [y~, X~] (README.md:55-66 of the reference, where numpy's lstsq does it on the host).
+// One pass over A in HBM: each workgroup stages chunks of up to 64 rows of A in LDS and
+// accumulates the upper triangle of A'A for them, each thread a 4 x 4 block of column pairs over
+// one group of rows; the workgroups' partial triangles are then summed in workgroup order, so
+// the result is deterministic.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+
+#include "device.hpp"
+
+namespace wx {
+
+namespace {
+constexpr int GB = 256;               // threads per workgroup
+constexpr uint32_t GRAM_LDS = 4160;   // doubles of the LDS chunk: rows x (m + 1)
+constexpr uint32_t GRAM_MAX_COLS = 64;
+constexpr uint32_t GRAM_MAX_BLOCKS = 2048;
+constexpr uint32_t TB = 4;            // a thread's 4 x 4 block of column pairs
+
+// rows per LDS chunk: the largest power of two up to 64 with R * (m + 1) doubles in GRAM_LDS
+// (larger chunks measured slower: fewer workgroups fit a CU, and a chunk's load and its
+// products do not overlap within a workgroup)
+__host__ __device__ inline uint32_t gram_chunk_rows(uint32_t m) {
+    uint32_t r = 64;
+    while (r > 16 && r * (m + 1) > GRAM_LDS) r >>= 1;
+    return r;
+}
+
+// pair p of the upper triangle (row-major over i <= j) -> (i, j)
+__device__ inline void pair_of(uint32_t p, uint32_t m, uint32_t &i, uint32_t &j) {
+    i = 0;
+    uint32_t left = p;
+    while (left >= m - i) {
+        left -= m - i;
+        ++i;
+    }
+    j = i + left;
+}
+
+// Per workgroup: the upper triangle of A'A over its chunks.  Columns are grouped by TB; a
+// thread owns one block (bi <= bj) of TB x TB column pairs and one of `rg` row groups (rows
+// g, g + rg, ... of each chunk), so a row's 2 TB values from LDS feed TB^2 products.  At the end
+// the row groups' blocks are added in row-group order.
+__global__ __launch_bounds__(GB) void k_gram(const double *__restrict__ A, uint64_t n, uint32_t m, uint64_t lda,
+                                             double *__restrict__ part) {
+    __shared__ double sh[GRAM_LDS];
+    const uint32_t ld = m + 1; // padded row stride
+    const uint32_t R = gram_chunk_rows(m);
+    const uint32_t mt = (m + TB - 1) / TB, nb = mt * (mt + 1) / 2;
+    uint32_t rg = GB / nb; // row groups: a power of two dividing R
+    rg = rg >= 64 ? 64 : rg >= 32 ? 32 : rg >= 16 ? 16 : rg >= 8 ? 8 : rg >= 4 ? 4 : rg >= 2 ? 2 : 1;
+    if (rg > R) rg = R;
+    const bool on = threadIdx.x < nb * rg;
+    const uint32_t b = threadIdx.x % nb, g = threadIdx.x / nb;
+    uint32_t bi = 0, bj = 0;
+    if (on) pair_of(b, mt, bi, bj);
+    double acc[TB][TB];
+#pragma unroll
+    for (uint32_t x = 0; x < TB; ++x)
+#pragma unroll
+        for (uint32_t y = 0; y < TB; ++y) acc[x][y] = 0.0;
+    const uint64_t n_chunks = (n + R - 1) / R;
+    for (uint64_t ch = blockIdx.x; ch < n_chunks; ch += gridDim.x) {
+        const uint64_t r0 = ch * R;
+        const uint32_t rows = n - r0 < R ? (uint32_t)(n - r0) : R;
+        // column c's rows are contiguous: consecutive threads read consecutive rows; columns
+        // past m (padding of the last column block) read as zero
+        for (uint32_t e = threadIdx.x; e < R * mt * TB; e += GB) {
+            const uint32_t c = e / R, r = e % R;
+            if (c < ld) sh[r * ld + c] = (r < rows && c < m) ? A[c * lda + r0 + r] : 0.0;
+        }
+        __syncthreads();
+        if (on)
+            for (uint32_t r = g; r < R; r += rg) {
+                double vi[TB], vj[TB];
+#pragma unroll
+                for (uint32_t x = 0; x < TB; ++x) {
+                    const uint32_t ci = bi * TB + x, cj = bj * TB + x;
+                    vi[x] = ci < m ? sh[r * ld + ci] : 0.0;
+                    vj[x] = cj < m ? sh[r * ld + cj] : 0.0;
+                }
+#pragma unroll
+                for (uint32_t x = 0; x < TB; ++x)
+#pragma unroll
+                    for (uint32_t y = 0; y < TB; ++y) acc[x][y] += vi[x] * vj[y];
+            }
+        __syncthreads();
+    }
+    // row groups in order, through LDS (reused: the nb * TB^2 block entries fit)
+    const uint32_t npairs = m * (m + 1) / 2;
+    for (uint32_t step = 0; step < rg; ++step) {
+        if (on && g == step)
+#pragma unroll
+            for (uint32_t x = 0; x < TB; ++x)
+#pragma unroll
+                for (uint32_t y = 0; y < TB; ++y)
+                    sh[(b * TB + x) * TB + y] = (step == 0 ? 0.0 : sh[(b * TB + x) * TB + y]) + acc[x][y];
+        __syncthreads();
+    }
+    for (uint32_t q = threadIdx.x; q < nb * TB * TB; q += GB) {
+        const uint32_t bb = q / (TB * TB), x = (q / TB) % TB, y = q % TB;
+        uint32_t ii, jj;
+        pair_of(bb, mt, ii, jj);
+        const uint32_t i = ii * TB + x, j = jj * TB + y;
+        if (i < m && j < m && i <= j) {
+            // index of pair (i, j) in the upper triangle
+            const uint32_t p = i * m - i * (i - 1) / 2 + (j - i);
+            part[(uint64_t)blockIdx.x * npairs + p] = sh[q];
+        }
+    }
+}
+
+// G[i, j] = G[j, i] = the workgroups' partials of pair (i, j) summed in a fixed order: one
+// workgroup per pair, thread t adding workgroups t, t + 256, ... in turn, then an LDS tree
+__global__ __launch_bounds__(GB) void k_gram_reduce(const double *__restrict__ part, uint32_t nblocks, uint32_t m,
+                                                    double *__restrict__ G) {
+    __shared__ double sh[GB];
+    const uint32_t npairs = m * (m + 1) / 2;
+    const uint32_t p = blockIdx.x;
+    double a = 0.0;
+    for (uint32_t b = threadIdx.x; b < nblocks; b += GB) a += part[(uint64_t)b * npairs + p];
+    sh[threadIdx.x] = a;
+    __syncthreads();
+    for (int s = GB / 2; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s) sh[threadIdx.x] += sh[threadIdx.x + s];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        uint32_t i, j;
+        pair_of(p, m, i, j);
+        G[(uint64_t)j * m + i] = sh[0];
+        G[(uint64_t)i * m + j] = sh[0];
+    }
+}
+} // namespace
+
+uint32_t gram_max_cols() { return GRAM_MAX_COLS; }
+
+// G (m x m, column-major, device) = A'A for the column-major n x m device matrix A; `part` is a
+// device scratch of gram_scratch_doubles(n, m) doubles.  Enqueued on `st`.
+uint64_t gram_scratch_doubles(uint64_t n, uint32_t m) {
+    const uint32_t R = gram_chunk_rows(m);
+    const uint64_t chunks = (n + R - 1) / R;
+    return std::max<uint64_t>(1, std::min<uint64_t>(chunks, GRAM_MAX_BLOCKS)) * (uint64_t)m * (m + 1) / 2;
+}
+
+void gram_device(const double *A, uint64_t n, uint32_t m, uint64_t lda, double *G, double *part, hipStream_t st) {
+    const uint32_t R = gram_chunk_rows(m);
+    const uint64_t chunks = (n + R - 1) / R;
+    const uint32_t nb = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(chunks, GRAM_MAX_BLOCKS));
+    hipLaunchKernelGGL(k_gram, dim3(nb), dim3(GB), 0, st, A, n, m, lda, part);
+    const uint32_t npairs = m * (m + 1) / 2;
+    hipLaunchKernelGGL(k_gram_reduce, dim3(npairs), dim3(GB), 0, st, part, nb, m, G);
+    WX_HIP(hipGetLastError());
+}
+
+} // namespace wx

@@ -124,6 +124,11 @@ void upload_design(Solver &s, const std::vector<double> &weights_internal);
 // unweighted), the index structures copied on the device (s.design and s.precond set).
 void reweight_design(Solver &s, const Solver &base, const std::vector<double> &w);
 
+// Gram matrix of a column-major device matrix (fwl.hip)
+uint32_t gram_max_cols();
+uint64_t gram_scratch_doubles(uint64_t n, uint32_t m);
+void gram_device(const double *A, uint64_t n, uint32_t m, uint64_t lda, double *G, double *part, hipStream_t st);
+
 uint64_t next_solver_uid();
 
 // Independent problems of one shape packed into one block-diagonal design, so that every
