@@ -43,6 +43,8 @@ __device__ inline void pair_of(uint32_t p, uint32_t m, uint32_t &i, uint32_t &j)
 // thread owns one block (bi <= bj) of TB x TB column pairs and one of `rg` row groups (rows
 // g, g + rg, ... of each chunk), so a row's 2 TB values from LDS feed TB^2 products.  At the end
 // the row groups' blocks are added in row-group order.
+// PF: chunk elements per thread (R rows x 4 * ceil(m / 4) columns / GB, at most PF)
+template <uint32_t PF>
 __global__ __launch_bounds__(GB) void k_gram(const double *__restrict__ A, uint64_t n, uint32_t m, uint64_t lda,
                                              double *__restrict__ part) {
     __shared__ double sh[GRAM_LDS];
@@ -62,16 +64,36 @@ __global__ __launch_bounds__(GB) void k_gram(const double *__restrict__ A, uint6
 #pragma unroll
         for (uint32_t y = 0; y < TB; ++y) acc[x][y] = 0.0;
     const uint64_t n_chunks = (n + R - 1) / R;
-    for (uint64_t ch = blockIdx.x; ch < n_chunks; ch += gridDim.x) {
+    // a thread's share of a chunk (element e = threadIdx.x + q * GB: column e / R, row e % R;
+    // consecutive threads read consecutive rows of a column), held in registers so that the next
+    // chunk's loads are in flight while this one's products run
+    const uint32_t n_el = R * mt * TB;
+    double pre[PF];
+    auto fetch = [&](uint64_t ch) {
         const uint64_t r0 = ch * R;
         const uint32_t rows = n - r0 < R ? (uint32_t)(n - r0) : R;
-        // column c's rows are contiguous: consecutive threads read consecutive rows; columns
-        // past m (padding of the last column block) read as zero
-        for (uint32_t e = threadIdx.x; e < R * mt * TB; e += GB) {
-            const uint32_t c = e / R, r = e % R;
-            if (c < ld) sh[r * ld + c] = (r < rows && c < m) ? A[c * lda + r0 + r] : 0.0;
+#pragma unroll
+        for (uint32_t q = 0; q < PF; ++q) {
+            const uint32_t e = threadIdx.x + q * GB, c = e / R, r = e % R;
+            pre[q] = (e < n_el && r < rows && c < m) ? A[c * lda + r0 + r] : 0.0;
         }
-        __syncthreads();
+    };
+    auto stage = [&]() { // columns past m (padding of the last column block) are zero
+#pragma unroll
+        for (uint32_t q = 0; q < PF; ++q) {
+            const uint32_t e = threadIdx.x + q * GB, c = e / R, r = e % R;
+            if (e < n_el && c < ld) sh[r * ld + c] = pre[q];
+        }
+    };
+    uint64_t ch = blockIdx.x;
+    if (ch < n_chunks) {
+        fetch(ch);
+        stage();
+    }
+    __syncthreads();
+    for (; ch < n_chunks; ch += gridDim.x) {
+        const bool more = ch + gridDim.x < n_chunks;
+        if (more) fetch(ch + gridDim.x);
         if (on)
             for (uint32_t r = g; r < R; r += rg) {
                 double vi[TB], vj[TB];
@@ -86,6 +108,8 @@ __global__ __launch_bounds__(GB) void k_gram(const double *__restrict__ A, uint6
 #pragma unroll
                     for (uint32_t y = 0; y < TB; ++y) acc[x][y] += vi[x] * vj[y];
             }
+        __syncthreads();
+        if (more) stage();
         __syncthreads();
     }
     // row groups in order, through LDS (reused: the nb * TB^2 block entries fit)
@@ -150,7 +174,10 @@ void gram_device(const double *A, uint64_t n, uint32_t m, uint64_t lda, double *
     const uint32_t R = gram_chunk_rows(m);
     const uint64_t chunks = (n + R - 1) / R;
     const uint32_t nb = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(chunks, GRAM_MAX_BLOCKS));
-    hipLaunchKernelGGL(k_gram, dim3(nb), dim3(GB), 0, st, A, n, m, lda, part);
+    const uint32_t per_thread = (R * ((m + 3) / 4 * 4) + GB - 1) / GB;
+    if (per_thread <= 4) hipLaunchKernelGGL(k_gram<4>, dim3(nb), dim3(GB), 0, st, A, n, m, lda, part);
+    else if (per_thread <= 8) hipLaunchKernelGGL(k_gram<8>, dim3(nb), dim3(GB), 0, st, A, n, m, lda, part);
+    else hipLaunchKernelGGL(k_gram<16>, dim3(nb), dim3(GB), 0, st, A, n, m, lda, part);
     const uint32_t npairs = m * (m + 1) / 2;
     hipLaunchKernelGGL(k_gram_reduce, dim3(npairs), dim3(GB), 0, st, part, nb, m, G);
     WX_HIP(hipGetLastError());

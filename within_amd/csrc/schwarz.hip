@@ -1114,30 +1114,33 @@ std::shared_ptr<SchwarzDev<CT>> build_dev(const Precond::Schwarz &S) {
                 // M_T = (I + L_TT')^{-1} D_T^+ (I + L_TT)^{-1}, one column per unit vector: the
                 // tail's forward substitution, the pivots (zero: the gauge, giving 0) and the
                 // backward substitution, exactly the operator the sparse solve applies
+                // L_TT: each tail row's entries on tail rows (tail-local column, value), in entry order
+                std::vector<uint32_t> tt_ptr(nt + 1, 0), tt_col;
+                std::vector<double> tt_val;
+                for (uint32_t a = 0; a < nt; ++a) {
+                    const uint32_t i = trow[a];
+                    for (uint64_t p = S.l_ptr[ro + i]; p < S.l_ptr[ro + i + 1]; ++p)
+                        if (tix[S.l_idx[p]] >= 0) {
+                            tt_col.push_back((uint32_t)tix[S.l_idx[p]]);
+                            tt_val.push_back(S.l_val[p]);
+                        }
+                    tt_ptr[a + 1] = (uint32_t)tt_col.size();
+                }
                 std::vector<double> M((size_t)nt * nt), y(nt);
                 for (uint32_t sidx = 0; sidx < nt; ++sidx) {
                     std::fill(y.begin(), y.end(), 0.0);
                     y[sidx] = 1.0;
                     for (uint32_t a = 0; a < nt; ++a) {
-                        const uint32_t i = trow[a];
                         double v = y[a];
-                        for (uint64_t p = S.l_ptr[ro + i]; p < S.l_ptr[ro + i + 1]; ++p) {
-                            const int32_t b = tix[S.l_idx[p]];
-                            if (b >= 0) v -= S.l_val[p] * y[b];
-                        }
+                        for (uint32_t q = tt_ptr[a]; q < tt_ptr[a + 1]; ++q) v -= tt_val[q] * y[tt_col[q]];
                         y[a] = v;
                     }
                     for (uint32_t a = 0; a < nt; ++a) {
                         const double dv = S.piv[S.piv_off[c] + trow[a]];
                         y[a] = dv != 0.0 ? y[a] / dv : 0.0;
                     }
-                    for (uint32_t a = nt; a-- > 0;) { // column-oriented: row a is final here
-                        const uint32_t i = trow[a];
-                        for (uint64_t p = S.l_ptr[ro + i]; p < S.l_ptr[ro + i + 1]; ++p) {
-                            const int32_t b = tix[S.l_idx[p]];
-                            if (b >= 0) y[b] -= S.l_val[p] * y[a];
-                        }
-                    }
+                    for (uint32_t a = nt; a-- > 0;) // column-oriented: row a is final here
+                        for (uint32_t q = tt_ptr[a]; q < tt_ptr[a + 1]; ++q) y[tt_col[q]] -= tt_val[q] * y[a];
                     for (uint32_t a = 0; a < nt; ++a) M[(size_t)a * nt + sidx] = y[a];
                 }
                 for (double v : M) t_mat.push_back((CT)v);
