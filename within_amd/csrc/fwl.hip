@@ -1,6 +1,7 @@
 // fwl.hip -- the Frisch-Waugh-Lovell step after demeaning: G = A'A for the demeaned block
 // A = [y~, X~] (README.md:55-66 of the reference, where numpy's lstsq does it on the host).
-// One pass over A in HBM: each workgroup stages chunks of up to 64 rows of A in LDS and
+// One pass over A in HBM: each workgroup stages chunks of up to 64 rows of A in LDS (the next
+// chunk's loads in flight, in registers, while the current chunk's products run) and
 // accumulates the upper triangle of A'A for them, each thread a 4 x 4 block of column pairs over
 // one group of rows; the workgroups' partial triangles are then summed in workgroup order, so
 // the result is deterministic.
