@@ -41,6 +41,7 @@ constexpr uint32_t TAIL_LEVEL_ROWS = 32;  // factor levels this small at the end
 constexpr uint32_t TAIL_MAX_ROWS = 512;   // ... up to this many rows ...
 constexpr uint32_t TAIL_MIN_LEVELS = 8;   // ... and at least this many levels form the dense tail
 constexpr uint32_t TAIL_CHUNK = 32;       // tail-row entries per partial sum
+constexpr uint32_t TAIL_ROW_CHUNKS = 64;  // tail rows with more chunks are combined by the whole workgroup
 
 struct RowItem {
     uint32_t row;   // slot id of the output row
@@ -627,52 +628,68 @@ __global__ __launch_bounds__(SP_BLK) void sk_sparse(const SpMeta *__restrict__ m
                                    red, aux_src, ns, mu, xs, sh, k, kshift);
             __syncthreads();
         }
-        if (m.nt) { // the dense tail: x_T = M_T (rhs_T - L_TF y_F), final (no pivot step)
-            const uint32_t grp = threadIdx.x >> kshift, ngrp = SP_BLK >> kshift;
-            if (on) // partial sums of the tail rows' entries on earlier rows, per chunk
-                for (uint32_t c = grp; c < m.n_tchunks; c += ngrp) {
-                    const TailChunk ch = ta.chunks[m.tchunk_off + c];
-                    ta.csc[(m.tchunk_off + c) * k + j] =
-                        dot_entries<CT, false>(0.0, ch.p0, ch.p1, 1, ta.idx, ta.val, xs, k, j);
-                }
-            __syncthreads();
-            if (on) // reduced right-hand sides: rhs minus the row's chunks in order
-                for (uint32_t a = grp; a < m.nt; a += ngrp) {
-                    const uint64_t ra = m.tail_off + a;
-                    double r = sparse_rhs<CT>(m, ta.rows[ra], red, aux_src, ns, mu, k, j);
-                    const uint64_t c1 = ta.cptr[ra + 1];
-                    uint64_t c = ta.cptr[ra];
-                    for (; c + 8 <= c1; c += 8) { // eight loads in flight, subtracted in order
-                        double v[8];
+        return;
+    }
+    if (phase == 2) { // the dense tail: x_T = M_T (rhs_T - L_TF y_F), final (no pivot step)
+        if (!m.nt) return;
+        const uint32_t grp = threadIdx.x >> kshift, ngrp = SP_BLK >> kshift;
+        // reduced right-hand sides: rhs minus the row's chunk partials (sk_tail_chunks); a row
+        // with few chunks by one row group in chunk order, a hub row's by the whole workgroup
+        // (row groups stride over its chunks, then a fixed LDS tree)
+        if (on)
+            for (uint32_t a = grp; a < m.nt; a += ngrp) {
+                const uint64_t ra = m.tail_off + a;
+                const uint64_t c1 = ta.cptr[ra + 1];
+                uint64_t c = ta.cptr[ra];
+                if (c1 - c > TAIL_ROW_CHUNKS) continue;
+                double r = sparse_rhs<CT>(m, ta.rows[ra], red, aux_src, ns, mu, k, j);
+                for (; c + 8 <= c1; c += 8) { // eight loads in flight, subtracted in order
+                    double v[8];
 #pragma unroll
-                        for (int q = 0; q < 8; ++q) v[q] = ta.csc[(c + q) * k + j];
+                    for (int q = 0; q < 8; ++q) v[q] = ta.csc[(c + q) * k + j];
 #pragma unroll
-                        for (int q = 0; q < 8; ++q) r -= v[q];
-                    }
-                    for (; c < c1; ++c) r -= ta.csc[c * k + j];
-                    ta.tsc[ra * k + j] = r;
+                    for (int q = 0; q < 8; ++q) r -= v[q];
                 }
-            __syncthreads();
+                for (; c < c1; ++c) r -= ta.csc[c * k + j];
+                ta.tsc[ra * k + j] = r;
+            }
+        for (uint32_t a = 0; a < m.nt; ++a) { // uniform over the workgroup
+            const uint64_t ra = m.tail_off + a;
+            const uint64_t c0 = ta.cptr[ra], c1 = ta.cptr[ra + 1];
+            if (c1 - c0 <= TAIL_ROW_CHUNKS) continue;
+            double part = 0.0;
             if (on)
-                for (uint32_t a = grp; a < m.nt; a += ngrp) {
-                    const CT *Mr = ta.mat + m.tmat_off + (uint64_t)a * m.nt;
-                    const double *r = ta.tsc + m.tail_off * k + j;
-                    double acc = 0.0;
-                    uint32_t b = 0;
-                    for (; b + 8 <= m.nt; b += 8) { // eight products' loads in flight, summed in order
-                        double mv[8], rv[8];
-#pragma unroll
-                        for (int q = 0; q < 8; ++q) {
-                            mv[q] = (double)Mr[b + q];
-                            rv[q] = r[(uint64_t)(b + q) * k];
-                        }
-#pragma unroll
-                        for (int q = 0; q < 8; ++q) acc += mv[q] * rv[q];
-                    }
-                    for (; b < m.nt; ++b) acc += (double)Mr[b] * r[(uint64_t)b * k];
-                    xs[(uint64_t)ta.rows[m.tail_off + a] * k + j] = acc;
-                }
+                for (uint64_t c = c0 + grp; c < c1; c += ngrp) part += ta.csc[c * k + j];
+            sh[threadIdx.x] = part;
+            __syncthreads();
+            for (uint32_t s = SP_BLK / 2; s >= kp && s > 0; s >>= 1) {
+                if (threadIdx.x < s) sh[threadIdx.x] += sh[threadIdx.x + s];
+                __syncthreads();
+            }
+            if (threadIdx.x < kp && on)
+                ta.tsc[ra * k + j] = sparse_rhs<CT>(m, ta.rows[ra], red, aux_src, ns, mu, k, j) - sh[threadIdx.x];
+            __syncthreads();
         }
+        __syncthreads();
+        if (on)
+            for (uint32_t a = grp; a < m.nt; a += ngrp) {
+                const CT *Mr = ta.mat + m.tmat_off + (uint64_t)a * m.nt;
+                const double *r = ta.tsc + m.tail_off * k + j;
+                double acc = 0.0;
+                uint32_t b = 0;
+                for (; b + 8 <= m.nt; b += 8) { // eight products' loads in flight, summed in order
+                    double mv[8], rv[8];
+#pragma unroll
+                    for (int q = 0; q < 8; ++q) {
+                        mv[q] = (double)Mr[b + q];
+                        rv[q] = r[(uint64_t)(b + q) * k];
+                    }
+#pragma unroll
+                    for (int q = 0; q < 8; ++q) acc += mv[q] * rv[q];
+                }
+                for (; b < m.nt; ++b) acc += (double)Mr[b] * r[(uint64_t)b * k];
+                xs[(uint64_t)ta.rows[m.tail_off + a] * k + j] = acc;
+            }
         return;
     }
     // (pivots applied by sk_sparse_diag in between) backward substitution with L' (rows of L' =
@@ -682,6 +699,22 @@ __global__ __launch_bounds__(SP_BLK) void sk_sparse(const SpMeta *__restrict__ m
         sparse_level<CT, false>(m, lvl_b_ptr[o], lvl_b_long[o], lvl_b_ptr[o + 1], lvl_b_rows, lt_ptr, lt_idx, lt_val,
                                 red, aux_src, ns, mu, xs, sh, k, kshift);
         __syncthreads();
+    }
+}
+
+// The dense tails' chunk partials, all components across the grid (one row group per chunk,
+// entries summed in order): csc[chunk] = sum of L entries x xs over the chunk's entries on rows
+// before the tail, which the forward phase has solved.
+template <typename CT>
+__global__ __launch_bounds__(BLK) void sk_tail_chunks(TailArgs<CT> ta, uint64_t n_chunks, const double *xs, uint32_t k,
+                                                      uint32_t kshift, const int32_t *__restrict__ gate) {
+    if (gate_closed(gate, k)) return;
+    const uint32_t kp = 1u << kshift, j = threadIdx.x & (kp - 1);
+    if (j >= k) return;
+    const uint64_t stride = ((uint64_t)gridDim.x * BLK) >> kshift;
+    for (uint64_t c = ((uint64_t)blockIdx.x * BLK + threadIdx.x) >> kshift; c < n_chunks; c += stride) {
+        const TailChunk ch = ta.chunks[c];
+        ta.csc[c * k + j] = dot_entries<CT, false>(0.0, ch.p0, ch.p1, 1, ta.idx, ta.val, xs, k, j);
     }
 }
 
@@ -1357,7 +1390,12 @@ struct SchwarzRunImpl : SchwarzRun {
                                    dp<uint32_t>(d.l_idx), dp<CT>(d.l_val), redp, dp<double>(mu2),
                                    dp<uint32_t>(d.aux_src), d.ns, xsp, k, kshift, gate);
             }
-            for (uint32_t phase = 0; phase < 2; ++phase) {
+            const TailArgs<CT> ta{dp<uint32_t>(d.tail_rows), dp<uint64_t>(d.tail_cptr), dp<TailChunk>(d.tail_chunks),
+                                  dp<uint32_t>(d.tail_idx),  dp<CT>(d.tail_val),      dp<CT>(d.tail_mat),
+                                  dp<double>(tsc),           dp<double>(csc)};
+            // forward levels (0), the dense tails' chunks across the grid, the tails (2), the
+            // pivots, the backward levels (1)
+            auto sparse = [&](uint32_t phase) {
                 hipLaunchKernelGGL((sk_sparse<CT>), dim3(d.n_sparse), dim3(SP_BLK), 0, st, dp<SpMeta>(d.sp_meta),
                                    dp<uint64_t>(d.sp_lvl_f_ptr), dp<uint64_t>(d.sp_lvl_f_long),
                                    dp<uint32_t>(d.sp_lvl_f_rows), dp<uint64_t>(d.sp_lvl_b_ptr),
@@ -1365,15 +1403,22 @@ struct SchwarzRunImpl : SchwarzRun {
                                    dp<uint64_t>(d.l_ptr), dp<uint32_t>(d.l_idx), dp<CT>(d.l_val),
                                    dp<uint64_t>(d.lt_ptr), dp<uint32_t>(d.lt_idx), dp<CT>(d.lt_val), dp<CT>(d.piv),
                                    redp, dp<double>(mu2), dp<uint32_t>(d.aux_src), d.ns, xsp, k, kshift, gate, phase,
-                                   TailArgs<CT>{dp<uint32_t>(d.tail_rows), dp<uint64_t>(d.tail_cptr),
-                                                dp<TailChunk>(d.tail_chunks), dp<uint32_t>(d.tail_idx),
-                                                dp<CT>(d.tail_val), dp<CT>(d.tail_mat), dp<double>(tsc),
-                                                dp<double>(csc)});
-                if (phase == 0 && d.n_diag_rows)
-                    hipLaunchKernelGGL((sk_sparse_diag<CT>), dim3(grid_rows(d.n_diag_rows, kshift)), dim3(BLK), 0,
-                                       st, dp<GridRow>(d.diag_rows), d.n_diag_rows, dp<SpMeta>(d.sp_meta),
-                                       dp<CT>(d.piv), xsp, k, kshift, gate);
+                                   ta);
+            };
+            // forward levels, the dense tails (their chunks across the grid, then per component),
+            // the pivots, the backward levels
+            sparse(0);
+            if (d.n_tail_rows) {
+                if (d.n_tail_chunks)
+                    hipLaunchKernelGGL((sk_tail_chunks<CT>), dim3(grid_rows(d.n_tail_chunks, kshift)), dim3(BLK), 0,
+                                       st, ta, d.n_tail_chunks, xsp, k, kshift, gate);
+                sparse(2);
             }
+            if (d.n_diag_rows)
+                hipLaunchKernelGGL((sk_sparse_diag<CT>), dim3(grid_rows(d.n_diag_rows, kshift)), dim3(BLK), 0, st,
+                                   dp<GridRow>(d.diag_rows), d.n_diag_rows, dp<SpMeta>(d.sp_meta), dp<CT>(d.piv), xsp,
+                                   k, kshift, gate);
+            sparse(1);
             for (size_t h = 0; h + 1 < d.grid_b_ptr.size(); ++h) {
                 const uint64_t b = d.grid_b_ptr[h], n = d.grid_b_ptr[h + 1] - b;
                 hipLaunchKernelGGL((sk_sparse_grid<CT, false>), dim3(grid_rows(n, kshift)), dim3(BLK), 0, st,
