@@ -285,6 +285,29 @@ def test_sparse_factor_apply_matches_oracle(gpu, oracle, case):
         np.testing.assert_allclose(r.demeaned, SMALL[f"{case}__demeaned"], atol=1e-8)
 
 
+def test_large_dense_kept_block_is_the_schur_pseudo_inverse(gpu):
+    """Kept blocks above 1,024 levels get their Schur complement S = D_k - C' D_e^-1 C assembled
+    and pseudo-inverted on the GPU (rocSOLVER Cholesky of the grounded minor, projected).  The
+    stored S+ must be numpy's pseudo-inverse of S rebuilt from the exported C and D_e^-1."""
+    import scipy.sparse as sp
+    rng = np.random.default_rng(33)
+    n = 60_000
+    codes = np.asfortranarray(np.column_stack([rng.integers(0, 3000, n), rng.integers(0, 1500, n)]).astype(np.uint32))
+    s = wa.Solver(codes, weights=rng.uniform(0.5, 2.0, n))
+    p = parse_precond(s.preconditioner.to_bytes())
+    assert list(p["fac_kind"]) == [0] and int(p["nk"][0]) == 1500 and p["gauge"][0] == 0
+    ne, nk = int(p["ne"][0]), int(p["nk"][0])
+    cptr = p["c_ptr"][: ne + 1].astype(np.int64)
+    C = sp.csr_matrix((p["c_val"], p["c_idx"].astype(np.int64), cptr), shape=(ne, nk))
+    S = -(C.T @ sp.diags(p["inv_de"][:ne]) @ C).toarray()
+    np.fill_diagonal(S, 0.0)
+    np.fill_diagonal(S, -S.sum(axis=1))  # a Laplacian: rows sum to zero
+    Sp = p["dense"][int(p["dense_off"][0]):int(p["dense_off"][0]) + nk * nk].reshape(nk, nk)
+    ref = np.linalg.pinv(S, rcond=1e-12, hermitian=True)
+    np.testing.assert_allclose(Sp, ref, rtol=1e-7, atol=1e-9 * np.abs(ref).max())
+    np.testing.assert_allclose(S @ Sp @ S, S, rtol=1e-7, atol=1e-9 * np.abs(S).max())
+
+
 def test_sparse_factor_dense_tail_matches_oracle(gpu, oracle, capfd, monkeypatch):
     """A worker-firm panel (Zipf firms) eliminates the hub firms last, in a chain of one-row
     levels; the device solves that tail as one dense product M_T = (I+L_TT')^-1 D_T^+ (I+L_TT)^-1.
