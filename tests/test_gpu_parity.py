@@ -445,6 +445,46 @@ def test_wide_batch_with_small_factor_matches_oracle(gpu, oracle, k):
     assert max(abs(a - b.iterations) for a, b in zip(r.iterations, res)) <= 2
 
 
+ZW_LIVE = [2] + list(range(5, 16))
+
+
+def test_wide_batch_with_stopped_leading_columns(gpu, oracle):
+    """Columns 0 and 1 are zero (stopped before the first iteration), so the wavefront gather's
+    lanes of that column pair never enter its loop, and the other pairs' level sums must not
+    depend on them; column 4 is zero beside a live column 5, and column 3 (which the fixed
+    effects fit exactly) runs longer than most."""
+    rng = np.random.default_rng(97)
+    n, k = 60_000, 16
+    p = (np.arange(3000) + 1.0) ** -1.1
+    p /= p.sum()
+    codes = np.asfortranarray(np.column_stack([rng.choice(3000, n, p=p), rng.integers(0, 400, n),
+                                               rng.integers(0, 12, n)]).astype(np.uint32))
+    w = rng.uniform(0.5, 2.0, n)
+    Y = np.asfortranarray(rng.standard_normal((n, k)))
+    Y[:, 0] = Y[:, 1] = Y[:, 4] = 0.0
+    Y[:, 3] = rng.standard_normal(3000)[codes[:, 0]] + rng.standard_normal(400)[codes[:, 1]]
+    print(f"[wide batch, stopped leading columns] k={k}", flush=True)
+    opts = wa.LsmrOptions(tol=1e-10, maxiter=5000)
+    for pc in (wa.PreconditionerConfig.Diagonal, None):
+        r = wa.solve_batch(codes, Y, weights=w, options=opts, preconditioner=pc)
+        assert all(r.converged)
+        assert r.iterations[0] == r.iterations[1] == r.iterations[4] == 0
+        ex = exact_demean(sparse_design(codes), Y, w)
+        for j in (0, 1, 4):
+            assert np.array_equal(r.demeaned[:, j], np.zeros(n))
+        assert np.abs(r.demeaned[:, 3]).max() <= 1e-7 * np.abs(Y[:, 3]).max()
+        for j in ZW_LIVE:
+            assert rel(r.demeaned[:, j], ex[:, j]) <= 1e-8, j
+        # each column as its own solve
+        for j in (2, 7, 15):
+            one = wa.solve_batch(codes, Y[:, [j]], weights=w, options=opts, preconditioner=pc)
+            assert rel(r.demeaned[:, j], one.demeaned[:, 0]) <= 1e-9, j
+    _, Dm, res = oracle.solve_batch(oracle.design(codes), Y, w, prec="diagonal", tol=1e-10, maxiter=5000, threads=8)
+    r = wa.solve_batch(codes, Y, weights=w, options=opts, preconditioner=wa.PreconditionerConfig.Diagonal)
+    for j in ZW_LIVE:
+        assert rel(r.demeaned[:, j], Dm[:, j]) <= 1e-8, j
+
+
 @pytest.mark.parametrize("prec", ["off", "diagonal", "schwarz"])
 @pytest.mark.parametrize("local_size", [1, 5, 40])
 def test_local_size_matches_oracle(gpu, oracle, prec, local_size):

@@ -351,10 +351,12 @@ __global__ __launch_bounds__(BLK) __attribute__((amdgpu_waves_per_eu(4, 8))) voi
                 // the pass crosses levels: the first row group carries level cur's sum so far
                 // and adds the pass's rows in order, writing each level that ends
                 adj = sum_over_groups(adj, kv);
+                // every lane of a row group writes the row's level (the same value): a column
+                // pair that has stopped skips this loop, so no one pair can be relied on for it
 #pragma unroll
                 for (int q = 0; q < S; ++q) {
                     cbuf[wv][q * WAVE + lane] = cq[q];
-                    if (jv == 0) lbuf[wv][q * G + g] = lv[q];
+                    lbuf[wv][q * G + g] = lv[q];
                 }
                 wave_sync();
                 if (g == 0) {
