@@ -91,6 +91,7 @@ struct SchwarzDev {
     uint64_t n_diag_rows = 0;
     DevBuf l_ptr, l_idx, l_val, lt_ptr, lt_idx, lt_val, piv;
     DevBuf tail_rows, tail_cptr, tail_chunks, tail_idx, tail_val, tail_mat; // dense tails
+    DevBuf tail_prod; // TailProdRow of each tail row
     uint64_t n_tail_rows = 0, n_tail_chunks = 0;
     uint32_t n_sparse = 0;
     // auxiliary slots [ns, ns + n_aux) of the covers: real kept slot | sheet bit (second sheet)
@@ -129,6 +130,14 @@ struct TailChunk {
     uint32_t row;     // tail-local row
     uint32_t pad;
     uint64_t p0, p1;  // into tail_idx / tail_val
+};
+
+// A tail row's place in its component's M_T product: its row of M_T (offset into tail_mat), the
+// component's tail size and the offset of its tail rows (into tail_rows and tsc)
+struct TailProdRow {
+    uint64_t mrow;
+    uint64_t toff;
+    uint32_t nt, pad;
 };
 
 // Dense tails of the sparse factors (device pointers) and their scratch: per tail row the
@@ -630,7 +639,7 @@ __global__ __launch_bounds__(SP_BLK) void sk_sparse(const SpMeta *__restrict__ m
         }
         return;
     }
-    if (phase == 2) { // the dense tail: x_T = M_T (rhs_T - L_TF y_F), final (no pivot step)
+    if (phase == 2) { // the dense tail's reduced right-hand side tsc = rhs_T - L_TF y_F
         if (!m.nt) return;
         const uint32_t grp = threadIdx.x >> kshift, ngrp = SP_BLK >> kshift;
         // reduced right-hand sides: rhs minus the row's chunk partials (sk_tail_chunks); a row
@@ -670,27 +679,7 @@ __global__ __launch_bounds__(SP_BLK) void sk_sparse(const SpMeta *__restrict__ m
                 ta.tsc[ra * k + j] = sparse_rhs<CT>(m, ta.rows[ra], red, aux_src, ns, mu, k, j) - sh[threadIdx.x];
             __syncthreads();
         }
-        __syncthreads();
-        if (on)
-            for (uint32_t a = grp; a < m.nt; a += ngrp) {
-                const CT *Mr = ta.mat + m.tmat_off + (uint64_t)a * m.nt;
-                const double *r = ta.tsc + m.tail_off * k + j;
-                double acc = 0.0;
-                uint32_t b = 0;
-                for (; b + 8 <= m.nt; b += 8) { // eight products' loads in flight, summed in order
-                    double mv[8], rv[8];
-#pragma unroll
-                    for (int q = 0; q < 8; ++q) {
-                        mv[q] = (double)Mr[b + q];
-                        rv[q] = r[(uint64_t)(b + q) * k];
-                    }
-#pragma unroll
-                    for (int q = 0; q < 8; ++q) acc += mv[q] * rv[q];
-                }
-                for (; b < m.nt; ++b) acc += (double)Mr[b] * r[(uint64_t)b * k];
-                xs[(uint64_t)ta.rows[m.tail_off + a] * k + j] = acc;
-            }
-        return;
+        return; // x_T = M_T tsc follows across the grid (sk_tail_product)
     }
     // (pivots applied by sk_sparse_diag in between) backward substitution with L' (rows of L' =
     // columns of L); the last nbig_b levels are solved by grid-wide launches after this kernel
@@ -716,6 +705,39 @@ __global__ __launch_bounds__(BLK) void sk_tail_chunks(TailArgs<CT> ta, uint64_t 
         const TailChunk ch = ta.chunks[c];
         ta.csc[c * k + j] = dot_entries<CT, false>(0.0, ch.p0, ch.p1, 1, ta.idx, ta.val, xs, k, j);
     }
+}
+
+// The dense tails' products x_T = M_T tsc of every component across the grid (final: no pivot
+// step follows): one wave per (tail row, column); lane t takes the products b = t, t + 64, ...
+// (eight loads in flight), and a fixed shuffle tree adds the lanes' sums.
+template <typename CT>
+__global__ __launch_bounds__(BLK) void sk_tail_product(TailArgs<CT> ta, const TailProdRow *__restrict__ prod,
+                                                       uint64_t n_rows, double *__restrict__ xs, uint32_t k,
+                                                       const int32_t *__restrict__ gate) {
+    if (gate_closed(gate, k)) return;
+    const uint32_t lane = threadIdx.x & 63;
+    const uint64_t task = ((uint64_t)blockIdx.x * BLK + threadIdx.x) >> 6; // wave-uniform
+    if (task >= n_rows * k) return;
+    const uint64_t ra = task / k;
+    const uint32_t j = (uint32_t)(task % k);
+    const TailProdRow p = prod[ra];
+    const CT *Mr = ta.mat + p.mrow;
+    const double *r = ta.tsc + p.toff * k + j;
+    double acc = 0.0;
+    uint32_t b = lane;
+    for (; b + 7 * 64 < p.nt; b += 8 * 64) {
+        double mv[8], rv[8];
+#pragma unroll
+        for (int q = 0; q < 8; ++q) {
+            mv[q] = (double)Mr[b + q * 64];
+            rv[q] = r[(uint64_t)(b + q * 64) * k];
+        }
+#pragma unroll
+        for (int q = 0; q < 8; ++q) acc += mv[q] * rv[q];
+    }
+    for (; b < p.nt; b += 64) acc += (double)Mr[b] * r[(uint64_t)b * k];
+    for (int off = 32; off > 0; off >>= 1) acc += __shfl_xor(acc, off);
+    if (lane == 0) xs[(uint64_t)ta.rows[ra] * k + j] = acc;
 }
 
 // One large level of every component that has it, all rows in parallel across the grid (one
@@ -1026,6 +1048,7 @@ std::shared_ptr<SchwarzDev<CT>> build_dev(const Precond::Schwarz &S) {
         std::vector<uint32_t> lidx, ltidx, lvf_rows, lvb_rows;
         // dense tails: rows (slots), per-row first chunk, chunks, entries on earlier rows, M_T
         std::vector<uint32_t> t_rows, t_idx;
+        std::vector<TailProdRow> t_prod;
         std::vector<uint64_t> t_cptr;
         std::vector<TailChunk> t_chunks;
         std::vector<CT> t_val, t_mat;
@@ -1132,6 +1155,7 @@ std::shared_ptr<SchwarzDev<CT>> build_dev(const Precond::Schwarz &S) {
                 // entries of the tail rows on rows before the tail, in entry order, chunked
                 for (uint32_t a = 0; a < nt; ++a) {
                     const uint32_t i = trow[a];
+                    t_prod.push_back({m.tmat_off + (uint64_t)a * nt, m.tail_off, nt, 0u});
                     t_rows.push_back(kb + i);
                     t_cptr.push_back(t_chunks.size());
                     uint64_t p0 = t_idx.size();
@@ -1299,6 +1323,7 @@ std::shared_ptr<SchwarzDev<CT>> build_dev(const Precond::Schwarz &S) {
         D->tail_idx.upload(t_idx.data(), t_idx.size());
         D->tail_val.upload(t_val.data(), t_val.size());
         D->tail_mat.upload(t_mat.data(), t_mat.size());
+        D->tail_prod.upload(t_prod.data(), t_prod.size());
         D->sp_lvl_f_ptr.upload(lvf_ptr.data(), lvf_ptr.size());
         D->sp_lvl_f_long.upload(lvf_long.data(), lvf_long.size());
         D->sp_lvl_b_long.upload(lvb_long.data(), lvb_long.size());
@@ -1413,6 +1438,9 @@ struct SchwarzRunImpl : SchwarzRun {
                     hipLaunchKernelGGL((sk_tail_chunks<CT>), dim3(grid_rows(d.n_tail_chunks, kshift)), dim3(BLK), 0,
                                        st, ta, d.n_tail_chunks, xsp, k, kshift, gate);
                 sparse(2);
+                const uint64_t waves = d.n_tail_rows * k;
+                hipLaunchKernelGGL((sk_tail_product<CT>), dim3((uint32_t)((waves + BLK / 64 - 1) / (BLK / 64))),
+                                   dim3(BLK), 0, st, ta, dp<TailProdRow>(d.tail_prod), d.n_tail_rows, xsp, k, gate);
             }
             if (d.n_diag_rows)
                 hipLaunchKernelGGL((sk_sparse_diag<CT>), dim3(grid_rows(d.n_diag_rows, kshift)), dim3(BLK), 0, st,
