@@ -41,7 +41,6 @@ constexpr uint32_t TAIL_LEVEL_ROWS = 32;  // factor levels this small at the end
 constexpr uint32_t TAIL_MAX_ROWS = 512;   // ... up to this many rows ...
 constexpr uint32_t TAIL_MIN_LEVELS = 8;   // ... and at least this many levels form the dense tail
 constexpr uint32_t TAIL_CHUNK = 32;       // tail-row entries per partial sum
-constexpr uint32_t TAIL_ROW_CHUNKS = 64;  // tail rows with more chunks are combined by the whole workgroup
 
 struct RowItem {
     uint32_t row;   // slot id of the output row
@@ -132,12 +131,13 @@ struct TailChunk {
     uint64_t p0, p1;  // into tail_idx / tail_val
 };
 
-// A tail row's place in its component's M_T product: its row of M_T (offset into tail_mat), the
-// component's tail size and the offset of its tail rows (into tail_rows and tsc)
+// A tail row's place in its component's dense tail: its row of M_T (offset into tail_mat), the
+// offset of the component's tail rows (into tail_rows and tsc), the tail size and the
+// component's SpMeta
 struct TailProdRow {
     uint64_t mrow;
     uint64_t toff;
-    uint32_t nt, pad;
+    uint32_t nt, meta;
 };
 
 // Dense tails of the sparse factors (device pointers) and their scratch: per tail row the
@@ -620,8 +620,7 @@ __global__ __launch_bounds__(SP_BLK) void sk_sparse(const SpMeta *__restrict__ m
                                                  const CT *__restrict__ piv, const double *__restrict__ red,
                                                  const double *__restrict__ mu2, const uint32_t *__restrict__ aux_src,
                                                  uint64_t ns, double *xs, uint32_t k,
-                                                 uint32_t kshift, const int32_t *__restrict__ gate, uint32_t phase,
-                                                 TailArgs<CT> ta) {
+                                                 uint32_t kshift, const int32_t *__restrict__ gate, uint32_t phase) {
     if (gate_closed(gate, k)) return;
     __shared__ double sh[SP_BLK];
     const uint32_t kp = 1u << kshift, j = threadIdx.x & (kp - 1);
@@ -638,48 +637,6 @@ __global__ __launch_bounds__(SP_BLK) void sk_sparse(const SpMeta *__restrict__ m
             __syncthreads();
         }
         return;
-    }
-    if (phase == 2) { // the dense tail's reduced right-hand side tsc = rhs_T - L_TF y_F
-        if (!m.nt) return;
-        const uint32_t grp = threadIdx.x >> kshift, ngrp = SP_BLK >> kshift;
-        // reduced right-hand sides: rhs minus the row's chunk partials (sk_tail_chunks); a row
-        // with few chunks by one row group in chunk order, a hub row's by the whole workgroup
-        // (row groups stride over its chunks, then a fixed LDS tree)
-        if (on)
-            for (uint32_t a = grp; a < m.nt; a += ngrp) {
-                const uint64_t ra = m.tail_off + a;
-                const uint64_t c1 = ta.cptr[ra + 1];
-                uint64_t c = ta.cptr[ra];
-                if (c1 - c > TAIL_ROW_CHUNKS) continue;
-                double r = sparse_rhs<CT>(m, ta.rows[ra], red, aux_src, ns, mu, k, j);
-                for (; c + 8 <= c1; c += 8) { // eight loads in flight, subtracted in order
-                    double v[8];
-#pragma unroll
-                    for (int q = 0; q < 8; ++q) v[q] = ta.csc[(c + q) * k + j];
-#pragma unroll
-                    for (int q = 0; q < 8; ++q) r -= v[q];
-                }
-                for (; c < c1; ++c) r -= ta.csc[c * k + j];
-                ta.tsc[ra * k + j] = r;
-            }
-        for (uint32_t a = 0; a < m.nt; ++a) { // uniform over the workgroup
-            const uint64_t ra = m.tail_off + a;
-            const uint64_t c0 = ta.cptr[ra], c1 = ta.cptr[ra + 1];
-            if (c1 - c0 <= TAIL_ROW_CHUNKS) continue;
-            double part = 0.0;
-            if (on)
-                for (uint64_t c = c0 + grp; c < c1; c += ngrp) part += ta.csc[c * k + j];
-            sh[threadIdx.x] = part;
-            __syncthreads();
-            for (uint32_t s = SP_BLK / 2; s >= kp && s > 0; s >>= 1) {
-                if (threadIdx.x < s) sh[threadIdx.x] += sh[threadIdx.x + s];
-                __syncthreads();
-            }
-            if (threadIdx.x < kp && on)
-                ta.tsc[ra * k + j] = sparse_rhs<CT>(m, ta.rows[ra], red, aux_src, ns, mu, k, j) - sh[threadIdx.x];
-            __syncthreads();
-        }
-        return; // x_T = M_T tsc follows across the grid (sk_tail_product)
     }
     // (pivots applied by sk_sparse_diag in between) backward substitution with L' (rows of L' =
     // columns of L); the last nbig_b levels are solved by grid-wide launches after this kernel
@@ -704,6 +661,40 @@ __global__ __launch_bounds__(BLK) void sk_tail_chunks(TailArgs<CT> ta, uint64_t 
     for (uint64_t c = ((uint64_t)blockIdx.x * BLK + threadIdx.x) >> kshift; c < n_chunks; c += stride) {
         const TailChunk ch = ta.chunks[c];
         ta.csc[c * k + j] = dot_entries<CT, false>(0.0, ch.p0, ch.p1, 1, ta.idx, ta.val, xs, k, j);
+    }
+}
+
+// The dense tails' reduced right-hand sides tsc = rhs_T - L_TF y_F of every component across the
+// grid: one wave per (tail row, column); lane t adds the row's chunk partials t, t + 64, ...
+// (sk_tail_chunks), and a fixed shuffle tree adds the lanes' sums.
+template <typename CT>
+__global__ __launch_bounds__(BLK) void sk_tail_rhs(TailArgs<CT> ta, const TailProdRow *__restrict__ prod,
+                                                   uint64_t n_rows, const SpMeta *__restrict__ meta,
+                                                   const double *__restrict__ red, const double *__restrict__ mu2,
+                                                   const uint32_t *__restrict__ aux_src, uint64_t ns, uint32_t k,
+                                                   const int32_t *__restrict__ gate) {
+    if (gate_closed(gate, k)) return;
+    const uint32_t lane = threadIdx.x & 63;
+    const uint64_t task = ((uint64_t)blockIdx.x * BLK + threadIdx.x) >> 6; // wave-uniform
+    if (task >= n_rows * k) return;
+    const uint64_t ra = task / k;
+    const uint32_t j = (uint32_t)(task % k);
+    const uint64_t c0 = ta.cptr[ra], c1 = ta.cptr[ra + 1];
+    double part = 0.0;
+    uint64_t c = c0 + lane;
+    for (; c + 7 * 64 < c1; c += 8 * 64) {
+        double v[8];
+#pragma unroll
+        for (int q = 0; q < 8; ++q) v[q] = ta.csc[(c + q * 64) * k + j];
+#pragma unroll
+        for (int q = 0; q < 8; ++q) part += v[q];
+    }
+    for (; c < c1; c += 64) part += ta.csc[c * k + j];
+    for (int off = 32; off > 0; off >>= 1) part += __shfl_xor(part, off);
+    if (lane == 0) {
+        const SpMeta m = meta[prod[ra].meta];
+        const double mu = mu2[(uint64_t)m.comp * k + j];
+        ta.tsc[ra * k + j] = sparse_rhs<CT>(m, ta.rows[ra], red, aux_src, ns, mu, k, j) - part;
     }
 }
 
@@ -1155,7 +1146,7 @@ std::shared_ptr<SchwarzDev<CT>> build_dev(const Precond::Schwarz &S) {
                 // entries of the tail rows on rows before the tail, in entry order, chunked
                 for (uint32_t a = 0; a < nt; ++a) {
                     const uint32_t i = trow[a];
-                    t_prod.push_back({m.tmat_off + (uint64_t)a * nt, m.tail_off, nt, 0u});
+                    t_prod.push_back({m.tmat_off + (uint64_t)a * nt, m.tail_off, nt, (uint32_t)metas.size()});
                     t_rows.push_back(kb + i);
                     t_cptr.push_back(t_chunks.size());
                     uint64_t p0 = t_idx.size();
@@ -1418,8 +1409,6 @@ struct SchwarzRunImpl : SchwarzRun {
             const TailArgs<CT> ta{dp<uint32_t>(d.tail_rows), dp<uint64_t>(d.tail_cptr), dp<TailChunk>(d.tail_chunks),
                                   dp<uint32_t>(d.tail_idx),  dp<CT>(d.tail_val),      dp<CT>(d.tail_mat),
                                   dp<double>(tsc),           dp<double>(csc)};
-            // forward levels (0), the dense tails' chunks across the grid, the tails (2), the
-            // pivots, the backward levels (1)
             auto sparse = [&](uint32_t phase) {
                 hipLaunchKernelGGL((sk_sparse<CT>), dim3(d.n_sparse), dim3(SP_BLK), 0, st, dp<SpMeta>(d.sp_meta),
                                    dp<uint64_t>(d.sp_lvl_f_ptr), dp<uint64_t>(d.sp_lvl_f_long),
@@ -1427,20 +1416,21 @@ struct SchwarzRunImpl : SchwarzRun {
                                    dp<uint64_t>(d.sp_lvl_b_long), dp<uint32_t>(d.sp_lvl_b_rows),
                                    dp<uint64_t>(d.l_ptr), dp<uint32_t>(d.l_idx), dp<CT>(d.l_val),
                                    dp<uint64_t>(d.lt_ptr), dp<uint32_t>(d.lt_idx), dp<CT>(d.lt_val), dp<CT>(d.piv),
-                                   redp, dp<double>(mu2), dp<uint32_t>(d.aux_src), d.ns, xsp, k, kshift, gate, phase,
-                                   ta);
+                                   redp, dp<double>(mu2), dp<uint32_t>(d.aux_src), d.ns, xsp, k, kshift, gate, phase);
             };
-            // forward levels, the dense tails (their chunks across the grid, then per component),
-            // the pivots, the backward levels
+            // forward levels (phase 0), the dense tails across the grid (chunk partials, reduced
+            // right-hand sides, M_T products), the pivots, the backward levels (phase 1)
             sparse(0);
             if (d.n_tail_rows) {
                 if (d.n_tail_chunks)
                     hipLaunchKernelGGL((sk_tail_chunks<CT>), dim3(grid_rows(d.n_tail_chunks, kshift)), dim3(BLK), 0,
                                        st, ta, d.n_tail_chunks, xsp, k, kshift, gate);
-                sparse(2);
-                const uint64_t waves = d.n_tail_rows * k;
-                hipLaunchKernelGGL((sk_tail_product<CT>), dim3((uint32_t)((waves + BLK / 64 - 1) / (BLK / 64))),
-                                   dim3(BLK), 0, st, ta, dp<TailProdRow>(d.tail_prod), d.n_tail_rows, xsp, k, gate);
+                const uint32_t tgrid = (uint32_t)((d.n_tail_rows * k + BLK / 64 - 1) / (BLK / 64));
+                hipLaunchKernelGGL((sk_tail_rhs<CT>), dim3(tgrid), dim3(BLK), 0, st, ta, dp<TailProdRow>(d.tail_prod),
+                                   d.n_tail_rows, dp<SpMeta>(d.sp_meta), redp, dp<double>(mu2),
+                                   dp<uint32_t>(d.aux_src), d.ns, k, gate);
+                hipLaunchKernelGGL((sk_tail_product<CT>), dim3(tgrid), dim3(BLK), 0, st, ta,
+                                   dp<TailProdRow>(d.tail_prod), d.n_tail_rows, xsp, k, gate);
             }
             if (d.n_diag_rows)
                 hipLaunchKernelGGL((sk_sparse_diag<CT>), dim3(grid_rows(d.n_diag_rows, kshift)), dim3(BLK), 0, st,
