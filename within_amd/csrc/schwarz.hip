@@ -37,8 +37,8 @@ constexpr uint32_t DENSE_TILED_MIN = 256; // kept blocks at least this large use
 constexpr uint64_t DENSE_TILE_TARGET = 8192; // (row tile, column segment) pairs wanted in flight
 constexpr uint64_t LONG_ROW = 64;         // factor rows with more entries are summed by a whole workgroup
 constexpr uint64_t BIG_LEVEL = 256;       // factor levels with at least this many rows use grid-wide launches
-constexpr uint32_t TAIL_LEVEL_ROWS = 32;  // factor levels this small at the end of the schedule ...
-constexpr uint32_t TAIL_MAX_ROWS = 512;   // ... up to this many rows ...
+constexpr uint32_t TAIL_LEVEL_ROWS = 64;  // factor levels this small at the end of the schedule ...
+constexpr uint32_t TAIL_MAX_ROWS = 1024;  // ... up to this many rows ...
 constexpr uint32_t TAIL_MIN_LEVELS = 8;   // ... and at least this many levels form the dense tail
 constexpr uint32_t TAIL_CHUNK = 32;       // tail-row entries per partial sum
 
