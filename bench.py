@@ -139,7 +139,7 @@ def cpu_baseline(solver, codes, w, Y, seconds_budget=30.0):
     # sampled Schur + approximate Cholesky otherwise), built by this library's host builder
     import within_amd as wa
     from within_amd._within import build_preconditioner
-    cfg = wa.config.AdditiveSchwarz(wa.config.LocalSolverConfig(dense_threshold=24))
+    cfg = wa.config.AdditiveSchwarz(wa.config.LocalSolverConfig(dense_threshold=24, dense_policy="reference"))
     p = build_preconditioner(codes, w, cfg)
     sch = oracle_schwarz_from_bytes(lib, p.to_bytes())
     d = lib.design(codes)

@@ -73,11 +73,16 @@ typedef enum wx_precond_kind {
 /* Schur mode, crates/within/src/config.rs:47-59. */
 typedef enum wx_schur_mode { WX_SCHUR_APPROXIMATE = 0, WX_SCHUR_EXACT = 1 } wx_schur_mode;
 
+/* Which kept blocks get the dense exact factor.  The reference rule (block_elim/solver.rs:311-347)
+ * always applies: dense iff n_kept <= dense_threshold.  WX_DENSE_DEVICE (the default) also
+ * factors a kept block densely where the device cost model finds the dense apply cheaper (up
+ * to 4,096 kept levels, or up to 16,384 when the reduced graph is at least 0.5 % dense); the
+ * choice depends on the config and the design only, never on whether a GPU is visible. */
+typedef enum wx_dense_policy { WX_DENSE_REFERENCE = 0, WX_DENSE_DEVICE = 1 } wx_dense_policy;
+
 /* LocalSolverConfig / ApproxCholConfig / ApproxSchurConfig / ScalingConfig,
- * crates/within/src/config.rs:20-131.  `dense_threshold < 0` selects the device cost
- * model (the default): a kept block is factored densely and exactly when that is the
- * cheaper device apply; otherwise approximate Cholesky is used.  A value >= 0 keeps the
- * reference rule (dense exact factor iff n_kept <= dense_threshold). */
+ * crates/within/src/config.rs:20-131 (defaults: dense_threshold 24, dense_policy
+ * WX_DENSE_DEVICE).  dense_threshold < 0 disables the reference rule. */
 typedef struct wx_precond_config {
     int32_t kind;              /* wx_precond_kind */
     int32_t schur;             /* wx_schur_mode */
@@ -91,6 +96,7 @@ typedef struct wx_precond_config {
     int32_t scaling_error_on_failure; /* 0 Warn, 1 Error */
     int32_t reduction;         /* ReductionStrategy; accepted, the device combine is always
                                   the deterministic gather */
+    int32_t dense_policy;      /* wx_dense_policy; no reference counterpart */
 } wx_precond_config;
 
 /* LsmrOptions, crates/within/src/config.rs:160-178, plus the device precision. */
@@ -130,6 +136,13 @@ uint64_t wx_last_timing(double *out, uint64_t n);
 
 /* Thread-local message of the last failed call on this thread ("" if none). */
 const char *wx_last_error(void);
+
+/* Name of the reference error variant the last failed call on this thread stands for, as
+ * "BuildError::InvalidWeight", "SolveError::InvalidInput", ... (crates/within/src/error.rs:12-146,
+ * crates/schwarz-precond/src/error.rs:22-90), so that a binding can rebuild the exact variant
+ * that a wx_status collapses; "" when the failure has no reference counterpart (a HIP runtime
+ * failure, WX_ERR_DEVICE) or no call failed. */
+const char *wx_last_error_variant(void);
 
 /* Library / device information: fills `name` (NUL terminated) and returns the
  * device count seen by the HIP runtime (0 when no GPU is visible). */

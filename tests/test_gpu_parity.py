@@ -270,7 +270,7 @@ def test_single_factor_runs_plain_lsmr(gpu):
 def test_sparse_factor_apply_matches_oracle(gpu, oracle, case):
     # dense_threshold=0 routes every kept block through approximate Cholesky + level-scheduled solves
     codes, y, w = small(case)
-    cfg = wa.config.AdditiveSchwarz(wa.config.LocalSolverConfig(dense_threshold=0))
+    cfg = wa.config.AdditiveSchwarz(wa.config.LocalSolverConfig(dense_threshold=0, dense_policy="reference"))
     s = wa.Solver(codes, weights=w, preconditioner=cfg)
     p = s.preconditioner
     sch = oracle_schwarz_from_bytes(oracle, p.to_bytes())
@@ -324,7 +324,7 @@ def test_sparse_factor_dense_tail_matches_oracle(gpu, oracle, capfd, monkeypatch
         firm[:, t] = np.where(move, rng.integers(0, nf, nw), firm[:, t - 1])
     codes = np.asfortranarray(np.column_stack([np.repeat(np.arange(nw), T), firm.ravel(),
                                                np.tile(np.arange(T), nw)]).astype(np.uint32))
-    cfg = wa.config.AdditiveSchwarz(wa.config.LocalSolverConfig(dense_threshold=0))
+    cfg = wa.config.AdditiveSchwarz(wa.config.LocalSolverConfig(dense_threshold=0, dense_policy="reference"))
     s = wa.Solver(codes, preconditioner=cfg)
     pc = s.preconditioner
     x = rng.standard_normal(s.n_dofs)
@@ -358,10 +358,18 @@ def _slope_design(seed, n=3000, la=80, lb=30, lc=4):
     return rng, design, D, Dw, w
 
 
+def _local(dense_threshold):
+    """None: the default local solver (device dense policy); 0: approximate Cholesky for every
+    kept block (the reference rule with threshold 0)."""
+    if dense_threshold is None:
+        return wa.config.LocalSolverConfig()
+    return wa.config.LocalSolverConfig(dense_threshold=dense_threshold, dense_policy="reference")
+
+
 @pytest.mark.parametrize("dense_threshold", [None, 0])
 def test_slope_schwarz_apply_matches_oracle(gpu, oracle, dense_threshold):
     rng, design, _, _, w = _slope_design(51)
-    cfg = wa.config.AdditiveSchwarz(wa.config.LocalSolverConfig(dense_threshold=dense_threshold))
+    cfg = wa.config.AdditiveSchwarz(_local(dense_threshold))
     s = wa.Solver(design, weights=w, preconditioner=cfg)
     p = s.preconditioner
     kinds = set(parse_precond(p.to_bytes())["fac_kind"].tolist())
@@ -377,7 +385,7 @@ def test_slope_schwarz_apply_matches_oracle(gpu, oracle, dense_threshold):
 def test_slope_additive_solve_matches_exact(gpu, dense_threshold, precision):
     rng, design, D, Dw, w = _slope_design(52)
     Y = np.asfortranarray(rng.standard_normal((D.shape[0], 3)))
-    cfg = wa.config.AdditiveSchwarz(wa.config.LocalSolverConfig(dense_threshold=dense_threshold))
+    cfg = wa.config.AdditiveSchwarz(_local(dense_threshold))
     s = wa.Solver(design, weights=w, preconditioner=cfg)
     r = s.solve_batch(Y, wa.LsmrOptions(tol=1e-10, precision=precision))
     assert all(r.converged)

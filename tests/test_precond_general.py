@@ -84,7 +84,7 @@ def test_slope_design_sparse_factors_are_spd(oracle):
     z = rng.standard_normal(n)
     w = rng.uniform(0.5, 2.0, n)
     design = [wa.Effect(a, True), wa.Effect(b, True, [z])]
-    cfg = wa.config.AdditiveSchwarz(wa.config.LocalSolverConfig(dense_threshold=0))
+    cfg = wa.config.AdditiveSchwarz(wa.config.LocalSolverConfig(dense_threshold=0, dense_policy="reference"))
     pre = build_preconditioner(design, w, cfg)
     p = parse_precond(pre.to_bytes())
     assert 2 in set(p["fac_kind"].tolist())  # a frustrated slope component uses the cover
@@ -102,7 +102,7 @@ def test_preconditioner_bytes_round_trip_general():
     b = rng.integers(0, 15, n).astype(np.uint32)
     design = [wa.Effect(a, True), wa.Effect(b, True, [rng.standard_normal(n)])]
     for thr in (0, 10 ** 6):
-        cfg = wa.config.AdditiveSchwarz(wa.config.LocalSolverConfig(dense_threshold=thr))
+        cfg = wa.config.AdditiveSchwarz(wa.config.LocalSolverConfig(dense_threshold=thr, dense_policy="reference"))
         pre = build_preconditioner(design, None, cfg)
         b1 = pre.to_bytes()
         assert wa.Preconditioner(b1).to_bytes() == b1

@@ -16,7 +16,7 @@ def _blobs():
     b = (rng.pareto(1.2, n) * 4).astype(np.int64).clip(0, 59).astype(np.uint32)
     c = rng.integers(0, 6, n).astype(np.uint32)
     w = rng.uniform(0.5, 2.0, n)
-    sparse = wa.config.AdditiveSchwarz(wa.config.LocalSolverConfig(dense_threshold=0))
+    sparse = wa.config.AdditiveSchwarz(wa.config.LocalSolverConfig(dense_threshold=0, dense_policy="reference"))
     out = {
         "dense": build_preconditioner(np.asfortranarray(np.column_stack([a, b, c])), weights=w),
         "sparse": build_preconditioner(np.asfortranarray(np.column_stack([a, b, c])), weights=w,

@@ -42,6 +42,7 @@ class PrecondConfig(C.Structure):
         ("scaling_max_sweeps", C.c_int64),
         ("scaling_error_on_failure", C.c_int32),
         ("reduction", C.c_int32),
+        ("dense_policy", C.c_int32),
     ]
 
 
@@ -70,6 +71,7 @@ class RhsStats(C.Structure):
 
 EXPORTS = {
     "wx_last_error": ([], C.c_char_p),
+    "wx_last_error_variant": ([], C.c_char_p),
     "wx_last_timing": ([C.c_void_p, C.c_uint64], C.c_uint64),
     "wx_device_info": ([C.c_char_p, C.c_uint64], C.c_int32),
     "wx_precond_config_default": ([C.POINTER(PrecondConfig)], None),
@@ -141,11 +143,16 @@ def check(status: int) -> None:
     if status == WX_OK:
         return
     msg = lib().wx_last_error().decode("utf-8", "replace")
+    variant = lib().wx_last_error_variant().decode("ascii", "replace")
     if status in (WX_ERR_INVALID_INPUT, WX_ERR_BUILD, WX_ERR_NOT_POSITIVE_DEF):
-        raise ValueError(msg)
-    if status == WX_ERR_INDEX:
-        raise IndexError(msg)
-    raise RuntimeError(msg)
+        err = ValueError(msg)
+    elif status == WX_ERR_INDEX:
+        err = IndexError(msg)
+    else:
+        err = RuntimeError(msg)
+    # the reference error variant the status stands for ("BuildError::InvalidWeight", ...)
+    err.variant = variant
+    raise err
 
 
 def device_info() -> tuple[int, str]:

@@ -201,27 +201,39 @@ class ScalingConfig:
 class LocalSolverConfig:
     """Local solver: Schur reduction + factorisation of the kept block.
 
-    ``dense_threshold=None`` selects the device cost model (dense exact factor when that is
-    the cheaper device apply); an integer keeps the reference rule n_kept <= threshold."""
+    A kept block of at most ``dense_threshold`` levels (default 24, as in the reference) gets
+    the exact dense factor, approximate Cholesky otherwise.  ``dense_policy`` (an addition of
+    this library) widens that rule: with ``"device"`` (the default) a larger kept block is
+    also factored densely where the device cost model finds the dense apply cheaper (up to
+    4,096 kept levels, or 16,384 when the reduced graph is at least 0.5 % dense);
+    ``"reference"`` applies the threshold alone.  The choice depends on the config and the
+    design only."""
 
-    __slots__ = ("_approx_chol", "_schur", "_dense_threshold", "_scaling")
+    __slots__ = ("_approx_chol", "_schur", "_dense_threshold", "_scaling", "_dense_policy")
 
     def __init__(self, approx_chol: Optional[ApproxCholConfig] = None, schur: Optional[Schur] = None,
-                 dense_threshold: Optional[int] = None, scaling: Optional[ScalingConfig] = None):
+                 dense_threshold: Optional[int] = None, scaling: Optional[ScalingConfig] = None,
+                 dense_policy: str = "device"):
         if dense_threshold is not None and int(dense_threshold) < 0:
             raise OverflowError("dense_threshold must be non-negative")
+        if dense_policy not in ("device", "reference"):
+            raise ValueError("dense_policy must be 'device' or 'reference'")
         self._approx_chol = approx_chol
         self._schur = schur
-        self._dense_threshold = None if dense_threshold is None else int(dense_threshold)
+        self._dense_threshold = 24 if dense_threshold is None else int(dense_threshold)
         self._scaling = scaling
+        self._dense_policy = dense_policy
 
     approx_chol = property(lambda self: self._approx_chol)
     schur = property(lambda self: self._schur)
     scaling = property(lambda self: self._scaling)
+    dense_threshold = property(lambda self: self._dense_threshold)
+    dense_policy = property(lambda self: self._dense_policy)
 
-    @property
-    def dense_threshold(self) -> int:
-        return 24 if self._dense_threshold is None else self._dense_threshold
+    def __repr__(self) -> str:
+        return (f"LocalSolverConfig(approx_chol={self._approx_chol!r}, schur={self._schur!r}, "
+                f"dense_threshold={self._dense_threshold}, scaling={self._scaling!r}, "
+                f"dense_policy={self._dense_policy!r})")
 
 
 class AdditiveSchwarz:
@@ -262,7 +274,8 @@ def _native_config(pc) -> Optional[_lib.PrecondConfig]:
                 if not ls.schur._exact:
                     c.schur_seed = ls.schur._cfg.seed
                     c.schur_split = ls.schur._cfg.split
-            c.dense_threshold = -1 if ls._dense_threshold is None else ls._dense_threshold
+            c.dense_threshold = ls.dense_threshold
+            c.dense_policy = 1 if ls.dense_policy == "device" else 0
             if ls.scaling is not None:
                 c.scaling_tolerance = ls.scaling.tolerance
                 c.scaling_max_sweeps = ls.scaling.max_sweeps

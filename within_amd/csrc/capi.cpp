@@ -34,14 +34,17 @@ struct wx_precond {
 namespace {
 
 thread_local std::string g_last_error;
+thread_local const char *g_last_variant = "";
 
 template <typename F> wx_status guard(F &&f) {
     try {
         g_last_error.clear();
+        g_last_variant = "";
         f();
         return WX_OK;
     } catch (const Error &e) {
         g_last_error = e.what();
+        g_last_variant = e.variant;
         return e.code;
     } catch (const std::bad_alloc &) {
         g_last_error = "out of host memory";
@@ -81,6 +84,7 @@ LocalSolverSettings settings_from(const wx_precond_config &c) {
     st.schur_seed = c.schur_seed;
     st.schur_split = c.schur_split;
     st.dense_threshold = c.dense_threshold;
+    st.dense_device = c.dense_policy != WX_DENSE_REFERENCE;
     st.scaling_tolerance = c.scaling_tolerance;
     st.scaling_max_sweeps = c.scaling_max_sweeps;
     st.scaling_error = c.scaling_error_on_failure != 0;
@@ -92,6 +96,8 @@ LocalSolverSettings settings_from(const wx_precond_config &c) {
 extern "C" {
 
 const char *wx_last_error(void) { return g_last_error.c_str(); }
+
+const char *wx_last_error_variant(void) { return g_last_variant; }
 
 int32_t wx_device_info(char *name, uint64_t name_len) {
     int count = 0;
@@ -117,7 +123,8 @@ void wx_precond_config_default(wx_precond_config *cfg) {
     cfg->split_merge = 2;
     cfg->schur_seed = 0;
     cfg->schur_split = 1;
-    cfg->dense_threshold = -1;
+    cfg->dense_threshold = 24;
+    cfg->dense_policy = WX_DENSE_DEVICE;
     cfg->scaling_tolerance = 1e-9;
     cfg->scaling_max_sweeps = 2048;
     cfg->scaling_error_on_failure = 0;
@@ -176,12 +183,14 @@ wx_status wx_design_from_effects(uint64_t n_obs, uint64_t n_effects, const uint3
                 for (uint64_t i = 0; i < n_obs; ++i)
                     if (!std::isfinite(z[i]))
                         fail(WX_ERR_BUILD, "slope " + std::to_string(s) + " loading at index " + std::to_string(i) +
-                                               " must be finite, got " + std::to_string(z[i]));
+                                               " must be finite, got " + std::to_string(z[i]),
+                             "BuildError::InvalidLoading");
                 cols.push_back((int64_t)d->slopes.size());
                 d->slopes.emplace_back(z, z + n_obs);
             }
             sp += n_slopes[e];
-            if (cols.empty()) fail(WX_ERR_BUILD, "an effect must have an intercept or at least one slope");
+            if (cols.empty())
+                fail(WX_ERR_BUILD, "an effect must have an intercept or at least one slope", "BuildError::EmptyEffect");
             d->columns.emplace_back(levels[e], levels[e] + n_obs);
             d->structure.push_back(std::move(cols));
         }
@@ -228,7 +237,8 @@ wx_status wx_solver_new(const wx_design *design, const double *weights, const wx
             if (prebuilt->p->n != s.design.n_dofs)
                 fail(WX_ERR_BUILD, "preconditioner dimension mismatch: expected " + std::to_string(s.design.n_dofs) +
                                        ", got " + std::to_string(prebuilt->p->n) + "x" +
-                                       std::to_string(prebuilt->p->n));
+                                       std::to_string(prebuilt->p->n),
+                     "BuildError::PreconditionerDimensionMismatch");
             s.precond = prebuilt->p;
         } else {
             wx_precond_config c;
@@ -273,7 +283,8 @@ wx_status wx_solver_reweight(const wx_solver *base, const double *weights, const
         if (prebuilt) {
             if (prebuilt->p->n != s.design.n_dofs)
                 fail(WX_ERR_BUILD, "preconditioner dimension mismatch: expected " + std::to_string(s.design.n_dofs) +
-                                       ", got " + std::to_string(prebuilt->p->n));
+                                       ", got " + std::to_string(prebuilt->p->n),
+                     "BuildError::PreconditionerDimensionMismatch");
             s.precond = prebuilt->p;
         } else {
             wx_precond_config c;
@@ -493,7 +504,7 @@ wx_status wx_solve_many(const wx_solver *const *solvers, uint64_t n_problems, co
         }
         const wx_status rc = wx_solve_many_device(sv.data(), n_problems, yv.data(), lv.data(), k, opts, xv.data(),
                                                   dv.data(), stats, st.s);
-        if (rc != WX_OK) fail(rc, wx_last_error());
+        if (rc != WX_OK) fail(rc, std::string(wx_last_error()), g_last_variant);
         for (uint64_t p = 0; p < n_problems; ++p) {
             const uint64_t n = solvers[p]->s.design.n_obs, N = solvers[p]->s.design.n_dofs;
             if (xv[p]) WX_HIP(hipMemcpyAsync(X[p], xv[p], N * k[p] * 8, hipMemcpyDeviceToHost, st.s));

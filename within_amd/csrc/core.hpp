@@ -11,13 +11,27 @@
 
 namespace wx {
 
-// Error carrying a wx_status; converted to a status code at the C boundary.
+// Error carrying a wx_status and, where one applies, the name of the reference error variant
+// it stands for (e.g. "BuildError::InvalidWeight"); converted at the C boundary.
 struct Error : std::runtime_error {
     wx_status code;
-    Error(wx_status c, const std::string &msg) : std::runtime_error(msg), code(c) {}
+    const char *variant;
+    Error(wx_status c, const std::string &msg, const char *v) : std::runtime_error(msg), code(c), variant(v) {}
 };
 
-[[noreturn]] inline void fail(wx_status code, const std::string &msg) { throw Error(code, msg); }
+// The reference variant a status stands for when the throw site names none.
+inline const char *default_variant(wx_status code) {
+    switch (code) {
+    case WX_ERR_INVALID_INPUT: return "SolveError::InvalidInput";
+    case WX_ERR_NOT_POSITIVE_DEF: return "SolveError::InvalidInput";
+    case WX_ERR_BUILD: return "BuildError::LocalSolverBuild";
+    default: return "";
+    }
+}
+
+[[noreturn]] inline void fail(wx_status code, const std::string &msg, const char *variant = nullptr) {
+    throw Error(code, msg, variant ? variant : default_variant(code));
+}
 
 // One term (effect) of the design.  Coefficient column c of level l lives at
 // offset + c * n_levels + l (crates/within/src/domain.rs:80-103).

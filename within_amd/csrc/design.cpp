@@ -69,16 +69,18 @@ Design build_design(uint64_t n_obs, std::vector<std::vector<uint32_t>> columns,
                     std::vector<std::vector<int64_t>> structure, std::vector<std::vector<double>> slopes,
                     bool locality_sort) {
     // crates/within/src/domain.rs:187-247.
-    if (n_obs == 0) fail(WX_ERR_BUILD, "no observations provided");
+    if (n_obs == 0) fail(WX_ERR_BUILD, "no observations provided", "BuildError::EmptyObservations");
     const uint64_t T = columns.size();
     for (uint64_t t = 0; t < T; ++t)
         if (columns[t].size() != n_obs)
             fail(WX_ERR_BUILD, "column " + std::to_string(t) + " has " + std::to_string(columns[t].size()) +
-                                   " observations, expected " + std::to_string(n_obs));
+                                   " observations, expected " + std::to_string(n_obs),
+                 "BuildError::ObservationCountMismatch");
     for (uint64_t s = 0; s < slopes.size(); ++s)
         if (slopes[s].size() != n_obs)
             fail(WX_ERR_BUILD, "column " + std::to_string(T + s) + " has " + std::to_string(slopes[s].size()) +
-                                   " observations, expected " + std::to_string(n_obs));
+                                   " observations, expected " + std::to_string(n_obs),
+                 "BuildError::SlopeLengthMismatch");
     Design d;
     d.n_obs = n_obs;
     d.n_slopes = slopes.size();
@@ -102,7 +104,8 @@ Design build_design(uint64_t n_obs, std::vector<std::vector<uint32_t>> columns,
         offset += tm.n_dofs();
     }
     if (offset > 0xFFFFFFFFull)
-        fail(WX_ERR_BUILD, "DOF space of " + std::to_string(offset) + " coefficients exceeds u32 indexing");
+        fail(WX_ERR_BUILD, "DOF space of " + std::to_string(offset) + " coefficients exceeds u32 indexing",
+             "BuildError::DofSpaceExceedsU32");
     d.n_dofs = offset;
 
     // Dominant term = last argmax of n_dofs (Rust max_by_key, domain.rs:226).
@@ -150,11 +153,13 @@ void permute_in(const Design &d, const double *src, double *dst) {
 void validate_weights(const Design &d, const double *w, uint64_t len) {
     // crates/within/src/domain.rs:261-279.
     if (len != d.n_obs)
-        fail(WX_ERR_BUILD, "weights has length " + std::to_string(len) + ", expected " + std::to_string(d.n_obs));
+        fail(WX_ERR_BUILD, "weights has length " + std::to_string(len) + ", expected " + std::to_string(d.n_obs),
+             "BuildError::WeightCountMismatch");
     for (uint64_t i = 0; i < len; ++i)
         if (!(w[i] >= 0.0 && std::isfinite(w[i])))
             fail(WX_ERR_BUILD, "weight at index " + std::to_string(i) + " must be finite and non-negative, got " +
-                                   std::to_string(w[i]));
+                                   std::to_string(w[i]),
+                 "BuildError::InvalidWeight");
 }
 
 std::vector<std::vector<uint8_t>> find_active_levels(const Design &d) {

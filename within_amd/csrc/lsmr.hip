@@ -40,9 +40,11 @@ struct ColState {
     double *alpha_bar, *phi_bar, *c_bar, *s_bar, *zeta_bar, *zeta0, *a_norm_sq, *abs_tol;
     double *prev_rho, *prev_rho_bar, *t_x, *t_hbar, *t_h, *inv_alpha, *bnorm_scale;
     double *den_norm; // ||D'W y|| = beta_1 ||p~_1||, the denominator of the true residual
+    double *rel_tol;  // the solve's tol, written with beta_1: a replayed iteration graph reads
+                      // the current call's tolerance, never one fixed at capture
     int32_t *active, *upd, *brk, *stop, *iters, *err;
 };
-constexpr int N_DOUBLE_FIELDS = 22;
+constexpr int N_DOUBLE_FIELDS = 23;
 constexpr int N_INT_FIELDS = 6;
 
 __device__ inline double term_load(const DesignArgs &d, int32_t li, uint64_t i) {
@@ -653,6 +655,7 @@ __device__ inline double reduce_partials(const double *part, uint32_t nb, uint32
 __device__ inline void set_beta1(ColState &cs, uint32_t j, double beta, double tol) {
     cs.beta[j] = beta;
     cs.abs_tol[j] = tol * beta;
+    cs.rel_tol[j] = tol;
     cs.iters[j] = 0;
     cs.err[j] = 0;
     cs.brk[j] = 0;
@@ -967,7 +970,7 @@ __global__ __launch_bounds__(BLK) void k_init_solution(uint64_t N, double *__res
 // The iteration count is kept per column on the device, so the launch is identical every
 // iteration and a captured graph can replay it.
 __global__ __launch_bounds__(BLK) void k_alpha_step(const double *part, uint32_t nb, uint32_t kp, ColState cs,
-                                                    double rel_tol, uint32_t k, Ranges rg) {
+                                                    uint32_t k, Ranges rg) {
     __shared__ double sh[BLK];
     const uint32_t c = blockIdx.x, j = c % k;
     double vp = reduce_partials(part, nb, kp, 3, 0, j, sh, false, rg, c / k);
@@ -1032,7 +1035,8 @@ __global__ __launch_bounds__(BLK) void k_alpha_step(const double *part, uint32_t
     else {
         const double a_norm = fmax(sqrt(a_norm_sq), 2.2250738585072014e-308);
         const double normar = fabs(zeta_bar_new);
-        if (normar / (a_norm * fmax(residual, 2.2250738585072014e-308)) <= rel_tol) stop = WX_STOP_NORMAL_EQ_TOLERANCE;
+        if (normar / (a_norm * fmax(residual, 2.2250738585072014e-308)) <= cs.rel_tol[c])
+            stop = WX_STOP_NORMAL_EQ_TOLERANCE;
     }
     cs.upd[c] = 1;
     cs.iters[c] += 1; // steps taken (reset by set_beta1)
@@ -1787,7 +1791,7 @@ void setup_colstate(Workspace &ws) {
                                         &ws.cs.c_bar, &ws.cs.s_bar, &ws.cs.zeta_bar, &ws.cs.zeta0,
                                         &ws.cs.a_norm_sq, &ws.cs.abs_tol, &ws.cs.prev_rho, &ws.cs.prev_rho_bar,
                                         &ws.cs.t_x, &ws.cs.t_hbar, &ws.cs.t_h, &ws.cs.inv_alpha,
-                                        &ws.cs.bnorm_scale, &ws.cs.den_norm};
+                                        &ws.cs.bnorm_scale, &ws.cs.den_norm, &ws.cs.rel_tol};
     for (int f = 0; f < N_DOUBLE_FIELDS; ++f) *fields[f] = dp + (uint64_t)f * cap;
     int32_t *ip = ws.ints.as<int32_t>();
     int32_t **ifields[N_INT_FIELDS] = {&ws.cs.active, &ws.cs.upd, &ws.cs.brk, &ws.cs.stop, &ws.cs.iters, &ws.cs.err};
@@ -1800,7 +1804,7 @@ ColState colstate_view(const ColState &cs, uint32_t off) {
     double **d[N_DOUBLE_FIELDS] = {&v.alpha, &v.beta, &v.beta_prev_inv, &v.beta_inv, &v.p_coeff, &v.gscale,
                                    &v.alpha_bar, &v.phi_bar, &v.c_bar, &v.s_bar, &v.zeta_bar, &v.zeta0,
                                    &v.a_norm_sq, &v.abs_tol, &v.prev_rho, &v.prev_rho_bar, &v.t_x, &v.t_hbar,
-                                   &v.t_h, &v.inv_alpha, &v.bnorm_scale, &v.den_norm};
+                                   &v.t_h, &v.inv_alpha, &v.bnorm_scale, &v.den_norm, &v.rel_tol};
     for (auto *f : d) *f += off;
     int32_t **i[N_INT_FIELDS] = {&v.active, &v.upd, &v.brk, &v.stop, &v.iters, &v.err};
     for (auto *f : i) *f += off;
@@ -2220,7 +2224,7 @@ class ChunkRun {
                                    part_, k, kshift, dof_col_);
             }
         }
-        hipLaunchKernelGGL(k_alpha_step, dim3(ncol_), dim3(BLK), 0, st, part_, gN_, kp_, cs_, opts_.tol, k, rg_dof_);
+        hipLaunchKernelGGL(k_alpha_step, dim3(ncol_), dim3(BLK), 0, st, part_, gN_, kp_, cs_, k, rg_dof_);
         const uint64_t push = cap_ ? ring_advance() : 0;
         hipLaunchKernelGGL(k_update_solution, dim3(gN_), dim3(BLK), 0, st, N, ws.v.as<double>(), ws.h.as<double>(),
                            ws.hb.as<double>(), ws.x.as<double>(), cs_, k, kshift, ws.pt.as<double>(),
@@ -2358,26 +2362,49 @@ std::vector<std::pair<uint64_t, ManyShape>> g_tuned; // problem-count class -> s
 ManyShape g_last_shape;
 ManyShape g_forced{0, 1, false, false}; // streams == 0: the autotuner decides
 
-std::vector<hipStream_t> &stream_pool() {
-    static std::vector<hipStream_t> pool;
-    return pool;
-}
+// Streams of the many-problem runs.  A call leases its streams for its whole duration, so
+// concurrent calls never share one: a run that captures its iteration graph on a stream would
+// otherwise also capture another call's launches on it.  Returned streams are reused by later
+// calls on the same device and live until the process ends.
+std::mutex g_stream_mu;
+std::vector<std::pair<int, hipStream_t>> g_free_streams; // (device, stream)
 
-hipStream_t pool_stream(int i) {
-    auto &pool = stream_pool();
-    while ((int)pool.size() <= i) {
-        hipStream_t s;
-        WX_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
-        pool.push_back(s);
+struct StreamLease {
+    std::vector<hipStream_t> s;
+    int dev = 0;
+    explicit StreamLease(int n) {
+        WX_HIP(hipGetDevice(&dev));
+        {
+            std::lock_guard<std::mutex> lk(g_stream_mu);
+            for (size_t i = g_free_streams.size(); i-- > 0 && (int)s.size() < n;)
+                if (g_free_streams[i].first == dev) {
+                    s.push_back(g_free_streams[i].second);
+                    g_free_streams.erase(g_free_streams.begin() + (std::ptrdiff_t)i);
+                }
+        }
+        while ((int)s.size() < n) {
+            hipStream_t st;
+            if (hipStreamCreateWithFlags(&st, hipStreamNonBlocking) != hipSuccess) {
+                give_back();
+                fail(WX_ERR_DEVICE, "hipStreamCreateWithFlags failed");
+            }
+            s.push_back(st);
+        }
     }
-    return pool[i];
-}
+    void give_back() {
+        std::lock_guard<std::mutex> lk(g_stream_mu);
+        for (hipStream_t st : s) g_free_streams.emplace_back(dev, st);
+        s.clear();
+    }
+    ~StreamLease() { give_back(); }
+};
 
 void run_many(const std::vector<const Solver *> &solvers, const std::vector<const double *> &Y,
               const std::vector<uint64_t> &ldy, const std::vector<uint64_t> &k, const wx_lsmr_options &opts,
               const std::vector<double *> &X, const std::vector<double *> &Dm, const std::vector<wx_rhs_stats *> &stats,
               const ManyShape &shape, hipStream_t caller) {
     // the caller's stream work comes first
+    StreamLease lease(std::max(1, shape.streams));
     hipEvent_t ready;
     WX_HIP(hipEventCreateWithFlags(&ready, hipEventDisableTiming));
     WX_HIP(hipEventRecord(ready, caller));
@@ -2414,7 +2441,7 @@ void run_many(const std::vector<const Solver *> &solvers, const std::vector<cons
                 packed[p] = 1;
             }
             packs.push_back(get_pack(mem, kv));
-            hipStream_t st = pool_stream(next_stream);
+            hipStream_t st = lease.s[next_stream];
             next_stream = (next_stream + 1) % shape.streams;
             WX_HIP(hipStreamWaitEvent(st, ready, 0));
             runs.push_back(std::make_unique<ChunkRun>(*packs.back(), mY, mld, opts, mX, mDm, mSt, st, shape.group,
@@ -2424,7 +2451,7 @@ void run_many(const std::vector<const Solver *> &solvers, const std::vector<cons
     for (size_t p = 0; p < solvers.size(); ++p)
         for (uint64_t c0 = 0; c0 < k[p] && !packed[p]; c0 += MAX_BATCH) {
             const uint32_t kk = (uint32_t)std::min<uint64_t>(MAX_BATCH, k[p] - c0);
-            hipStream_t st = pool_stream(next_stream);
+            hipStream_t st = lease.s[next_stream];
             next_stream = (next_stream + 1) % shape.streams;
             WX_HIP(hipStreamWaitEvent(st, ready, 0));
             runs.push_back(std::make_unique<ChunkRun>(*solvers[p], Y[p], ldy[p], c0, kk, opts, X[p], Dm[p], stats[p], st,

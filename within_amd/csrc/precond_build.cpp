@@ -57,7 +57,9 @@ std::shared_ptr<Precond> build_diagonal(const Design &d, const std::vector<doubl
         double &v = p->inv_diag[j];
         if (v == 0.0) continue;
         const double inv = 1.0 / v;
-        if (!std::isfinite(inv)) fail(WX_ERR_BUILD, "zero diagonal in preconditioner at index " + std::to_string(j));
+        if (!std::isfinite(inv))
+            fail(WX_ERR_BUILD, "zero diagonal in preconditioner at index " + std::to_string(j),
+                 "BuildError::SingularDiagonal");
         v = inv;
     }
     return p;
@@ -840,7 +842,7 @@ std::shared_ptr<Precond> build_additive(const Design &d, const std::vector<doubl
                               pair_err[p]);
     phase("cross-tabs and components");
     for (size_t p = 0; p < pairs.size(); ++p) {
-        if (!pair_err[p].empty()) fail(WX_ERR_BUILD, pair_err[p]);
+        if (!pair_err[p].empty()) fail(WX_ERR_BUILD, pair_err[p], "BuildError::UnscalableComponent");
         warnings.insert(warnings.end(), pair_warn[p].begin(), pair_warn[p].end());
     }
     std::vector<Component> comps;
@@ -879,26 +881,32 @@ std::shared_ptr<Precond> build_additive(const Design &d, const std::vector<doubl
     S->fold.resize(ns);
     S->inv_de.assign(ns, 0.0);
     S->c_ptr.assign(ns + 1, 0);
-    // factor choice per component.  Device cost model (dense_threshold < 0): small kept blocks
-    // and dense-ish reduced graphs get the exact dense inverse -- one parallel symmetric
-    // product per apply -- while large sparse reduced graphs get approximate Cholesky (of the
-    // Gremban cover for a signed block), whose triangular solves run level by level.  The
-    // density estimate is the sampled Schur complement's edge count sum_e (deg(e) - 1) over
-    // nk^2 / 2.
+    // factor choice per component.  The reference rule (block_elim/solver.rs:311-347): the
+    // exact dense factor iff n_kept <= dense_threshold.  With the device policy a kept block is
+    // also factored densely where the device cost model prefers it: small kept blocks and
+    // dense-ish reduced graphs get the exact dense inverse -- one parallel symmetric product per
+    // apply -- while large sparse reduced graphs keep approximate Cholesky (of the Gremban cover
+    // for a signed block), whose triangular solves run level by level.  The density estimate is
+    // the sampled Schur complement's edge count sum_e (deg(e) - 1) over nk^2 / 2.  The choice
+    // depends on the config and the component only; a visible GPU merely computes the large
+    // dense blocks (`big`) instead of the host.
     const bool have_gpu = gpu_available();
     std::vector<uint8_t> dense(nc), big(nc, 0);
     for (uint64_t c = 0; c < nc; ++c) {
         const uint64_t nk = comps[c].nk;
-        if (cfg.dense_threshold >= 0) dense[c] = nk <= (uint64_t)cfg.dense_threshold;
-        else if (nk <= 4096) dense[c] = 1;
-        else if (nk <= 16384 && have_gpu) {
-            double m = 0.0;
-            for (uint32_t e = 0; e < comps[c].ne; ++e) {
-                const uint64_t deg = comps[c].cptr[e + 1] - comps[c].cptr[e];
-                if (deg > 1) m += (double)(deg - 1);
+        bool d = cfg.dense_threshold >= 0 && nk <= (uint64_t)cfg.dense_threshold;
+        if (!d && cfg.dense_device) {
+            if (nk <= 4096) d = true;
+            else if (nk <= 16384) {
+                double m = 0.0;
+                for (uint32_t e = 0; e < comps[c].ne; ++e) {
+                    const uint64_t deg = comps[c].cptr[e + 1] - comps[c].cptr[e];
+                    if (deg > 1) m += (double)(deg - 1);
+                }
+                d = 2.0 * m / ((double)nk * nk) >= 0.005;
             }
-            dense[c] = 2.0 * m / ((double)nk * nk) >= 0.005;
-        } else dense[c] = 0;
+        }
+        dense[c] = d;
         big[c] = dense[c] && nk > 1024 && have_gpu;
         S->fac_kind[c] = dense[c] ? 0 : (comps[c].signed_form ? 2 : 1);
     }
@@ -967,7 +975,9 @@ std::shared_ptr<Precond> build_additive(const Design &d, const std::vector<doubl
     }
     phase("large dense blocks");
     for (uint64_t c = 0; c < nc; ++c) {
-        if (failed[c] == 1) fail(WX_ERR_BUILD, "zero diagonal in preconditioner at index " + std::to_string(c));
+        if (failed[c] == 1)
+            fail(WX_ERR_BUILD, "zero diagonal in preconditioner at index " + std::to_string(c),
+                 "BuildError::SingularDiagonal");
         if (failed[c] == 2) fail(WX_ERR_BUILD, "local solver build failed: Schur complement is not positive definite");
     }
     // assemble slots (kept block of sparse components relabelled into elimination order;

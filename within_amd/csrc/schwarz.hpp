@@ -17,7 +17,8 @@ struct LocalSolverSettings {
     bool schur_exact = false;
     uint64_t schur_seed = 0;
     int32_t schur_split = 1;
-    int64_t dense_threshold = -1; // < 0: device cost model
+    int64_t dense_threshold = 24; // dense exact factor iff n_kept <= dense_threshold (< 0: never)
+    bool dense_device = true;     // additionally dense where the device cost model prefers it
     double scaling_tolerance = 1e-9;
     int64_t scaling_max_sweeps = 2048;
     bool scaling_error = false;
