@@ -4,6 +4,15 @@ import sys
 import numpy as np
 import pytest
 
+# PyTorch-ROCm bundles its own HIP runtime (libamdhip64.so.7, the same soname as ROCm's).
+# libwithin_amd.so binds to whichever copy the process has loaded first; when the library comes
+# first and torch later, torch maps its second copy and cannot initialise the GPU.  The GPU
+# tests that hand torch device buffers to the library therefore need torch loaded first.
+try:
+    import torch  # noqa: F401
+except ImportError:  # the CPU suite does not need it
+    torch = None
+
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 if ROOT not in sys.path:
     sys.path.insert(0, ROOT)

@@ -152,6 +152,34 @@ def test_many_autotuned_mixed_precision(gpu, problems):
     print(f"autotuned shape {shape}")
 
 
+@pytest.mark.parametrize("shape", [(4, 2, True, False), (4, 2, True, True)])
+def test_many_graph_replay_reads_the_current_tolerance(gpu, problems, shape):
+    """A later call on the same Solvers replays the iteration graphs captured by an earlier one:
+    it must still stop at its own tolerance (the tolerance lives in the per-column device
+    state, not in the captured launches)."""
+    sub = problems[:4] + problems[-6:]
+    solvers, Ys = [s for s, _ in sub], [Y for _, Y in sub]
+    wa.set_many_shape(*shape)
+    try:
+        loose = wa.solve_many(solvers, Ys, wa.LsmrOptions(tol=1e-6))
+        tight = wa.solve_many(solvers, Ys, wa.LsmrOptions(tol=1e-12))
+    finally:
+        wa.set_many_shape(0)
+    more = 0
+    for p, ((s, Y), rl, rt) in enumerate(zip(sub, loose, tight)):
+        ref = s.solve_batch(Y, wa.LsmrOptions(tol=1e-12))
+        assert all(rt.converged), p
+        more += sum(int(a > b) for a, b in zip(rt.iterations, rl.iterations))
+        for j in range(Y.shape[1]):
+            if not shape[3]:
+                assert rt.iterations[j] == ref.iterations[j], (p, j)
+                assert rel(rt.demeaned[:, j], ref.demeaned[:, j]) <= 1e-13
+            else:
+                assert abs(rt.iterations[j] - ref.iterations[j]) <= 1, (p, j)
+                assert rel(rt.demeaned[:, j], ref.demeaned[:, j]) <= 1e-9
+    assert more > 0
+
+
 def test_many_maxiter_and_zero_rhs(gpu, problems):
     s, Y = problems[0]
     Z = np.zeros_like(Y)
