@@ -12,7 +12,8 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
 
 def declared_symbols():
-    text = open(os.path.join(ROOT, "include", "within_amd.h")).read()
+    inc = os.path.join(ROOT, "include")
+    text = "".join(open(os.path.join(inc, f)).read() for f in sorted(os.listdir(inc)) if f.endswith(".h"))
     return sorted(set(re.findall(r"\b(wx_[a-z0-9_]+)\s*\(", text)))
 
 
@@ -21,7 +22,7 @@ def test_library_exports_every_declared_symbol():
     syms = declared_symbols()
     assert len(syms) >= 25
     for s in syms:
-        assert hasattr(L, s), f"{s} declared in include/within_amd.h but not exported"
+        assert hasattr(L, s), f"{s} declared in include/*.h but not exported"
 
 
 def test_binding_covers_every_declared_symbol():

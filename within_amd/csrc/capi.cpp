@@ -8,6 +8,7 @@
 #include <cstring>
 #include <new>
 
+#include "../../include/within_amd_testing.h"
 #include "schwarz.hpp"
 #include "solver.hpp"
 
@@ -341,6 +342,59 @@ wx_status wx_gram(const double *A, uint64_t n, uint64_t m, uint64_t lda, double 
         part.alloc(gram_scratch_doubles(n, (uint32_t)m) * 8);
         gram_device(a.as<double>(), n, (uint32_t)m, n, g.as<double>(), part.as<double>(), nullptr);
         WX_HIP(hipMemcpy(G, g.p, m * m * 8, hipMemcpyDeviceToHost));
+    });
+}
+
+wx_status wx_testing_design_index(const wx_design *design, uint64_t n_terms, uint64_t *n_levels, uint64_t *offset,
+                                  uint8_t *sorted, int64_t *dominant, int32_t *permuted, uint32_t *obs_perm) {
+    return guard([&] {
+        if (!design) fail(WX_ERR_INVALID_INPUT, "null design");
+        if (n_terms != design->columns.size())
+            fail(WX_ERR_INVALID_INPUT, "n_terms disagrees with the design's term count");
+        const Design d = build_design(design->n_obs, design->columns, design->structure, design->slopes, true);
+        int64_t dom = -1;
+        uint64_t best = 0;
+        for (uint64_t t = 0; t < d.terms.size(); ++t) {
+            if (n_levels) n_levels[t] = d.terms[t].n_levels;
+            if (offset) offset[t] = d.terms[t].offset;
+            if (sorted) sorted[t] = d.terms[t].sorted ? 1 : 0;
+            if (dom < 0 || d.terms[t].n_dofs() >= best) {
+                dom = (int64_t)t;
+                best = d.terms[t].n_dofs();
+            }
+        }
+        if (dominant) *dominant = dom;
+        if (permuted) *permuted = d.obs_perm.empty() ? 0 : 1;
+        if (obs_perm && !d.obs_perm.empty()) std::copy(d.obs_perm.begin(), d.obs_perm.end(), obs_perm);
+    });
+}
+
+wx_status wx_testing_block_precond(uint32_t n_rows, uint32_t n_cols, const double *B, const double *diag,
+                                   const wx_precond_config *cfg, wx_precond **out) {
+    return guard([&] {
+        if (!out || !diag || (!B && n_rows && n_cols)) fail(WX_ERR_INVALID_INPUT, "null pointer");
+        if (n_rows + n_cols == 0) fail(WX_ERR_INVALID_INPUT, "empty block");
+        wx_precond_config c;
+        if (cfg) c = *cfg;
+        else wx_precond_config_default(&c);
+        auto h = std::make_unique<wx_precond>();
+        h->p = build_block_precond(n_rows, n_cols, B, diag, settings_from(c));
+        *out = h.release();
+    });
+}
+
+wx_status wx_testing_reduced_laplacian(uint32_t n_rows, uint32_t n_cols, const double *B, const double *diag,
+                                       const wx_precond_config *cfg, double *out, uint64_t cap, uint32_t *n_out) {
+    return guard([&] {
+        if (!out || !n_out || !diag || (!B && n_rows && n_cols)) fail(WX_ERR_INVALID_INPUT, "null pointer");
+        wx_precond_config c;
+        if (cfg) c = *cfg;
+        else wx_precond_config_default(&c);
+        uint32_t n = 0;
+        const std::vector<double> L = block_reduced_laplacian(n_rows, n_cols, B, diag, settings_from(c), n);
+        if ((uint64_t)n * n > cap) fail(WX_ERR_INVALID_INPUT, "output buffer too small");
+        std::copy(L.begin(), L.end(), out);
+        *n_out = n;
     });
 }
 

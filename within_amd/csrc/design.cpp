@@ -25,45 +25,6 @@ std::vector<uint32_t> stable_argsort(const uint32_t *key, uint64_t n, uint64_t n
     return perm;
 }
 
-namespace {
-// Stable reorder of `perm` by key[perm[i]] (a counting sort when n >= n_levels).
-void stable_sort_by(std::vector<uint32_t> &perm, std::vector<uint32_t> &tmp, const uint32_t *key,
-                    uint64_t n_levels) {
-    const uint64_t n = perm.size();
-    if (n < n_levels) {
-        std::stable_sort(perm.begin(), perm.end(), [&](uint32_t a, uint32_t b) { return key[a] < key[b]; });
-        return;
-    }
-    std::vector<uint64_t> cursor(n_levels + 1, 0);
-    for (uint64_t i = 0; i < n; ++i) cursor[key[i] + 1]++;
-    for (uint64_t l = 1; l <= n_levels; ++l) cursor[l] += cursor[l - 1];
-    tmp.resize(n);
-    for (uint64_t i = 0; i < n; ++i) tmp[cursor[key[perm[i]]]++] = perm[i];
-    perm.swap(tmp);
-}
-
-// Observation order of the device design: by the dominant term's level, as the reference's
-// locality sort (domain.rs:226-234), and within one dominant level by the other terms' levels,
-// the term with the fewest levels first.  Rows of one dominant level that share a level of a
-// small factor become adjacent, so a wavefront's per-row coefficient gathers of that factor
-// touch fewer distinct cache lines (the gather is bound by line requests, not bytes), and that
-// factor's D' reads runs of consecutive rows.  The secondary order is internal: coefficients,
-// caller-order inputs and outputs are unaffected.
-std::vector<uint32_t> locality_order(const std::vector<std::vector<uint32_t>> &columns, const Design &d,
-                                     uint64_t dom, uint64_t n_obs) {
-    std::vector<uint64_t> keys;
-    for (uint64_t t = 0; t < columns.size(); ++t)
-        if (t != dom) keys.push_back(t);
-    std::stable_sort(keys.begin(), keys.end(),
-                     [&](uint64_t a, uint64_t b) { return d.terms[a].n_levels < d.terms[b].n_levels; });
-    keys.insert(keys.begin(), dom);
-    std::vector<uint32_t> perm(n_obs), tmp;
-    std::iota(perm.begin(), perm.end(), 0u);
-    for (size_t q = keys.size(); q-- > 0;) // least significant key first
-        stable_sort_by(perm, tmp, columns[keys[q]].data(), d.terms[keys[q]].n_levels);
-    return perm;
-}
-} // namespace
 
 Design build_design(uint64_t n_obs, std::vector<std::vector<uint32_t>> columns,
                     std::vector<std::vector<int64_t>> structure, std::vector<std::vector<double>> slopes,
@@ -120,7 +81,9 @@ Design build_design(uint64_t n_obs, std::vector<std::vector<uint32_t>> columns,
     d.codes.resize(T * n_obs);
     d.slopes.resize(d.n_slopes * n_obs);
     if (sort) {
-        d.obs_perm = locality_order(columns, d, (uint64_t)dom, n_obs);
+        // the reference's locality sort (domain.rs:226-234): a stable argsort by the dominant
+        // term, so obs_perm, the permuted columns and the sorted flags are the reference's
+        d.obs_perm = stable_argsort(columns[dom].data(), n_obs, d.terms[dom].n_levels);
         const uint32_t *perm = d.obs_perm.data();
         for (uint64_t t = 0; t < T; ++t) {
             uint32_t *dst = d.codes.data() + t * n_obs;

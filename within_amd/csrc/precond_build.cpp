@@ -12,6 +12,7 @@
 //     factor in elimination order, ready for level-scheduled triangular solves.
 #include <algorithm>
 #include <cmath>
+#include <functional>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -810,6 +811,12 @@ Component make_cover(const Component &cm) {
 
 } // namespace
 
+namespace {
+std::shared_ptr<Precond> assemble_additive(std::vector<Component> &comps, uint64_t n_dofs, bool any_slope,
+                                           const LocalSolverSettings &cfg,
+                                           const std::function<void(const char *)> &phase);
+} // namespace
+
 std::shared_ptr<Precond> build_additive(const Design &d, const std::vector<double> &w, const LocalSolverSettings &cfg,
                                         std::vector<std::string> &warnings) {
     // WITHIN_AMD_VERBOSE: per-phase wall times of the build on stderr
@@ -849,14 +856,23 @@ std::shared_ptr<Precond> build_additive(const Design &d, const std::vector<doubl
     for (auto &v : per_pair)
         for (auto &c : v) comps.push_back(std::move(c));
     if (comps.empty()) return nullptr;
+    return assemble_additive(comps, d.n_dofs, any_slope, cfg, phase);
+}
 
+namespace {
+
+// Factorisation of every component's kept block and assembly of the slot arrays
+// (operator/schwarz.rs:90-113, block_elim/solver.rs:311-347).
+std::shared_ptr<Precond> assemble_additive(std::vector<Component> &comps, uint64_t n_dofs, bool any_slope,
+                                           const LocalSolverSettings &cfg,
+                                           const std::function<void(const char *)> &phase) {
     auto pre = std::make_shared<Precond>();
     pre->kind = Precond::ADDITIVE;
-    pre->n = d.n_dofs;
+    pre->n = n_dofs;
     auto S = std::make_shared<Precond::Schwarz>();
-    S->n_dofs = d.n_dofs;
+    S->n_dofs = n_dofs;
     // partition of unity 1/sqrt(count), only without slope channels (factor_pairs.rs:84-89,155-194)
-    std::vector<uint32_t> counts(d.n_dofs, 0);
+    std::vector<uint32_t> counts(n_dofs, 0);
     for (auto &c : comps)
         for (uint32_t g : c.glob) counts[g]++;
     const uint64_t nc = comps.size();
@@ -1066,6 +1082,68 @@ std::shared_ptr<Precond> build_additive(const Design &d, const std::vector<doubl
     pre->schwarz = S;
     phase("assembly");
     return pre;
+}
+
+// One general bipartite block [[diag_r, B], [B', diag_c]] (B: n_rows x n_cols row-major, any
+// signs) as an oriented, SDDM-converted component with global DOFs 0..n_rows + n_cols - 1, as
+// build_pair_components forms a slope pair's component (sddm.rs:86-106, 199-463).
+Component component_from_block(uint32_t nr, uint32_t nc, const double *B, const double *diag,
+                               const LocalSolverSettings &cfg, std::vector<std::string> &warnings) {
+    Component comp;
+    const bool swap = nc > nr;
+    const uint32_t ne = swap ? nc : nr, nk = swap ? nr : nc;
+    comp.ne = ne;
+    comp.nk = nk;
+    for (uint32_t e = 0; e < ne; ++e) comp.glob.push_back(swap ? nr + e : e);
+    for (uint32_t k = 0; k < nk; ++k) comp.glob.push_back(swap ? k : nr + k);
+    comp.cptr.push_back(0);
+    for (uint32_t e = 0; e < ne; ++e) {
+        for (uint32_t k = 0; k < nk; ++k) {
+            const double v = swap ? B[(uint64_t)k * nc + e] : B[(uint64_t)e * nc + k];
+            if (v != 0.0) {
+                comp.cidx.push_back(k);
+                comp.cval.push_back(v);
+            }
+        }
+        comp.cptr.push_back(comp.cidx.size());
+    }
+    for (uint32_t i = 0; i < ne + nk; ++i) comp.diag.push_back(diag[comp.glob[i]]);
+    int64_t sweeps = 0;
+    double violation = 0.0;
+    if (!convert_general(comp, cfg, sweeps, violation))
+        fail(WX_ERR_BUILD, "signed component is not diagonally scalable to SDDM form", "BuildError::UnscalableComponent");
+    if (violation > cfg.scaling_tolerance) warnings.push_back("dominance scaling uncertified");
+    return comp;
+}
+
+} // namespace
+
+std::shared_ptr<Precond> build_block_precond(uint32_t n_rows, uint32_t n_cols, const double *B, const double *diag,
+                                             const LocalSolverSettings &cfg) {
+    std::vector<std::string> warnings;
+    std::vector<Component> comps;
+    comps.push_back(component_from_block(n_rows, n_cols, B, diag, cfg, warnings));
+    return assemble_additive(comps, (uint64_t)n_rows + n_cols, true, cfg, [](const char *) {});
+}
+
+std::vector<double> block_reduced_laplacian(uint32_t n_rows, uint32_t n_cols, const double *B, const double *diag,
+                                            const LocalSolverSettings &cfg, uint32_t &n_out) {
+    std::vector<std::string> warnings;
+    Component cm = component_from_block(n_rows, n_cols, B, diag, cfg, warnings);
+    const Component cover = cm.signed_form ? make_cover(cm) : Component{};
+    const Component &lf = cm.signed_form ? cover : cm;
+    std::vector<Edge> edges;
+    schur_edges(lf, cfg, 0, edges);
+    const uint32_t n = lf.nk + (lf.grounded ? 1 : 0);
+    n_out = n;
+    std::vector<double> L((size_t)n * n, 0.0);
+    for (const Edge &e : edges) {
+        L[(size_t)e.lo * n + e.hi] -= e.w;
+        L[(size_t)e.hi * n + e.lo] -= e.w;
+        L[(size_t)e.lo * n + e.lo] += e.w;
+        L[(size_t)e.hi * n + e.hi] += e.w;
+    }
+    return L;
 }
 
 } // namespace wx

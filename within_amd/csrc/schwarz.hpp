@@ -108,6 +108,13 @@ bool dense_schur_pinv_gpu(const DenseSchurArgs &a, bool floating, double *out);
 // Diagonal preconditioner (operator/schwarz.rs:185-229).
 std::shared_ptr<Precond> build_diagonal(const Design &d, const std::vector<double> &weights);
 
+// Test hooks (include/within_amd_testing.h): the builder's component pipeline on one general
+// bipartite block [[diag_r, B], [B', diag_c]] given directly.
+std::shared_ptr<Precond> build_block_precond(uint32_t n_rows, uint32_t n_cols, const double *B, const double *diag,
+                                             const LocalSolverSettings &cfg);
+std::vector<double> block_reduced_laplacian(uint32_t n_rows, uint32_t n_cols, const double *B, const double *diag,
+                                            const LocalSolverSettings &cfg, uint32_t &n_out);
+
 // Versioned binary format of a preconditioner.
 std::vector<uint8_t> serialize_precond(const Precond &p);
 std::shared_ptr<Precond> deserialize_precond(const uint8_t *buf, uint64_t len);
