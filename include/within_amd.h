@@ -188,6 +188,14 @@ void wx_solver_release(wx_solver *s);
 wx_status wx_gram_device(const double *A, uint64_t n, uint64_t m, uint64_t lda, double *G, void *stream);
 wx_status wx_gram(const double *A, uint64_t n, uint64_t m, uint64_t lda, double *G);
 
+/* wx_gram_device without any device allocation, for callers that cannot allocate (code under
+ * HIP graph capture, serving loops, harnesses that forbid hipMalloc).  No counterpart in the
+ * reference.  Same arguments, kernels and launch grid as wx_gram_device, so G is bitwise the
+ * same; 1 <= m <= 32.  The partial sums live in a static device array of 8.6 MB per GPU, held
+ * until the process ends and shared by every call on that GPU: calls that overlap on one GPU
+ * (from several threads or streams) race and must be serialised by the caller. */
+wx_status wx_gram_device_noalloc(const double *A, uint64_t n, uint64_t m, uint64_t lda, double *G, void *stream);
+
 /* A Solver for the same design with new observation weights (NULL: unweighted), as
  * wx_solver_new(design, weights, cfg, prebuilt) would build it, reusing `base`'s design and
  * device index structures (copied on the device) so that only the weights and the

@@ -86,6 +86,7 @@ EXPORTS = {
     "wx_solver_release": ([C.c_void_p], None),
     "wx_gram_device": ([C.c_void_p, C.c_uint64, C.c_uint64, C.c_uint64, C.c_void_p, C.c_void_p], C.c_int),
     "wx_gram": ([C.c_void_p, C.c_uint64, C.c_uint64, C.c_uint64, C.c_void_p], C.c_int),
+    "wx_gram_device_noalloc": ([C.c_void_p, C.c_uint64, C.c_uint64, C.c_uint64, C.c_void_p, C.c_void_p], C.c_int),
     "wx_solver_n_obs": ([C.c_void_p], C.c_uint64),
     "wx_solver_n_dofs": ([C.c_void_p], C.c_uint64),
     "wx_solver_time_setup": ([C.c_void_p], C.c_double),

@@ -328,6 +328,25 @@ wx_status wx_gram_device(const double *A, uint64_t n, uint64_t m, uint64_t lda, 
     });
 }
 
+wx_status wx_gram_device_noalloc(const double *A, uint64_t n, uint64_t m, uint64_t lda, double *G, void *stream) {
+    return guard([&] {
+        if (!A || !G) fail(WX_ERR_INVALID_INPUT, "null matrix pointer");
+        if (m == 0 || m > gram_noalloc_max_cols())
+            fail(WX_ERR_INVALID_INPUT, "the allocation-free Gram matrix takes 1 to " +
+                                           std::to_string(gram_noalloc_max_cols()) + " columns");
+        if (lda < n) fail(WX_ERR_INVALID_INPUT, "leading dimension smaller than the number of rows");
+        require_device();
+        std::string err;
+        double *scratch = gram_scratch_ptr(err);
+        if (!scratch) fail(WX_ERR_DEVICE, err);
+        hipStream_t st = (hipStream_t)stream;
+        // the same kernels and grid as wx_gram_device, with the partial triangles in this
+        // device's static scratch; every partial the reduction reads is written by this call
+        gram_device(A, n, (uint32_t)m, lda, G, scratch, st);
+        WX_HIP(hipStreamSynchronize(st));
+    });
+}
+
 wx_status wx_gram(const double *A, uint64_t n, uint64_t m, uint64_t lda, double *G) {
     return guard([&] {
         if (!A || !G) fail(WX_ERR_INVALID_INPUT, "null matrix pointer");

@@ -8,6 +8,9 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <mutex>
+#include <string>
+#include <vector>
 
 #include "device.hpp"
 
@@ -18,6 +21,7 @@ constexpr int GB = 256;               // threads per workgroup
 constexpr uint32_t GRAM_LDS = 4160;   // doubles of the LDS chunk: rows x (m + 1)
 constexpr uint32_t GRAM_MAX_COLS = 64;
 constexpr uint32_t GRAM_MAX_BLOCKS = 2048;
+constexpr uint32_t GRAM_NOALLOC_MAX_COLS = 32; // columns of the allocation-free entry point
 constexpr uint32_t TB = 4;            // a thread's 4 x 4 block of column pairs
 
 // rows per LDS chunk: the largest power of two up to 64 with R * (m + 1) doubles in GRAM_LDS
@@ -169,6 +173,40 @@ uint64_t gram_scratch_doubles(uint64_t n, uint32_t m) {
     const uint32_t R = gram_chunk_rows(m);
     const uint64_t chunks = (n + R - 1) / R;
     return std::max<uint64_t>(1, std::min<uint64_t>(chunks, GRAM_MAX_BLOCKS)) * (uint64_t)m * (m + 1) / 2;
+}
+
+// Scratch of the allocation-free Gram entry point: the partial triangles of up to
+// GRAM_MAX_BLOCKS workgroups for up to GRAM_NOALLOC_MAX_COLS columns (8.6 MB), one copy per
+// GPU held until the process ends.  Host code never passes `gram_scratch` itself to a kernel:
+// in host code the name is a stand-in, not a device address; gram_scratch_ptr() returns this
+// device's address of its copy.
+__device__ __align__(256) double gram_scratch[(uint64_t)GRAM_MAX_BLOCKS * GRAM_NOALLOC_MAX_COLS *
+                                             (GRAM_NOALLOC_MAX_COLS + 1) / 2];
+
+uint32_t gram_noalloc_max_cols() { return GRAM_NOALLOC_MAX_COLS; }
+
+// This device's address of `gram_scratch`, looked up once per device (hipGetSymbolAddress with
+// the device current) and kept; nullptr with `err` set when the lookup fails.
+double *gram_scratch_ptr(std::string &err) {
+    static std::mutex mu;
+    static std::vector<double *> ptr; // per device
+    int dev = 0, count = 0;
+    if (hipGetDevice(&dev) != hipSuccess || hipGetDeviceCount(&count) != hipSuccess || dev < 0 || dev >= count) {
+        err = "no current HIP device";
+        return nullptr;
+    }
+    std::lock_guard<std::mutex> lk(mu);
+    if ((int)ptr.size() < count) ptr.resize(count, nullptr);
+    if (!ptr[dev]) {
+        void *p = nullptr;
+        const hipError_t rc = hipGetSymbolAddress(&p, HIP_SYMBOL(gram_scratch));
+        if (rc != hipSuccess || !p) {
+            err = std::string("hipGetSymbolAddress(gram_scratch) failed: ") + hipGetErrorString(rc);
+            return nullptr;
+        }
+        ptr[dev] = (double *)p;
+    }
+    return ptr[dev];
 }
 
 void gram_device(const double *A, uint64_t n, uint32_t m, uint64_t lda, double *G, double *part, hipStream_t st) {

@@ -128,6 +128,8 @@ void reweight_design(Solver &s, const Solver &base, const std::vector<double> &w
 uint32_t gram_max_cols();
 uint64_t gram_scratch_doubles(uint64_t n, uint32_t m);
 void gram_device(const double *A, uint64_t n, uint32_t m, uint64_t lda, double *G, double *part, hipStream_t st);
+uint32_t gram_noalloc_max_cols();
+double *gram_scratch_ptr(std::string &err);
 
 uint64_t next_solver_uid();
 
