@@ -77,6 +77,64 @@ def test_operator_matches_oracle_bitwise(gpu, oracle, weighted, k):
     np.testing.assert_allclose(lhs, rhs, rtol=1e-11)
 
 
+def _pair_design(seed=23, n=200_000):
+    """Three factors where the two unsorted ones (B: 600 levels, C: 9) qualify for the fused D'
+    (pieces = runs of equal C inside B's 64-row work items, at most n / 4 of them)."""
+    rng = np.random.default_rng(seed)
+    a = np.minimum((rng.pareto(1.1, n) * 30).astype(np.int64), 2999)
+    b = rng.integers(0, 600, n)
+    c = rng.integers(0, 9, n)
+    codes = np.asfortranarray(np.column_stack([a, b, c]).astype(np.uint32))
+    # the library's rule, restated: internal rows by A (stable), B's positions in (B, C) order,
+    # work items of at most 64 positions per B level, a piece per run of equal C in an item
+    rows = np.argsort(a, kind="stable")
+    bi, ci = b[rows], c[rows]
+    pos = np.lexsort((ci, bi))
+    bp, cp = bi[pos], ci[pos]
+    starts = np.searchsorted(bp, np.arange(601))
+    pieces = 0
+    for l in range(600):
+        for p0 in range(starts[l], starts[l + 1], 64):
+            seg = cp[p0:min(starts[l + 1], p0 + 64)]
+            pieces += 1 + int(np.count_nonzero(seg[1:] != seg[:-1]))
+    assert pieces <= n // 4
+    return rng, codes
+
+
+@pytest.mark.parametrize("weighted", [False, True])
+@pytest.mark.parametrize("k", [2, 3, 16])
+def test_fused_pair_adjoint_matches_oracle(gpu, oracle, weighted, k):
+    """D' of the two unsorted terms in one pass (even widths; odd widths reduce term by term)."""
+    rng, codes = _pair_design()
+    n = codes.shape[0]
+    w = rng.uniform(0.5, 2.0, n) if weighted else None
+    s = wa.Solver(codes, weights=w, preconditioner=wa.PreconditionerConfig.Off)
+    d = oracle.design(codes)
+    sw = None if w is None else np.sqrt(w)
+    U = rng.standard_normal((n, k))
+    X = rng.standard_normal((s.n_dofs, k))
+    got_t = apply_op(s, U, True)
+    got = apply_op(s, X, False)
+    for j in range(k):
+        np.testing.assert_allclose(got_t[:, j], oracle.scatter(d, U[:, j], sw), rtol=1e-13, atol=1e-11)
+    np.testing.assert_allclose(np.sum(got * U, axis=0), np.sum(X * got_t, axis=0), rtol=1e-11)
+    # bitwise reproducible run to run
+    assert np.array_equal(apply_op(s, U, True), got_t)
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_fused_pair_solve_matches_exact(gpu, precision):
+    rng, codes = _pair_design(seed=29)
+    n = codes.shape[0]
+    w = rng.uniform(0.5, 2.0, n)
+    Y = np.asfortranarray(rng.standard_normal((n, 4)))
+    r = wa.solve_batch(codes, Y, weights=w, options=wa.LsmrOptions(tol=1e-10, precision=precision))
+    assert all(r.converged)
+    ex = exact_demean(sparse_design(codes), Y, w)
+    for j in range(4):
+        assert rel(r.demeaned[:, j], ex[:, j]) <= 1e-8, j
+
+
 # ---------------------------------------------------------------------------
 # full solves
 # ---------------------------------------------------------------------------

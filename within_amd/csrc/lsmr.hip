@@ -518,6 +518,98 @@ __global__ __launch_bounds__(BLK) void k_segreduce2(TermInfo ti, const uint32_t 
     }
 }
 
+// The primary term's reduction of a fused pair (PairPlan): k_segreduce2 over P's rows in (P, Q)
+// order, plus the sum of every run of equal Q codes inside the item (a piece), written to
+// pieces[piece] in run order from the item's first piece.  Each row's (scaled) u is read once
+// and added to both sums; P's sums are exactly k_segreduce2's for the same row order.
+template <bool SCALE>
+__global__ __launch_bounds__(BLK) void k_segreduce2_pair(TermInfo ti, const uint32_t *__restrict__ perm,
+                                                         const SegItem *__restrict__ items, uint32_t n_items,
+                                                         const uint32_t *__restrict__ piece0,
+                                                         const uint32_t *__restrict__ qcode,
+                                                         const double *__restrict__ scale_pos,
+                                                         const double *__restrict__ src, double *__restrict__ out,
+                                                         double *__restrict__ partials, double *__restrict__ pieces,
+                                                         const int32_t *__restrict__ active, uint32_t k,
+                                                         uint32_t kshift) {
+    const uint32_t kv = (1u << kshift) >> 1;
+    const uint32_t j0 = 2 * (threadIdx.x & (kv - 1));
+    if (j0 >= k || (active && !active[j0] && !active[j0 + 1])) return;
+    const uint32_t groups = (gridDim.x * BLK) / kv;
+    for (uint32_t it = (blockIdx.x * BLK + threadIdx.x) / kv; it < n_items; it += groups) {
+        const SegItem item = items[it];
+        const bool any = item.begin < item.end; // an unobserved level still writes its zero sum
+        double a0 = 0.0, a1 = 0.0, r0 = 0.0, r1 = 0.0;
+        uint32_t piece = piece0[it], cq = any ? qcode[item.begin] : 0u;
+        constexpr uint32_t U = 8;
+        uint32_t p = item.begin;
+        for (; p + U <= item.end; p += U) {
+            uint32_t row[U], qc[U];
+            double sc[U];
+            dbl2 b[U];
+#pragma unroll
+            for (int q = 0; q < (int)U; ++q) {
+                row[q] = perm[p + q];
+                qc[q] = qcode[p + q];
+                sc[q] = SCALE ? scale_pos[p + q] : 1.0;
+            }
+#pragma unroll
+            for (int q = 0; q < (int)U; ++q) b[q] = *(const dbl2 *)&src[(uint64_t)row[q] * k + j0];
+#pragma unroll
+            for (int q = 0; q < (int)U; ++q) {
+                const double x0 = SCALE ? sc[q] * b[q].x : b[q].x;
+                const double x1 = SCALE ? sc[q] * b[q].y : b[q].y;
+                a0 += x0;
+                a1 += x1;
+                if (qc[q] != cq) {
+                    dbl2 rv;
+                    rv.x = r0;
+                    rv.y = r1;
+                    *(dbl2 *)&pieces[(uint64_t)piece * k + j0] = rv;
+                    ++piece;
+                    r0 = 0.0;
+                    r1 = 0.0;
+                    cq = qc[q];
+                }
+                r0 += x0;
+                r1 += x1;
+            }
+        }
+        for (; p < item.end; ++p) {
+            const uint64_t row = perm[p];
+            const uint32_t qc = qcode[p];
+            const dbl2 b = *(const dbl2 *)&src[row * k + j0];
+            const double x0 = SCALE ? scale_pos[p] * b.x : b.x;
+            const double x1 = SCALE ? scale_pos[p] * b.y : b.y;
+            a0 += x0;
+            a1 += x1;
+            if (qc != cq) {
+                dbl2 rv;
+                rv.x = r0;
+                rv.y = r1;
+                *(dbl2 *)&pieces[(uint64_t)piece * k + j0] = rv;
+                ++piece;
+                r0 = 0.0;
+                r1 = 0.0;
+                cq = qc;
+            }
+            r0 += x0;
+            r1 += x1;
+        }
+        if (any) {
+            dbl2 rv;
+            rv.x = r0;
+            rv.y = r1;
+            *(dbl2 *)&pieces[(uint64_t)piece * k + j0] = rv;
+        }
+        dbl2 av;
+        av.x = a0;
+        av.y = a1;
+        if (item.dest >= 0) *(dbl2 *)&out[(ti.offset + item.level) * k + j0] = av;
+        else if (item.dest != SEG_SKIP) *(dbl2 *)&partials[(uint64_t)(-item.dest - 1) * k + j0] = av;
+    }
+}
+
 template <int NC>
 __global__ void k_segfix(TermInfo ti, const SegFix *__restrict__ fixes, uint32_t n_fixes,
                          const double *__restrict__ partials, double *__restrict__ out,
@@ -1255,6 +1347,151 @@ DesignArgs DevDesign::args() const {
 // ---------------------------------------------------------------------------
 // Design upload: codes, slopes, weights and the per-term D' segment plans
 // ---------------------------------------------------------------------------
+namespace {
+// D' plan of one term (or of the pieces of a fused pair) over positions in its level-sorted
+// order, level l at positions [start[l], start[l + 1]): a work item per level of at most
+// SEG_CHUNK positions, larger levels split into SEG_CHUNK chunks whose partial sums are
+// combined in a fixed order (a one-stage fixup per level, or for single-column terms a
+// sequence of segmented-sum stages of at most FIX_CHUNK partials each).  Returns the partial
+// slots the plan needs; `items` receives the host copy of the work items.
+uint64_t build_seg_plan(DevTerm &dt, const std::vector<uint64_t> &start, uint64_t n_levels,
+                        std::vector<SegItem> &items) {
+    std::vector<SegFix> fixes;
+    uint32_t partials = 0;
+    items.clear();
+    items.reserve(n_levels + start[n_levels] / SEG_CHUNK + 1);
+    for (uint64_t l = 0; l < n_levels; ++l) {
+        uint64_t b = start[l], e = start[l + 1];
+        if (e - b <= SEG_CHUNK) {
+            items.push_back({(uint32_t)l, (uint32_t)b, (uint32_t)e, (int32_t)l});
+        } else {
+            uint32_t first = partials;
+            uint32_t count = 0;
+            for (uint64_t p = b; p < e; p += SEG_CHUNK) {
+                items.push_back({(uint32_t)l, (uint32_t)p, (uint32_t)std::min<uint64_t>(e, p + SEG_CHUNK),
+                                 -(int32_t)(partials + 1)});
+                ++partials;
+                ++count;
+            }
+            fixes.push_back({(uint32_t)l, first, count, 0});
+        }
+    }
+    dt.items.upload(items.data(), items.size());
+    dt.n_items = (uint32_t)items.size();
+    dt.fixes.upload(fixes.data(), fixes.size());
+    dt.n_fixes = (uint32_t)fixes.size();
+    dt.n_partials = partials;
+    uint64_t slots = (uint64_t)partials * dt.info.ncols;
+    dt.stages.clear();
+    if (dt.info.ncols == 1) {
+        // combine stages: a split level with <= FIX_CHUNK partials is summed into its output,
+        // a larger one into ceil(count / FIX_CHUNK) partials of the next region
+        std::vector<SegFix> cur = fixes;
+        uint64_t in_base = 0, out_base = partials;
+        while (!cur.empty()) {
+            std::vector<SegItem> st_items;
+            std::vector<SegFix> next;
+            uint32_t np = 0;
+            for (const SegFix &f : cur) {
+                if (f.count <= FIX_CHUNK) {
+                    st_items.push_back({f.level, f.first, f.first + f.count, (int32_t)f.level});
+                    continue;
+                }
+                const uint32_t first = np;
+                uint32_t cnt = 0;
+                for (uint32_t q = 0; q < f.count; q += FIX_CHUNK) {
+                    st_items.push_back({f.level, f.first + q, f.first + std::min(f.count, q + FIX_CHUNK),
+                                        -(int32_t)(np + 1)});
+                    ++np;
+                    ++cnt;
+                }
+                next.push_back({f.level, first, cnt, 0});
+            }
+            DevTerm::FixStage stage;
+            stage.items.upload(st_items.data(), st_items.size());
+            stage.n_items = (uint32_t)st_items.size();
+            stage.src_base = in_base;
+            stage.dst_base = out_base;
+            dt.stages.push_back(std::move(stage));
+            in_base = out_base;
+            out_base += np;
+            cur.swap(next);
+        }
+        slots = out_base;
+    }
+    return slots;
+}
+
+// The fused D' of two unsorted single-intercept terms of a plain design (k_segreduce2_pair):
+// the primary term P (the unsorted term with the most levels) is reduced over its rows in
+// (P, Q) order, Q being the unsorted term with the fewest levels.  A run of equal Q codes
+// inside one of P's work items is a piece; the pass that sums P's levels also writes every
+// piece's sum, and Q's D' is then a segmented sum over the pieces in Q order.  So u is read
+// once for both terms, and the extra traffic is one write and one read of the pieces.  Used
+// only when the pieces number at most a quarter of the rows.
+void build_pair_plan(const Design &d, const std::vector<double> &w, std::vector<std::vector<SegItem>> &host_items,
+                     DevDesign &dv) {
+    dv.pair = PairPlan{};
+    if (!dv.plain) return;
+    int32_t P = -1, Q = -1;
+    for (size_t t = 0; t < d.terms.size(); ++t) {
+        if (d.terms[t].sorted) continue;
+        if (P < 0 || d.terms[t].n_levels > d.terms[P].n_levels) P = (int32_t)t;
+        if (Q < 0 || d.terms[t].n_levels <= d.terms[Q].n_levels) Q = (int32_t)t;
+    }
+    if (P < 0 || Q < 0 || P == Q) return;
+    const uint64_t n = d.n_obs, LQ = d.terms[Q].n_levels, LP = d.terms[P].n_levels;
+    const uint32_t *cp = d.level_column(P), *cq = d.level_column(Q);
+    // rows in (P, Q) order: stable by Q, then a stable counting sort by P
+    const std::vector<uint32_t> byq = stable_argsort(cq, n, LQ);
+    std::vector<uint64_t> cur(LP + 1, 0);
+    for (uint64_t i = 0; i < n; ++i) cur[cp[i] + 1]++;
+    for (uint64_t l = 0; l < LP; ++l) cur[l + 1] += cur[l];
+    std::vector<uint32_t> perm(n);
+    for (uint64_t q = 0; q < n; ++q) perm[cur[cp[byq[q]]]++] = byq[q];
+    std::vector<uint32_t> qpos(n);
+    for (uint64_t q = 0; q < n; ++q) qpos[q] = cq[perm[q]];
+    // pieces: runs of equal Q code inside each of P's items (the kernel's rule)
+    const std::vector<SegItem> &items = host_items[P];
+    std::vector<uint32_t> piece0(items.size() + 1), piece_q;
+    for (size_t it = 0; it < items.size(); ++it) {
+        piece0[it] = (uint32_t)piece_q.size();
+        for (uint32_t q = items[it].begin; q < items[it].end; ++q)
+            if (q == items[it].begin || qpos[q] != qpos[q - 1]) piece_q.push_back(qpos[q]);
+        if (piece_q.size() > n / 4) return;
+    }
+    piece0[items.size()] = (uint32_t)piece_q.size();
+    PairPlan &pp = dv.pair;
+    pp.p = P;
+    pp.q = Q;
+    pp.n_pieces = (uint32_t)piece_q.size();
+    DevTerm &pt = dv.terms[P];
+    pt.perm.upload(perm.data(), perm.size());
+    if (!w.empty()) {
+        std::vector<double> wp(n), swp(n);
+        for (uint64_t q = 0; q < n; ++q) {
+            wp[q] = w[perm[q]];
+            swp[q] = std::sqrt(w[perm[q]]);
+        }
+        pt.w_pos.upload(wp.data(), wp.size());
+        pt.sw_pos.upload(swp.data(), swp.size());
+    }
+    pp.qcode_pos.upload(qpos.data(), qpos.size());
+    pp.item_piece0.upload(piece0.data(), piece0.size());
+    // Q's reduction over the pieces, in Q order
+    DevTerm &qt = pp.qt;
+    qt.info = dv.terms[Q].info;
+    const std::vector<uint32_t> pperm = stable_argsort(piece_q.data(), piece_q.size(), LQ);
+    qt.perm.upload(pperm.data(), pperm.size());
+    std::vector<uint64_t> start(LQ + 1, 0);
+    for (uint32_t c : piece_q) start[c + 1]++;
+    for (uint64_t l = 0; l < LQ; ++l) start[l + 1] += start[l];
+    std::vector<SegItem> qitems;
+    const uint64_t slots = build_seg_plan(qt, start, LQ, qitems);
+    dv.max_partials = std::max<uint32_t>(dv.max_partials, (uint32_t)slots);
+}
+} // namespace
+
 void upload_design(Solver &s, const std::vector<double> &weights_internal) {
     const Design &d = s.design;
     DevDesign &dv = s.dev;
@@ -1286,7 +1523,7 @@ void upload_design(Solver &s, const std::vector<double> &weights_internal) {
     dv.terms.clear();
     dv.terms.resize(d.terms.size());
     dv.max_partials = 0;
-    std::vector<std::vector<SegItem>> host_items(d.terms.size()); // of the fusable terms
+    std::vector<std::vector<SegItem>> host_items(d.terms.size()); // every term's D' work items
     dv.plain = true;
     for (const Term &tm : d.terms) dv.plain = dv.plain && tm.ncols() == 1 && tm.loads[0] < 0;
     for (size_t t = 0; t < d.terms.size(); ++t) {
@@ -1316,72 +1553,10 @@ void upload_design(Solver &s, const std::vector<double> &weights_internal) {
                 dt.sw_pos.upload(swp.data(), swp.size());
             }
         }
-        std::vector<SegItem> items;
-        std::vector<SegFix> fixes;
-        uint32_t partials = 0;
-        items.reserve(tm.n_levels + d.n_obs / SEG_CHUNK + 1);
-        for (uint64_t l = 0; l < tm.n_levels; ++l) {
-            uint64_t b = start[l], e = start[l + 1];
-            if (e - b <= SEG_CHUNK) {
-                items.push_back({(uint32_t)l, (uint32_t)b, (uint32_t)e, (int32_t)l});
-            } else {
-                uint32_t first = partials;
-                uint32_t count = 0;
-                for (uint64_t p = b; p < e; p += SEG_CHUNK) {
-                    items.push_back({(uint32_t)l, (uint32_t)p, (uint32_t)std::min<uint64_t>(e, p + SEG_CHUNK),
-                                     -(int32_t)(partials + 1)});
-                    ++partials;
-                    ++count;
-                }
-                fixes.push_back({(uint32_t)l, first, count, 0});
-            }
-        }
-        dt.items.upload(items.data(), items.size());
-        dt.n_items = (uint32_t)items.size();
-        if (tm.sorted && tm.ncols() == 1 && tm.loads[0] < 0) host_items[t] = items;
-        dt.fixes.upload(fixes.data(), fixes.size());
-        dt.n_fixes = (uint32_t)fixes.size();
-        dt.n_partials = partials;
-        uint64_t slots = (uint64_t)partials * dt.info.ncols;
-        dt.stages.clear();
-        if (dt.info.ncols == 1) {
-            // combine stages: a split level with <= FIX_CHUNK partials is summed into its output,
-            // a larger one into ceil(count / FIX_CHUNK) partials of the next region
-            std::vector<SegFix> cur = fixes;
-            uint64_t in_base = 0, out_base = partials;
-            while (!cur.empty()) {
-                std::vector<SegItem> st_items;
-                std::vector<SegFix> next;
-                uint32_t np = 0;
-                for (const SegFix &f : cur) {
-                    if (f.count <= FIX_CHUNK) {
-                        st_items.push_back({f.level, f.first, f.first + f.count, (int32_t)f.level});
-                        continue;
-                    }
-                    const uint32_t first = np;
-                    uint32_t cnt = 0;
-                    for (uint32_t q = 0; q < f.count; q += FIX_CHUNK) {
-                        st_items.push_back({f.level, f.first + q, f.first + std::min(f.count, q + FIX_CHUNK),
-                                            -(int32_t)(np + 1)});
-                        ++np;
-                        ++cnt;
-                    }
-                    next.push_back({f.level, first, cnt, 0});
-                }
-                DevTerm::FixStage stage;
-                stage.items.upload(st_items.data(), st_items.size());
-                stage.n_items = (uint32_t)st_items.size();
-                stage.src_base = in_base;
-                stage.dst_base = out_base;
-                dt.stages.push_back(std::move(stage));
-                in_base = out_base;
-                out_base += np;
-                cur.swap(next);
-            }
-            slots = out_base;
-        }
+        const uint64_t slots = build_seg_plan(dt, start, tm.n_levels, host_items[t]);
         dv.max_partials = std::max<uint32_t>(dv.max_partials, (uint32_t)slots);
     }
+    build_pair_plan(d, weights_internal, host_items, dv);
     // a row's codes as one aligned 16-byte record: the fused gather and the demean read them with
     // one load instead of one per term
     dv.codes4.release();
@@ -1435,6 +1610,26 @@ void clone_buf(DevBuf &dst, const DevBuf &src) {
     dst.alloc(src.bytes);
     WX_HIP(hipMemcpy(dst.p, src.p, src.bytes, hipMemcpyDeviceToDevice));
 }
+
+// A term's D' plan (everything but its position-ordered weights) copied on the device.
+void clone_term_plan(DevTerm &dt, const DevTerm &bt) {
+    dt.info = bt.info;
+    clone_buf(dt.perm, bt.perm);
+    clone_buf(dt.items, bt.items);
+    dt.n_items = bt.n_items;
+    clone_buf(dt.fixes, bt.fixes);
+    dt.n_fixes = bt.n_fixes;
+    dt.n_partials = bt.n_partials;
+    clone_buf(dt.gitems, bt.gitems);
+    dt.n_gitems = bt.n_gitems;
+    dt.stages.resize(bt.stages.size());
+    for (size_t q = 0; q < bt.stages.size(); ++q) {
+        clone_buf(dt.stages[q].items, bt.stages[q].items);
+        dt.stages[q].n_items = bt.stages[q].n_items;
+        dt.stages[q].src_base = bt.stages[q].src_base;
+        dt.stages[q].dst_base = bt.stages[q].dst_base;
+    }
+}
 } // namespace
 
 // The device design of `base` with new observation weights (internal order; empty:
@@ -1473,25 +1668,19 @@ void reweight_design(Solver &s, const Solver &base, const std::vector<double> &w
     dv.terms.clear();
     dv.terms.resize(b.terms.size());
     std::vector<uint32_t> perm;
+    // the fused pair's index structures (its primary term's weights follow from that term's
+    // permutation below)
+    dv.pair = PairPlan{};
+    dv.pair.p = b.pair.p;
+    dv.pair.q = b.pair.q;
+    dv.pair.n_pieces = b.pair.n_pieces;
+    clone_buf(dv.pair.qcode_pos, b.pair.qcode_pos);
+    clone_buf(dv.pair.item_piece0, b.pair.item_piece0);
+    clone_term_plan(dv.pair.qt, b.pair.qt);
     for (size_t t = 0; t < b.terms.size(); ++t) {
         const DevTerm &bt = b.terms[t];
         DevTerm &dt = dv.terms[t];
-        dt.info = bt.info;
-        clone_buf(dt.perm, bt.perm);
-        clone_buf(dt.items, bt.items);
-        dt.n_items = bt.n_items;
-        clone_buf(dt.fixes, bt.fixes);
-        dt.n_fixes = bt.n_fixes;
-        dt.n_partials = bt.n_partials;
-        clone_buf(dt.gitems, bt.gitems);
-        dt.n_gitems = bt.n_gitems;
-        dt.stages.resize(bt.stages.size());
-        for (size_t q = 0; q < bt.stages.size(); ++q) {
-            clone_buf(dt.stages[q].items, bt.stages[q].items);
-            dt.stages[q].n_items = bt.stages[q].n_items;
-            dt.stages[q].src_base = bt.stages[q].src_base;
-            dt.stages[q].dst_base = bt.stages[q].dst_base;
-        }
+        clone_term_plan(dt, bt);
         if (bt.perm.p && !w.empty()) { // weights in this term's sorted order
             perm.resize(b.n);
             WX_HIP(hipMemcpy(perm.data(), bt.perm.p, b.n * sizeof(uint32_t), hipMemcpyDeviceToHost));
@@ -1543,13 +1732,41 @@ void launch_segfix(const DevTerm &dt, double *partials, double *out, const int32
 
 // Launches the D' reduction of every term (but `skip`): out (N x k) = D' (f .* src) with
 // f = 1 (which = 0), sqrt(w) (which = 1) or w (which = 2).
+// `pieces` (n_pieces x k, may be NULL) holds the fused pair's piece sums; the pair runs fused
+// when it is given and the width is even.
 void launch_adjoint(const DevDesign &dv, const DesignArgs &da, int which, const double *src, double *out,
                     double *partials, const int32_t *active, uint32_t k, uint32_t kshift, hipStream_t st,
-                    int32_t skip = -1) {
+                    int32_t skip = -1, double *pieces = nullptr) {
+    const PairPlan &pp = dv.pair;
+    const bool pair = pp.p >= 0 && pieces && k >= 2 && (k & 1) == 0 && pp.p != skip && pp.q != skip;
     for (size_t t = 0; t < dv.terms.size(); ++t) {
-        if ((int32_t)t == skip) continue;
+        if ((int32_t)t == skip || (pair && (int32_t)t == pp.q)) continue;
         const DevTerm &dt = dv.terms[t];
         const uint32_t groups_per_block = BLK >> kshift;
+        if (pair && (int32_t)t == pp.p) {
+            // P's levels and the pieces in one pass over u, then Q's levels from the pieces
+            const double *f = which == 1 ? dt.sw_pos.as<double>() : which == 2 ? dt.w_pos.as<double>() : nullptr;
+            const uint32_t grid2 = std::max<uint32_t>(
+                1, std::min<uint32_t>((dt.n_items + 2 * groups_per_block - 1) / (2 * groups_per_block), 65535u * 8));
+            if (f)
+                hipLaunchKernelGGL((k_segreduce2_pair<true>), dim3(grid2), dim3(BLK), 0, st, dt.info,
+                                   dt.perm.as<uint32_t>(), dt.items.as<SegItem>(), dt.n_items,
+                                   pp.item_piece0.as<uint32_t>(), pp.qcode_pos.as<uint32_t>(), f, src, out, partials,
+                                   pieces, active, k, kshift);
+            else
+                hipLaunchKernelGGL((k_segreduce2_pair<false>), dim3(grid2), dim3(BLK), 0, st, dt.info,
+                                   dt.perm.as<uint32_t>(), dt.items.as<SegItem>(), dt.n_items,
+                                   pp.item_piece0.as<uint32_t>(), pp.qcode_pos.as<uint32_t>(), f, src, out, partials,
+                                   pieces, active, k, kshift);
+            launch_segfix(dt, partials, out, active, k, kshift, st);
+            const DevTerm &qt = pp.qt;
+            const uint32_t gq = std::max<uint32_t>(
+                1, std::min<uint32_t>((qt.n_items + 2 * groups_per_block - 1) / (2 * groups_per_block), 65535u * 8));
+            hipLaunchKernelGGL((k_segreduce2<false>), dim3(gq), dim3(BLK), 0, st, qt.info, qt.perm.as<uint32_t>(),
+                               qt.items.as<SegItem>(), qt.n_items, nullptr, pieces, out, partials, active, k, kshift);
+            launch_segfix(qt, partials, out, active, k, kshift, st);
+            continue;
+        }
         uint32_t grid = (dt.n_items + groups_per_block - 1) / groups_per_block;
         grid = std::max<uint32_t>(1, std::min<uint32_t>(grid, 65535u * 8));
         const uint32_t *perm = dt.perm.p ? dt.perm.as<uint32_t>() : nullptr;
@@ -1756,6 +1973,8 @@ struct Workspace {
     uint32_t k = 0;
     uint32_t cap = MAX_BATCH; // column-state entries (packed members x k)
     DevBuf u, v, pt, atu, h, hb, x, yint, part, segpart, cols, ints, bad, red;
+    DevBuf pieces; // the fused pair's piece sums (n_pieces x k)
+    double *pieces_or_null() { return pieces.p ? pieces.as<double>() : nullptr; }
     ColState cs{};
     std::unique_ptr<SchwarzRun> runs[2];
     // pinned copies of the per-column "active" flags of the last two iteration groups, and
@@ -1825,6 +2044,7 @@ std::shared_ptr<Workspace> new_workspace(const DevDesign &dv, uint32_t k, uint32
     for (DevBuf *b : {&w->v, &w->pt, &w->atu, &w->h, &w->hb, &w->x}) b->alloc(N * k * 8);
     w->part.alloc((size_t)std::max<uint64_t>(std::max(MAX_PART_BLOCKS, MAX_FUSED_BLOCKS), part_blocks) * kp * 3 * 8);
     w->segpart.alloc((size_t)std::max<uint32_t>(dv.max_partials, 1) * k * 8);
+    if (dv.pair.p >= 0) w->pieces.alloc((size_t)std::max<uint32_t>(dv.pair.n_pieces, 1) * k * 8);
     w->bad.alloc(8);
     w->red.alloc(2 * (size_t)w->cap * 8);
     WX_HIP(hipHostMalloc((void **)&w->hflags, (2 * (size_t)w->cap + 2) * sizeof(int32_t), hipHostMallocDefault));
@@ -1963,7 +2183,7 @@ class ChunkRun {
         }
         // init: p~ = A'b / beta_1, v = M^{-1} p~, alpha_1
         launch_adjoint(dv_, da_, weighted_ ? 2 : 0, ws_->yint.as<double>(), ws_->pt.as<double>(),
-                       ws_->segpart.as<double>(), gate(), k_, kshift_, st_);
+                       ws_->segpart.as<double>(), gate(), k_, kshift_, st_, -1, ws_->pieces_or_null());
         const uint64_t N = N_;
         if (prec_kind_ == 2) {
             hipLaunchKernelGGL((k_ptilde<2, true>), dim3(gN_), dim3(BLK), 0, st_, N, nullptr, ws_->pt.as<double>(),
@@ -2072,7 +2292,7 @@ class ChunkRun {
         // residual numerator ||D'W(y - Dx)||; the denominator ||D'W y|| = beta_1 ||p~_1|| was
         // kept at initialisation
         launch_adjoint(dv_, da, weighted_ ? 2 : 0, dm_int, ws.h.as<double>(), ws.segpart.as<double>(), nullptr, k,
-                       kshift, st);
+                       kshift, st, -1, ws.pieces_or_null());
         hipLaunchKernelGGL(k_sumsq, dim3(gN_), dim3(BLK), 0, st, N, ws.h.as<double>(), part_, k, kshift);
         hipLaunchKernelGGL(k_reduce_to, dim3(ncol_), dim3(BLK), 0, st, part_, gN_, kp, ws.red.as<double>(), k, rg_dof_);
         WX_HIP(hipGetLastError());
@@ -2200,7 +2420,7 @@ class ChunkRun {
             launch_segfix(dv_.terms[dv_.fuse_term], ws.segpart.as<double>(), ws.atu.as<double>(), gate(), k, kshift,
                           st);
         launch_adjoint(dv_, da_, weighted_ ? 1 : 0, ws.u.as<double>(), ws.atu.as<double>(), ws.segpart.as<double>(),
-                       gate(), k, kshift, st, dv_.fuse_term);
+                       gate(), k, kshift, st, dv_.fuse_term, ws.pieces_or_null());
         if (timed) timer_->end(1);
         if (prec_kind_ == 2) {
             hipLaunchKernelGGL((k_ptilde<2, false>), dim3(gN_), dim3(BLK), 0, st, N, ws.atu.as<double>(),
@@ -2696,10 +2916,11 @@ void design_apply_device(const Solver &s, bool adjoint, const double *X, uint64_
     for (uint64_t c0 = 0; c0 < k_total; c0 += MAX_BATCH) {
         const uint32_t k = (uint32_t)std::min<uint64_t>(MAX_BATCH, k_total - c0);
         const uint32_t kshift = pow2_shift(k);
-        DevBuf in, out, part;
+        DevBuf in, out, part, pieces;
         in.alloc((adjoint ? n : N) * k * 8);
         out.alloc((adjoint ? N : n) * k * 8);
         part.alloc((uint64_t)std::max<uint32_t>(dv.max_partials, 1) * k * 8);
+        if (adjoint && dv.pair.p >= 0) pieces.alloc((uint64_t)std::max<uint32_t>(dv.pair.n_pieces, 1) * k * 8);
         if (!adjoint) {
             hipLaunchKernelGGL(k_interleave, dim3((N * k + BLK - 1) / BLK), dim3(BLK), 0, st, N, X + c0 * N, N,
                                nullptr, in.as<double>(), k);
@@ -2715,8 +2936,8 @@ void design_apply_device(const Solver &s, bool adjoint, const double *X, uint64_
         } else {
             hipLaunchKernelGGL(k_interleave, dim3((n * k + BLK - 1) / BLK), dim3(BLK), 0, st, n, X + c0 * n, n,
                                obs_perm, in.as<double>(), k);
-            launch_adjoint(dv, da, weighted ? 1 : 0, in.as<double>(),
-                           out.as<double>(), part.as<double>(), nullptr, k, kshift, st);
+            launch_adjoint(dv, da, weighted ? 1 : 0, in.as<double>(), out.as<double>(), part.as<double>(), nullptr, k,
+                           kshift, st, -1, pieces.p ? pieces.as<double>() : nullptr);
             hipLaunchKernelGGL(k_deinterleave, dim3((N * k + BLK - 1) / BLK), dim3(BLK), 0, st, N, out.as<double>(),
                                Y + c0 * N, N, nullptr, k);
         }

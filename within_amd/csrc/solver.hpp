@@ -75,6 +75,17 @@ struct DevTerm {
     std::vector<FixStage> stages;
 };
 
+// The fused D' of two unsorted terms P and Q of a plain design: P's rows in (P, Q) order, a
+// piece per run of equal Q codes inside one of P's work items, and Q's D' as a segmented sum over
+// the pieces (lsmr.hip, build_pair_plan).
+struct PairPlan {
+    int32_t p = -1, q = -1;
+    DevBuf qcode_pos;   // u32[n]: Q's code at each of P's positions
+    DevBuf item_piece0; // u32[P's n_items + 1]: first piece of each of P's work items
+    uint32_t n_pieces = 0;
+    DevTerm qt;         // Q's reduction over the pieces: perm = pieces in Q order, items, fixups
+};
+
 struct DevDesign {
     uint64_t n = 0, N = 0;
     DevBuf codes, slopes, sw, w, obs_perm, inv_perm, inv_diag;
@@ -84,6 +95,7 @@ struct DevDesign {
     uint32_t max_partials = 0;
     bool plain = false; // every term is a single intercept column
     int32_t fuse_term = -1; // sorted single-intercept term whose D' is fused into the gather
+    PairPlan pair;          // p < 0: no fused pair
     DesignArgs args() const;
 };
 
