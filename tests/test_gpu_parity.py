@@ -79,7 +79,7 @@ def test_operator_matches_oracle_bitwise(gpu, oracle, weighted, k):
 
 def _pair_design(seed=23, n=200_000):
     """Three factors where the two unsorted ones (B: 600 levels, C: 9) qualify for the fused D'
-    (pieces = runs of equal C inside B's 64-row work items, at most n / 4 of them)."""
+    (pieces = runs of equal C inside B's work items, at most n / 4 of them)."""
     rng = np.random.default_rng(seed)
     a = np.minimum((rng.pareto(1.1, n) * 30).astype(np.int64), 2999)
     b = rng.integers(0, 600, n)

@@ -30,6 +30,9 @@ namespace {
 constexpr int BLK = (int)KERNEL_BLOCK;
 constexpr uint32_t SEG_CHUNK = 64;  // rows per D' work item before a level is split
 constexpr uint32_t FIX_CHUNK = 64;  // partials per combine item
+// rows per work item of a fused pair's primary term (64, 128, 256, 512 and 1,024 measured the
+// same or slower at config 2: fewer, longer items keep fewer loads in flight)
+constexpr uint32_t PAIR_CHUNK = 64;
 constexpr uint32_t MAX_FUSED_BLOCKS = 32768; // grid cap of the fused gather (one item per row group)
 constexpr int GATHER_WAVE_STEPS = 2;             // row steps per pass of k_gather_wave
 constexpr uint32_t GATHER_WAVE_BLOCKS = 8192;    // its grid cap (4 waves, each looping over items)
@@ -526,6 +529,7 @@ template <bool SCALE>
 __global__ __launch_bounds__(BLK) void k_segreduce2_pair(TermInfo ti, const uint32_t *__restrict__ perm,
                                                          const SegItem *__restrict__ items, uint32_t n_items,
                                                          const uint32_t *__restrict__ piece0,
+                                                         const uint32_t *__restrict__ piece_slot,
                                                          const uint32_t *__restrict__ qcode,
                                                          const double *__restrict__ scale_pos,
                                                          const double *__restrict__ src, double *__restrict__ out,
@@ -565,7 +569,7 @@ __global__ __launch_bounds__(BLK) void k_segreduce2_pair(TermInfo ti, const uint
                     dbl2 rv;
                     rv.x = r0;
                     rv.y = r1;
-                    *(dbl2 *)&pieces[(uint64_t)piece * k + j0] = rv;
+                    *(dbl2 *)&pieces[(uint64_t)piece_slot[piece] * k + j0] = rv;
                     ++piece;
                     r0 = 0.0;
                     r1 = 0.0;
@@ -587,7 +591,7 @@ __global__ __launch_bounds__(BLK) void k_segreduce2_pair(TermInfo ti, const uint
                 dbl2 rv;
                 rv.x = r0;
                 rv.y = r1;
-                *(dbl2 *)&pieces[(uint64_t)piece * k + j0] = rv;
+                *(dbl2 *)&pieces[(uint64_t)piece_slot[piece] * k + j0] = rv;
                 ++piece;
                 r0 = 0.0;
                 r1 = 0.0;
@@ -600,7 +604,7 @@ __global__ __launch_bounds__(BLK) void k_segreduce2_pair(TermInfo ti, const uint
             dbl2 rv;
             rv.x = r0;
             rv.y = r1;
-            *(dbl2 *)&pieces[(uint64_t)piece * k + j0] = rv;
+            *(dbl2 *)&pieces[(uint64_t)piece_slot[piece] * k + j0] = rv;
         }
         dbl2 av;
         av.x = a0;
@@ -1355,20 +1359,20 @@ namespace {
 // sequence of segmented-sum stages of at most FIX_CHUNK partials each).  Returns the partial
 // slots the plan needs; `items` receives the host copy of the work items.
 uint64_t build_seg_plan(DevTerm &dt, const std::vector<uint64_t> &start, uint64_t n_levels,
-                        std::vector<SegItem> &items) {
+                        std::vector<SegItem> &items, uint32_t chunk = SEG_CHUNK) {
     std::vector<SegFix> fixes;
     uint32_t partials = 0;
     items.clear();
-    items.reserve(n_levels + start[n_levels] / SEG_CHUNK + 1);
+    items.reserve(n_levels + start[n_levels] / chunk + 1);
     for (uint64_t l = 0; l < n_levels; ++l) {
         uint64_t b = start[l], e = start[l + 1];
-        if (e - b <= SEG_CHUNK) {
+        if (e - b <= chunk) {
             items.push_back({(uint32_t)l, (uint32_t)b, (uint32_t)e, (int32_t)l});
         } else {
             uint32_t first = partials;
             uint32_t count = 0;
-            for (uint64_t p = b; p < e; p += SEG_CHUNK) {
-                items.push_back({(uint32_t)l, (uint32_t)p, (uint32_t)std::min<uint64_t>(e, p + SEG_CHUNK),
+            for (uint64_t p = b; p < e; p += chunk) {
+                items.push_back({(uint32_t)l, (uint32_t)p, (uint32_t)std::min<uint64_t>(e, p + chunk),
                                  -(int32_t)(partials + 1)});
                 ++partials;
                 ++count;
@@ -1451,8 +1455,15 @@ void build_pair_plan(const Design &d, const std::vector<double> &w, std::vector<
     for (uint64_t q = 0; q < n; ++q) perm[cur[cp[byq[q]]]++] = byq[q];
     std::vector<uint32_t> qpos(n);
     for (uint64_t q = 0; q < n; ++q) qpos[q] = cq[perm[q]];
+    // P's work items of up to PAIR_CHUNK rows
+    std::vector<uint64_t> pstart(LP + 1, 0);
+    for (uint64_t i = 0; i < n; ++i) pstart[cp[i] + 1]++;
+    for (uint64_t l = 0; l < LP; ++l) pstart[l + 1] += pstart[l];
+    DevTerm pplan;
+    pplan.info = dv.terms[P].info;
+    std::vector<SegItem> items;
+    const uint64_t pslots = build_seg_plan(pplan, pstart, LP, items, PAIR_CHUNK);
     // pieces: runs of equal Q code inside each of P's items (the kernel's rule)
-    const std::vector<SegItem> &items = host_items[P];
     std::vector<uint32_t> piece0(items.size() + 1), piece_q;
     for (size_t it = 0; it < items.size(); ++it) {
         piece0[it] = (uint32_t)piece_q.size();
@@ -1466,6 +1477,13 @@ void build_pair_plan(const Design &d, const std::vector<double> &w, std::vector<
     pp.q = Q;
     pp.n_pieces = (uint32_t)piece_q.size();
     DevTerm &pt = dv.terms[P];
+    pt.items = std::move(pplan.items);
+    pt.n_items = pplan.n_items;
+    pt.fixes = std::move(pplan.fixes);
+    pt.n_fixes = pplan.n_fixes;
+    pt.n_partials = pplan.n_partials;
+    pt.stages = std::move(pplan.stages);
+    dv.max_partials = std::max<uint32_t>(dv.max_partials, (uint32_t)pslots);
     pt.perm.upload(perm.data(), perm.size());
     if (!w.empty()) {
         std::vector<double> wp(n), swp(n);
@@ -1478,11 +1496,14 @@ void build_pair_plan(const Design &d, const std::vector<double> &w, std::vector<
     }
     pp.qcode_pos.upload(qpos.data(), qpos.size());
     pp.item_piece0.upload(piece0.data(), piece0.size());
-    // Q's reduction over the pieces, in Q order
+    // the pieces are stored in Q order (piece -> its slot there), so Q's reduction reads them
+    // contiguously
     DevTerm &qt = pp.qt;
     qt.info = dv.terms[Q].info;
     const std::vector<uint32_t> pperm = stable_argsort(piece_q.data(), piece_q.size(), LQ);
-    qt.perm.upload(pperm.data(), pperm.size());
+    std::vector<uint32_t> slot(pperm.size());
+    for (uint32_t q = 0; q < (uint32_t)pperm.size(); ++q) slot[pperm[q]] = q;
+    pp.piece_slot.upload(slot.data(), slot.size());
     std::vector<uint64_t> start(LQ + 1, 0);
     for (uint32_t c : piece_q) start[c + 1]++;
     for (uint64_t l = 0; l < LQ; ++l) start[l + 1] += start[l];
@@ -1676,6 +1697,7 @@ void reweight_design(Solver &s, const Solver &base, const std::vector<double> &w
     dv.pair.n_pieces = b.pair.n_pieces;
     clone_buf(dv.pair.qcode_pos, b.pair.qcode_pos);
     clone_buf(dv.pair.item_piece0, b.pair.item_piece0);
+    clone_buf(dv.pair.piece_slot, b.pair.piece_slot);
     clone_term_plan(dv.pair.qt, b.pair.qt);
     for (size_t t = 0; t < b.terms.size(); ++t) {
         const DevTerm &bt = b.terms[t];
@@ -1746,23 +1768,25 @@ void launch_adjoint(const DevDesign &dv, const DesignArgs &da, int which, const 
         if (pair && (int32_t)t == pp.p) {
             // P's levels and the pieces in one pass over u, then Q's levels from the pieces
             const double *f = which == 1 ? dt.sw_pos.as<double>() : which == 2 ? dt.w_pos.as<double>() : nullptr;
+            // one item per row group
             const uint32_t grid2 = std::max<uint32_t>(
-                1, std::min<uint32_t>((dt.n_items + 2 * groups_per_block - 1) / (2 * groups_per_block), 65535u * 8));
+                1, std::min<uint32_t>((dt.n_items + 2 * groups_per_block - 1) / (2 * groups_per_block) * 2,
+                                      65535u * 8));
             if (f)
                 hipLaunchKernelGGL((k_segreduce2_pair<true>), dim3(grid2), dim3(BLK), 0, st, dt.info,
                                    dt.perm.as<uint32_t>(), dt.items.as<SegItem>(), dt.n_items,
-                                   pp.item_piece0.as<uint32_t>(), pp.qcode_pos.as<uint32_t>(), f, src, out, partials,
-                                   pieces, active, k, kshift);
+                                   pp.item_piece0.as<uint32_t>(), pp.piece_slot.as<uint32_t>(),
+                                   pp.qcode_pos.as<uint32_t>(), f, src, out, partials, pieces, active, k, kshift);
             else
                 hipLaunchKernelGGL((k_segreduce2_pair<false>), dim3(grid2), dim3(BLK), 0, st, dt.info,
                                    dt.perm.as<uint32_t>(), dt.items.as<SegItem>(), dt.n_items,
-                                   pp.item_piece0.as<uint32_t>(), pp.qcode_pos.as<uint32_t>(), f, src, out, partials,
-                                   pieces, active, k, kshift);
+                                   pp.item_piece0.as<uint32_t>(), pp.piece_slot.as<uint32_t>(),
+                                   pp.qcode_pos.as<uint32_t>(), f, src, out, partials, pieces, active, k, kshift);
             launch_segfix(dt, partials, out, active, k, kshift, st);
             const DevTerm &qt = pp.qt;
             const uint32_t gq = std::max<uint32_t>(
                 1, std::min<uint32_t>((qt.n_items + 2 * groups_per_block - 1) / (2 * groups_per_block), 65535u * 8));
-            hipLaunchKernelGGL((k_segreduce2<false>), dim3(gq), dim3(BLK), 0, st, qt.info, qt.perm.as<uint32_t>(),
+            hipLaunchKernelGGL((k_segreduce2<false>), dim3(gq), dim3(BLK), 0, st, qt.info, nullptr,
                                qt.items.as<SegItem>(), qt.n_items, nullptr, pieces, out, partials, active, k, kshift);
             launch_segfix(qt, partials, out, active, k, kshift, st);
             continue;

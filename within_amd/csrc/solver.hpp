@@ -82,8 +82,9 @@ struct PairPlan {
     int32_t p = -1, q = -1;
     DevBuf qcode_pos;   // u32[n]: Q's code at each of P's positions
     DevBuf item_piece0; // u32[P's n_items + 1]: first piece of each of P's work items
+    DevBuf piece_slot;  // u32[n_pieces]: where each piece is stored (the pieces in Q order)
     uint32_t n_pieces = 0;
-    DevTerm qt;         // Q's reduction over the pieces: perm = pieces in Q order, items, fixups
+    DevTerm qt;         // Q's reduction over the stored pieces: items over Q's levels, fixups
 };
 
 struct DevDesign {
