@@ -44,6 +44,12 @@ wx_status wx_testing_reduced_laplacian(uint32_t n_rows, uint32_t n_cols, const d
 wx_status wx_testing_design_index(const wx_design *design, uint64_t n_terms, uint64_t *n_levels, uint64_t *offset,
                                   uint8_t *sorted, int64_t *dominant, int32_t *permuted, uint32_t *obs_perm);
 
+/* wx_precond_apply with the arithmetic of a solve of the given wx_precision: WX_PRECISION_MIXED
+ * applies M^{-1} as a mixed-precision solve does (FP32 coefficients and FP32 slot-space vectors,
+ * FP64 accumulation), so tests can hold that operator to symmetry and definiteness bounds.
+ * Needs a GPU. */
+wx_status wx_testing_precond_apply(const wx_precond *p, const double *x, uint64_t k, double *y, int32_t precision);
+
 #ifdef __cplusplus
 }
 #endif

@@ -592,3 +592,51 @@ def test_local_size_mixed_precision_and_batch(gpu):
     for j in range(2):
         r1 = s.solve_batch(np.asfortranarray(Y[:, j:j + 1]), wa.LsmrOptions(tol=1e-10, local_size=8))
         np.testing.assert_allclose(r1.demeaned[:, 0], rb.demeaned[:, j], rtol=0, atol=1e-12)
+
+
+def _apply_prec(p, X, precision):
+    """M^{-1} X (n x k) with the arithmetic of a solve of `precision` (0 FP64, 1 mixed)."""
+    X = np.asfortranarray(X, dtype=np.float64)
+    Y = np.empty_like(X)
+    _lib.check(_lib.lib().wx_testing_precond_apply(p._h, X.ctypes.data, X.shape[1], Y.ctypes.data, precision))
+    return Y
+
+
+@pytest.mark.parametrize("kind", ["dense", "sparse", "slope_cover"])
+def test_mixed_precision_apply_is_symmetric_and_definite(gpu, kind):
+    """The preconditioner of a mixed-precision solve (FP32 coefficients and FP32 slot-space
+    vectors, FP64 accumulation) is applied as an explicit matrix: it stays symmetric and
+    positive semi-definite to FP32 rounding -- far inside the sqrt(eps) margin of the
+    indefiniteness check (crates/schwarz-precond/src/lsmr/bidiag.rs:99-111) -- and within FP32
+    rounding of the FP64 apply."""
+    rng = np.random.default_rng(71)
+    n = 6000
+    if kind == "slope_cover":
+        _, design, _, _, w = _slope_design(53)
+        cfg = wa.config.AdditiveSchwarz(wa.config.LocalSolverConfig(dense_threshold=0, dense_policy="reference"))
+    else:
+        a, b, c = rng.integers(0, 300, n), rng.integers(0, 40, n), rng.integers(0, 6, n)
+        design = np.asfortranarray(np.column_stack([a, b, c]).astype(np.uint32))
+        w = rng.uniform(0.5, 2.0, n)
+        ls = (wa.config.LocalSolverConfig() if kind == "dense"
+              else wa.config.LocalSolverConfig(dense_threshold=0, dense_policy="reference"))
+        cfg = wa.config.AdditiveSchwarz(ls)
+    s = wa.Solver(design, weights=w, preconditioner=cfg)
+    p = s.preconditioner
+    N = s.n_dofs
+    E = np.eye(N)
+    M32 = np.column_stack([_apply_prec(p, E[:, c0:c0 + 16], 1) for c0 in range(0, N, 16)])
+    M64 = np.column_stack([_apply_prec(p, E[:, c0:c0 + 16], 0) for c0 in range(0, N, 16)])
+    np.testing.assert_allclose(M64[:, 0], p.apply(np.ascontiguousarray(E[:, 0])), rtol=0,
+                               atol=1e-14 * np.abs(M64).max())
+    scale = np.abs(M64).max()
+    assert np.abs(M32 - M64).max() <= 1e-5 * scale
+    assert np.abs(M32 - M32.T).max() <= 1e-5 * scale
+    ev = np.linalg.eigvalsh(0.5 * (M32 + M32.T))
+    assert ev.min() >= -1e-5 * ev.max()
+    # the FP32 vectors change only rounding: a mixed solve reaches the FP64 solve's answer
+    Y = np.asfortranarray(rng.standard_normal((s.n_obs, 2)))
+    rm = s.solve_batch(Y, wa.LsmrOptions(tol=1e-10, precision="mixed"))
+    rf = s.solve_batch(Y, wa.LsmrOptions(tol=1e-10))
+    assert all(rm.converged) and all(rf.converged)
+    assert max(rel(rm.demeaned[:, j], rf.demeaned[:, j]) for j in range(2)) <= 1e-8

@@ -665,9 +665,12 @@ const char *wx_precond_variant(const wx_precond *p) {
     return p->p->kind == Precond::ADDITIVE ? "Additive" : "Diagonal";
 }
 
-wx_status wx_precond_apply(const wx_precond *p, const double *x, uint64_t k, double *y) {
+namespace {
+wx_status precond_apply_host(const wx_precond *p, const double *x, uint64_t k, double *y, int32_t precision) {
     return guard([&] {
         if (!p || (!x && k) || (!y && k)) fail(WX_ERR_INVALID_INPUT, "null pointer");
+        if (precision != WX_PRECISION_FP64 && precision != WX_PRECISION_MIXED)
+            fail(WX_ERR_INVALID_INPUT, "precision must be WX_PRECISION_FP64 or WX_PRECISION_MIXED");
         require_device();
         const uint64_t n = p->p->n;
         if (k == 0) return;
@@ -676,10 +679,19 @@ wx_status wx_precond_apply(const wx_precond *p, const double *x, uint64_t k, dou
         dx.alloc(n * k * 8);
         dy.alloc(n * k * 8);
         WX_HIP(hipMemcpyAsync(dx.p, x, n * k * 8, hipMemcpyHostToDevice, st.s));
-        precond_apply_device(*p->p, dx.as<double>(), k, dy.as<double>(), st.s);
+        precond_apply_device(*p->p, dx.as<double>(), k, dy.as<double>(), st.s, precision);
         WX_HIP(hipMemcpyAsync(y, dy.p, n * k * 8, hipMemcpyDeviceToHost, st.s));
         WX_HIP(hipStreamSynchronize(st.s));
     });
+}
+} // namespace
+
+wx_status wx_precond_apply(const wx_precond *p, const double *x, uint64_t k, double *y) {
+    return precond_apply_host(p, x, k, y, WX_PRECISION_FP64);
+}
+
+wx_status wx_testing_precond_apply(const wx_precond *p, const double *x, uint64_t k, double *y, int32_t precision) {
+    return precond_apply_host(p, x, k, y, precision);
 }
 
 wx_status wx_precond_serialize(const wx_precond *p, uint8_t **buf, uint64_t *len) {

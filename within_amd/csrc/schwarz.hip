@@ -149,7 +149,7 @@ template <typename CT> struct TailArgs {
     const uint32_t *idx;
     const CT *val;
     const CT *mat;         // M_T of each component, row-major
-    double *tsc, *csc;
+    CT *tsc, *csc;         // reduced tail right-hand sides, chunk partials (slot-space precision)
 };
 
 // A row of a large level, solved by a grid-wide launch: the component's SpMeta and the row slot.
@@ -173,18 +173,19 @@ __device__ inline bool gate_closed(const int32_t *gate, uint32_t k) {
 template <typename CT>
 __global__ __launch_bounds__(BLK) void sk_restrict(uint64_t ns, const uint32_t *__restrict__ glob,
                                                    const CT *__restrict__ sgnpw, const double *__restrict__ r,
-                                                   double *__restrict__ loc, uint32_t k, uint32_t kshift, const int32_t *__restrict__ gate) {
+                                                   CT *__restrict__ loc, uint32_t k, uint32_t kshift, const int32_t *__restrict__ gate) {
     if (gate_closed(gate, k)) return;
     const uint32_t kp = 1u << kshift, j = threadIdx.x & (kp - 1);
     if (j >= k) return;
     const uint64_t stride = ((uint64_t)gridDim.x * BLK) >> kshift;
     for (uint64_t s = ((uint64_t)blockIdx.x * BLK + threadIdx.x) >> kshift; s < ns; s += stride)
-        loc[s * k + j] = (double)sgnpw[s] * r[(uint64_t)glob[s] * k + j];
+        loc[s * k + j] = (CT)((double)sgnpw[s] * r[(uint64_t)glob[s] * k + j]);
 }
 
-// partial sums of src over chunks of slots
+// partial sums of src over chunks of slots (FP64 sums of the stored slot values)
+template <typename VT>
 __global__ __launch_bounds__(BLK) void sk_chunk_sum(const Chunk *__restrict__ chunks, uint32_t n_chunks,
-                                                    const double *__restrict__ src, double *__restrict__ part,
+                                                    const VT *__restrict__ src, double *__restrict__ part,
                                                     uint32_t k, uint32_t kshift, const int32_t *__restrict__ gate) {
     if (gate_closed(gate, k)) return;
     const uint32_t kp = 1u << kshift, j = threadIdx.x & (kp - 1);
@@ -197,13 +198,13 @@ __global__ __launch_bounds__(BLK) void sk_chunk_sum(const Chunk *__restrict__ ch
     for (; s + 4 <= ch.end; s += 4) { // four loads in flight, summed in slot order
         double t[4];
 #pragma unroll
-        for (int q = 0; q < 4; ++q) t[q] = src[(uint64_t)(s + q) * k + j];
+        for (int q = 0; q < 4; ++q) t[q] = (double)src[(uint64_t)(s + q) * k + j];
         acc += t[0];
         acc += t[1];
         acc += t[2];
         acc += t[3];
     }
-    for (; s < ch.end; ++s) acc += src[(uint64_t)s * k + j];
+    for (; s < ch.end; ++s) acc += (double)src[(uint64_t)s * k + j];
     part[(uint64_t)g * k + j] = acc;
 }
 
@@ -244,34 +245,39 @@ __global__ __launch_bounds__(BLK) void sk_comp_mean(const uint32_t *__restrict__
     }
 }
 
-// MODE 0: reduced rhs of kept rows:  red[row] = (loc[row]-mu) + sum_p ct[p]*inv_de[e]*(loc[e]-mu)
+// MODE 0: reduced rhs of kept rows:  red[row] = (loc[row]-mu) + sum_p ct[p]*(loc[e]-mu), the C'
+//         values ct already scaled by their eliminated row's 1 / D_e
 // MODE 1: back-substitution of eliminated rows: xs[row] = inv_de[row]*((loc[row]-mu) + sum_p c[p]*xs[kept])
 template <typename CT, int MODE>
-__device__ inline double row_term(const uint32_t *idx, const CT *val, uint64_t p, const double *loc,
-                                  const double *xs, const CT *inv_de, const double *mu_c, uint32_t k, uint32_t j) {
+__device__ inline double row_term(const uint32_t *idx, const CT *val, uint64_t p, const CT *loc,
+                                  const CT *xs, const CT *inv_de, const double *mu_c, uint32_t k, uint32_t j) {
     const uint64_t e = idx[p];
-    if (MODE == 0) return (double)val[p] * ((double)inv_de[e] * (loc[e * k + j] - mu_c[j]));
-    return (double)val[p] * xs[e * k + j];
+    if (MODE == 0) return (double)val[p] * ((double)loc[e * k + j] - mu_c[j]);
+    return (double)val[p] * (double)xs[e * k + j];
 }
 
+// the output of an elimination SpMV (the reduced right-hand side, MODE 0, or the back-substituted
+// slots, MODE 1): a slot-space vector in the coefficient precision
+template <typename CT, int MODE> using OutT = CT;
+
 template <typename CT, int MODE>
-__device__ inline void row_finish(uint32_t row, double sum, const double *loc, double *out, const CT *inv_de,
+__device__ inline void row_finish(uint32_t row, double sum, const CT *loc, OutT<CT, MODE> *out, const CT *inv_de,
                                   const double *mu_c, uint32_t k, uint32_t j) {
-    const double b = loc[(uint64_t)row * k + j] - mu_c[j];
-    if (MODE == 0) out[(uint64_t)row * k + j] = b + sum;
+    const double b = (double)loc[(uint64_t)row * k + j] - mu_c[j];
+    if (MODE == 0) out[(uint64_t)row * k + j] = (CT)(b + sum);
     else {
         const double id = (double)inv_de[row];
-        out[(uint64_t)row * k + j] = id * b + id * sum;
+        out[(uint64_t)row * k + j] = (CT)(id * b + id * sum);
     }
 }
 
 template <typename CT, int MODE>
 __global__ __launch_bounds__(BLK) void sk_rows(const RowItem *__restrict__ items, uint32_t n_items,
                                                const uint32_t *__restrict__ idx, const CT *__restrict__ val,
-                                               const uint32_t *__restrict__ slot_comp, const double *__restrict__ loc,
-                                               const double *__restrict__ xs, const CT *__restrict__ inv_de,
-                                               const double *__restrict__ mu, double *__restrict__ out,
-                                               double *__restrict__ part, uint32_t k, uint32_t kshift, const int32_t *__restrict__ gate) {
+                                               const uint32_t *__restrict__ slot_comp, const CT *__restrict__ loc,
+                                               const CT *__restrict__ xs, const CT *__restrict__ inv_de,
+                                               const double *__restrict__ mu, OutT<CT, MODE> *__restrict__ out,
+                                               CT *__restrict__ part, uint32_t k, uint32_t kshift, const int32_t *__restrict__ gate) {
     if (gate_closed(gate, k)) return;
     const uint32_t kp = 1u << kshift, j = threadIdx.x & (kp - 1);
     if (j >= k) return;
@@ -292,28 +298,46 @@ __global__ __launch_bounds__(BLK) void sk_rows(const RowItem *__restrict__ items
         }
         for (; p < item.end; ++p) acc += row_term<CT, MODE>(idx, val, p, loc, xs, inv_de, mu_c, k, j);
         if (item.dest >= 0) row_finish<CT, MODE>(item.row, acc, loc, out, inv_de, mu_c, k, j);
-        else part[(uint64_t)(-item.dest - 1) * k + j] = acc;
+        else part[(uint64_t)(-item.dest - 1) * k + j] = (CT)acc;
     }
 }
 
-// sk_rows with two columns per lane (16-byte loads of the gathered vector rows, half the lanes
-// per row) for an even number of columns; per column the arithmetic and the summation order
-// are those of sk_rows.
+// sk_rows with two columns per lane (one load of the two columns of a gathered vector row,
+// half the lanes per row) for an even number of columns; per column the arithmetic and the
+// summation order are those of sk_rows.
 typedef double dbl2 __attribute__((ext_vector_type(2)));
+typedef float flt2 __attribute__((ext_vector_type(2)));
+template <typename CT> struct Vec2;
+template <> struct Vec2<double> { typedef dbl2 type; };
+template <> struct Vec2<float> { typedef flt2 type; };
+
+// two consecutive columns of a slot-space vector, widened to FP64 / narrowed for a store
+template <typename CT> __device__ inline dbl2 load2(const CT *p) {
+    const typename Vec2<CT>::type v = *(const typename Vec2<CT>::type *)p;
+    dbl2 r;
+    r.x = (double)v.x;
+    r.y = (double)v.y;
+    return r;
+}
+template <typename CT> __device__ inline void store2(CT *p, double x, double y) {
+    typename Vec2<CT>::type v;
+    v.x = (CT)x;
+    v.y = (CT)y;
+    *(typename Vec2<CT>::type *)p = v;
+}
 
 template <typename CT, int MODE>
-__device__ inline dbl2 row_term2(const uint32_t *idx, const CT *val, uint64_t p, const double *loc, const double *xs,
+__device__ inline dbl2 row_term2(const uint32_t *idx, const CT *val, uint64_t p, const CT *loc, const CT *xs,
                                  const CT *inv_de, dbl2 mu, uint32_t k, uint32_t j0) {
     const uint64_t e = idx[p];
     const double v = (double)val[p];
     dbl2 r;
     if (MODE == 0) {
-        const double id = (double)inv_de[e];
-        const dbl2 l = *(const dbl2 *)&loc[e * k + j0];
-        r.x = v * (id * (l.x - mu.x));
-        r.y = v * (id * (l.y - mu.y));
+        const dbl2 l = load2<CT>(&loc[e * k + j0]);
+        r.x = v * (l.x - mu.x);
+        r.y = v * (l.y - mu.y);
     } else {
-        const dbl2 x = *(const dbl2 *)&xs[e * k + j0];
+        const dbl2 x = load2<CT>(&xs[e * k + j0]);
         r.x = v * x.x;
         r.y = v * x.y;
     }
@@ -323,10 +347,10 @@ __device__ inline dbl2 row_term2(const uint32_t *idx, const CT *val, uint64_t p,
 template <typename CT, int MODE>
 __global__ __launch_bounds__(BLK) void sk_rows2(const RowItem *__restrict__ items, uint32_t n_items,
                                                 const uint32_t *__restrict__ idx, const CT *__restrict__ val,
-                                                const uint32_t *__restrict__ slot_comp, const double *__restrict__ loc,
-                                                const double *__restrict__ xs, const CT *__restrict__ inv_de,
-                                                const double *__restrict__ mu, double *__restrict__ out,
-                                                double *__restrict__ part, uint32_t k, uint32_t kshift,
+                                                const uint32_t *__restrict__ slot_comp, const CT *__restrict__ loc,
+                                                const CT *__restrict__ xs, const CT *__restrict__ inv_de,
+                                                const double *__restrict__ mu, OutT<CT, MODE> *__restrict__ out,
+                                                CT *__restrict__ part, uint32_t k, uint32_t kshift,
                                                 const int32_t *__restrict__ gate) {
     if (gate_closed(gate, k)) return;
     const uint32_t kv = (1u << kshift) >> 1, j0 = 2 * (threadIdx.x & (kv - 1));
@@ -353,23 +377,111 @@ __global__ __launch_bounds__(BLK) void sk_rows2(const RowItem *__restrict__ item
             a0 += t.x;
             a1 += t.y;
         }
-        dbl2 o;
         if (item.dest >= 0) {
-            const dbl2 b = *(const dbl2 *)&loc[(uint64_t)item.row * k + j0];
+            const dbl2 b = load2<CT>(&loc[(uint64_t)item.row * k + j0]);
             const double b0 = b.x - m.x, b1 = b.y - m.y;
             if (MODE == 0) {
-                o.x = b0 + a0;
-                o.y = b1 + a1;
+                store2<OutT<CT, MODE>>(&out[(uint64_t)item.row * k + j0], b0 + a0, b1 + a1);
             } else {
                 const double id = (double)inv_de[item.row];
-                o.x = id * b0 + id * a0;
-                o.y = id * b1 + id * a1;
+                store2<OutT<CT, MODE>>(&out[(uint64_t)item.row * k + j0], id * b0 + id * a0, id * b1 + id * a1);
             }
-            *(dbl2 *)&out[(uint64_t)item.row * k + j0] = o;
         } else {
-            o.x = a0;
-            o.y = a1;
-            *(dbl2 *)&part[(uint64_t)(-item.dest - 1) * k + j0] = o;
+            store2<CT>(&part[(uint64_t)(-item.dest - 1) * k + j0], a0, a1);
+        }
+    }
+}
+
+// sk_rows with four columns per lane for FP32 slot-space vectors (one 16-byte load of four
+// columns of a gathered row, a quarter of the lanes per row, so a wave has twice the rows of
+// sk_rows2 in flight) for a width divisible by four; per column the arithmetic and the
+// summation order are those of sk_rows.
+typedef float flt4 __attribute__((ext_vector_type(4)));
+
+template <int MODE>
+__global__ __launch_bounds__(BLK) void sk_rows4(const RowItem *__restrict__ items, uint32_t n_items,
+                                                const uint32_t *__restrict__ idx, const float *__restrict__ val,
+                                                const uint32_t *__restrict__ slot_comp, const float *__restrict__ loc,
+                                                const float *__restrict__ xs, const float *__restrict__ inv_de,
+                                                const double *__restrict__ mu, OutT<float, MODE> *__restrict__ out,
+                                                float *__restrict__ part, uint32_t k, uint32_t kshift,
+                                                const int32_t *__restrict__ gate) {
+    if (gate_closed(gate, k)) return;
+    const uint32_t kv = (1u << kshift) >> 2, j0 = 4 * (threadIdx.x & (kv - 1));
+    if (j0 >= k) return;
+    const uint32_t groups = (gridDim.x * BLK) / kv;
+    for (uint32_t it = (blockIdx.x * BLK + threadIdx.x) / kv; it < n_items; it += groups) {
+        const RowItem item = items[it];
+        const double *mu_c = mu + (uint64_t)slot_comp[item.row] * k + j0;
+        const double m0 = mu_c[0], m1 = mu_c[1], m2 = mu_c[2], m3 = mu_c[3];
+        double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
+        uint64_t p = item.begin;
+        for (; p + 8 <= item.end; p += 8) { // eight independent gathers in flight, summed in order
+            double v[8];
+            flt4 g[8];
+#pragma unroll
+            for (int q = 0; q < 8; ++q) {
+                const uint64_t e = idx[p + q];
+                v[q] = (double)val[p + q];
+                g[q] = *(const flt4 *)&(MODE == 0 ? loc : xs)[e * k + j0];
+            }
+#pragma unroll
+            for (int q = 0; q < 8; ++q) {
+                if (MODE == 0) {
+                    a0 += v[q] * ((double)g[q].x - m0);
+                    a1 += v[q] * ((double)g[q].y - m1);
+                    a2 += v[q] * ((double)g[q].z - m2);
+                    a3 += v[q] * ((double)g[q].w - m3);
+                } else {
+                    a0 += v[q] * (double)g[q].x;
+                    a1 += v[q] * (double)g[q].y;
+                    a2 += v[q] * (double)g[q].z;
+                    a3 += v[q] * (double)g[q].w;
+                }
+            }
+        }
+        for (; p < item.end; ++p) {
+            const uint64_t e = idx[p];
+            const double v = (double)val[p];
+            const flt4 g = *(const flt4 *)&(MODE == 0 ? loc : xs)[e * k + j0];
+            if (MODE == 0) {
+                a0 += v * ((double)g.x - m0);
+                a1 += v * ((double)g.y - m1);
+                a2 += v * ((double)g.z - m2);
+                a3 += v * ((double)g.w - m3);
+            } else {
+                a0 += v * (double)g.x;
+                a1 += v * (double)g.y;
+                a2 += v * (double)g.z;
+                a3 += v * (double)g.w;
+            }
+        }
+        if (item.dest >= 0) {
+            const flt4 b = *(const flt4 *)&loc[(uint64_t)item.row * k + j0];
+            const double b0 = (double)b.x - m0, b1 = (double)b.y - m1, b2 = (double)b.z - m2, b3 = (double)b.w - m3;
+            if (MODE == 0) {
+                flt4 o;
+                o.x = (float)(b0 + a0);
+                o.y = (float)(b1 + a1);
+                o.z = (float)(b2 + a2);
+                o.w = (float)(b3 + a3);
+                *(flt4 *)&out[(uint64_t)item.row * k + j0] = o;
+            } else {
+                const double id = (double)inv_de[item.row];
+                flt4 o;
+                o.x = (float)(id * b0 + id * a0);
+                o.y = (float)(id * b1 + id * a1);
+                o.z = (float)(id * b2 + id * a2);
+                o.w = (float)(id * b3 + id * a3);
+                *(flt4 *)&out[(uint64_t)item.row * k + j0] = o;
+            }
+        } else {
+            flt4 o;
+            o.x = (float)a0;
+            o.y = (float)a1;
+            o.z = (float)a2;
+            o.w = (float)a3;
+            *(flt4 *)&part[(uint64_t)(-item.dest - 1) * k + j0] = o;
         }
     }
 }
@@ -377,9 +489,10 @@ __global__ __launch_bounds__(BLK) void sk_rows2(const RowItem *__restrict__ item
 template <typename CT, int MODE>
 __global__ __launch_bounds__(BLK) void sk_rows_fix(const RowFix *__restrict__ fixes, uint32_t n_fix,
                                                    const uint32_t *__restrict__ slot_comp,
-                                                   const double *__restrict__ loc, const CT *__restrict__ inv_de,
-                                                   const double *__restrict__ mu, const double *__restrict__ part,
-                                                   double *__restrict__ out, uint32_t k, uint32_t kshift, const int32_t *__restrict__ gate) {
+                                                   const CT *__restrict__ loc, const CT *__restrict__ inv_de,
+                                                   const double *__restrict__ mu, const CT *__restrict__ part,
+                                                   OutT<CT, MODE> *__restrict__ out, uint32_t k, uint32_t kshift,
+                                                   const int32_t *__restrict__ gate) {
     // one workgroup per split row: its row groups stride over the partials (a hub row can have
     // thousands), then a fixed LDS tree adds the groups' sums
     if (gate_closed(gate, k)) return;
@@ -390,7 +503,7 @@ __global__ __launch_bounds__(BLK) void sk_rows_fix(const RowFix *__restrict__ fi
         const RowFix f = fixes[fi];
         double acc = 0.0;
         if (j < k)
-            for (uint32_t q = grp; q < f.count; q += ngrp) acc += part[(uint64_t)(f.first + q) * k + j];
+            for (uint32_t q = grp; q < f.count; q += ngrp) acc += (double)part[(uint64_t)(f.first + q) * k + j];
         sh[threadIdx.x] = acc;
         __syncthreads();
         for (uint32_t s = BLK / 2; s >= kp && s > 0; s >>= 1) {
@@ -423,46 +536,50 @@ struct DenseComp {
 
 template <typename CT, int KC>
 __global__ __launch_bounds__(64) void sk_dense_sym(const DenseTile *__restrict__ tiles, const CT *__restrict__ mat,
-                                                   const double *__restrict__ red, double *__restrict__ part,
+                                                   const CT *__restrict__ red, CT *__restrict__ part,
                                                    uint32_t kstride, uint32_t c0, const int32_t *__restrict__ gate) {
     if (gate_closed(gate, kstride)) return;
     const DenseTile T = tiles[blockIdx.x];
     const uint32_t row = T.row0 + threadIdx.x;
     const bool on = row < T.nk;
     const CT *M = mat + T.mat_off + (on ? row : 0);
-    const double *R = red + (uint64_t)T.kept_base * kstride + c0;
-    double acc[KC];
+    const CT *R = red + (uint64_t)T.kept_base * kstride + c0;
+    // the segment's products accumulate in the coefficient precision: FP64 runs in FP64; FP32
+    // runs in FP32 (full-rate FMAs on the wave-uniform right-hand-side operands, a segment being
+    // a few hundred terms), their segment partials summed in FP64 by sk_dense_reduce
+    CT acc[KC];
 #pragma unroll
-    for (int c = 0; c < KC; ++c) acc[c] = 0.0;
+    for (int c = 0; c < KC; ++c) acc[c] = (CT)0;
     uint32_t l = T.l0;
     for (; l + 8 <= T.l1; l += 8) { // eight coefficient loads in flight per lane
-        double m[8];
+        CT m[8];
 #pragma unroll
-        for (int q = 0; q < 8; ++q) m[q] = (double)M[(uint64_t)(l + q) * T.nk];
+        for (int q = 0; q < 8; ++q) m[q] = M[(uint64_t)(l + q) * T.nk];
 #pragma unroll
         for (int q = 0; q < 8; ++q) {
-            const double *rl = R + (uint64_t)(l + q) * kstride;
+            const CT *rl = R + (uint64_t)(l + q) * kstride;
 #pragma unroll
             for (int c = 0; c < KC; ++c) acc[c] += m[q] * rl[c];
         }
     }
     for (; l < T.l1; ++l) {
-        const double m = (double)M[(uint64_t)l * T.nk];
-        const double *rl = R + (uint64_t)l * kstride;
+        const CT m = M[(uint64_t)l * T.nk];
+        const CT *rl = R + (uint64_t)l * kstride;
 #pragma unroll
         for (int c = 0; c < KC; ++c) acc[c] += m * rl[c];
     }
     if (on) {
-        double *o = part + (T.part_off + (uint64_t)T.seg * T.nk + row) * kstride + c0;
+        CT *o = part + (T.part_off + (uint64_t)T.seg * T.nk + row) * kstride + c0;
 #pragma unroll
         for (int c = 0; c < KC; ++c) o[c] = acc[c];
     }
 }
 
 // xs[kept_base + i] = sum over segments (in order) of the partials
+template <typename CT>
 __global__ __launch_bounds__(BLK) void sk_dense_reduce(const DenseComp *__restrict__ comps, uint32_t n_comps,
                                                        const uint64_t *__restrict__ row_start,
-                                                       const double *__restrict__ part, double *__restrict__ xs,
+                                                       const CT *__restrict__ part, CT *__restrict__ xs,
                                                        uint32_t k, uint32_t kshift, const int32_t *__restrict__ gate) {
     if (gate_closed(gate, k)) return;
     const uint32_t kp = 1u << kshift, j = threadIdx.x & (kp - 1);
@@ -479,8 +596,8 @@ __global__ __launch_bounds__(BLK) void sk_dense_reduce(const DenseComp *__restri
         const DenseComp C = comps[lo];
         const uint64_t i = q - row_start[lo];
         double acc = 0.0;
-        for (uint32_t s = 0; s < C.n_seg; ++s) acc += part[(C.part_off + (uint64_t)s * C.nk + i) * k + j];
-        xs[(uint64_t)(C.kept_base + i) * k + j] = acc;
+        for (uint32_t s = 0; s < C.n_seg; ++s) acc += (double)part[(C.part_off + (uint64_t)s * C.nk + i) * k + j];
+        xs[(uint64_t)(C.kept_base + i) * k + j] = (CT)acc;
     }
 }
 
@@ -490,7 +607,7 @@ __global__ __launch_bounds__(BLK) void sk_dense(uint32_t n_rows, const uint64_t 
                                                 const uint32_t *__restrict__ row_base,
                                                 const uint32_t *__restrict__ row_nk,
                                                 const uint32_t *__restrict__ row_slot, const CT *__restrict__ mat,
-                                                const double *__restrict__ red, double *__restrict__ xs, uint32_t k,
+                                                const CT *__restrict__ red, CT *__restrict__ xs, uint32_t k,
                                                 uint32_t kshift, const int32_t *__restrict__ gate) {
     if (gate_closed(gate, k)) return;
     const uint32_t kp = 1u << kshift, j = threadIdx.x & (kp - 1);
@@ -498,7 +615,7 @@ __global__ __launch_bounds__(BLK) void sk_dense(uint32_t n_rows, const uint64_t 
     const uint32_t groups = (gridDim.x * BLK) >> kshift;
     for (uint32_t r = (blockIdx.x * BLK + threadIdx.x) >> kshift; r < n_rows; r += groups) {
         const CT *m = mat + row_off[r];
-        const double *rd = red + row_base[r] * k + j;
+        const CT *rd = red + row_base[r] * k + j;
         const uint32_t nk = row_nk[r];
         double acc = 0.0;
         uint32_t l = 0;
@@ -507,27 +624,27 @@ __global__ __launch_bounds__(BLK) void sk_dense(uint32_t n_rows, const uint64_t 
 #pragma unroll
             for (int q = 0; q < 8; ++q) {
                 mv[q] = (double)m[l + q];
-                rv[q] = rd[(uint64_t)(l + q) * k];
+                rv[q] = (double)rd[(uint64_t)(l + q) * k];
             }
 #pragma unroll
             for (int q = 0; q < 8; ++q) acc += mv[q] * rv[q];
         }
-        for (; l < nk; ++l) acc += (double)m[l] * rd[(uint64_t)l * k];
-        xs[(uint64_t)row_slot[r] * k + j] = acc;
+        for (; l < nk; ++l) acc += (double)m[l] * (double)rd[(uint64_t)l * k];
+        xs[(uint64_t)row_slot[r] * k + j] = (CT)acc;
     }
 }
 
 // Right-hand side of a forward-substitution row: the reduced rhs minus the component mean, or
 // for a Gremban cover's auxiliary slot the antisymmetric embed of its kept slot's.
 template <typename CT>
-__device__ inline double sparse_rhs(const SpMeta &m, uint32_t row, const double *red, const uint32_t *aux_src,
+__device__ inline double sparse_rhs(const SpMeta &m, uint32_t row, const CT *red, const uint32_t *aux_src,
                                     uint64_t ns, double mu, uint32_t k, uint32_t j) {
     if (m.cover) { // antisymmetric embed [b, -b] of the kept block's reduced rhs
         const uint32_t a = aux_src[row - ns];
-        const double b = red[(uint64_t)(a & 0x7FFFFFFFu) * k + j];
+        const double b = (double)red[(uint64_t)(a & 0x7FFFFFFFu) * k + j];
         return (a >> 31) ? -b : b;
     }
-    return red[(uint64_t)row * k + j] - mu;
+    return (double)red[(uint64_t)row * k + j] - mu;
 }
 
 // sum(val[p] * xs[idx[p]]) over [p0, p1) for one column, accumulated into acc in entry order
@@ -536,7 +653,7 @@ __device__ inline double sparse_rhs(const SpMeta &m, uint32_t row, const double 
 // leave a hub row of thousands of entries latency-bound.
 template <typename CT, bool SUB>
 __device__ inline double dot_entries(double acc, uint64_t p0, uint64_t p1, uint64_t step, const uint32_t *idx,
-                                     const CT *val, const double *xs, uint32_t k, uint32_t j) {
+                                     const CT *val, const CT *xs, uint32_t k, uint32_t j) {
     constexpr int U = 8;
     uint64_t p = p0;
     for (; p + (U - 1) * step < p1; p += U * step) {
@@ -548,7 +665,7 @@ __device__ inline double dot_entries(double acc, uint64_t p0, uint64_t p1, uint6
             v[q] = (double)val[p + q * step];
         }
 #pragma unroll
-        for (int q = 0; q < U; ++q) x[q] = xs[(uint64_t)e[q] * k + j];
+        for (int q = 0; q < U; ++q) x[q] = (double)xs[(uint64_t)e[q] * k + j];
 #pragma unroll
         for (int q = 0; q < U; ++q) {
             if (SUB) acc -= v[q] * x[q];
@@ -556,8 +673,8 @@ __device__ inline double dot_entries(double acc, uint64_t p0, uint64_t p1, uint6
         }
     }
     for (; p < p1; p += step) {
-        if (SUB) acc -= (double)val[p] * xs[(uint64_t)idx[p] * k + j];
-        else acc += (double)val[p] * xs[(uint64_t)idx[p] * k + j];
+        if (SUB) acc -= (double)val[p] * (double)xs[(uint64_t)idx[p] * k + j];
+        else acc += (double)val[p] * (double)xs[(uint64_t)idx[p] * k + j];
     }
     return acc;
 }
@@ -568,8 +685,8 @@ __device__ inline double dot_entries(double acc, uint64_t p0, uint64_t p1, uint6
 // adds the partial sums in a fixed LDS tree.  FORWARD: x = rhs - L x; else x = x - L' x.
 template <typename CT, bool FORWARD>
 __device__ inline void sparse_level(const SpMeta &m, uint64_t b, uint64_t el, uint64_t e, const uint32_t *rows,
-                                    const uint64_t *ptr, const uint32_t *idx, const CT *val, const double *red,
-                                    const uint32_t *aux_src, uint64_t ns, double mu, double *xs, double *sh,
+                                    const uint64_t *ptr, const uint32_t *idx, const CT *val, const CT *red,
+                                    const uint32_t *aux_src, uint64_t ns, double mu, CT *xs, double *sh,
                                     uint32_t k, uint32_t kshift) {
     const uint32_t kp = 1u << kshift, j = threadIdx.x & (kp - 1);
     const uint32_t grp = threadIdx.x >> kshift, ngrp = SP_BLK >> kshift;
@@ -578,9 +695,10 @@ __device__ inline void sparse_level(const SpMeta &m, uint64_t b, uint64_t el, ui
         for (uint64_t q = b + grp; q < el; q += ngrp) {
             const uint32_t row = rows[q];
             const uint64_t lr = (uint64_t)(row - m.kept_base) + m.row_off;
-            double acc = FORWARD ? sparse_rhs<CT>(m, row, red, aux_src, ns, mu, k, j) : xs[(uint64_t)row * k + j];
+            double acc = FORWARD ? sparse_rhs<CT>(m, row, red, aux_src, ns, mu, k, j)
+                                 : (double)xs[(uint64_t)row * k + j];
             acc = dot_entries<CT, true>(acc, ptr[lr], ptr[lr + 1], 1, idx, val, xs, k, j);
-            xs[(uint64_t)row * k + j] = acc;
+            xs[(uint64_t)row * k + j] = (CT)acc;
         }
     for (uint64_t q = el; q < e; ++q) {
         const uint32_t row = rows[q];
@@ -594,8 +712,9 @@ __device__ inline void sparse_level(const SpMeta &m, uint64_t b, uint64_t el, ui
             __syncthreads();
         }
         if (threadIdx.x < kp && on) {
-            const double base = FORWARD ? sparse_rhs<CT>(m, row, red, aux_src, ns, mu, k, j) : xs[(uint64_t)row * k + j];
-            xs[(uint64_t)row * k + j] = base - sh[threadIdx.x];
+            const double base = FORWARD ? sparse_rhs<CT>(m, row, red, aux_src, ns, mu, k, j)
+                                        : (double)xs[(uint64_t)row * k + j];
+            xs[(uint64_t)row * k + j] = (CT)(base - sh[threadIdx.x]);
         }
         __syncthreads();
     }
@@ -617,9 +736,9 @@ __global__ __launch_bounds__(SP_BLK) void sk_sparse(const SpMeta *__restrict__ m
                                                  const uint64_t *__restrict__ l_ptr, const uint32_t *__restrict__ l_idx,
                                                  const CT *__restrict__ l_val, const uint64_t *__restrict__ lt_ptr,
                                                  const uint32_t *__restrict__ lt_idx, const CT *__restrict__ lt_val,
-                                                 const CT *__restrict__ piv, const double *__restrict__ red,
+                                                 const CT *__restrict__ piv, const CT *__restrict__ red,
                                                  const double *__restrict__ mu2, const uint32_t *__restrict__ aux_src,
-                                                 uint64_t ns, double *xs, uint32_t k,
+                                                 uint64_t ns, CT *xs, uint32_t k,
                                                  uint32_t kshift, const int32_t *__restrict__ gate, uint32_t phase) {
     if (gate_closed(gate, k)) return;
     __shared__ double sh[SP_BLK];
@@ -652,7 +771,7 @@ __global__ __launch_bounds__(SP_BLK) void sk_sparse(const SpMeta *__restrict__ m
 // entries summed in order): csc[chunk] = sum of L entries x xs over the chunk's entries on rows
 // before the tail, which the forward phase has solved.
 template <typename CT>
-__global__ __launch_bounds__(BLK) void sk_tail_chunks(TailArgs<CT> ta, uint64_t n_chunks, const double *xs, uint32_t k,
+__global__ __launch_bounds__(BLK) void sk_tail_chunks(TailArgs<CT> ta, uint64_t n_chunks, const CT *xs, uint32_t k,
                                                       uint32_t kshift, const int32_t *__restrict__ gate) {
     if (gate_closed(gate, k)) return;
     const uint32_t kp = 1u << kshift, j = threadIdx.x & (kp - 1);
@@ -660,7 +779,7 @@ __global__ __launch_bounds__(BLK) void sk_tail_chunks(TailArgs<CT> ta, uint64_t 
     const uint64_t stride = ((uint64_t)gridDim.x * BLK) >> kshift;
     for (uint64_t c = ((uint64_t)blockIdx.x * BLK + threadIdx.x) >> kshift; c < n_chunks; c += stride) {
         const TailChunk ch = ta.chunks[c];
-        ta.csc[c * k + j] = dot_entries<CT, false>(0.0, ch.p0, ch.p1, 1, ta.idx, ta.val, xs, k, j);
+        ta.csc[c * k + j] = (CT)dot_entries<CT, false>(0.0, ch.p0, ch.p1, 1, ta.idx, ta.val, xs, k, j);
     }
 }
 
@@ -670,7 +789,7 @@ __global__ __launch_bounds__(BLK) void sk_tail_chunks(TailArgs<CT> ta, uint64_t 
 template <typename CT>
 __global__ __launch_bounds__(BLK) void sk_tail_rhs(TailArgs<CT> ta, const TailProdRow *__restrict__ prod,
                                                    uint64_t n_rows, const SpMeta *__restrict__ meta,
-                                                   const double *__restrict__ red, const double *__restrict__ mu2,
+                                                   const CT *__restrict__ red, const double *__restrict__ mu2,
                                                    const uint32_t *__restrict__ aux_src, uint64_t ns, uint32_t k,
                                                    const int32_t *__restrict__ gate) {
     if (gate_closed(gate, k)) return;
@@ -685,16 +804,16 @@ __global__ __launch_bounds__(BLK) void sk_tail_rhs(TailArgs<CT> ta, const TailPr
     for (; c + 7 * 64 < c1; c += 8 * 64) {
         double v[8];
 #pragma unroll
-        for (int q = 0; q < 8; ++q) v[q] = ta.csc[(c + q * 64) * k + j];
+        for (int q = 0; q < 8; ++q) v[q] = (double)ta.csc[(c + q * 64) * k + j];
 #pragma unroll
         for (int q = 0; q < 8; ++q) part += v[q];
     }
-    for (; c < c1; c += 64) part += ta.csc[c * k + j];
+    for (; c < c1; c += 64) part += (double)ta.csc[c * k + j];
     for (int off = 32; off > 0; off >>= 1) part += __shfl_xor(part, off);
     if (lane == 0) {
         const SpMeta m = meta[prod[ra].meta];
         const double mu = mu2[(uint64_t)m.comp * k + j];
-        ta.tsc[ra * k + j] = sparse_rhs<CT>(m, ta.rows[ra], red, aux_src, ns, mu, k, j) - part;
+        ta.tsc[ra * k + j] = (CT)(sparse_rhs<CT>(m, ta.rows[ra], red, aux_src, ns, mu, k, j) - part);
     }
 }
 
@@ -703,7 +822,7 @@ __global__ __launch_bounds__(BLK) void sk_tail_rhs(TailArgs<CT> ta, const TailPr
 // (eight loads in flight), and a fixed shuffle tree adds the lanes' sums.
 template <typename CT>
 __global__ __launch_bounds__(BLK) void sk_tail_product(TailArgs<CT> ta, const TailProdRow *__restrict__ prod,
-                                                       uint64_t n_rows, double *__restrict__ xs, uint32_t k,
+                                                       uint64_t n_rows, CT *__restrict__ xs, uint32_t k,
                                                        const int32_t *__restrict__ gate) {
     if (gate_closed(gate, k)) return;
     const uint32_t lane = threadIdx.x & 63;
@@ -713,7 +832,7 @@ __global__ __launch_bounds__(BLK) void sk_tail_product(TailArgs<CT> ta, const Ta
     const uint32_t j = (uint32_t)(task % k);
     const TailProdRow p = prod[ra];
     const CT *Mr = ta.mat + p.mrow;
-    const double *r = ta.tsc + p.toff * k + j;
+    const CT *r = ta.tsc + p.toff * k + j;
     double acc = 0.0;
     uint32_t b = lane;
     for (; b + 7 * 64 < p.nt; b += 8 * 64) {
@@ -721,14 +840,14 @@ __global__ __launch_bounds__(BLK) void sk_tail_product(TailArgs<CT> ta, const Ta
 #pragma unroll
         for (int q = 0; q < 8; ++q) {
             mv[q] = (double)Mr[b + q * 64];
-            rv[q] = r[(uint64_t)(b + q * 64) * k];
+            rv[q] = (double)r[(uint64_t)(b + q * 64) * k];
         }
 #pragma unroll
         for (int q = 0; q < 8; ++q) acc += mv[q] * rv[q];
     }
-    for (; b < p.nt; b += 64) acc += (double)Mr[b] * r[(uint64_t)b * k];
+    for (; b < p.nt; b += 64) acc += (double)Mr[b] * (double)r[(uint64_t)b * k];
     for (int off = 32; off > 0; off >>= 1) acc += __shfl_xor(acc, off);
-    if (lane == 0) xs[(uint64_t)ta.rows[ra] * k + j] = acc;
+    if (lane == 0) xs[(uint64_t)ta.rows[ra] * k + j] = (CT)acc;
 }
 
 // One large level of every component that has it, all rows in parallel across the grid (one
@@ -737,8 +856,8 @@ template <typename CT, bool FORWARD>
 __global__ __launch_bounds__(BLK) void sk_sparse_grid(const GridRow *__restrict__ rows, uint64_t n_rows,
                                                       const SpMeta *__restrict__ meta, const uint64_t *__restrict__ ptr,
                                                       const uint32_t *__restrict__ idx, const CT *__restrict__ val,
-                                                      const double *__restrict__ red, const double *__restrict__ mu2,
-                                                      const uint32_t *__restrict__ aux_src, uint64_t ns, double *xs,
+                                                      const CT *__restrict__ red, const double *__restrict__ mu2,
+                                                      const uint32_t *__restrict__ aux_src, uint64_t ns, CT *xs,
                                                       uint32_t k, uint32_t kshift, const int32_t *__restrict__ gate) {
     if (gate_closed(gate, k)) return;
     const uint32_t kp = 1u << kshift, j = threadIdx.x & (kp - 1);
@@ -749,9 +868,9 @@ __global__ __launch_bounds__(BLK) void sk_sparse_grid(const GridRow *__restrict_
         const SpMeta &m = meta[gr.meta];
         const uint64_t lr = (uint64_t)(gr.row - m.kept_base) + m.row_off;
         double acc = FORWARD ? sparse_rhs<CT>(m, gr.row, red, aux_src, ns, mu2[(uint64_t)m.comp * k + j], k, j)
-                             : xs[(uint64_t)gr.row * k + j];
+                             : (double)xs[(uint64_t)gr.row * k + j];
         acc = dot_entries<CT, true>(acc, ptr[lr], ptr[lr + 1], 1, idx, val, xs, k, j);
-        xs[(uint64_t)gr.row * k + j] = acc;
+        xs[(uint64_t)gr.row * k + j] = (CT)acc;
     }
 }
 
@@ -760,7 +879,7 @@ __global__ __launch_bounds__(BLK) void sk_sparse_grid(const GridRow *__restrict_
 template <typename CT>
 __global__ __launch_bounds__(BLK) void sk_sparse_diag(const GridRow *__restrict__ rows, uint64_t n_rows,
                                                       const SpMeta *__restrict__ meta, const CT *__restrict__ piv,
-                                                      double *xs, uint32_t k, uint32_t kshift,
+                                                      CT *xs, uint32_t k, uint32_t kshift,
                                                       const int32_t *__restrict__ gate) {
     if (gate_closed(gate, k)) return;
     const uint32_t kp = 1u << kshift, j = threadIdx.x & (kp - 1);
@@ -771,14 +890,15 @@ __global__ __launch_bounds__(BLK) void sk_sparse_diag(const GridRow *__restrict_
         const SpMeta &m = meta[gr.meta];
         const double dv = (double)piv[m.piv_off + (gr.row - m.kept_base)];
         const uint64_t x = (uint64_t)gr.row * k + j;
-        xs[x] = dv != 0.0 ? xs[x] / dv : 0.0;
+        xs[x] = dv != 0.0 ? (CT)((double)xs[x] / dv) : (CT)0;
     }
 }
 
 // Gremban covers: read back x = (y+ - y-) / 2 into the real kept slots, the first sheet's
 // rows assigning and then the second sheet's subtracting (two launches, sheet 0 then 1).
+template <typename CT>
 __global__ __launch_bounds__(BLK) void sk_cover_readback(const uint32_t *__restrict__ cover_rows, uint64_t n,
-                                                         const uint32_t *__restrict__ aux_src, uint64_t ns, double *xs,
+                                                         const uint32_t *__restrict__ aux_src, uint64_t ns, CT *xs,
                                                          uint32_t sheet, uint32_t k, uint32_t kshift,
                                                          const int32_t *__restrict__ gate) {
     if (gate_closed(gate, k)) return;
@@ -789,9 +909,9 @@ __global__ __launch_bounds__(BLK) void sk_cover_readback(const uint32_t *__restr
         const uint32_t r = cover_rows[q];
         const uint32_t a = aux_src[r - ns];
         if ((a >> 31) != sheet) continue;
-        const double y = 0.5 * xs[(uint64_t)r * k + j];
-        double &dst = xs[(uint64_t)(a & 0x7FFFFFFFu) * k + j];
-        dst = sheet ? dst - y : y;
+        const double y = 0.5 * (double)xs[(uint64_t)r * k + j];
+        CT &dst = xs[(uint64_t)(a & 0x7FFFFFFFu) * k + j];
+        dst = sheet ? (CT)((double)dst - y) : (CT)y;
     }
 }
 
@@ -800,7 +920,7 @@ template <typename CT>
 __global__ __launch_bounds__(BLK) void sk_prolong(uint64_t N, const uint64_t *__restrict__ d_ptr,
                                                   const uint32_t *__restrict__ d_slot, const CT *__restrict__ sgnpw,
                                                   const uint32_t *__restrict__ slot_comp,
-                                                  const double *__restrict__ xs, const double *__restrict__ mu3,
+                                                  const CT *__restrict__ xs, const double *__restrict__ mu3,
                                                   double *__restrict__ z, uint32_t k, uint32_t kshift, const int32_t *__restrict__ gate) {
     if (gate_closed(gate, k)) return;
     const uint32_t kp = 1u << kshift, j = threadIdx.x & (kp - 1);
@@ -810,7 +930,7 @@ __global__ __launch_bounds__(BLK) void sk_prolong(uint64_t N, const uint64_t *__
         double acc = 0.0;
         for (uint64_t q = d_ptr[d]; q < d_ptr[d + 1]; ++q) {
             const uint64_t s = d_slot[q];
-            acc += (double)sgnpw[s] * (xs[s * k + j] - mu3[(uint64_t)slot_comp[s] * k + j]);
+            acc += (double)sgnpw[s] * ((double)xs[s * k + j] - mu3[(uint64_t)slot_comp[s] * k + j]);
         }
         z[d * k + j] = acc;
     }
@@ -900,8 +1020,10 @@ std::shared_ptr<SchwarzDev<CT>> build_dev(const Precond::Schwarz &S) {
             for (uint64_t e = base; e < kb; ++e)
                 for (uint64_t p = S.c_ptr[e]; p < S.c_ptr[e + 1]; ++p) {
                     const uint64_t row = kb + S.c_idx[p];
+                    // C' entries carry the eliminated row's 1 / D_e: the reduced right-hand side's
+                    // terms are then ct_val * (loc - mu) with no second gather per entry
                     ct_idx[cur[row]] = (uint32_t)e;
-                    ct_val[cur[row]] = S.c_val[p];
+                    ct_val[cur[row]] = S.c_val[p] * S.inv_de[e];
                     cur[row]++;
                 }
         }
@@ -1338,42 +1460,57 @@ struct SchwarzRunImpl : SchwarzRun {
         if (k <= kcap) return;
         kcap = k;
         const uint64_t ns = D->ns;
-        loc.alloc(ns * k * 8);
-        red.alloc(ns * k * 8);
-        xs.alloc((ns + D->n_aux) * k * 8);
+        // slot-space vectors in the coefficient precision (FP32 in mixed runs); the component
+        // means and the chunk partials behind them stay FP64
+        constexpr uint64_t V = sizeof(CT);
+        loc.alloc(ns * k * V);
+        red.alloc(ns * k * V);
+        xs.alloc((ns + D->n_aux) * k * V);
         part.alloc((uint64_t)std::max(D->n_all_chunks, D->n_kept_chunks) * k * 8 + 8);
-        rowpart.alloc((uint64_t)std::max(D->n_c_part, D->n_ct_part) * k * 8 + 8);
+        rowpart.alloc((uint64_t)std::max(D->n_c_part, D->n_ct_part) * k * V + 8);
         mu1.alloc(D->nc * k * 8);
         mu2.alloc(D->nc * k * 8);
         mu3.alloc(D->nc * k * 8);
-        dpart.alloc(D->dense_part_rows * k * 8 + 8);
-        tsc.alloc(D->n_tail_rows * k * 8 + 8);
-        csc.alloc(D->n_tail_chunks * k * 8 + 8);
+        dpart.alloc(D->dense_part_rows * k * V + 8);
+        tsc.alloc(D->n_tail_rows * k * V + 8);
+        csc.alloc(D->n_tail_chunks * k * V + 8);
     }
 
     // the elimination SpMVs (C' rows into the reduced right-hand side, C rows in the
     // back-substitution); two columns per lane for an even width
     template <int MODE>
-    void rows_spmv(const DevBuf &items, uint32_t n_items, const DevBuf &idx, const DevBuf &val, double *locp,
-                   double *xsp, double *out, uint32_t k, uint32_t kshift, hipStream_t st, const int32_t *gate) {
+    void rows_spmv(const DevBuf &items, uint32_t n_items, const DevBuf &idx, const DevBuf &val, CT *locp,
+                   CT *xsp, OutT<CT, MODE> *out, uint32_t k, uint32_t kshift, hipStream_t st, const int32_t *gate) {
         const SchwarzDev<CT> &d = *D;
+        if constexpr (std::is_same<CT, float>::value) {
+            if (k >= 4 && (k & 3) == 0) { // FP32 slot vectors: four columns per lane
+                const uint64_t per = 4 * (uint64_t)(BLK >> kshift);
+                const uint32_t g =
+                    (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>((n_items + per - 1) / per, 65535));
+                hipLaunchKernelGGL((sk_rows4<MODE>), dim3(g), dim3(BLK), 0, st, dp<RowItem>(items), n_items,
+                                   dp<uint32_t>(idx), dp<float>(val), dp<uint32_t>(d.slot_comp), locp, xsp,
+                                   dp<float>(d.inv_de), dp<double>(mu1), out, dp<float>(rowpart), k, kshift, gate);
+                return;
+            }
+        }
         if (k >= 2 && (k & 1) == 0) {
             const uint64_t per = 2 * (uint64_t)(BLK >> kshift);
             const uint32_t g = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>((n_items + per - 1) / per, 65535));
             hipLaunchKernelGGL((sk_rows2<CT, MODE>), dim3(g), dim3(BLK), 0, st, dp<RowItem>(items), n_items,
                                dp<uint32_t>(idx), dp<CT>(val), dp<uint32_t>(d.slot_comp), locp, xsp,
-                               dp<CT>(d.inv_de), dp<double>(mu1), out, dp<double>(rowpart), k, kshift, gate);
+                               dp<CT>(d.inv_de), dp<double>(mu1), out, dp<CT>(rowpart), k, kshift, gate);
         } else {
             hipLaunchKernelGGL((sk_rows<CT, MODE>), dim3(grid_rows(n_items, kshift)), dim3(BLK), 0, st,
                                dp<RowItem>(items), n_items, dp<uint32_t>(idx), dp<CT>(val), dp<uint32_t>(d.slot_comp),
-                               locp, xsp, dp<CT>(d.inv_de), dp<double>(mu1), out, dp<double>(rowpart), k, kshift,
+                               locp, xsp, dp<CT>(d.inv_de), dp<double>(mu1), out, dp<CT>(rowpart), k, kshift,
                                gate);
         }
     }
 
-    void mean(const DevBuf &chunks, uint32_t n_chunks, const DevBuf &cptr, const DevBuf &count, const double *src,
+    template <typename VT>
+    void mean(const DevBuf &chunks, uint32_t n_chunks, const DevBuf &cptr, const DevBuf &count, const VT *src,
               double *mu, uint32_t k, uint32_t kshift, hipStream_t st, const int32_t *gate) {
-        hipLaunchKernelGGL(sk_chunk_sum, dim3(grid_rows(n_chunks, kshift)), dim3(BLK), 0, st, dp<Chunk>(chunks),
+        hipLaunchKernelGGL((sk_chunk_sum<VT>), dim3(grid_rows(n_chunks, kshift)), dim3(BLK), 0, st, dp<Chunk>(chunks),
                            n_chunks, src, dp<double>(part), k, kshift, gate);
         hipLaunchKernelGGL(sk_comp_mean, dim3((uint32_t)D->nc), dim3(BLK), 0, st, dp<uint32_t>(cptr), dp<double>(part),
                            dp<uint32_t>(count), mu, k, kshift, gate);
@@ -1383,7 +1520,7 @@ struct SchwarzRunImpl : SchwarzRun {
         ensure(k);
         const uint32_t kshift = pow2_shift(k);
         const SchwarzDev<CT> &d = *D;
-        double *locp = dp<double>(loc), *redp = dp<double>(red), *xsp = dp<double>(xs);
+        CT *locp = dp<CT>(loc), *redp = dp<CT>(red), *xsp = dp<CT>(xs);
         hipLaunchKernelGGL((sk_restrict<CT>), dim3(grid_rows(d.ns, kshift)), dim3(BLK), 0, st, d.ns,
                            dp<uint32_t>(d.glob), dp<CT>(d.sgnpw), r, locp, k, kshift, gate);
         mean(d.all_chunks, d.n_all_chunks, d.all_cptr, d.comp_count, locp, dp<double>(mu1), k, kshift, st, gate);
@@ -1392,7 +1529,7 @@ struct SchwarzRunImpl : SchwarzRun {
         if (d.n_ct_fix)
             hipLaunchKernelGGL((sk_rows_fix<CT, 0>), dim3(std::min<uint32_t>(d.n_ct_fix, 65535)), dim3(BLK), 0, st,
                                dp<RowFix>(d.ct_fix), d.n_ct_fix, dp<uint32_t>(d.slot_comp), locp, dp<CT>(d.inv_de),
-                               dp<double>(mu1), dp<double>(rowpart), redp, k, kshift, gate);
+                               dp<double>(mu1), dp<CT>(rowpart), redp, k, kshift, gate);
         // kept-block solves
         if (d.n_sparse) {
             mean(d.kept_chunks, d.n_kept_chunks, d.kept_cptr, d.comp_kcount, redp, dp<double>(mu2), k, kshift, st, gate);
@@ -1408,7 +1545,7 @@ struct SchwarzRunImpl : SchwarzRun {
             }
             const TailArgs<CT> ta{dp<uint32_t>(d.tail_rows), dp<uint64_t>(d.tail_cptr), dp<TailChunk>(d.tail_chunks),
                                   dp<uint32_t>(d.tail_idx),  dp<CT>(d.tail_val),      dp<CT>(d.tail_mat),
-                                  dp<double>(tsc),           dp<double>(csc)};
+                                  dp<CT>(tsc),               dp<CT>(csc)};
             auto sparse = [&](uint32_t phase) {
                 hipLaunchKernelGGL((sk_sparse<CT>), dim3(d.n_sparse), dim3(SP_BLK), 0, st, dp<SpMeta>(d.sp_meta),
                                    dp<uint64_t>(d.sp_lvl_f_ptr), dp<uint64_t>(d.sp_lvl_f_long),
@@ -1446,7 +1583,7 @@ struct SchwarzRunImpl : SchwarzRun {
             }
             if (d.n_cover_rows)
                 for (uint32_t sheet = 0; sheet < 2; ++sheet)
-                    hipLaunchKernelGGL(sk_cover_readback, dim3(grid_rows(d.n_cover_rows, kshift)), dim3(BLK), 0, st,
+                    hipLaunchKernelGGL((sk_cover_readback<CT>), dim3(grid_rows(d.n_cover_rows, kshift)), dim3(BLK), 0, st,
                                        dp<uint32_t>(d.cover_rows), d.n_cover_rows, dp<uint32_t>(d.aux_src), d.ns,
                                        xsp, sheet, k, kshift, gate);
         }
@@ -1458,7 +1595,7 @@ struct SchwarzRunImpl : SchwarzRun {
                 while (kc > k - c0) kc >>= 1;
 #define WX_DS(N)                                                                                                   \
     hipLaunchKernelGGL((sk_dense_sym<CT, N>), dim3(d.n_dtiles), dim3(64), 0, st, dp<DenseTile>(d.dtiles),         \
-                       dp<CT>(d.dmat), redp, dp<double>(dpart), k, c0, gate)
+                       dp<CT>(d.dmat), redp, dp<CT>(dpart), k, c0, gate)
                 if (kc == 16) WX_DS(16);
                 else if (kc == 8) WX_DS(8);
                 else if (kc == 4) WX_DS(4);
@@ -1467,8 +1604,8 @@ struct SchwarzRunImpl : SchwarzRun {
 #undef WX_DS
                 c0 += kc;
             }
-            hipLaunchKernelGGL(sk_dense_reduce, dim3(grid_rows(d.dense_part_rows, kshift)), dim3(BLK), 0, st,
-                               dp<DenseComp>(d.dcomps), d.n_dcomps, dp<uint64_t>(d.drow_start), dp<double>(dpart), xsp,
+            hipLaunchKernelGGL((sk_dense_reduce<CT>), dim3(grid_rows(d.dense_part_rows, kshift)), dim3(BLK), 0, st,
+                               dp<DenseComp>(d.dcomps), d.n_dcomps, dp<uint64_t>(d.drow_start), dp<CT>(dpart), xsp,
                                k, kshift, gate);
         }
         if (d.n_drows)
@@ -1480,7 +1617,7 @@ struct SchwarzRunImpl : SchwarzRun {
         if (d.n_c_fix)
             hipLaunchKernelGGL((sk_rows_fix<CT, 1>), dim3(std::min<uint32_t>(d.n_c_fix, 65535)), dim3(BLK), 0, st,
                                dp<RowFix>(d.c_fix), d.n_c_fix, dp<uint32_t>(d.slot_comp), locp, dp<CT>(d.inv_de),
-                               dp<double>(mu1), dp<double>(rowpart), xsp, k, kshift, gate);
+                               dp<double>(mu1), dp<CT>(rowpart), xsp, k, kshift, gate);
         mean(d.all_chunks, d.n_all_chunks, d.all_cptr, d.comp_count, xsp, dp<double>(mu3), k, kshift, st, gate);
         hipLaunchKernelGGL((sk_prolong<CT>), dim3(grid_rows(d.N, kshift)), dim3(BLK), 0, st, d.N,
                            dp<uint64_t>(d.d_ptr), dp<uint32_t>(d.d_slot), dp<CT>(d.sgnpw),
@@ -1527,7 +1664,8 @@ std::unique_ptr<SchwarzRun> make_schwarz_run(const Precond &p, int32_t precision
     return r;
 }
 
-void precond_apply_device(const Precond &p, const double *x, uint64_t k, double *y, hipStream_t st) {
+void precond_apply_device(const Precond &p, const double *x, uint64_t k, double *y, hipStream_t st,
+                          int32_t precision) {
     const uint64_t n = p.n;
     if (p.kind == Precond::DIAGONAL) {
         DevBuf inv;
@@ -1538,7 +1676,7 @@ void precond_apply_device(const Precond &p, const double *x, uint64_t k, double 
         WX_HIP(hipStreamSynchronize(st));
         return;
     }
-    auto run = make_schwarz_run(p, WX_PRECISION_FP64, st);
+    auto run = make_schwarz_run(p, precision, st);
     for (uint64_t c0 = 0; c0 < k; c0 += MAX_BATCH) {
         const uint32_t kk = (uint32_t)std::min<uint64_t>(MAX_BATCH, k - c0);
         DevBuf xi, yi;

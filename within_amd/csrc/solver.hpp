@@ -215,6 +215,9 @@ std::vector<double> &last_timing();
 void design_apply_device(const Solver &s, bool adjoint, const double *X, uint64_t k, double *Y, hipStream_t stream);
 
 // y = M^{-1} x on the device for n x k column-major arrays (FP64 coefficients).
-void precond_apply_device(const Precond &p, const double *x, uint64_t k, double *y, hipStream_t stream);
+// y = M^{-1} x (n x k column-major device arrays); `precision` selects the apply of the FP64 or
+// the mixed solve (FP32 coefficients and slot-space vectors).
+void precond_apply_device(const Precond &p, const double *x, uint64_t k, double *y, hipStream_t stream,
+                          int32_t precision = WX_PRECISION_FP64);
 
 } // namespace wx
