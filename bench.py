@@ -129,6 +129,37 @@ def run_plain_fp64(solver, Y_dev, n, N, tol, n_rhs):
     return n_rhs / (time.perf_counter() - t0), its
 
 
+def schwarz_bytes(pre_bytes: bytes, k: int, value_bytes: int, N: int) -> dict:
+    """Algorithmic bytes of one additive Schwarz apply (the per-apply formula of the hot-path
+    scope: factor and cross-tab entries at (index + value) bytes, each read twice -- once by each
+    elimination SpMV, once by each triangular sweep --, dense kept blocks read once, the
+    slot-space vectors written and read four times, plus the FP64 input and output DOF vectors),
+    from the preconditioner's exported arrays (little-endian WXPC layout)."""
+    import struct
+    assert pre_bytes[:4] == b"WXPC"
+    _, kind, n = struct.unpack_from("<IIQ", pre_bytes, 4)
+    if kind != 1:
+        return {}
+    sizes = {}
+    pos = 20
+    fields = [("slot_start", 8), ("ne", 4), ("nk", 4), ("fac_kind", 1), ("gauge", 1), ("glob", 4), ("pw", 8),
+              ("fold", 8), ("inv_de", 8), ("c_ptr", 8), ("c_idx", 4), ("c_val", 8), ("dense_off", 8), ("dense", 8),
+              ("l_row_off", 8), ("l_ptr", 8), ("l_idx", 4), ("l_val", 8), ("piv_off", 8), ("piv", 8),
+              ("cover_off", 8), ("cover_perm", 4)]
+    for name, sz in fields:
+        m, = struct.unpack_from("<Q", pre_bytes, pos)
+        sizes[name] = m
+        pos += 8 + m * sz
+    s = value_bytes
+    slots = sizes["glob"]
+    b_c = sizes["c_idx"] * (4 + s) * 2
+    b_dense = sizes["dense"] * s
+    b_l = sizes["l_idx"] * (4 + s) * 2
+    b_vec = s * k * slots * 4 + 8 * k * N * 2
+    return {"bytes": b_c + b_dense + b_l + b_vec, "cross_tab_entries": sizes["c_idx"], "dense_entries": sizes["dense"],
+            "factor_entries": sizes["l_idx"], "slots": slots}
+
+
 def host_info():
     """The host the CPU baseline runs on: logical CPUs, the CPUs this process may use, the thread
     share it is given (OMP_NUM_THREADS, 16 per GPU on the GPU box) and the CPU model (as lscpu
@@ -569,12 +600,24 @@ def main():
         "roofline_adjoint_rest": {"achieved": b_rest / adj_dur / 1e9, "unit": "GB/s",
                                   "frac": b_rest / adj_dur / 1e9 / HBM_PEAK_GBS,
                                   "algorithmic_bytes_per_iteration": b_rest, "ms_per_iteration": adj_dur * 1e3,
-                                  "what": "D' of the other terms (k_segreduce + k_segfix launches)"},
+                                  "what": "D' of the other terms (fused pair pass, piece and split-level sums)"},
         "iterations": [s.iterations for s in st],
         "true_residual_max": max(s.true_residual for s in st),
         "setup_s": setup,
         "phases_ms": {p: {"total_ms": v[0], "launches": v[1]} for p, v in ph.items()},
     }
+    pre = solver.preconditioner
+    if pre is not None:
+        sb = schwarz_bytes(pre.to_bytes(), k, 4, N)
+        p_ms, p_launches = ph["precond"]
+        if sb and p_launches:
+            # as for the gather, the launch after the last column converged returns at once
+            per = p_ms / iters * 1e-3
+            out["roofline_precond"] = {"achieved": sb["bytes"] / per / 1e9, "unit": "GB/s",
+                                       "frac": sb["bytes"] / per / 1e9 / HBM_PEAK_GBS,
+                                       "algorithmic_bytes_per_apply": sb["bytes"], "ms_per_apply": per * 1e3,
+                                       "what": "additive Schwarz apply (mixed: FP32 coefficients and slot vectors)",
+                                       **{kk: vv for kk, vv in sb.items() if kk != "bytes"}}
     if "fp64" in res:
         el64, st64, ph64 = res["fp64"]
         out["fp64_value"] = k * args.steps / el64

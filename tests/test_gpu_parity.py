@@ -640,3 +640,49 @@ def test_mixed_precision_apply_is_symmetric_and_definite(gpu, kind):
     rf = s.solve_batch(Y, wa.LsmrOptions(tol=1e-10))
     assert all(rm.converged) and all(rf.converged)
     assert max(rel(rm.demeaned[:, j], rf.demeaned[:, j]) for j in range(2)) <= 1e-8
+
+
+def test_dense_tails_stay_bounded_on_many_mid_size_components(gpu, oracle):
+    """Approximate Cholesky everywhere (dense threshold 0) on a design of 40 disconnected blocks:
+    many mid-size sparse kept blocks.  The apply matches the oracle's level-by-level restatement of
+    the same factors, and the dense tails' entries stay within 8 times the factors' entries and
+    rows (the device forms a tail only where it does not outweigh the factor it replaces)."""
+    import re
+    import subprocess
+    import sys
+    code = r'''
+import numpy as np, within_amd as wa
+rng = np.random.default_rng(91)
+blocks = []
+for b in range(40):
+    n = 3000
+    a = rng.integers(0, 150, n) + 150 * b
+    c = rng.integers(0, 70, n) + 70 * b
+    blocks.append(np.column_stack([a, c]))
+codes = np.asfortranarray(np.concatenate(blocks).astype(np.uint32))
+np.save("gpurun_out/_tails_codes.npy", codes)
+cfg = wa.config.AdditiveSchwarz(wa.config.LocalSolverConfig(dense_threshold=0, dense_policy="reference"))
+s = wa.Solver(codes, preconditioner=cfg)
+p = s.preconditioner
+open("gpurun_out/_tails_pre.bin", "wb").write(p.to_bytes())
+x = np.random.default_rng(5).standard_normal(s.n_dofs)
+np.save("gpurun_out/_tails_x.npy", x)
+np.save("gpurun_out/_tails_y.npy", p.apply(x))
+r = s.solve(np.random.default_rng(6).standard_normal(codes.shape[0]), wa.LsmrOptions(tol=1e-10))
+print("converged", r.converged, r.iterations)
+'''
+    import os
+    os.makedirs("gpurun_out", exist_ok=True)
+    out = subprocess.run([sys.executable, "-c", code], capture_output=True, text=True, timeout=300,
+                         env={**os.environ, "WITHIN_AMD_VERBOSE": "1"})
+    assert out.returncode == 0, out.stderr[-3000:]
+    assert "converged True" in out.stdout
+    m = re.findall(r"sparse factors: (\d+) rows, (\d+) factor entries, dense tails of (\d+) rows \((\d+) entries",
+                   out.stderr)
+    assert m, out.stderr[-2000:]
+    rows, entries, t_rows, t_entries = map(int, m[-1])
+    print(f"factors: {rows} rows, {entries} entries; tails: {t_rows} rows, {t_entries} entries")
+    assert t_entries <= 8 * (entries + rows)
+    sch = oracle_schwarz_from_bytes(oracle, open("gpurun_out/_tails_pre.bin", "rb").read())
+    x = np.load("gpurun_out/_tails_x.npy")
+    assert rel(np.load("gpurun_out/_tails_y.npy"), sch.apply(x)) <= 1e-11

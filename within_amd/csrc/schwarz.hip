@@ -39,7 +39,9 @@ constexpr uint64_t LONG_ROW = 64;         // factor rows with more entries are s
 constexpr uint64_t BIG_LEVEL = 256;       // factor levels with at least this many rows use grid-wide launches
 constexpr uint32_t TAIL_LEVEL_ROWS = 64;  // factor levels this small at the end of the schedule ...
 constexpr uint32_t TAIL_MAX_ROWS = 1024;  // ... up to this many rows ...
-constexpr uint32_t TAIL_MIN_LEVELS = 8;   // ... and at least this many levels form the dense tail
+constexpr uint32_t TAIL_MIN_LEVELS = 8;   // ... and at least this many levels form the dense tail,
+constexpr uint64_t TAIL_FILL = 8;         // ... whose nt^2 entries stay within this many times the
+                                          // component's factor entries and rows
 constexpr uint32_t TAIL_CHUNK = 32;       // tail-row entries per partial sum
 
 struct RowItem {
@@ -1231,13 +1233,17 @@ std::shared_ptr<SchwarzDev<CT>> build_dev(const Precond::Schwarz &S) {
             }
             // dense tail: the trailing levels with at most TAIL_LEVEL_ROWS rows each (a chain of
             // hubs eliminated last), at most TAIL_MAX_ROWS rows, when they span TAIL_MIN_LEVELS
-            // levels or more; the schedules then cover the rows before the cut only
+            // levels or more and the tail's nt^2 dense entries stay within TAIL_FILL times the
+            // component's factor entries and rows (so no tail outweighs the factor it replaces);
+            // the schedules then cover the rows before the cut only
             uint32_t cut = df;
             {
                 std::vector<uint32_t> per(df, 0);
                 for (uint32_t i = 0; i < nk; ++i) per[lf[i]]++;
-                uint32_t rows_t = 0;
-                while (cut > 1 && per[cut - 1] <= TAIL_LEVEL_ROWS && rows_t + per[cut - 1] <= TAIL_MAX_ROWS)
+                const uint64_t fill_cap = TAIL_FILL * ((S.l_ptr[ro + nk] - S.l_ptr[ro]) + nk);
+                uint64_t rows_t = 0;
+                while (cut > 1 && per[cut - 1] <= TAIL_LEVEL_ROWS && rows_t + per[cut - 1] <= TAIL_MAX_ROWS &&
+                       (rows_t + per[cut - 1]) * (rows_t + per[cut - 1]) <= fill_cap)
                     rows_t += per[--cut];
                 if (df - cut < TAIL_MIN_LEVELS) cut = df;
             }
@@ -1437,6 +1443,12 @@ std::shared_ptr<SchwarzDev<CT>> build_dev(const Precond::Schwarz &S) {
         D->tail_val.upload(t_val.data(), t_val.size());
         D->tail_mat.upload(t_mat.data(), t_mat.size());
         D->tail_prod.upload(t_prod.data(), t_prod.size());
+        if (std::getenv("WITHIN_AMD_VERBOSE"))
+            std::fprintf(stderr, "[within_amd] sparse factors: %llu rows, %llu factor entries, dense tails of %llu rows "
+                                 "(%llu entries, %.1f MB)\n",
+                         (unsigned long long)piv.size(), (unsigned long long)lidx.size(),
+                         (unsigned long long)t_rows.size(),
+                         (unsigned long long)t_mat.size(), (double)(t_mat.size() * sizeof(CT)) / 1e6);
         D->sp_lvl_f_ptr.upload(lvf_ptr.data(), lvf_ptr.size());
         D->sp_lvl_f_long.upload(lvf_long.data(), lvf_long.size());
         D->sp_lvl_b_long.upload(lvb_long.data(), lvb_long.size());
