@@ -686,3 +686,37 @@ print("converged", r.converged, r.iterations)
     sch = oracle_schwarz_from_bytes(oracle, open("gpurun_out/_tails_pre.bin", "rb").read())
     x = np.load("gpurun_out/_tails_x.npy")
     assert rel(np.load("gpurun_out/_tails_y.npy"), sch.apply(x)) <= 1e-11
+
+
+@pytest.mark.parametrize("case", ["weighted_3f", "unweighted_2f_gaps", "zero_weights"])
+def test_device_cross_tabs_give_the_host_preconditioner(gpu, case):
+    """The plain pairs' cross-tabs built on the device (stable sort by cell, each cell's weights
+    added in observation order) make the preconditioner byte for byte equal to the host-built one
+    (WITHIN_AMD_HOST_SETUP=1), for the default local solver and for approximate Cholesky
+    everywhere."""
+    from within_amd._within import build_preconditioner
+    rng = np.random.default_rng(61)
+    n = 40_000
+    if case == "weighted_3f":
+        codes = np.column_stack([rng.integers(0, 2000, n), rng.integers(0, 300, n), rng.integers(0, 9, n)])
+        w = rng.uniform(0.5, 2.0, n)
+    elif case == "unweighted_2f_gaps":
+        codes = np.column_stack([rng.choice(np.r_[0:3000:3], n), rng.integers(0, 120, n)])
+        w = None
+    else:
+        codes = np.column_stack([rng.integers(0, 1500, n), rng.integers(0, 200, n), rng.integers(0, 4, n)])
+        w = rng.uniform(0.5, 2.0, n)
+        w[rng.uniform(size=n) < 0.1] = 0.0
+    codes = np.asfortranarray(codes.astype(np.uint32))
+    for ls in (wa.config.LocalSolverConfig(), wa.config.LocalSolverConfig(dense_threshold=0, dense_policy="reference")):
+        cfg = wa.config.AdditiveSchwarz(ls)
+        dev = build_preconditioner(codes, w, cfg).to_bytes()
+        os.environ["WITHIN_AMD_HOST_SETUP"] = "1"
+        try:
+            host = build_preconditioner(codes, w, cfg).to_bytes()
+        finally:
+            del os.environ["WITHIN_AMD_HOST_SETUP"]
+        assert dev == host
+    # and the Solver (which builds through the same path) solves with it
+    r = wa.solve(codes, np.asfortranarray(rng.standard_normal(n)), weights=w)
+    assert r.converged

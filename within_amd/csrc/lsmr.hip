@@ -1433,49 +1433,61 @@ uint64_t build_seg_plan(DevTerm &dt, const std::vector<uint64_t> &start, uint64_
 // piece's sum, and Q's D' is then a segmented sum over the pieces in Q order.  So u is read
 // once for both terms, and the extra traffic is one write and one read of the pieces.  Used
 // only when the pieces number at most a quarter of the rows.
-void build_pair_plan(const Design &d, const std::vector<double> &w, std::vector<std::vector<SegItem>> &host_items,
-                     DevDesign &dv) {
-    dv.pair = PairPlan{};
-    if (!dv.plain) return;
+std::pair<int32_t, int32_t> pair_candidate(const Design &d, bool plain) {
     int32_t P = -1, Q = -1;
+    if (!plain) return {-1, -1};
     for (size_t t = 0; t < d.terms.size(); ++t) {
         if (d.terms[t].sorted) continue;
         if (P < 0 || d.terms[t].n_levels > d.terms[P].n_levels) P = (int32_t)t;
         if (Q < 0 || d.terms[t].n_levels <= d.terms[Q].n_levels) Q = (int32_t)t;
     }
-    if (P < 0 || Q < 0 || P == Q) return;
+    if (P < 0 || Q < 0 || P == Q) return {-1, -1};
+    return {P, Q};
+}
+
+// An unsorted term's code-sorted permutation (stable) and its position-ordered weights, on the
+// device from the uploaded codes and weights.
+void term_permutation(DevDesign &dv, DevTerm &dt, uint32_t t, uint64_t n_levels) {
+    const uint64_t n = dv.n;
+    dt.perm.alloc(n * 4);
+    dev_stable_argsort(dv.codes.as<uint32_t>() + (uint64_t)t * n, n, n_levels, dt.perm.as<uint32_t>(), nullptr);
+    if (dv.w.p) { // weights in this term's sorted order (sequential reads)
+        dt.w_pos.alloc(n * 8);
+        dt.sw_pos.alloc(n * 8);
+        dev_gather(dv.w.as<double>(), dt.perm.as<uint32_t>(), n, dt.w_pos.as<double>(), nullptr);
+        dev_gather(dv.sw.as<double>(), dt.perm.as<uint32_t>(), n, dt.sw_pos.as<double>(), nullptr);
+        WX_HIP(hipStreamSynchronize(nullptr));
+    }
+}
+
+// Returns false when the design has no qualifying pair (the terms keep their own plans).
+bool build_pair_plan(const Design &d, const std::vector<std::vector<uint64_t>> &starts, DevDesign &dv) {
+    dv.pair = PairPlan{};
+    const auto [P, Q] = pair_candidate(d, dv.plain);
+    if (P < 0) return false;
     const uint64_t n = d.n_obs, LQ = d.terms[Q].n_levels, LP = d.terms[P].n_levels;
-    const uint32_t *cp = d.level_column(P), *cq = d.level_column(Q);
-    // rows in (P, Q) order: stable by Q, then a stable counting sort by P
-    const std::vector<uint32_t> byq = stable_argsort(cq, n, LQ);
-    std::vector<uint64_t> cur(LP + 1, 0);
-    for (uint64_t i = 0; i < n; ++i) cur[cp[i] + 1]++;
-    for (uint64_t l = 0; l < LP; ++l) cur[l + 1] += cur[l];
-    std::vector<uint32_t> perm(n);
-    for (uint64_t q = 0; q < n; ++q) perm[cur[cp[byq[q]]]++] = byq[q];
-    std::vector<uint32_t> qpos(n);
-    for (uint64_t q = 0; q < n; ++q) qpos[q] = cq[perm[q]];
-    // P's work items of up to PAIR_CHUNK rows
-    std::vector<uint64_t> pstart(LP + 1, 0);
-    for (uint64_t i = 0; i < n; ++i) pstart[cp[i] + 1]++;
-    for (uint64_t l = 0; l < LP; ++l) pstart[l + 1] += pstart[l];
+    const uint32_t *cp = dv.codes.as<uint32_t>() + (uint64_t)P * n, *cq = dv.codes.as<uint32_t>() + (uint64_t)Q * n;
+    hipStream_t st = nullptr;
+    // rows in (P, Q) order (a stable sort of the composite key) and Q's code at each position
+    DevBuf perm, qpos;
+    perm.alloc(n * 4);
+    qpos.alloc(n * 4);
+    dev_stable_argsort2(cp, cq, LP, LQ, n, perm.as<uint32_t>(), st);
+    dev_gather_u32(cq, perm.as<uint32_t>(), n, qpos.as<uint32_t>(), st);
+    // P's work items of up to PAIR_CHUNK rows, and the pieces they hold (the kernel's rule)
     DevTerm pplan;
     pplan.info = dv.terms[P].info;
     std::vector<SegItem> items;
-    const uint64_t pslots = build_seg_plan(pplan, pstart, LP, items, PAIR_CHUNK);
-    // pieces: runs of equal Q code inside each of P's items (the kernel's rule)
-    std::vector<uint32_t> piece0(items.size() + 1), piece_q;
-    for (size_t it = 0; it < items.size(); ++it) {
-        piece0[it] = (uint32_t)piece_q.size();
-        for (uint32_t q = items[it].begin; q < items[it].end; ++q)
-            if (q == items[it].begin || qpos[q] != qpos[q - 1]) piece_q.push_back(qpos[q]);
-        if (piece_q.size() > n / 4) return;
-    }
-    piece0[items.size()] = (uint32_t)piece_q.size();
+    const uint64_t pslots = build_seg_plan(pplan, starts[P], LP, items, PAIR_CHUNK);
+    DevBuf piece0, piece_q;
+    piece0.alloc((items.size() + 1) * 4);
+    const uint64_t n_pieces =
+        dev_pieces(qpos.as<uint32_t>(), n, pplan.items.as<SegItem>(), pplan.n_items, piece0.as<uint32_t>(), piece_q, st);
+    if (n_pieces > n / 4 || n_pieces > 0xFFFFFFFFull) return false;
     PairPlan &pp = dv.pair;
     pp.p = P;
     pp.q = Q;
-    pp.n_pieces = (uint32_t)piece_q.size();
+    pp.n_pieces = (uint32_t)n_pieces;
     DevTerm &pt = dv.terms[P];
     pt.items = std::move(pplan.items);
     pt.n_items = pplan.n_items;
@@ -1484,69 +1496,96 @@ void build_pair_plan(const Design &d, const std::vector<double> &w, std::vector<
     pt.n_partials = pplan.n_partials;
     pt.stages = std::move(pplan.stages);
     dv.max_partials = std::max<uint32_t>(dv.max_partials, (uint32_t)pslots);
-    pt.perm.upload(perm.data(), perm.size());
-    if (!w.empty()) {
-        std::vector<double> wp(n), swp(n);
-        for (uint64_t q = 0; q < n; ++q) {
-            wp[q] = w[perm[q]];
-            swp[q] = std::sqrt(w[perm[q]]);
-        }
-        pt.w_pos.upload(wp.data(), wp.size());
-        pt.sw_pos.upload(swp.data(), swp.size());
+    pt.perm = std::move(perm);
+    if (dv.w.p) {
+        pt.w_pos.alloc(n * 8);
+        pt.sw_pos.alloc(n * 8);
+        dev_gather(dv.w.as<double>(), pt.perm.as<uint32_t>(), n, pt.w_pos.as<double>(), st);
+        dev_gather(dv.sw.as<double>(), pt.perm.as<uint32_t>(), n, pt.sw_pos.as<double>(), st);
     }
-    pp.qcode_pos.upload(qpos.data(), qpos.size());
-    pp.item_piece0.upload(piece0.data(), piece0.size());
+    pp.qcode_pos = std::move(qpos);
+    pp.item_piece0 = std::move(piece0);
     // the pieces are stored in Q order (piece -> its slot there), so Q's reduction reads them
     // contiguously
     DevTerm &qt = pp.qt;
     qt.info = dv.terms[Q].info;
-    const std::vector<uint32_t> pperm = stable_argsort(piece_q.data(), piece_q.size(), LQ);
-    std::vector<uint32_t> slot(pperm.size());
-    for (uint32_t q = 0; q < (uint32_t)pperm.size(); ++q) slot[pperm[q]] = q;
-    pp.piece_slot.upload(slot.data(), slot.size());
+    DevBuf pperm;
+    pperm.alloc(n_pieces * 4);
+    dev_stable_argsort(piece_q.as<uint32_t>(), n_pieces, LQ, pperm.as<uint32_t>(), st);
+    pp.piece_slot.alloc(n_pieces * 4);
+    dev_invert_perm(pperm.as<uint32_t>(), n_pieces, pp.piece_slot.as<uint32_t>(), st);
+    std::vector<uint32_t> hq(n_pieces);
+    WX_HIP(hipMemcpyAsync(hq.data(), piece_q.p, n_pieces * 4, hipMemcpyDeviceToHost, st));
+    WX_HIP(hipStreamSynchronize(st));
     std::vector<uint64_t> start(LQ + 1, 0);
-    for (uint32_t c : piece_q) start[c + 1]++;
+    for (uint32_t c : hq) start[c + 1]++;
     for (uint64_t l = 0; l < LQ; ++l) start[l + 1] += start[l];
     std::vector<SegItem> qitems;
     const uint64_t slots = build_seg_plan(qt, start, LQ, qitems);
     dv.max_partials = std::max<uint32_t>(dv.max_partials, (uint32_t)slots);
+    return true;
 }
 } // namespace
 
 void upload_design(Solver &s, const std::vector<double> &weights_internal) {
     const Design &d = s.design;
     DevDesign &dv = s.dev;
+    // WITHIN_AMD_VERBOSE: per-phase wall times of the upload on stderr
+    const bool verbose = std::getenv("WITHIN_AMD_VERBOSE") != nullptr;
+    auto t_last = std::chrono::steady_clock::now();
+    auto phase = [&](const char *name) {
+        if (!verbose) return;
+        const auto now = std::chrono::steady_clock::now();
+        std::fprintf(stderr, "[within_amd] device design: %s %.1f ms\n", name,
+                     std::chrono::duration<double, std::milli>(now - t_last).count());
+        t_last = now;
+    };
     if (d.terms.size() > MAX_TERMS)
         fail(WX_ERR_INVALID_INPUT, "the device path supports at most " + std::to_string(MAX_TERMS) + " terms");
     dv.n = d.n_obs;
     dv.N = d.n_dofs;
+    const uint64_t n = d.n_obs;
+    hipStream_t st0 = nullptr;
+    // codes, weights and the row permutation go up once; everything derived from them (square
+    // roots, inverse permutation, the terms' sort permutations, position-ordered weights, code
+    // records) is formed on the device
     dv.codes.upload(d.codes.data(), d.codes.size());
     dv.slopes.upload(d.slopes.data(), d.slopes.size());
     if (!weights_internal.empty()) {
-        std::vector<double> sw(weights_internal.size());
-        for (size_t i = 0; i < sw.size(); ++i) sw[i] = std::sqrt(weights_internal[i]);
-        dv.sw.upload(sw.data(), sw.size());
         dv.w.upload(weights_internal.data(), weights_internal.size());
+        dv.sw.alloc(n * 8);
+        dev_sqrt(dv.w.as<double>(), n, dv.sw.as<double>(), st0);
     }
     if (!d.obs_perm.empty()) {
         dv.obs_perm.upload(d.obs_perm.data(), d.obs_perm.size());
-        std::vector<uint32_t> inv(d.obs_perm.size());
-        for (uint64_t i = 0; i < inv.size(); ++i) inv[d.obs_perm[i]] = (uint32_t)i;
-        dv.inv_perm.upload(inv.data(), inv.size());
+        dv.inv_perm.alloc(n * 4);
+        dev_invert_perm(dv.obs_perm.as<uint32_t>(), n, dv.inv_perm.as<uint32_t>(), st0);
         if (!weights_internal.empty()) { // sqrt-weights in caller order for the right-hand-side load
-            std::vector<double> swc(inv.size());
-            for (uint64_t r = 0; r < inv.size(); ++r) swc[r] = std::sqrt(weights_internal[inv[r]]);
-            dv.sw_caller.upload(swc.data(), swc.size());
+            dv.sw_caller.alloc(n * 8);
+            dev_gather(dv.sw.as<double>(), dv.inv_perm.as<uint32_t>(), n, dv.sw_caller.as<double>(), st0);
         }
     }
+    WX_HIP(hipStreamSynchronize(st0));
     if (s.precond && s.precond->kind == Precond::DIAGONAL)
         dv.inv_diag.upload(s.precond->inv_diag.data(), s.precond->inv_diag.size());
+    phase("codes, weights, permutation");
     dv.terms.clear();
     dv.terms.resize(d.terms.size());
     dv.max_partials = 0;
     std::vector<std::vector<SegItem>> host_items(d.terms.size()); // every term's D' work items
     dv.plain = true;
     for (const Term &tm : d.terms) dv.plain = dv.plain && tm.ncols() == 1 && tm.loads[0] < 0;
+    // level starts of every term (counts in parallel over terms)
+    std::vector<std::vector<uint64_t>> starts(d.terms.size());
+#pragma omp parallel for schedule(dynamic, 1)
+    for (int64_t t = 0; t < (int64_t)d.terms.size(); ++t) {
+        const uint32_t *col = d.level_column((uint64_t)t);
+        std::vector<uint64_t> &start = starts[t];
+        start.assign(d.terms[t].n_levels + 1, 0);
+        for (uint64_t i = 0; i < n; ++i) start[col[i] + 1]++;
+        for (uint64_t l = 0; l < d.terms[t].n_levels; ++l) start[l + 1] += start[l];
+    }
+    const int32_t pair_p = pair_candidate(d, dv.plain).first;
     for (size_t t = 0; t < d.terms.size(); ++t) {
         const Term &tm = d.terms[t];
         DevTerm &dt = dv.terms[t];
@@ -1557,27 +1596,15 @@ void upload_design(Solver &s, const std::vector<double> &weights_internal) {
         dt.info.offset = tm.offset;
         dt.info.ncols = (uint32_t)tm.ncols();
         for (uint32_t c = 0; c < MAX_COLS; ++c) dt.info.load[c] = c < tm.ncols() ? (int32_t)tm.loads[c] : -1;
-        const uint32_t *col = d.level_column(t);
-        std::vector<uint64_t> start(tm.n_levels + 1, 0);
-        for (uint64_t i = 0; i < d.n_obs; ++i) start[col[i] + 1]++;
-        for (uint64_t l = 0; l < tm.n_levels; ++l) start[l + 1] += start[l];
-        if (!tm.sorted) {
-            std::vector<uint32_t> perm = stable_argsort(col, d.n_obs, tm.n_levels);
-            dt.perm.upload(perm.data(), perm.size());
-            if (!weights_internal.empty()) { // weights in this term's sorted order (sequential reads)
-                std::vector<double> wp(perm.size()), swp(perm.size());
-                for (size_t q = 0; q < perm.size(); ++q) {
-                    wp[q] = weights_internal[perm[q]];
-                    swp[q] = std::sqrt(weights_internal[perm[q]]);
-                }
-                dt.w_pos.upload(wp.data(), wp.size());
-                dt.sw_pos.upload(swp.data(), swp.size());
-            }
-        }
-        const uint64_t slots = build_seg_plan(dt, start, tm.n_levels, host_items[t]);
+        // the fused pair's primary term gets its permutation from the pair plan
+        if (!tm.sorted && (int32_t)t != pair_p) term_permutation(dv, dt, (uint32_t)t, tm.n_levels);
+        const uint64_t slots = build_seg_plan(dt, starts[t], tm.n_levels, host_items[t]);
         dv.max_partials = std::max<uint32_t>(dv.max_partials, (uint32_t)slots);
     }
-    build_pair_plan(d, weights_internal, host_items, dv);
+    phase("D' plans");
+    if (!build_pair_plan(d, starts, dv) && pair_p >= 0) // no fused pair: P's own permutation
+        term_permutation(dv, dv.terms[pair_p], (uint32_t)pair_p, d.terms[pair_p].n_levels);
+    phase("fused pair plan");
     // a row's codes as one aligned 16-byte record: the fused gather and the demean read them with
     // one load instead of one per term
     dv.codes4.release();
@@ -1620,6 +1647,7 @@ void upload_design(Solver &s, const std::vector<double> &weights_internal) {
         ft.gitems.upload(g.data(), g.size());
         ft.n_gitems = (uint32_t)g.size();
     }
+    phase("code records and gather items");
 }
 
 namespace {

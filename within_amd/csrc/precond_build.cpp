@@ -493,9 +493,12 @@ std::string pair_name(Channel a, Channel b) {
 // Cross-tab of one channel pair and its components (cross_tab.rs, accumulate.rs, factor_pairs.rs:95-152),
 // each converted to SDDM form (sddm.rs).  Runs inside a parallel loop, so a failure is returned
 // as a message in `error` rather than thrown.
+// `cells`, when given (a plain pair), holds the cross-tab computed on the device, bitwise equal to
+// the host accumulation below.
 void build_pair_components(const Design &d, const std::vector<double> &w, Channel rc, Channel cc,
                            const std::vector<std::vector<uint8_t>> &active, const LocalSolverSettings &cfg,
-                           std::vector<Component> &out, std::vector<std::string> &warnings, std::string &error) {
+                           std::vector<Component> &out, std::vector<std::string> &warnings, std::string &error,
+                           const CellList *cells = nullptr) {
     const Term &tr = d.terms[rc.term], &tc = d.terms[cc.term];
     const int64_t lr = tr.loads[rc.column], lc = tc.loads[cc.column];
     const bool general = lr >= 0 || lc >= 0;
@@ -514,67 +517,86 @@ void build_pair_components(const Design &d, const std::vector<double> &w, Channe
         if (active[rc.term][l]) l2g.push_back((uint32_t)(base_r + l));
     for (uint64_t l = 0; l < tc.n_levels; ++l)
         if (active[cc.term][l]) l2g.push_back((uint32_t)(base_c + l));
-    // accumulate: bucket by row (observation order), dedup per row with sorted columns
-    const uint32_t *rlev = d.level_column(rc.term), *clev = d.level_column(cc.term);
-    const double *zr = lr >= 0 ? d.slope_column((uint64_t)lr) : nullptr;
-    const double *zc = lc >= 0 ? d.slope_column((uint64_t)lc) : nullptr;
     std::vector<double> rdiag(nr, 0.0), cdiag(nc, 0.0);
-    std::vector<uint64_t> rcount(nr + 1, 0);
-    for (uint64_t i = 0; i < d.n_obs; ++i) {
-        const uint32_t cj = rmap[rlev[i]], ck = cmap[clev[i]];
-        const double wi = w.empty() ? 1.0 : w[i];
-        const double a = zr ? zr[i] : 1.0, b = zc ? zc[i] : 1.0;
-        rdiag[cj] += wi * a * a;
-        cdiag[ck] += wi * b * b;
-        rcount[cj + 1]++;
-    }
-    for (uint32_t r = 0; r < nr; ++r) rcount[r + 1] += rcount[r];
-    std::vector<uint32_t> bcol(d.n_obs);
-    std::vector<double> bval(d.n_obs);
-    {
-        std::vector<uint64_t> cur(rcount.begin(), rcount.end() - 1);
-        for (uint64_t i = 0; i < d.n_obs; ++i) {
-            const uint32_t cj = rmap[rlev[i]];
-            const double wi = w.empty() ? 1.0 : w[i];
-            const double a = zr ? zr[i] : 1.0, b = zc ? zc[i] : 1.0;
-            bcol[cur[cj]] = cmap[clev[i]];
-            bval[cur[cj]] = wi * a * b;
-            cur[cj]++;
-        }
-    }
     std::vector<uint64_t> cptr(nr + 1, 0);
     std::vector<uint32_t> cidx;
     std::vector<double> cval;
-    cidx.reserve(d.n_obs);
-    cval.reserve(d.n_obs);
-    {
-        std::vector<double> work(nc, 0.0);
-        std::vector<uint32_t> touched;
-        std::vector<uint8_t> seen(nc, 0);
-        for (uint32_t r = 0; r < nr; ++r) {
-            for (uint64_t p = rcount[r]; p < rcount[r + 1]; ++p) {
-                const uint32_t c = bcol[p];
-                if (!seen[c]) {
-                    seen[c] = 1;
-                    touched.push_back(c);
-                }
-                work[c] += bval[p];
+    if (cells && !general) {
+        // the device's cells (row-major keys level_r * L_c + level_c); zero-weight cells dropped as below
+        const uint64_t lcn = tc.n_levels;
+        cidx.reserve(cells->key.size());
+        cval.reserve(cells->key.size());
+        for (size_t q = 0; q < cells->key.size(); ++q) {
+            const uint32_t cj = rmap[cells->key[q] / lcn], ck = cmap[cells->key[q] % lcn];
+            const double v = cells->val[q];
+            rdiag[cj] += v;
+            cdiag[ck] += v;
+            if (v != 0.0) {
+                cidx.push_back(ck);
+                cval.push_back(v);
+                cptr[cj + 1]++;
             }
-            std::sort(touched.begin(), touched.end());
-            for (uint32_t c : touched) {
-                if (work[c] != 0.0) {
-                    cidx.push_back(c);
-                    cval.push_back(work[c]);
-                }
-                work[c] = 0.0;
-                seen[c] = 0;
-            }
-            touched.clear();
-            cptr[r + 1] = cidx.size();
         }
+        for (uint32_t r = 0; r < nr; ++r) cptr[r + 1] += cptr[r];
+    } else {
+        // accumulate: bucket by row (observation order), dedup per row with sorted columns
+        const uint32_t *rlev = d.level_column(rc.term), *clev = d.level_column(cc.term);
+        const double *zr = lr >= 0 ? d.slope_column((uint64_t)lr) : nullptr;
+        const double *zc = lc >= 0 ? d.slope_column((uint64_t)lc) : nullptr;
+        std::vector<uint64_t> rcount(nr + 1, 0);
+        for (uint64_t i = 0; i < d.n_obs; ++i) {
+            const uint32_t cj = rmap[rlev[i]], ck = cmap[clev[i]];
+            const double wi = w.empty() ? 1.0 : w[i];
+            const double a = zr ? zr[i] : 1.0, b = zc ? zc[i] : 1.0;
+            rdiag[cj] += wi * a * a;
+            cdiag[ck] += wi * b * b;
+            rcount[cj + 1]++;
+        }
+        for (uint32_t r = 0; r < nr; ++r) rcount[r + 1] += rcount[r];
+        std::vector<uint32_t> bcol(d.n_obs);
+        std::vector<double> bval(d.n_obs);
+        {
+            std::vector<uint64_t> cur(rcount.begin(), rcount.end() - 1);
+            for (uint64_t i = 0; i < d.n_obs; ++i) {
+                const uint32_t cj = rmap[rlev[i]];
+                const double wi = w.empty() ? 1.0 : w[i];
+                const double a = zr ? zr[i] : 1.0, b = zc ? zc[i] : 1.0;
+                bcol[cur[cj]] = cmap[clev[i]];
+                bval[cur[cj]] = wi * a * b;
+                cur[cj]++;
+            }
+        }
+        cidx.reserve(d.n_obs);
+        cval.reserve(d.n_obs);
+        {
+            std::vector<double> work(nc, 0.0);
+            std::vector<uint32_t> touched;
+            std::vector<uint8_t> seen(nc, 0);
+            for (uint32_t r = 0; r < nr; ++r) {
+                for (uint64_t p = rcount[r]; p < rcount[r + 1]; ++p) {
+                    const uint32_t c = bcol[p];
+                    if (!seen[c]) {
+                        seen[c] = 1;
+                        touched.push_back(c);
+                    }
+                    work[c] += bval[p];
+                }
+                std::sort(touched.begin(), touched.end());
+                for (uint32_t c : touched) {
+                    if (work[c] != 0.0) {
+                        cidx.push_back(c);
+                        cval.push_back(work[c]);
+                    }
+                    work[c] = 0.0;
+                    seen[c] = 0;
+                }
+                touched.clear();
+                cptr[r + 1] = cidx.size();
+            }
+        }
+        std::vector<uint32_t>().swap(bcol);
+        std::vector<double>().swap(bval);
     }
-    std::vector<uint32_t>().swap(bcol);
-    std::vector<double>().swap(bval);
     // transpose
     std::vector<uint64_t> tptr(nc + 1, 0);
     for (uint32_t c : cidx) tptr[c + 1]++;
@@ -840,13 +862,26 @@ std::shared_ptr<Precond> build_additive(const Design &d, const std::vector<doubl
     bool any_slope = false;
     for (auto &c : channels) any_slope |= d.terms[c.term].loads[c.column] >= 0;
     const auto active = find_active_levels(d);
+    // the cross-tabs of the plain (intercept) pairs on the device when a GPU is visible (the same
+    // cells as the host accumulation, bit for bit)
+    std::vector<std::pair<uint32_t, uint32_t>> plain;
+    std::vector<int64_t> plain_of(pairs.size(), -1);
+    for (size_t p = 0; p < pairs.size(); ++p)
+        if (d.terms[pairs[p].first.term].loads[pairs[p].first.column] < 0 &&
+            d.terms[pairs[p].second.term].loads[pairs[p].second.column] < 0) {
+            plain_of[p] = (int64_t)plain.size();
+            plain.push_back({pairs[p].first.term, pairs[p].second.term});
+        }
+    std::vector<CellList> cells;
+    const bool on_device = d.n_obs <= 0x7FFFFFFFull && device_cross_tabs(d, w, plain, cells);
+    if (on_device) phase("cross-tabs on the device");
     std::vector<std::vector<Component>> per_pair(pairs.size());
     std::vector<std::vector<std::string>> pair_warn(pairs.size());
     std::vector<std::string> pair_err(pairs.size());
 #pragma omp parallel for schedule(dynamic, 1)
     for (int64_t p = 0; p < (int64_t)pairs.size(); ++p)
         build_pair_components(d, w, pairs[p].first, pairs[p].second, active, cfg, per_pair[p], pair_warn[p],
-                              pair_err[p]);
+                              pair_err[p], on_device && plain_of[p] >= 0 ? &cells[plain_of[p]] : nullptr);
     phase("cross-tabs and components");
     for (size_t p = 0; p < pairs.size(); ++p) {
         if (!pair_err[p].empty()) fail(WX_ERR_BUILD, pair_err[p], "BuildError::UnscalableComponent");
@@ -933,14 +968,42 @@ std::shared_ptr<Precond> assemble_additive(std::vector<Component> &comps, uint64
             S->dense_off[c] = dense_total;
             dense_total += (uint64_t)comps[c].nk * comps[c].nk;
         }
-    S->dense.resize(dense_total);
-    // per-component factorisation (parallel)
+    S->dense.resize(dense_total); // uninitialised; pages first touched by all threads
+#pragma omp parallel for schedule(static)
+    for (int64_t q = 0; q < (int64_t)((dense_total + 4095) / 4096); ++q)
+        std::fill(S->dense.begin() + q * 4096, S->dense.begin() + std::min<uint64_t>(dense_total, (q + 1) * 4096), 0.0);
+    // per-component factorisation (parallel over components).  A dense block whose reduced
+    // block costs many adjacency products (a small kept side next to a large eliminated one) is
+    // built first, its rows in parallel; each row sums its terms in adjacency order, so the
+    // result does not depend on the split.
     std::vector<std::vector<double>> dense_mats(nc);
     std::vector<AcFactor> acs(nc);
     std::vector<int> failed(nc, 0);
+    std::vector<uint8_t> heavy(nc, 0);
+    for (uint64_t c = 0; c < nc; ++c) {
+        if (!dense[c] || big[c]) continue;
+        double work = 0.0;
+        for (uint32_t e = 0; e < comps[c].ne; ++e) {
+            const double deg = (double)(comps[c].cptr[e + 1] - comps[c].cptr[e]);
+            work += deg * deg;
+        }
+        heavy[c] = work > 4e6;
+    }
+    for (uint64_t c = 0; c < nc; ++c) {
+        if (!heavy[c]) continue;
+        const Component &cm = comps[c];
+        for (uint32_t e = 0; e < cm.ne; ++e)
+            if (!(cm.diag[e] > 0.0)) failed[c] = 1;
+        if (failed[c]) continue;
+        std::vector<double> Sm;
+        dense_schur(cm, Sm, true);
+        if (!dense_laplacian_pinv(Sm, cm.nk, S->gauge[c] == 0)) failed[c] = 2;
+        dense_mats[c] = std::move(Sm);
+    }
 #pragma omp parallel for schedule(dynamic, 1)
     for (int64_t c = 0; c < (int64_t)nc; ++c) {
         const Component &cm = comps[c];
+        if (heavy[c]) continue;
         for (uint32_t e = 0; e < cm.ne; ++e)
             if (!(cm.diag[e] > 0.0)) failed[c] = 1;
         if (failed[c] || big[c]) continue;

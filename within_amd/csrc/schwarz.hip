@@ -944,7 +944,7 @@ uint32_t grid_rows(uint64_t rows, uint32_t kshift, uint32_t cap = 65535) {
     return (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(g, cap));
 }
 
-template <typename T, typename S> std::vector<T> cast_vec(const std::vector<S> &v) {
+template <typename T, typename V> std::vector<T> cast_vec(const V &v) {
     std::vector<T> o(v.size());
 #pragma omp parallel for schedule(static) if (v.size() > (1u << 20))
     for (int64_t i = 0; i < (int64_t)v.size(); ++i) o[i] = (T)v[i];

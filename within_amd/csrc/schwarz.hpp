@@ -24,6 +24,19 @@ struct LocalSolverSettings {
     bool scaling_error = false;
 };
 
+// Allocator whose resize leaves doubles uninitialised: the dense kept blocks (hundreds of MB) are
+// written in full by their builders, so zero-filling them first would only cost time.
+template <typename T> struct NoInitAlloc : std::allocator<T> {
+    template <typename U> struct rebind { using other = NoInitAlloc<U>; };
+    NoInitAlloc() = default;
+    template <typename U> NoInitAlloc(const NoInitAlloc<U> &) {}
+    template <typename U, typename... A> void construct(U *p, A &&...a) {
+        if constexpr (sizeof...(A) == 0) ::new ((void *)p) U;
+        else ::new ((void *)p) U(std::forward<A>(a)...);
+    }
+};
+using DenseVec = std::vector<double, NoInitAlloc<double>>;
+
 // Host form of a one-level additive Schwarz preconditioner over factor-pair components.
 // Every component is stored eliminated-block first: slots [slot_start[c], +ne[c]) are the
 // eliminated levels, the next nk[c] slots the kept levels.  A component's matrix is stored
@@ -51,7 +64,7 @@ struct Precond::Schwarz {
     std::vector<double> c_val;
     // dense kept-block operators: S^+ (nk x nk row-major) at dense_off[c]
     std::vector<uint64_t> dense_off;  // n_comp
-    std::vector<double> dense;
+    DenseVec dense;
     // sparse LDL' factors: L strictly lower, CSR by rows (local indices), pivots d
     std::vector<uint64_t> l_row_off;  // n_comp: offset of the component's first row pointer
     std::vector<uint64_t> l_ptr;
@@ -103,6 +116,17 @@ struct DenseSchurArgs {
     bool signed_form;
 };
 bool gpu_available();
+
+// The cross-tab cells of a plain factor pair (r, c) in row-major order: key = level_r * L_c +
+// level_c, val = the weight sum of the cell's rows added in observation order (cells with no rows
+// are absent).  device_cross_tabs computes every listed pair's cells on the GPU, bitwise equal to
+// the host accumulation; it returns false (and computes nothing) when no GPU is visible.
+struct CellList {
+    std::vector<uint64_t> key;
+    std::vector<double> val;
+};
+bool device_cross_tabs(const Design &d, const std::vector<double> &w,
+                       const std::vector<std::pair<uint32_t, uint32_t>> &pairs, std::vector<CellList> &out);
 bool dense_schur_pinv_gpu(const DenseSchurArgs &a, bool floating, double *out);
 
 // Diagonal preconditioner (operator/schwarz.rs:185-229).

@@ -23,10 +23,10 @@ struct Writer {
         const uint8_t *p = reinterpret_cast<const uint8_t *>(&v);
         b.insert(b.end(), p, p + sizeof(T));
     }
-    template <typename T> void vec(const std::vector<T> &v) {
+    template <typename V> void vec(const V &v) {
         put<uint64_t>(v.size());
         const uint8_t *p = reinterpret_cast<const uint8_t *>(v.data());
-        b.insert(b.end(), p, p + v.size() * sizeof(T));
+        b.insert(b.end(), p, p + v.size() * sizeof(typename V::value_type));
     }
 };
 
@@ -122,7 +122,10 @@ std::shared_ptr<Precond> deserialize_precond(const uint8_t *buf, uint64_t len) {
         S->c_idx = r.vec<uint32_t>();
         S->c_val = r.vec<double>();
         S->dense_off = r.vec<uint64_t>();
-        S->dense = r.vec<double>();
+        {
+            const std::vector<double> dn = r.vec<double>();
+            S->dense.assign(dn.begin(), dn.end());
+        }
         S->l_row_off = r.vec<uint64_t>();
         S->l_ptr = r.vec<uint64_t>();
         S->l_idx = r.vec<uint32_t>();

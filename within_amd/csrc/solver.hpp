@@ -130,6 +130,20 @@ struct Solver {
     mutable std::vector<uint32_t> pool_k;
 };
 
+// Device-side index structures (setup_gpu.hip): stable argsorts by one key or by a pair of keys
+// (radix sorts of (key, row)), gathers, square roots, inverse permutations, and the pieces of a
+// fused pair (runs of equal Q code inside P's work items; piece0 receives each item's first
+// piece, piece_q each piece's Q code; returns the piece count).  Enqueued on `st`.
+void dev_sqrt(const double *src, uint64_t n, double *out, hipStream_t st);
+void dev_gather(const double *src, const uint32_t *perm, uint64_t n, double *out, hipStream_t st);
+void dev_gather_u32(const uint32_t *src, const uint32_t *perm, uint64_t n, uint32_t *out, hipStream_t st);
+void dev_invert_perm(const uint32_t *perm, uint64_t n, uint32_t *inv, hipStream_t st);
+void dev_stable_argsort(const uint32_t *key, uint64_t n, uint64_t n_levels, uint32_t *perm, hipStream_t st);
+void dev_stable_argsort2(const uint32_t *kp, const uint32_t *kq, uint64_t lp, uint64_t lq, uint64_t n, uint32_t *perm,
+                         hipStream_t st);
+uint64_t dev_pieces(const uint32_t *qpos, uint64_t n, const SegItem *items, uint32_t n_items, uint32_t *piece0,
+                    DevBuf &piece_q, hipStream_t st);
+
 // Uploads the design (codes, slopes, weights, D' segment plans) to the current device.
 void upload_design(Solver &s, const std::vector<double> &weights_internal);
 
