@@ -44,6 +44,18 @@ wx_status wx_testing_reduced_laplacian(uint32_t n_rows, uint32_t n_cols, const d
 wx_status wx_testing_design_index(const wx_design *design, uint64_t n_terms, uint64_t *n_levels, uint64_t *offset,
                                   uint8_t *sorted, int64_t *dominant, int32_t *permuted, uint32_t *obs_perm);
 
+/* The slope reparametrisation of a design (crates/within/src/solver/reparam.rs:78-276), built by
+ * the host (on_device = 0) or on the device (on_device = 1, needs a GPU), as a Solver would build
+ * it.  slopes_out (may be NULL): the whitened loading columns, n_obs x n_slope_columns, F-order,
+ * caller row order.  transform_out / mean_out (may be NULL): every slope term's per-level V x V
+ * whitening rows and V centres, terms in order.  unidentified_out (may be NULL): the
+ * (term, level, column) triples the reparametrisation could not identify, ascending; their count
+ * goes to *n_unidentified.  Stands in for the reference's crate-internal SlopeReparam fields, which
+ * its own tests read through solve results (tests/slopes.rs:19-195). */
+wx_status wx_testing_reparam(const wx_design *design, const double *weights, int32_t on_device, double *slopes_out,
+                             double *transform_out, uint64_t transform_cap, double *mean_out, uint64_t mean_cap,
+                             uint64_t *unidentified_out, uint64_t unidentified_cap, uint64_t *n_unidentified);
+
 /* wx_precond_apply with the arithmetic of a solve of the given wx_precision: WX_PRECISION_MIXED
  * applies M^{-1} as a mixed-precision solve does (FP32 coefficients and FP32 slot-space vectors,
  * FP64 accumulation), so tests can hold that operator to symmetry and definiteness bounds.

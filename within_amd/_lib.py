@@ -120,6 +120,8 @@ EXPORTS = {
     "wx_buffer_free": ([C.c_void_p], None),
     # include/within_amd_testing.h
     "wx_testing_precond_apply": ([C.c_void_p, C.c_void_p, C.c_uint64, C.c_void_p, C.c_int32], C.c_int),
+    "wx_testing_reparam": ([C.c_void_p, C.c_void_p, C.c_int32, C.c_void_p, C.c_void_p, C.c_uint64, C.c_void_p,
+                            C.c_uint64, C.c_void_p, C.c_uint64, C.c_void_p], C.c_int),
     "wx_testing_design_index": ([C.c_void_p, C.c_uint64, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
                                  C.c_void_p], C.c_int),
     "wx_testing_block_precond": ([C.c_uint32, C.c_uint32, C.c_void_p, C.c_void_p, C.c_void_p, C.POINTER(C.c_void_p)],

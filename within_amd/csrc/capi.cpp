@@ -233,7 +233,13 @@ wx_status wx_solver_new(const wx_design *design, const double *weights, const wx
         }
         s.weighted = weights != nullptr;
         if (weights) s.min_weight = *std::min_element(w.begin(), w.end());
-        if (s.design.has_slopes()) s.reparam = build_reparam(s.design, weights ? w.data() : nullptr);
+        if (s.design.has_slopes()) {
+            // the per-level moments and whitening run on the device (WITHIN_AMD_HOST_SETUP=1: host)
+            static const bool host_setup = std::getenv("WITHIN_AMD_HOST_SETUP") != nullptr;
+            s.reparam = host_setup ? build_reparam(s.design, weights ? w.data() : nullptr)
+                                   : build_reparam_device(s.design, weights ? w.data() : nullptr);
+            setup_phase("slope reparametrisation", t0);
+        }
         if (prebuilt) {
             if (prebuilt->p->n != s.design.n_dofs)
                 fail(WX_ERR_BUILD, "preconditioner dimension mismatch: expected " + std::to_string(s.design.n_dofs) +
@@ -385,6 +391,51 @@ wx_status wx_testing_design_index(const wx_design *design, uint64_t n_terms, uin
         if (dominant) *dominant = dom;
         if (permuted) *permuted = d.obs_perm.empty() ? 0 : 1;
         if (obs_perm && !d.obs_perm.empty()) std::copy(d.obs_perm.begin(), d.obs_perm.end(), obs_perm);
+    });
+}
+
+wx_status wx_testing_reparam(const wx_design *design, const double *weights, int32_t on_device, double *slopes_out,
+                             double *transform_out, uint64_t transform_cap, double *mean_out, uint64_t mean_cap,
+                             uint64_t *unidentified_out, uint64_t unidentified_cap, uint64_t *n_unidentified) {
+    return guard([&] {
+        if (!design) fail(WX_ERR_INVALID_INPUT, "null design");
+        Design d = build_design(design->n_obs, design->columns, design->structure, design->slopes, true);
+        std::vector<double> w;
+        if (weights) {
+            validate_weights(d, weights, d.n_obs);
+            w.resize(d.n_obs);
+            permute_in(d, weights, w.data());
+        }
+        if (on_device) require_device();
+        std::unique_ptr<SlopeReparam> rp;
+        if (d.has_slopes())
+            rp = on_device ? build_reparam_device(d, weights ? w.data() : nullptr)
+                           : build_reparam(d, weights ? w.data() : nullptr);
+        const uint64_t n = d.n_obs;
+        if (slopes_out) // caller row order
+            for (uint64_t c = 0; c < d.slopes.size() / std::max<uint64_t>(n, 1); ++c)
+                for (uint64_t i = 0; i < n; ++i)
+                    slopes_out[c * n + (d.obs_perm.empty() ? i : d.obs_perm[i])] = d.slopes[c * n + i];
+        uint64_t nt = 0, nm = 0;
+        if (rp)
+            for (const auto &blk : rp->blocks) {
+                if (transform_out) {
+                    if (nt + blk.transform.size() > transform_cap) fail(WX_ERR_INVALID_INPUT, "transform buffer too small");
+                    std::copy(blk.transform.begin(), blk.transform.end(), transform_out + nt);
+                }
+                if (mean_out) {
+                    if (nm + blk.mean.size() > mean_cap) fail(WX_ERR_INVALID_INPUT, "mean buffer too small");
+                    std::copy(blk.mean.begin(), blk.mean.end(), mean_out + nm);
+                }
+                nt += blk.transform.size();
+                nm += blk.mean.size();
+            }
+        const uint64_t nu = rp ? rp->unidentified.size() / 3 : 0;
+        if (n_unidentified) *n_unidentified = nu;
+        if (unidentified_out) {
+            if (nu * 3 > unidentified_cap) fail(WX_ERR_INVALID_INPUT, "unidentified buffer too small");
+            std::copy(rp->unidentified.begin(), rp->unidentified.end(), unidentified_out);
+        }
     });
 }
 

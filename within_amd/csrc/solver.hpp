@@ -134,6 +134,9 @@ struct Solver {
 // (radix sorts of (key, row)), gathers, square roots, inverse permutations, and the pieces of a
 // fused pair (runs of equal Q code inside P's work items; piece0 receives each item's first
 // piece, piece_q each piece's Q code; returns the piece count).  Enqueued on `st`.
+// Slope reparametrisation with the per-level moments and whitening on the device; the same
+// contract as build_reparam.
+std::unique_ptr<SlopeReparam> build_reparam_device(Design &d, const double *weights);
 void dev_sqrt(const double *src, uint64_t n, double *out, hipStream_t st);
 void dev_gather(const double *src, const uint32_t *perm, uint64_t n, double *out, hipStream_t st);
 void dev_gather_u32(const uint32_t *src, const uint32_t *perm, uint64_t n, uint32_t *out, hipStream_t st);
