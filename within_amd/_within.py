@@ -13,6 +13,7 @@ per-level reparametrised slope columns the solver works in) / ``stop_reason`` on
 from __future__ import annotations
 
 import ctypes as C
+import struct
 import sys
 import time
 import warnings
@@ -501,6 +502,8 @@ class Preconditioner:
         if not isinstance(data, (bytes, bytearray, memoryview)):
             raise TypeError("Preconditioner(data) expects bytes")
         b = bytes(data)
+        if b[:4] != b"WXPC":
+            b = _from_reference_payload(b)
         out = C.c_void_p()
         buf = (C.c_uint8 * max(1, len(b))).from_buffer_copy(b if b else b"\0")
         st = _lib.lib().wx_precond_deserialize(buf, len(b), C.byref(out))
@@ -551,6 +554,29 @@ class Preconditioner:
 
     def __repr__(self) -> str:
         return f"Preconditioner({_lib.lib().wx_precond_variant(self._h).decode()}, n={self.nrows})"
+
+
+def _from_reference_payload(b: bytes) -> bytes:
+    """Bytes pickled by the reference (postcard wire v13): a Diagonal preconditioner converts to
+    this library's format; an additive Schwarz one is decoded and reported, since its local
+    factors are in approx-chol's own format.  Other bytes pass through unchanged (the native
+    reader reports them)."""
+    from . import wire_v13
+
+    try:
+        ref = wire_v13.decode(b)
+    except wire_v13.WireError as e:
+        if b[:1] in (b"\x00", b"\x01") and len(b) > 1:
+            raise ValueError(f"failed to deserialize preconditioner: not a valid v13 payload ({e})") from None
+        return b
+    if ref.variant == "Diagonal":
+        inv = np.ascontiguousarray(ref.inv_diag, "<f8")
+        return b"WXPC" + struct.pack("<IIQQ", 2, 0, len(inv), len(inv)) + inv.tobytes()
+    raise ValueError(
+        f"a reference additive Schwarz preconditioner (wire v13: {len(ref.subdomains)} subdomains, "
+        f"n_dofs={ref.n_dofs}) stores its local factors in approx-chol's format, which this library "
+        "does not read; rebuild it with Solver(design, preconditioner=PreconditionerConfig.Additive) "
+        "or build_preconditioner(design)")
 
 
 def build_preconditioner(design, weights: Optional[np.ndarray] = None, preconditioner=None) -> Optional[Preconditioner]:
