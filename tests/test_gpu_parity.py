@@ -688,7 +688,7 @@ print("converged", r.converged, r.iterations)
     assert rel(np.load("gpurun_out/_tails_y.npy"), sch.apply(x)) <= 1e-11
 
 
-@pytest.mark.parametrize("case", ["weighted_3f", "unweighted_2f_gaps", "zero_weights"])
+@pytest.mark.parametrize("case", ["weighted_3f", "unweighted_2f_gaps", "zero_weights", "large_disconnected"])
 def test_device_cross_tabs_give_the_host_preconditioner(gpu, case):
     """The plain pairs' cross-tabs built on the device (stable sort by cell, each cell's weights
     added in observation order) make the preconditioner byte for byte equal to the host-built one
@@ -703,10 +703,21 @@ def test_device_cross_tabs_give_the_host_preconditioner(gpu, case):
     elif case == "unweighted_2f_gaps":
         codes = np.column_stack([rng.choice(np.r_[0:3000:3], n), rng.integers(0, 120, n)])
         w = None
-    else:
+    elif case == "zero_weights":
         codes = np.column_stack([rng.integers(0, 1500, n), rng.integers(0, 200, n), rng.integers(0, 4, n)])
         w = rng.uniform(0.5, 2.0, n)
         w[rng.uniform(size=n) < 0.1] = 0.0
+    else:
+        # 1.5M rows in two disconnected blocks of levels plus isolated zero-weight levels: enough
+        # cells for the parallel transposes and union-find, several components per pair
+        n = 1_500_000
+        half = rng.uniform(size=n) < 0.6
+        a = np.where(half, rng.integers(0, 100_000, n), rng.integers(100_000, 150_000, n))
+        b = np.where(half, rng.integers(0, 2_500, n), rng.integers(2_500, 4_000, n))
+        c = np.where(half, rng.integers(0, 6, n), rng.integers(6, 9, n))
+        codes = np.column_stack([a, b, c])
+        w = rng.uniform(0.5, 2.0, n)
+        w[a % 997 == 5] = 0.0
     codes = np.asfortranarray(codes.astype(np.uint32))
     for ls in (wa.config.LocalSolverConfig(), wa.config.LocalSolverConfig(dense_threshold=0, dense_policy="reference")):
         cfg = wa.config.AdditiveSchwarz(ls)

@@ -61,9 +61,9 @@ struct Design {
 
 // Builds a design from per-term code columns (caller order) and term structure.
 // Restates Design::build (crates/within/src/domain.rs:187-247).
-Design build_design(uint64_t n_obs, std::vector<std::vector<uint32_t>> columns,
-                    std::vector<std::vector<int64_t>> structure, std::vector<std::vector<double>> slopes,
-                    bool locality_sort = true);
+Design build_design(uint64_t n_obs, const std::vector<std::vector<uint32_t>> &columns,
+                    const std::vector<std::vector<int64_t>> &structure,
+                    const std::vector<std::vector<double>> &slopes, bool locality_sort = true);
 
 // Stable ascending argsort by level code (crates/within/src/domain.rs:110-140).
 std::vector<uint32_t> stable_argsort(const uint32_t *key, uint64_t n, uint64_t n_levels);

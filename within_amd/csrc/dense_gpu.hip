@@ -155,13 +155,33 @@ bool gpu_available() {
     return hipGetDeviceCount(&n) == hipSuccess && n > 0;
 }
 
-bool dense_schur_pinv_gpu(const DenseSchurArgs &a, bool floating, double *out) {
+namespace {
+std::shared_ptr<double> adopt(DevBuf &b) {
+    double *p = b.as<double>();
+    b.p = nullptr;
+    b.bytes = 0;
+    return std::shared_ptr<double>(p, [](double *q) { (void)hipFree(q); });
+}
+} // namespace
+
+void Precond::Schwarz::host_dense() const {
+    std::lock_guard<std::mutex> lk(*dense_mu);
+    if (dense_on_host) return;
+    auto *self = const_cast<Precond::Schwarz *>(this);
+    for (const DeviceBlock &b : dense_dev)
+        WX_HIP(hipMemcpy(self->dense.data() + b.off, b.dev.get(), b.count * 8, hipMemcpyDeviceToHost));
+    self->dense_on_host = true;
+}
+
+bool dense_schur_pinv_gpu(const DenseSchurArgs &a, bool floating, std::shared_ptr<double> &out) {
     const uint32_t n = a.nk;
+    DevBuf cptr, cidx, cval, diag, sur, rabs, tp, te, tv, S, mean, info;
     if (floating && n <= 1) {
-        std::fill(out, out + (size_t)n * n, 0.0);
+        S.alloc((size_t)n * n * 8);
+        WX_HIP(hipMemset(S.p, 0, (size_t)n * n * 8));
+        out = adopt(S);
         return true;
     }
-    DevBuf cptr, cidx, cval, diag, sur, rabs, tp, te, tv, S, mean, info;
     upload_vec(cptr, a.cptr, (size_t)a.ne + 1);
     upload_vec(cidx, a.cidx, a.nnz);
     upload_vec(cval, a.cval, a.nnz);
@@ -203,7 +223,8 @@ bool dense_schur_pinv_gpu(const DenseSchurArgs &a, bool floating, double *out) {
         hipLaunchKernelGGL(k_project, dim3(eg), dim3(DB), 0, 0, s, n, (uint64_t)n, mean.as<double>());
     }
     WX_HIP(hipGetLastError());
-    WX_HIP(hipMemcpy(out, s, nn * 8, hipMemcpyDeviceToHost));
+    WX_HIP(hipDeviceSynchronize());
+    out = adopt(S);
     return true;
 }
 

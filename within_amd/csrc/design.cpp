@@ -5,6 +5,8 @@
 #include <cstring>
 #include <numeric>
 
+#include <omp.h>
+
 #include "core.hpp"
 
 namespace wx {
@@ -18,17 +20,35 @@ std::vector<uint32_t> stable_argsort(const uint32_t *key, uint64_t n, uint64_t n
         std::stable_sort(perm.begin(), perm.end(), [&](uint32_t a, uint32_t b) { return key[a] < key[b]; });
         return perm;
     }
-    std::vector<uint64_t> cursor(n_levels + 1, 0);
-    for (uint64_t i = 0; i < n; ++i) cursor[key[i] + 1]++;
-    for (uint64_t l = 1; l <= n_levels; ++l) cursor[l] += cursor[l - 1];
-    for (uint64_t i = 0; i < n; ++i) perm[cursor[key[i]]++] = (uint32_t)i;
+    // The counting sort over contiguous row blocks, one per thread: block b's rows of level l
+    // go after level l's rows of blocks < b, so the order is the serial one.
+    const int nb = n >= (1u << 20) ? std::max(1, std::min(omp_get_max_threads(), 32)) : 1;
+    std::vector<std::vector<uint64_t>> cnt(nb, std::vector<uint64_t>(n_levels, 0));
+    auto lo = [&](int b) { return n * (uint64_t)b / (uint64_t)nb; };
+#pragma omp parallel for schedule(static, 1) num_threads(nb)
+    for (int b = 0; b < nb; ++b) {
+        uint64_t *c = cnt[b].data();
+        for (uint64_t i = lo(b); i < lo(b + 1); ++i) c[key[i]]++;
+    }
+    uint64_t run = 0;
+    for (uint64_t l = 0; l < n_levels; ++l)
+        for (int b = 0; b < nb; ++b) {
+            const uint64_t c = cnt[b][l];
+            cnt[b][l] = run;
+            run += c;
+        }
+#pragma omp parallel for schedule(static, 1) num_threads(nb)
+    for (int b = 0; b < nb; ++b) {
+        uint64_t *c = cnt[b].data();
+        for (uint64_t i = lo(b); i < lo(b + 1); ++i) perm[c[key[i]]++] = (uint32_t)i;
+    }
     return perm;
 }
 
 
-Design build_design(uint64_t n_obs, std::vector<std::vector<uint32_t>> columns,
-                    std::vector<std::vector<int64_t>> structure, std::vector<std::vector<double>> slopes,
-                    bool locality_sort) {
+Design build_design(uint64_t n_obs, const std::vector<std::vector<uint32_t>> &columns,
+                    const std::vector<std::vector<int64_t>> &structure,
+                    const std::vector<std::vector<double>> &slopes, bool locality_sort) {
     // crates/within/src/domain.rs:187-247.
     if (n_obs == 0) fail(WX_ERR_BUILD, "no observations provided", "BuildError::EmptyObservations");
     const uint64_t T = columns.size();
@@ -48,15 +68,15 @@ Design build_design(uint64_t n_obs, std::vector<std::vector<uint32_t>> columns,
     d.terms.resize(T);
     uint64_t offset = 0;
     for (uint64_t t = 0; t < T; ++t) {
-        const auto &col = columns[t];
-        uint32_t mx = 0, prev = 0;
-        bool sorted = true;
-        for (uint64_t i = 0; i < n_obs; ++i) {
-            uint32_t v = col[i];
-            mx = std::max(mx, v);
-            sorted &= v >= prev;
-            prev = v;
+        const uint32_t *col = columns[t].data();
+        uint32_t mx = 0;
+        int unsorted = 0;
+#pragma omp parallel for schedule(static) reduction(max : mx) reduction(| : unsorted) if (n_obs >= (1u << 20))
+        for (int64_t i = 0; i < (int64_t)n_obs; ++i) {
+            mx = std::max(mx, col[i]);
+            unsorted |= i > 0 && col[i] < col[i - 1];
         }
+        const bool sorted = !unsorted;
         Term &tm = d.terms[t];
         tm.n_levels = (uint64_t)mx + 1;
         tm.offset = offset;
@@ -85,17 +105,22 @@ Design build_design(uint64_t n_obs, std::vector<std::vector<uint32_t>> columns,
         // term, so obs_perm, the permuted columns and the sorted flags are the reference's
         d.obs_perm = stable_argsort(columns[dom].data(), n_obs, d.terms[dom].n_levels);
         const uint32_t *perm = d.obs_perm.data();
+        const bool par = n_obs >= (1u << 20);
         for (uint64_t t = 0; t < T; ++t) {
             uint32_t *dst = d.codes.data() + t * n_obs;
             const uint32_t *src = columns[t].data();
-            for (uint64_t i = 0; i < n_obs; ++i) dst[i] = src[perm[i]];
-            bool s = true;
-            for (uint64_t i = 1; i < n_obs; ++i) s &= dst[i] >= dst[i - 1];
-            d.terms[t].sorted = s; // nested factors come out sorted (domain.rs:231-234)
+            int unsorted = 0;
+#pragma omp parallel for schedule(static) if (par)
+            for (int64_t i = 0; i < (int64_t)n_obs; ++i) dst[i] = src[perm[i]];
+#pragma omp parallel for schedule(static) reduction(| : unsorted) if (par)
+            for (int64_t i = 1; i < (int64_t)n_obs; ++i) unsorted |= dst[i] < dst[i - 1];
+            d.terms[t].sorted = !unsorted; // nested factors come out sorted (domain.rs:231-234)
         }
         for (uint64_t s = 0; s < d.n_slopes; ++s) {
             double *dst = d.slopes.data() + s * n_obs;
-            for (uint64_t i = 0; i < n_obs; ++i) dst[i] = slopes[s][perm[i]];
+            const double *src = slopes[s].data();
+#pragma omp parallel for schedule(static) if (par)
+            for (int64_t i = 0; i < (int64_t)n_obs; ++i) dst[i] = src[perm[i]];
         }
     } else {
         for (uint64_t t = 0; t < T; ++t) std::memcpy(d.codes.data() + t * n_obs, columns[t].data(), n_obs * 4);
@@ -110,7 +135,8 @@ void permute_in(const Design &d, const double *src, double *dst) {
         std::memcpy(dst, src, d.n_obs * sizeof(double));
         return;
     }
-    for (uint64_t k = 0; k < d.n_obs; ++k) dst[k] = src[d.obs_perm[k]];
+#pragma omp parallel for schedule(static) if (d.n_obs >= (1u << 20))
+    for (int64_t k = 0; k < (int64_t)d.n_obs; ++k) dst[k] = src[d.obs_perm[k]];
 }
 
 void validate_weights(const Design &d, const double *w, uint64_t len) {
@@ -118,7 +144,11 @@ void validate_weights(const Design &d, const double *w, uint64_t len) {
     if (len != d.n_obs)
         fail(WX_ERR_BUILD, "weights has length " + std::to_string(len) + ", expected " + std::to_string(d.n_obs),
              "BuildError::WeightCountMismatch");
-    for (uint64_t i = 0; i < len; ++i)
+    int bad = 0;
+#pragma omp parallel for schedule(static) reduction(| : bad) if (len >= (1u << 20))
+    for (int64_t i = 0; i < (int64_t)len; ++i) bad |= !(w[i] >= 0.0 && std::isfinite(w[i]));
+    if (!bad) return;
+    for (uint64_t i = 0; i < len; ++i) // the first offending weight
         if (!(w[i] >= 0.0 && std::isfinite(w[i])))
             fail(WX_ERR_BUILD, "weight at index " + std::to_string(i) + " must be finite and non-negative, got " +
                                    std::to_string(w[i]),

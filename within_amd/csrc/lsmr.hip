@@ -1426,6 +1426,19 @@ uint64_t build_seg_plan(DevTerm &dt, const std::vector<uint64_t> &start, uint64_
     return slots;
 }
 
+// a row's level codes as one 16-byte record (terms beyond T read as 0)
+__global__ __launch_bounds__(BLK) void k_codes4(uint64_t n, uint32_t T, const uint32_t *__restrict__ codes,
+                                                uint4 *__restrict__ out) {
+    for (uint64_t i = (uint64_t)blockIdx.x * BLK + threadIdx.x; i < n; i += (uint64_t)gridDim.x * BLK) {
+        uint4 r;
+        r.x = codes[i];
+        r.y = T > 1 ? codes[n + i] : 0u;
+        r.z = T > 2 ? codes[2 * n + i] : 0u;
+        r.w = T > 3 ? codes[3 * n + i] : 0u;
+        out[i] = r;
+    }
+}
+
 // The fused D' of two unsorted single-intercept terms of a plain design (k_segreduce2_pair):
 // the primary term P (the unsorted term with the most levels) is reduced over its rows in
 // (P, Q) order, Q being the unsorted term with the fewest levels.  A run of equal Q codes
@@ -1609,12 +1622,12 @@ void upload_design(Solver &s, const std::vector<double> &weights_internal) {
     // one load instead of one per term
     dv.codes4.release();
     if (dv.plain && d.terms.size() <= 4 && !d.terms.empty()) {
-        std::vector<uint32_t> c4(d.n_obs * 4, 0u);
-        for (size_t t = 0; t < d.terms.size(); ++t) {
-            const uint32_t *col = d.level_column(t);
-            for (uint64_t i = 0; i < d.n_obs; ++i) c4[i * 4 + t] = col[i];
-        }
-        dv.codes4.upload(c4.data(), c4.size());
+        dv.codes4.alloc(n * 16);
+        const uint32_t g = (uint32_t)std::min<uint64_t>((n + BLK - 1) / BLK, 1u << 20);
+        hipLaunchKernelGGL(k_codes4, dim3(g), dim3(BLK), 0, st0, n, (uint32_t)d.terms.size(),
+                           dv.codes.as<uint32_t>(), dv.codes4.as<uint4>());
+        WX_HIP(hipGetLastError());
+        WX_HIP(hipStreamSynchronize(st0));
     }
     // the sorted single-intercept term with the most work items gets its D' fused into the gather
     dv.fuse_term = -1;

@@ -76,6 +76,7 @@ std::shared_ptr<Precond> merge_schwarz(const std::vector<const Solver *> &ms, co
         S->inv_de.insert(S->inv_de.end(), A.inv_de.begin(), A.inv_de.end());
         S->c_idx.insert(S->c_idx.end(), A.c_idx.begin(), A.c_idx.end());
         S->c_val.insert(S->c_val.end(), A.c_val.begin(), A.c_val.end());
+        A.host_dense();
         S->dense.insert(S->dense.end(), A.dense.begin(), A.dense.end());
         for (uint64_t v : A.l_ptr) S->l_ptr.push_back(lidx_base + v);
         S->l_idx.insert(S->l_idx.end(), A.l_idx.begin(), A.l_idx.end());

@@ -11,6 +11,7 @@
 //     this library's own implementation of the published algorithm, as a unit-lower LDL'
 //     factor in elimination order, ready for level-scheduled triangular solves.
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <functional>
 #include <cstdio>
@@ -20,11 +21,9 @@
 #include <queue>
 #include <string>
 #include <tuple>
-#ifdef _OPENMP
 #include <chrono>
 
 #include <omp.h>
-#endif
 
 #include "schwarz.hpp"
 
@@ -323,6 +322,49 @@ struct Channel {
     uint32_t term, column;
 };
 
+// Stable CSR transpose: output row c lists the input rows holding column c in ascending order
+// (with their values when `val` is given) -- the serial counting transpose's result, formed over
+// contiguous blocks of input rows, one per thread, each placing its entries after the earlier
+// blocks' entries of the same column.
+void par_transpose(uint64_t nrows, uint64_t ncols, const uint64_t *ptr, const uint32_t *idx, const double *val,
+                   std::vector<uint64_t> &tptr, std::vector<uint32_t> &tidx, std::vector<double> *tval) {
+    const uint64_t nnz = ptr[nrows];
+    tptr.assign(ncols + 1, 0);
+    tidx.resize(nnz);
+    if (tval) tval->resize(nnz);
+    int nb = nnz >= (1u << 18) ? std::max(1, std::min(omp_get_max_threads(), 32)) : 1;
+    if ((uint64_t)nb * ncols > 8 * nnz + 4096) nb = 1; // counters would outweigh the entries
+    std::vector<uint64_t> rb(nb + 1, nrows); // row blocks of about equal entry counts
+    rb[0] = 0;
+    for (int b = 1; b < nb; ++b)
+        rb[b] = (uint64_t)(std::upper_bound(ptr, ptr + nrows + 1, nnz * (uint64_t)b / nb) - ptr) - 1;
+    for (int b = 1; b <= nb; ++b) rb[b] = std::max(rb[b], rb[b - 1]);
+    std::vector<std::vector<uint64_t>> cnt(nb, std::vector<uint64_t>(ncols, 0));
+#pragma omp parallel for schedule(static, 1) num_threads(nb)
+    for (int b = 0; b < nb; ++b)
+        for (uint64_t p = ptr[rb[b]]; p < ptr[rb[b + 1]]; ++p) cnt[b][idx[p]]++;
+    uint64_t run = 0;
+    for (uint64_t c = 0; c < ncols; ++c) {
+        tptr[c] = run;
+        for (int b = 0; b < nb; ++b) {
+            const uint64_t k = cnt[b][c];
+            cnt[b][c] = run;
+            run += k;
+        }
+    }
+    tptr[ncols] = run;
+#pragma omp parallel for schedule(static, 1) num_threads(nb)
+    for (int b = 0; b < nb; ++b) {
+        uint64_t *cur = cnt[b].data();
+        for (uint64_t r = rb[b]; r < rb[b + 1]; ++r)
+            for (uint64_t p = ptr[r]; p < ptr[r + 1]; ++p) {
+                const uint64_t q = cur[idx[p]]++;
+                tidx[q] = (uint32_t)r;
+                if (tval) (*tval)[q] = val[p];
+            }
+    }
+}
+
 struct Component {
     // eliminated-major after orientation, in SDDM coordinates
     std::vector<uint32_t> glob;  // ne + nk
@@ -344,16 +386,8 @@ struct Adjacency {
     std::vector<uint64_t> tp;
     std::vector<uint32_t> te;
     std::vector<double> tv;
-    explicit Adjacency(const Component &cm) : tp(cm.nk + 1, 0), te(cm.cidx.size()), tv(cm.cidx.size()) {
-        for (uint32_t k : cm.cidx) tp[k + 1]++;
-        for (uint32_t k = 0; k < cm.nk; ++k) tp[k + 1] += tp[k];
-        std::vector<uint64_t> cur(tp.begin(), tp.end() - 1);
-        for (uint32_t e = 0; e < cm.ne; ++e)
-            for (uint64_t p = cm.cptr[e]; p < cm.cptr[e + 1]; ++p) {
-                te[cur[cm.cidx[p]]] = e;
-                tv[cur[cm.cidx[p]]] = cm.cval[p];
-                cur[cm.cidx[p]]++;
-            }
+    explicit Adjacency(const Component &cm) {
+        par_transpose(cm.ne, cm.nk, cm.cptr.data(), cm.cidx.data(), cm.cval.data(), tp, te, &tv);
     }
     template <typename F> void each(const Component &cm, uint32_t i, F &&fn) const {
         if (i < cm.ne) {
@@ -495,6 +529,215 @@ std::string pair_name(Channel a, Channel b) {
 // as a message in `error` rather than thrown.
 // `cells`, when given (a plain pair), holds the cross-tab computed on the device, bitwise equal to
 // the host accumulation below.
+// Connected components of a bipartite graph (rows 0..nr-1, columns nr..nr+nc-1, edges from a
+// CSR) by a lock-free union-find: every edge hooks the larger of its two roots under the smaller,
+// so each component's root is its smallest node.  label[v] = that root.
+std::vector<uint32_t> bipartite_roots(uint32_t nr, uint32_t nc, const std::vector<uint64_t> &cptr,
+                                      const std::vector<uint32_t> &cidx) {
+    const uint32_t nl = nr + nc;
+    std::vector<std::atomic<uint32_t>> parent(nl);
+#pragma omp parallel for schedule(static) if (nl >= (1u << 16))
+    for (int64_t v = 0; v < (int64_t)nl; ++v) parent[v].store((uint32_t)v, std::memory_order_relaxed);
+    auto find = [&](uint32_t x) {
+        for (;;) {
+            const uint32_t p = parent[x].load(std::memory_order_relaxed);
+            if (p == x) return x;
+            const uint32_t g = parent[p].load(std::memory_order_relaxed);
+            if (g != p) { // path halving: point x at its grandparent (still an ancestor)
+                uint32_t seen = p;
+                parent[x].compare_exchange_weak(seen, g, std::memory_order_relaxed);
+            }
+            x = g;
+        }
+    };
+#pragma omp parallel for schedule(dynamic, 1024) if (cidx.size() >= (1u << 16))
+    for (int64_t r = 0; r < (int64_t)nr; ++r)
+        for (uint64_t q = cptr[r]; q < cptr[r + 1]; ++q) {
+            uint32_t a = find((uint32_t)r), b = find(nr + cidx[q]);
+            while (a != b) {
+                if (a > b) std::swap(a, b); // hook b (the larger root) under a
+                uint32_t expect = b;
+                if (parent[b].compare_exchange_strong(expect, a, std::memory_order_relaxed)) break;
+                a = find(a);
+                b = find(b);
+            }
+        }
+    std::vector<uint32_t> label(nl);
+#pragma omp parallel for schedule(static) if (nl >= (1u << 16))
+    for (int64_t v = 0; v < (int64_t)nl; ++v) label[v] = find((uint32_t)v);
+    return label;
+}
+
+// A plain pair's components from its device cross-tab cells: the same components, in the same
+// order and with the same bits as the serial path below (components ordered by their smallest
+// node, each row's entries in column order, every kept level's sum over rows in row order), with
+// the cross block, its transpose, the components and each component's extraction in parallel.
+void plain_pair_components_from_cells(const Design &d, Channel rc, Channel cc,
+                                      const std::vector<std::vector<uint8_t>> &active, const CellList &cells,
+                                      std::vector<Component> &out, std::string &error) {
+    const Term &tr = d.terms[rc.term], &tc = d.terms[cc.term];
+    std::vector<uint32_t> rmap(tr.n_levels, UINT32_MAX), cmap(tc.n_levels, UINT32_MAX);
+    uint32_t nr = 0, nc = 0;
+    for (uint64_t l = 0; l < tr.n_levels; ++l)
+        if (active[rc.term][l]) rmap[l] = nr++;
+    for (uint64_t l = 0; l < tc.n_levels; ++l)
+        if (active[cc.term][l]) cmap[l] = nc++;
+    if (nr == 0 || nc == 0) return;
+    std::vector<uint32_t> l2g;
+    l2g.reserve(nr + nc);
+    const uint64_t base_r = tr.offset + rc.column * tr.n_levels, base_c = tc.offset + cc.column * tc.n_levels;
+    for (uint64_t l = 0; l < tr.n_levels; ++l)
+        if (active[rc.term][l]) l2g.push_back((uint32_t)(base_r + l));
+    for (uint64_t l = 0; l < tc.n_levels; ++l)
+        if (active[cc.term][l]) l2g.push_back((uint32_t)(base_c + l));
+    // the cross block C (rows = compact row levels, nonzero cells only) from the key-sorted cells
+    const uint64_t ncell = cells.key.size(), lcn = tc.n_levels;
+    const bool par = ncell >= (1u << 16);
+    const int nt = par ? std::max(1, std::min(omp_get_max_threads(), 32)) : 1;
+    std::vector<uint64_t> cptr(nr + 1, 0), chunk_nz(nt + 1, 0);
+    auto clo = [&](int b) { return ncell * (uint64_t)b / (uint64_t)nt; };
+#pragma omp parallel for schedule(static, 1) num_threads(nt)
+    for (int b = 0; b < nt; ++b) {
+        uint64_t z = 0;
+        for (uint64_t q = clo(b); q < clo(b + 1); ++q) z += cells.val[q] != 0.0;
+        chunk_nz[b + 1] = z;
+    }
+    for (int b = 0; b < nt; ++b) chunk_nz[b + 1] += chunk_nz[b];
+    std::vector<uint32_t> cidx(chunk_nz[nt]), crow(chunk_nz[nt]);
+    std::vector<double> cval(chunk_nz[nt]);
+#pragma omp parallel for schedule(static, 1) num_threads(nt)
+    for (int b = 0; b < nt; ++b) {
+        uint64_t o = chunk_nz[b];
+        for (uint64_t q = clo(b); q < clo(b + 1); ++q) {
+            const double v = cells.val[q];
+            if (v == 0.0) continue;
+            crow[o] = rmap[cells.key[q] / lcn];
+            cidx[o] = cmap[cells.key[q] % lcn];
+            cval[o] = v;
+            ++o;
+        }
+    }
+    for (uint64_t q = 0; q < crow.size(); ++q) // rows ascending: count, then prefix
+        cptr[crow[q] + 1]++;
+    std::vector<uint32_t>().swap(crow);
+    for (uint32_t r = 0; r < nr; ++r) cptr[r + 1] += cptr[r];
+    // weight sums per level (zero-valued cells add nothing), for the dead-component test
+    std::vector<double> rdiag(nr, 0.0), cdiag(nc, 0.0);
+    std::vector<uint64_t> tptr;
+    std::vector<uint32_t> tidx;
+    std::vector<double> tval;
+    par_transpose(nr, nc, cptr.data(), cidx.data(), cval.data(), tptr, tidx, &tval);
+#pragma omp parallel for schedule(static) if (par)
+    for (int64_t r = 0; r < (int64_t)nr; ++r) {
+        double a = 0.0;
+        for (uint64_t q = cptr[r]; q < cptr[r + 1]; ++q) a += cval[q];
+        rdiag[r] = a;
+    }
+#pragma omp parallel for schedule(static) if (par)
+    for (int64_t c = 0; c < (int64_t)nc; ++c) {
+        double a = 0.0;
+        for (uint64_t q = tptr[c]; q < tptr[c + 1]; ++q) a += tval[q];
+        cdiag[c] = a;
+    }
+    // components, ordered by their smallest node; members ascending
+    const std::vector<uint32_t> root = bipartite_roots(nr, nc, cptr, cidx);
+    const uint32_t nl = nr + nc;
+    std::vector<uint32_t> comp_of(nl, UINT32_MAX), first;
+    for (uint32_t v = 0; v < nl; ++v)
+        if (root[v] == v) {
+            comp_of[v] = (uint32_t)first.size();
+            first.push_back(v);
+        }
+    const uint32_t ncomp = (uint32_t)first.size();
+    std::vector<uint64_t> mptr(ncomp + 1, 0);
+    for (uint32_t v = 0; v < nl; ++v) mptr[comp_of[root[v]] + 1]++;
+    for (uint32_t c = 0; c < ncomp; ++c) mptr[c + 1] += mptr[c];
+    std::vector<uint32_t> members(nl);
+    {
+        std::vector<uint64_t> cur(mptr.begin(), mptr.end() - 1);
+        for (uint32_t v = 0; v < nl; ++v) members[cur[comp_of[root[v]]]++] = v;
+    }
+    std::vector<uint32_t> kmap(std::max(nr, nc), UINT32_MAX); // kept level -> local index (shared, reset per component)
+    for (uint32_t ci = 0; ci < ncomp; ++ci) {
+        const uint32_t *mb = members.data() + mptr[ci], *me = members.data() + mptr[ci + 1];
+        const uint32_t *split = std::lower_bound(mb, me, nr); // rows, then columns (as nr + c)
+        const uint64_t n_rows = split - mb, n_cols = me - split;
+        bool dead = true;
+        for (const uint32_t *v = mb; v < split; ++v) dead &= rdiag[*v] == 0.0;
+        for (const uint32_t *v = split; v < me; ++v) dead &= cdiag[*v - nr] == 0.0;
+        if (dead) continue;
+        // orientation: eliminate the larger block; swap iff n_cols > n_rows (sddm.rs:86-106)
+        const bool swap = n_cols > n_rows;
+        std::vector<uint32_t> E, K;
+        for (const uint32_t *v = mb; v < split; ++v) (swap ? K : E).push_back(*v);
+        for (const uint32_t *v = split; v < me; ++v) (swap ? E : K).push_back(*v - nr);
+        Component comp;
+        comp.ne = (uint32_t)E.size();
+        comp.nk = (uint32_t)K.size();
+        comp.glob.reserve(E.size() + K.size());
+        for (uint32_t e : E) comp.glob.push_back(swap ? l2g[nr + e] : l2g[e]);
+        for (uint32_t k : K) comp.glob.push_back(swap ? l2g[k] : l2g[nr + k]);
+        for (uint32_t i = 0; i < (uint32_t)K.size(); ++i) kmap[K[i]] = i;
+        // an eliminated node's entries: its C row (no swap) or its C' row (swap), kept-ascending
+        const uint64_t *eptr = swap ? tptr.data() : cptr.data();
+        const uint32_t *eidx = swap ? tidx.data() : cidx.data();
+        const double *evalp = swap ? tval.data() : cval.data();
+        const uint64_t *kptr = swap ? cptr.data() : tptr.data();
+        const double *kval = swap ? cval.data() : tval.data();
+        comp.cptr.assign(E.size() + 1, 0);
+        for (size_t ei = 0; ei < E.size(); ++ei) comp.cptr[ei + 1] = comp.cptr[ei] + (eptr[E[ei] + 1] - eptr[E[ei]]);
+        const uint64_t nnz = comp.cptr[E.size()];
+        comp.cidx.resize(nnz);
+        comp.cval.resize(nnz);
+        std::vector<double> rowsum_e(E.size(), 0.0), rowsum_k(K.size(), 0.0);
+        const bool cpar = nnz >= (1u << 16);
+        (void)cpar;
+#pragma omp parallel for schedule(dynamic, 256) if (cpar)
+        for (int64_t ei = 0; ei < (int64_t)E.size(); ++ei) {
+            uint64_t o = comp.cptr[ei];
+            double a = 0.0;
+            for (uint64_t q = eptr[E[ei]]; q < eptr[E[ei] + 1]; ++q, ++o) {
+                comp.cidx[o] = kmap[eidx[q]];
+                comp.cval[o] = evalp[q];
+                a += evalp[q];
+            }
+            rowsum_e[ei] = a;
+        }
+        // a kept level's sum over the eliminated levels in ascending order
+#pragma omp parallel for schedule(dynamic, 64) if (cpar)
+        for (int64_t ki = 0; ki < (int64_t)K.size(); ++ki) {
+            double a = 0.0;
+            for (uint64_t q = kptr[K[ki]]; q < kptr[K[ki] + 1]; ++q) a += kval[q];
+            rowsum_k[ki] = a;
+        }
+        for (uint32_t k : K) kmap[k] = UINT32_MAX;
+        // validate the Laplacian claim and adopt the row sums (sddm.rs:173-197)
+        double total_diag = 0.0, mismatch = 0.0;
+        for (size_t ei = 0; ei < E.size(); ++ei) {
+            const double dg = swap ? cdiag[E[ei]] : rdiag[E[ei]];
+            total_diag += dg;
+            mismatch += std::fabs(dg - rowsum_e[ei]);
+        }
+        for (size_t ki = 0; ki < K.size(); ++ki) {
+            const double dg = swap ? rdiag[K[ki]] : cdiag[K[ki]];
+            total_diag += dg;
+            mismatch += std::fabs(dg - rowsum_k[ki]);
+        }
+        const double tol =
+            64.0 * 2.220446049250313e-16 * std::sqrt((double)std::max<size_t>(1, E.size() + K.size())) * total_diag;
+        if (mismatch > tol) {
+            error = "signed component between " + pair_name(rc, cc) + " is not diagonally scalable to SDDM form";
+            return;
+        }
+        comp.diag = std::move(rowsum_e);
+        comp.diag.insert(comp.diag.end(), rowsum_k.begin(), rowsum_k.end());
+        comp.fold.assign(E.size() + K.size(), 1.0);
+        std::fill(comp.fold.begin() + E.size(), comp.fold.end(), -1.0);
+        comp.surplus.assign(E.size() + K.size(), 0.0);
+        out.push_back(std::move(comp));
+    }
+}
+
 void build_pair_components(const Design &d, const std::vector<double> &w, Channel rc, Channel cc,
                            const std::vector<std::vector<uint8_t>> &active, const LocalSolverSettings &cfg,
                            std::vector<Component> &out, std::vector<std::string> &warnings, std::string &error,
@@ -502,6 +745,10 @@ void build_pair_components(const Design &d, const std::vector<double> &w, Channe
     const Term &tr = d.terms[rc.term], &tc = d.terms[cc.term];
     const int64_t lr = tr.loads[rc.column], lc = tc.loads[cc.column];
     const bool general = lr >= 0 || lc >= 0;
+    if (cells && !general) {
+        plain_pair_components_from_cells(d, rc, cc, active, *cells, out, error);
+        return;
+    }
     // compact active levels
     std::vector<uint32_t> rmap(tr.n_levels, UINT32_MAX), cmap(tc.n_levels, UINT32_MAX);
     uint32_t nr = 0, nc = 0;
@@ -968,10 +1215,18 @@ std::shared_ptr<Precond> assemble_additive(std::vector<Component> &comps, uint64
             S->dense_off[c] = dense_total;
             dense_total += (uint64_t)comps[c].nk * comps[c].nk;
         }
-    S->dense.resize(dense_total); // uninitialised; pages first touched by all threads
-#pragma omp parallel for schedule(static)
-    for (int64_t q = 0; q < (int64_t)((dense_total + 4095) / 4096); ++q)
-        std::fill(S->dense.begin() + q * 4096, S->dense.begin() + std::min<uint64_t>(dense_total, (q + 1) * 4096), 0.0);
+    S->dense.resize(dense_total); // uninitialised
+    // the host-built blocks' pages first touched by all threads; the GPU-built blocks stay on the
+    // device and their host pages untouched until the host needs them
+    {
+        std::vector<std::pair<uint64_t, uint64_t>> host_ranges;
+        for (uint64_t c = 0; c < nc; ++c)
+            if (dense[c] && !big[c]) host_ranges.push_back({S->dense_off[c], (uint64_t)comps[c].nk * comps[c].nk});
+#pragma omp parallel for schedule(dynamic, 1)
+        for (int64_t r = 0; r < (int64_t)host_ranges.size(); ++r)
+            std::fill(S->dense.begin() + host_ranges[r].first,
+                      S->dense.begin() + host_ranges[r].first + host_ranges[r].second, 0.0);
+    }
     // per-component factorisation (parallel over components).  A dense block whose reduced
     // block costs many adjacency products (a small kept side next to a large eliminated one) is
     // built first, its rows in parallel; each row sums its terms in adjacency order, so the
@@ -1050,7 +1305,13 @@ std::shared_ptr<Precond> assemble_additive(std::vector<Component> &comps, uint64
                                cm.signed_form ? rowabs.data() : nullptr,
                                adj.tp.data(),  adj.te.data(),  adj.tv.data(),
                                cm.signed_form};
-        if (!dense_schur_pinv_gpu(a, S->gauge[c] == 0, S->dense.data() + S->dense_off[c])) failed[c] = 2;
+        std::shared_ptr<double> dev;
+        if (!dense_schur_pinv_gpu(a, S->gauge[c] == 0, dev)) {
+            failed[c] = 2;
+            continue;
+        }
+        S->dense_dev.push_back({S->dense_off[c], (uint64_t)cm.nk * cm.nk, std::move(dev)});
+        S->dense_on_host = false;
     }
     phase("large dense blocks");
     for (uint64_t c = 0; c < nc; ++c) {
@@ -1080,16 +1341,37 @@ std::shared_ptr<Precond> assemble_additive(std::vector<Component> &comps, uint64
     for (auto &cm : comps) cnnz += cm.cidx.size();
     S->c_idx.resize(cnnz);
     S->c_val.resize(cnnz);
-    uint64_t loff = 0, poff = 0, rptr_off = 0, coff = 0, cpos = 0;
+    // every component's place in the concatenated arrays, then the components one after another,
+    // each filling its own ranges with its rows in parallel
+    std::vector<uint64_t> c_at(nc + 1, 0), l_at(nc, 0), p_at(nc, 0), r_at(nc, 0), v_at(nc, 0);
+    {
+        uint64_t loff = 0, poff = 0, rptr_off = 0, coff = 0;
+        for (uint64_t c = 0; c < nc; ++c) {
+            c_at[c + 1] = c_at[c] + comps[c].cidx.size();
+            if (dense[c]) continue;
+            l_at[c] = loff;
+            p_at[c] = poff;
+            r_at[c] = rptr_off;
+            v_at[c] = coff;
+            const uint64_t nf = acs[c].d.size();
+            loff += acs[c].ridx.size();
+            rptr_off += nf + 1;
+            poff += nf;
+            if (S->fac_kind[c] == 2) coff += nf;
+        }
+    }
     for (uint64_t c = 0; c < nc; ++c) {
         const Component &cm = comps[c];
         const uint64_t base = S->slot_start[c];
         const uint32_t ne = cm.ne, nk = cm.nk;
+        const bool par = cm.cidx.size() + ne + nk >= (1u << 16);
+        const bool relabelled = S->fac_kind[c] == 1;
         std::vector<uint32_t> relabel(nk); // old kept index -> new
-        if (S->fac_kind[c] == 1)
+        if (relabelled)
             for (uint32_t t = 0; t < nk; ++t) relabel[acs[c].order[t]] = t;
         else std::iota(relabel.begin(), relabel.end(), 0u);
-        for (uint32_t e = 0; e < ne; ++e) {
+#pragma omp parallel for schedule(static) if (par)
+        for (int64_t e = 0; e < (int64_t)ne; ++e) {
             S->glob[base + e] = cm.glob[e];
             S->fold[base + e] = cm.fold[e];
             S->inv_de[base + e] = 1.0 / cm.diag[e];
@@ -1098,50 +1380,56 @@ std::shared_ptr<Precond> assemble_additive(std::vector<Component> &comps, uint64
             S->glob[base + ne + relabel[k]] = cm.glob[ne + k];
             S->fold[base + ne + relabel[k]] = cm.fold[ne + k];
         }
-        for (uint32_t s = 0; s < ne + nk; ++s) {
+#pragma omp parallel for schedule(static) if (par)
+        for (int64_t s = 0; s < (int64_t)(ne + nk); ++s) {
             const uint32_t g = S->glob[base + s];
             S->pw[base + s] = (!any_slope && counts[g] > 1) ? 1.0 / std::sqrt((double)counts[g]) : 1.0;
         }
-        for (uint32_t e = 0; e < ne; ++e) {
-            S->c_ptr[base + e] = cpos;
-            for (uint64_t p = cm.cptr[e]; p < cm.cptr[e + 1]; ++p) {
-                S->c_idx[cpos] = relabel[cm.cidx[p]];
-                S->c_val[cpos] = cm.cval[p];
-                ++cpos;
+        // C rows, each sorted by its (relabelled) kept index; rows of a component come out of
+        // the cross-tab kept-ascending, so only a relabelled block needs the sort
+#pragma omp parallel for schedule(dynamic, 512) if (par)
+        for (int64_t e = 0; e < (int64_t)ne; ++e) {
+            const uint64_t at = c_at[c] + cm.cptr[e];
+            S->c_ptr[base + e] = at;
+            const uint64_t len = cm.cptr[e + 1] - cm.cptr[e];
+            if (!relabelled) {
+                std::copy(cm.cidx.begin() + cm.cptr[e], cm.cidx.begin() + cm.cptr[e + 1], S->c_idx.begin() + at);
+                std::copy(cm.cval.begin() + cm.cptr[e], cm.cval.begin() + cm.cptr[e + 1], S->c_val.begin() + at);
+                continue;
             }
-            // keep each C row sorted by (relabelled) kept index
-            std::vector<std::pair<uint32_t, double>> row;
-            for (uint64_t p = S->c_ptr[base + e]; p < cpos; ++p) row.push_back({S->c_idx[p], S->c_val[p]});
+            std::vector<std::pair<uint32_t, double>> row(len);
+            for (uint64_t q = 0; q < len; ++q)
+                row[q] = {relabel[cm.cidx[cm.cptr[e] + q]], cm.cval[cm.cptr[e] + q]};
             std::sort(row.begin(), row.end(), [](auto &a, auto &b) { return a.first < b.first; });
-            for (size_t q = 0; q < row.size(); ++q) {
-                S->c_idx[S->c_ptr[base + e] + q] = row[q].first;
-                S->c_val[S->c_ptr[base + e] + q] = row[q].second;
+            for (uint64_t q = 0; q < len; ++q) {
+                S->c_idx[at + q] = row[q].first;
+                S->c_val[at + q] = row[q].second;
             }
         }
-        for (uint32_t k = 0; k < nk; ++k) S->c_ptr[base + ne + k] = cpos;
-        if (dense[c]) { // the large blocks are in place already
-            if (!big[c]) std::copy(dense_mats[c].begin(), dense_mats[c].end(), S->dense.begin() + S->dense_off[c]);
+        for (uint32_t k = 0; k < nk; ++k) S->c_ptr[base + ne + k] = c_at[c + 1];
+        if (dense[c]) { // the large blocks are on the device already
+            if (!big[c]) {
+                const uint64_t m = dense_mats[c].size();
+#pragma omp parallel for schedule(static) if (m >= (1u << 20))
+                for (int64_t q = 0; q < (int64_t)m; ++q) S->dense[S->dense_off[c] + q] = dense_mats[c][q];
+            }
             std::vector<double>().swap(dense_mats[c]);
         } else {
             const AcFactor &f = acs[c];
             const uint64_t nf = f.d.size();
-            S->l_row_off[c] = rptr_off;
-            S->piv_off[c] = poff;
-            for (uint64_t i = 0; i <= nf; ++i) S->l_ptr[rptr_off + i] = loff + f.rptr[i];
-            std::copy(f.ridx.begin(), f.ridx.end(), S->l_idx.begin() + loff);
-            std::copy(f.rval.begin(), f.rval.end(), S->l_val.begin() + loff);
-            std::copy(f.d.begin(), f.d.end(), S->piv.begin() + poff);
-            loff += f.ridx.size();
-            rptr_off += nf + 1;
-            poff += nf;
+            S->l_row_off[c] = r_at[c];
+            S->piv_off[c] = p_at[c];
+            for (uint64_t q = 0; q <= nf; ++q) S->l_ptr[r_at[c] + q] = l_at[c] + f.rptr[q];
+            std::copy(f.ridx.begin(), f.ridx.end(), S->l_idx.begin() + l_at[c]);
+            std::copy(f.rval.begin(), f.rval.end(), S->l_val.begin() + l_at[c]);
+            std::copy(f.d.begin(), f.d.end(), S->piv.begin() + p_at[c]);
             if (S->fac_kind[c] == 2) {
-                S->cover_off[c] = coff;
-                std::copy(f.order.begin(), f.order.end(), S->cover_perm.begin() + coff);
-                coff += nf;
+                S->cover_off[c] = v_at[c];
+                std::copy(f.order.begin(), f.order.end(), S->cover_perm.begin() + v_at[c]);
             }
         }
     }
-    S->c_ptr[ns] = cpos;
+    S->c_ptr[ns] = c_at[nc];
     pre->schwarz = S;
     phase("assembly");
     return pre;

@@ -944,6 +944,11 @@ uint32_t grid_rows(uint64_t rows, uint32_t kshift, uint32_t cap = 65535) {
     return (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(g, cap));
 }
 
+__global__ __launch_bounds__(256) void sk_cast_f32(uint64_t n, const double *__restrict__ src, float *__restrict__ dst) {
+    for (uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (uint64_t)gridDim.x * 256)
+        dst[i] = (float)src[i];
+}
+
 template <typename T, typename V> std::vector<T> cast_vec(const V &v) {
     std::vector<T> o(v.size());
 #pragma omp parallel for schedule(static) if (v.size() > (1u << 20))
@@ -1138,11 +1143,44 @@ std::shared_ptr<SchwarzDev<CT>> build_dev(const Precond::Schwarz &S) {
     D->drow_base.upload(drow_base.data(), drow_base.size());
     D->drow_nk.upload(drow_nk.data(), drow_nk.size());
     D->drow_slot.upload(drow_slot.data(), drow_slot.size());
-    if constexpr (std::is_same<CT, double>::value) {
-        D->dmat.upload(S.dense.data(), S.dense.size());
-    } else {
-        auto v = cast_vec<CT>(S.dense);
-        D->dmat.upload(v.data(), v.size());
+    // the dense operators: host-built blocks uploaded, GPU-built ones copied (and for FP32
+    // converted) on the device
+    {
+        const uint64_t total = S.dense.size();
+        D->dmat.alloc(total * sizeof(CT));
+        std::vector<std::pair<uint64_t, uint64_t>> dev_ranges;
+        for (const auto &b : S.dense_dev) dev_ranges.push_back({b.off, b.count});
+        std::sort(dev_ranges.begin(), dev_ranges.end());
+        uint64_t at = 0;
+        auto upload_host = [&](uint64_t from, uint64_t to) {
+            if (to <= from) return;
+            if constexpr (std::is_same<CT, double>::value) {
+                WX_HIP(hipMemcpy(D->dmat.template as<CT>() + from, S.dense.data() + from, (to - from) * 8,
+                                 hipMemcpyHostToDevice));
+            } else {
+                std::vector<CT> v(to - from);
+#pragma omp parallel for schedule(static) if (to - from > (1u << 20))
+                for (int64_t i = 0; i < (int64_t)(to - from); ++i) v[i] = (CT)S.dense[from + i];
+                WX_HIP(hipMemcpy(D->dmat.template as<CT>() + from, v.data(), v.size() * sizeof(CT),
+                                 hipMemcpyHostToDevice));
+            }
+        };
+        for (const auto &r : dev_ranges) {
+            upload_host(at, r.first);
+            at = r.first + r.second;
+        }
+        upload_host(at, total);
+        for (const auto &b : S.dense_dev) {
+            if constexpr (std::is_same<CT, double>::value) {
+                WX_HIP(hipMemcpy(D->dmat.template as<CT>() + b.off, b.dev.get(), b.count * 8, hipMemcpyDeviceToDevice));
+            } else {
+                const uint32_t g = (uint32_t)std::min<uint64_t>((b.count + 255) / 256, 1u << 20);
+                hipLaunchKernelGGL(sk_cast_f32, dim3(g), dim3(256), 0, 0, b.count, b.dev.get(),
+                                   D->dmat.template as<CT>() + b.off);
+                WX_HIP(hipGetLastError());
+            }
+        }
+        WX_HIP(hipDeviceSynchronize());
     }
     // prolongation map
     std::vector<uint64_t> d_ptr(S.n_dofs + 1, 0);

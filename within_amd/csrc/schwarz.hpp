@@ -65,6 +65,18 @@ struct Precond::Schwarz {
     // dense kept-block operators: S^+ (nk x nk row-major) at dense_off[c]
     std::vector<uint64_t> dense_off;  // n_comp
     DenseVec dense;
+    // Blocks formed on the GPU stay resident there (FP64, `count` entries that belong at `off` in
+    // `dense`); the device apply copies them from there, and their host copy in `dense` is
+    // written only when the host needs it (serialisation, merging): host_dense().
+    struct DeviceBlock {
+        uint64_t off = 0, count = 0;
+        std::shared_ptr<double> dev;
+    };
+    std::vector<DeviceBlock> dense_dev;
+    std::shared_ptr<std::mutex> dense_mu = std::make_shared<std::mutex>();
+    bool dense_on_host = true; // guarded by *dense_mu
+    // Completes `dense` on the host (downloads device blocks once); safe from several threads.
+    void host_dense() const;
     // sparse LDL' factors: L strictly lower, CSR by rows (local indices), pivots d
     std::vector<uint64_t> l_row_off;  // n_comp: offset of the component's first row pointer
     std::vector<uint64_t> l_ptr;
@@ -127,7 +139,8 @@ struct CellList {
 };
 bool device_cross_tabs(const Design &d, const std::vector<double> &w,
                        const std::vector<std::pair<uint32_t, uint32_t>> &pairs, std::vector<CellList> &out);
-bool dense_schur_pinv_gpu(const DenseSchurArgs &a, bool floating, double *out);
+// The operator (n_kept x n_kept, row-major, FP64) stays on the device in `out`.
+bool dense_schur_pinv_gpu(const DenseSchurArgs &a, bool floating, std::shared_ptr<double> &out);
 
 // Diagonal preconditioner (operator/schwarz.rs:185-229).
 std::shared_ptr<Precond> build_diagonal(const Design &d, const std::vector<double> &weights);

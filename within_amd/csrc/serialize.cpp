@@ -76,6 +76,7 @@ std::vector<uint8_t> serialize_precond(const Precond &p) {
         w.vec(S.c_idx);
         w.vec(S.c_val);
         w.vec(S.dense_off);
+        S.host_dense();
         w.vec(S.dense);
         w.vec(S.l_row_off);
         w.vec(S.l_ptr);
