@@ -343,14 +343,22 @@ def test_sparse_factor_apply_matches_oracle(gpu, oracle, case):
         np.testing.assert_allclose(r.demeaned, SMALL[f"{case}__demeaned"], atol=1e-8)
 
 
-def test_large_dense_kept_block_is_the_schur_pseudo_inverse(gpu):
+@pytest.mark.parametrize("rows", ["uniform", "zipf"])
+def test_large_dense_kept_block_is_the_schur_pseudo_inverse(gpu, rows):
     """Kept blocks above 1,024 levels get their Schur complement S = D_k - C' D_e^-1 C assembled
     and pseudo-inverted on the GPU (rocSOLVER Cholesky of the grounded minor, projected).  The
-    stored S+ must be numpy's pseudo-inverse of S rebuilt from the exported C and D_e^-1."""
+    stored S+ must be numpy's pseudo-inverse of S rebuilt from the exported C and D_e^-1.  With
+    Zipf-distributed eliminated levels the largest ones touch most kept levels, and their terms
+    come from the dense product on the matrix cores instead of the row kernel."""
     import scipy.sparse as sp
     rng = np.random.default_rng(33)
     n = 60_000
-    codes = np.asfortranarray(np.column_stack([rng.integers(0, 3000, n), rng.integers(0, 1500, n)]).astype(np.uint32))
+    if rows == "uniform":
+        a = rng.integers(0, 3000, n)
+    else:
+        pz = (np.arange(3000) + 1.0) ** -1.1
+        a = rng.choice(3000, n, p=pz / pz.sum())
+    codes = np.asfortranarray(np.column_stack([a, rng.integers(0, 1500, n)]).astype(np.uint32))
     s = wa.Solver(codes, weights=rng.uniform(0.5, 2.0, n))
     p = parse_precond(s.preconditioner.to_bytes())
     assert list(p["fac_kind"]) == [0] and int(p["nk"][0]) == 1500 and p["gauge"][0] == 0

@@ -43,17 +43,19 @@ constexpr uint32_t ROW_CHUNK = 8192; // kept columns per assembly workgroup (64 
 // Off-diagonal part of kept row i, columns [c0, c0 + chunk): -sum over the eliminated rows e
 // adjacent to i, in adjacency order, of (w_ie * C_ej) / D_e.  Entries of one eliminated row
 // have distinct columns, so the threads of a workgroup update disjoint slots between barriers.
+// Eliminated rows flagged in `heavy` are skipped here: their terms come from one dense product.
 __global__ __launch_bounds__(DB) void k_schur_rows(uint32_t nk, const uint64_t *__restrict__ cptr,
                                                    const uint32_t *__restrict__ cidx, const double *__restrict__ cval,
                                                    const double *__restrict__ diag, const uint64_t *__restrict__ tp,
                                                    const uint32_t *__restrict__ te, const double *__restrict__ tv,
-                                                   double *__restrict__ S) {
+                                                   const uint8_t *__restrict__ heavy, double *__restrict__ S) {
     extern __shared__ double row[];
     const uint32_t i = blockIdx.x, c0 = blockIdx.y * ROW_CHUNK, c1 = min(nk, c0 + ROW_CHUNK), w = c1 - c0;
     for (uint32_t j = threadIdx.x; j < w; j += DB) row[j] = 0.0;
     __syncthreads();
     for (uint64_t q = tp[i]; q < tp[i + 1]; ++q) {
         const uint32_t e = te[q];
+        if (heavy && heavy[e]) continue; // uniform over the workgroup
         const double wi = tv[q], inv = 1.0 / diag[e];
         for (uint64_t p = cptr[e] + threadIdx.x; p < cptr[e + 1]; p += DB) {
             const uint32_t j = cidx[p];
@@ -62,6 +64,17 @@ __global__ __launch_bounds__(DB) void k_schur_rows(uint32_t nk, const uint64_t *
         __syncthreads();
     }
     for (uint32_t j = threadIdx.x; j < w; j += DB) S[(uint64_t)i * nk + c0 + j] = row[j];
+}
+
+// H (n_h x nk, row-major, zeroed): row h = C row heavy_rows[h] scaled by 1 / sqrt(D_e), so the
+// heavy rows' part of the off-diagonal block is -H' H.
+__global__ __launch_bounds__(DB) void k_heavy_rows(uint32_t nk, const uint32_t *__restrict__ rows,
+                                                   const uint64_t *__restrict__ cptr,
+                                                   const uint32_t *__restrict__ cidx, const double *__restrict__ cval,
+                                                   const double *__restrict__ diag, double *__restrict__ H) {
+    const uint32_t h = blockIdx.x, e = rows[h];
+    const double s = 1.0 / sqrt(diag[e]);
+    for (uint64_t p = cptr[e] + threadIdx.x; p < cptr[e + 1]; p += DB) H[(uint64_t)h * nk + cidx[p]] = cval[p] * s;
 }
 
 // Diagonal of kept row i: minus the off-diagonal row sum (a fixed-order tree; the magnitude
@@ -193,18 +206,49 @@ bool dense_schur_pinv_gpu(const DenseSchurArgs &a, bool floating, std::shared_pt
     upload_vec(tv, a.tv, a.nnz);
     S.alloc((size_t)n * n * 8);
     double *s = S.as<double>();
+    // Eliminated rows touching many kept levels (deg >= n / 8) cost deg^2 scattered updates each in
+    // the row kernel; they are gathered into a dense block whose product -H'H runs on the matrix
+    // cores instead, and the row kernel adds only the others' terms.
+    std::vector<uint32_t> heavy_rows;
+    std::vector<uint8_t> heavy_flag(a.ne, 0);
+    for (uint32_t e = 0; e < a.ne; ++e)
+        if ((a.cptr[e + 1] - a.cptr[e]) * 8 >= n && n >= 1024) {
+            heavy_rows.push_back(e);
+            heavy_flag[e] = 1;
+        }
+    DevBuf hflag, hrows, hmat;
+    if (!heavy_rows.empty()) {
+        upload_vec(hflag, heavy_flag.data(), heavy_flag.size());
+        upload_vec(hrows, heavy_rows.data(), heavy_rows.size());
+    }
     const uint32_t chunks = (n + ROW_CHUNK - 1) / ROW_CHUNK;
     const size_t lds = (size_t)std::min(n, ROW_CHUNK) * 8;
     hipLaunchKernelGGL(k_schur_rows, dim3(n, chunks), dim3(DB), lds, 0, n, cptr.as<uint64_t>(), cidx.as<uint32_t>(),
-                       cval.as<double>(), diag.as<double>(), tp.as<uint64_t>(), te.as<uint32_t>(), tv.as<double>(), s);
+                       cval.as<double>(), diag.as<double>(), tp.as<uint64_t>(), te.as<uint32_t>(), tv.as<double>(),
+                       heavy_rows.empty() ? nullptr : hflag.as<uint8_t>(), s);
     WX_HIP(hipGetLastError());
+    Handle H0;
+    if (!heavy_rows.empty()) {
+        const uint32_t nh = (uint32_t)heavy_rows.size();
+        hmat.alloc((size_t)nh * n * 8);
+        WX_HIP(hipMemsetAsync(hmat.p, 0, (size_t)nh * n * 8, 0));
+        hipLaunchKernelGGL(k_heavy_rows, dim3(nh), dim3(DB), 0, 0, n, hrows.as<uint32_t>(), cptr.as<uint64_t>(),
+                           cidx.as<uint32_t>(), cval.as<double>(), diag.as<double>(), hmat.as<double>());
+        WX_HIP(hipGetLastError());
+        // S (row-major n x n == column-major, symmetric) += -H' H; H row-major nh x n is the
+        // column-major n x nh matrix A, so the product is A A'
+        const double alpha = -1.0, beta = 1.0;
+        WX_RB(rocblas_dgemm(H0.h, rocblas_operation_none, rocblas_operation_transpose, (rocblas_int)n,
+                            (rocblas_int)n, (rocblas_int)nh, &alpha, hmat.as<double>(), (rocblas_int)n, hmat.as<double>(),
+                            (rocblas_int)n, &beta, s, (rocblas_int)n));
+    }
     hipLaunchKernelGGL(k_schur_diag, dim3(n), dim3(DB), 0, 0, a.ne, n, diag.as<double>(),
                        a.surplus ? sur.as<double>() : nullptr, a.rowabs ? rabs.as<double>() : nullptr,
                        tp.as<uint64_t>(), te.as<uint32_t>(), tv.as<double>(), a.signed_form ? 1 : 0, s);
     WX_HIP(hipGetLastError());
     // Cholesky and inverse of the (grounded) minor in place: row-major == column-major
     const uint32_t m = floating ? n - 1 : n;
-    Handle H;
+    Handle &H = H0;
     info.alloc(sizeof(rocblas_int));
     rocblas_int hinfo = 0;
     WX_RB(rocsolver_dpotrf(H.h, rocblas_fill_lower, (rocblas_int)m, s, (rocblas_int)n, info.as<rocblas_int>()));

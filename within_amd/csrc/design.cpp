@@ -160,7 +160,10 @@ std::vector<std::vector<uint8_t>> find_active_levels(const Design &d) {
     for (uint64_t t = 0; t < d.terms.size(); ++t) {
         active[t].assign(d.terms[t].n_levels, 0);
         const uint32_t *col = d.level_column(t);
-        for (uint64_t i = 0; i < d.n_obs; ++i) active[t][col[i]] = 1;
+        uint8_t *a = active[t].data();
+#pragma omp parallel for schedule(static) if (d.n_obs >= (1u << 20))
+        for (int64_t i = 0; i < (int64_t)d.n_obs; ++i) // read first: stores only the first time
+            if (!__atomic_load_n(&a[col[i]], __ATOMIC_RELAXED)) __atomic_store_n(&a[col[i]], (uint8_t)1, __ATOMIC_RELAXED);
     }
     return active;
 }

@@ -1125,10 +1125,18 @@ std::shared_ptr<Precond> build_additive(const Design &d, const std::vector<doubl
     std::vector<std::vector<Component>> per_pair(pairs.size());
     std::vector<std::vector<std::string>> pair_warn(pairs.size());
     std::vector<std::string> pair_err(pairs.size());
+    // pairs accumulated on the host run side by side; pairs with device cells run one after
+    // another, each using every thread inside
+    auto from_cells = [&](int64_t p) { return on_device && plain_of[p] >= 0; };
 #pragma omp parallel for schedule(dynamic, 1)
     for (int64_t p = 0; p < (int64_t)pairs.size(); ++p)
-        build_pair_components(d, w, pairs[p].first, pairs[p].second, active, cfg, per_pair[p], pair_warn[p],
-                              pair_err[p], on_device && plain_of[p] >= 0 ? &cells[plain_of[p]] : nullptr);
+        if (!from_cells(p))
+            build_pair_components(d, w, pairs[p].first, pairs[p].second, active, cfg, per_pair[p], pair_warn[p],
+                                  pair_err[p], nullptr);
+    for (int64_t p = 0; p < (int64_t)pairs.size(); ++p)
+        if (from_cells(p))
+            build_pair_components(d, w, pairs[p].first, pairs[p].second, active, cfg, per_pair[p], pair_warn[p],
+                                  pair_err[p], &cells[plain_of[p]]);
     phase("cross-tabs and components");
     for (size_t p = 0; p < pairs.size(); ++p) {
         if (!pair_err[p].empty()) fail(WX_ERR_BUILD, pair_err[p], "BuildError::UnscalableComponent");
