@@ -79,7 +79,10 @@ def run_gpu(solver, Y_dev, k, n, N, precision, steps, warmup, tol):
     stream = torch.cuda.current_stream().cuda_stream
 
     def step(timing=False):
+        # the timed solves do the reference's work (x, demeaned, the LSMR residual estimate); the
+        # event-timed solve after them also forms the FP64 true residual that the line reports
         opts.timing = 1 if timing else 0
+        opts.true_residual = 1 if timing else 0
         _lib.check(L.wx_solver_solve_batch_device(solver._h, Y_dev.data_ptr(), n, k, C.byref(opts), X.data_ptr(),
                                                   Dm.data_ptr(), stats, C.c_void_p(stream)))
 
@@ -110,6 +113,7 @@ def run_plain_fp64(solver, Y_dev, n, N, tol, n_rhs):
     L.wx_lsmr_options_default(C.byref(opts))
     opts.tol = tol
     opts.precision = 0
+    opts.true_residual = 0
     X = torch.empty((1, N), dtype=torch.float64, device="cuda")
     Dm = torch.empty((1, n), dtype=torch.float64, device="cuda")
     stats = (_lib.RhsStats * 1)()

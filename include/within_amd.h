@@ -107,6 +107,9 @@ typedef struct wx_lsmr_options {
     int32_t precision;         /* wx_precision */
     int32_t check_every;       /* host convergence poll interval in iterations; <= 0: auto */
     int32_t timing;            /* != 0: time the solve's kernels with HIP events (wx_last_timing) */
+    int32_t true_residual;     /* != 0 (default): compute wx_rhs_stats.true_residual, one more D' pass
+                                  over the demeaned output; 0: skip it (reported as NaN), leaving the
+                                  reference's work: x, demeaned and the LSMR estimate `residual` */
 } wx_lsmr_options;
 
 /* Per-RHS result, the per-column fields of BatchSolveResult (crates/within/src/solver.rs:248-293)

@@ -739,3 +739,20 @@ def test_device_cross_tabs_give_the_host_preconditioner(gpu, case):
     # and the Solver (which builds through the same path) solves with it
     r = wa.solve(codes, np.asfortranarray(rng.standard_normal(n)), weights=w)
     assert r.converged
+
+
+def test_true_residual_can_be_skipped(gpu):
+    """LsmrOptions(true_residual=False) leaves the reference's work only: the same x, demeaned
+    output and LSMR estimate bit for bit, with the FP64 true residual reported as NaN."""
+    rng = np.random.default_rng(71)
+    n = 30_000
+    codes = np.asfortranarray(np.column_stack([rng.integers(0, 900, n), rng.integers(0, 40, n)]).astype(np.uint32))
+    Y = np.asfortranarray(rng.standard_normal((n, 4)))
+    w = rng.uniform(0.5, 2.0, n)
+    s = wa.Solver(codes, weights=w)
+    for precision in ("fp64", "mixed"):
+        a = s.solve_batch(Y, wa.LsmrOptions(precision=precision))
+        b = s.solve_batch(Y, wa.LsmrOptions(precision=precision, true_residual=False))
+        assert np.array_equal(a.x, b.x) and np.array_equal(a.demeaned, b.demeaned)
+        assert list(a.residual) == list(b.residual) and list(a.iterations) == list(b.iterations)
+        assert all(np.isfinite(a.true_residual)) and all(np.isnan(b.true_residual))

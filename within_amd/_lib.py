@@ -54,6 +54,7 @@ class LsmrOpts(C.Structure):
         ("precision", C.c_int32),
         ("check_every", C.c_int32),
         ("timing", C.c_int32),
+        ("true_residual", C.c_int32),
     ]
 
 

@@ -88,12 +88,14 @@ ReductionStrategy.ParallelReduction = ReductionStrategy(2, "ParallelReduction")
 
 class LsmrOptions:
     """LSMR tuning (crates/within/src/config.rs:160-178): ``tol=1e-8``, ``maxiter=1000``,
-    ``local_size=None``; plus ``precision`` ("fp64" or "mixed")."""
+    ``local_size=None``; plus ``precision`` ("fp64" or "mixed") and ``true_residual`` (default
+    True: each result's ``true_residual`` is computed, one more D' pass over the demeaned output;
+    False leaves exactly the reference's work and reports it as NaN)."""
 
-    __slots__ = ("_tol", "_maxiter", "_local_size", "_precision")
+    __slots__ = ("_tol", "_maxiter", "_local_size", "_precision", "_true_residual")
 
     def __init__(self, tol: float = 1e-8, maxiter: int = 1000, local_size: Optional[int] = None,
-                 precision: str = "fp64"):
+                 precision: str = "fp64", true_residual: bool = True):
         if precision not in ("fp64", "mixed"):
             raise ValueError("precision must be 'fp64' or 'mixed'")
         if int(maxiter) < 0:
@@ -104,15 +106,17 @@ class LsmrOptions:
         self._maxiter = int(maxiter)
         self._local_size = None if local_size is None else int(local_size)
         self._precision = precision
+        self._true_residual = bool(true_residual)
 
     tol = property(lambda self: self._tol)
     maxiter = property(lambda self: self._maxiter)
     local_size = property(lambda self: self._local_size)
     precision = property(lambda self: self._precision)
+    true_residual = property(lambda self: self._true_residual)
 
     def __repr__(self) -> str:
         return (f"LsmrOptions(tol={self._tol}, maxiter={self._maxiter}, local_size={self._local_size}, "
-                f"precision={self._precision!r})")
+                f"precision={self._precision!r}, true_residual={self._true_residual})")
 
     def _native(self) -> _lib.LsmrOpts:
         o = _lib.LsmrOpts()
@@ -121,6 +125,7 @@ class LsmrOptions:
         o.maxiter = self._maxiter
         o.local_size = self._local_size or 0
         o.precision = 1 if self._precision == "mixed" else 0
+        o.true_residual = 1 if self._true_residual else 0
         return o
 
 

@@ -140,6 +140,7 @@ void wx_lsmr_options_default(wx_lsmr_options *o) {
     o->precision = WX_PRECISION_FP64;
     o->check_every = 0;
     o->timing = 0;
+    o->true_residual = 1;
 }
 
 uint64_t wx_last_timing(double *out, uint64_t n) {

@@ -2356,15 +2356,20 @@ class ChunkRun {
         }
         // residual numerator ||D'W(y - Dx)||; the denominator ||D'W y|| = beta_1 ||p~_1|| was
         // kept at initialisation
-        launch_adjoint(dv_, da, weighted_ ? 2 : 0, dm_int, ws.h.as<double>(), ws.segpart.as<double>(), nullptr, k,
-                       kshift, st, -1, ws.pieces_or_null());
-        hipLaunchKernelGGL(k_sumsq, dim3(gN_), dim3(BLK), 0, st, N, ws.h.as<double>(), part_, k, kshift);
-        hipLaunchKernelGGL(k_reduce_to, dim3(ncol_), dim3(BLK), 0, st, part_, gN_, kp, ws.red.as<double>(), k, rg_dof_);
+        const bool true_res = opts_.true_residual != 0;
+        if (true_res) {
+            launch_adjoint(dv_, da, weighted_ ? 2 : 0, dm_int, ws.h.as<double>(), ws.segpart.as<double>(), nullptr,
+                           k, kshift, st, -1, ws.pieces_or_null());
+            hipLaunchKernelGGL(k_sumsq, dim3(gN_), dim3(BLK), 0, st, N, ws.h.as<double>(), part_, k, kshift);
+            hipLaunchKernelGGL(k_reduce_to, dim3(ncol_), dim3(BLK), 0, st, part_, gN_, kp, ws.red.as<double>(), k,
+                               rg_dof_);
+        }
         WX_HIP(hipGetLastError());
         const size_t cap = ws.cap;
-        std::vector<double> num(ncol_), host_d(N_DOUBLE_FIELDS * cap);
+        std::vector<double> num(ncol_, std::nan("")), host_d(N_DOUBLE_FIELDS * cap);
         std::vector<int32_t> host_i(N_INT_FIELDS * cap);
-        WX_HIP(hipMemcpyAsync(num.data(), ws.red.as<double>(), ncol_ * 8, hipMemcpyDeviceToHost, st));
+        if (true_res)
+            WX_HIP(hipMemcpyAsync(num.data(), ws.red.as<double>(), ncol_ * 8, hipMemcpyDeviceToHost, st));
         WX_HIP(hipMemcpyAsync(host_d.data(), ws.cols.p, host_d.size() * 8, hipMemcpyDeviceToHost, st));
         WX_HIP(hipMemcpyAsync(host_i.data(), ws.ints.p, host_i.size() * 4, hipMemcpyDeviceToHost, st));
         WX_HIP(hipStreamSynchronize(st));
@@ -2407,7 +2412,7 @@ class ChunkRun {
                 o.residual = std::fabs(zeta_bar[c]) / zeta0[c];
                 o.residual_norm = std::fabs(phi_bar[c]);
             }
-            o.true_residual = den_norm[c] > 0.0 ? std::sqrt(num[c]) / den_norm[c] : 0.0;
+            o.true_residual = !true_res ? std::nan("") : den_norm[c] > 0.0 ? std::sqrt(num[c]) / den_norm[c] : 0.0;
             o.time_solve = secs / k;
         }
     }

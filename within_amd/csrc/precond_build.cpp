@@ -524,11 +524,6 @@ std::string pair_name(Channel a, Channel b) {
            std::to_string(b.term) + " column " + std::to_string(b.column);
 }
 
-// Cross-tab of one channel pair and its components (cross_tab.rs, accumulate.rs, factor_pairs.rs:95-152),
-// each converted to SDDM form (sddm.rs).  Runs inside a parallel loop, so a failure is returned
-// as a message in `error` rather than thrown.
-// `cells`, when given (a plain pair), holds the cross-tab computed on the device, bitwise equal to
-// the host accumulation below.
 // Connected components of a bipartite graph (rows 0..nr-1, columns nr..nr+nc-1, edges from a
 // CSR) by a lock-free union-find: every edge hooks the larger of its two roots under the smaller,
 // so each component's root is its smallest node.  label[v] = that root.
@@ -738,6 +733,11 @@ void plain_pair_components_from_cells(const Design &d, Channel rc, Channel cc,
     }
 }
 
+// Cross-tab of one channel pair and its components (cross_tab.rs, accumulate.rs, factor_pairs.rs:95-152),
+// each converted to SDDM form (sddm.rs).  Runs inside a parallel loop, so a failure is returned
+// as a message in `error` rather than thrown.
+// `cells`, when given (a plain pair), holds the cross-tab computed on the device, bitwise equal to
+// the host accumulation below.
 void build_pair_components(const Design &d, const std::vector<double> &w, Channel rc, Channel cc,
                            const std::vector<std::vector<uint8_t>> &active, const LocalSolverSettings &cfg,
                            std::vector<Component> &out, std::vector<std::string> &warnings, std::string &error,
