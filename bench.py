@@ -285,7 +285,8 @@ def bench_many(args):
     sweep = {}
     if args.sweep:
         for shp in [(1, 1, False, False), (8, 2, True, False), (1, 1, False, True), (2, 1, True, True),
-                    (4, 1, True, True), (4, 2, True, True), (8, 2, True, True), (4, 4, True, True)]:
+                    (4, 1, True, True), (4, 2, True, True), (8, 2, True, True), (4, 4, True, True),
+                    (8, 2, True, True, 32), (8, 2, True, True, 128), (8, 2, True, True, 256)]:
             wa.set_many_shape(*shp)
             sweep[str(shp)] = timed(many) * 1e3
             log(f"[bench] shape {shp}: {sweep[str(shp)]:.1f} ms")
@@ -510,7 +511,7 @@ def main():
                          "fwl: the Gram matrix of a demeaned block")
     ap.add_argument("--problems", type=int, default=64)
     ap.add_argument("--sweep", action="store_true", help="many: also time every launch shape")
-    ap.add_argument("--shape", default="", help="many: fixed launch shape 'streams,iterations_per_graph,graph,pack'")
+    ap.add_argument("--shape", default="", help="many: fixed launch shape 'streams,iterations_per_graph,graph,pack[,tile]'")
     ap.add_argument("--no-seq", action="store_true", help="many: skip the one-after-another comparison")
     ap.add_argument("--copy-probe", action="store_true",
                     help="also run a plain 1 GiB device-to-device copy (calibrates HBM byte counters)")

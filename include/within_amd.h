@@ -260,6 +260,15 @@ void wx_last_many_shape(int32_t *streams, int32_t *iterations_per_graph, int32_t
  * autotuner. */
 void wx_set_many_shape(int32_t streams, int32_t iterations_per_graph, int32_t graph, int32_t pack);
 
+/* Work-item tile of the packed many-problem runs: the most rows of one D' or fused-gather work
+ * item (a larger level is split into pieces of that many rows whose partial sums are combined in
+ * order).  wx_set_many_tile fixes it for later calls (16-4096; 0 returns it to the autotuner,
+ * which tries 32, 64, 128 and 256 for the packed shape it picked); wx_last_many_tile reports
+ * the last call's.  No reference counterpart: the reference runs problems one after another
+ * (crates/within/src/solver.rs:530-577). */
+void wx_set_many_tile(int32_t rows);
+int32_t wx_last_many_tile(void);
+
 /* Adjoint of the weighted projection demeaned = z - D x(z, w) for reverse-mode
  * differentiation.  No counterpart in the reference (its bindings are not differentiable);
  * it serves torch.autograd callers of within_amd.autograd.  Given the upstream gradient

@@ -69,8 +69,10 @@ def singles(problems):
     return [s.solve_batch(Y, opts) for s, Y in problems]
 
 
+# (streams, iterations per graph, graph, pack[, work-item tile of the packs])
 SHAPES = [(1, 1, False, False), (4, 1, True, False), (8, 2, True, False), (16, 4, True, False),
-          (1, 1, False, True), (4, 1, True, True), (4, 4, True, True)]
+          (1, 1, False, True), (4, 1, True, True), (4, 4, True, True), (4, 2, True, True, 32),
+          (4, 2, True, True, 256)]
 
 
 @pytest.mark.parametrize("shape", SHAPES)
@@ -82,7 +84,8 @@ def test_many_matches_individual_solves(gpu, problems, singles, shape):
     finally:
         wa.set_many_shape(0)
     got = wa.last_many_shape()
-    assert (got["streams"], got["iterations_per_graph"], got["graph"], got["pack"]) == shape
+    assert (got["streams"], got["iterations_per_graph"], got["graph"], got["pack"]) == shape[:4]
+    assert got["tile"] == (shape[4] if len(shape) > 4 else 64)
     packed = shape[3]
     worst_d = worst_x = worst_off = worst_plain = 0.0
     diter = n_off = n_cols = 0

@@ -102,6 +102,8 @@ EXPORTS = {
                        C.c_void_p, C.c_void_p], C.c_int),
     "wx_last_many_shape": ([C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p], None),
     "wx_set_many_shape": ([C.c_int32, C.c_int32, C.c_int32, C.c_int32], None),
+    "wx_set_many_tile": ([C.c_int32], None),
+    "wx_last_many_tile": ([], C.c_int32),
     "wx_solver_projection_grad_device": ([C.c_void_p, C.c_void_p, C.c_uint64, C.c_void_p, C.c_uint64, C.c_uint64,
                                           C.POINTER(LsmrOpts), C.c_void_p, C.c_void_p, C.POINTER(RhsStats),
                                           C.c_void_p], C.c_int),

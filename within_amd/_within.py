@@ -842,10 +842,14 @@ def last_many_shape() -> dict:
     """The launch shape the autotuner picked for the last solve_many call."""
     s, g, gr, pk = C.c_int32(), C.c_int32(), C.c_int32(), C.c_int32()
     _lib.lib().wx_last_many_shape(C.byref(s), C.byref(g), C.byref(gr), C.byref(pk))
-    return {"streams": s.value, "iterations_per_graph": g.value, "graph": bool(gr.value), "pack": bool(pk.value)}
+    return {"streams": s.value, "iterations_per_graph": g.value, "graph": bool(gr.value), "pack": bool(pk.value),
+            "tile": int(_lib.lib().wx_last_many_tile())}
 
 
-def set_many_shape(streams: int = 0, iterations_per_graph: int = 1, graph: bool = True, pack: bool = False) -> None:
+def set_many_shape(streams: int = 0, iterations_per_graph: int = 1, graph: bool = True, pack: bool = False,
+                   tile: int = 0) -> None:
     """Fixes the launch shape of later solve_many calls (pack: problems of one shape run as one
-    block-diagonal batch); streams=0 returns the choice to the autotuner."""
+    block-diagonal batch; tile: rows per D' and gather work item of the packs, 0 = 64 when the
+    shape is fixed); streams=0 returns the choice to the autotuner."""
     _lib.lib().wx_set_many_shape(int(streams), int(iterations_per_graph), 1 if graph else 0, 1 if pack else 0)
+    _lib.lib().wx_set_many_tile(int(tile) if streams else 0)

@@ -643,6 +643,10 @@ void wx_set_many_shape(int32_t streams, int32_t iterations_per_graph, int32_t gr
     force_many_shape(streams, iterations_per_graph, graph, pack);
 }
 
+void wx_set_many_tile(int32_t rows) { force_many_tile(rows); }
+
+int32_t wx_last_many_tile(void) { return last_many_tile(); }
+
 void wx_last_many_shape(int32_t *streams, int32_t *iterations_per_graph, int32_t *graph, int32_t *pack) {
     last_many_shape(streams, iterations_per_graph, graph, pack);
 }

@@ -1540,7 +1540,7 @@ bool build_pair_plan(const Design &d, const std::vector<std::vector<uint64_t>> &
 }
 } // namespace
 
-void upload_design(Solver &s, const std::vector<double> &weights_internal) {
+void upload_design(Solver &s, const std::vector<double> &weights_internal, uint32_t tile) {
     const Design &d = s.design;
     DevDesign &dv = s.dev;
     // WITHIN_AMD_VERBOSE: per-phase wall times of the upload on stderr
@@ -1555,8 +1555,10 @@ void upload_design(Solver &s, const std::vector<double> &weights_internal) {
     };
     if (d.terms.size() > MAX_TERMS)
         fail(WX_ERR_INVALID_INPUT, "the device path supports at most " + std::to_string(MAX_TERMS) + " terms");
+    if (tile < 16 || tile > 4096) fail(WX_ERR_INVALID_INPUT, "work-item tile must be 16..4096 rows");
     dv.n = d.n_obs;
     dv.N = d.n_dofs;
+    dv.tile = tile;
     const uint64_t n = d.n_obs;
     hipStream_t st0 = nullptr;
     // codes, weights and the row permutation go up once; everything derived from them (square
@@ -1611,7 +1613,7 @@ void upload_design(Solver &s, const std::vector<double> &weights_internal) {
         for (uint32_t c = 0; c < MAX_COLS; ++c) dt.info.load[c] = c < tm.ncols() ? (int32_t)tm.loads[c] : -1;
         // the fused pair's primary term gets its permutation from the pair plan
         if (!tm.sorted && (int32_t)t != pair_p) term_permutation(dv, dt, (uint32_t)t, tm.n_levels);
-        const uint64_t slots = build_seg_plan(dt, starts[t], tm.n_levels, host_items[t]);
+        const uint64_t slots = build_seg_plan(dt, starts[t], tm.n_levels, host_items[t], tile);
         dv.max_partials = std::max<uint32_t>(dv.max_partials, (uint32_t)slots);
     }
     phase("D' plans");
@@ -1648,7 +1650,7 @@ void upload_design(Solver &s, const std::vector<double> &weights_internal) {
             if (!g.empty() && it.dest >= 0) {
                 SegItem &m = g.back();
                 if (m.dest >= 0 && it.level == (uint32_t)m.dest + 1 && it.begin == m.end &&
-                    (it.end - m.begin) <= SEG_CHUNK && (it.level - m.level) < GATHER_SPAN) {
+                    (it.end - m.begin) <= tile && (it.level - m.level) < GATHER_SPAN) {
                     m.end = it.end;
                     m.dest = it.dest;
                     continue;
@@ -2646,11 +2648,13 @@ struct ManyShape {
     int streams = 1, group = 1;
     bool graph = false;
     bool pack = false; // problems of one shape packed into one block-diagonal run
+    uint32_t tile = DEFAULT_TILE; // rows per D' / gather work item of the packs
 };
 std::mutex g_tune_mu;
 std::vector<std::pair<uint64_t, ManyShape>> g_tuned; // problem-count class -> shape
 ManyShape g_last_shape;
 ManyShape g_forced{0, 1, false, false}; // streams == 0: the autotuner decides
+uint32_t g_forced_tile = 0;               // 0: the autotuner decides
 
 // Streams of the many-problem runs.  A call leases its streams for its whole duration, so
 // concurrent calls never share one: a run that captures its iteration graph on a stream would
@@ -2730,7 +2734,7 @@ void run_many(const std::vector<const Solver *> &solvers, const std::vector<cons
                 mSt.push_back(stats[p]);
                 packed[p] = 1;
             }
-            packs.push_back(get_pack(mem, kv));
+            packs.push_back(get_pack(mem, kv, shape.tile));
             hipStream_t st = lease.s[next_stream];
             next_stream = (next_stream + 1) % shape.streams;
             WX_HIP(hipStreamWaitEvent(st, ready, 0));
@@ -2790,6 +2794,7 @@ void solve_many_device(const std::vector<const Solver *> &solvers, const std::ve
         std::lock_guard<std::mutex> lk(g_tune_mu);
         if (g_forced.streams > 0) {
             shape = g_forced;
+            shape.tile = g_forced_tile ? g_forced_tile : DEFAULT_TILE;
             tuned = true;
         } else {
             for (auto &e : g_tuned)
@@ -2810,7 +2815,7 @@ void solve_many_device(const std::vector<const Solver *> &solvers, const std::ve
         probe.maxiter = std::min<int64_t>(opts.maxiter, 24);
         probe.timing = 0;
         double best = 1e300;
-        for (const ManyShape &c : cand) {
+        auto trial = [&](const ManyShape &c) {
             // the first run of a shape builds what later calls reuse (packs, captured graphs,
             // workspaces); the second is timed
             run_many(solvers, Y, ldy, k, probe, X, Dm, stats, c, caller);
@@ -2819,11 +2824,22 @@ void solve_many_device(const std::vector<const Solver *> &solvers, const std::ve
             run_many(solvers, Y, ldy, k, probe, X, Dm, stats, c, caller);
             const double dt = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
             if (std::getenv("WITHIN_AMD_VERBOSE"))
-                fprintf(stderr, "[within_amd] autotune streams %d group %d graph %d pack %d: %.2f ms\n", c.streams,
-                        c.group, (int)c.graph, (int)c.pack, dt * 1e3);
+                fprintf(stderr, "[within_amd] autotune streams %d group %d graph %d pack %d tile %u: %.2f ms\n",
+                        c.streams, c.group, (int)c.graph, (int)c.pack, c.tile, dt * 1e3);
             if (dt < best) {
                 best = dt;
                 shape = c;
+            }
+        };
+        for (const ManyShape &c : cand) trial(c);
+        // then the packs' work-item tile: rows per D' and gather item (more items and more
+        // split-level partials when small, fewer and longer items when large)
+        if (shape.pack) {
+            const ManyShape base = shape;
+            for (uint32_t t : {32u, 128u, 256u}) {
+                ManyShape c = base;
+                c.tile = t;
+                trial(c);
             }
         }
         std::lock_guard<std::mutex> lk(g_tune_mu);
@@ -2834,6 +2850,16 @@ void solve_many_device(const std::vector<const Solver *> &solvers, const std::ve
         g_last_shape = shape;
     }
     run_many(solvers, Y, ldy, k, opts, X, Dm, stats, shape, caller);
+}
+
+void force_many_tile(int32_t rows) {
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    g_forced_tile = rows <= 0 ? 0u : (uint32_t)std::max(16, std::min(rows, 4096));
+}
+
+int32_t last_many_tile() {
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    return (int32_t)g_last_shape.tile;
 }
 
 void force_many_shape(int32_t streams, int32_t group, int32_t graph, int32_t pack) {

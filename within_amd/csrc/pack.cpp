@@ -88,7 +88,7 @@ std::shared_ptr<Precond> merge_schwarz(const std::vector<const Solver *> &ms, co
     return p;
 }
 
-std::shared_ptr<Pack> build_pack(const std::vector<const Solver *> &ms, const std::vector<uint32_t> &kv) {
+std::shared_ptr<Pack> build_pack(const std::vector<const Solver *> &ms, const std::vector<uint32_t> &kv, uint32_t tile) {
     const uint32_t k = *std::max_element(kv.begin(), kv.end());
     auto pk = std::make_shared<Pack>();
     pk->k = k;
@@ -183,7 +183,7 @@ std::shared_ptr<Pack> build_pack(const std::vector<const Solver *> &ms, const st
         sv->precond = merge_schwarz(ms, maps, md.n_dofs);
     }
     sv->uid = next_solver_uid();
-    upload_design(*sv, w);
+    upload_design(*sv, w, tile);
     DevDesign &dv = sv->dev;
     if (dv.fuse_term < 0) fail(WX_ERR_INVALID_INPUT, "packed design has no sorted term to fuse");
     // fused term: the gather walks one level (or level chunk) per item, each member's items
@@ -261,7 +261,7 @@ std::shared_ptr<Pack> build_pack(const std::vector<const Solver *> &ms, const st
 
 std::mutex g_pack_mu;
 std::list<std::pair<std::vector<uint64_t>, std::shared_ptr<Pack>>> g_packs; // most recent first
-constexpr size_t PACK_CACHE = 8;
+constexpr size_t PACK_CACHE = 16;
 
 } // namespace
 
@@ -275,8 +275,9 @@ bool pack_signature(const Solver &s, std::string &sig) {
     return true;
 }
 
-std::shared_ptr<Pack> get_pack(const std::vector<const Solver *> &members, const std::vector<uint32_t> &kv) {
-    std::vector<uint64_t> key;
+std::shared_ptr<Pack> get_pack(const std::vector<const Solver *> &members, const std::vector<uint32_t> &kv,
+                               uint32_t tile) {
+    std::vector<uint64_t> key{tile};
     for (size_t m = 0; m < members.size(); ++m) {
         key.push_back(members[m]->uid);
         key.push_back(kv[m]);
@@ -289,7 +290,7 @@ std::shared_ptr<Pack> get_pack(const std::vector<const Solver *> &members, const
                 return it->second;
             }
     }
-    auto pk = build_pack(members, kv);
+    auto pk = build_pack(members, kv, tile);
     std::lock_guard<std::mutex> lk(g_pack_mu);
     g_packs.emplace_front(key, pk);
     while (g_packs.size() > PACK_CACHE) g_packs.pop_back();

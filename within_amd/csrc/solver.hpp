@@ -12,6 +12,7 @@
 namespace wx {
 
 constexpr uint32_t MAX_TERMS = 8;
+constexpr uint32_t DEFAULT_TILE = 64; // rows per D' / fused-gather work item
 constexpr uint32_t MAX_COLS = 4;   // coefficient columns per term on the device path
 constexpr uint32_t MAX_BATCH = 64; // RHS columns interleaved per device pass
 constexpr uint32_t KERNEL_BLOCK = 256;        // threads per workgroup of the solver kernels
@@ -94,6 +95,7 @@ struct DevDesign {
     DevBuf codes4;    // plain designs with at most 4 terms: the row's codes as one 16-byte record
     std::vector<DevTerm> terms;
     uint32_t max_partials = 0;
+    uint32_t tile = DEFAULT_TILE; // rows per D' work item (upload_design)
     bool plain = false; // every term is a single intercept column
     int32_t fuse_term = -1; // sorted single-intercept term whose D' is fused into the gather
     PairPlan pair;          // p < 0: no fused pair
@@ -148,7 +150,9 @@ uint64_t dev_pieces(const uint32_t *qpos, uint64_t n, const SegItem *items, uint
                     DevBuf &piece_q, hipStream_t st);
 
 // Uploads the design (codes, slopes, weights, D' segment plans) to the current device.
-void upload_design(Solver &s, const std::vector<double> &weights_internal);
+// `tile`: the most rows of one D' work item (a larger level is split into pieces of that many
+// rows, whose partial sums are combined in order), and of one fused-gather item.
+void upload_design(Solver &s, const std::vector<double> &weights_internal, uint32_t tile = DEFAULT_TILE);
 
 // s.dev = base's device design with the observation weights w (internal order; empty:
 // unweighted), the index structures copied on the device (s.design and s.precond set).
@@ -197,7 +201,8 @@ bool pack_signature(const Solver &s, std::string &sig);
 
 // Builds (or returns from a small cache) the pack of `members` with kv[m] columns each, run at
 // batch width k = max kv.
-std::shared_ptr<Pack> get_pack(const std::vector<const Solver *> &members, const std::vector<uint32_t> &kv);
+std::shared_ptr<Pack> get_pack(const std::vector<const Solver *> &members, const std::vector<uint32_t> &kv,
+                               uint32_t tile = DEFAULT_TILE);
 
 // Runs the batched solve with device-resident inputs/outputs on `stream`.  x_solver_basis:
 // X is left in the solver's (reparametrised) coefficient basis instead of the caller's.
@@ -216,6 +221,8 @@ void force_many_shape(int32_t streams, int32_t group, int32_t graph, int32_t pac
 
 // The (streams, iterations per graph, graph on, packed) shape the last solve_many_device used.
 void last_many_shape(int32_t *streams, int32_t *group, int32_t *graph, int32_t *pack);
+void force_many_tile(int32_t rows);
+int32_t last_many_tile();
 
 // Adjoint of the weighted projection R = Z - D x(Z, w) for reverse-mode differentiation:
 // grad_z = G - w .* (D lambda), grad_w = -sum_j R_j .* (D lambda)_j with D'WD lambda = D'G.
