@@ -726,23 +726,40 @@ struct Ranges {
 __device__ inline double reduce_partials(const double *part, uint32_t nb, uint32_t kp, uint32_t stride_fields,
                                          uint32_t field, uint32_t j, double *sh, bool is_max, Ranges rg = {},
                                          uint32_t m = 0) {
-    double acc = 0.0;
+    // each thread keeps four accumulators (four loads in flight), combined in a fixed order;
+    // then a butterfly inside each wavefront and the wavefronts' results in order
+    double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
     const uint32_t r0 = rg.ptr ? rg.ptr[m] : 0u, r1 = rg.ptr ? rg.ptr[m + 1] : 1u;
+    auto at = [&](uint32_t b) { return part[((uint64_t)b * kp + j) * stride_fields + field]; };
     for (uint32_t r = r0; r < r1; ++r) {
         const uint32_t lo = rg.ptr ? rg.lh[2 * r] : 0u, hi = rg.ptr ? rg.lh[2 * r + 1] : nb;
-        for (uint32_t b = lo + threadIdx.x; b < hi; b += BLK) {
-            double v = part[((uint64_t)b * kp + j) * stride_fields + field];
-            acc = is_max ? fmax(acc, v) : acc + v;
+        uint32_t b = lo + threadIdx.x;
+        for (; b + 3 * BLK < hi; b += 4 * BLK) {
+            const double v0 = at(b), v1 = at(b + BLK), v2 = at(b + 2 * BLK), v3 = at(b + 3 * BLK);
+            if (is_max) {
+                a0 = fmax(a0, v0);
+                a1 = fmax(a1, v1);
+                a2 = fmax(a2, v2);
+                a3 = fmax(a3, v3);
+            } else {
+                a0 += v0;
+                a1 += v1;
+                a2 += v2;
+                a3 += v3;
+            }
         }
+        for (; b < hi; b += BLK) a0 = is_max ? fmax(a0, at(b)) : a0 + at(b);
     }
-    sh[threadIdx.x] = acc;
+    double acc = is_max ? fmax(fmax(a0, a1), fmax(a2, a3)) : (a0 + a1) + (a2 + a3);
+    for (int off = 32; off > 0; off >>= 1) {
+        const double o = __shfl_xor(acc, off, 64);
+        acc = is_max ? fmax(acc, o) : acc + o;
+    }
+    constexpr uint32_t NWAVE = BLK / 64;
+    if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = acc;
     __syncthreads();
-    for (uint32_t s = BLK / 2; s > 0; s >>= 1) {
-        if (threadIdx.x < s)
-            sh[threadIdx.x] = is_max ? fmax(sh[threadIdx.x], sh[threadIdx.x + s]) : sh[threadIdx.x] + sh[threadIdx.x + s];
-        __syncthreads();
-    }
     double r = sh[0];
+    for (uint32_t w = 1; w < NWAVE; ++w) r = is_max ? fmax(r, sh[w]) : r + sh[w];
     __syncthreads();
     return r;
 }

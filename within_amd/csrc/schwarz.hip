@@ -184,43 +184,13 @@ __global__ __launch_bounds__(BLK) void sk_restrict(uint64_t ns, const uint32_t *
         loc[s * k + j] = (CT)((double)sgnpw[s] * r[(uint64_t)glob[s] * k + j]);
 }
 
-// partial sums of src over chunks of slots (FP64 sums of the stored slot values)
-template <typename VT>
-__global__ __launch_bounds__(BLK) void sk_chunk_sum(const Chunk *__restrict__ chunks, uint32_t n_chunks,
-                                                    const VT *__restrict__ src, double *__restrict__ part,
-                                                    uint32_t k, uint32_t kshift, const int32_t *__restrict__ gate) {
-    if (gate_closed(gate, k)) return;
-    const uint32_t kp = 1u << kshift, j = threadIdx.x & (kp - 1);
-    if (j >= k) return;
-    const uint32_t g = (blockIdx.x * BLK + threadIdx.x) >> kshift;
-    if (g >= n_chunks) return;
-    const Chunk ch = chunks[g];
-    double acc = 0.0;
-    uint32_t s = ch.begin;
-    for (; s + 4 <= ch.end; s += 4) { // four loads in flight, summed in slot order
-        double t[4];
-#pragma unroll
-        for (int q = 0; q < 4; ++q) t[q] = (double)src[(uint64_t)(s + q) * k + j];
-        acc += t[0];
-        acc += t[1];
-        acc += t[2];
-        acc += t[3];
-    }
-    for (; s < ch.end; ++s) acc += (double)src[(uint64_t)s * k + j];
-    part[(uint64_t)g * k + j] = acc;
-}
-
-// mean per component = sum of its chunk partials / count; one workgroup per component,
-// row groups stride over the chunks, then a fixed-order LDS tree across the groups
-__global__ __launch_bounds__(BLK) void sk_comp_mean(const uint32_t *__restrict__ cptr, const double *__restrict__ part,
-                                                    const uint32_t *__restrict__ count, double *__restrict__ mu,
-                                                    uint32_t k, uint32_t kshift, const int32_t *__restrict__ gate) {
-    if (gate_closed(gate, k)) return;
-    __shared__ double sh[BLK];
+// Mean of component c over its chunk partials, by the whole workgroup: row groups stride over
+// the chunks with eight accumulators (partial q goes to q / ngrp mod 8, eight loads in flight),
+// combined in a fixed order, then a fixed-order LDS tree across the groups.
+__device__ inline void comp_mean_one(uint32_t c, const uint32_t *__restrict__ cptr, const double *__restrict__ part,
+                                     const uint32_t *__restrict__ count, double *__restrict__ mu, uint32_t k,
+                                     uint32_t kshift, double *sh) {
     const uint32_t kp = 1u << kshift, j = threadIdx.x & (kp - 1), grp = threadIdx.x >> kshift;
-    const uint32_t c = blockIdx.x;
-    // eight accumulators (partial q goes to q / ngrp mod 8) keep eight loads in flight; they
-    // are combined in a fixed order
     const uint32_t ngrp = BLK >> kshift;
     double a8[8] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
     if (j < k) {
@@ -245,6 +215,42 @@ __global__ __launch_bounds__(BLK) void sk_comp_mean(const uint32_t *__restrict__
         const uint32_t n = count[c];
         mu[(uint64_t)c * k + threadIdx.x] = n ? sh[threadIdx.x] / (double)n : 0.0;
     }
+    __syncthreads();
+}
+
+// mean per component = sum of its chunk partials / count; one workgroup per component
+__global__ __launch_bounds__(BLK) void sk_comp_mean(const uint32_t *__restrict__ cptr, const double *__restrict__ part,
+                                                    const uint32_t *__restrict__ count, double *__restrict__ mu,
+                                                    uint32_t k, uint32_t kshift, const int32_t *__restrict__ gate) {
+    if (gate_closed(gate, k)) return;
+    __shared__ double sh[BLK];
+    comp_mean_one(blockIdx.x, cptr, part, count, mu, k, kshift, sh);
+}
+
+// partial sums of src over chunks of slots (FP64 sums of the stored slot values)
+template <typename VT>
+__global__ __launch_bounds__(BLK) void sk_chunk_sum(const Chunk *__restrict__ chunks, uint32_t n_chunks,
+                                                    const VT *__restrict__ src, double *__restrict__ part,
+                                                    uint32_t k, uint32_t kshift, const int32_t *__restrict__ gate) {
+    if (gate_closed(gate, k)) return;
+    const uint32_t kp = 1u << kshift, j = threadIdx.x & (kp - 1);
+    if (j >= k) return;
+    const uint32_t g = (blockIdx.x * BLK + threadIdx.x) >> kshift;
+    if (g >= n_chunks) return;
+    const Chunk ch = chunks[g];
+    double acc = 0.0;
+    uint32_t s = ch.begin;
+    for (; s + 4 <= ch.end; s += 4) { // four loads in flight, summed in slot order
+        double t[4];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) t[q] = (double)src[(uint64_t)(s + q) * k + j];
+        acc += t[0];
+        acc += t[1];
+        acc += t[2];
+        acc += t[3];
+    }
+    for (; s < ch.end; ++s) acc += (double)src[(uint64_t)s * k + j];
+    part[(uint64_t)g * k + j] = acc;
 }
 
 // MODE 0: reduced rhs of kept rows:  red[row] = (loc[row]-mu) + sum_p ct[p]*(loc[e]-mu), the C'
@@ -597,8 +603,17 @@ __global__ __launch_bounds__(BLK) void sk_dense_reduce(const DenseComp *__restri
         }
         const DenseComp C = comps[lo];
         const uint64_t i = q - row_start[lo];
+        // the segments' partials added in segment order, eight loads in flight
         double acc = 0.0;
-        for (uint32_t s = 0; s < C.n_seg; ++s) acc += (double)part[(C.part_off + (uint64_t)s * C.nk + i) * k + j];
+        uint32_t s = 0;
+        for (; s + 8 <= C.n_seg; s += 8) {
+            double t[8];
+#pragma unroll
+            for (int q = 0; q < 8; ++q) t[q] = (double)part[(C.part_off + (uint64_t)(s + q) * C.nk + i) * k + j];
+#pragma unroll
+            for (int q = 0; q < 8; ++q) acc += t[q];
+        }
+        for (; s < C.n_seg; ++s) acc += (double)part[(C.part_off + (uint64_t)s * C.nk + i) * k + j];
         xs[(uint64_t)(C.kept_base + i) * k + j] = (CT)acc;
     }
 }
