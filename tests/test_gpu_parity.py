@@ -746,13 +746,19 @@ def test_true_residual_can_be_skipped(gpu):
     output and LSMR estimate bit for bit, with the FP64 true residual reported as NaN."""
     rng = np.random.default_rng(71)
     n = 30_000
-    codes = np.asfortranarray(np.column_stack([rng.integers(0, 900, n), rng.integers(0, 40, n)]).astype(np.uint32))
-    Y = np.asfortranarray(rng.standard_normal((n, 4)))
+    a0 = rng.integers(0, 900, n)
+    designs = [
+        np.column_stack([a0, rng.integers(0, 40, n)]),                          # rows permuted
+        np.column_stack([np.sort(a0), rng.integers(0, 40, n), rng.integers(0, 7, n)]),  # caller order kept
+    ]
     w = rng.uniform(0.5, 2.0, n)
-    s = wa.Solver(codes, weights=w)
-    for precision in ("fp64", "mixed"):
-        a = s.solve_batch(Y, wa.LsmrOptions(precision=precision))
-        b = s.solve_batch(Y, wa.LsmrOptions(precision=precision, true_residual=False))
-        assert np.array_equal(a.x, b.x) and np.array_equal(a.demeaned, b.demeaned)
-        assert list(a.residual) == list(b.residual) and list(a.iterations) == list(b.iterations)
-        assert all(np.isfinite(a.true_residual)) and all(np.isnan(b.true_residual))
+    for codes in designs:
+        s = wa.Solver(np.asfortranarray(codes.astype(np.uint32)), weights=w)
+        for k in (4, 3):
+            Y = np.asfortranarray(rng.standard_normal((n, k)))
+            for precision in ("fp64", "mixed"):
+                a = s.solve_batch(Y, wa.LsmrOptions(precision=precision))
+                b = s.solve_batch(Y, wa.LsmrOptions(precision=precision, true_residual=False))
+                assert np.array_equal(a.x, b.x) and np.array_equal(a.demeaned, b.demeaned)
+                assert list(a.residual) == list(b.residual) and list(a.iterations) == list(b.iterations)
+                assert all(np.isfinite(a.true_residual)) and all(np.isnan(b.true_residual))

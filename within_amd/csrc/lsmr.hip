@@ -1295,6 +1295,69 @@ __global__ __launch_bounds__(BLK) void k_store_caller(uint64_t n, const uint32_t
     }
 }
 
+// The demeaned output y - D x straight into caller order, column-major (k_demean2 followed by
+// k_store_caller, without the internal-order round trip): per 64-row tile of caller rows, each
+// row's internal row i = inv_perm[r] gives y_int, the code record and the coefficient rows; the
+// terms are added in order from 0.0 and subtracted from y, as k_demean2 does, so the values are
+// the same bits.  For plain designs of up to four terms (code records present).
+template <int TT>
+__global__ __launch_bounds__(BLK) void k_demean_store(DesignArgs d, const uint4 *__restrict__ codes4,
+                                                      const uint32_t *__restrict__ inv_perm,
+                                                      const double *__restrict__ x, const double *__restrict__ y_int,
+                                                      double *__restrict__ Dm, uint64_t ldd, uint32_t k,
+                                                      uint32_t kshift) {
+    extern __shared__ double tile[]; // 64 x (k + 1)
+    const uint32_t ld = k + 1;
+    const uint32_t kp = 1u << kshift, j = threadIdx.x & (kp - 1), grp = threadIdx.x >> kshift;
+    const uint32_t ngrp = BLK >> kshift;
+    const uint64_t n = d.n, n_tiles = (n + 63) / 64;
+    constexpr int R = 4;
+    for (uint64_t tb = blockIdx.x; tb < n_tiles; tb += gridDim.x) {
+        const uint64_t r0 = tb * 64;
+        if (j < k)
+            for (uint32_t rb = grp; rb < 64; rb += R * ngrp) {
+                uint64_t row[R];
+                bool ok[R];
+#pragma unroll
+                for (int q = 0; q < R; ++q) {
+                    const uint64_t r = r0 + rb + q * ngrp;
+                    ok[q] = rb + q * ngrp < 64 && r < n;
+                    row[q] = ok[q] ? (inv_perm ? inv_perm[r] : r) : 0;
+                }
+                double yv[R];
+                uint32_t code[TT][R];
+#pragma unroll
+                for (int q = 0; q < R; ++q) {
+                    yv[q] = y_int[row[q] * k + j];
+                    const uint4 c4 = codes4[row[q]];
+                    const uint32_t cc[4] = {c4.x, c4.y, c4.z, c4.w};
+#pragma unroll
+                    for (int t = 0; t < TT; ++t) code[t][q] = cc[t];
+                }
+                double val[TT][R];
+#pragma unroll
+                for (int t = 0; t < TT; ++t)
+#pragma unroll
+                    for (int q = 0; q < R; ++q) val[t][q] = x[(d.terms[t].offset + code[t][q]) * k + j];
+#pragma unroll
+                for (int q = 0; q < R; ++q) {
+                    if (!ok[q]) continue;
+                    double a = 0.0;
+#pragma unroll
+                    for (int t = 0; t < TT; ++t) a += val[t][q];
+                    tile[(rb + q * ngrp) * ld + j] = yv[q] - a;
+                }
+            }
+        __syncthreads();
+        for (uint32_t q = threadIdx.x; q < 64 * k; q += BLK) {
+            const uint32_t c = q >> 6, rr = q & 63;
+            const uint64_t r = r0 + rr;
+            if (r < n) Dm[r + c * ldd] = tile[rr * ld + c];
+        }
+        __syncthreads();
+    }
+}
+
 // X[dof + j*ldx] = x[dof*k + j]
 __global__ void k_store_x(uint64_t N, const double *__restrict__ x, double *__restrict__ X, uint64_t ldx, uint32_t k) {
     const uint64_t idx = (uint64_t)blockIdx.x * BLK + threadIdx.x;
@@ -2342,8 +2405,25 @@ class ChunkRun {
         hipStream_t st = st_;
         Workspace &ws = *ws_;
         double *dm_int = ws.u.as<double>();
-        const uint32_t gd = grid_for(n, kshift, 65535);
-        launch_demean(dv_, da, gd, ws.x.as<double>(), ws.yint.as<double>(), dm_int, k, kshift, st);
+        const bool true_res = opts_.true_residual != 0;
+        // without the true residual nothing needs y - D x in internal order: a plain design's
+        // demeaned output goes straight to caller order
+        const bool direct = !pk_ && Dm_ && !true_res && dv_.codes4.p && da.T >= 1 && da.T <= 4;
+        if (direct) {
+#define WX_DS(TT)                                                                                                \
+    hipLaunchKernelGGL((k_demean_store<TT>), dim3(gt_), dim3(BLK), tile_bytes_, st, da, dv_.codes4.as<uint4>(),   \
+                       inv_perm_, ws.x.as<double>(), ws.yint.as<double>(), Dm_ + c0_ * n, n, k, kshift)
+            switch (da.T) {
+            case 1: WX_DS(1); break;
+            case 2: WX_DS(2); break;
+            case 3: WX_DS(3); break;
+            default: WX_DS(4); break;
+            }
+#undef WX_DS
+        } else {
+            const uint32_t gd = grid_for(n, kshift, 65535);
+            launch_demean(dv_, da, gd, ws.x.as<double>(), ws.yint.as<double>(), dm_int, k, kshift, st);
+        }
         DevBuf xtab;
         if (pk_) {
             bool any_x = false;
@@ -2364,7 +2444,7 @@ class ChunkRun {
                                    xtab.as<double *>(), pk_->own_ld.as<uint64_t>(), pk_->own_kv.as<uint32_t>());
             }
         } else {
-            if (Dm_)
+            if (Dm_ && !direct)
                 hipLaunchKernelGGL(k_store_caller, dim3(gt_), dim3(BLK), tile_bytes_, st, n, inv_perm_, dm_int,
                                    Dm_ + c0_ * n, n, k, kshift, k);
             if (X_ && (!s_.reparam || x_solver_basis_)) {
@@ -2375,7 +2455,6 @@ class ChunkRun {
         }
         // residual numerator ||D'W(y - Dx)||; the denominator ||D'W y|| = beta_1 ||p~_1|| was
         // kept at initialisation
-        const bool true_res = opts_.true_residual != 0;
         if (true_res) {
             launch_adjoint(dv_, da, weighted_ ? 2 : 0, dm_int, ws.h.as<double>(), ws.segpart.as<double>(), nullptr,
                            k, kshift, st, -1, ws.pieces_or_null());
