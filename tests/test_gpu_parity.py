@@ -735,7 +735,17 @@ def test_device_cross_tabs_give_the_host_preconditioner(gpu, case):
             host = build_preconditioner(codes, w, cfg).to_bytes()
         finally:
             del os.environ["WITHIN_AMD_HOST_SETUP"]
-        assert dev == host
+        pd, ph = parse_precond(dev), parse_precond(host)
+        if not np.any((pd["fac_kind"] == 0) & (pd["nk"] > 1024)):
+            assert dev == host
+        else:
+            # kept blocks above 1,024 levels are pseudo-inverted on the GPU by the device build
+            # and on the host by the host build: equal to rounding; everything else bit for bit
+            for f in pd:
+                if f != "dense":
+                    assert np.array_equal(pd[f], ph[f]), f
+            scale = np.abs(ph["dense"]).max()
+            assert np.max(np.abs(pd["dense"] - ph["dense"])) <= 1e-9 * scale
     # and the Solver (which builds through the same path) solves with it
     r = wa.solve(codes, np.asfortranarray(rng.standard_normal(n)), weights=w)
     assert r.converged
@@ -762,3 +772,32 @@ def test_true_residual_can_be_skipped(gpu):
                 assert np.array_equal(a.x, b.x) and np.array_equal(a.demeaned, b.demeaned)
                 assert list(a.residual) == list(b.residual) and list(a.iterations) == list(b.iterations)
                 assert all(np.isfinite(a.true_residual)) and all(np.isnan(b.true_residual))
+
+
+def test_device_built_large_dense_block_applies_like_the_host_built_one(gpu, oracle):
+    """A 1,500-level kept block: its Schur complement and pseudo-inverse built on the GPU (row
+    kernel, the heavy eliminated levels' DGEMM, rocSOLVER) against the host build of the same
+    design (WITHIN_AMD_HOST_SETUP=1: host cross-tabs, host Schur complement and Cholesky).  The
+    two preconditioners share every subdomain, weight and cross block bit for bit, and their
+    applies, both by the oracle from the exported bytes, agree to 1e-11."""
+    from within_amd._within import build_preconditioner
+    rng = np.random.default_rng(91)
+    n = 60_000
+    pz = (np.arange(3000) + 1.0) ** -1.1
+    codes = np.asfortranarray(np.column_stack([rng.choice(3000, n, p=pz / pz.sum()),
+                                               rng.integers(0, 1500, n)]).astype(np.uint32))
+    w = rng.uniform(0.5, 2.0, n)
+    dev = build_preconditioner(codes, w).to_bytes()
+    os.environ["WITHIN_AMD_HOST_SETUP"] = "1"
+    try:
+        host = build_preconditioner(codes, w).to_bytes()
+    finally:
+        del os.environ["WITHIN_AMD_HOST_SETUP"]
+    pd, ph = parse_precond(dev), parse_precond(host)
+    assert int(pd["nk"].max()) == 1500
+    for f in ("slot_start", "ne", "nk", "fac_kind", "gauge", "glob", "pw", "fold", "inv_de", "c_ptr", "c_idx", "c_val"):
+        assert np.array_equal(pd[f], ph[f]), f
+    sd, sh = oracle_schwarz_from_bytes(oracle, dev), oracle_schwarz_from_bytes(oracle, host)
+    x = np.random.default_rng(3).standard_normal(int(pd["n"]))
+    yd, yh = sd.apply(x), sh.apply(x)
+    assert rel(yd, yh) <= 1e-11, rel(yd, yh)

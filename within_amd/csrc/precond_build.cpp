@@ -1196,7 +1196,9 @@ std::shared_ptr<Precond> assemble_additive(std::vector<Component> &comps, uint64
     // the sampled Schur complement's edge count sum_e (deg(e) - 1) over nk^2 / 2.  The choice
     // depends on the config and the component only; a visible GPU merely computes the large
     // dense blocks (`big`) instead of the host.
-    const bool have_gpu = gpu_available();
+    // WITHIN_AMD_HOST_SETUP=1 builds the large blocks on the host too (the device-independent
+    // reference the device build is tested against)
+    const bool have_gpu = gpu_available() && std::getenv("WITHIN_AMD_HOST_SETUP") == nullptr;
     std::vector<uint8_t> dense(nc), big(nc, 0);
     for (uint64_t c = 0; c < nc; ++c) {
         const uint64_t nk = comps[c].nk;
