@@ -103,7 +103,9 @@ void sample_clique_tree(std::vector<std::pair<uint32_t, double>> &nb, double W, 
     std::sort(nb.begin(), nb.end(), [](const auto &a, const auto &b) {
         return a.second < b.second || (a.second == b.second && a.first < b.first);
     });
-    std::vector<double> suffix(d + 1, 0.0);
+    thread_local std::vector<double> suffix; // reused across stars: no allocation per star
+    if (suffix.size() < d + 1) suffix.resize(2 * d + 2);
+    suffix[d] = 0.0;
     for (size_t i = d; i-- > 0;) suffix[i] = suffix[i + 1] + nb[i].second;
     const double inv_split = 1.0 / split;
     for (int rep = 0; rep < split; ++rep) {
@@ -128,11 +130,12 @@ void sample_clique_tree(std::vector<std::pair<uint32_t, double>> &nb, double W, 
 
 // Merges edges by (lo, hi), summing weights in a fixed (lo, hi, w) order.
 void sort_and_merge(std::vector<Edge> &e) {
-    std::sort(e.begin(), e.end(), [](const Edge &a, const Edge &b) {
+    auto less = [](const Edge &a, const Edge &b) {
         if (a.lo != b.lo) return a.lo < b.lo;
         if (a.hi != b.hi) return a.hi < b.hi;
         return a.w < b.w;
-    });
+    };
+    std::sort(e.begin(), e.end(), less);
     size_t wr = 0;
     for (size_t r = 0; r < e.size(); ++r) {
         if (wr > 0 && e[wr - 1].lo == e[r].lo && e[wr - 1].hi == e[r].hi) e[wr - 1].w += e[r].w;
@@ -1283,10 +1286,18 @@ std::shared_ptr<Precond> assemble_additive(std::vector<Component> &comps, uint64
             const Component cover = cm.signed_form ? make_cover(cm) : Component{};
             const Component &lf = cm.signed_form ? cover : cm;
             std::vector<Edge> edges;
+            const auto t0 = std::chrono::steady_clock::now();
             schur_edges(lf, cfg, (uint64_t)c, edges);
+            const auto t1 = std::chrono::steady_clock::now();
             const uint32_t n_ac = lf.nk + (lf.grounded ? 1 : 0);
             acs[c] = approx_cholesky(n_ac, edges, mix_seed(cfg.approx_chol_seed, c, 0xC401),
                                      cfg.split_merge > 0 ? cfg.split_merge : 1, lf.grounded ? (int64_t)lf.nk : -1);
+            static const bool verbose_factor = std::getenv("WITHIN_AMD_VERBOSE") != nullptr;
+            if (verbose_factor && lf.ne + lf.nk > 10000)
+                std::fprintf(stderr, "[within_amd] component %lld: %u eliminated, %u kept, %zu Schur edges: "
+                             "sampling %.1f ms, approximate Cholesky %.1f ms\n", (long long)c, lf.ne, lf.nk,
+                             edges.size(), std::chrono::duration<double, std::milli>(t1 - t0).count(),
+                             std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t1).count());
             if (lf.grounded) { // drop the ground vertex, eliminated last
                 AcFactor &f = acs[c];
                 f.order.resize(lf.nk);
@@ -1375,6 +1386,7 @@ std::shared_ptr<Precond> assemble_additive(std::vector<Component> &comps, uint64
         const uint64_t base = S->slot_start[c];
         const uint32_t ne = cm.ne, nk = cm.nk;
         const bool par = cm.cidx.size() + ne + nk >= (1u << 16);
+        (void)par;
         const bool relabelled = S->fac_kind[c] == 1;
         std::vector<uint32_t> relabel(nk); // old kept index -> new
         if (relabelled)
