@@ -81,6 +81,7 @@ struct SchwarzDev {
     DevBuf dtiles, dcomps, drow_start;
     uint32_t n_dtiles = 0, n_dcomps = 0;
     uint64_t dense_part_rows = 0;
+    uint64_t dense_tiled_rows = 0; // kept rows of the tiled dense blocks (sk_dense_reduce's outputs)
     // sparse kept blocks (one workgroup per component): levels and factor rows
     DevBuf sp_comp, sp_lvl_f_ptr, sp_lvl_b_ptr, sp_lvl_f_rows, sp_lvl_b_rows, sp_meta;
     DevBuf sp_lvl_f_long, sp_lvl_b_long; // per level: first long row
@@ -945,7 +946,17 @@ __global__ __launch_bounds__(BLK) void sk_prolong(uint64_t N, const uint64_t *__
     const uint64_t stride = ((uint64_t)gridDim.x * BLK) >> kshift;
     for (uint64_t d = ((uint64_t)blockIdx.x * BLK + threadIdx.x) >> kshift; d < N; d += stride) {
         double acc = 0.0;
-        for (uint64_t q = d_ptr[d]; q < d_ptr[d + 1]; ++q) {
+        const uint64_t q1 = d_ptr[d + 1];
+        uint64_t q = d_ptr[d];
+        for (; q + 2 <= q1; q += 2) { // two slots' loads in flight, added in slot order
+            const uint64_t s0 = d_slot[q], s1 = d_slot[q + 1];
+            const double w0 = (double)sgnpw[s0], w1 = (double)sgnpw[s1];
+            const double x0 = (double)xs[s0 * k + j], x1 = (double)xs[s1 * k + j];
+            const double m0 = mu3[(uint64_t)slot_comp[s0] * k + j], m1 = mu3[(uint64_t)slot_comp[s1] * k + j];
+            acc += w0 * (x0 - m0);
+            acc += w1 * (x1 - m1);
+        }
+        for (; q < q1; ++q) {
             const uint64_t s = d_slot[q];
             acc += (double)sgnpw[s] * ((double)xs[s * k + j] - mu3[(uint64_t)slot_comp[s] * k + j]);
         }
@@ -1153,6 +1164,7 @@ std::shared_ptr<SchwarzDev<CT>> build_dev(const Precond::Schwarz &S) {
     D->dcomps.upload(dcomps.data(), dcomps.size());
     D->drow_start.upload(drow_start.data(), drow_start.size());
     D->dense_part_rows = part_rows;
+    D->dense_tiled_rows = drow_start.back();
     D->n_drows = (uint32_t)drow_off.size();
     D->drow_off.upload(drow_off.data(), drow_off.size());
     D->drow_base.upload(drow_base.data(), drow_base.size());
@@ -1669,7 +1681,7 @@ struct SchwarzRunImpl : SchwarzRun {
 #undef WX_DS
                 c0 += kc;
             }
-            hipLaunchKernelGGL((sk_dense_reduce<CT>), dim3(grid_rows(d.dense_part_rows, kshift)), dim3(BLK), 0, st,
+            hipLaunchKernelGGL((sk_dense_reduce<CT>), dim3(grid_rows(d.dense_tiled_rows, kshift)), dim3(BLK), 0, st,
                                dp<DenseComp>(d.dcomps), d.n_dcomps, dp<uint64_t>(d.drow_start), dp<CT>(dpart), xsp,
                                k, kshift, gate);
         }
