@@ -67,9 +67,9 @@ struct SchwarzDev {
     DevBuf glob, sgnpw, inv_de, slot_comp;
     // C' over kept rows (entry index = eliminated slot) and C over eliminated rows (entry = kept slot)
     DevBuf ct_idx, ct_val, ct_items, ct_fix;
-    uint32_t n_ct_items = 0, n_ct_fix = 0, n_ct_part = 0;
+    uint32_t n_ct_items = 0, n_ct_fix = 0, n_ct_part = 0, n_ct_fix_small = 0;
     DevBuf c_idx, c_val, c_items, c_fix;
-    uint32_t n_c_items = 0, n_c_fix = 0, n_c_part = 0;
+    uint32_t n_c_items = 0, n_c_fix = 0, n_c_part = 0, n_c_fix_small = 0;
     // means: chunks over all slots (per component) and over kept slots of sparse components
     DevBuf all_chunks, all_cptr, kept_chunks, kept_cptr;
     uint32_t n_all_chunks = 0, n_kept_chunks = 0;
@@ -181,8 +181,16 @@ __global__ __launch_bounds__(BLK) void sk_restrict(uint64_t ns, const uint32_t *
     const uint32_t kp = 1u << kshift, j = threadIdx.x & (kp - 1);
     if (j >= k) return;
     const uint64_t stride = ((uint64_t)gridDim.x * BLK) >> kshift;
-    for (uint64_t s = ((uint64_t)blockIdx.x * BLK + threadIdx.x) >> kshift; s < ns; s += stride)
-        loc[s * k + j] = (CT)((double)sgnpw[s] * r[(uint64_t)glob[s] * k + j]);
+    uint64_t s = ((uint64_t)blockIdx.x * BLK + threadIdx.x) >> kshift;
+    for (; s + stride < ns; s += 2 * stride) { // two slots' loads in flight
+        const uint64_t s1 = s + stride;
+        const uint64_t g0 = glob[s], g1 = glob[s1];
+        const double w0 = (double)sgnpw[s], w1 = (double)sgnpw[s1];
+        const double r0 = r[g0 * k + j], r1 = r[g1 * k + j];
+        loc[s * k + j] = (CT)(w0 * r0);
+        loc[s1 * k + j] = (CT)(w1 * r1);
+    }
+    if (s < ns) loc[s * k + j] = (CT)((double)sgnpw[s] * r[(uint64_t)glob[s] * k + j]);
 }
 
 // Mean of component c over its chunk partials, by the whole workgroup: row groups stride over
@@ -522,6 +530,40 @@ __global__ __launch_bounds__(BLK) void sk_rows_fix(const RowFix *__restrict__ fi
         if (threadIdx.x < kp && j < k)
             row_finish<CT, MODE>(f.row, sh[threadIdx.x], loc, out, inv_de, mu + (uint64_t)slot_comp[f.row] * k, k, j);
         __syncthreads();
+    }
+}
+
+// sk_rows_fix for split rows of at most FIX_SMALL partials, one row group per row instead of
+// one workgroup.  The workgroup version gives row group g the value 0.0 + partial g and then
+// adds group g + s into group g for s = ngrp/2, ..., 1; for at most FIX_SMALL <= ngrp partials
+// the levels above the partial count only add zeros, so this applies the remaining levels to
+// the same values and returns the same bits.
+constexpr uint32_t FIX_SMALL = 8;
+template <typename CT, int MODE>
+__global__ __launch_bounds__(BLK) void sk_rows_fix_small(const RowFix *__restrict__ fixes, uint32_t n_fix,
+                                                         const uint32_t *__restrict__ slot_comp,
+                                                         const CT *__restrict__ loc, const CT *__restrict__ inv_de,
+                                                         const double *__restrict__ mu, const CT *__restrict__ part,
+                                                         OutT<CT, MODE> *__restrict__ out, uint32_t k,
+                                                         uint32_t kshift, const int32_t *__restrict__ gate) {
+    if (gate_closed(gate, k)) return;
+    const uint32_t kp = 1u << kshift, j = threadIdx.x & (kp - 1);
+    if (j >= k) return;
+    const uint32_t stride = (gridDim.x * BLK) >> kshift;
+    for (uint32_t fi = (blockIdx.x * BLK + threadIdx.x) >> kshift; fi < n_fix; fi += stride) {
+        const RowFix f = fixes[fi];
+        double v[FIX_SMALL];
+#pragma unroll
+        for (uint32_t g = 0; g < FIX_SMALL; ++g)
+            v[g] = g < f.count ? 0.0 + (double)part[(f.first + g) * k + j] : 0.0;
+        uint32_t w = 1;
+        while (w < f.count) w <<= 1;
+#pragma unroll
+        for (uint32_t h = FIX_SMALL / 2; h >= 1; h >>= 1)
+            if (h < w)
+#pragma unroll
+                for (uint32_t g = 0; g < h; ++g) v[g] += v[g + h];
+        row_finish<CT, MODE>(f.row, v[0], loc, out, inv_de, mu + (uint64_t)slot_comp[f.row] * k, k, j);
     }
 }
 
@@ -1072,9 +1114,15 @@ std::shared_ptr<SchwarzDev<CT>> build_dev(const Precond::Schwarz &S) {
     {
         std::vector<RowItem> items;
         std::vector<RowFix> fixes;
+        // split rows with few partials first (a row group each), then the rest (a workgroup each)
+        auto small_first = [](std::vector<RowFix> &fx) {
+            std::stable_partition(fx.begin(), fx.end(), [](const RowFix &f) { return f.count <= FIX_SMALL; });
+            return (uint32_t)std::count_if(fx.begin(), fx.end(), [](const RowFix &f) { return f.count <= FIX_SMALL; });
+        };
         make_items(c_rows, c_b, c_e, items, fixes, D->n_c_part);
         D->c_items.upload(items.data(), items.size());
         D->n_c_items = (uint32_t)items.size();
+        D->n_c_fix_small = small_first(fixes);
         D->c_fix.upload(fixes.data(), fixes.size());
         D->n_c_fix = (uint32_t)fixes.size();
         items.clear();
@@ -1082,6 +1130,7 @@ std::shared_ptr<SchwarzDev<CT>> build_dev(const Precond::Schwarz &S) {
         make_items(ct_rows, ct_b, ct_e, items, fixes, D->n_ct_part);
         D->ct_items.upload(items.data(), items.size());
         D->n_ct_items = (uint32_t)items.size();
+        D->n_ct_fix_small = small_first(fixes);
         D->ct_fix.upload(fixes.data(), fixes.size());
         D->n_ct_fix = (uint32_t)fixes.size();
     }
@@ -1584,6 +1633,23 @@ struct SchwarzRunImpl : SchwarzRun {
         }
     }
 
+    // split rows: the ones with at most FIX_SMALL partials (stored first) a row group each when a
+    // workgroup holds at least FIX_SMALL row groups, the rest a workgroup each
+    template <int MODE>
+    void fix_rows(const DevBuf &fixes, uint32_t n_fix, uint32_t n_small, CT *locp, const double *mu,
+                  OutT<CT, MODE> *out, uint32_t k, uint32_t kshift, hipStream_t st, const int32_t *gate) {
+        const SchwarzDev<CT> &d = *D;
+        if ((BLK >> kshift) < FIX_SMALL) n_small = 0;
+        if (n_small)
+            hipLaunchKernelGGL((sk_rows_fix_small<CT, MODE>), dim3(grid_rows(n_small, kshift)), dim3(BLK), 0, st,
+                               dp<RowFix>(fixes), n_small, dp<uint32_t>(d.slot_comp), locp, dp<CT>(d.inv_de), mu,
+                               dp<CT>(rowpart), out, k, kshift, gate);
+        if (n_fix > n_small)
+            hipLaunchKernelGGL((sk_rows_fix<CT, MODE>), dim3(std::min<uint32_t>(n_fix - n_small, 65535)), dim3(BLK), 0,
+                               st, dp<RowFix>(fixes) + n_small, n_fix - n_small, dp<uint32_t>(d.slot_comp), locp,
+                               dp<CT>(d.inv_de), mu, dp<CT>(rowpart), out, k, kshift, gate);
+    }
+
     template <typename VT>
     void mean(const DevBuf &chunks, uint32_t n_chunks, const DevBuf &cptr, const DevBuf &count, const VT *src,
               double *mu, uint32_t k, uint32_t kshift, hipStream_t st, const int32_t *gate) {
@@ -1598,15 +1664,12 @@ struct SchwarzRunImpl : SchwarzRun {
         const uint32_t kshift = pow2_shift(k);
         const SchwarzDev<CT> &d = *D;
         CT *locp = dp<CT>(loc), *redp = dp<CT>(red), *xsp = dp<CT>(xs);
-        hipLaunchKernelGGL((sk_restrict<CT>), dim3(grid_rows(d.ns, kshift)), dim3(BLK), 0, st, d.ns,
+        hipLaunchKernelGGL((sk_restrict<CT>), dim3(grid_rows((d.ns + 1) / 2, kshift)), dim3(BLK), 0, st, d.ns,
                            dp<uint32_t>(d.glob), dp<CT>(d.sgnpw), r, locp, k, kshift, gate);
         mean(d.all_chunks, d.n_all_chunks, d.all_cptr, d.comp_count, locp, dp<double>(mu1), k, kshift, st, gate);
         // reduced right-hand side on kept rows
         rows_spmv<0>(d.ct_items, d.n_ct_items, d.ct_idx, d.ct_val, locp, xsp, redp, k, kshift, st, gate);
-        if (d.n_ct_fix)
-            hipLaunchKernelGGL((sk_rows_fix<CT, 0>), dim3(std::min<uint32_t>(d.n_ct_fix, 65535)), dim3(BLK), 0, st,
-                               dp<RowFix>(d.ct_fix), d.n_ct_fix, dp<uint32_t>(d.slot_comp), locp, dp<CT>(d.inv_de),
-                               dp<double>(mu1), dp<CT>(rowpart), redp, k, kshift, gate);
+        fix_rows<0>(d.ct_fix, d.n_ct_fix, d.n_ct_fix_small, locp, dp<double>(mu1), redp, k, kshift, st, gate);
         // kept-block solves
         if (d.n_sparse) {
             mean(d.kept_chunks, d.n_kept_chunks, d.kept_cptr, d.comp_kcount, redp, dp<double>(mu2), k, kshift, st, gate);
@@ -1691,10 +1754,7 @@ struct SchwarzRunImpl : SchwarzRun {
                                dp<uint32_t>(d.drow_slot), dp<CT>(d.dmat), redp, xsp, k, kshift, gate);
         // back-substitution on eliminated rows
         rows_spmv<1>(d.c_items, d.n_c_items, d.c_idx, d.c_val, locp, xsp, xsp, k, kshift, st, gate);
-        if (d.n_c_fix)
-            hipLaunchKernelGGL((sk_rows_fix<CT, 1>), dim3(std::min<uint32_t>(d.n_c_fix, 65535)), dim3(BLK), 0, st,
-                               dp<RowFix>(d.c_fix), d.n_c_fix, dp<uint32_t>(d.slot_comp), locp, dp<CT>(d.inv_de),
-                               dp<double>(mu1), dp<CT>(rowpart), xsp, k, kshift, gate);
+        fix_rows<1>(d.c_fix, d.n_c_fix, d.n_c_fix_small, locp, dp<double>(mu1), xsp, k, kshift, st, gate);
         mean(d.all_chunks, d.n_all_chunks, d.all_cptr, d.comp_count, xsp, dp<double>(mu3), k, kshift, st, gate);
         hipLaunchKernelGGL((sk_prolong<CT>), dim3(grid_rows(d.N, kshift)), dim3(BLK), 0, st, d.N,
                            dp<uint64_t>(d.d_ptr), dp<uint32_t>(d.d_slot), dp<CT>(d.sgnpw),
