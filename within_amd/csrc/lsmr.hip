@@ -1618,6 +1618,76 @@ bool build_pair_plan(const Design &d, const std::vector<std::vector<uint64_t>> &
     dv.max_partials = std::max<uint32_t>(dv.max_partials, (uint32_t)slots);
     return true;
 }
+
+std::vector<SegItem> download_items(const DevBuf &b, uint32_t n) {
+    std::vector<SegItem> h(n);
+    if (n) WX_HIP(hipMemcpy(h.data(), b.p, (size_t)n * sizeof(SegItem), hipMemcpyDeviceToHost));
+    return h;
+}
+
+// `in`'s items in absolute terms: positions in region `src`, partial slots in region `dst`,
+// output rows at `offset` (complete items name their output row as their destination)
+void append_absolute(std::vector<SegItem> &out, const std::vector<SegItem> &in, uint64_t src, uint64_t dst,
+                     uint64_t offset) {
+    for (SegItem it : in) {
+        it.begin += (uint32_t)src;
+        it.end += (uint32_t)src;
+        it.level += (uint32_t)offset;
+        if (it.dest >= 0) it.dest = (int32_t)it.level;
+        else if (it.dest != SEG_SKIP) it.dest = -(int32_t)(dst + (uint64_t)(-(int64_t)it.dest - 1)) - 1;
+        out.push_back(it);
+    }
+}
+
+// The merged fix-up launches (MergedFix) of a plain design of exactly the fused sorted term S and
+// the fused pair (P, Q); off when the slots or the output rows would not fit the items' 31 bits.
+void build_merged_fix(const Design &d, DevDesign &dv) {
+    dv.mfix = MergedFix{};
+    const PairPlan &pp = dv.pair;
+    if (!dv.plain || d.terms.size() != 3 || pp.p < 0 || dv.fuse_term < 0 || dv.fuse_term == pp.p ||
+        dv.fuse_term == pp.q)
+        return;
+    const DevTerm &S = dv.terms[dv.fuse_term], &P = dv.terms[pp.p], &Q = pp.qt;
+    auto slots_of = [](const DevTerm &t) {
+        return t.stages.empty() ? (uint64_t)t.n_partials : t.stages.back().dst_base;
+    };
+    // every stage's output region follows its input region, so a term's slots end where its last
+    // stage's output region starts (that stage writes complete levels only)
+    const uint64_t s_slots = std::max<uint64_t>(slots_of(S), S.n_partials);
+    const uint64_t p_slots = std::max<uint64_t>(slots_of(P), P.n_partials);
+    const uint64_t q_slots = std::max<uint64_t>(slots_of(Q), Q.n_partials);
+    MergedFix m;
+    m.base_p = s_slots;
+    m.base_pieces = m.base_p + p_slots;
+    const uint64_t base_q = m.base_pieces + pp.n_pieces;
+    m.slots = base_q + q_slots;
+    if (m.slots >= (1ull << 31) || dv.N >= (1ull << 31)) return;
+    const size_t L = std::max({S.stages.size(), P.stages.size(), Q.stages.size() + 1});
+    for (size_t lv = 0; lv < L; ++lv) {
+        std::vector<SegItem> items;
+        if (lv < S.stages.size()) {
+            const DevTerm::FixStage &g = S.stages[lv];
+            append_absolute(items, download_items(g.items, g.n_items), g.src_base, g.dst_base, S.info.offset);
+        }
+        if (lv < P.stages.size()) {
+            const DevTerm::FixStage &g = P.stages[lv];
+            append_absolute(items, download_items(g.items, g.n_items), m.base_p + g.src_base, m.base_p + g.dst_base,
+                            P.info.offset);
+        }
+        if (lv == 0) {
+            append_absolute(items, download_items(Q.items, Q.n_items), m.base_pieces, base_q, Q.info.offset);
+        } else if (lv - 1 < Q.stages.size()) {
+            const DevTerm::FixStage &g = Q.stages[lv - 1];
+            append_absolute(items, download_items(g.items, g.n_items), base_q + g.src_base, base_q + g.dst_base,
+                            Q.info.offset);
+        }
+        m.levels.emplace_back();
+        m.levels.back().upload(items.data(), items.size());
+        m.n_items.push_back((uint32_t)items.size());
+    }
+    dv.max_partials = std::max<uint32_t>(dv.max_partials, (uint32_t)m.slots);
+    dv.mfix = std::move(m);
+}
 } // namespace
 
 void upload_design(Solver &s, const std::vector<double> &weights_internal, uint32_t tile) {
@@ -1742,6 +1812,7 @@ void upload_design(Solver &s, const std::vector<double> &weights_internal, uint3
         ft.gitems.upload(g.data(), g.size());
         ft.n_gitems = (uint32_t)g.size();
     }
+    build_merged_fix(d, dv);
     phase("code records and gather items");
 }
 
@@ -1822,6 +1893,13 @@ void reweight_design(Solver &s, const Solver &base, const std::vector<double> &w
     clone_buf(dv.pair.item_piece0, b.pair.item_piece0);
     clone_buf(dv.pair.piece_slot, b.pair.piece_slot);
     clone_term_plan(dv.pair.qt, b.pair.qt);
+    dv.mfix = MergedFix{};
+    dv.mfix.base_p = b.mfix.base_p;
+    dv.mfix.base_pieces = b.mfix.base_pieces;
+    dv.mfix.slots = b.mfix.slots;
+    dv.mfix.n_items = b.mfix.n_items;
+    dv.mfix.levels.resize(b.mfix.levels.size());
+    for (size_t q = 0; q < b.mfix.levels.size(); ++q) clone_buf(dv.mfix.levels[q], b.mfix.levels[q]);
     for (size_t t = 0; t < b.terms.size(); ++t) {
         const DevTerm &bt = b.terms[t];
         DevTerm &dt = dv.terms[t];
@@ -1946,6 +2024,43 @@ void launch_adjoint(const DevDesign &dv, const DesignArgs &da, int which, const 
 #undef WX_SEG_NC
 #undef WX_SEG
         launch_segfix(dt, partials, out, active, k, kshift, st);
+    }
+    WX_HIP(hipGetLastError());
+}
+
+// The fused pair's D' and the merged fix-ups (MergedFix) after the fused gather has left the
+// sorted term's level sums in `out` and its partials at the start of `partials`; P's pass writes
+// its partials and the pieces into their regions of the same buffer.
+void launch_adjoint_merged(const DevDesign &dv, int which, const double *src, double *out, double *partials,
+                           const int32_t *active, uint32_t k, uint32_t kshift, hipStream_t st) {
+    const PairPlan &pp = dv.pair;
+    const MergedFix &m = dv.mfix;
+    const DevTerm &dt = dv.terms[pp.p];
+    const uint32_t groups_per_block = BLK >> kshift;
+    const double *f = which == 1 ? dt.sw_pos.as<double>() : which == 2 ? dt.w_pos.as<double>() : nullptr;
+    const uint32_t grid2 = std::max<uint32_t>(
+        1, std::min<uint32_t>((dt.n_items + 2 * groups_per_block - 1) / (2 * groups_per_block) * 2, 65535u * 8));
+    double *pparts = partials + m.base_p * k, *pieces = partials + m.base_pieces * k;
+    if (f)
+        hipLaunchKernelGGL((k_segreduce2_pair<true>), dim3(grid2), dim3(BLK), 0, st, dt.info, dt.perm.as<uint32_t>(),
+                           dt.items.as<SegItem>(), dt.n_items, pp.item_piece0.as<uint32_t>(),
+                           pp.piece_slot.as<uint32_t>(), pp.qcode_pos.as<uint32_t>(), f, src, out, pparts, pieces,
+                           active, k, kshift);
+    else
+        hipLaunchKernelGGL((k_segreduce2_pair<false>), dim3(grid2), dim3(BLK), 0, st, dt.info,
+                           dt.perm.as<uint32_t>(), dt.items.as<SegItem>(), dt.n_items,
+                           pp.item_piece0.as<uint32_t>(), pp.piece_slot.as<uint32_t>(),
+                           pp.qcode_pos.as<uint32_t>(), f, src, out, pparts, pieces, active, k, kshift);
+    TermInfo z = dt.info; // absolute output rows, plain sums
+    z.offset = 0;
+    z.load[0] = -1;
+    for (size_t lv = 0; lv < m.levels.size(); ++lv) {
+        const uint32_t n = m.n_items[lv];
+        if (!n) continue;
+        const uint32_t g =
+            std::max<uint32_t>(1, std::min<uint32_t>((n + 2 * groups_per_block - 1) / (2 * groups_per_block), 65535u * 8));
+        hipLaunchKernelGGL((k_segreduce2<false>), dim3(g), dim3(BLK), 0, st, z, nullptr, m.levels[lv].as<SegItem>(), n,
+                           nullptr, partials, out, partials, active, k, kshift);
     }
     WX_HIP(hipGetLastError());
 }
@@ -2583,12 +2698,18 @@ class ChunkRun {
         if (timed) timer_->end(0);
         hipLaunchKernelGGL(k_beta, dim3(ncol_), dim3(BLK), 0, st, part_, gpart, kp_, cs_, k, rg_item_);
         if (timed) timer_->begin(1);
-        // split levels of the fused term first: the other terms reuse the partial buffer
-        if (dv_.fuse_term >= 0)
-            launch_segfix(dv_.terms[dv_.fuse_term], ws.segpart.as<double>(), ws.atu.as<double>(), gate(), k, kshift,
-                          st);
-        launch_adjoint(dv_, da_, weighted_ ? 1 : 0, ws.u.as<double>(), ws.atu.as<double>(), ws.segpart.as<double>(),
-                       gate(), k, kshift, st, dv_.fuse_term, ws.pieces_or_null());
+        if (dv_.fuse_term >= 0 && dv_.mfix.on() && k >= 2 && (k & 1) == 0) {
+            // the fused pair, then every term's split levels in merged launches
+            launch_adjoint_merged(dv_, weighted_ ? 1 : 0, ws.u.as<double>(), ws.atu.as<double>(),
+                                  ws.segpart.as<double>(), gate(), k, kshift, st);
+        } else {
+            // split levels of the fused term first: the other terms reuse the partial buffer
+            if (dv_.fuse_term >= 0)
+                launch_segfix(dv_.terms[dv_.fuse_term], ws.segpart.as<double>(), ws.atu.as<double>(), gate(), k,
+                              kshift, st);
+            launch_adjoint(dv_, da_, weighted_ ? 1 : 0, ws.u.as<double>(), ws.atu.as<double>(),
+                           ws.segpart.as<double>(), gate(), k, kshift, st, dv_.fuse_term, ws.pieces_or_null());
+        }
         if (timed) timer_->end(1);
         if (prec_kind_ == 2) {
             hipLaunchKernelGGL((k_ptilde<2, false>), dim3(gN_), dim3(BLK), 0, st, N, ws.atu.as<double>(),

@@ -88,6 +88,18 @@ struct PairPlan {
     DevTerm qt;         // Q's reduction over the stored pieces: items over Q's levels, fixups
 };
 
+// The fix-ups of a three-term plain design's D' after the fused gather and the fused pair,
+// merged across the terms: launch d runs the sorted term's and P's combine stage d together
+// with Q's reduction over the pieces (d = 0) or Q's stage d - 1.  Every item is addressed in one
+// partial-slot buffer (the sorted term's slots first, then P's, the pieces and Q's) and names its
+// absolute output row (lsmr.hip, build_merged_fix).
+struct MergedFix {
+    std::vector<DevBuf> levels; // SegItem[] per launch
+    std::vector<uint32_t> n_items;
+    uint64_t base_p = 0, base_pieces = 0, slots = 0;
+    bool on() const { return !levels.empty(); }
+};
+
 struct DevDesign {
     uint64_t n = 0, N = 0;
     DevBuf codes, slopes, sw, w, obs_perm, inv_perm, inv_diag;
@@ -99,6 +111,7 @@ struct DevDesign {
     bool plain = false; // every term is a single intercept column
     int32_t fuse_term = -1; // sorted single-intercept term whose D' is fused into the gather
     PairPlan pair;          // p < 0: no fused pair
+    MergedFix mfix;         // off unless the design is the sorted term plus a fused pair
     DesignArgs args() const;
 };
 
