@@ -574,8 +574,8 @@ __global__ __launch_bounds__(BLK) void sk_rows_fix_small(const RowFix *__restric
 // lane (a wave-uniform load).  Each lane keeps KC FP64 accumulators.  Segment partials are
 // summed in a fixed order by sk_dense_reduce, so the result is deterministic.
 struct DenseTile {
-    uint64_t mat_off;   // S^+ of the component (row-major nk x nk, symmetric)
-    uint32_t nk, kept_base;
+    uint64_t mat_off;   // S^+ of the component (row-major nk x nk, symmetric, rows ld apart)
+    uint32_t nk, ld, kept_base;
     uint32_t row0, l0, l1;
     uint32_t seg;       // segment index of [l0, l1)
     uint64_t part_off;  // partial rows of this component: part[(part_off + seg*nk + i)*k + c]
@@ -605,7 +605,7 @@ __global__ __launch_bounds__(64) void sk_dense_sym(const DenseTile *__restrict__
     for (; l + 8 <= T.l1; l += 8) { // eight coefficient loads in flight per lane
         CT m[8];
 #pragma unroll
-        for (int q = 0; q < 8; ++q) m[q] = M[(uint64_t)(l + q) * T.nk];
+        for (int q = 0; q < 8; ++q) m[q] = M[(uint64_t)(l + q) * T.ld];
 #pragma unroll
         for (int q = 0; q < 8; ++q) {
             const CT *rl = R + (uint64_t)(l + q) * kstride;
@@ -614,7 +614,7 @@ __global__ __launch_bounds__(64) void sk_dense_sym(const DenseTile *__restrict__
         }
     }
     for (; l < T.l1; ++l) {
-        const CT m = M[(uint64_t)l * T.nk];
+        const CT m = M[(uint64_t)l * T.ld];
         const CT *rl = R + (uint64_t)l * kstride;
 #pragma unroll
         for (int c = 0; c < KC; ++c) acc[c] += m * rl[c];
@@ -1176,6 +1176,27 @@ std::shared_ptr<SchwarzDev<CT>> build_dev(const Precond::Schwarz &S) {
     uint64_t row_tiles = 0;
     for (uint64_t c = 0; c < nc; ++c)
         if (S.fac_kind[c] == 0 && S.nk[c] >= DENSE_TILED_MIN) row_tiles += (S.nk[c] + 63) / 64;
+    // device layout of the dense operators: a tiled block's rows get a leading dimension padded
+    // to a multiple of 64 coefficients and the block a 64-coefficient-aligned start, so each
+    // wave's load of 64 consecutive coefficients covers whole 128-byte lines; the other blocks
+    // stay packed in the host order
+    std::vector<uint32_t> dorder; // dense components by host offset
+    for (uint64_t c = 0; c < nc; ++c)
+        if (S.fac_kind[c] == 0) dorder.push_back((uint32_t)c);
+    std::sort(dorder.begin(), dorder.end(), [&](uint32_t a, uint32_t b) { return S.dense_off[a] < S.dense_off[b]; });
+    std::vector<uint64_t> doff(nc, 0);
+    std::vector<uint32_t> dld(nc, 0);
+    uint64_t dtotal = 0;
+    bool relayout = false;
+    for (uint32_t c : dorder) {
+        const uint32_t nk = S.nk[c];
+        const bool tiled = nk >= DENSE_TILED_MIN;
+        dld[c] = tiled ? (nk + 63) / 64 * 64 : nk;
+        if (tiled) dtotal = (dtotal + 63) / 64 * 64;
+        doff[c] = dtotal;
+        dtotal += (uint64_t)nk * dld[c];
+        relayout = relayout || doff[c] != S.dense_off[c] || dld[c] != nk;
+    }
     std::vector<DenseTile> tiles;
     uint64_t part_rows = 0;
     for (uint64_t c = 0; c < nc; ++c) {
@@ -1191,7 +1212,7 @@ std::shared_ptr<SchwarzDev<CT>> build_dev(const Precond::Schwarz &S) {
             DenseComp dc{nk, kb, 0, 0, part_rows};
             for (uint32_t s = 0; s * seg_len < nk; ++s) {
                 for (uint32_t r0 = 0; r0 < nk; r0 += 64)
-                    tiles.push_back({S.dense_off[c], nk, kb, r0, s * seg_len, std::min(nk, (s + 1) * seg_len), s,
+                    tiles.push_back({doff[c], nk, dld[c], kb, r0, s * seg_len, std::min(nk, (s + 1) * seg_len), s,
                                      part_rows});
                 dc.n_seg++;
             }
@@ -1201,7 +1222,7 @@ std::shared_ptr<SchwarzDev<CT>> build_dev(const Precond::Schwarz &S) {
             continue;
         }
         for (uint32_t i = 0; i < nk; ++i) {
-            drow_off.push_back(S.dense_off[c] + (uint64_t)i * nk);
+            drow_off.push_back(doff[c] + (uint64_t)i * nk);
             drow_base.push_back(kb);
             drow_nk.push_back(nk);
             drow_slot.push_back(kb + i);
@@ -1223,7 +1244,8 @@ std::shared_ptr<SchwarzDev<CT>> build_dev(const Precond::Schwarz &S) {
     // converted) on the device
     {
         const uint64_t total = S.dense.size();
-        D->dmat.alloc(total * sizeof(CT));
+        DevBuf raw; // the host layout
+        raw.alloc(total * sizeof(CT));
         std::vector<std::pair<uint64_t, uint64_t>> dev_ranges;
         for (const auto &b : S.dense_dev) dev_ranges.push_back({b.off, b.count});
         std::sort(dev_ranges.begin(), dev_ranges.end());
@@ -1231,13 +1253,13 @@ std::shared_ptr<SchwarzDev<CT>> build_dev(const Precond::Schwarz &S) {
         auto upload_host = [&](uint64_t from, uint64_t to) {
             if (to <= from) return;
             if constexpr (std::is_same<CT, double>::value) {
-                WX_HIP(hipMemcpy(D->dmat.template as<CT>() + from, S.dense.data() + from, (to - from) * 8,
+                WX_HIP(hipMemcpy(raw.template as<CT>() + from, S.dense.data() + from, (to - from) * 8,
                                  hipMemcpyHostToDevice));
             } else {
                 std::vector<CT> v(to - from);
 #pragma omp parallel for schedule(static) if (to - from > (1u << 20))
                 for (int64_t i = 0; i < (int64_t)(to - from); ++i) v[i] = (CT)S.dense[from + i];
-                WX_HIP(hipMemcpy(D->dmat.template as<CT>() + from, v.data(), v.size() * sizeof(CT),
+                WX_HIP(hipMemcpy(raw.template as<CT>() + from, v.data(), v.size() * sizeof(CT),
                                  hipMemcpyHostToDevice));
             }
         };
@@ -1248,15 +1270,44 @@ std::shared_ptr<SchwarzDev<CT>> build_dev(const Precond::Schwarz &S) {
         upload_host(at, total);
         for (const auto &b : S.dense_dev) {
             if constexpr (std::is_same<CT, double>::value) {
-                WX_HIP(hipMemcpy(D->dmat.template as<CT>() + b.off, b.dev.get(), b.count * 8, hipMemcpyDeviceToDevice));
+                WX_HIP(hipMemcpy(raw.template as<CT>() + b.off, b.dev.get(), b.count * 8, hipMemcpyDeviceToDevice));
             } else {
                 const uint32_t g = (uint32_t)std::min<uint64_t>((b.count + 255) / 256, 1u << 20);
                 hipLaunchKernelGGL(sk_cast_f32, dim3(g), dim3(256), 0, 0, b.count, b.dev.get(),
-                                   D->dmat.template as<CT>() + b.off);
+                                   raw.template as<CT>() + b.off);
                 WX_HIP(hipGetLastError());
             }
         }
         WX_HIP(hipDeviceSynchronize());
+        if (!relayout) {
+            D->dmat = std::move(raw);
+        } else {
+            // packed runs copied whole, tiled blocks row by row into their padded rows
+            D->dmat.alloc(std::max<uint64_t>(dtotal, 1) * sizeof(CT));
+            CT *dst = D->dmat.template as<CT>();
+            const CT *src = raw.template as<CT>();
+            for (size_t q = 0; q < dorder.size();) {
+                const uint32_t c = dorder[q];
+                const uint64_t nk = S.nk[c];
+                if (dld[c] != nk) {
+                    WX_HIP(hipMemcpy2D(dst + doff[c], dld[c] * sizeof(CT), src + S.dense_off[c], nk * sizeof(CT),
+                                       nk * sizeof(CT), nk, hipMemcpyDeviceToDevice));
+                    ++q;
+                    continue;
+                }
+                uint64_t len = nk * nk;
+                size_t r = q + 1;
+                for (; r < dorder.size(); ++r) {
+                    const uint32_t e = dorder[r];
+                    if (dld[e] != S.nk[e] || doff[e] != doff[c] + len || S.dense_off[e] != S.dense_off[c] + len) break;
+                    len += (uint64_t)S.nk[e] * S.nk[e];
+                }
+                if (len)
+                    WX_HIP(hipMemcpy(dst + doff[c], src + S.dense_off[c], len * sizeof(CT), hipMemcpyDeviceToDevice));
+                q = r;
+            }
+            WX_HIP(hipDeviceSynchronize());
+        }
     }
     // prolongation map
     std::vector<uint64_t> d_ptr(S.n_dofs + 1, 0);
