@@ -50,6 +50,22 @@ struct ColState {
 constexpr int N_DOUBLE_FIELDS = 23;
 constexpr int N_INT_FIELDS = 6;
 
+// Partial blocks of one packed member: the ranges lh[2r], lh[2r+1] for r in
+// [ptr[m], ptr[m+1]).  A null ptr means one problem and the blocks [0, nb).
+struct Ranges {
+    const uint32_t *ptr;
+    const uint32_t *lh;
+};
+
+// The beta reduction (k_beta) of ncol columns, run by ncol workgroups appended to the grid of
+// a launch that neither writes the column state nor reads the fields k_beta writes; part
+// holds nb norm partials per column.
+struct BetaSide {
+    const double *part;
+    uint32_t nb, kp, ncol, k;
+    Ranges rg;
+};
+
 __device__ inline double term_load(const DesignArgs &d, int32_t li, uint64_t i) {
     return li < 0 ? 1.0 : d.slopes[(uint64_t)li * d.n + i];
 }
@@ -525,6 +541,9 @@ __global__ __launch_bounds__(BLK) void k_segreduce2(TermInfo ti, const uint32_t 
 // order, plus the sum of every run of equal Q codes inside the item (a piece), written to
 // pieces[piece] in run order from the item's first piece.  Each row's (scaled) u is read once
 // and added to both sums; P's sums are exactly k_segreduce2's for the same row order.
+__device__ inline void beta_column(const double *part, uint32_t nb, uint32_t kp, const ColState &cs, uint32_t k,
+                                   Ranges rg, uint32_t c, double *sh);
+
 template <bool SCALE>
 __global__ __launch_bounds__(BLK) void k_segreduce2_pair(TermInfo ti, const uint32_t *__restrict__ perm,
                                                          const SegItem *__restrict__ items, uint32_t n_items,
@@ -535,11 +554,17 @@ __global__ __launch_bounds__(BLK) void k_segreduce2_pair(TermInfo ti, const uint
                                                          const double *__restrict__ src, double *__restrict__ out,
                                                          double *__restrict__ partials, double *__restrict__ pieces,
                                                          const int32_t *__restrict__ active, uint32_t k,
-                                                         uint32_t kshift) {
+                                                         uint32_t kshift, ColState cs, BetaSide bs) {
+    const uint32_t nmain = gridDim.x - bs.ncol;
+    if (blockIdx.x >= nmain) { // the appended beta workgroups
+        __shared__ double sh[BLK];
+        beta_column(bs.part, bs.nb, bs.kp, cs, bs.k, bs.rg, blockIdx.x - nmain, sh);
+        return;
+    }
     const uint32_t kv = (1u << kshift) >> 1;
     const uint32_t j0 = 2 * (threadIdx.x & (kv - 1));
     if (j0 >= k || (active && !active[j0] && !active[j0 + 1])) return;
-    const uint32_t groups = (gridDim.x * BLK) / kv;
+    const uint32_t groups = (nmain * BLK) / kv;
     for (uint32_t it = (blockIdx.x * BLK + threadIdx.x) / kv; it < n_items; it += groups) {
         const SegItem item = items[it];
         const bool any = item.begin < item.end; // an unobserved level still writes its zero sum
@@ -712,13 +737,6 @@ __global__ __launch_bounds__(BLK) void k_load_rhs(uint64_t n, const uint32_t *__
         part[((uint64_t)blockIdx.x * kp + threadIdx.x) * 2 + 1] = sh[threadIdx.x];
     }
 }
-
-// Partial blocks of one packed member: the ranges lh[2r], lh[2r+1] for r in
-// [ptr[m], ptr[m+1]).  A null ptr means one problem and the blocks [0, nb).
-struct Ranges {
-    const uint32_t *ptr;
-    const uint32_t *lh;
-};
 
 // Per-column reduction of block partials (one block per column, fixed tree order).  The
 // reduction kernels run one workgroup per column-state index c = m * k + j: column j of
@@ -981,11 +999,11 @@ __device__ inline double alpha_from_parts(double vp, double vv, double pp, bool 
     return sqrt(vp > 0.0 ? vp : 0.0);
 }
 
-// beta = ||u|| after the gather; breakdown flags and the p~ update coefficients.
-__global__ __launch_bounds__(BLK) void k_beta(const double *part, uint32_t nb, uint32_t kp, ColState cs, uint32_t k,
-                                              Ranges rg) {
-    __shared__ double sh[BLK];
-    const uint32_t c = blockIdx.x, j = c % k;
+// beta = ||u|| after the gather; breakdown flags and the p~ update coefficients (column c,
+// one workgroup).
+__device__ inline void beta_column(const double *part, uint32_t nb, uint32_t kp, const ColState &cs, uint32_t k,
+                                   Ranges rg, uint32_t c, double *sh) {
+    const uint32_t j = c % k;
     double s = reduce_partials(part, nb, kp, 1, 0, j, sh, false, rg, c / k);
     if (threadIdx.x == 0 && cs.active[c]) {
         double beta = sqrt(s);
@@ -1000,6 +1018,12 @@ __global__ __launch_bounds__(BLK) void k_beta(const double *part, uint32_t nb, u
             cs.p_coeff[c] = beta / cs.alpha[c];
         }
     }
+}
+
+__global__ __launch_bounds__(BLK) void k_beta(const double *part, uint32_t nb, uint32_t kp, ColState cs, uint32_t k,
+                                              Ranges rg) {
+    __shared__ double sh[BLK];
+    beta_column(part, nb, kp, cs, k, rg, blockIdx.x, sh);
 }
 
 // First alpha (ModifiedGolubKahan::init, bidiag.rs:428-472) and the recurrence seed
@@ -1082,10 +1106,9 @@ __global__ __launch_bounds__(BLK) void k_init_solution(uint64_t N, double *__res
 // (recurrence.rs:143-161) and ConvergenceState::{observe, check} (recurrence.rs:239-255).
 // The iteration count is kept per column on the device, so the launch is identical every
 // iteration and a captured graph can replay it.
-__global__ __launch_bounds__(BLK) void k_alpha_step(const double *part, uint32_t nb, uint32_t kp, ColState cs,
-                                                    uint32_t k, Ranges rg) {
-    __shared__ double sh[BLK];
-    const uint32_t c = blockIdx.x, j = c % k;
+__device__ inline void alpha_step_column(const double *part, uint32_t nb, uint32_t kp, const ColState &cs, uint32_t k,
+                                         Ranges rg, uint32_t c, double *sh) {
+    const uint32_t j = c % k;
     double vp = reduce_partials(part, nb, kp, 3, 0, j, sh, false, rg, c / k);
     double vv = reduce_partials(part, nb, kp, 3, 1, j, sh, false, rg, c / k);
     double pp = reduce_partials(part, nb, kp, 3, 2, j, sh, false, rg, c / k);
@@ -1157,6 +1180,16 @@ __global__ __launch_bounds__(BLK) void k_alpha_step(const double *part, uint32_t
         cs.stop[c] = stop;
         cs.active[c] = 0;
     }
+}
+
+// One workgroup per column; the column's new "active" flag is also stored to `mirror` (pinned
+// host memory the host polls), so no copy of the flags is enqueued after an iteration.
+__global__ __launch_bounds__(BLK) void k_alpha_step(const double *part, uint32_t nb, uint32_t kp, ColState cs,
+                                                    uint32_t k, Ranges rg, int32_t *mirror) {
+    __shared__ double sh[BLK];
+    const uint32_t c = blockIdx.x;
+    alpha_step_column(part, nb, kp, cs, k, rg, c, sh);
+    if (threadIdx.x == 0 && mirror) mirror[c] = cs.active[c];
 }
 
 // x/h/h_bar update with the normalized v (recurrence.rs:163-190); v <- v / alpha first
@@ -1977,12 +2010,14 @@ void launch_adjoint(const DevDesign &dv, const DesignArgs &da, int which, const 
                 hipLaunchKernelGGL((k_segreduce2_pair<true>), dim3(grid2), dim3(BLK), 0, st, dt.info,
                                    dt.perm.as<uint32_t>(), dt.items.as<SegItem>(), dt.n_items,
                                    pp.item_piece0.as<uint32_t>(), pp.piece_slot.as<uint32_t>(),
-                                   pp.qcode_pos.as<uint32_t>(), f, src, out, partials, pieces, active, k, kshift);
+                                   pp.qcode_pos.as<uint32_t>(), f, src, out, partials, pieces, active, k, kshift,
+                                   ColState{}, BetaSide{});
             else
                 hipLaunchKernelGGL((k_segreduce2_pair<false>), dim3(grid2), dim3(BLK), 0, st, dt.info,
                                    dt.perm.as<uint32_t>(), dt.items.as<SegItem>(), dt.n_items,
                                    pp.item_piece0.as<uint32_t>(), pp.piece_slot.as<uint32_t>(),
-                                   pp.qcode_pos.as<uint32_t>(), f, src, out, partials, pieces, active, k, kshift);
+                                   pp.qcode_pos.as<uint32_t>(), f, src, out, partials, pieces, active, k, kshift,
+                                   ColState{}, BetaSide{});
             launch_segfix(dt, partials, out, active, k, kshift, st);
             const DevTerm &qt = pp.qt;
             const uint32_t gq = std::max<uint32_t>(
@@ -2031,8 +2066,10 @@ void launch_adjoint(const DevDesign &dv, const DesignArgs &da, int which, const 
 // The fused pair's D' and the merged fix-ups (MergedFix) after the fused gather has left the
 // sorted term's level sums in `out` and its partials at the start of `partials`; P's pass writes
 // its partials and the pieces into their regions of the same buffer.
+// With bs.ncol > 0 the pair launch also runs the beta reduction (BetaSide) on column state cs.
 void launch_adjoint_merged(const DevDesign &dv, int which, const double *src, double *out, double *partials,
-                           const int32_t *active, uint32_t k, uint32_t kshift, hipStream_t st) {
+                           const int32_t *active, uint32_t k, uint32_t kshift, hipStream_t st,
+                           const ColState &cs = ColState{}, const BetaSide &bs = BetaSide{}) {
     const PairPlan &pp = dv.pair;
     const MergedFix &m = dv.mfix;
     const DevTerm &dt = dv.terms[pp.p];
@@ -2041,16 +2078,17 @@ void launch_adjoint_merged(const DevDesign &dv, int which, const double *src, do
     const uint32_t grid2 = std::max<uint32_t>(
         1, std::min<uint32_t>((dt.n_items + 2 * groups_per_block - 1) / (2 * groups_per_block) * 2, 65535u * 8));
     double *pparts = partials + m.base_p * k, *pieces = partials + m.base_pieces * k;
+    const uint32_t gtot = grid2 + bs.ncol;
     if (f)
-        hipLaunchKernelGGL((k_segreduce2_pair<true>), dim3(grid2), dim3(BLK), 0, st, dt.info, dt.perm.as<uint32_t>(),
+        hipLaunchKernelGGL((k_segreduce2_pair<true>), dim3(gtot), dim3(BLK), 0, st, dt.info, dt.perm.as<uint32_t>(),
                            dt.items.as<SegItem>(), dt.n_items, pp.item_piece0.as<uint32_t>(),
                            pp.piece_slot.as<uint32_t>(), pp.qcode_pos.as<uint32_t>(), f, src, out, pparts, pieces,
-                           active, k, kshift);
+                           active, k, kshift, cs, bs);
     else
-        hipLaunchKernelGGL((k_segreduce2_pair<false>), dim3(grid2), dim3(BLK), 0, st, dt.info,
+        hipLaunchKernelGGL((k_segreduce2_pair<false>), dim3(gtot), dim3(BLK), 0, st, dt.info,
                            dt.perm.as<uint32_t>(), dt.items.as<SegItem>(), dt.n_items,
                            pp.item_piece0.as<uint32_t>(), pp.piece_slot.as<uint32_t>(),
-                           pp.qcode_pos.as<uint32_t>(), f, src, out, pparts, pieces, active, k, kshift);
+                           pp.qcode_pos.as<uint32_t>(), f, src, out, pparts, pieces, active, k, kshift, cs, bs);
     TermInfo z = dt.info; // absolute output rows, plain sums
     z.offset = 0;
     z.load[0] = -1;
@@ -2239,11 +2277,14 @@ struct Workspace {
     double *pieces_or_null() { return pieces.p ? pieces.as<double>() : nullptr; }
     ColState cs{};
     std::unique_ptr<SchwarzRun> runs[2];
-    // pinned copies of the per-column "active" flags of the last two iteration groups, and
-    // the events that mark them complete (the host polls one group behind the device);
-    // hbad: the right-hand-side load's (non-finite count, needs-scaled-norm) flags
-    int32_t *hflags = nullptr, *hbad = nullptr;
+    // hbad (pinned): the right-hand-side load's (non-finite count, needs-scaled-norm) flags;
+    // the events that mark the last two iteration groups complete (the host polls one group
+    // behind the device)
+    int32_t *hbad = nullptr;
     hipEvent_t flag_ev[2] = {nullptr, nullptr};
+    // host-coherent pinned copy of the "active" flags that k_alpha_step stores every
+    // iteration (host and device addresses); a flag, once 0, stays 0 for the rest of a solve
+    int32_t *hmirror = nullptr, *dmirror = nullptr;
     // captured iteration group (replayed by later solves on this workspace), its key
     hipGraph_t graph = nullptr;
     hipGraphExec_t graph_exec = nullptr;
@@ -2256,7 +2297,8 @@ struct Workspace {
     }
     ~Workspace() {
         drop_graph();
-        if (hflags) (void)hipHostFree(hflags);
+        if (hbad) (void)hipHostFree(hbad);
+        if (hmirror) (void)hipHostFree(hmirror);
         for (auto &e : flag_ev)
             if (e) (void)hipEventDestroy(e);
     }
@@ -2309,8 +2351,10 @@ std::shared_ptr<Workspace> new_workspace(const DevDesign &dv, uint32_t k, uint32
     if (dv.pair.p >= 0) w->pieces.alloc((size_t)std::max<uint32_t>(dv.pair.n_pieces, 1) * k * 8);
     w->bad.alloc(8);
     w->red.alloc(2 * (size_t)w->cap * 8);
-    WX_HIP(hipHostMalloc((void **)&w->hflags, (2 * (size_t)w->cap + 2) * sizeof(int32_t), hipHostMallocDefault));
-    w->hbad = w->hflags + 2 * w->cap;
+    WX_HIP(hipHostMalloc((void **)&w->hbad, 2 * sizeof(int32_t), hipHostMallocDefault));
+    WX_HIP(hipHostMalloc((void **)&w->hmirror, (size_t)w->cap * sizeof(int32_t), hipHostMallocCoherent));
+    std::fill(w->hmirror, w->hmirror + w->cap, 0);
+    WX_HIP(hipHostGetDevicePointer((void **)&w->dmirror, w->hmirror, 0));
     for (auto &e : w->flag_ev) WX_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
     return w;
 }
@@ -2354,10 +2398,10 @@ namespace {
 //   launch_load()  right-hand side load and beta_1 partials (asynchronous)
 //   finish_init()  waits for the load's validation flags, then the bidiagonalisation init
 //   launch_group() one group of `group` iterations: the first iteration directly, later
-//                  groups as one replay of a captured hipGraph; the per-column active flags
-//                  of the group are copied to pinned memory behind an event
-//   poll(wait)     the flags of the group before the last one (lag one): 1 when no column
-//                  is active any more, 0 when one is, -1 when not yet known
+//                  groups as one replay of a captured hipGraph; k_alpha_step stores the
+//                  per-column active flags to pinned memory, and an event marks the group
+//   poll(wait)     once the group before the last one (lag one) is complete, the flags: 1
+//                  when no column is active any more, 0 when one is, -1 when not yet known
 //   finish()       demeaned output, x, FP64 true residual, statistics (synchronous)
 class ChunkRun {
   public:
@@ -2490,9 +2534,9 @@ class ChunkRun {
             for (int q = 0; q < g; ++q) iteration(false);
             itn_ += g;
         }
+        // the group's k_alpha_step launches store the flags to the pinned mirror; the event
+        // marks the group complete
         const int slot = launches_ & 1;
-        WX_HIP(hipMemcpyAsync(ws_->hflags + (size_t)slot * ws_->cap, cs_.active, (size_t)ncol_ * 4,
-                              hipMemcpyDeviceToHost, st_));
         WX_HIP(hipEventRecord(ws_->flag_ev[slot], st_));
         ++launches_;
         WX_HIP(hipGetLastError());
@@ -2507,7 +2551,9 @@ class ChunkRun {
             if (q == hipErrorNotReady) return -1;
             WX_HIP(q);
         }
-        const int32_t *f = ws_->hflags + (size_t)slot * ws_->cap;
+        // the mirror holds the flags of that group or of a later one: a column that reads 0
+        // has stopped, so "all 0" is final either way
+        const volatile int32_t *f = ws_->hmirror;
         for (uint32_t c = 0; c < ncol_; ++c)
             if (f[c]) return 0;
         return 1;
@@ -2696,12 +2742,15 @@ class ChunkRun {
         else
             launch_gather(dv_, da_, first, gn_, ws.v.as<double>(), uin, ws.u.as<double>(), cs_, part_, k, kshift, st);
         if (timed) timer_->end(0);
-        hipLaunchKernelGGL(k_beta, dim3(ncol_), dim3(BLK), 0, st, part_, gpart, kp_, cs_, k, rg_item_);
+        const bool merged = dv_.fuse_term >= 0 && dv_.mfix.on() && k >= 2 && (k & 1) == 0;
+        if (!merged) hipLaunchKernelGGL(k_beta, dim3(ncol_), dim3(BLK), 0, st, part_, gpart, kp_, cs_, k, rg_item_);
         if (timed) timer_->begin(1);
-        if (dv_.fuse_term >= 0 && dv_.mfix.on() && k >= 2 && (k & 1) == 0) {
-            // the fused pair, then every term's split levels in merged launches
+        if (merged) {
+            // the fused pair (its launch also reduces beta), then every term's split levels in
+            // merged launches
             launch_adjoint_merged(dv_, weighted_ ? 1 : 0, ws.u.as<double>(), ws.atu.as<double>(),
-                                  ws.segpart.as<double>(), gate(), k, kshift, st);
+                                  ws.segpart.as<double>(), gate(), k, kshift, st, cs_,
+                                  BetaSide{part_, gpart, kp_, ncol_, k, rg_item_});
         } else {
             // split levels of the fused term first: the other terms reuse the partial buffer
             if (dv_.fuse_term >= 0)
@@ -2711,7 +2760,21 @@ class ChunkRun {
                            ws.segpart.as<double>(), gate(), k, kshift, st, dv_.fuse_term, ws.pieces_or_null());
         }
         if (timed) timer_->end(1);
-        if (prec_kind_ == 2) {
+        if (prec_kind_ == 2 && cap_ == 0) {
+            // p~ update folded into the restriction, the alpha dot products into the prolongation
+            ApplyFusion fz;
+            fz.blk_col = dof_col_;
+            fz.atu = ws.atu.as<double>();
+            fz.beta_inv = cs_.beta_inv;
+            fz.p_coeff = cs_.p_coeff;
+            fz.brk = cs_.brk;
+            fz.active = cs_.active;
+            fz.dots = part_;
+            fz.dots_blocks = gN_;
+            if (timed) timer_->begin(2);
+            schwarz_->apply_fused(ws.pt.as<double>(), ws.v.as<double>(), k, st, gate(), fz);
+            if (timed) timer_->end(2);
+        } else if (prec_kind_ == 2) {
             hipLaunchKernelGGL((k_ptilde<2, false>), dim3(gN_), dim3(BLK), 0, st, N, ws.atu.as<double>(),
                                ws.pt.as<double>(), ws.v.as<double>(), nullptr, cs_, part_, k, kshift, dof_col_);
             if (timed) timer_->begin(2);
@@ -2733,7 +2796,8 @@ class ChunkRun {
                                    part_, k, kshift, dof_col_);
             }
         }
-        hipLaunchKernelGGL(k_alpha_step, dim3(ncol_), dim3(BLK), 0, st, part_, gN_, kp_, cs_, k, rg_dof_);
+        hipLaunchKernelGGL(k_alpha_step, dim3(ncol_), dim3(BLK), 0, st, part_, gN_, kp_, cs_, k, rg_dof_,
+                           ws.dmirror);
         const uint64_t push = cap_ ? ring_advance() : 0;
         hipLaunchKernelGGL(k_update_solution, dim3(gN_), dim3(BLK), 0, st, N, ws.v.as<double>(), ws.h.as<double>(),
                            ws.hb.as<double>(), ws.x.as<double>(), cs_, k, kshift, ws.pt.as<double>(),

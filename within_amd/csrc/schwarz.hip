@@ -170,6 +170,18 @@ __device__ inline bool gate_closed(const int32_t *gate, uint32_t k) {
     return a == 0;
 }
 
+// Sum over the threads of a workgroup that share (tid & (kp-1)), by a fixed-order LDS tree;
+// every thread of the workgroup must call it.
+__device__ inline double cols_block_sum(double v, uint32_t kp, double *sh) {
+    sh[threadIdx.x] = v;
+    __syncthreads();
+    for (uint32_t s = BLK / 2; s >= kp && s > 0; s >>= 1) {
+        if (threadIdx.x < s) sh[threadIdx.x] += sh[threadIdx.x + s];
+        __syncthreads();
+    }
+    return sh[threadIdx.x];
+}
+
 // ---------------------------------------------------------------------------
 // kernels (kp lanes per row; lane j = RHS column)
 // ---------------------------------------------------------------------------
@@ -191,6 +203,46 @@ __global__ __launch_bounds__(BLK) void sk_restrict(uint64_t ns, const uint32_t *
         loc[s1 * k + j] = (CT)(w1 * r1);
     }
     if (s < ns) loc[s * k + j] = (CT)((double)sgnpw[s] * r[(uint64_t)glob[s] * k + j]);
+}
+
+// The restriction by DOF with the p~ update of the LSMR step folded in (ApplyFusion): a row
+// group per DOF forms p~ for its active columns (an inactive column keeps its p~), stores it,
+// and writes each of the DOF's slots, loc[s] = sgnpw[s] * p~.  Every slot belongs to one DOF,
+// so every slot is written once, with the value sk_restrict gives from the stored p~.
+template <typename CT>
+__global__ __launch_bounds__(BLK) void sk_restrict_dof(uint64_t N, const uint64_t *__restrict__ d_ptr,
+                                                       const uint32_t *__restrict__ d_slot,
+                                                       const CT *__restrict__ sgnpw, const double *__restrict__ atu,
+                                                       double *__restrict__ pt, ApplyFusion fz, CT *__restrict__ loc,
+                                                       uint32_t k, uint32_t kshift, const int32_t *__restrict__ gate) {
+    if (gate_closed(gate, k)) return;
+    const uint32_t kp = 1u << kshift, j = threadIdx.x & (kp - 1);
+    if (j >= k) return;
+    const uint32_t c = (fz.blk_col ? fz.blk_col[blockIdx.x] : 0u) + j;
+    const bool act = fz.active[c] != 0, brk = fz.brk[c] != 0;
+    const double bi = fz.beta_inv[c], pc = fz.p_coeff[c];
+    const uint64_t stride = ((uint64_t)gridDim.x * BLK) >> kshift;
+    for (uint64_t d = ((uint64_t)blockIdx.x * BLK + threadIdx.x) >> kshift; d < N; d += stride) {
+        const uint64_t q = d * k + j;
+        double p;
+        if (!act) p = pt[q];
+        else {
+            p = brk ? 0.0 : bi * atu[q] + (-pc) * pt[q];
+            pt[q] = p;
+        }
+        const uint64_t e = d_ptr[d + 1];
+        uint64_t i = d_ptr[d];
+        for (; i + 2 <= e; i += 2) { // two slots' stores in flight
+            const uint64_t s0 = d_slot[i], s1 = d_slot[i + 1];
+            const double w0 = (double)sgnpw[s0], w1 = (double)sgnpw[s1];
+            loc[s0 * k + j] = (CT)(w0 * p);
+            loc[s1 * k + j] = (CT)(w1 * p);
+        }
+        if (i < e) {
+            const uint64_t s = d_slot[i];
+            loc[s * k + j] = (CT)((double)sgnpw[s] * p);
+        }
+    }
 }
 
 // Mean of component c over its chunk partials, by the whole workgroup: row groups stride over
@@ -976,17 +1028,20 @@ __global__ __launch_bounds__(BLK) void sk_cover_readback(const uint32_t *__restr
 }
 
 // z[dof] = sum over the DOF's slots of sgnpw[s] * (xs[s] - mu3[comp(s)]); uncovered DOFs -> 0.
-template <typename CT>
+// DOTS: also the block partials of <z,r>, <z,z>, <r,r> (r = the apply's input p~), each lane
+// adding its DOFs in loop order, then a fixed-order tree over the lanes of a column.
+template <typename CT, bool DOTS>
 __global__ __launch_bounds__(BLK) void sk_prolong(uint64_t N, const uint64_t *__restrict__ d_ptr,
                                                   const uint32_t *__restrict__ d_slot, const CT *__restrict__ sgnpw,
                                                   const uint32_t *__restrict__ slot_comp,
                                                   const CT *__restrict__ xs, const double *__restrict__ mu3,
-                                                  double *__restrict__ z, uint32_t k, uint32_t kshift, const int32_t *__restrict__ gate) {
+                                                  double *__restrict__ z, uint32_t k, uint32_t kshift, const int32_t *__restrict__ gate,
+                                                  const double *__restrict__ r, double *__restrict__ dots) {
     if (gate_closed(gate, k)) return;
     const uint32_t kp = 1u << kshift, j = threadIdx.x & (kp - 1);
-    if (j >= k) return;
+    double vp = 0.0, vv = 0.0, pp = 0.0;
     const uint64_t stride = ((uint64_t)gridDim.x * BLK) >> kshift;
-    for (uint64_t d = ((uint64_t)blockIdx.x * BLK + threadIdx.x) >> kshift; d < N; d += stride) {
+    for (uint64_t d = ((uint64_t)blockIdx.x * BLK + threadIdx.x) >> kshift; j < k && d < N; d += stride) {
         double acc = 0.0;
         const uint64_t q1 = d_ptr[d + 1];
         uint64_t q = d_ptr[d];
@@ -1003,6 +1058,26 @@ __global__ __launch_bounds__(BLK) void sk_prolong(uint64_t N, const uint64_t *__
             acc += (double)sgnpw[s] * ((double)xs[s * k + j] - mu3[(uint64_t)slot_comp[s] * k + j]);
         }
         z[d * k + j] = acc;
+        if (DOTS) {
+            const double p = r[d * k + j];
+            vp += acc * p;
+            vv += acc * acc;
+            pp += p * p;
+        }
+    }
+    if (DOTS) {
+        __shared__ double sh[BLK];
+        const double a = cols_block_sum(vp, kp, sh);
+        __syncthreads();
+        const double b = cols_block_sum(vv, kp, sh);
+        __syncthreads();
+        const double c = cols_block_sum(pp, kp, sh);
+        if (threadIdx.x < kp) {
+            double *o = dots + ((uint64_t)blockIdx.x * kp + threadIdx.x) * 3;
+            o[0] = a;
+            o[1] = b;
+            o[2] = c;
+        }
     }
 }
 
@@ -1710,13 +1785,20 @@ struct SchwarzRunImpl : SchwarzRun {
                            dp<uint32_t>(count), mu, k, kshift, gate);
     }
 
-    void apply(const double *r, double *z, uint32_t k, hipStream_t st, const int32_t *gate) override {
+    void apply_fused(double *r, double *z, uint32_t k, hipStream_t st, const int32_t *gate,
+                     const ApplyFusion &fz) override {
         ensure(k);
         const uint32_t kshift = pow2_shift(k);
         const SchwarzDev<CT> &d = *D;
         CT *locp = dp<CT>(loc), *redp = dp<CT>(red), *xsp = dp<CT>(xs);
-        hipLaunchKernelGGL((sk_restrict<CT>), dim3(grid_rows((d.ns + 1) / 2, kshift)), dim3(BLK), 0, st, d.ns,
-                           dp<uint32_t>(d.glob), dp<CT>(d.sgnpw), r, locp, k, kshift, gate);
+        if (fz.atu)
+            hipLaunchKernelGGL((sk_restrict_dof<CT>), dim3(fz.blk_col ? fz.dots_blocks : grid_rows(d.N, kshift)),
+                               dim3(BLK), 0, st,
+                               d.N, dp<uint64_t>(d.d_ptr), dp<uint32_t>(d.d_slot), dp<CT>(d.sgnpw), fz.atu, r, fz,
+                               locp, k, kshift, gate);
+        else
+            hipLaunchKernelGGL((sk_restrict<CT>), dim3(grid_rows((d.ns + 1) / 2, kshift)), dim3(BLK), 0, st, d.ns,
+                               dp<uint32_t>(d.glob), dp<CT>(d.sgnpw), r, locp, k, kshift, gate);
         mean(d.all_chunks, d.n_all_chunks, d.all_cptr, d.comp_count, locp, dp<double>(mu1), k, kshift, st, gate);
         // reduced right-hand side on kept rows
         rows_spmv<0>(d.ct_items, d.n_ct_items, d.ct_idx, d.ct_val, locp, xsp, redp, k, kshift, st, gate);
@@ -1807,9 +1889,15 @@ struct SchwarzRunImpl : SchwarzRun {
         rows_spmv<1>(d.c_items, d.n_c_items, d.c_idx, d.c_val, locp, xsp, xsp, k, kshift, st, gate);
         fix_rows<1>(d.c_fix, d.n_c_fix, d.n_c_fix_small, locp, dp<double>(mu1), xsp, k, kshift, st, gate);
         mean(d.all_chunks, d.n_all_chunks, d.all_cptr, d.comp_count, xsp, dp<double>(mu3), k, kshift, st, gate);
-        hipLaunchKernelGGL((sk_prolong<CT>), dim3(grid_rows(d.N, kshift)), dim3(BLK), 0, st, d.N,
-                           dp<uint64_t>(d.d_ptr), dp<uint32_t>(d.d_slot), dp<CT>(d.sgnpw),
-                           dp<uint32_t>(d.slot_comp), xsp, dp<double>(mu3), z, k, kshift, gate);
+        if (fz.dots)
+            hipLaunchKernelGGL((sk_prolong<CT, true>), dim3(fz.dots_blocks), dim3(BLK), 0, st, d.N,
+                               dp<uint64_t>(d.d_ptr), dp<uint32_t>(d.d_slot), dp<CT>(d.sgnpw),
+                               dp<uint32_t>(d.slot_comp), xsp, dp<double>(mu3), z, k, kshift, gate, r, fz.dots);
+        else
+            hipLaunchKernelGGL((sk_prolong<CT, false>), dim3(grid_rows(d.N, kshift)), dim3(BLK), 0, st, d.N,
+                               dp<uint64_t>(d.d_ptr), dp<uint32_t>(d.d_slot), dp<CT>(d.sgnpw),
+                               dp<uint32_t>(d.slot_comp), xsp, dp<double>(mu3), z, k, kshift, gate, nullptr,
+                               nullptr);
         WX_HIP(hipGetLastError());
     }
 };

@@ -93,12 +93,34 @@ struct Precond::Schwarz {
     uint64_t n_slots() const { return glob.size(); }
 };
 
+// The LSMR steps on either side of a preconditioner apply that a solve can fold into it
+// (column j's state at index j, or at blk_col[b] + j in a packed run):
+//  - before: p~ <- beta_inv A'u - p_coeff p~ for every active column (0 after a breakdown),
+//    formed per DOF inside the restriction, which then reads the new p~ (bidiag.rs:475-485);
+//  - after:  per-block partial sums of <v,p~>, <v,v>, <p~,p~> (bidiag.rs:99-111), formed inside
+//    the prolongation, laid out part[(b * kp + j) * 3 + f] over `dots_blocks` blocks.
+struct ApplyFusion {
+    const double *atu = nullptr; // A'u (null: p~ is the input as it stands)
+    const double *beta_inv = nullptr, *p_coeff = nullptr;
+    const int32_t *brk = nullptr, *active = nullptr;
+    double *dots = nullptr; // null: no partial sums
+    uint32_t dots_blocks = 0;
+    // a packed run: both launches take `dots_blocks` blocks, block b covering whole DOF blocks
+    // of one member, whose column-state base is blk_col[b] (null: one problem, base 0)
+    const uint32_t *blk_col = nullptr;
+};
+
 // Device apply of an additive Schwarz preconditioner for one batch width.
 struct SchwarzRun {
     virtual ~SchwarzRun() = default;
     // v = M^{-1} pt for N x k interleaved FP64 arrays, enqueued on `st`.  `gate` (k flags on
     // the device, or null) lets every kernel return at once when no flag is set.
-    virtual void apply(const double *pt, double *v, uint32_t k, hipStream_t st, const int32_t *gate) = 0;
+    void apply(const double *pt, double *v, uint32_t k, hipStream_t st, const int32_t *gate) {
+        apply_fused(const_cast<double *>(pt), v, k, st, gate, ApplyFusion{});
+    }
+    // The same with the steps of `fz` folded in; pt is updated in place when fz.atu is set.
+    virtual void apply_fused(double *pt, double *v, uint32_t k, hipStream_t st, const int32_t *gate,
+                             const ApplyFusion &fz) = 0;
 };
 
 std::unique_ptr<SchwarzRun> make_schwarz_run(const Precond &p, int32_t precision, hipStream_t st);
