@@ -809,8 +809,11 @@ __device__ inline void set_beta1(ColState &cs, uint32_t j, double beta, double t
 // columns raise `need_scaled` and get the scaled two-pass form (k_scaled_sumsq).
 constexpr double NORM_SAFE_LO = 1e-140, NORM_SAFE_HI = 1e140;
 
+// nonfinite: the load's count of non-finite entries; when it is not 0 every column stops at once
+// (the solve then reports the error).  host_bad (pinned, or null): receives that count.
 __global__ __launch_bounds__(BLK) void k_beta1(const double *part, uint32_t nb, uint32_t kp, ColState cs, double tol,
-                                               int32_t *need_scaled, uint32_t k, Ranges rg) {
+                                               int32_t *need_scaled, uint32_t k, Ranges rg,
+                                               const int32_t *nonfinite, int32_t *host_bad) {
     __shared__ double sh[BLK];
     const uint32_t c = blockIdx.x, j = c % k;
     const double s = reduce_partials(part, nb, kp, 2, 0, j, sh, false, rg, c / k);
@@ -823,14 +826,20 @@ __global__ __launch_bounds__(BLK) void k_beta1(const double *part, uint32_t nb, 
             need_scaled[0] = 1;
         }
         set_beta1(cs, c, m == 0.0 ? 0.0 : beta, tol);
+        const int32_t bad = nonfinite ? *nonfinite : 0;
+        if (bad) cs.active[c] = 0;
+        if (host_bad && c == 0) host_bad[0] = bad;
     }
 }
 
 // sum((b/scale)^2) partials, b = sqrt(w) .* y_int, for the columns outside the safe range
 template <bool WEIGHTED>
+// (need: when set, the launch returns at once unless *need, k_beta1's flag, is set)
 __global__ __launch_bounds__(BLK) void k_scaled_sumsq(uint64_t n, const double *__restrict__ sw,
                                                       const double *__restrict__ y_int, ColState cs,
-                                                      double *__restrict__ part, uint32_t k, uint32_t kshift) {
+                                                      double *__restrict__ part, uint32_t k, uint32_t kshift,
+                                                      const int32_t *need) {
+    if (need && *need == 0) return;
     __shared__ double sh[BLK];
     const uint32_t kp = 1u << kshift;
     const uint32_t j = threadIdx.x & (kp - 1);
@@ -849,13 +858,15 @@ __global__ __launch_bounds__(BLK) void k_scaled_sumsq(uint64_t n, const double *
 }
 
 __global__ __launch_bounds__(BLK) void k_beta1_scaled(const double *part, uint32_t nb, uint32_t kp, ColState cs,
-                                                      double tol) {
+                                                      double tol, const int32_t *need, const int32_t *nonfinite) {
+    if (need && *need == 0) return;
     __shared__ double sh[BLK];
     const uint32_t j = blockIdx.x;
     const double s = reduce_partials(part, nb, kp, 1, 0, j, sh, false);
     if (threadIdx.x == 0) {
         const double m = cs.bnorm_scale[j];
         if (m != 0.0 && !(m >= NORM_SAFE_LO && m <= NORM_SAFE_HI)) set_beta1(cs, j, m * sqrt(s), tol);
+        if (nonfinite && *nonfinite) cs.active[j] = 0;
     }
 }
 
@@ -1106,16 +1117,16 @@ __global__ __launch_bounds__(BLK) void k_init_solution(uint64_t N, double *__res
 // (recurrence.rs:143-161) and ConvergenceState::{observe, check} (recurrence.rs:239-255).
 // The iteration count is kept per column on the device, so the launch is identical every
 // iteration and a captured graph can replay it.
-__device__ inline void alpha_step_column(const double *part, uint32_t nb, uint32_t kp, const ColState &cs, uint32_t k,
+__device__ inline float alpha_step_column(const double *part, uint32_t nb, uint32_t kp, const ColState &cs, uint32_t k,
                                          Ranges rg, uint32_t c, double *sh) {
     const uint32_t j = c % k;
     double vp = reduce_partials(part, nb, kp, 3, 0, j, sh, false, rg, c / k);
     double vv = reduce_partials(part, nb, kp, 3, 1, j, sh, false, rg, c / k);
     double pp = reduce_partials(part, nb, kp, 3, 2, j, sh, false, rg, c / k);
-    if (threadIdx.x != 0) return;
+    if (threadIdx.x != 0) return 0.0f;
     if (!cs.active[c]) {
         cs.upd[c] = 0;
-        return;
+        return 0.0f;
     }
     double alpha, beta = cs.beta[c];
     if (cs.brk[c]) {
@@ -1128,7 +1139,7 @@ __device__ inline void alpha_step_column(const double *part, uint32_t nb, uint32
             cs.active[c] = 0;
             cs.upd[c] = 0;
             cs.iters[c] += 1; // steps taken (reset by set_beta1)
-            return;
+            return 0.0f;
         }
     }
     cs.inv_alpha[c] = alpha > 0.0 ? 1.0 / alpha : 0.0;
@@ -1167,12 +1178,14 @@ __device__ inline void alpha_step_column(const double *part, uint32_t nb, uint32
     // convergence.check
     const double residual = fabs(phi_bar_new);
     int32_t stop = -1;
+    double progress = 0.0; // min over the two tests of value / tolerance (scheduling only)
     if (residual <= cs.abs_tol[c]) stop = WX_STOP_RESIDUAL_TOLERANCE;
     else {
         const double a_norm = fmax(sqrt(a_norm_sq), 2.2250738585072014e-308);
         const double normar = fabs(zeta_bar_new);
-        if (normar / (a_norm * fmax(residual, 2.2250738585072014e-308)) <= cs.rel_tol[c])
-            stop = WX_STOP_NORMAL_EQ_TOLERANCE;
+        const double test2 = normar / (a_norm * fmax(residual, 2.2250738585072014e-308));
+        if (test2 <= cs.rel_tol[c]) stop = WX_STOP_NORMAL_EQ_TOLERANCE;
+        else progress = fmin(residual / cs.abs_tol[c], test2 / cs.rel_tol[c]);
     }
     cs.upd[c] = 1;
     cs.iters[c] += 1; // steps taken (reset by set_beta1)
@@ -1180,16 +1193,21 @@ __device__ inline void alpha_step_column(const double *part, uint32_t nb, uint32
         cs.stop[c] = stop;
         cs.active[c] = 0;
     }
+    return (float)progress;
 }
 
-// One workgroup per column; the column's new "active" flag is also stored to `mirror` (pinned
-// host memory the host polls), so no copy of the flags is enqueued after an iteration.
+// One workgroup per column; the column's new "active" flag and its progress towards the stopping
+// test are also stored to pinned host memory the host polls (mirror, progress), so no copy of
+// the flags is enqueued after an iteration.
 __global__ __launch_bounds__(BLK) void k_alpha_step(const double *part, uint32_t nb, uint32_t kp, ColState cs,
-                                                    uint32_t k, Ranges rg, int32_t *mirror) {
+                                                    uint32_t k, Ranges rg, int32_t *mirror, float *progress) {
     __shared__ double sh[BLK];
     const uint32_t c = blockIdx.x;
-    alpha_step_column(part, nb, kp, cs, k, rg, c, sh);
-    if (threadIdx.x == 0 && mirror) mirror[c] = cs.active[c];
+    const float pg = alpha_step_column(part, nb, kp, cs, k, rg, c, sh);
+    if (threadIdx.x == 0 && mirror) {
+        mirror[c] = cs.active[c];
+        progress[c] = pg;
+    }
 }
 
 // x/h/h_bar update with the normalized v (recurrence.rs:163-190); v <- v / alpha first
@@ -1329,32 +1347,35 @@ __global__ __launch_bounds__(BLK) void k_store_caller(uint64_t n, const uint32_t
 }
 
 // The demeaned output y - D x straight into caller order, column-major (k_demean2 followed by
-// k_store_caller, without the internal-order round trip): per 64-row tile of caller rows, each
-// row's internal row i = inv_perm[r] gives y_int, the code record and the coefficient rows; the
-// terms are added in order from 0.0 and subtracted from y, as k_demean2 does, so the values are
-// the same bits.  For plain designs of up to four terms (code records present).
+// k_store_caller, without the internal-order round trip): per DEMEAN_TILE-row tile of caller
+// rows, each row's internal row i = inv_perm[r] gives y_int, the code record and the
+// coefficient rows; the terms are added in order from 0.0 and subtracted from y, as k_demean2
+// does, so the values are the same bits.  For plain designs of up to four terms (code records
+// present).  (Tiles of 128 and 256 rows measured slower: 485 and 612 against 458 us at n = 5M,
+// k = 16.)
+constexpr uint32_t DEMEAN_TILE = 64;
 template <int TT>
 __global__ __launch_bounds__(BLK) void k_demean_store(DesignArgs d, const uint4 *__restrict__ codes4,
                                                       const uint32_t *__restrict__ inv_perm,
                                                       const double *__restrict__ x, const double *__restrict__ y_int,
                                                       double *__restrict__ Dm, uint64_t ldd, uint32_t k,
                                                       uint32_t kshift) {
-    extern __shared__ double tile[]; // 64 x (k + 1)
+    extern __shared__ double tile[]; // DEMEAN_TILE x (k + 1)
     const uint32_t ld = k + 1;
     const uint32_t kp = 1u << kshift, j = threadIdx.x & (kp - 1), grp = threadIdx.x >> kshift;
     const uint32_t ngrp = BLK >> kshift;
-    const uint64_t n = d.n, n_tiles = (n + 63) / 64;
-    constexpr int R = 4;
+    const uint64_t n = d.n, n_tiles = (n + DEMEAN_TILE - 1) / DEMEAN_TILE;
+    constexpr int R = DEMEAN_TILE / 16 < 4 ? 4 : DEMEAN_TILE / 16;
     for (uint64_t tb = blockIdx.x; tb < n_tiles; tb += gridDim.x) {
-        const uint64_t r0 = tb * 64;
+        const uint64_t r0 = tb * DEMEAN_TILE;
         if (j < k)
-            for (uint32_t rb = grp; rb < 64; rb += R * ngrp) {
+            for (uint32_t rb = grp; rb < DEMEAN_TILE; rb += R * ngrp) {
                 uint64_t row[R];
                 bool ok[R];
 #pragma unroll
                 for (int q = 0; q < R; ++q) {
                     const uint64_t r = r0 + rb + q * ngrp;
-                    ok[q] = rb + q * ngrp < 64 && r < n;
+                    ok[q] = rb + q * ngrp < DEMEAN_TILE && r < n;
                     row[q] = ok[q] ? (inv_perm ? inv_perm[r] : r) : 0;
                 }
                 double yv[R];
@@ -1382,8 +1403,8 @@ __global__ __launch_bounds__(BLK) void k_demean_store(DesignArgs d, const uint4 
                 }
             }
         __syncthreads();
-        for (uint32_t q = threadIdx.x; q < 64 * k; q += BLK) {
-            const uint32_t c = q >> 6, rr = q & 63;
+        for (uint32_t q = threadIdx.x; q < DEMEAN_TILE * k; q += BLK) {
+            const uint32_t c = q / DEMEAN_TILE, rr = q % DEMEAN_TILE;
             const uint64_t r = r0 + rr;
             if (r < n) Dm[r + c * ldd] = tile[rr * ld + c];
         }
@@ -2280,11 +2301,13 @@ struct Workspace {
     // hbad (pinned): the right-hand-side load's (non-finite count, needs-scaled-norm) flags;
     // the events that mark the last two iteration groups complete (the host polls one group
     // behind the device)
-    int32_t *hbad = nullptr;
+    int32_t *hbad = nullptr, *dbad = nullptr; // host and device addresses
     hipEvent_t flag_ev[2] = {nullptr, nullptr};
     // host-coherent pinned copy of the "active" flags that k_alpha_step stores every
-    // iteration (host and device addresses); a flag, once 0, stays 0 for the rest of a solve
+    // iteration (host and device addresses); a flag, once 0, stays 0 for the rest of a solve.
+    // progress: each column's stopping-test value over its tolerance (0 once stopped)
     int32_t *hmirror = nullptr, *dmirror = nullptr;
+    float *hprogress = nullptr, *dprogress = nullptr;
     // captured iteration group (replayed by later solves on this workspace), its key
     hipGraph_t graph = nullptr;
     hipGraphExec_t graph_exec = nullptr;
@@ -2351,10 +2374,14 @@ std::shared_ptr<Workspace> new_workspace(const DevDesign &dv, uint32_t k, uint32
     if (dv.pair.p >= 0) w->pieces.alloc((size_t)std::max<uint32_t>(dv.pair.n_pieces, 1) * k * 8);
     w->bad.alloc(8);
     w->red.alloc(2 * (size_t)w->cap * 8);
-    WX_HIP(hipHostMalloc((void **)&w->hbad, 2 * sizeof(int32_t), hipHostMallocDefault));
-    WX_HIP(hipHostMalloc((void **)&w->hmirror, (size_t)w->cap * sizeof(int32_t), hipHostMallocCoherent));
+    WX_HIP(hipHostMalloc((void **)&w->hbad, 2 * sizeof(int32_t), hipHostMallocCoherent));
+    WX_HIP(hipHostGetDevicePointer((void **)&w->dbad, w->hbad, 0));
+    WX_HIP(hipHostMalloc((void **)&w->hmirror, (size_t)w->cap * 8, hipHostMallocCoherent));
     std::fill(w->hmirror, w->hmirror + w->cap, 0);
+    w->hprogress = reinterpret_cast<float *>(w->hmirror + w->cap);
+    std::fill(w->hprogress, w->hprogress + w->cap, 0.0f);
     WX_HIP(hipHostGetDevicePointer((void **)&w->dmirror, w->hmirror, 0));
+    w->dprogress = reinterpret_cast<float *>(w->dmirror + w->cap);
     for (auto &e : w->flag_ev) WX_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
     return w;
 }
@@ -2452,7 +2479,8 @@ class ChunkRun {
                                        nullptr, mY_[m], mld_[m], yint, pp, badp, k_, kshift_, mb.kv);
             }
             hipLaunchKernelGGL(k_beta1, dim3(ncol_), dim3(BLK), 0, st_, part_, pk_->n_tiles, kp_, cs_, opts_.tol,
-                               badp + 1, k_, rg_tile_);
+                               badp + 1, k_, rg_tile_, badp, nullptr);
+            WX_HIP(hipMemcpyAsync(ws_->hbad, ws_->bad.p, 8, hipMemcpyDeviceToHost, st_));
         } else {
             if (weighted_)
                 hipLaunchKernelGGL((k_load_rhs<true>), dim3(gt_), dim3(BLK), tile_bytes_, st_, n_, inv_perm_,
@@ -2461,30 +2489,45 @@ class ChunkRun {
             else
                 hipLaunchKernelGGL((k_load_rhs<false>), dim3(gt_), dim3(BLK), tile_bytes_, st_, n_, inv_perm_, nullptr,
                                    Y_, ldy_, ws_->yint.as<double>(), part_, badp, k_, kshift_, k_);
+            // the non-finite count goes straight to pinned memory; finish() reads it
             hipLaunchKernelGGL(k_beta1, dim3(k_), dim3(BLK), 0, st_, part_, gt_, kp_, cs_, opts_.tol, badp + 1, k_,
-                               Ranges{});
+                               Ranges{}, badp, ws_->dbad);
         }
-        WX_HIP(hipMemcpyAsync(ws_->hbad, ws_->bad.p, 8, hipMemcpyDeviceToHost, st_));
         WX_HIP(hipGetLastError());
     }
 
     void finish_init() {
-        WX_HIP(hipStreamSynchronize(st_));
-        if (ws_->hbad[0]) fail(WX_ERR_INVALID_INPUT, "Solver::solve: response contains non-finite values");
-        if (ws_->hbad[1]) { // a column outside the safe range of the unscaled sum of squares
-            const size_t nm = pk_ ? pk_->members.size() : 1;
-            for (size_t m = 0; m < nm; ++m) {
-                const uint64_t r0 = pk_ ? pk_->members[m].row_off : 0, nr = pk_ ? pk_->members[m].n : n_;
+        if (!pk_) {
+            // no wait for the load: a non-finite response has stopped every column (finish()
+            // reports it), and the scaled norm pass returns at once unless a column needs it
+            const uint32_t g = grid_for(n_, kshift_);
+            int32_t *badp = ws_->bad.as<int32_t>();
+            if (weighted_)
+                hipLaunchKernelGGL((k_scaled_sumsq<true>), dim3(g), dim3(BLK), 0, st_, n_, dv_.sw.as<double>(),
+                                   ws_->yint.as<double>(), cs_, part_, k_, kshift_, badp + 1);
+            else
+                hipLaunchKernelGGL((k_scaled_sumsq<false>), dim3(g), dim3(BLK), 0, st_, n_, nullptr,
+                                   ws_->yint.as<double>(), cs_, part_, k_, kshift_, badp + 1);
+            hipLaunchKernelGGL(k_beta1_scaled, dim3(k_), dim3(BLK), 0, st_, part_, g, kp_, cs_, opts_.tol, badp + 1,
+                               badp);
+        } else {
+            WX_HIP(hipStreamSynchronize(st_));
+            check_load();
+        }
+        if (pk_ && ws_->hbad[1]) { // a column outside the safe range of the unscaled sum of squares
+            for (size_t m = 0; m < pk_->members.size(); ++m) {
+                const uint64_t r0 = pk_->members[m].row_off, nr = pk_->members[m].n;
                 const ColState v = colstate_view(cs_, (uint32_t)(m * k_));
                 const uint32_t g = grid_for(nr, kshift_);
                 const double *y = ws_->yint.as<double>() + r0 * k_;
                 if (weighted_)
                     hipLaunchKernelGGL((k_scaled_sumsq<true>), dim3(g), dim3(BLK), 0, st_, nr,
-                                       dv_.sw.as<double>() + r0, y, v, part_, k_, kshift_);
+                                       dv_.sw.as<double>() + r0, y, v, part_, k_, kshift_, nullptr);
                 else
                     hipLaunchKernelGGL((k_scaled_sumsq<false>), dim3(g), dim3(BLK), 0, st_, nr, nullptr, y, v, part_,
-                                       k_, kshift_);
-                hipLaunchKernelGGL(k_beta1_scaled, dim3(k_), dim3(BLK), 0, st_, part_, g, kp_, v, opts_.tol);
+                                       k_, kshift_, nullptr);
+                hipLaunchKernelGGL(k_beta1_scaled, dim3(k_), dim3(BLK), 0, st_, part_, g, kp_, v, opts_.tol, nullptr,
+                                   nullptr);
             }
         }
         // init: p~ = A'b / beta_1, v = M^{-1} p~, alpha_1
@@ -2492,11 +2535,19 @@ class ChunkRun {
                        ws_->segpart.as<double>(), gate(), k_, kshift_, st_, -1, ws_->pieces_or_null());
         const uint64_t N = N_;
         if (prec_kind_ == 2) {
-            hipLaunchKernelGGL((k_ptilde<2, true>), dim3(gN_), dim3(BLK), 0, st_, N, nullptr, ws_->pt.as<double>(),
-                               ws_->v.as<double>(), nullptr, cs_, part_, k_, kshift_, dof_col_);
-            schwarz_->apply(ws_->pt.as<double>(), ws_->v.as<double>(), k_, st_, gate());
-            hipLaunchKernelGGL(k_dots, dim3(gN_), dim3(BLK), 0, st_, N, ws_->pt.as<double>(), ws_->v.as<double>(), cs_,
-                               part_, k_, kshift_, dof_col_);
+            // p~ = A'b / beta_1 inside the restriction, the alpha_1 dot products inside the
+            // prolongation
+            ApplyFusion fz;
+            fz.atu = ws_->pt.as<double>();
+            fz.init = true;
+            fz.beta_inv = cs_.beta_inv;
+            fz.p_coeff = cs_.p_coeff;
+            fz.brk = cs_.brk;
+            fz.active = cs_.active;
+            fz.dots = part_;
+            fz.dots_blocks = gN_;
+            fz.blk_col = dof_col_;
+            schwarz_->apply_fused(ws_->pt.as<double>(), ws_->v.as<double>(), k_, st_, gate(), fz);
         } else if (prec_kind_ == 1) {
             hipLaunchKernelGGL((k_ptilde<1, true>), dim3(gN_), dim3(BLK), 0, st_, N, nullptr, ws_->pt.as<double>(),
                                ws_->v.as<double>(), inv_diag_, cs_, part_, k_, kshift_, dof_col_);
@@ -2517,6 +2568,11 @@ class ChunkRun {
                            ws_->hb.as<double>(), ws_->x.as<double>(), cs_, k_, kshift_, ws_->pt.as<double>(),
                            cap_ ? ring_v(push) : nullptr, cap_ ? ring_p(push) : nullptr, dof_col_);
         WX_HIP(hipGetLastError());
+    }
+
+    // the load's flags in pinned memory: a non-finite response is an error
+    void check_load() const {
+        if (ws_->hbad[0]) fail(WX_ERR_INVALID_INPUT, "Solver::solve: response contains non-finite values");
     }
 
     bool exhausted() const { return itn_ >= maxiter_; }
@@ -2542,15 +2598,17 @@ class ChunkRun {
         WX_HIP(hipGetLastError());
     }
 
-    int poll(bool wait) {
-        if (launches_ < 2) return 0;
-        const int slot = (launches_ - 2) & 1;
+    // lag 2: the group before the last one; lag 1: the last group itself
+    int poll(bool wait, int lag = 2) {
+        if (launches_ < lag) return 0;
+        const int slot = (launches_ - lag) & 1;
         if (wait) WX_HIP(hipEventSynchronize(ws_->flag_ev[slot]));
         else {
             const hipError_t q = hipEventQuery(ws_->flag_ev[slot]);
             if (q == hipErrorNotReady) return -1;
             WX_HIP(q);
         }
+        load_done_ = true;
         // the mirror holds the flags of that group or of a later one: a column that reads 0
         // has stopped, so "all 0" is final either way
         const volatile int32_t *f = ws_->hmirror;
@@ -2559,7 +2617,29 @@ class ChunkRun {
         return 1;
     }
 
+    // True when the slowest active column of a completed group is expected to meet its test
+    // within the next group, from its progress (k_alpha_step's test value over the tolerance,
+    // <= 1 once met) and the rate at which that fell since the previous reading.  A driver
+    // then waits for the last group (lag 1) rather than launch one more that would find every
+    // column stopped.  Only the launch schedule depends on it, never a result.
+    bool near_done() {
+        const volatile int32_t *f = ws_->hmirror;
+        const volatile float *pg = ws_->hprogress;
+        float m = 0.0f;
+        for (uint32_t c = 0; c < ncol_; ++c)
+            if (f[c]) m = std::max(m, (float)pg[c]);
+        const float prev = prev_progress_;
+        prev_progress_ = m;
+        if (!(m > 0.0f)) return false;
+        const float rate = prev > m ? prev / m : 1.0f;
+        return m / rate <= 2.0f;
+    }
+
     void finish() {
+        if (!pk_) { // the load's validation (finish_init did not wait for it)
+            if (!load_done_) WX_HIP(hipStreamSynchronize(st_));
+            check_load();
+        }
         const DesignArgs &da = da_;
         const uint64_t n = n_, N = N_;
         const uint32_t k = k_, kshift = kshift_, kp = kp_;
@@ -2571,8 +2651,12 @@ class ChunkRun {
         // demeaned output goes straight to caller order
         const bool direct = !pk_ && Dm_ && !true_res && dv_.codes4.p && da.T >= 1 && da.T <= 4;
         if (direct) {
+            const uint32_t gd = (uint32_t)std::max<uint64_t>(
+                1, std::min<uint64_t>((n + DEMEAN_TILE - 1) / DEMEAN_TILE, MAX_PART_BLOCKS));
 #define WX_DS(TT)                                                                                                \
-    hipLaunchKernelGGL((k_demean_store<TT>), dim3(gt_), dim3(BLK), tile_bytes_, st, da, dv_.codes4.as<uint4>(),   \
+    hipLaunchKernelGGL((k_demean_store<TT>), dim3(gd), dim3(BLK), (size_t)DEMEAN_TILE * (k + 1) * sizeof(double), \
+                       st, da,                                                                                     \
+                       dv_.codes4.as<uint4>(),                                                                     \
                        inv_perm_, ws.x.as<double>(), ws.yint.as<double>(), Dm_ + c0_ * n, n, k, kshift)
             switch (da.T) {
             case 1: WX_DS(1); break;
@@ -2797,7 +2881,7 @@ class ChunkRun {
             }
         }
         hipLaunchKernelGGL(k_alpha_step, dim3(ncol_), dim3(BLK), 0, st, part_, gN_, kp_, cs_, k, rg_dof_,
-                           ws.dmirror);
+                           ws.dmirror, ws.dprogress);
         const uint64_t push = cap_ ? ring_advance() : 0;
         hipLaunchKernelGGL(k_update_solution, dim3(gN_), dim3(BLK), 0, st, N, ws.v.as<double>(), ws.h.as<double>(),
                            ws.hb.as<double>(), ws.x.as<double>(), cs_, k, kshift, ws.pt.as<double>(),
@@ -2878,6 +2962,8 @@ class ChunkRun {
     uint64_t cap_ = 0, ring_next_ = 0, ring_count_ = 0;
     DevBuf ringV_, ringP_, ringc_;
     int group_ = 1, itn_ = 0, launches_ = 0;
+    bool load_done_ = false; // an event recorded after the load has completed
+    float prev_progress_ = -1.0f;
     bool use_graph_ = false;
     uint64_t graph_key_ = 0;
 };
@@ -2907,6 +2993,8 @@ void solve_batch_device(const Solver &s, const double *Y, uint64_t ldy, uint64_t
         while (!run.exhausted()) {
             run.launch_group();
             if (run.poll(true) == 1) break;
+            // near convergence, wait for the last group too rather than launch one more
+            if (run.near_done() && run.poll(true, 1) == 1) break;
         }
         run.finish();
     }

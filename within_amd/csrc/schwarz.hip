@@ -227,7 +227,8 @@ __global__ __launch_bounds__(BLK) void sk_restrict_dof(uint64_t N, const uint64_
         double p;
         if (!act) p = pt[q];
         else {
-            p = brk ? 0.0 : bi * atu[q] + (-pc) * pt[q];
+            if (fz.init) p = bi * pt[q];
+            else p = brk ? 0.0 : bi * atu[q] + (-pc) * pt[q];
             pt[q] = p;
         }
         const uint64_t e = d_ptr[d + 1];

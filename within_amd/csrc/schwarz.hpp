@@ -101,6 +101,7 @@ struct Precond::Schwarz {
 //    the prolongation, laid out part[(b * kp + j) * 3 + f] over `dots_blocks` blocks.
 struct ApplyFusion {
     const double *atu = nullptr; // A'u (null: p~ is the input as it stands)
+    bool init = false;           // the first step instead: p~ <- beta_inv p~ (p~ holds A'b)
     const double *beta_inv = nullptr, *p_coeff = nullptr;
     const int32_t *brk = nullptr, *active = nullptr;
     double *dots = nullptr; // null: no partial sums
