@@ -523,11 +523,26 @@ __global__ __launch_bounds__(BLK) void k_segreduce2(TermInfo ti, const uint32_t 
                 a1 += SCALE ? sc[q] * b[q].y : b[q].y;
             }
         }
-        for (; p < item.end; ++p) {
-            const uint64_t row = perm ? perm[p] : p;
-            const dbl2 b = *(const dbl2 *)&src[row * k + j0];
-            a0 += SCALE ? scale_pos[p] * b.x : b.x;
-            a1 += SCALE ? scale_pos[p] * b.y : b.y;
+        if (p < item.end) { // the last rows: their loads in flight together, added in order
+            const uint32_t rem = item.end - p;
+            uint32_t row[U - 1];
+            double sc[U - 1];
+            dbl2 b[U - 1];
+#pragma unroll
+            for (int q = 0; q < (int)U - 1; ++q)
+                if ((uint32_t)q < rem) {
+                    row[q] = perm ? perm[p + q] : p + q;
+                    sc[q] = SCALE ? scale_pos[p + q] : 1.0;
+                }
+#pragma unroll
+            for (int q = 0; q < (int)U - 1; ++q)
+                if ((uint32_t)q < rem) b[q] = *(const dbl2 *)&src[(uint64_t)row[q] * k + j0];
+#pragma unroll
+            for (int q = 0; q < (int)U - 1; ++q)
+                if ((uint32_t)q < rem) {
+                    a0 += SCALE ? sc[q] * b[q].x : b[q].x;
+                    a1 += SCALE ? sc[q] * b[q].y : b[q].y;
+                }
         }
         dbl2 av;
         av.x = a0;

@@ -31,7 +31,8 @@ template <typename T> inline T *dp(const DevBuf &b) { return static_cast<T *>(b.
 
 constexpr int BLK = 256;
 constexpr int SP_BLK = 1024; // threads of the per-component sparse triangular solve (one workgroup per component)
-constexpr uint32_t ROW_CHUNK = 256;   // CSR entries per work item before a row is split
+constexpr uint32_t ROW_CHUNK = 128;   // CSR entries per work item before a row is split
+                                      // (256: too few items for the C' SpMV's long rows)
 constexpr uint32_t MEAN_CHUNK = 64;   // slots per partial-sum chunk
 constexpr uint32_t DENSE_TILED_MIN = 256; // kept blocks at least this large use the tiled product
 constexpr uint64_t DENSE_TILE_TARGET = 8192; // (row tile, column segment) pairs wanted in flight
@@ -510,21 +511,32 @@ __global__ __launch_bounds__(BLK) void sk_rows4(const RowItem *__restrict__ item
                 }
             }
         }
-        for (; p < item.end; ++p) {
-            const uint64_t e = idx[p];
-            const double v = (double)val[p];
-            const flt4 g = *(const flt4 *)&(MODE == 0 ? loc : xs)[e * k + j0];
-            if (MODE == 0) {
-                a0 += v * ((double)g.x - m0);
-                a1 += v * ((double)g.y - m1);
-                a2 += v * ((double)g.z - m2);
-                a3 += v * ((double)g.w - m3);
-            } else {
-                a0 += v * (double)g.x;
-                a1 += v * (double)g.y;
-                a2 += v * (double)g.z;
-                a3 += v * (double)g.w;
-            }
+        if (p < item.end) { // the last entries: their gathers in flight together, summed in order
+            const uint32_t rem = (uint32_t)(item.end - p);
+            double v[7];
+            flt4 g[7];
+#pragma unroll
+            for (int q = 0; q < 7; ++q)
+                if ((uint32_t)q < rem) {
+                    const uint64_t e = idx[p + q];
+                    v[q] = (double)val[p + q];
+                    g[q] = *(const flt4 *)&(MODE == 0 ? loc : xs)[e * k + j0];
+                }
+#pragma unroll
+            for (int q = 0; q < 7; ++q)
+                if ((uint32_t)q < rem) {
+                    if (MODE == 0) {
+                        a0 += v[q] * ((double)g[q].x - m0);
+                        a1 += v[q] * ((double)g[q].y - m1);
+                        a2 += v[q] * ((double)g[q].z - m2);
+                        a3 += v[q] * ((double)g[q].w - m3);
+                    } else {
+                        a0 += v[q] * (double)g[q].x;
+                        a1 += v[q] * (double)g[q].y;
+                        a2 += v[q] * (double)g[q].z;
+                        a3 += v[q] * (double)g[q].w;
+                    }
+                }
         }
         if (item.dest >= 0) {
             const flt4 b = *(const flt4 *)&loc[(uint64_t)item.row * k + j0];
