@@ -81,6 +81,10 @@ struct SchwarzDev {
     // large dense kept blocks: split-K tiles, per-component segment counts and row starts
     DevBuf dtiles, dcomps, drow_start;
     uint32_t n_dtiles = 0, n_dcomps = 0;
+    // FP32 runs: the large blocks as their upper triangle of 64 x 64 blocks (sk_dense_upper),
+    // one work item per super-tile of 4 x 4 blocks
+    DevBuf dsym, stiles;
+    uint32_t n_stiles = 0;
     uint64_t dense_part_rows = 0;
     uint64_t dense_tiled_rows = 0; // kept rows of the tiled dense blocks (sk_dense_reduce's outputs)
     // sparse kept blocks (one workgroup per component): levels and factor rows
@@ -691,6 +695,165 @@ __global__ __launch_bounds__(64) void sk_dense_sym(const DenseTile *__restrict__
     }
 }
 
+// Large dense blocks of an FP32 run by their upper triangle.  S^+ is symmetric, so a component
+// stores only its 64 x 64 blocks (I, J) with I <= J (row-major inside a block, zero past nk),
+// block row by block row; the product reads each stored block once and applies it twice:
+// y_I += B x_J and, off the diagonal, y_J += B' x_I.  A workgroup takes a super-tile of 4 x 4
+// blocks (super-row P, super-column Q, P <= Q), a wavefront per block row.  Each wavefront
+// stages its block in LDS (coalesced loads, XOR-swizzled so both the row and the column reads
+// are free of bank conflicts), accumulates y_I over the super-tile's block columns, and the
+// four wavefronts' y_J are added in wavefront order through LDS.  Every super-tile writes one
+// partial per block row (source Q + 1) and one per block column (source P) of its component,
+// so output row i of block K has nt + 1 partials (nt super-rows), which sk_dense_reduce adds in
+// source order: the result is deterministic, and exactly symmetric in the stored coefficients.
+struct SymTile {
+    uint64_t blk_off;  // the component's first stored block, in floats
+    uint32_t nk, nb;   // kept levels, 64-row blocks
+    uint32_t kept_base;
+    uint32_t P, Q;     // super-row, super-column
+    uint32_t pad;
+    uint64_t part_off; // partial rows of the component: part[(part_off + src * nk + i) * k + c]
+};
+
+__device__ inline uint64_t upper_block(uint32_t I, uint32_t J, uint32_t nb) {
+    return (uint64_t)I * nb - (uint64_t)I * (I - 1) / 2 + (J - I);
+}
+// float index of element (r, c) of a block staged in LDS: rows of 64 floats, the 16-byte chunks
+// of row r permuted by r mod 8, so the 16-byte row reads and writes of eight lanes, and the
+// column reads, fall in distinct banks
+__device__ inline uint32_t sym_lds(uint32_t r, uint32_t c) { return r * 64 + ((((c >> 2) ^ (r & 7)) << 2) | (c & 3)); }
+
+typedef float sym_f4 __attribute__((ext_vector_type(4)));
+
+// One super-tile per workgroup, a wavefront per block row I; the products on the matrix cores
+// (v_mfma_f32_16x16x4f32: D[i][j] += sum_k A[i][k] B[k][j], lane l holding A[l % 16][l / 16],
+// B[l / 16][l % 16] and D[4 (l / 16) + v][l % 16] in register v).  The 64-term inner dimension
+// is taken as 16 steps s, lane group g = l / 16 supplying term 16 g + s, so a lane's A values
+// for y_I = B x_J are 16 consecutive coefficients of one row of B, and for y_J = B' x_I 16
+// consecutive coefficients of one column.  kc <= 16 right-hand-side columns from c0.
+__global__ __launch_bounds__(256) void sk_dense_upper(const SymTile *__restrict__ tiles,
+                                                      const float *__restrict__ blocks,
+                                                      const float *__restrict__ red, float *__restrict__ part,
+                                                      uint32_t kstride, uint32_t c0, uint32_t kc,
+                                                      const int32_t *__restrict__ gate) {
+    if (gate_closed(gate, kstride)) return;
+    __shared__ float lds[4 * 64 * 64];
+    const SymTile T = tiles[blockIdx.x];
+    const uint32_t w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
+    const uint32_t li = lane & 15, lg = lane >> 4; // MFMA row / column index, lane group
+    float *B = lds + w * 4096;
+    const uint32_t I = 4 * T.P + w;
+    const bool row_ok = I < T.nb;
+    const uint32_t rows_I = row_ok ? min(64u, T.nk - 64 * I) : 0u;
+    const float *X = red + (uint64_t)T.kept_base * kstride + c0;
+    // x_I as the B operand: lane holds x[64 I + 16 lg + s][li], s = 0 .. 15 (0 past nk or kc)
+    float xi[16];
+#pragma unroll
+    for (int q = 0; q < 16; ++q) {
+        const uint32_t r = 16 * lg + q;
+        xi[q] = (r < rows_I && li < kc) ? X[(uint64_t)(64 * I + r) * kstride + li] : 0.0f;
+    }
+    sym_f4 acc_row[4];
+#pragma unroll
+    for (int m = 0; m < 4; ++m) acc_row[m] = sym_f4{0.0f, 0.0f, 0.0f, 0.0f};
+    for (uint32_t jj = 0; jj < 4; ++jj) {
+        const uint32_t J = 4 * T.Q + jj;
+        const bool blk = row_ok && J < T.nb && I <= J;
+        const uint32_t cols_J = J < T.nb ? min(64u, T.nk - 64 * J) : 0u;
+        float xj[16];
+        if (blk) {
+            // stage block (I, J): 16 coalesced loads of 64 x 16 bytes, chunk m = rows 4m .. 4m + 3
+            const float4 *src = (const float4 *)(blocks + T.blk_off + upper_block(I, J, T.nb) * 4096);
+#pragma unroll
+            for (int h = 0; h < 2; ++h) {
+                float4 v[8];
+#pragma unroll
+                for (int m = 0; m < 8; ++m) v[m] = src[(8 * h + m) * 64 + lane];
+#pragma unroll
+                for (int m = 0; m < 8; ++m) {
+                    const uint32_t r = 4 * (8 * h + m) + (lane >> 4), c = 4 * (lane & 15);
+                    *(float4 *)&B[sym_lds(r, c)] = v[m];
+                }
+            }
+#pragma unroll
+            for (int q = 0; q < 16; ++q) {
+                const uint32_t r = 16 * lg + q;
+                xj[q] = (r < cols_J && li < kc) ? X[(uint64_t)(64 * J + r) * kstride + li] : 0.0f;
+            }
+        }
+        __syncthreads();
+        sym_f4 acc_col[4];
+#pragma unroll
+        for (int m = 0; m < 4; ++m) acc_col[m] = sym_f4{0.0f, 0.0f, 0.0f, 0.0f};
+        if (blk) {
+#pragma unroll
+            for (int m = 0; m < 4; ++m) { // y_I rows 16 m .. 16 m + 15: A = B[16 m + li][16 lg + s]
+                float a[16];
+#pragma unroll
+                for (int q = 0; q < 4; ++q) {
+                    const float4 t = *(const float4 *)&B[sym_lds(16 * m + li, 16 * lg + 4 * q)];
+                    a[4 * q] = t.x;
+                    a[4 * q + 1] = t.y;
+                    a[4 * q + 2] = t.z;
+                    a[4 * q + 3] = t.w;
+                }
+#pragma unroll
+                for (int q = 0; q < 16; ++q)
+                    acc_row[m] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[q], xj[q], acc_row[m], 0, 0, 0);
+            }
+            if (J != I) {
+#pragma unroll
+                for (int m = 0; m < 4; ++m) { // y_J rows 16 m .. 16 m + 15: A = B[16 lg + s][16 m + li]
+#pragma unroll
+                    for (int q = 0; q < 16; ++q)
+                        acc_col[m] = __builtin_amdgcn_mfma_f32_16x16x4f32(B[sym_lds(16 * lg + q, 16 * m + li)], xi[q],
+                                                                          acc_col[m], 0, 0, 0);
+                }
+            }
+        }
+        __syncthreads(); // every wavefront is done with its block
+        // this wavefront's y_J to LDS as [column][row], columns 65 floats apart
+#pragma unroll
+        for (int m = 0; m < 4; ++m)
+#pragma unroll
+            for (int v = 0; v < 4; ++v) B[li * 65 + 16 * m + 4 * lg + v] = acc_col[m][v];
+        __syncthreads();
+        if (J < T.nb) { // block column J's partial from this super-row: the four rows' sums in order
+            for (uint32_t q = threadIdx.x; q < 64 * kc; q += 256) {
+                const uint32_t c = q >> 6, l = q & 63;
+                const float sum = ((lds[c * 65 + l] + lds[4096 + c * 65 + l]) + lds[8192 + c * 65 + l]) +
+                                  lds[12288 + c * 65 + l];
+                if (l < cols_J) part[(T.part_off + (uint64_t)T.P * T.nk + 64 * J + l) * kstride + c0 + c] = sum;
+            }
+        }
+        __syncthreads();
+    }
+    if (li < kc) { // block row I's partial from this super-column
+#pragma unroll
+        for (int m = 0; m < 4; ++m)
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                const uint32_t r = 16 * m + 4 * lg + v;
+                if (r < rows_I)
+                    part[(T.part_off + (uint64_t)(T.Q + 1) * T.nk + 64 * I + r) * kstride + c0 + li] = acc_row[m][v];
+            }
+    }
+}
+
+// Packs component blocks: dst block b (I <= J, block-row order) from the row-major nk x nk src.
+template <typename CT>
+__global__ __launch_bounds__(256) void sk_pack_upper(const CT *__restrict__ src, uint32_t nk, uint32_t nb,
+                                                     float *__restrict__ dst) {
+    const uint64_t b = blockIdx.x;
+    uint32_t I = 0;
+    while (upper_block(I + 1, I + 1, nb) <= b && I + 1 < nb) ++I;
+    const uint32_t J = (uint32_t)(b - upper_block(I, I, nb)) + I;
+    for (uint32_t e = threadIdx.x; e < 4096; e += 256) {
+        const uint32_t r = 64 * I + (e >> 6), c = 64 * J + (e & 63);
+        dst[b * 4096 + e] = (r < nk && c < nk) ? (float)src[(uint64_t)r * nk + c] : 0.0f;
+    }
+}
+
 // xs[kept_base + i] = sum over segments (in order) of the partials
 template <typename CT>
 __global__ __launch_bounds__(BLK) void sk_dense_reduce(const DenseComp *__restrict__ comps, uint32_t n_comps,
@@ -1264,14 +1427,17 @@ std::shared_ptr<SchwarzDev<CT>> build_dev(const Precond::Schwarz &S) {
     uint64_t row_tiles = 0;
     for (uint64_t c = 0; c < nc; ++c)
         if (S.fac_kind[c] == 0 && S.nk[c] >= DENSE_TILED_MIN) row_tiles += (S.nk[c] + 63) / 64;
-    // device layout of the dense operators: a tiled block's rows get a leading dimension padded
-    // to a multiple of 64 coefficients and the block a 64-coefficient-aligned start, so each
-    // wave's load of 64 consecutive coefficients covers whole 128-byte lines; the other blocks
-    // stay packed in the host order
+    // device layout of the dense operators: in an FP64 run a tiled block's rows get a leading
+    // dimension padded to a multiple of 64 coefficients and the block a 64-coefficient-aligned
+    // start, so each wave's load of 64 consecutive coefficients covers whole 128-byte lines (an
+    // FP32 run keeps its tiled blocks in dsym instead); the other blocks stay packed in the host
+    // order
     std::vector<uint32_t> dorder; // dense components by host offset
     for (uint64_t c = 0; c < nc; ++c)
         if (S.fac_kind[c] == 0) dorder.push_back((uint32_t)c);
     std::sort(dorder.begin(), dorder.end(), [&](uint32_t a, uint32_t b) { return S.dense_off[a] < S.dense_off[b]; });
+    // FP32 runs keep the tiled blocks only as their upper block triangle (dsym), outside dmat
+    constexpr bool sym = std::is_same<CT, float>::value;
     std::vector<uint64_t> doff(nc, 0);
     std::vector<uint32_t> dld(nc, 0);
     uint64_t dtotal = 0;
@@ -1279,6 +1445,10 @@ std::shared_ptr<SchwarzDev<CT>> build_dev(const Precond::Schwarz &S) {
     for (uint32_t c : dorder) {
         const uint32_t nk = S.nk[c];
         const bool tiled = nk >= DENSE_TILED_MIN;
+        if (tiled && sym) { // dld 0: not in dmat
+            relayout = true;
+            continue;
+        }
         dld[c] = tiled ? (nk + 63) / 64 * 64 : nk;
         if (tiled) dtotal = (dtotal + 63) / 64 * 64;
         doff[c] = dtotal;
@@ -1286,11 +1456,25 @@ std::shared_ptr<SchwarzDev<CT>> build_dev(const Precond::Schwarz &S) {
         relayout = relayout || doff[c] != S.dense_off[c] || dld[c] != nk;
     }
     std::vector<DenseTile> tiles;
-    uint64_t part_rows = 0;
+    std::vector<SymTile> stiles;
+    std::vector<uint64_t> sym_off(nc, 0);
+    uint64_t part_rows = 0, sym_total = 0;
     for (uint64_t c = 0; c < nc; ++c) {
         if (S.fac_kind[c] != 0) continue;
         const uint32_t nk = S.nk[c];
         const uint32_t kb = (uint32_t)(S.slot_start[c] + S.ne[c]);
+        if (nk >= DENSE_TILED_MIN && sym) {
+            // super-tiles (P, Q), P <= Q, of 4 x 4 blocks; nt + 1 partial sources per row
+            const uint32_t nb = (nk + 63) / 64, nt = (nb + 3) / 4;
+            sym_off[c] = sym_total;
+            for (uint32_t P = 0; P < nt; ++P)
+                for (uint32_t Q = P; Q < nt; ++Q) stiles.push_back({sym_total, nk, nb, kb, P, Q, 0, part_rows});
+            sym_total += (uint64_t)nb * (nb + 1) / 2 * 4096;
+            dcomps.push_back(DenseComp{nk, kb, nt + 1, 0, part_rows});
+            part_rows += (uint64_t)(nt + 1) * nk;
+            drow_start.push_back(drow_start.back() + nk);
+            continue;
+        }
         if (nk >= DENSE_TILED_MIN) {
             // enough (row tile, column segment) pairs to fill the device, segments >= 256 columns
             const uint32_t max_seg = (nk + 255) / 256;
@@ -1318,6 +1502,8 @@ std::shared_ptr<SchwarzDev<CT>> build_dev(const Precond::Schwarz &S) {
     }
     D->n_dtiles = (uint32_t)tiles.size();
     D->dtiles.upload(tiles.data(), tiles.size());
+    D->n_stiles = (uint32_t)stiles.size();
+    D->stiles.upload(stiles.data(), stiles.size());
     D->n_dcomps = (uint32_t)dcomps.size();
     D->dcomps.upload(dcomps.data(), dcomps.size());
     D->drow_start.upload(drow_start.data(), drow_start.size());
@@ -1367,6 +1553,18 @@ std::shared_ptr<SchwarzDev<CT>> build_dev(const Precond::Schwarz &S) {
             }
         }
         WX_HIP(hipDeviceSynchronize());
+        if (sym_total) { // the tiled blocks' upper block triangles
+            D->dsym.alloc(sym_total * sizeof(float));
+            for (uint64_t c = 0; c < nc; ++c) {
+                if (S.fac_kind[c] != 0 || S.nk[c] < DENSE_TILED_MIN) continue;
+                const uint32_t nk = S.nk[c], nb = (nk + 63) / 64;
+                hipLaunchKernelGGL((sk_pack_upper<CT>), dim3((uint32_t)((uint64_t)nb * (nb + 1) / 2)), dim3(256), 0, 0,
+                                   raw.template as<CT>() + S.dense_off[c], nk, nb,
+                                   D->dsym.template as<float>() + sym_off[c]);
+                WX_HIP(hipGetLastError());
+            }
+            WX_HIP(hipDeviceSynchronize());
+        }
         if (!relayout) {
             D->dmat = std::move(raw);
         } else {
@@ -1377,6 +1575,10 @@ std::shared_ptr<SchwarzDev<CT>> build_dev(const Precond::Schwarz &S) {
             for (size_t q = 0; q < dorder.size();) {
                 const uint32_t c = dorder[q];
                 const uint64_t nk = S.nk[c];
+                if (dld[c] == 0) { // kept in dsym only
+                    ++q;
+                    continue;
+                }
                 if (dld[c] != nk) {
                     WX_HIP(hipMemcpy2D(dst + doff[c], dld[c] * sizeof(CT), src + S.dense_off[c], nk * sizeof(CT),
                                        nk * sizeof(CT), nk, hipMemcpyDeviceToDevice));
@@ -1873,6 +2075,11 @@ struct SchwarzRunImpl : SchwarzRun {
                                        dp<uint32_t>(d.cover_rows), d.n_cover_rows, dp<uint32_t>(d.aux_src), d.ns,
                                        xsp, sheet, k, kshift, gate);
         }
+        if constexpr (std::is_same<CT, float>::value)
+            for (uint32_t c0 = 0; d.n_stiles && c0 < k; c0 += 16)
+                hipLaunchKernelGGL(sk_dense_upper, dim3(d.n_stiles), dim3(256), 0, st, dp<SymTile>(d.stiles),
+                                   dp<float>(d.dsym), redp, dp<float>(dpart), k, c0, std::min<uint32_t>(16, k - c0),
+                                   gate);
         if (d.n_dtiles) {
             // column windows of exact power-of-two widths (<= 16): the wave-uniform red loads
             // never read past column k
@@ -1890,6 +2097,8 @@ struct SchwarzRunImpl : SchwarzRun {
 #undef WX_DS
                 c0 += kc;
             }
+        }
+        if (d.n_dtiles || d.n_stiles) {
             hipLaunchKernelGGL((sk_dense_reduce<CT>), dim3(grid_rows(d.dense_tiled_rows, kshift)), dim3(BLK), 0, st,
                                dp<DenseComp>(d.dcomps), d.n_dcomps, dp<uint64_t>(d.drow_start), dp<CT>(dpart), xsp,
                                k, kshift, gate);
