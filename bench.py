@@ -136,9 +136,12 @@ def run_plain_fp64(solver, Y_dev, n, N, tol, n_rhs):
 def schwarz_bytes(pre_bytes: bytes, k: int, value_bytes: int, N: int) -> dict:
     """Algorithmic bytes of one additive Schwarz apply (the per-apply formula of the hot-path
     scope: factor and cross-tab entries at (index + value) bytes, each read twice -- once by each
-    elimination SpMV, once by each triangular sweep --, dense kept blocks read once, the
-    slot-space vectors written and read four times, plus the FP64 input and output DOF vectors),
-    from the preconditioner's exported arrays (little-endian WXPC layout)."""
+    elimination SpMV, once by each triangular sweep --, dense kept blocks read once, counted as
+    their nk (nk + 1) / 2 distinct entries since S^+ is symmetric, the slot-space vectors written
+    and read four times, plus the FP64 DOF vectors: input and output, and the LSMR steps folded
+    into the apply -- A'u read, p~ read and written by the restriction, p~ read again by the
+    prolongation for the alpha dot products), from the preconditioner's exported arrays
+    (little-endian WXPC layout)."""
     import struct
     assert pre_bytes[:4] == b"WXPC"
     _, kind, n = struct.unpack_from("<IIQ", pre_bytes, 4)
@@ -156,11 +159,17 @@ def schwarz_bytes(pre_bytes: bytes, k: int, value_bytes: int, N: int) -> dict:
         pos += 8 + m * sz
     s = value_bytes
     slots = sizes["glob"]
+    # the kept-block sizes of the dense components (fac_kind 0), from the arrays' positions
+    nc = sizes["ne"]
+    pos = 20 + 8 + sizes["slot_start"] * 8 + 8 + nc * 4
+    nk = struct.unpack_from(f"<{nc}I", pre_bytes, pos + 8)
+    kinds = pre_bytes[pos + 8 + nc * 4 + 8: pos + 8 + nc * 4 + 8 + nc]
+    dense_sym = sum(n * (n + 1) // 2 for n, kd in zip(nk, kinds) if kd == 0)
     b_c = sizes["c_idx"] * (4 + s) * 2
-    b_dense = sizes["dense"] * s
+    b_dense = dense_sym * s
     b_l = sizes["l_idx"] * (4 + s) * 2
-    b_vec = s * k * slots * 4 + 8 * k * N * 2
-    return {"bytes": b_c + b_dense + b_l + b_vec, "cross_tab_entries": sizes["c_idx"], "dense_entries": sizes["dense"],
+    b_vec = s * k * slots * 4 + 8 * k * N * 2 + 8 * k * N * 4
+    return {"bytes": b_c + b_dense + b_l + b_vec, "cross_tab_entries": sizes["c_idx"], "dense_entries": dense_sym,
             "factor_entries": sizes["l_idx"], "slots": slots}
 
 
