@@ -1362,18 +1362,18 @@ __global__ __launch_bounds__(BLK) void k_store_caller(uint64_t n, const uint32_t
 }
 
 // The demeaned output y - D x straight into caller order, column-major (k_demean2 followed by
-// k_store_caller, without the internal-order round trip): per DEMEAN_TILE-row tile of caller
-// rows, each row's internal row i = inv_perm[r] gives y_int, the code record and the
-// coefficient rows; the terms are added in order from 0.0 and subtracted from y, as k_demean2
-// does, so the values are the same bits.  For plain designs of up to four terms (code records
-// present).  (Tiles of 128 and 256 rows measured slower: 485 and 612 against 458 us at n = 5M,
-// k = 16.)
+// k_store_caller, without the internal-order round trip), per DEMEAN_TILE-row tile of caller
+// rows: the tile of the caller's response Y (the same values y_int holds in internal order) is
+// read column by column into LDS, each row's code record in caller order gives its coefficient
+// rows, the terms are added in order from 0.0 and subtracted from y, as k_demean2 does (so the
+// values are the same bits), and the tile is written column by column.  Every stream but the
+// coefficient rows is sequential.  For plain designs of up to four terms (code records present).
+// (Tiles of 128 and 256 rows measured slower.)
 constexpr uint32_t DEMEAN_TILE = 64;
 template <int TT>
-__global__ __launch_bounds__(BLK) void k_demean_store(DesignArgs d, const uint4 *__restrict__ codes4,
-                                                      const uint32_t *__restrict__ inv_perm,
-                                                      const double *__restrict__ x, const double *__restrict__ y_int,
-                                                      double *__restrict__ Dm, uint64_t ldd, uint32_t k,
+__global__ __launch_bounds__(BLK) void k_demean_store(DesignArgs d, const uint4 *__restrict__ codes4c,
+                                                      const double *__restrict__ x, const double *__restrict__ Y,
+                                                      uint64_t ldy, double *__restrict__ Dm, uint64_t ldd, uint32_t k,
                                                       uint32_t kshift) {
     extern __shared__ double tile[]; // DEMEAN_TILE x (k + 1)
     const uint32_t ld = k + 1;
@@ -1383,22 +1383,21 @@ __global__ __launch_bounds__(BLK) void k_demean_store(DesignArgs d, const uint4 
     constexpr int R = DEMEAN_TILE / 16 < 4 ? 4 : DEMEAN_TILE / 16;
     for (uint64_t tb = blockIdx.x; tb < n_tiles; tb += gridDim.x) {
         const uint64_t r0 = tb * DEMEAN_TILE;
+        for (uint32_t q = threadIdx.x; q < DEMEAN_TILE * k; q += BLK) {
+            const uint32_t c = q / DEMEAN_TILE, rr = q % DEMEAN_TILE;
+            const uint64_t r = r0 + rr;
+            if (r < n) tile[rr * ld + c] = Y[r + c * ldy];
+        }
+        __syncthreads();
         if (j < k)
             for (uint32_t rb = grp; rb < DEMEAN_TILE; rb += R * ngrp) {
-                uint64_t row[R];
                 bool ok[R];
+                uint32_t code[TT][R];
 #pragma unroll
                 for (int q = 0; q < R; ++q) {
                     const uint64_t r = r0 + rb + q * ngrp;
                     ok[q] = rb + q * ngrp < DEMEAN_TILE && r < n;
-                    row[q] = ok[q] ? (inv_perm ? inv_perm[r] : r) : 0;
-                }
-                double yv[R];
-                uint32_t code[TT][R];
-#pragma unroll
-                for (int q = 0; q < R; ++q) {
-                    yv[q] = y_int[row[q] * k + j];
-                    const uint4 c4 = codes4[row[q]];
+                    const uint4 c4 = codes4c[ok[q] ? r : 0];
                     const uint32_t cc[4] = {c4.x, c4.y, c4.z, c4.w};
 #pragma unroll
                     for (int t = 0; t < TT; ++t) code[t][q] = cc[t];
@@ -1414,7 +1413,8 @@ __global__ __launch_bounds__(BLK) void k_demean_store(DesignArgs d, const uint4 
                     double a = 0.0;
 #pragma unroll
                     for (int t = 0; t < TT; ++t) a += val[t][q];
-                    tile[(rb + q * ngrp) * ld + j] = yv[q] - a;
+                    double &yv = tile[(rb + q * ngrp) * ld + j];
+                    yv = yv - a;
                 }
             }
         __syncthreads();
@@ -1586,6 +1586,13 @@ __global__ __launch_bounds__(BLK) void k_codes4(uint64_t n, uint32_t T, const ui
         r.w = T > 3 ? codes[3 * n + i] : 0u;
         out[i] = r;
     }
+}
+
+// the records in caller row order: out[r] = codes4[inv_perm[r]]
+__global__ __launch_bounds__(BLK) void k_codes4_caller(uint64_t n, const uint32_t *__restrict__ inv_perm,
+                                                       const uint4 *__restrict__ codes4, uint4 *__restrict__ out) {
+    for (uint64_t r = (uint64_t)blockIdx.x * BLK + threadIdx.x; r < n; r += (uint64_t)gridDim.x * BLK)
+        out[r] = codes4[inv_perm[r]];
 }
 
 // The fused D' of two unsorted single-intercept terms of a plain design (k_segreduce2_pair):
@@ -1842,11 +1849,17 @@ void upload_design(Solver &s, const std::vector<double> &weights_internal, uint3
     // a row's codes as one aligned 16-byte record: the fused gather and the demean read them with
     // one load instead of one per term
     dv.codes4.release();
+    dv.codes4_caller.release();
     if (dv.plain && d.terms.size() <= 4 && !d.terms.empty()) {
         dv.codes4.alloc(n * 16);
         const uint32_t g = (uint32_t)std::min<uint64_t>((n + BLK - 1) / BLK, 1u << 20);
         hipLaunchKernelGGL(k_codes4, dim3(g), dim3(BLK), 0, st0, n, (uint32_t)d.terms.size(),
                            dv.codes.as<uint32_t>(), dv.codes4.as<uint4>());
+        if (dv.inv_perm.p) { // the demeaned output's caller-order pass reads them row by row
+            dv.codes4_caller.alloc(n * 16);
+            hipLaunchKernelGGL(k_codes4_caller, dim3(g), dim3(BLK), 0, st0, n, dv.inv_perm.as<uint32_t>(),
+                               dv.codes4.as<uint4>(), dv.codes4_caller.as<uint4>());
+        }
         WX_HIP(hipGetLastError());
         WX_HIP(hipStreamSynchronize(st0));
     }
@@ -1929,6 +1942,7 @@ void reweight_design(Solver &s, const Solver &base, const std::vector<double> &w
     clone_buf(dv.obs_perm, b.obs_perm);
     clone_buf(dv.inv_perm, b.inv_perm);
     clone_buf(dv.codes4, b.codes4);
+    clone_buf(dv.codes4_caller, b.codes4_caller);
     dv.plain = b.plain;
     dv.fuse_term = b.fuse_term;
     dv.max_partials = b.max_partials;
@@ -2666,13 +2680,15 @@ class ChunkRun {
         // demeaned output goes straight to caller order
         const bool direct = !pk_ && Dm_ && !true_res && dv_.codes4.p && da.T >= 1 && da.T <= 4;
         if (direct) {
+            // a tile per workgroup where the grid allows (4,096 workgroups looping over tiles
+            // measured 455 against 415 us at n = 5M, k = 16)
             const uint32_t gd = (uint32_t)std::max<uint64_t>(
-                1, std::min<uint64_t>((n + DEMEAN_TILE - 1) / DEMEAN_TILE, MAX_PART_BLOCKS));
+                1, std::min<uint64_t>((n + DEMEAN_TILE - 1) / DEMEAN_TILE, 65535));
 #define WX_DS(TT)                                                                                                \
     hipLaunchKernelGGL((k_demean_store<TT>), dim3(gd), dim3(BLK), (size_t)DEMEAN_TILE * (k + 1) * sizeof(double), \
                        st, da,                                                                                     \
-                       dv_.codes4.as<uint4>(),                                                                     \
-                       inv_perm_, ws.x.as<double>(), ws.yint.as<double>(), Dm_ + c0_ * n, n, k, kshift)
+                       (dv_.codes4_caller.p ? dv_.codes4_caller : dv_.codes4).as<uint4>(), ws.x.as<double>(), Y_,  \
+                       ldy_, Dm_ + c0_ * n, n, k, kshift)
             switch (da.T) {
             case 1: WX_DS(1); break;
             case 2: WX_DS(2); break;

@@ -105,6 +105,7 @@ struct DevDesign {
     DevBuf codes, slopes, sw, w, obs_perm, inv_perm, inv_diag;
     DevBuf sw_caller; // sqrt-weights in caller row order (when the rows are permuted)
     DevBuf codes4;    // plain designs with at most 4 terms: the row's codes as one 16-byte record
+    DevBuf codes4_caller; // the same records in caller row order (when the rows are permuted)
     std::vector<DevTerm> terms;
     uint32_t max_partials = 0;
     uint32_t tile = DEFAULT_TILE; // rows per D' work item (upload_design)
