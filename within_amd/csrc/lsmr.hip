@@ -2331,6 +2331,9 @@ struct Workspace {
     // the events that mark the last two iteration groups complete (the host polls one group
     // behind the device)
     int32_t *hbad = nullptr, *dbad = nullptr; // host and device addresses
+    // pinned copies of the column state and the true-residual numerators read at the end
+    double *hcols = nullptr, *hnum = nullptr;
+    int32_t *hints = nullptr;
     hipEvent_t flag_ev[2] = {nullptr, nullptr};
     // host-coherent pinned copy of the "active" flags that k_alpha_step stores every
     // iteration (host and device addresses); a flag, once 0, stays 0 for the rest of a solve.
@@ -2350,6 +2353,9 @@ struct Workspace {
     ~Workspace() {
         drop_graph();
         if (hbad) (void)hipHostFree(hbad);
+        if (hcols) (void)hipHostFree(hcols);
+        if (hnum) (void)hipHostFree(hnum);
+        if (hints) (void)hipHostFree(hints);
         if (hmirror) (void)hipHostFree(hmirror);
         for (auto &e : flag_ev)
             if (e) (void)hipEventDestroy(e);
@@ -2404,6 +2410,9 @@ std::shared_ptr<Workspace> new_workspace(const DevDesign &dv, uint32_t k, uint32
     w->bad.alloc(8);
     w->red.alloc(2 * (size_t)w->cap * 8);
     WX_HIP(hipHostMalloc((void **)&w->hbad, 2 * sizeof(int32_t), hipHostMallocCoherent));
+    WX_HIP(hipHostMalloc((void **)&w->hcols, (size_t)N_DOUBLE_FIELDS * w->cap * 8, hipHostMallocDefault));
+    WX_HIP(hipHostMalloc((void **)&w->hnum, (size_t)w->cap * 8, hipHostMallocDefault));
+    WX_HIP(hipHostMalloc((void **)&w->hints, (size_t)N_INT_FIELDS * w->cap * 4, hipHostMallocDefault));
     WX_HIP(hipHostGetDevicePointer((void **)&w->dbad, w->hbad, 0));
     WX_HIP(hipHostMalloc((void **)&w->hmirror, (size_t)w->cap * 8, hipHostMallocCoherent));
     std::fill(w->hmirror, w->hmirror + w->cap, 0);
@@ -2740,13 +2749,15 @@ class ChunkRun {
         }
         WX_HIP(hipGetLastError());
         const size_t cap = ws.cap;
-        std::vector<double> num(ncol_, std::nan("")), host_d(N_DOUBLE_FIELDS * cap);
-        std::vector<int32_t> host_i(N_INT_FIELDS * cap);
+        // the column state through the workspace's pinned buffers (asynchronous copies, one wait)
         if (true_res)
-            WX_HIP(hipMemcpyAsync(num.data(), ws.red.as<double>(), ncol_ * 8, hipMemcpyDeviceToHost, st));
-        WX_HIP(hipMemcpyAsync(host_d.data(), ws.cols.p, host_d.size() * 8, hipMemcpyDeviceToHost, st));
-        WX_HIP(hipMemcpyAsync(host_i.data(), ws.ints.p, host_i.size() * 4, hipMemcpyDeviceToHost, st));
+            WX_HIP(hipMemcpyAsync(ws.hnum, ws.red.as<double>(), ncol_ * 8, hipMemcpyDeviceToHost, st));
+        WX_HIP(hipMemcpyAsync(ws.hcols, ws.cols.p, N_DOUBLE_FIELDS * cap * 8, hipMemcpyDeviceToHost, st));
+        WX_HIP(hipMemcpyAsync(ws.hints, ws.ints.p, N_INT_FIELDS * cap * 4, hipMemcpyDeviceToHost, st));
         WX_HIP(hipStreamSynchronize(st));
+        std::vector<double> num(ncol_, std::nan("")), host_d(ws.hcols, ws.hcols + N_DOUBLE_FIELDS * cap);
+        std::vector<int32_t> host_i(ws.hints, ws.hints + N_INT_FIELDS * cap);
+        if (true_res) std::copy(ws.hnum, ws.hnum + ncol_, num.begin());
         if (!pk_ && X_ && s_.reparam && !x_solver_basis_) {
             std::vector<double> xi(N * k), xo(N);
             WX_HIP(hipMemcpy(xi.data(), ws.x.p, N * k * 8, hipMemcpyDeviceToHost));
