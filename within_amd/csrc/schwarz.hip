@@ -34,6 +34,7 @@ constexpr int SP_BLK = 1024; // threads of the per-component sparse triangular s
 constexpr uint32_t ROW_CHUNK = 128;   // CSR entries per work item before a row is split
                                       // (256: too few items for the C' SpMV's long rows)
 constexpr uint32_t MEAN_CHUNK = 64;   // slots per partial-sum chunk
+constexpr uint32_t MEAN_ITEM = 16;    // chunks per workgroup of the single-launch means
 constexpr uint32_t DENSE_TILED_MIN = 256; // kept blocks at least this large use the tiled product
 constexpr uint64_t DENSE_TILE_TARGET = 8192; // (row tile, column segment) pairs wanted in flight
 constexpr uint64_t LONG_ROW = 64;         // factor rows with more entries are summed by a whole workgroup
@@ -74,6 +75,10 @@ struct SchwarzDev {
     // means: chunks over all slots (per component) and over kept slots of sparse components
     DevBuf all_chunks, all_cptr, kept_chunks, kept_cptr;
     uint32_t n_all_chunks = 0, n_kept_chunks = 0;
+    // the single-launch means (sk_mean): per component, its first work item of MEAN_ITEM
+    // chunks in each list (nc + 1 entries)
+    DevBuf all_iptr, kept_iptr;
+    uint32_t n_all_items = 0, n_kept_items = 0;
     DevBuf comp_count, comp_kcount;
     // dense kept blocks: one entry per kept row: (matrix row offset, first kept slot, nk)
     DevBuf drow_off, drow_base, drow_nk, drow_slot, dmat;
@@ -318,6 +323,62 @@ __global__ __launch_bounds__(BLK) void sk_chunk_sum(const Chunk *__restrict__ ch
     }
     for (; s < ch.end; ++s) acc += (double)src[(uint64_t)s * k + j];
     part[(uint64_t)g * k + j] = acc;
+}
+
+// Component means in one launch: workgroup b takes MEAN_ITEM chunks of one component (its item
+// b - iptr[c]); its row groups add their chunks' slots in order, a fixed LDS tree adds the
+// groups, and the item's sums go to part[b].  The workgroup that finishes a component's last
+// item (a per-component counter, agent-scope fences around it) adds the component's item sums
+// in item order (comp_mean_one) and resets the counter for the next launch.  No workgroup
+// waits on another, and the sums do not depend on the order in which workgroups finish.
+template <typename VT>
+__global__ __launch_bounds__(BLK) void sk_mean(const Chunk *__restrict__ chunks, const uint32_t *__restrict__ cptr,
+                                               const uint32_t *__restrict__ iptr, uint32_t nc,
+                                               const VT *__restrict__ src, const uint32_t *__restrict__ count,
+                                               double *part, uint32_t *counter, double *__restrict__ mu, uint32_t k,
+                                               uint32_t kshift, const int32_t *__restrict__ gate) {
+    if (gate_closed(gate, k)) return;
+    __shared__ double sh[BLK];
+    __shared__ uint32_t last;
+    const uint32_t b = blockIdx.x;
+    uint32_t lo = 0, hi = nc - 1; // the component of item b: the last c with iptr[c] <= b
+    while (lo < hi) {
+        const uint32_t mid = (lo + hi + 1) / 2;
+        if (iptr[mid] <= b) lo = mid;
+        else hi = mid - 1;
+    }
+    const uint32_t c = lo;
+    const uint32_t kp = 1u << kshift, j = threadIdx.x & (kp - 1), grp = threadIdx.x >> kshift;
+    const uint32_t ngrp = BLK >> kshift;
+    const uint32_t ch0 = cptr[c] + (b - iptr[c]) * MEAN_ITEM, ch1 = min(cptr[c + 1], ch0 + MEAN_ITEM);
+    double acc = 0.0;
+    if (j < k)
+        for (uint32_t ch = ch0 + grp; ch < ch1; ch += ngrp) {
+            const Chunk C = chunks[ch];
+            uint32_t q = C.begin;
+            for (; q + 4 <= C.end; q += 4) { // four loads in flight, added in slot order
+                double t[4];
+#pragma unroll
+                for (int u = 0; u < 4; ++u) t[u] = (double)src[(uint64_t)(q + u) * k + j];
+                acc += t[0];
+                acc += t[1];
+                acc += t[2];
+                acc += t[3];
+            }
+            for (; q < C.end; ++q) acc += (double)src[(uint64_t)q * k + j];
+        }
+    const double tot = cols_block_sum(acc, kp, sh);
+    if (threadIdx.x < kp && j < k) part[(uint64_t)b * k + j] = tot;
+    __syncthreads();
+    if (threadIdx.x == 0) { // (the lanes that stored sums, kp <= 64 of them, are this wavefront's)
+        __threadfence(); // release: this item's sums before its count
+        last = atomicAdd(&counter[c], 1u) == iptr[c + 1] - iptr[c] - 1;
+    }
+    __syncthreads();
+    if (!last) return;
+    __threadfence(); // acquire: every item's sums after the count
+    comp_mean_one(c, iptr, part, count, mu, k, kshift, sh); // the item sums in item order
+    if (threadIdx.x == 0) counter[c] = 0;                  // for the next launch
 }
 
 // MODE 0: reduced rhs of kept rows:  red[row] = (loc[row]-mu) + sum_p ct[p]*(loc[e]-mu), the C'
@@ -572,20 +633,18 @@ __global__ __launch_bounds__(BLK) void sk_rows4(const RowItem *__restrict__ item
     }
 }
 
+// A split row's partials added by one workgroup: its row groups stride over the partials (a
+// hub row can have thousands), then a fixed LDS tree adds the groups' sums.  Block bid of nblk.
 template <typename CT, int MODE>
-__global__ __launch_bounds__(BLK) void sk_rows_fix(const RowFix *__restrict__ fixes, uint32_t n_fix,
-                                                   const uint32_t *__restrict__ slot_comp,
-                                                   const CT *__restrict__ loc, const CT *__restrict__ inv_de,
-                                                   const double *__restrict__ mu, const CT *__restrict__ part,
-                                                   OutT<CT, MODE> *__restrict__ out, uint32_t k, uint32_t kshift,
-                                                   const int32_t *__restrict__ gate) {
-    // one workgroup per split row: its row groups stride over the partials (a hub row can have
-    // thousands), then a fixed LDS tree adds the groups' sums
-    if (gate_closed(gate, k)) return;
+__device__ inline void rows_fix_wide(const RowFix *__restrict__ fixes, uint32_t n_fix,
+                                     const uint32_t *__restrict__ slot_comp, const CT *__restrict__ loc,
+                                     const CT *__restrict__ inv_de, const double *__restrict__ mu,
+                                     const CT *__restrict__ part, OutT<CT, MODE> *__restrict__ out, uint32_t k,
+                                     uint32_t kshift, uint32_t bid, uint32_t nblk) {
     __shared__ double sh[BLK];
     const uint32_t kp = 1u << kshift, j = threadIdx.x & (kp - 1);
     const uint32_t grp = threadIdx.x >> kshift, ngrp = BLK >> kshift;
-    for (uint32_t fi = blockIdx.x; fi < n_fix; fi += gridDim.x) {
+    for (uint32_t fi = bid; fi < n_fix; fi += nblk) {
         const RowFix f = fixes[fi];
         double acc = 0.0;
         if (j < k)
@@ -602,24 +661,22 @@ __global__ __launch_bounds__(BLK) void sk_rows_fix(const RowFix *__restrict__ fi
     }
 }
 
-// sk_rows_fix for split rows of at most FIX_SMALL partials, one row group per row instead of
-// one workgroup.  The workgroup version gives row group g the value 0.0 + partial g and then
-// adds group g + s into group g for s = ngrp/2, ..., 1; for at most FIX_SMALL <= ngrp partials
-// the levels above the partial count only add zeros, so this applies the remaining levels to
-// the same values and returns the same bits.
+// The same for split rows of at most FIX_SMALL partials, one row group per row instead of one
+// workgroup.  The workgroup version gives row group g the value 0.0 + partial g and then adds
+// group g + s into group g for s = ngrp/2, ..., 1; for at most FIX_SMALL <= ngrp partials the
+// levels above the partial count only add zeros, so this applies the remaining levels to the
+// same values and returns the same bits.
 constexpr uint32_t FIX_SMALL = 8;
 template <typename CT, int MODE>
-__global__ __launch_bounds__(BLK) void sk_rows_fix_small(const RowFix *__restrict__ fixes, uint32_t n_fix,
-                                                         const uint32_t *__restrict__ slot_comp,
-                                                         const CT *__restrict__ loc, const CT *__restrict__ inv_de,
-                                                         const double *__restrict__ mu, const CT *__restrict__ part,
-                                                         OutT<CT, MODE> *__restrict__ out, uint32_t k,
-                                                         uint32_t kshift, const int32_t *__restrict__ gate) {
-    if (gate_closed(gate, k)) return;
+__device__ inline void rows_fix_narrow(const RowFix *__restrict__ fixes, uint32_t n_fix,
+                                       const uint32_t *__restrict__ slot_comp, const CT *__restrict__ loc,
+                                       const CT *__restrict__ inv_de, const double *__restrict__ mu,
+                                       const CT *__restrict__ part, OutT<CT, MODE> *__restrict__ out, uint32_t k,
+                                       uint32_t kshift, uint32_t bid, uint32_t nblk) {
     const uint32_t kp = 1u << kshift, j = threadIdx.x & (kp - 1);
     if (j >= k) return;
-    const uint32_t stride = (gridDim.x * BLK) >> kshift;
-    for (uint32_t fi = (blockIdx.x * BLK + threadIdx.x) >> kshift; fi < n_fix; fi += stride) {
+    const uint32_t stride = (nblk * BLK) >> kshift;
+    for (uint32_t fi = (bid * BLK + threadIdx.x) >> kshift; fi < n_fix; fi += stride) {
         const RowFix f = fixes[fi];
         double v[FIX_SMALL];
 #pragma unroll
@@ -634,6 +691,25 @@ __global__ __launch_bounds__(BLK) void sk_rows_fix_small(const RowFix *__restric
                 for (uint32_t g = 0; g < h; ++g) v[g] += v[g + h];
         row_finish<CT, MODE>(f.row, v[0], loc, out, inv_de, mu + (uint64_t)slot_comp[f.row] * k, k, j);
     }
+}
+
+// Every split row in one launch: blocks [0, g_small) take the first n_small rows (at most
+// FIX_SMALL partials each) a row group per row, the other blocks the rest a workgroup per row.
+template <typename CT, int MODE>
+__global__ __launch_bounds__(BLK) void sk_rows_fix(const RowFix *__restrict__ fixes, uint32_t n_small,
+                                                   uint32_t n_fix, uint32_t g_small,
+                                                   const uint32_t *__restrict__ slot_comp,
+                                                   const CT *__restrict__ loc, const CT *__restrict__ inv_de,
+                                                   const double *__restrict__ mu, const CT *__restrict__ part,
+                                                   OutT<CT, MODE> *__restrict__ out, uint32_t k, uint32_t kshift,
+                                                   const int32_t *__restrict__ gate) {
+    if (gate_closed(gate, k)) return;
+    if (blockIdx.x < g_small)
+        rows_fix_narrow<CT, MODE>(fixes, n_small, slot_comp, loc, inv_de, mu, part, out, k, kshift, blockIdx.x,
+                                  g_small);
+    else
+        rows_fix_wide<CT, MODE>(fixes + n_small, n_fix - n_small, slot_comp, loc, inv_de, mu, part, out, k, kshift,
+                                blockIdx.x - g_small, gridDim.x - g_small);
 }
 
 // Large dense kept blocks: xs = S^+ red as a split-K product that streams S^+ once.  S^+ is
@@ -1263,6 +1339,10 @@ uint32_t grid_rows(uint64_t rows, uint32_t kshift, uint32_t cap = 65535) {
     return (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(g, cap));
 }
 
+__global__ __launch_bounds__(256) void sk_zero_words(uint32_t *__restrict__ p, uint64_t n) {
+    for (uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (uint64_t)gridDim.x * 256) p[i] = 0u;
+}
+
 __global__ __launch_bounds__(256) void sk_cast_f32(uint64_t n, const double *__restrict__ src, float *__restrict__ dst) {
     for (uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (uint64_t)gridDim.x * 256)
         dst[i] = (float)src[i];
@@ -1419,6 +1499,17 @@ std::shared_ptr<SchwarzDev<CT>> build_dev(const Precond::Schwarz &S) {
     D->kept_cptr.upload(kept_cptr.data(), kept_cptr.size());
     D->comp_count.upload(cnt.data(), cnt.size());
     D->comp_kcount.upload(kcnt.data(), kcnt.size());
+    {
+        std::vector<uint32_t> ai(nc + 1, 0), ki(nc + 1, 0);
+        for (uint64_t c = 0; c < nc; ++c) {
+            ai[c + 1] = ai[c] + (all_cptr[c + 1] - all_cptr[c] + MEAN_ITEM - 1) / MEAN_ITEM;
+            ki[c + 1] = ki[c] + (kept_cptr[c + 1] - kept_cptr[c] + MEAN_ITEM - 1) / MEAN_ITEM;
+        }
+        D->all_iptr.upload(ai.data(), ai.size());
+        D->kept_iptr.upload(ki.data(), ki.size());
+        D->n_all_items = ai[nc];
+        D->n_kept_items = ki[nc];
+    }
     // dense rows: small blocks row by row, blocks with nk >= DENSE_TILED_MIN as split-K tiles
     std::vector<uint64_t> drow_off;
     std::vector<uint32_t> drow_base, drow_nk, drow_slot;
@@ -1922,8 +2013,9 @@ struct SchwarzRunImpl : SchwarzRun {
     std::shared_ptr<SchwarzDev<CT>> D;
     uint32_t kcap = 0;
     DevBuf loc, red, xs, part, rowpart, mu1, mu2, mu3, dpart, tsc, csc;
+    DevBuf counters; // per component: items of the current sk_mean launch finished
 
-    void ensure(uint32_t k) {
+    void ensure(uint32_t k, hipStream_t st) {
         if (k <= kcap) return;
         kcap = k;
         const uint64_t ns = D->ns;
@@ -1938,6 +2030,20 @@ struct SchwarzRunImpl : SchwarzRun {
         mu1.alloc(D->nc * k * 8);
         mu2.alloc(D->nc * k * 8);
         mu3.alloc(D->nc * k * 8);
+        // a component without chunks in a list (the kept list of a dense or grounded one) has
+        // no sk_mean item and keeps the mean 0 set here; the counters start at 0 and sk_mean
+        // leaves them 0.  Cleared by a kernel on the run's stream: a memset on the default
+        // stream while another thread captures a graph can end up in that graph.
+        const bool fresh = !counters.p;
+        if (fresh) counters.alloc(D->nc * 4 + 4);
+        const uint64_t nmu = (uint64_t)D->nc * k * 2; // 32-bit words
+        for (DevBuf *m : {&mu1, &mu2, &mu3})
+            hipLaunchKernelGGL(sk_zero_words, dim3((uint32_t)std::min<uint64_t>((nmu + 255) / 256 + 1, 65535)),
+                               dim3(256), 0, st, m->as<uint32_t>(), nmu);
+        if (fresh)
+            hipLaunchKernelGGL(sk_zero_words, dim3((uint32_t)std::min<uint64_t>((D->nc + 256) / 256 + 1, 65535)),
+                               dim3(256), 0, st, counters.as<uint32_t>(), (uint64_t)D->nc + 1);
+        WX_HIP(hipGetLastError());
         dpart.alloc(D->dense_part_rows * k * V + 8);
         tsc.alloc(D->n_tail_rows * k * V + 8);
         csc.alloc(D->n_tail_chunks * k * V + 8);
@@ -1975,34 +2081,33 @@ struct SchwarzRunImpl : SchwarzRun {
     }
 
     // split rows: the ones with at most FIX_SMALL partials (stored first) a row group each when a
-    // workgroup holds at least FIX_SMALL row groups, the rest a workgroup each
+    // workgroup holds at least FIX_SMALL row groups, the rest a workgroup each; one launch
     template <int MODE>
     void fix_rows(const DevBuf &fixes, uint32_t n_fix, uint32_t n_small, CT *locp, const double *mu,
                   OutT<CT, MODE> *out, uint32_t k, uint32_t kshift, hipStream_t st, const int32_t *gate) {
         const SchwarzDev<CT> &d = *D;
         if ((BLK >> kshift) < FIX_SMALL) n_small = 0;
-        if (n_small)
-            hipLaunchKernelGGL((sk_rows_fix_small<CT, MODE>), dim3(grid_rows(n_small, kshift)), dim3(BLK), 0, st,
-                               dp<RowFix>(fixes), n_small, dp<uint32_t>(d.slot_comp), locp, dp<CT>(d.inv_de), mu,
-                               dp<CT>(rowpart), out, k, kshift, gate);
-        if (n_fix > n_small)
-            hipLaunchKernelGGL((sk_rows_fix<CT, MODE>), dim3(std::min<uint32_t>(n_fix - n_small, 65535)), dim3(BLK), 0,
-                               st, dp<RowFix>(fixes) + n_small, n_fix - n_small, dp<uint32_t>(d.slot_comp), locp,
-                               dp<CT>(d.inv_de), mu, dp<CT>(rowpart), out, k, kshift, gate);
+        if (!n_fix) return;
+        // both parts loop over their rows, so either grid may be capped
+        const uint32_t g_small = n_small ? grid_rows(n_small, kshift, 32768) : 0u;
+        const uint32_t g_wide = std::min<uint32_t>(n_fix - n_small, 32767u);
+        hipLaunchKernelGGL((sk_rows_fix<CT, MODE>), dim3(g_small + g_wide), dim3(BLK), 0, st, dp<RowFix>(fixes),
+                           n_small, n_fix, g_small, dp<uint32_t>(d.slot_comp), locp, dp<CT>(d.inv_de), mu,
+                           dp<CT>(rowpart), out, k, kshift, gate);
     }
 
     template <typename VT>
-    void mean(const DevBuf &chunks, uint32_t n_chunks, const DevBuf &cptr, const DevBuf &count, const VT *src,
-              double *mu, uint32_t k, uint32_t kshift, hipStream_t st, const int32_t *gate) {
-        hipLaunchKernelGGL((sk_chunk_sum<VT>), dim3(grid_rows(n_chunks, kshift)), dim3(BLK), 0, st, dp<Chunk>(chunks),
-                           n_chunks, src, dp<double>(part), k, kshift, gate);
-        hipLaunchKernelGGL(sk_comp_mean, dim3((uint32_t)D->nc), dim3(BLK), 0, st, dp<uint32_t>(cptr), dp<double>(part),
-                           dp<uint32_t>(count), mu, k, kshift, gate);
+    void mean(const DevBuf &chunks, const DevBuf &cptr, const DevBuf &iptr, uint32_t n_items, const DevBuf &count,
+              const VT *src, double *mu, uint32_t k, uint32_t kshift, hipStream_t st, const int32_t *gate) {
+        if (!n_items) return;
+        hipLaunchKernelGGL((sk_mean<VT>), dim3(n_items), dim3(BLK), 0, st, dp<Chunk>(chunks), dp<uint32_t>(cptr),
+                           dp<uint32_t>(iptr), (uint32_t)D->nc, src, dp<uint32_t>(count), dp<double>(part),
+                           dp<uint32_t>(counters), mu, k, kshift, gate);
     }
 
     void apply_fused(double *r, double *z, uint32_t k, hipStream_t st, const int32_t *gate,
                      const ApplyFusion &fz) override {
-        ensure(k);
+        ensure(k, st);
         const uint32_t kshift = pow2_shift(k);
         const SchwarzDev<CT> &d = *D;
         CT *locp = dp<CT>(loc), *redp = dp<CT>(red), *xsp = dp<CT>(xs);
@@ -2014,13 +2119,15 @@ struct SchwarzRunImpl : SchwarzRun {
         else
             hipLaunchKernelGGL((sk_restrict<CT>), dim3(grid_rows((d.ns + 1) / 2, kshift)), dim3(BLK), 0, st, d.ns,
                                dp<uint32_t>(d.glob), dp<CT>(d.sgnpw), r, locp, k, kshift, gate);
-        mean(d.all_chunks, d.n_all_chunks, d.all_cptr, d.comp_count, locp, dp<double>(mu1), k, kshift, st, gate);
+        mean(d.all_chunks, d.all_cptr, d.all_iptr, d.n_all_items, d.comp_count, locp, dp<double>(mu1), k, kshift, st,
+             gate);
         // reduced right-hand side on kept rows
         rows_spmv<0>(d.ct_items, d.n_ct_items, d.ct_idx, d.ct_val, locp, xsp, redp, k, kshift, st, gate);
         fix_rows<0>(d.ct_fix, d.n_ct_fix, d.n_ct_fix_small, locp, dp<double>(mu1), redp, k, kshift, st, gate);
         // kept-block solves
         if (d.n_sparse) {
-            mean(d.kept_chunks, d.n_kept_chunks, d.kept_cptr, d.comp_kcount, redp, dp<double>(mu2), k, kshift, st, gate);
+            mean(d.kept_chunks, d.kept_cptr, d.kept_iptr, d.n_kept_items, d.comp_kcount, redp, dp<double>(mu2), k,
+                 kshift, st, gate);
             // large leading forward levels across the grid, then each component's workgroup
             // (remaining forward levels, pivots, leading backward levels), then the large
             // trailing backward levels across the grid, then the covers' read-back
@@ -2110,7 +2217,8 @@ struct SchwarzRunImpl : SchwarzRun {
         // back-substitution on eliminated rows
         rows_spmv<1>(d.c_items, d.n_c_items, d.c_idx, d.c_val, locp, xsp, xsp, k, kshift, st, gate);
         fix_rows<1>(d.c_fix, d.n_c_fix, d.n_c_fix_small, locp, dp<double>(mu1), xsp, k, kshift, st, gate);
-        mean(d.all_chunks, d.n_all_chunks, d.all_cptr, d.comp_count, xsp, dp<double>(mu3), k, kshift, st, gate);
+        mean(d.all_chunks, d.all_cptr, d.all_iptr, d.n_all_items, d.comp_count, xsp, dp<double>(mu3), k, kshift, st,
+             gate);
         if (fz.dots)
             hipLaunchKernelGGL((sk_prolong<CT, true>), dim3(fz.dots_blocks), dim3(BLK), 0, st, d.N,
                                dp<uint64_t>(d.d_ptr), dp<uint32_t>(d.d_slot), dp<CT>(d.sgnpw),
