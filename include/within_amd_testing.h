@@ -62,6 +62,14 @@ wx_status wx_testing_reparam(const wx_design *design, const double *weights, int
  * Needs a GPU. */
 wx_status wx_testing_precond_apply(const wx_precond *p, const double *x, uint64_t k, double *y, int32_t precision);
 
+/* Process-wide switch for later solves with an additive Schwarz preconditioner: on (the default)
+ * the LSMR steps on either side of the apply -- p~ <- A'u / beta - (theta / rho) p~ before it and
+ * the partial sums of <v,p~>, <v,v>, <p~,p~> after it (lsmr/bidiag.rs:99-111, 475-485) -- run
+ * inside the apply's first and last launch; off they run as launches of their own.  Both forms
+ * compute the same values in the same order, which tests hold them to.  Returns the previous
+ * setting (1 on, 0 off).  Stands in for nothing in the reference, whose steps are separate loops. */
+int32_t wx_testing_set_fused_steps(int32_t on);
+
 #ifdef __cplusplus
 }
 #endif

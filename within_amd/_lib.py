@@ -123,6 +123,7 @@ EXPORTS = {
     "wx_buffer_free": ([C.c_void_p], None),
     # include/within_amd_testing.h
     "wx_testing_precond_apply": ([C.c_void_p, C.c_void_p, C.c_uint64, C.c_void_p, C.c_int32], C.c_int),
+    "wx_testing_set_fused_steps": ([C.c_int32], C.c_int32),
     "wx_testing_reparam": ([C.c_void_p, C.c_void_p, C.c_int32, C.c_void_p, C.c_void_p, C.c_uint64, C.c_void_p,
                             C.c_uint64, C.c_void_p, C.c_uint64, C.c_void_p], C.c_int),
     "wx_testing_design_index": ([C.c_void_p, C.c_uint64, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,

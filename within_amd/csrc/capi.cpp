@@ -775,6 +775,8 @@ wx_status wx_testing_precond_apply(const wx_precond *p, const double *x, uint64_
     return precond_apply_host(p, x, k, y, precision);
 }
 
+int32_t wx_testing_set_fused_steps(int32_t on) { return set_fused_steps(on != 0) ? 1 : 0; }
+
 wx_status wx_precond_serialize(const wx_precond *p, uint8_t **buf, uint64_t *len) {
     return guard([&] {
         if (!p || !buf || !len) fail(WX_ERR_INVALID_INPUT, "null pointer");

@@ -25,6 +25,11 @@
 
 namespace wx {
 
+// Whether runs fold the LSMR steps around an additive Schwarz apply into it (set_fused_steps).
+std::atomic<bool> g_fused_steps{true};
+
+bool set_fused_steps(bool on) { return g_fused_steps.exchange(on); }
+
 namespace {
 
 constexpr int BLK = (int)KERNEL_BLOCK;
@@ -2572,7 +2577,7 @@ class ChunkRun {
         launch_adjoint(dv_, da_, weighted_ ? 2 : 0, ws_->yint.as<double>(), ws_->pt.as<double>(),
                        ws_->segpart.as<double>(), gate(), k_, kshift_, st_, -1, ws_->pieces_or_null());
         const uint64_t N = N_;
-        if (prec_kind_ == 2) {
+        if (fused_) {
             // p~ = A'b / beta_1 inside the restriction, the alpha_1 dot products inside the
             // prolongation
             ApplyFusion fz;
@@ -2586,6 +2591,12 @@ class ChunkRun {
             fz.dots_blocks = gN_;
             fz.blk_col = dof_col_;
             schwarz_->apply_fused(ws_->pt.as<double>(), ws_->v.as<double>(), k_, st_, gate(), fz);
+        } else if (prec_kind_ == 2) {
+            hipLaunchKernelGGL((k_ptilde<2, true>), dim3(gN_), dim3(BLK), 0, st_, N, nullptr, ws_->pt.as<double>(),
+                               ws_->v.as<double>(), nullptr, cs_, part_, k_, kshift_, dof_col_);
+            schwarz_->apply(ws_->pt.as<double>(), ws_->v.as<double>(), k_, st_, gate());
+            hipLaunchKernelGGL(k_dots, dim3(gN_), dim3(BLK), 0, st_, N, ws_->pt.as<double>(), ws_->v.as<double>(),
+                               cs_, part_, k_, kshift_, dof_col_);
         } else if (prec_kind_ == 1) {
             hipLaunchKernelGGL((k_ptilde<1, true>), dim3(gN_), dim3(BLK), 0, st_, N, nullptr, ws_->pt.as<double>(),
                                ws_->v.as<double>(), inv_diag_, cs_, part_, k_, kshift_, dof_col_);
@@ -2815,6 +2826,7 @@ class ChunkRun {
         const Precond *pc = s.precond.get();
         prec_kind_ = pc == nullptr ? 0 : (pc->kind == Precond::DIAGONAL ? 1 : 2);
         inv_diag_ = prec_kind_ == 1 ? dv_.inv_diag.as<double>() : nullptr;
+        fused_ = prec_kind_ == 2 && g_fused_steps.load();
         const int prec_slot = opts.precision == WX_PRECISION_MIXED ? 1 : 0;
         kshift_ = pow2_shift(k);
         kp_ = 1u << kshift_;
@@ -2845,7 +2857,7 @@ class ChunkRun {
         // a graph replays fixed launch arguments: not with the reorthogonalisation ring (its
         // slot changes every iteration) nor with the event timer
         use_graph_ = use_graph && cap_ == 0 && !(timer_ && timer_->on);
-        graph_key_ = (uint64_t)group_ * 4 + (uint64_t)prec_slot;
+        graph_key_ = ((uint64_t)group_ * 4 + (uint64_t)prec_slot) * 2 + (fused_ ? 1 : 0);
     }
 
     // convergence gate of the kernels: the run's active flags (a pack's members converge
@@ -2886,7 +2898,7 @@ class ChunkRun {
                            ws.segpart.as<double>(), gate(), k, kshift, st, dv_.fuse_term, ws.pieces_or_null());
         }
         if (timed) timer_->end(1);
-        if (prec_kind_ == 2 && cap_ == 0) {
+        if (fused_ && cap_ == 0) {
             // p~ update folded into the restriction, the alpha dot products into the prolongation
             ApplyFusion fz;
             fz.blk_col = dof_col_;
@@ -2994,6 +3006,7 @@ class ChunkRun {
     bool weighted_ = false;
     const uint32_t *inv_perm_ = nullptr;
     int maxiter_ = 0, prec_kind_ = 0;
+    bool fused_ = false; // the LSMR steps around the Schwarz apply folded into it
     const double *inv_diag_ = nullptr;
     uint32_t gn_ = 1, gN_ = 1, gt_ = 1, ncol_ = 1;
     const uint32_t *dof_col_ = nullptr;

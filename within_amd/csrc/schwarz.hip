@@ -35,6 +35,7 @@ constexpr uint32_t ROW_CHUNK = 128;   // CSR entries per work item before a row 
                                       // (256: too few items for the C' SpMV's long rows)
 constexpr uint32_t MEAN_CHUNK = 64;   // slots per partial-sum chunk
 constexpr uint32_t MEAN_ITEM = 16;    // chunks per workgroup of the single-launch means
+constexpr uint32_t MEAN_SMALL = 16;   // one-chunk components per workgroup of the means
 constexpr uint32_t DENSE_TILED_MIN = 256; // kept blocks at least this large use the tiled product
 constexpr uint64_t DENSE_TILE_TARGET = 8192; // (row tile, column segment) pairs wanted in flight
 constexpr uint64_t LONG_ROW = 64;         // factor rows with more entries are summed by a whole workgroup
@@ -62,6 +63,12 @@ struct Chunk {
     uint32_t begin, end; // slot range
     uint32_t pad;
 };
+// A work item of the component means.  kind 0: chunks [chunk0, chunk0 + n) (n <= MEAN_ITEM) of
+// component comp, whose items are first .. first + nitems - 1 of the list; kind 1: the n chunks
+// from chunk0 (n <= MEAN_SMALL), each the only chunk of its component, a row group per chunk.
+struct MeanItem {
+    uint32_t kind, comp, chunk0, n, first, nitems;
+};
 
 template <typename CT>
 struct SchwarzDev {
@@ -73,12 +80,14 @@ struct SchwarzDev {
     DevBuf c_idx, c_val, c_items, c_fix;
     uint32_t n_c_items = 0, n_c_fix = 0, n_c_part = 0, n_c_fix_small = 0;
     // means: chunks over all slots (per component) and over kept slots of sparse components
-    DevBuf all_chunks, all_cptr, kept_chunks, kept_cptr;
+    DevBuf all_chunks, kept_chunks;
     uint32_t n_all_chunks = 0, n_kept_chunks = 0;
-    // the single-launch means (sk_mean): per component, its first work item of MEAN_ITEM
-    // chunks in each list (nc + 1 entries)
-    DevBuf all_iptr, kept_iptr;
+    // the single-launch means (sk_mean): the work items over each chunk list
+    DevBuf all_items, kept_items;
     uint32_t n_all_items = 0, n_kept_items = 0;
+    // the components of several items in each list (sk_mean_combine): comp, first, nitems
+    DevBuf all_comb, kept_comb;
+    uint32_t n_all_comb = 0, n_kept_comb = 0;
     DevBuf comp_count, comp_kcount;
     // dense kept blocks: one entry per kept row: (matrix row offset, first kept slot, nk)
     DevBuf drow_off, drow_base, drow_nk, drow_slot, dmat;
@@ -256,25 +265,24 @@ __global__ __launch_bounds__(BLK) void sk_restrict_dof(uint64_t N, const uint64_
     }
 }
 
-// Mean of component c over its chunk partials, by the whole workgroup: row groups stride over
-// the chunks with eight accumulators (partial q goes to q / ngrp mod 8, eight loads in flight),
-// combined in a fixed order, then a fixed-order LDS tree across the groups.
-__device__ inline void comp_mean_one(uint32_t c, const uint32_t *__restrict__ cptr, const double *__restrict__ part,
+// Mean of component c over the item sums part[q0 .. q1), by the whole workgroup: row groups
+// stride over the items with eight accumulators (item q goes to (q - q0) / ngrp mod 8, eight
+// loads in flight), combined in a fixed order, then a fixed-order LDS tree across the groups.
+__device__ inline void comp_mean_one(uint32_t c, uint32_t q0, uint32_t q1, const double *__restrict__ part,
                                      const uint32_t *__restrict__ count, double *__restrict__ mu, uint32_t k,
                                      uint32_t kshift, double *sh) {
     const uint32_t kp = 1u << kshift, j = threadIdx.x & (kp - 1), grp = threadIdx.x >> kshift;
     const uint32_t ngrp = BLK >> kshift;
     double a8[8] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
     if (j < k) {
-        const uint32_t e = cptr[c + 1];
-        uint32_t q = cptr[c] + grp;
-        for (; q + 7 * ngrp < e; q += 8 * ngrp) {
+        uint32_t q = q0 + grp;
+        for (; q + 7 * ngrp < q1; q += 8 * ngrp) {
 #pragma unroll
             for (int m = 0; m < 8; ++m) a8[m] += part[(uint64_t)(q + m * ngrp) * k + j];
         }
 #pragma unroll
         for (int m = 0; m < 7; ++m)
-            if (q + m * ngrp < e) a8[m] += part[(uint64_t)(q + m * ngrp) * k + j];
+            if (q + m * ngrp < q1) a8[m] += part[(uint64_t)(q + m * ngrp) * k + j];
     }
     const double acc = ((a8[0] + a8[1]) + (a8[2] + a8[3])) + ((a8[4] + a8[5]) + (a8[6] + a8[7]));
     sh[threadIdx.x] = acc;
@@ -290,95 +298,72 @@ __device__ inline void comp_mean_one(uint32_t c, const uint32_t *__restrict__ cp
     __syncthreads();
 }
 
-// mean per component = sum of its chunk partials / count; one workgroup per component
-__global__ __launch_bounds__(BLK) void sk_comp_mean(const uint32_t *__restrict__ cptr, const double *__restrict__ part,
-                                                    const uint32_t *__restrict__ count, double *__restrict__ mu,
-                                                    uint32_t k, uint32_t kshift, const int32_t *__restrict__ gate) {
-    if (gate_closed(gate, k)) return;
-    __shared__ double sh[BLK];
-    comp_mean_one(blockIdx.x, cptr, part, count, mu, k, kshift, sh);
-}
-
-// partial sums of src over chunks of slots (FP64 sums of the stored slot values)
+// acc plus column j of the chunk's slots, added in slot order with four loads in flight
 template <typename VT>
-__global__ __launch_bounds__(BLK) void sk_chunk_sum(const Chunk *__restrict__ chunks, uint32_t n_chunks,
-                                                    const VT *__restrict__ src, double *__restrict__ part,
-                                                    uint32_t k, uint32_t kshift, const int32_t *__restrict__ gate) {
-    if (gate_closed(gate, k)) return;
-    const uint32_t kp = 1u << kshift, j = threadIdx.x & (kp - 1);
-    if (j >= k) return;
-    const uint32_t g = (blockIdx.x * BLK + threadIdx.x) >> kshift;
-    if (g >= n_chunks) return;
-    const Chunk ch = chunks[g];
-    double acc = 0.0;
-    uint32_t s = ch.begin;
-    for (; s + 4 <= ch.end; s += 4) { // four loads in flight, summed in slot order
+__device__ inline double chunk_add(double acc, const Chunk &C, const VT *__restrict__ src, uint32_t k, uint32_t j) {
+    uint32_t q = C.begin;
+    for (; q + 4 <= C.end; q += 4) {
         double t[4];
 #pragma unroll
-        for (int q = 0; q < 4; ++q) t[q] = (double)src[(uint64_t)(s + q) * k + j];
+        for (int u = 0; u < 4; ++u) t[u] = (double)src[(uint64_t)(q + u) * k + j];
         acc += t[0];
         acc += t[1];
         acc += t[2];
         acc += t[3];
     }
-    for (; s < ch.end; ++s) acc += (double)src[(uint64_t)s * k + j];
-    part[(uint64_t)g * k + j] = acc;
+    for (; q < C.end; ++q) acc += (double)src[(uint64_t)q * k + j];
+    return acc;
 }
 
-// Component means in one launch: workgroup b takes MEAN_ITEM chunks of one component (its item
-// b - iptr[c]); its row groups add their chunks' slots in order, a fixed LDS tree adds the
-// groups, and the item's sums go to part[b].  The workgroup that finishes a component's last
-// item (a per-component counter, agent-scope fences around it) adds the component's item sums
-// in item order (comp_mean_one) and resets the counter for the next launch.  No workgroup
-// waits on another, and the sums do not depend on the order in which workgroups finish.
+// Component means, workgroup b taking work item b (MeanItem).  A kind-0 item's row groups add
+// their chunks' slots in order and a fixed LDS tree adds the groups.  When the item is its
+// component's only one, that is the component's sum and the item writes the mean; otherwise the
+// item's sums go to part[b] and sk_mean_combine adds them.  A kind-1 item's row groups each form
+// the mean of a one-chunk component.  Every path divides the same sum, (total + 0.0), by the
+// count.
 template <typename VT>
-__global__ __launch_bounds__(BLK) void sk_mean(const Chunk *__restrict__ chunks, const uint32_t *__restrict__ cptr,
-                                               const uint32_t *__restrict__ iptr, uint32_t nc,
+__global__ __launch_bounds__(BLK) void sk_mean(const MeanItem *__restrict__ items, const Chunk *__restrict__ chunks,
                                                const VT *__restrict__ src, const uint32_t *__restrict__ count,
-                                               double *part, uint32_t *counter, double *__restrict__ mu, uint32_t k,
+                                               double *__restrict__ part, double *__restrict__ mu, uint32_t k,
                                                uint32_t kshift, const int32_t *__restrict__ gate) {
     if (gate_closed(gate, k)) return;
     __shared__ double sh[BLK];
-    __shared__ uint32_t last;
-    const uint32_t b = blockIdx.x;
-    uint32_t lo = 0, hi = nc - 1; // the component of item b: the last c with iptr[c] <= b
-    while (lo < hi) {
-        const uint32_t mid = (lo + hi + 1) / 2;
-        if (iptr[mid] <= b) lo = mid;
-        else hi = mid - 1;
-    }
-    const uint32_t c = lo;
+    const MeanItem it = items[blockIdx.x];
     const uint32_t kp = 1u << kshift, j = threadIdx.x & (kp - 1), grp = threadIdx.x >> kshift;
     const uint32_t ngrp = BLK >> kshift;
-    const uint32_t ch0 = cptr[c] + (b - iptr[c]) * MEAN_ITEM, ch1 = min(cptr[c + 1], ch0 + MEAN_ITEM);
+    if (it.kind == 1) {
+        if (j < k)
+            for (uint32_t q = grp; q < it.n; q += ngrp) {
+                const Chunk C = chunks[it.chunk0 + q];
+                const double tot = chunk_add(0.0, C, src, k, j);
+                const uint32_t n = count[C.comp];
+                mu[(uint64_t)C.comp * k + j] = n ? (tot + 0.0) / (double)n : 0.0;
+            }
+        return;
+    }
     double acc = 0.0;
     if (j < k)
-        for (uint32_t ch = ch0 + grp; ch < ch1; ch += ngrp) {
-            const Chunk C = chunks[ch];
-            uint32_t q = C.begin;
-            for (; q + 4 <= C.end; q += 4) { // four loads in flight, added in slot order
-                double t[4];
-#pragma unroll
-                for (int u = 0; u < 4; ++u) t[u] = (double)src[(uint64_t)(q + u) * k + j];
-                acc += t[0];
-                acc += t[1];
-                acc += t[2];
-                acc += t[3];
-            }
-            for (; q < C.end; ++q) acc += (double)src[(uint64_t)q * k + j];
-        }
+        for (uint32_t ch = it.chunk0 + grp; ch < it.chunk0 + it.n; ch += ngrp) acc = chunk_add(acc, chunks[ch], src, k, j);
     const double tot = cols_block_sum(acc, kp, sh);
-    if (threadIdx.x < kp && j < k) part[(uint64_t)b * k + j] = tot;
-    __syncthreads();
-    if (threadIdx.x == 0) { // (the lanes that stored sums, kp <= 64 of them, are this wavefront's)
-        __threadfence(); // release: this item's sums before its count
-        last = atomicAdd(&counter[c], 1u) == iptr[c + 1] - iptr[c] - 1;
+    if (threadIdx.x < kp && j < k) {
+        if (it.nitems == 1) {
+            const uint32_t n = count[it.comp];
+            mu[(uint64_t)it.comp * k + j] = n ? (tot + 0.0) / (double)n : 0.0;
+        } else {
+            part[(uint64_t)blockIdx.x * k + j] = tot;
+        }
     }
-    __syncthreads();
-    if (!last) return;
-    __threadfence(); // acquire: every item's sums after the count
-    comp_mean_one(c, iptr, part, count, mu, k, kshift, sh); // the item sums in item order
-    if (threadIdx.x == 0) counter[c] = 0;                  // for the next launch
+}
+
+// The means of the components with several work items: workgroup b adds component
+// comb[b].comp's item sums in item order (comp_mean_one).
+__global__ __launch_bounds__(BLK) void sk_mean_combine(const MeanItem *__restrict__ comb, const double *__restrict__ part,
+                                                       const uint32_t *__restrict__ count, double *__restrict__ mu,
+                                                       uint32_t k, uint32_t kshift, const int32_t *__restrict__ gate) {
+    if (gate_closed(gate, k)) return;
+    __shared__ double sh[BLK];
+    const MeanItem it = comb[blockIdx.x];
+    comp_mean_one(it.comp, it.first, it.first + it.nitems, part, count, mu, k, kshift, sh);
 }
 
 // MODE 0: reduced rhs of kept rows:  red[row] = (loc[row]-mu) + sum_p ct[p]*(loc[e]-mu), the C'
@@ -1493,22 +1478,45 @@ std::shared_ptr<SchwarzDev<CT>> build_dev(const Precond::Schwarz &S) {
     }
     D->all_chunks.upload(all.data(), all.size());
     D->n_all_chunks = (uint32_t)all.size();
-    D->all_cptr.upload(all_cptr.data(), all_cptr.size());
     D->kept_chunks.upload(kept.data(), kept.size());
     D->n_kept_chunks = (uint32_t)kept.size();
-    D->kept_cptr.upload(kept_cptr.data(), kept_cptr.size());
     D->comp_count.upload(cnt.data(), cnt.size());
     D->comp_kcount.upload(kcnt.data(), kcnt.size());
     {
-        std::vector<uint32_t> ai(nc + 1, 0), ki(nc + 1, 0);
-        for (uint64_t c = 0; c < nc; ++c) {
-            ai[c + 1] = ai[c] + (all_cptr[c + 1] - all_cptr[c] + MEAN_ITEM - 1) / MEAN_ITEM;
-            ki[c + 1] = ki[c] + (kept_cptr[c + 1] - kept_cptr[c] + MEAN_ITEM - 1) / MEAN_ITEM;
-        }
-        D->all_iptr.upload(ai.data(), ai.size());
-        D->kept_iptr.upload(ki.data(), ki.size());
-        D->n_all_items = ai[nc];
-        D->n_kept_items = ki[nc];
+        // work items of the means: components of one chunk MEAN_SMALL to an item, larger ones
+        // MEAN_ITEM chunks to an item
+        auto mean_items = [nc](const std::vector<uint32_t> &cp) {
+            std::vector<MeanItem> it;
+            for (uint64_t c = 0; c < nc; ++c) {
+                const uint32_t c0 = cp[c], nch = cp[c + 1] - c0;
+                if (nch == 1) {
+                    MeanItem *o = it.empty() ? nullptr : &it.back();
+                    if (o && o->kind == 1 && o->chunk0 + o->n == c0 && o->n < MEAN_SMALL) ++o->n;
+                    else it.push_back({1u, (uint32_t)c, c0, 1u, 0u, 0u});
+                    continue;
+                }
+                const uint32_t ni = (nch + MEAN_ITEM - 1) / MEAN_ITEM, first = (uint32_t)it.size();
+                for (uint32_t i = 0; i < ni; ++i)
+                    it.push_back({0u, (uint32_t)c, c0 + i * MEAN_ITEM, std::min(MEAN_ITEM, nch - i * MEAN_ITEM), first, ni});
+            }
+            return it;
+        };
+        auto combined = [](const std::vector<MeanItem> &it) {
+            std::vector<MeanItem> cb;
+            for (size_t i = 0; i < it.size(); ++i)
+                if (it[i].kind == 0 && it[i].nitems > 1 && it[i].first == (uint32_t)i) cb.push_back(it[i]);
+            return cb;
+        };
+        const std::vector<MeanItem> ai = mean_items(all_cptr), ki = mean_items(kept_cptr);
+        const std::vector<MeanItem> ac = combined(ai), kc = combined(ki);
+        D->all_items.upload(ai.data(), ai.size());
+        D->kept_items.upload(ki.data(), ki.size());
+        D->n_all_items = (uint32_t)ai.size();
+        D->n_kept_items = (uint32_t)ki.size();
+        D->all_comb.upload(ac.data(), ac.size());
+        D->kept_comb.upload(kc.data(), kc.size());
+        D->n_all_comb = (uint32_t)ac.size();
+        D->n_kept_comb = (uint32_t)kc.size();
     }
     // dense rows: small blocks row by row, blocks with nk >= DENSE_TILED_MIN as split-K tiles
     std::vector<uint64_t> drow_off;
@@ -2013,7 +2021,6 @@ struct SchwarzRunImpl : SchwarzRun {
     std::shared_ptr<SchwarzDev<CT>> D;
     uint32_t kcap = 0;
     DevBuf loc, red, xs, part, rowpart, mu1, mu2, mu3, dpart, tsc, csc;
-    DevBuf counters; // per component: items of the current sk_mean launch finished
 
     void ensure(uint32_t k, hipStream_t st) {
         if (k <= kcap) return;
@@ -2031,18 +2038,13 @@ struct SchwarzRunImpl : SchwarzRun {
         mu2.alloc(D->nc * k * 8);
         mu3.alloc(D->nc * k * 8);
         // a component without chunks in a list (the kept list of a dense or grounded one) has
-        // no sk_mean item and keeps the mean 0 set here; the counters start at 0 and sk_mean
-        // leaves them 0.  Cleared by a kernel on the run's stream: a memset on the default
-        // stream while another thread captures a graph can end up in that graph.
-        const bool fresh = !counters.p;
-        if (fresh) counters.alloc(D->nc * 4 + 4);
+        // no sk_mean item and keeps the mean 0 set here.  Cleared by a kernel on the run's
+        // stream: a memset on the default stream while another thread captures a graph can end
+        // up in that graph.
         const uint64_t nmu = (uint64_t)D->nc * k * 2; // 32-bit words
         for (DevBuf *m : {&mu1, &mu2, &mu3})
             hipLaunchKernelGGL(sk_zero_words, dim3((uint32_t)std::min<uint64_t>((nmu + 255) / 256 + 1, 65535)),
                                dim3(256), 0, st, m->as<uint32_t>(), nmu);
-        if (fresh)
-            hipLaunchKernelGGL(sk_zero_words, dim3((uint32_t)std::min<uint64_t>((D->nc + 256) / 256 + 1, 65535)),
-                               dim3(256), 0, st, counters.as<uint32_t>(), (uint64_t)D->nc + 1);
         WX_HIP(hipGetLastError());
         dpart.alloc(D->dense_part_rows * k * V + 8);
         tsc.alloc(D->n_tail_rows * k * V + 8);
@@ -2097,12 +2099,15 @@ struct SchwarzRunImpl : SchwarzRun {
     }
 
     template <typename VT>
-    void mean(const DevBuf &chunks, const DevBuf &cptr, const DevBuf &iptr, uint32_t n_items, const DevBuf &count,
-              const VT *src, double *mu, uint32_t k, uint32_t kshift, hipStream_t st, const int32_t *gate) {
+    void mean(const DevBuf &chunks, const DevBuf &items, uint32_t n_items, const DevBuf &comb, uint32_t n_comb,
+              const DevBuf &count, const VT *src, double *mu, uint32_t k, uint32_t kshift, hipStream_t st,
+              const int32_t *gate) {
         if (!n_items) return;
-        hipLaunchKernelGGL((sk_mean<VT>), dim3(n_items), dim3(BLK), 0, st, dp<Chunk>(chunks), dp<uint32_t>(cptr),
-                           dp<uint32_t>(iptr), (uint32_t)D->nc, src, dp<uint32_t>(count), dp<double>(part),
-                           dp<uint32_t>(counters), mu, k, kshift, gate);
+        hipLaunchKernelGGL((sk_mean<VT>), dim3(n_items), dim3(BLK), 0, st, dp<MeanItem>(items), dp<Chunk>(chunks),
+                           src, dp<uint32_t>(count), dp<double>(part), mu, k, kshift, gate);
+        if (n_comb)
+            hipLaunchKernelGGL(sk_mean_combine, dim3(n_comb), dim3(BLK), 0, st, dp<MeanItem>(comb), dp<double>(part),
+                               dp<uint32_t>(count), mu, k, kshift, gate);
     }
 
     void apply_fused(double *r, double *z, uint32_t k, hipStream_t st, const int32_t *gate,
@@ -2119,15 +2124,15 @@ struct SchwarzRunImpl : SchwarzRun {
         else
             hipLaunchKernelGGL((sk_restrict<CT>), dim3(grid_rows((d.ns + 1) / 2, kshift)), dim3(BLK), 0, st, d.ns,
                                dp<uint32_t>(d.glob), dp<CT>(d.sgnpw), r, locp, k, kshift, gate);
-        mean(d.all_chunks, d.all_cptr, d.all_iptr, d.n_all_items, d.comp_count, locp, dp<double>(mu1), k, kshift, st,
-             gate);
+        mean(d.all_chunks, d.all_items, d.n_all_items, d.all_comb, d.n_all_comb, d.comp_count, locp, dp<double>(mu1),
+             k, kshift, st, gate);
         // reduced right-hand side on kept rows
         rows_spmv<0>(d.ct_items, d.n_ct_items, d.ct_idx, d.ct_val, locp, xsp, redp, k, kshift, st, gate);
         fix_rows<0>(d.ct_fix, d.n_ct_fix, d.n_ct_fix_small, locp, dp<double>(mu1), redp, k, kshift, st, gate);
         // kept-block solves
         if (d.n_sparse) {
-            mean(d.kept_chunks, d.kept_cptr, d.kept_iptr, d.n_kept_items, d.comp_kcount, redp, dp<double>(mu2), k,
-                 kshift, st, gate);
+            mean(d.kept_chunks, d.kept_items, d.n_kept_items, d.kept_comb, d.n_kept_comb, d.comp_kcount, redp,
+                 dp<double>(mu2), k, kshift, st, gate);
             // large leading forward levels across the grid, then each component's workgroup
             // (remaining forward levels, pivots, leading backward levels), then the large
             // trailing backward levels across the grid, then the covers' read-back
@@ -2217,8 +2222,8 @@ struct SchwarzRunImpl : SchwarzRun {
         // back-substitution on eliminated rows
         rows_spmv<1>(d.c_items, d.n_c_items, d.c_idx, d.c_val, locp, xsp, xsp, k, kshift, st, gate);
         fix_rows<1>(d.c_fix, d.n_c_fix, d.n_c_fix_small, locp, dp<double>(mu1), xsp, k, kshift, st, gate);
-        mean(d.all_chunks, d.all_cptr, d.all_iptr, d.n_all_items, d.comp_count, xsp, dp<double>(mu3), k, kshift, st,
-             gate);
+        mean(d.all_chunks, d.all_items, d.n_all_items, d.all_comb, d.n_all_comb, d.comp_count, xsp, dp<double>(mu3),
+             k, kshift, st, gate);
         if (fz.dots)
             hipLaunchKernelGGL((sk_prolong<CT, true>), dim3(fz.dots_blocks), dim3(BLK), 0, st, d.N,
                                dp<uint64_t>(d.d_ptr), dp<uint32_t>(d.d_slot), dp<CT>(d.sgnpw),

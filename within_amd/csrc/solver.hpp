@@ -238,6 +238,11 @@ void last_many_shape(int32_t *streams, int32_t *group, int32_t *graph, int32_t *
 void force_many_tile(int32_t rows);
 int32_t last_many_tile();
 
+// Whether later solves fold the LSMR steps on either side of an additive Schwarz apply into its
+// first and last launch (default) or run them as launches of their own; returns the previous
+// setting.  The two forms compute the same values in the same order.
+bool set_fused_steps(bool on);
+
 // Adjoint of the weighted projection R = Z - D x(Z, w) for reverse-mode differentiation:
 // grad_z = G - w .* (D lambda), grad_w = -sum_j R_j .* (D lambda)_j with D'WD lambda = D'G.
 // Column-major device arrays in caller row order; grad_w may be null.
