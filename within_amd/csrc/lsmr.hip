@@ -712,32 +712,61 @@ __global__ __launch_bounds__(BLK) void k_load_rhs(uint64_t n, const uint32_t *__
     // narrower than its pack)
     extern __shared__ double tile[]; // 64 x (k + 1)
     __shared__ double sh[BLK];
-    const uint32_t ld = k + 1;
+    __shared__ uint64_t t_int[64]; // the tile rows' internal rows
+    __shared__ double t_sw[64];    // and their sqrt-weights
+    const uint32_t ld = k + 1, nq = 64 * k;
     const uint32_t kp = 1u << kshift, j = threadIdx.x & (kp - 1), grp = threadIdx.x >> kshift;
     const uint32_t ngrp = BLK >> kshift;
     const uint64_t n_tiles = (n + 63) / 64;
     double mx = 0.0, sq = 0.0;
     int32_t nbad = 0;
-    for (uint64_t tb = blockIdx.x; tb < n_tiles; tb += gridDim.x) {
+    // a tile's loads into registers: element q = threadIdx.x + m * BLK of the 64 x k tile (column
+    // q / 64, row q % 64), and for threads < 64 their row's internal row and sqrt-weight; the
+    // next tile's are issued before the current tile's stores
+    constexpr uint32_t QM = 64 * MAX_BATCH / BLK;
+    double yr[QM];
+    uint64_t ir = 0;
+    double wr = 1.0;
+    auto fetch = [&](uint64_t tb) {
         const uint64_t r0 = tb * 64;
-        for (uint32_t q = threadIdx.x; q < 64 * k; q += BLK) {
-            const uint32_t c = q >> 6, rr = q & 63;
-            const uint64_t r = r0 + rr;
+#pragma unroll
+        for (uint32_t m = 0; m < QM; ++m) {
+            const uint32_t q = threadIdx.x + m * BLK, c = q >> 6;
+            const uint64_t r = r0 + (q & 63);
+            yr[m] = q < nq && r < n && c < kv ? Y[r + c * ldy] : 0.0;
+        }
+        if (threadIdx.x < 64) {
+            const uint64_t r = r0 + threadIdx.x;
             if (r < n) {
-                const double y = c < kv ? Y[r + c * ldy] : 0.0;
-                nbad += !isfinite(y);
-                tile[rr * ld + c] = y;
+                ir = inv_perm ? (uint64_t)inv_perm[r] : r;
+                if (WEIGHTED) wr = sw_caller[r];
             }
         }
+    };
+    if (blockIdx.x < n_tiles) fetch(blockIdx.x);
+    for (uint64_t tb = blockIdx.x; tb < n_tiles; tb += gridDim.x) {
+        const uint64_t r0 = tb * 64;
+#pragma unroll
+        for (uint32_t m = 0; m < QM; ++m) {
+            const uint32_t q = threadIdx.x + m * BLK;
+            if (q < nq && r0 + (q & 63) < n) {
+                nbad += !isfinite(yr[m]);
+                tile[(q & 63) * ld + (q >> 6)] = yr[m];
+            }
+        }
+        if (threadIdx.x < 64) {
+            t_int[threadIdx.x] = ir;
+            t_sw[threadIdx.x] = wr;
+        }
         __syncthreads();
+        if (tb + gridDim.x < n_tiles) fetch(tb + gridDim.x);
         if (j < k)
             for (uint32_t rr = grp; rr < 64; rr += ngrp) {
-                const uint64_t r = r0 + rr;
-                if (r >= n) break;
-                const uint64_t i = inv_perm ? inv_perm[r] : r;
+                if (r0 + rr >= n) break;
+                const uint64_t i = t_int[rr];
                 const double y = tile[rr * ld + j];
                 y_int[i * k + j] = y;
-                const double b = WEIGHTED ? sw_caller[r] * y : y;
+                const double b = WEIGHTED ? t_sw[rr] * y : y;
                 mx = fmax(mx, fabs(b));
                 sq += b * b;
             }
@@ -1388,6 +1417,35 @@ __global__ __launch_bounds__(BLK) void k_demean_store(DesignArgs d, const uint4 
     constexpr int R = DEMEAN_TILE / 16 < 4 ? 4 : DEMEAN_TILE / 16;
     for (uint64_t tb = blockIdx.x; tb < n_tiles; tb += gridDim.x) {
         const uint64_t r0 = tb * DEMEAN_TILE;
+        // (D x)_r of rows rb + q * ngrp (q < R), the terms added in order; ok[q] = the row exists
+        bool ok[R];
+        double dx[R];
+        auto rows_dx = [&](uint32_t rb) {
+            uint32_t code[TT][R];
+#pragma unroll
+            for (int q = 0; q < R; ++q) {
+                const uint64_t r = r0 + rb + q * ngrp;
+                ok[q] = rb + q * ngrp < DEMEAN_TILE && r < n;
+                const uint4 c4 = codes4c[ok[q] ? r : 0];
+                const uint32_t cc[4] = {c4.x, c4.y, c4.z, c4.w};
+#pragma unroll
+                for (int t = 0; t < TT; ++t) code[t][q] = cc[t];
+            }
+            double val[TT][R];
+#pragma unroll
+            for (int t = 0; t < TT; ++t)
+#pragma unroll
+                for (int q = 0; q < R; ++q) val[t][q] = x[(d.terms[t].offset + code[t][q]) * k + j];
+#pragma unroll
+            for (int q = 0; q < R; ++q) {
+                double a = 0.0;
+#pragma unroll
+                for (int t = 0; t < TT; ++t) a += val[t][q];
+                dx[q] = a;
+            }
+        };
+        // the first rows' codes and coefficient gathers in flight with the tile's loads
+        if (j < k) rows_dx(grp);
         for (uint32_t q = threadIdx.x; q < DEMEAN_TILE * k; q += BLK) {
             const uint32_t c = q / DEMEAN_TILE, rr = q % DEMEAN_TILE;
             const uint64_t r = r0 + rr;
@@ -1396,30 +1454,12 @@ __global__ __launch_bounds__(BLK) void k_demean_store(DesignArgs d, const uint4 
         __syncthreads();
         if (j < k)
             for (uint32_t rb = grp; rb < DEMEAN_TILE; rb += R * ngrp) {
-                bool ok[R];
-                uint32_t code[TT][R];
-#pragma unroll
-                for (int q = 0; q < R; ++q) {
-                    const uint64_t r = r0 + rb + q * ngrp;
-                    ok[q] = rb + q * ngrp < DEMEAN_TILE && r < n;
-                    const uint4 c4 = codes4c[ok[q] ? r : 0];
-                    const uint32_t cc[4] = {c4.x, c4.y, c4.z, c4.w};
-#pragma unroll
-                    for (int t = 0; t < TT; ++t) code[t][q] = cc[t];
-                }
-                double val[TT][R];
-#pragma unroll
-                for (int t = 0; t < TT; ++t)
-#pragma unroll
-                    for (int q = 0; q < R; ++q) val[t][q] = x[(d.terms[t].offset + code[t][q]) * k + j];
+                if (rb != grp) rows_dx(rb);
 #pragma unroll
                 for (int q = 0; q < R; ++q) {
                     if (!ok[q]) continue;
-                    double a = 0.0;
-#pragma unroll
-                    for (int t = 0; t < TT; ++t) a += val[t][q];
                     double &yv = tile[(rb + q * ngrp) * ld + j];
-                    yv = yv - a;
+                    yv = yv - dx[q];
                 }
             }
         __syncthreads();
