@@ -703,7 +703,7 @@ __global__ void k_segfix(TermInfo ti, const SegFix *__restrict__ fixes, uint32_t
 // values are written as one contiguous segment.  Non-finite entries are counted, and every
 // block leaves per-column partials of sum(b^2) and max|b| for b = sqrt(w) .* y, so beta_1
 // needs no further pass over the observations.
-template <bool WEIGHTED>
+template <bool WEIGHTED, uint32_t QM>
 __global__ __launch_bounds__(BLK) void k_load_rhs(uint64_t n, const uint32_t *__restrict__ inv_perm,
                                                   const double *__restrict__ sw_caller, const double *__restrict__ Y,
                                                   uint64_t ldy, double *__restrict__ y_int, double *__restrict__ part,
@@ -721,9 +721,8 @@ __global__ __launch_bounds__(BLK) void k_load_rhs(uint64_t n, const uint32_t *__
     double mx = 0.0, sq = 0.0;
     int32_t nbad = 0;
     // a tile's loads into registers: element q = threadIdx.x + m * BLK of the 64 x k tile (column
-    // q / 64, row q % 64), and for threads < 64 their row's internal row and sqrt-weight; the
-    // next tile's are issued before the current tile's stores
-    constexpr uint32_t QM = 64 * MAX_BATCH / BLK;
+    // q / 64, row q % 64; QM * BLK >= 64 k), and for threads < 64 their row's internal row and
+    // sqrt-weight; the next tile's are issued before the current tile's stores
     double yr[QM];
     uint64_t ir = 0;
     double wr = 1.0;
@@ -1369,6 +1368,7 @@ __global__ __launch_bounds__(BLK) void k_demean2(DesignArgs d, const uint4 *__re
 
 // Internal-order interleaved dm_int -> caller-order column-major Dm (solver.rs:483), through
 // 64-row LDS tiles (k+1 doubles per tile row) so the column-major writes are coalesced.
+template <uint32_t RM>
 __global__ __launch_bounds__(BLK) void k_store_caller(uint64_t n, const uint32_t *__restrict__ inv_perm,
                                                       const double *__restrict__ dm_int, double *__restrict__ Dm,
                                                       uint64_t ldd, uint32_t k, uint32_t kshift, uint32_t kv) {
@@ -1377,15 +1377,28 @@ __global__ __launch_bounds__(BLK) void k_store_caller(uint64_t n, const uint32_t
     const uint32_t kp = 1u << kshift, j = threadIdx.x & (kp - 1), grp = threadIdx.x >> kshift;
     const uint32_t ngrp = BLK >> kshift;
     const uint64_t n_tiles = (n + 63) / 64;
+    // a tile's gathers into registers: this thread's rows grp + m * ngrp (m < RM, RM * ngrp >=
+    // 64), column j; the next tile's are issued before the current tile's stores
+    double v[RM];
+    auto fetch = [&](uint64_t tb) {
+        const uint64_t r0 = tb * 64;
+#pragma unroll
+        for (uint32_t m = 0; m < RM; ++m) {
+            const uint32_t rr = grp + m * ngrp;
+            const uint64_t r = r0 + rr;
+            v[m] = j < k && rr < 64 && r < n ? dm_int[(inv_perm ? (uint64_t)inv_perm[r] : r) * k + j] : 0.0;
+        }
+    };
+    if (blockIdx.x < n_tiles) fetch(blockIdx.x);
     for (uint64_t tb = blockIdx.x; tb < n_tiles; tb += gridDim.x) {
         const uint64_t r0 = tb * 64;
-        if (j < k)
-            for (uint32_t rr = grp; rr < 64; rr += ngrp) {
-                const uint64_t r = r0 + rr;
-                if (r >= n) break;
-                tile[rr * ld + j] = dm_int[(inv_perm ? inv_perm[r] : r) * k + j];
-            }
+#pragma unroll
+        for (uint32_t m = 0; m < RM; ++m) {
+            const uint32_t rr = grp + m * ngrp;
+            if (j < k && rr < 64 && r0 + rr < n) tile[rr * ld + j] = v[m];
+        }
         __syncthreads();
+        if (tb + gridDim.x < n_tiles) fetch(tb + gridDim.x);
         for (uint32_t q = threadIdx.x; q < 64 * k; q += BLK) {
             const uint32_t c = q >> 6, rr = q & 63;
             const uint64_t r = r0 + rr;
@@ -2553,25 +2566,17 @@ class ChunkRun {
                 const uint32_t *ip = md.inv_perm.p ? md.inv_perm.as<uint32_t>() : nullptr;
                 double *yint = ws_->yint.as<double>() + mb.row_off * k_;
                 double *pp = part_ + (uint64_t)mb.tile_off * kp_ * 2;
-                if (weighted_)
-                    hipLaunchKernelGGL((k_load_rhs<true>), dim3(mb.n_tiles), dim3(BLK), tile_bytes_, st_, mb.n, ip,
-                                       ip ? md.sw_caller.as<double>() : md.sw.as<double>(), mY_[m], mld_[m], yint, pp,
-                                       badp, k_, kshift_, mb.kv);
-                else
-                    hipLaunchKernelGGL((k_load_rhs<false>), dim3(mb.n_tiles), dim3(BLK), tile_bytes_, st_, mb.n, ip,
-                                       nullptr, mY_[m], mld_[m], yint, pp, badp, k_, kshift_, mb.kv);
+                launch_load_rhs(mb.n_tiles, mb.n, ip,
+                                weighted_ ? (ip ? md.sw_caller.as<double>() : md.sw.as<double>()) : nullptr, mY_[m],
+                                mld_[m], yint, pp, badp, mb.kv);
             }
             hipLaunchKernelGGL(k_beta1, dim3(ncol_), dim3(BLK), 0, st_, part_, pk_->n_tiles, kp_, cs_, opts_.tol,
                                badp + 1, k_, rg_tile_, badp, nullptr);
             WX_HIP(hipMemcpyAsync(ws_->hbad, ws_->bad.p, 8, hipMemcpyDeviceToHost, st_));
         } else {
-            if (weighted_)
-                hipLaunchKernelGGL((k_load_rhs<true>), dim3(gt_), dim3(BLK), tile_bytes_, st_, n_, inv_perm_,
-                                   inv_perm_ ? dv_.sw_caller.as<double>() : dv_.sw.as<double>(), Y_, ldy_,
-                                   ws_->yint.as<double>(), part_, badp, k_, kshift_, k_);
-            else
-                hipLaunchKernelGGL((k_load_rhs<false>), dim3(gt_), dim3(BLK), tile_bytes_, st_, n_, inv_perm_, nullptr,
-                                   Y_, ldy_, ws_->yint.as<double>(), part_, badp, k_, kshift_, k_);
+            launch_load_rhs(gt_, n_, inv_perm_,
+                            weighted_ ? (inv_perm_ ? dv_.sw_caller.as<double>() : dv_.sw.as<double>()) : nullptr, Y_,
+                            ldy_, ws_->yint.as<double>(), part_, badp, k_);
             // the non-finite count goes straight to pinned memory; finish() reads it
             hipLaunchKernelGGL(k_beta1, dim3(k_), dim3(BLK), 0, st_, part_, gt_, kp_, cs_, opts_.tol, badp + 1, k_,
                                Ranges{}, badp, ws_->dbad);
@@ -2724,6 +2729,36 @@ class ChunkRun {
         return m / rate <= 2.0f;
     }
 
+    // caller-order rows (kv of the k columns) -> internal order, sqrt(w) y partials; sw null:
+    // unweighted
+    void launch_load_rhs(uint64_t grid, uint64_t n, const uint32_t *inv_perm, const double *sw, const double *Y,
+                         uint64_t ldy, double *y_int, double *part, int32_t *bad, uint32_t kv) {
+        const dim3 g((uint32_t)grid), b(BLK);
+        if (sw && k_ <= 16)
+            hipLaunchKernelGGL((k_load_rhs<true, 4>), g, b, tile_bytes_, st_, n, inv_perm, sw, Y, ldy, y_int, part,
+                               bad, k_, kshift_, kv);
+        else if (sw)
+            hipLaunchKernelGGL((k_load_rhs<true, 16>), g, b, tile_bytes_, st_, n, inv_perm, sw, Y, ldy, y_int, part,
+                               bad, k_, kshift_, kv);
+        else if (k_ <= 16)
+            hipLaunchKernelGGL((k_load_rhs<false, 4>), g, b, tile_bytes_, st_, n, inv_perm, nullptr, Y, ldy, y_int,
+                               part, bad, k_, kshift_, kv);
+        else
+            hipLaunchKernelGGL((k_load_rhs<false, 16>), g, b, tile_bytes_, st_, n, inv_perm, nullptr, Y, ldy, y_int,
+                               part, bad, k_, kshift_, kv);
+    }
+
+    // internal-order rows -> the caller's column-major output (kv of the k columns)
+    void launch_store_caller(uint64_t grid, uint64_t n, const uint32_t *inv_perm, const double *dm_int, double *Dm,
+                             uint64_t ldd, uint32_t kv) {
+        if (k_ <= 16)
+            hipLaunchKernelGGL((k_store_caller<4>), dim3((uint32_t)grid), dim3(BLK), tile_bytes_, st_, n, inv_perm,
+                               dm_int, Dm, ldd, k_, kshift_, kv);
+        else
+            hipLaunchKernelGGL((k_store_caller<16>), dim3((uint32_t)grid), dim3(BLK), tile_bytes_, st_, n, inv_perm,
+                               dm_int, Dm, ldd, k_, kshift_, kv);
+    }
+
     void finish() {
         if (!pk_) { // the load's validation (finish_init did not wait for it)
             if (!load_done_) WX_HIP(hipStreamSynchronize(st_));
@@ -2767,9 +2802,8 @@ class ChunkRun {
                 const PackMember &mb = pk_->members[m];
                 const DevDesign &md = mb.s->dev;
                 if (mDm_[m])
-                    hipLaunchKernelGGL(k_store_caller, dim3(mb.n_tiles), dim3(BLK), tile_bytes_, st, mb.n,
-                                       md.inv_perm.p ? md.inv_perm.as<uint32_t>() : nullptr, dm_int + mb.row_off * k,
-                                       mDm_[m], mb.n, k, kshift, mb.kv);
+                    launch_store_caller(mb.n_tiles, mb.n, md.inv_perm.p ? md.inv_perm.as<uint32_t>() : nullptr,
+                                        dm_int + mb.row_off * k, mDm_[m], mb.n, mb.kv);
                 any_x = any_x || mX_[m];
             }
             if (any_x) {
@@ -2780,9 +2814,7 @@ class ChunkRun {
                                    xtab.as<double *>(), pk_->own_ld.as<uint64_t>(), pk_->own_kv.as<uint32_t>());
             }
         } else {
-            if (Dm_ && !direct)
-                hipLaunchKernelGGL(k_store_caller, dim3(gt_), dim3(BLK), tile_bytes_, st, n, inv_perm_, dm_int,
-                                   Dm_ + c0_ * n, n, k, kshift, k);
+            if (Dm_ && !direct) launch_store_caller(gt_, n, inv_perm_, dm_int, Dm_ + c0_ * n, n, k);
             if (X_ && (!s_.reparam || x_solver_basis_)) {
                 const uint64_t tot = N * k;
                 hipLaunchKernelGGL(k_store_x, dim3((tot + BLK - 1) / BLK), dim3(BLK), 0, st, N, ws.x.as<double>(),
