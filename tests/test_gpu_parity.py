@@ -610,7 +610,7 @@ def _apply_prec(p, X, precision):
     return Y
 
 
-@pytest.mark.parametrize("kind", ["dense", "sparse", "slope_cover"])
+@pytest.mark.parametrize("kind", ["dense", "dense_tiled", "sparse", "slope_cover"])
 def test_mixed_precision_apply_is_symmetric_and_definite(gpu, kind):
     """The preconditioner of a mixed-precision solve (FP32 coefficients and FP32 slot-space
     vectors, FP64 accumulation) is applied as an explicit matrix: it stays symmetric and
@@ -619,7 +619,15 @@ def test_mixed_precision_apply_is_symmetric_and_definite(gpu, kind):
     rounding of the FP64 apply."""
     rng = np.random.default_rng(71)
     n = 6000
-    if kind == "slope_cover":
+    if kind == "dense_tiled":
+        # a 300-level kept block: the FP32 run applies its upper triangle of 64 x 64 blocks on
+        # the matrix cores (five block rows, the last one 44 rows, two super-rows of four)
+        n = 20000
+        a, b, c = rng.integers(0, 600, n), rng.integers(0, 300, n), rng.integers(0, 6, n)
+        design = np.asfortranarray(np.column_stack([a, b, c]).astype(np.uint32))
+        w = rng.uniform(0.5, 2.0, n)
+        cfg = wa.config.AdditiveSchwarz(wa.config.LocalSolverConfig(dense_threshold=1024))
+    elif kind == "slope_cover":
         _, design, _, _, w = _slope_design(53)
         cfg = wa.config.AdditiveSchwarz(wa.config.LocalSolverConfig(dense_threshold=0, dense_policy="reference"))
     else:
@@ -642,6 +650,9 @@ def test_mixed_precision_apply_is_symmetric_and_definite(gpu, kind):
     assert np.abs(M32 - M32.T).max() <= 1e-5 * scale
     ev = np.linalg.eigvalsh(0.5 * (M32 + M32.T))
     assert ev.min() >= -1e-5 * ev.max()
+    # more than 16 columns: the products run in 16-column windows
+    X = rng.standard_normal((N, 21))
+    np.testing.assert_allclose(_apply_prec(p, X, 1), M64 @ X, rtol=0, atol=1e-5 * np.abs(M64 @ X).max())
     # the FP32 vectors change only rounding: a mixed solve reaches the FP64 solve's answer
     Y = np.asfortranarray(rng.standard_normal((s.n_obs, 2)))
     rm = s.solve_batch(Y, wa.LsmrOptions(tol=1e-10, precision="mixed"))
