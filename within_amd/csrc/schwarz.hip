@@ -2058,7 +2058,11 @@ struct SchwarzRunImpl : SchwarzRun {
                    CT *xsp, OutT<CT, MODE> *out, uint32_t k, uint32_t kshift, hipStream_t st, const int32_t *gate) {
         const SchwarzDev<CT> &d = *D;
         if constexpr (std::is_same<CT, float>::value) {
-            if (k >= 4 && (k & 3) == 0) { // FP32 slot vectors: four columns per lane
+            // FP32 slot vectors: four columns per lane.  The C' rows (MODE 0) are the long ones,
+            // split into items of up to ROW_CHUNK entries that a lane walks serially: below 16
+            // columns two columns per lane keep twice the lanes (and waves) on them (worker-firm
+            // panel at k = 4: 103 -> 52 us; 64 uneven problems: +3 %)
+            if ((k & 3) == 0 && (MODE == 1 ? k >= 4 : k >= 16)) {
                 const uint64_t per = 4 * (uint64_t)(BLK >> kshift);
                 const uint32_t g =
                     (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>((n_items + per - 1) / per, 65535));
