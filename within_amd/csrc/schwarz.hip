@@ -34,7 +34,7 @@ constexpr int SP_BLK = 1024; // threads of the per-component sparse triangular s
 constexpr uint32_t ROW_CHUNK = 128;   // CSR entries per work item before a row is split
                                       // (256: too few items for the C' SpMV's long rows)
 constexpr uint32_t MEAN_CHUNK = 64;   // slots per partial-sum chunk
-constexpr uint32_t MEAN_ITEM = 16;    // chunks per workgroup of the single-launch means
+constexpr uint32_t MEAN_ITEM = 16;    // chunks per workgroup of the component means
 constexpr uint32_t MEAN_SMALL = 16;   // one-chunk components per workgroup of the means
 constexpr uint32_t DENSE_TILED_MIN = 256; // kept blocks at least this large use the tiled product
 constexpr uint64_t DENSE_TILE_TARGET = 8192; // (row tile, column segment) pairs wanted in flight
@@ -82,7 +82,7 @@ struct SchwarzDev {
     // means: chunks over all slots (per component) and over kept slots of sparse components
     DevBuf all_chunks, kept_chunks;
     uint32_t n_all_chunks = 0, n_kept_chunks = 0;
-    // the single-launch means (sk_mean): the work items over each chunk list
+    // the component means (sk_mean): the work items over each chunk list
     DevBuf all_items, kept_items;
     uint32_t n_all_items = 0, n_kept_items = 0;
     // the components of several items in each list (sk_mean_combine): comp, first, nitems
