@@ -120,6 +120,7 @@ struct SchwarzDev {
     uint64_t n_aux = 0;
     // prolongation: DOF -> slots
     DevBuf d_ptr, d_slot;
+    DevBuf d_w, d_comp; // per d_slot entry: its slot's sgnpw and component (no dependent lookup)
     bool has_sparse = false;
 };
 
@@ -231,7 +232,7 @@ __global__ __launch_bounds__(BLK) void sk_restrict(uint64_t ns, const uint32_t *
 template <typename CT>
 __global__ __launch_bounds__(BLK) void sk_restrict_dof(uint64_t N, const uint64_t *__restrict__ d_ptr,
                                                        const uint32_t *__restrict__ d_slot,
-                                                       const CT *__restrict__ sgnpw, const double *__restrict__ atu,
+                                                       const CT *__restrict__ d_w, const double *__restrict__ atu,
                                                        double *__restrict__ pt, ApplyFusion fz, CT *__restrict__ loc,
                                                        uint32_t k, uint32_t kshift, const int32_t *__restrict__ gate) {
     if (gate_closed(gate, k)) return;
@@ -254,13 +255,13 @@ __global__ __launch_bounds__(BLK) void sk_restrict_dof(uint64_t N, const uint64_
         uint64_t i = d_ptr[d];
         for (; i + 2 <= e; i += 2) { // two slots' stores in flight
             const uint64_t s0 = d_slot[i], s1 = d_slot[i + 1];
-            const double w0 = (double)sgnpw[s0], w1 = (double)sgnpw[s1];
+            const double w0 = (double)d_w[i], w1 = (double)d_w[i + 1];
             loc[s0 * k + j] = (CT)(w0 * p);
             loc[s1 * k + j] = (CT)(w1 * p);
         }
         if (i < e) {
             const uint64_t s = d_slot[i];
-            loc[s * k + j] = (CT)((double)sgnpw[s] * p);
+            loc[s * k + j] = (CT)((double)d_w[i] * p);
         }
     }
 }
@@ -1265,12 +1266,13 @@ __global__ __launch_bounds__(BLK) void sk_cover_readback(const uint32_t *__restr
 }
 
 // z[dof] = sum over the DOF's slots of sgnpw[s] * (xs[s] - mu3[comp(s)]); uncovered DOFs -> 0.
+// sgnpw[s] and comp(s) come per d_slot entry (d_w, d_comp), so only xs and mu3 are gathered.
 // DOTS: also the block partials of <z,r>, <z,z>, <r,r> (r = the apply's input p~), each lane
 // adding its DOFs in loop order, then a fixed-order tree over the lanes of a column.
 template <typename CT, bool DOTS>
 __global__ __launch_bounds__(BLK) void sk_prolong(uint64_t N, const uint64_t *__restrict__ d_ptr,
-                                                  const uint32_t *__restrict__ d_slot, const CT *__restrict__ sgnpw,
-                                                  const uint32_t *__restrict__ slot_comp,
+                                                  const uint32_t *__restrict__ d_slot, const CT *__restrict__ d_w,
+                                                  const uint32_t *__restrict__ d_comp,
                                                   const CT *__restrict__ xs, const double *__restrict__ mu3,
                                                   double *__restrict__ z, uint32_t k, uint32_t kshift, const int32_t *__restrict__ gate,
                                                   const double *__restrict__ r, double *__restrict__ dots) {
@@ -1284,15 +1286,15 @@ __global__ __launch_bounds__(BLK) void sk_prolong(uint64_t N, const uint64_t *__
         uint64_t q = d_ptr[d];
         for (; q + 2 <= q1; q += 2) { // two slots' loads in flight, added in slot order
             const uint64_t s0 = d_slot[q], s1 = d_slot[q + 1];
-            const double w0 = (double)sgnpw[s0], w1 = (double)sgnpw[s1];
+            const double w0 = (double)d_w[q], w1 = (double)d_w[q + 1];
             const double x0 = (double)xs[s0 * k + j], x1 = (double)xs[s1 * k + j];
-            const double m0 = mu3[(uint64_t)slot_comp[s0] * k + j], m1 = mu3[(uint64_t)slot_comp[s1] * k + j];
+            const double m0 = mu3[(uint64_t)d_comp[q] * k + j], m1 = mu3[(uint64_t)d_comp[q + 1] * k + j];
             acc += w0 * (x0 - m0);
             acc += w1 * (x1 - m1);
         }
         for (; q < q1; ++q) {
             const uint64_t s = d_slot[q];
-            acc += (double)sgnpw[s] * ((double)xs[s * k + j] - mu3[(uint64_t)slot_comp[s] * k + j]);
+            acc += (double)d_w[q] * ((double)xs[s * k + j] - mu3[(uint64_t)d_comp[q] * k + j]);
         }
         z[d * k + j] = acc;
         if (DOTS) {
@@ -1709,6 +1711,16 @@ std::shared_ptr<SchwarzDev<CT>> build_dev(const Precond::Schwarz &S) {
     }
     D->d_ptr.upload(d_ptr.data(), d_ptr.size());
     D->d_slot.upload(d_slot.data(), d_slot.size());
+    {
+        std::vector<CT> dw(ns);
+        std::vector<uint32_t> dc(ns);
+        for (uint64_t i = 0; i < ns; ++i) {
+            dw[i] = sgnpw[d_slot[i]];
+            dc[i] = slot_comp[d_slot[i]];
+        }
+        D->d_w.upload(dw.data(), ns);
+        D->d_comp.upload(dc.data(), ns);
+    }
     for (uint64_t c = 0; c < nc; ++c) D->has_sparse |= S.fac_kind[c] == 1 || S.fac_kind[c] == 2;
     if (D->has_sparse) {
         // level schedules of L (forward) and L' (backward) per sparse component
@@ -2123,7 +2135,7 @@ struct SchwarzRunImpl : SchwarzRun {
         if (fz.atu)
             hipLaunchKernelGGL((sk_restrict_dof<CT>), dim3(fz.blk_col ? fz.dots_blocks : grid_rows(d.N, kshift)),
                                dim3(BLK), 0, st,
-                               d.N, dp<uint64_t>(d.d_ptr), dp<uint32_t>(d.d_slot), dp<CT>(d.sgnpw), fz.atu, r, fz,
+                               d.N, dp<uint64_t>(d.d_ptr), dp<uint32_t>(d.d_slot), dp<CT>(d.d_w), fz.atu, r, fz,
                                locp, k, kshift, gate);
         else
             hipLaunchKernelGGL((sk_restrict<CT>), dim3(grid_rows((d.ns + 1) / 2, kshift)), dim3(BLK), 0, st, d.ns,
@@ -2230,12 +2242,12 @@ struct SchwarzRunImpl : SchwarzRun {
              k, kshift, st, gate);
         if (fz.dots)
             hipLaunchKernelGGL((sk_prolong<CT, true>), dim3(fz.dots_blocks), dim3(BLK), 0, st, d.N,
-                               dp<uint64_t>(d.d_ptr), dp<uint32_t>(d.d_slot), dp<CT>(d.sgnpw),
-                               dp<uint32_t>(d.slot_comp), xsp, dp<double>(mu3), z, k, kshift, gate, r, fz.dots);
+                               dp<uint64_t>(d.d_ptr), dp<uint32_t>(d.d_slot), dp<CT>(d.d_w),
+                               dp<uint32_t>(d.d_comp), xsp, dp<double>(mu3), z, k, kshift, gate, r, fz.dots);
         else
             hipLaunchKernelGGL((sk_prolong<CT, false>), dim3(grid_rows(d.N, kshift)), dim3(BLK), 0, st, d.N,
-                               dp<uint64_t>(d.d_ptr), dp<uint32_t>(d.d_slot), dp<CT>(d.sgnpw),
-                               dp<uint32_t>(d.slot_comp), xsp, dp<double>(mu3), z, k, kshift, gate, nullptr,
+                               dp<uint64_t>(d.d_ptr), dp<uint32_t>(d.d_slot), dp<CT>(d.d_w),
+                               dp<uint32_t>(d.d_comp), xsp, dp<double>(mu3), z, k, kshift, gate, nullptr,
                                nullptr);
         WX_HIP(hipGetLastError());
     }
